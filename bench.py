@@ -1,0 +1,101 @@
+"""Benchmark: WHIR commit + prove of one 2^24-variable KoalaBear polynomial on one MI355X.
+
+Workload (the single-GPU configuration of the headline metric): num_variables
+24, folding factor 4, log inverse rate 1 (rate 1/2, 2^25-entry codeword), RS
+domain initial reduction 3, 100-bit security with 16 PoW bits, CapacityBound,
+Poseidon2 width-16 Merkle tree, two seeded extension-field evaluation
+constraints.  The polynomial's 2^24 coefficients are drawn from SmallRng seed 0
+and uploaded once; every timed step runs statement evaluation, commit and
+prove from that device-resident input and returns the proof bytes to the host.
+
+Prints one JSON line.  --gpus N > 1 is not supported (one GPU per box).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=24)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    import __graft_entry__ as ge
+
+    W = ge._package()
+    n = args.n
+    prm = W.params(n)
+    cfg = W.WhirConfig(prm, device=0)
+    poly = W.sample_field(0, 1 << n)
+    pts = [W.sample_field(1 + j, 4 * n).reshape(n, 4) for j in range(2)]
+    d_poly = W.DeviceBuffer.from_array(poly)
+
+    def step():
+        proof, _ = W.prove_once(cfg, None, pts, device_ptr=d_poly.ptr)
+        return proof
+
+    for _ in range(args.warmup):
+        proof = step()
+    times = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s = time.perf_counter()
+        proof = step()
+        times.append((time.perf_counter() - s) * 1e3)
+    total_ms = (time.perf_counter() - t0) * 1e3
+    ms = total_ms / args.steps
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from helpers import oracle_params, oracle_prove
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        t = time.perf_counter()
+        cproof, _, cms = oracle_prove(oracle_params(n), poly, pts)
+        wall = (time.perf_counter() - t) * 1e3
+        cpu = {"value": round(cms[0] + cms[1], 1), "unit": "ms", "cores": threads, "kind": "port",
+               "sample": f"one full n={n} commit+prove of the same inputs on the CPU oracle ({wall:.0f} ms wall "
+                         f"incl. statement evaluation)",
+               "bit_exact": cproof == proof}
+
+    out = {
+        "metric": "prove ms @2^24 vars" if n == 24 else f"prove ms @2^{n} vars",
+        "value": round(ms, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32 (KoalaBear, Montgomery form)",
+        "data": "synthetic (SmallRng seed 0 coefficients, seeds 1-2 points)",
+        "config": {"workload": f"WHIR commit+prove n={n} KoalaBear/EF4, folding 4, rate 1/2, rs-reduction 3, "
+                               f"lambda 100, pow 16, CapacityBound, Poseidon2-16 Merkle, 2 eval constraints",
+                   "num_variables": n, "proofs_per_step": 1},
+        "proofs_per_s": round(1e3 / ms, 2),
+        "step_ms": [round(t, 3) for t in times],
+        "proof_bytes": len(proof),
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,202 @@
+/*
+ * whir_gpu.h - C ABI of the MI355X WHIR prover (KoalaBear / EF4 / Poseidon2).
+ *
+ * Drop-in boundary for the commit/prove path of tcoratger/whir-p3.  Each entry
+ * point names the reference interface it replaces (file:line in whir-p3).
+ *
+ * Conventions
+ *  - Base-field elements are uint32_t in Montgomery form (a * 2^32 mod p), the
+ *    in-memory layout of Plonky3's KoalaBear, so a Rust &[KoalaBear] can be
+ *    passed as-is.  An extension element is 4 consecutive uint32_t (c0..c3 of
+ *    c0 + c1 x + c2 x^2 + c3 x^3, x^4 = 3).  A multilinear point of n variables
+ *    is n extension elements.  Digests are 8 base elements.
+ *  - Every call returns WHIRGPU_OK (0) or an error code; no exception or panic
+ *    crosses the ABI.  whirgpu_last_error() describes the last failure of the
+ *    calling thread.
+ *  - Contexts are independent and may be used from different threads; one
+ *    context must not be used by two threads at once.  Calls block until
+ *    their results are on the host.
+ *  - The caller owns every host buffer.  Device memory belongs to the context,
+ *    statement, prover-data and proof handles and is released by their
+ *    destroy calls.
+ */
+#ifndef WHIR_GPU_H
+#define WHIR_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  WHIRGPU_OK = 0,
+  WHIRGPU_BAD_PARAMS = 1,       /* the reference asserts/panics (src/whir/parameters.rs:84-114, prover/mod.rs:110) */
+  WHIRGPU_OOM = 2,
+  WHIRGPU_DEVICE = 3,
+  WHIRGPU_GRIND_FAILED = 4,     /* FiatShamirError::InvalidGrindingWitness (src/fiat_shamir/errors.rs:9-13) */
+  WHIRGPU_BUFFER_TOO_SMALL = 5,
+  WHIRGPU_BAD_STATE = 6
+};
+
+/* ProtocolParameters (src/parameters/mod.rs:147-166) plus the number of
+ * variables given to WhirConfig::new (src/whir/parameters.rs:81). */
+typedef struct whirgpu_params {
+  uint32_t num_variables;
+  uint32_t security_level;
+  uint32_t pow_bits;
+  /* FoldingFactor (src/parameters/mod.rs:31-38): Constant(k) when
+   * folding_factor_first == folding_factor == k, otherwise
+   * ConstantFromSecondRound(folding_factor_first, folding_factor). */
+  uint32_t folding_factor_first;
+  uint32_t folding_factor;
+  uint32_t starting_log_inv_rate;
+  uint32_t rs_domain_initial_reduction_factor;
+  uint32_t soundness_type;      /* SecurityAssumption: 0 UniqueDecoding, 1 JohnsonBound, 2 CapacityBound */
+  uint32_t sumcheck_strategy;   /* SumcheckStrategy (src/whir/parameters.rs:16-29): 0 Classic, 1 Svo */
+  uint32_t leaf_hash_width;     /* 16: PaddingFreeSponge<P2-16,16,8,8>; 24: PaddingFreeSponge<P2-24,24,16,8> */
+  uint32_t digest_elems;        /* DIGEST_ELEMS of DomainSeparator::commit_statement (domain_separator.rs:95-105) */
+} whirgpu_params;
+
+/* Poseidon2 round constants (Montgomery form) for the width-16 permutation
+ * (challenger, node compression, width-16 leaves) and the width-24 one
+ * (width-24 leaves), as held by Plonky3's Poseidon2KoalaBear<16/24>. */
+typedef struct whirgpu_poseidon2 {
+  uint32_t w16_external_initial[4 * 16];
+  uint32_t w16_external_terminal[4 * 16];
+  uint32_t w16_internal[20];
+  uint32_t w24_external_initial[4 * 24];
+  uint32_t w24_external_terminal[4 * 24];
+  uint32_t w24_internal[23];
+} whirgpu_poseidon2;
+
+/* DuplexChallenger<F, Poseidon2KoalaBear<16>, 16, 8> state (p3-challenger):
+ * sponge state, pending inputs and unread outputs (popped from the end). */
+typedef struct whirgpu_challenger {
+  uint32_t sponge[16];
+  uint32_t input[8];
+  uint32_t output[8];
+  uint32_t input_len;
+  uint32_t output_len;
+} whirgpu_challenger;
+
+typedef struct whirgpu_ctx whirgpu_ctx;                 /* WhirConfig + device + MMCS */
+typedef struct whirgpu_statement whirgpu_statement;     /* InitialStatement */
+typedef struct whirgpu_prover_data whirgpu_prover_data; /* MT::ProverData<DenseMatrix<F>> */
+typedef struct whirgpu_proof whirgpu_proof;             /* WhirProof */
+
+const char* whirgpu_last_error(void);
+
+/* Poseidon2KoalaBear<16>::new_from_rng_128 then Poseidon2KoalaBear<24>::new_from_rng_128
+ * on SmallRng::seed_from_u64(seed) (src/bin/main.rs:100-102, benches/whir.rs:66-68). */
+int whirgpu_poseidon2_from_seed(uint64_t seed, whirgpu_poseidon2* out);
+
+/* `count` draws of rng.random::<KoalaBear>() on SmallRng::seed_from_u64(seed)
+ * (the polynomial of benches/whir.rs:96-97), Montgomery form. */
+int whirgpu_sample_field(uint64_t seed, uint32_t* out, size_t count);
+
+/* One Poseidon2 permutation on the host (width 16 or 24), for checking. */
+int whirgpu_poseidon2_permute(const whirgpu_poseidon2* perms, uint32_t width, uint32_t* state);
+
+/* WhirConfig::new (src/whir/parameters.rs:81-234) on device `device`. */
+int whirgpu_create(int device, const whirgpu_params* params, const whirgpu_poseidon2* perms, whirgpu_ctx** out);
+int whirgpu_destroy(whirgpu_ctx* ctx);
+
+/* The derived schedule: [commitment_ood_samples, starting_folding_pow_bits,
+ * n_rounds, final_queries, final_pow_bits, final_sumcheck_rounds,
+ * final_folding_pow_bits] then per round [pow_bits, folding_pow_bits,
+ * num_queries, ood_samples, num_variables, folding_factor, log2(domain_size),
+ * folded_domain_gen]. */
+int whirgpu_config(const whirgpu_ctx* ctx, uint32_t* out, size_t cap, size_t* len);
+
+/* DomainSeparator::{commit_statement, add_whir_proof} pattern
+ * (src/fiat_shamir/domain_separator.rs:95-169), Montgomery form. */
+int whirgpu_domain_separator(const whirgpu_ctx* ctx, uint32_t* out, size_t cap, size_t* len);
+
+/* DuplexChallenger::new (empty state), and
+ * DomainSeparator::observe_domain_separator (domain_separator.rs:81-86). */
+int whirgpu_challenger_new(whirgpu_challenger* ch);
+int whirgpu_challenger_observe(const whirgpu_ctx* ctx, whirgpu_challenger* ch, const uint32_t* values, size_t n);
+int whirgpu_challenger_observe_domain_separator(const whirgpu_ctx* ctx, whirgpu_challenger* ch);
+/* CanSample::sample: `n` draws into out (Montgomery form). */
+int whirgpu_challenger_sample(const whirgpu_ctx* ctx, whirgpu_challenger* ch, uint32_t* out, size_t n);
+
+/* WhirConfig::initial_statement (src/whir/parameters.rs:419-429): takes the
+ * 2^n evaluations (host pointer, or device pointer when evals_on_device != 0;
+ * a device buffer is borrowed and must outlive the statement). */
+int whirgpu_statement_create(whirgpu_ctx* ctx, const uint32_t* evals, int evals_on_device, whirgpu_statement** out);
+/* InitialStatement::evaluate (src/whir/constraints/statement/initial.rs:143-152). */
+int whirgpu_statement_evaluate(whirgpu_statement* st, const uint32_t* point, uint32_t* eval_out);
+int whirgpu_statement_destroy(whirgpu_statement* st);
+
+/* WhirProof::from_protocol_parameters (src/whir/proof.rs:209-233). */
+int whirgpu_proof_create(whirgpu_ctx* ctx, whirgpu_proof** out);
+int whirgpu_proof_destroy(whirgpu_proof* proof);
+/* Proof bytes: field values as Montgomery u32, lengths as u64, Option tags as
+ * one byte, little endian, fields in WhirProof declaration order
+ * (src/whir/proof.rs:17-141). */
+int whirgpu_proof_serialize(const whirgpu_proof* proof, uint8_t* out, size_t cap, size_t* len);
+
+/* CommitmentWriter::commit (src/whir/committer/writer.rs:56-109). */
+int whirgpu_commit(whirgpu_ctx* ctx, whirgpu_proof* proof, whirgpu_challenger* ch, whirgpu_statement* st,
+                   whirgpu_prover_data** out);
+/* Prover::prove (src/whir/prover/mod.rs:97-128); consumes prover_data
+ * (the reference moves it in), which is destroyed on return. */
+int whirgpu_prove(whirgpu_ctx* ctx, whirgpu_proof* proof, whirgpu_challenger* ch, const whirgpu_statement* st,
+                  whirgpu_prover_data* prover_data);
+int whirgpu_prover_data_destroy(whirgpu_prover_data* pd);
+
+/* Per-stage device time of the last commit/prove on this context, in ms:
+ * [lde, merkle, grind, sumcheck, mle, other, total_wall]. */
+int whirgpu_last_timings(const whirgpu_ctx* ctx, double* out7);
+
+/* Device memory for callers that keep inputs resident in HBM (the Rust side
+ * would hold such a buffer between proofs).  kind: 0 host->device,
+ * 1 device->host, 2 device->device.  No reference counterpart. */
+int whirgpu_device_alloc(int device, size_t bytes, void** out);
+int whirgpu_device_free(int device, void* ptr);
+int whirgpu_memcpy(int device, void* dst, const void* src, size_t bytes, int kind);
+
+/* ---------------- kernel-level entry points (device pointers, blocking) ---------------- */
+
+/* Transpose + zero-pad + column DFT (writer.rs:70-86 with dft_batch):
+ * d_src holds 2^m base elements; d_dst receives 2^(m-k+log_blowup) rows of
+ * 2^k elements.  whirgpu_lde_ext: the same on extension elements
+ * (prover/mod.rs:166-180 with dft_algebra_batch). */
+int whirgpu_lde_base(whirgpu_ctx* ctx, const uint32_t* d_src, uint32_t m, uint32_t k, uint32_t log_blowup,
+                     uint32_t* d_dst);
+int whirgpu_lde_ext(whirgpu_ctx* ctx, const uint32_t* d_src, uint32_t m, uint32_t k, uint32_t log_blowup,
+                    uint32_t* d_dst);
+
+/* MerkleTreeMmcs::commit_matrix of `height` rows of `row_len` base elements;
+ * d_tree receives 2*height-1 digests (leaves first, root last). */
+int whirgpu_merkle_commit(whirgpu_ctx* ctx, const uint32_t* d_rows, uint64_t height, uint32_t row_len,
+                          uint32_t* d_tree, uint32_t* root_out);
+
+/* GrindingChallenger::grind (smallest witness); updates *ch as the
+ * reference does (observe witness, sample_bits). */
+int whirgpu_grind(whirgpu_ctx* ctx, whirgpu_challenger* ch, uint32_t bits, uint32_t* witness_out);
+
+/* EvaluationsList::new_from_point (src/poly/evals.rs:312-320): 2^n extension
+ * elements scale * eq(point, x) into d_out. */
+int whirgpu_eq_expand(whirgpu_ctx* ctx, const uint32_t* point, uint32_t n, const uint32_t* scale, uint32_t* d_out);
+
+/* evaluate_hypercube_base / _ext (src/poly/evals.rs:334-366). */
+int whirgpu_mle_eval_base(whirgpu_ctx* ctx, const uint32_t* d_f, uint32_t n, const uint32_t* point, uint32_t* out);
+int whirgpu_mle_eval_ext(whirgpu_ctx* ctx, const uint32_t* d_f, uint32_t n, const uint32_t* point, uint32_t* out);
+
+/* sumcheck_coefficients (src/poly/evals.rs:493-526): out = (h(0), h(2)) of
+ * arrays of 2^n extension elements. */
+int whirgpu_sumcheck_coeffs(whirgpu_ctx* ctx, const uint32_t* d_evals, const uint32_t* d_weights, uint32_t n,
+                            uint32_t* out);
+/* compress (src/poly/evals.rs:127-148) of both arrays with r; when n >= 2
+ * `out` (may be NULL) receives the next round's (h(0), h(2)). */
+int whirgpu_sumcheck_fold(whirgpu_ctx* ctx, uint32_t* d_evals, uint32_t* d_weights, uint32_t n, const uint32_t* r,
+                          uint32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WHIR_GPU_H */

@@ -264,3 +264,17 @@ int orc_verify(const int* params, uint64_t perm_seed, const uint8_t* proof, size
 }
 
 }  // extern "C"
+
+extern "C" {
+// SplitMix64-derived xoshiro256++ state of SmallRng::seed_from_u64(seed), and
+// raw outputs of xoshiro256++ from an explicit state (known-answer checks).
+void orc_rng_state(uint64_t seed, uint64_t* out4) {
+  SmallRng r(seed);
+  for (int i = 0; i < 4; i++) out4[i] = r.s[i];
+}
+void orc_xoshiro_outputs(const uint64_t* state4, uint64_t* out, size_t n) {
+  SmallRng r(0);
+  for (int i = 0; i < 4; i++) r.s[i] = state4[i];
+  for (size_t i = 0; i < n; i++) out[i] = r.next_u64();
+}
+}

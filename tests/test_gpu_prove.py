@@ -1,0 +1,90 @@
+"""Whole commit + prove on the device against the oracle: byte-identical
+proofs, and the oracle's verifier accepts the device's proofs."""
+import numpy as np
+import pytest
+
+from helpers import oracle_params, oracle_prove, oracle_sample, oracle_verify, product
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    (10, dict(security_level=32, pow_bits=5, ff=2, rs=1)),
+    (10, dict(security_level=32, pow_bits=0, ff=3, rs=2, soundness=0)),
+    (9, dict(security_level=32, pow_bits=5, ff=2, rs=1, soundness=1)),
+    (11, dict(security_level=32, pow_bits=10, ff=4, ff0=3, rs=3)),
+    (8, dict(security_level=32, pow_bits=5, ff=1, rs=1)),
+    (6, dict(security_level=32, pow_bits=5, ff=2, rs=2)),
+    (4, dict(security_level=32, pow_bits=5, ff=4, rs=1)),
+    (14, dict(security_level=100, pow_bits=16, ff=4, rs=3)),
+    (16, dict(security_level=100, pow_bits=16, ff=4, rs=3)),
+    (13, dict(security_level=32, pow_bits=5, ff=2, rs=1, leaf_width=24)),
+    (12, dict(security_level=60, pow_bits=8, ff=3, rs=1, digest_elems=32)),
+]
+
+
+def product_cfg(n, security_level=100, pow_bits=16, ff=4, ff0=None, rate=1, rs=3, soundness=2, svo=1,
+                leaf_width=16, digest_elems=8):
+    W = product()
+    prm = W.params(n, security_level=security_level, pow_bits=pow_bits, folding_factor=ff, folding_factor_first=ff0,
+                   starting_log_inv_rate=rate, rs_domain_initial_reduction_factor=rs, soundness=soundness,
+                   svo=bool(svo), leaf_hash_width=leaf_width, digest_elems=digest_elems)
+    return W.WhirConfig(prm, device=0)
+
+
+def inputs(n, npts, seed=0):
+    poly = oracle_sample(seed, 1 << n)
+    pts = [oracle_sample(seed + 1 + j, 4 * n).reshape(n, 4) for j in range(npts)]
+    return poly, pts
+
+
+@pytest.mark.parametrize("n,kw", CASES)
+@pytest.mark.parametrize("npts", [0, 1, 2])
+def test_proof_bytes_match_oracle(n, kw, npts):
+    W = product()
+    poly, pts = inputs(n, npts, seed=n)
+    cfg = product_cfg(n, **kw)
+    got, evals = W.prove_once(cfg, poly, pts)
+    want, want_evals, _ = oracle_prove(oracle_params(n, **kw), poly, pts)
+    for a, b in zip(evals, want_evals):
+        assert np.array_equal(a, b)
+    assert got == want
+    rc, err = oracle_verify(oracle_params(n, **kw), got, pts, want_evals)
+    assert rc == 0, err
+
+
+def test_config1_n18_matches_oracle():
+    # n = 18, folding 4, rate 1/2, rs-reduction 3, lambda 100, one seeded constraint
+    W = product()
+    poly, pts = inputs(18, 1)
+    cfg = product_cfg(18)
+    got, _ = W.prove_once(cfg, poly, pts)
+    want, evals, _ = oracle_prove(oracle_params(18), poly, pts)
+    assert got == want
+    assert oracle_verify(oracle_params(18), got, pts, evals)[0] == 0
+
+
+def test_device_resident_input_and_repeat():
+    W = product()
+    n = 14
+    poly, pts = inputs(n, 2, seed=3)
+    cfg = product_cfg(n)
+    d = W.DeviceBuffer.from_array(poly)
+    a, _ = W.prove_once(cfg, poly, pts)
+    b, _ = W.prove_once(cfg, None, pts, device_ptr=d.ptr)
+    c, _ = W.prove_once(cfg, None, pts, device_ptr=d.ptr)
+    assert a == b == c
+
+
+def test_config2_n24_matches_oracle():
+    # the 2^24-variable benchmark proof: two seeded constraints, Poseidon2-16 Merkle
+    W = product()
+    n = 24
+    poly = W.sample_field(0, 1 << n)
+    pts = [W.sample_field(1 + j, 4 * n).reshape(n, 4) for j in range(2)]
+    cfg = product_cfg(n)
+    d = W.DeviceBuffer.from_array(poly)
+    got, evals = W.prove_once(cfg, None, pts, device_ptr=d.ptr)
+    want, want_evals, _ = oracle_prove(oracle_params(n), poly, pts)
+    assert all(np.array_equal(a, b) for a, b in zip(evals, want_evals))
+    assert got == want
+    assert oracle_verify(oracle_params(n), got, pts, want_evals)[0] == 0

@@ -1,0 +1,321 @@
+"""Python bindings (ctypes) for the MI355X WHIR prover library.
+
+The prover is native: HIP kernels plus a C++ host behind the C ABI declared in
+include/whir_gpu.h.  This module only loads that library and wraps its entry
+points with the names of the reference crate's API (tcoratger/whir-p3):
+``WhirConfig``, ``DomainSeparator``, ``DuplexChallenger``, ``InitialStatement``,
+``CommitmentWriter.commit`` and ``Prover.prove``.  There is no fallback: if
+libwhirgpu.so is missing, importing the library raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwhirgpu.so")
+P = 0x7F000001
+
+U32P = ctypes.POINTER(ctypes.c_uint32)
+
+
+class Params(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_uint32) for name in (
+        "num_variables", "security_level", "pow_bits", "folding_factor_first", "folding_factor",
+        "starting_log_inv_rate", "rs_domain_initial_reduction_factor", "soundness_type",
+        "sumcheck_strategy", "leaf_hash_width", "digest_elems")]
+
+
+class Poseidon2Constants(ctypes.Structure):
+    _fields_ = [
+        ("w16_external_initial", ctypes.c_uint32 * 64),
+        ("w16_external_terminal", ctypes.c_uint32 * 64),
+        ("w16_internal", ctypes.c_uint32 * 20),
+        ("w24_external_initial", ctypes.c_uint32 * 96),
+        ("w24_external_terminal", ctypes.c_uint32 * 96),
+        ("w24_internal", ctypes.c_uint32 * 23),
+    ]
+
+
+class ChallengerState(ctypes.Structure):
+    _fields_ = [("sponge", ctypes.c_uint32 * 16), ("input", ctypes.c_uint32 * 8), ("output", ctypes.c_uint32 * 8),
+                ("input_len", ctypes.c_uint32), ("output_len", ctypes.c_uint32)]
+
+
+SOUNDNESS = {"UniqueDecoding": 0, "JohnsonBound": 1, "CapacityBound": 2}
+
+# Function signatures of include/whir_gpu.h: name -> (restype, argtypes)
+_VP = ctypes.c_void_p
+_SIGS = {
+    "whirgpu_last_error": (ctypes.c_char_p, []),
+    "whirgpu_poseidon2_from_seed": (ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(Poseidon2Constants)]),
+    "whirgpu_sample_field": (ctypes.c_int, [ctypes.c_uint64, U32P, ctypes.c_size_t]),
+    "whirgpu_poseidon2_permute": (ctypes.c_int, [ctypes.POINTER(Poseidon2Constants), ctypes.c_uint32, U32P]),
+    "whirgpu_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Params), ctypes.POINTER(Poseidon2Constants),
+                                      ctypes.POINTER(_VP)]),
+    "whirgpu_destroy": (ctypes.c_int, [_VP]),
+    "whirgpu_config": (ctypes.c_int, [_VP, U32P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "whirgpu_domain_separator": (ctypes.c_int, [_VP, U32P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "whirgpu_challenger_new": (ctypes.c_int, [ctypes.POINTER(ChallengerState)]),
+    "whirgpu_challenger_observe": (ctypes.c_int, [_VP, ctypes.POINTER(ChallengerState), U32P, ctypes.c_size_t]),
+    "whirgpu_challenger_observe_domain_separator": (ctypes.c_int, [_VP, ctypes.POINTER(ChallengerState)]),
+    "whirgpu_challenger_sample": (ctypes.c_int, [_VP, ctypes.POINTER(ChallengerState), U32P, ctypes.c_size_t]),
+    "whirgpu_statement_create": (ctypes.c_int, [_VP, _VP, ctypes.c_int, ctypes.POINTER(_VP)]),
+    "whirgpu_statement_evaluate": (ctypes.c_int, [_VP, U32P, U32P]),
+    "whirgpu_statement_destroy": (ctypes.c_int, [_VP]),
+    "whirgpu_proof_create": (ctypes.c_int, [_VP, ctypes.POINTER(_VP)]),
+    "whirgpu_proof_destroy": (ctypes.c_int, [_VP]),
+    "whirgpu_proof_serialize": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_uint8), ctypes.c_size_t,
+                                               ctypes.POINTER(ctypes.c_size_t)]),
+    "whirgpu_commit": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ChallengerState), _VP, ctypes.POINTER(_VP)]),
+    "whirgpu_prove": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ChallengerState), _VP, _VP]),
+    "whirgpu_prover_data_destroy": (ctypes.c_int, [_VP]),
+    "whirgpu_last_timings": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double)]),
+    "whirgpu_device_alloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(_VP)]),
+    "whirgpu_device_free": (ctypes.c_int, [ctypes.c_int, _VP]),
+    "whirgpu_memcpy": (ctypes.c_int, [ctypes.c_int, _VP, _VP, ctypes.c_size_t, ctypes.c_int]),
+    "whirgpu_lde_base": (ctypes.c_int, [_VP, _VP, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _VP]),
+    "whirgpu_lde_ext": (ctypes.c_int, [_VP, _VP, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _VP]),
+    "whirgpu_merkle_commit": (ctypes.c_int, [_VP, _VP, ctypes.c_uint64, ctypes.c_uint32, _VP, U32P]),
+    "whirgpu_grind": (ctypes.c_int, [_VP, ctypes.POINTER(ChallengerState), ctypes.c_uint32, U32P]),
+    "whirgpu_eq_expand": (ctypes.c_int, [_VP, U32P, ctypes.c_uint32, U32P, _VP]),
+    "whirgpu_mle_eval_base": (ctypes.c_int, [_VP, _VP, ctypes.c_uint32, U32P, U32P]),
+    "whirgpu_mle_eval_ext": (ctypes.c_int, [_VP, _VP, ctypes.c_uint32, U32P, U32P]),
+    "whirgpu_sumcheck_coeffs": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_uint32, U32P]),
+    "whirgpu_sumcheck_fold": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_uint32, U32P, U32P]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded native library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"native library missing: {LIB_PATH} (run whir-p3_amd/build.py)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class WhirError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc != 0:
+        raise WhirError(f"whirgpu error {rc}: {lib().whirgpu_last_error().decode()}")
+
+
+def u32p(a):
+    return a.ctypes.data_as(U32P)
+
+
+def params(num_variables, *, security_level=100, pow_bits=16, folding_factor=4, folding_factor_first=None,
+           starting_log_inv_rate=1, rs_domain_initial_reduction_factor=3, soundness="CapacityBound",
+           svo=True, leaf_hash_width=16, digest_elems=8):
+    """ProtocolParameters + num_variables, with the 2^24 benchmark's defaults."""
+    return Params(num_variables, security_level, pow_bits,
+                  folding_factor if folding_factor_first is None else folding_factor_first, folding_factor,
+                  starting_log_inv_rate, rs_domain_initial_reduction_factor,
+                  SOUNDNESS.get(soundness, soundness), 1 if svo else 0, leaf_hash_width, digest_elems)
+
+
+def poseidon2_from_seed(seed=1):
+    c = Poseidon2Constants()
+    check(lib().whirgpu_poseidon2_from_seed(seed, ctypes.byref(c)))
+    return c
+
+
+def sample_field(seed, count):
+    """`count` draws of rng.random::<KoalaBear>() from SmallRng::seed_from_u64(seed), Montgomery form."""
+    out = np.empty(count, dtype=np.uint32)
+    check(lib().whirgpu_sample_field(seed, u32p(out), count))
+    return out
+
+
+class DeviceBuffer:
+    """A device allocation owned by Python (inputs kept resident in HBM)."""
+
+    def __init__(self, nbytes, device=0):
+        self.device = device
+        self.nbytes = nbytes
+        p = ctypes.c_void_p()
+        check(lib().whirgpu_device_alloc(device, nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, a, device=0):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes, device)
+        b.upload(a)
+        return b
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib().whirgpu_memcpy(self.device, ctypes.c_void_p(self.ptr), a.ctypes.data_as(ctypes.c_void_p),
+                                   a.nbytes, 0))
+
+    def download(self, dtype=np.uint32, count=None):
+        dt = np.dtype(dtype)
+        count = self.nbytes // dt.itemsize if count is None else count
+        out = np.empty(count, dtype=dt)
+        check(lib().whirgpu_memcpy(self.device, out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(self.ptr),
+                                   out.nbytes, 1))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().whirgpu_device_free(self.device, ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class WhirConfig:
+    """WhirConfig::new + the device context.  device=-1 gives a host-only context
+    (schedule and transcript only)."""
+
+    def __init__(self, prm, perm_seed=1, device=0):
+        self.params = prm
+        self.perms = poseidon2_from_seed(perm_seed)
+        h = ctypes.c_void_p()
+        check(lib().whirgpu_create(device, ctypes.byref(prm), ctypes.byref(self.perms), ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().whirgpu_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def schedule(self):
+        out = (ctypes.c_uint32 * 512)()
+        n = ctypes.c_size_t()
+        check(lib().whirgpu_config(self.handle, out, 512, ctypes.byref(n)))
+        return list(out[:n.value])
+
+    def domain_separator(self):
+        out = (ctypes.c_uint32 * 4096)()
+        n = ctypes.c_size_t()
+        check(lib().whirgpu_domain_separator(self.handle, out, 4096, ctypes.byref(n)))
+        return np.array(out[:n.value], dtype=np.uint32)
+
+    def challenger(self, observe_domain_separator=True):
+        ch = ChallengerState()
+        check(lib().whirgpu_challenger_new(ctypes.byref(ch)))
+        if observe_domain_separator:
+            check(lib().whirgpu_challenger_observe_domain_separator(self.handle, ctypes.byref(ch)))
+        return ch
+
+    def timings(self):
+        out = (ctypes.c_double * 7)()
+        check(lib().whirgpu_last_timings(self.handle, out))
+        return list(out)
+
+
+class InitialStatement:
+    """WhirConfig::initial_statement + InitialStatement::evaluate."""
+
+    def __init__(self, cfg, evals=None, device_ptr=None):
+        self.cfg = cfg
+        h = ctypes.c_void_p()
+        if device_ptr is not None:
+            check(lib().whirgpu_statement_create(cfg.handle, ctypes.c_void_p(device_ptr), 1, ctypes.byref(h)))
+        else:
+            self._host = np.ascontiguousarray(evals, dtype=np.uint32)
+            check(lib().whirgpu_statement_create(cfg.handle, self._host.ctypes.data_as(ctypes.c_void_p), 0,
+                                                 ctypes.byref(h)))
+        self.handle = h
+
+    def evaluate(self, point):
+        pt = np.ascontiguousarray(point, dtype=np.uint32).reshape(-1)
+        out = np.empty(4, dtype=np.uint32)
+        check(lib().whirgpu_statement_evaluate(self.handle, u32p(pt), u32p(out)))
+        return out
+
+    def close(self):
+        if self.handle:
+            lib().whirgpu_statement_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class WhirProof:
+    def __init__(self, cfg):
+        h = ctypes.c_void_p()
+        check(lib().whirgpu_proof_create(cfg.handle, ctypes.byref(h)))
+        self.handle = h
+
+    def to_bytes(self):
+        n = ctypes.c_size_t()
+        rc = lib().whirgpu_proof_serialize(self.handle, None, 0, ctypes.byref(n))
+        buf = (ctypes.c_uint8 * n.value)()
+        check(lib().whirgpu_proof_serialize(self.handle, buf, n.value, ctypes.byref(n)))
+        del rc
+        return bytes(buf)
+
+    def close(self):
+        if self.handle:
+            lib().whirgpu_proof_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CommitmentWriter:
+    def __init__(self, cfg):
+        self.cfg = cfg
+
+    def commit(self, proof, challenger, statement):
+        pd = ctypes.c_void_p()
+        check(lib().whirgpu_commit(self.cfg.handle, proof.handle, ctypes.byref(challenger), statement.handle,
+                                   ctypes.byref(pd)))
+        return pd
+
+
+class Prover:
+    def __init__(self, cfg):
+        self.cfg = cfg
+
+    def prove(self, proof, challenger, statement, prover_data):
+        check(lib().whirgpu_prove(self.cfg.handle, proof.handle, ctypes.byref(challenger), statement.handle,
+                                  prover_data))
+
+
+def prove_once(cfg, poly, points=(), device_ptr=None):
+    """Commit + prove as benches/whir.rs does; returns (proof bytes, evaluations)."""
+    st = InitialStatement(cfg, poly, device_ptr=device_ptr)
+    evals = [st.evaluate(p) for p in points]
+    ch = cfg.challenger()
+    proof = WhirProof(cfg)
+    pd = CommitmentWriter(cfg).commit(proof, ch, st)
+    Prover(cfg).prove(proof, ch, st, pd)
+    out = proof.to_bytes()
+    proof.close()
+    st.close()
+    return out, evals

@@ -1,0 +1,191 @@
+// Poseidon2 Merkle commitment of a row-major matrix (MerkleTreeMmcs::commit_matrix,
+// called at src/whir/committer/writer.rs:88-89 and, through ExtensionMmcs with
+// each extension element flattened to its 4 coefficients, at
+// src/whir/prover/mod.rs:182-184), and batched openings (open_batch at
+// src/whir/prover/mod.rs:251-289, 380-400).
+//
+// Leaf digest: PaddingFreeSponge<Poseidon2-LW, LW, LW-8, 8> over the row
+// (overwrite the rate part with each chunk, permute, output state[0..8]).
+// Node digest: TruncatedPermutation<Poseidon2-16, 2, 8, 16> (left || right,
+// permute, keep state[0..8]).
+//
+// The digests of all levels live in one buffer: level 0 (the leaves) first,
+// then each parent level, ending with the root; level j starts at digest
+// 2H - 2H/2^j.  Leaves are hashed one row per lane and each level above is
+// one launch with one compression per lane.
+#include "kcommon.h"
+#include "kernels.h"
+#include "poseidon2.h"
+
+namespace kb {
+
+template <int LW>
+__global__ void __launch_bounds__(256) k_leaf_hash(const u32* __restrict__ rows, int row_len, u64 nrows,
+                                                   u32* __restrict__ dig, P2Consts<LW> c) {
+  constexpr int RATE = LW - 8;
+  u64 i = (u64)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nrows) return;
+  u32 s[LW];
+#pragma unroll
+  for (int j = 0; j < LW; j++) s[j] = 0;
+  const u32* r = rows + i * (u64)row_len;
+  if ((row_len % RATE) == 0) {
+    for (int pos = 0; pos < row_len; pos += RATE) {
+      const uint4* v = reinterpret_cast<const uint4*>(r + pos);
+#pragma unroll
+      for (int q = 0; q < RATE / 4; q++) {
+        uint4 x = v[q];
+        s[4 * q] = x.x;
+        s[4 * q + 1] = x.y;
+        s[4 * q + 2] = x.z;
+        s[4 * q + 3] = x.w;
+      }
+      permute<LW>(s, c);
+    }
+  } else {
+    for (int pos = 0; pos < row_len; pos += RATE) {
+      int cnt = row_len - pos < RATE ? row_len - pos : RATE;
+#pragma unroll
+      for (int j = 0; j < RATE; j++)
+        if (j < cnt) s[j] = r[pos + j];
+      permute<LW>(s, c);
+    }
+  }
+  uint4* o = reinterpret_cast<uint4*>(dig + i * 8);
+  o[0] = make_uint4(s[0], s[1], s[2], s[3]);
+  o[1] = make_uint4(s[4], s[5], s[6], s[7]);
+}
+
+__device__ __forceinline__ void compress_pair(const u32* l, const u32* r, u32* out, const P2C16& c) {
+  u32 s[16];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    s[k] = l[k];
+    s[8 + k] = r[k];
+  }
+  permute<16>(s, c);
+#pragma unroll
+  for (int k = 0; k < 8; k++) out[k] = s[k];
+}
+
+// One tree level: out[j] = compress(in[2j], in[2j+1]).
+__global__ void __launch_bounds__(256) k_tree_level(const u32* __restrict__ in, u64 n_out, u32* __restrict__ out,
+                                                    P2C16 c) {
+  u64 j = (u64)blockIdx.x * 256 + threadIdx.x;
+  if (j >= n_out) return;
+  const uint4* v = reinterpret_cast<const uint4*>(in + 16 * j);
+  u32 s[16];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    uint4 x = v[q];
+    s[4 * q] = x.x;
+    s[4 * q + 1] = x.y;
+    s[4 * q + 2] = x.z;
+    s[4 * q + 3] = x.w;
+  }
+  permute<16>(s, c);
+  uint4* o = reinterpret_cast<uint4*>(out + 8 * j);
+  o[0] = make_uint4(s[0], s[1], s[2], s[3]);
+  o[1] = make_uint4(s[4], s[5], s[6], s[7]);
+}
+
+// Opening of `nq` rows: out[q] = row (row_len) followed by depth sibling digests.
+__global__ void k_open(const u32* __restrict__ rows, int row_len, const u32* __restrict__ tree, u64 height, int depth,
+                       const u64* __restrict__ idx, int nq, u32* __restrict__ out) {
+  const u64 per = (u64)row_len + 8ull * depth;
+  for (u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x; e < per * nq; e += (u64)gridDim.x * blockDim.x) {
+    u64 q = e / per, o = e % per;
+    u64 ix = idx[q];
+    if (o < (u64)row_len) {
+      out[e] = rows[ix * row_len + o];
+    } else {
+      u64 l = (o - row_len) / 8, k = (o - row_len) % 8;
+      u64 off = 2 * height - ((2 * height) >> l);
+      u64 node = (ix >> l) ^ 1;
+      out[e] = tree[(off + node) * 8 + k];
+    }
+  }
+}
+
+template <int LW>
+int merkle_leaves_impl(const u32* rows, int row_len, u64 height, u32* tree, const P2Consts<LW>& c, hipStream_t st) {
+  unsigned blocks = (unsigned)((height + 255) / 256);
+  hipLaunchKernelGGL((k_leaf_hash<LW>), dim3(blocks), dim3(256), 0, st, rows, row_len, height, tree, c);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+int merkle_commit(const u32* rows, int row_len, u64 height, int leaf_width, const P2C16& c16, const P2C24& c24,
+                  u32* tree, hipStream_t st) {
+  if (height == 0 || (height & (height - 1))) return -1;
+  int rc = leaf_width == 24 ? merkle_leaves_impl<24>(rows, row_len, height, tree, c24, st)
+                            : merkle_leaves_impl<16>(rows, row_len, height, tree, c16, st);
+  if (rc) return rc;
+  u64 off = 0, n = height;
+  while (n > 1) {
+    u64 n_out = n >> 1;
+    unsigned blocks = (unsigned)((n_out + 255) / 256);
+    hipLaunchKernelGGL(k_tree_level, dim3(blocks), dim3(256), 0, st, tree + off * 8, n_out, tree + (off + n) * 8, c16);
+    if (hipGetLastError() != hipSuccess) return -5;
+    off += n;
+    n = n_out;
+  }
+  return 0;
+}
+
+int merkle_open(const u32* rows, int row_len, const u32* tree, u64 height, const u64* d_idx, int nq, u32* d_out,
+                hipStream_t st) {
+  int depth = 0;
+  while ((1ull << depth) < height) depth++;
+  u64 total = ((u64)row_len + 8ull * depth) * nq;
+  unsigned blocks = (unsigned)((total + 255) / 256);
+  if (blocks == 0) return 0;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_open, dim3(blocks), dim3(256), 0, st, rows, row_len, tree, height, depth, d_idx, nq, d_out);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+// ------------------------------------------------------------------ grinding
+// GrindingChallenger::grind for DuplexChallenger (used at src/whir/proof.rs:187-189,
+// src/whir/prover/mod.rs:223-225, 362-364): the smallest w in [0, p) such that,
+// after observing w, sample_bits(bits) == 0.  Observing w with L < 8 buffered
+// inputs overwrites state[0..L] with the buffer and state[L] with w, and the
+// next sample is state[7] of the permuted state, so each candidate costs one
+// permutation.  Candidates are checked in rounds of gridDim*256; a block stops
+// once the best witness found so far is below its round, so every candidate
+// smaller than the result has been checked.
+struct Sponge16 {
+  u32 s[16];
+};
+
+__global__ void __launch_bounds__(256) k_grind(P2C16 c, Sponge16 base, int wpos, u32 mask, u32* result, u32 limit) {
+  __shared__ u32 found_sh;
+  const u64 per_round = (u64)gridDim.x * 256;
+  for (u64 start = 0; start < limit; start += per_round) {
+    if (threadIdx.x == 0) found_sh = __hip_atomic_load(result, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    u32 f = found_sh;
+    __syncthreads();
+    if ((u64)f <= start) break;
+    u64 w = start + (u64)blockIdx.x * 256 + threadIdx.x;
+    if (w < limit) {
+      u32 s[16];
+#pragma unroll
+      for (int k = 0; k < 16; k++) s[k] = base.s[k];
+#pragma unroll
+      for (int k = 0; k < 16; k++)
+        if (k == wpos) s[k] = to_monty((u32)w);
+      permute<16>(s, c);
+      if ((from_monty(s[7]) & mask) == 0) atomicMin(result, (u32)w);
+    }
+  }
+}
+
+int grind(const P2C16& c, const u32* state16, int wpos, int bits, u32* d_result, int num_blocks, hipStream_t st) {
+  Sponge16 b;
+  for (int k = 0; k < 16; k++) b.s[k] = state16[k];
+  u32 mask = bits >= 32 ? 0xffffffffu : ((1u << bits) - 1);
+  hipLaunchKernelGGL(k_grind, dim3(num_blocks), dim3(256), 0, st, c, b, wpos, mask, d_result, P);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+}  // namespace kb

@@ -1,0 +1,315 @@
+// Multilinear kernels on evaluation lists (index bit n-1-j <-> variable j):
+//   eq tables            EvaluationsList::new_from_point (src/poly/evals.rs:312-320)
+//   eq-weighted sums     eval_multilinear_{base,ext} (evals.rs:648-738),
+//                        SplitEqInner::partial_evals (src/sumcheck/svo.rs:465-502)
+//   weight accumulation  EqStatement::combine_hypercube_packed (statement/eq.rs:248-312),
+//                        SplitEq::combine_into_packed (svo.rs:504-559),
+//                        SelectStatement::combine_packed (statement/select.rs:427-496)
+//   multi-variable fold  compress_multi_into_packed (evals.rs:422-447)
+//   sumcheck round       sumcheck_coefficients (evals.rs:493-526) and compress
+//                        (evals.rs:127-148), fused as fold(r) -> next (h(0), h(2))
+//
+// eq(z, x) is always split as Ehi[x >> lo] * Elo[x & (2^lo - 1)], where Ehi and
+// Elo are the eq tables of the leading and trailing coordinates of z; both are
+// small and built on the device from the point, which arrives by value.
+#include "kcommon.h"
+#include "kernels.h"
+
+namespace kb {
+
+// ---------------------------------------------------------------- eq tables
+// out[x] = scale * prod_j (x_j ? z_j : 1 - z_j), x < 2^t, z_0 on the MSB.
+__global__ void k_eq_small(PointArg z, int first, int t, ef scale, ef* __restrict__ out) {
+  u32 x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= (1u << t)) return;
+  ef acc = scale;
+  for (int j = 0; j < t; j++) {
+    const ef& zj = z.z[first + j];
+    int bit = (x >> (t - 1 - j)) & 1;
+    acc = ef_mul(acc, bit ? zj : ef_sub(ef_one(), zj));
+  }
+  out[x] = acc;
+}
+
+int eq_table(const PointArg& z, int first, int t, const ef& scale, ef* out, hipStream_t st) {
+  u32 n = 1u << t;
+  hipLaunchKernelGGL(k_eq_small, dim3((n + 255) / 256), dim3(256), 0, st, z, first, t, scale, out);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+// Full table out[x] = Ehi[x >> lo] * Elo[x & mask] for x < 2^(hi+lo).
+__global__ void k_eq_outer(const ef* __restrict__ Ehi, const ef* __restrict__ Elo, int lo, u64 n, ef* __restrict__ out) {
+  for (u64 x = (u64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (u64)gridDim.x * blockDim.x)
+    out[x] = ef_mul(Ehi[x >> lo], Elo[x & ((1u << lo) - 1)]);
+}
+
+int eq_outer(const ef* Ehi, const ef* Elo, int lo, u64 n, ef* out, hipStream_t st) {
+  u64 blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_eq_outer, dim3((unsigned)blocks), dim3(256), 0, st, Ehi, Elo, lo, n, out);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+// ---------------------------------------------------------------- eq-weighted sums
+// Segmented dot product of f with K split eq tables.
+// f has 2^nf entries in 2^sb segments of 2^(nf - sb); within a segment,
+// x = xh * 2^lo + xl and the weight of table k is Ehi[k][xh] * Elo[k][xl].
+// Thread t owns xl = t mod 2^lo and rows xh = (t >> lo) + i * (256 >> lo).
+// partial[(seg * gridDim.x + block) * K + k] receives the block's sum.
+template <typename T, int K>
+__global__ void __launch_bounds__(256) k_eq_dot(const T* __restrict__ f, int nf, int sb, int lo, EqTables tabs,
+                                                ef* __restrict__ partial) {
+  __shared__ ef sh[256 * K];
+  const int t = threadIdx.x;
+  const int seg = blockIdx.y;
+  const int nrest = nf - sb;
+  const u64 nrows = 1ull << (nrest - lo);
+  const int rows_per_iter = 256 >> lo;
+  const u32 xl = t & ((1u << lo) - 1);
+  const u64 rows_per_block = (nrows + gridDim.x - 1) / gridDim.x;
+  const u64 r0 = (u64)blockIdx.x * rows_per_block;
+  u64 r1 = r0 + rows_per_block;
+  if (r1 > nrows) r1 = nrows;
+  const T* fs = f + ((u64)seg << nrest);
+  ef acc[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) acc[k] = ef_zero();
+  for (u64 r = r0 + (t >> lo); r < r1; r += rows_per_iter) {
+    T v = fs[(r << lo) + xl];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      ef e = tabs.hi[k][r];
+      if constexpr (sizeof(T) == 4) acc[k] = ef_add(acc[k], ef_mul_base(e, v));
+      else acc[k] = ef_add(acc[k], ef_mul(e, v));
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++) acc[k] = ef_mul(acc[k], tabs.lo[k][xl]);
+  block_reduce_ef<K>(acc, sh);
+  if (t == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) partial[((u64)seg * gridDim.x + blockIdx.x) * K + k] = acc[k];
+  }
+}
+
+// out[seg * K + k] = sum_b partial[(seg * nb + b) * K + k]; one block per (seg, k).
+__global__ void __launch_bounds__(256) k_reduce_strided(const ef* __restrict__ partial, int nb, int K,
+                                                        ef* __restrict__ out) {
+  __shared__ ef sh[256];
+  const int seg = blockIdx.x / K, k = blockIdx.x % K;
+  ef acc[1] = {ef_zero()};
+  for (int b = threadIdx.x; b < nb; b += 256) acc[0] = ef_add(acc[0], partial[((u64)seg * nb + b) * K + k]);
+  block_reduce_ef<1>(acc, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = acc[0];
+}
+
+template <typename T>
+static int eq_dot_impl(const T* f, int nf, int sb, int lo, const EqTables& tabs, int K, ef* partial, int nblocks,
+                       hipStream_t st) {
+  dim3 g(nblocks, 1u << sb);
+  switch (K) {
+    case 1: hipLaunchKernelGGL((k_eq_dot<T, 1>), g, dim3(256), 0, st, f, nf, sb, lo, tabs, partial); break;
+    case 2: hipLaunchKernelGGL((k_eq_dot<T, 2>), g, dim3(256), 0, st, f, nf, sb, lo, tabs, partial); break;
+    case 3: hipLaunchKernelGGL((k_eq_dot<T, 3>), g, dim3(256), 0, st, f, nf, sb, lo, tabs, partial); break;
+    case 4: hipLaunchKernelGGL((k_eq_dot<T, 4>), g, dim3(256), 0, st, f, nf, sb, lo, tabs, partial); break;
+    default: return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+// Result layout: out[(seg * K) + k].  `partial` needs nblocks * 2^sb * K entries.
+int eq_dot_base(const u32* f, int nf, int sb, int lo, const EqTables& tabs, int K, ef* partial, int nblocks, ef* out,
+                hipStream_t st) {
+  int rc = eq_dot_impl<u32>(f, nf, sb, lo, tabs, K, partial, nblocks, st);
+  if (rc) return rc;
+  // partial is [seg][block][k]; reduce over blocks for each (seg, k)
+  hipLaunchKernelGGL(k_reduce_strided, dim3((1u << sb) * K), dim3(256), 0, st, partial, nblocks, K, out);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+int eq_dot_ext(const ef* f, int nf, int sb, int lo, const EqTables& tabs, int K, ef* partial, int nblocks, ef* out,
+               hipStream_t st) {
+  int rc = eq_dot_impl<ef>(f, nf, sb, lo, tabs, K, partial, nblocks, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_reduce_strided, dim3((1u << sb) * K), dim3(256), 0, st, partial, nblocks, K, out);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+// ---------------------------------------------------------------- weight accumulation
+// W[x] = (acc ? W[x] : 0) + sum_k Ehi[k][x >> lo] * Elo[k][x & mask]
+__global__ void __launch_bounds__(256) k_eq_combine(ef* __restrict__ Wt, u64 n, int lo, EqTables tabs, int K,
+                                                    int accumulate) {
+  const u32 mask = (1u << lo) - 1;
+  for (u64 x = (u64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (u64)gridDim.x * blockDim.x) {
+    ef acc = accumulate ? Wt[x] : ef_zero();
+    for (int k = 0; k < K; k++) acc = ef_add(acc, ef_mul(tabs.hi[k][x >> lo], tabs.lo[k][x & mask]));
+    Wt[x] = acc;
+  }
+}
+
+int eq_combine(ef* Wt, u64 n, int lo, const EqTables& tabs, int K, int accumulate, hipStream_t st) {
+  u64 blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_eq_combine, dim3((unsigned)blocks), dim3(256), 0, st, Wt, n, lo, tabs, K, accumulate);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+// Select tables for z_j in F and weight g_j in EF:
+//   A[j][xh] = g_j * (z_j^(2^lo))^xh  (EF),   B[j][xl] = z_j^xl  (F)
+__global__ void k_select_tables(const u32* __restrict__ z, const ef* __restrict__ g, int nsel, int hi, int lo,
+                                ef* __restrict__ A, u32* __restrict__ B) {
+  const u32 nh = 1u << hi, nl = 1u << lo;
+  u64 total = (u64)nsel * (nh + nl);
+  for (u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (u64)gridDim.x * blockDim.x) {
+    u32 j = (u32)(e / (nh + nl)), r = (u32)(e % (nh + nl));
+    if (r < nh) {
+      u32 zz = pow(z[j], (u64)r << lo);
+      A[(u64)j * nh + r] = ef_mul_base(g[j], zz);
+    } else {
+      B[(u64)j * nl + (r - nh)] = pow(z[j], r - nh);
+    }
+  }
+}
+
+// W[x] += sum_j A[j][x >> lo] * B[j][x & mask]
+__global__ void __launch_bounds__(256) k_select_combine(ef* __restrict__ Wt, u64 n, int lo, int hi,
+                                                        const ef* __restrict__ A, const u32* __restrict__ B,
+                                                        int nsel) {
+  const u32 mask = (1u << lo) - 1;
+  const u64 nh = 1ull << hi, nl = 1ull << lo;
+  for (u64 x = (u64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (u64)gridDim.x * blockDim.x) {
+    ef acc = Wt[x];
+    const u64 xh = x >> lo, xl = x & mask;
+    for (int j = 0; j < nsel; j++) acc = ef_add(acc, ef_mul_base(A[j * nh + xh], B[j * nl + xl]));
+    Wt[x] = acc;
+  }
+}
+
+int select_combine(ef* Wt, int m, const u32* d_z, const ef* d_g, int nsel, ef* A, u32* B, hipStream_t st) {
+  if (nsel == 0) return 0;
+  int lo = m / 2, hi = m - lo;
+  u64 tt = (u64)nsel * ((1ull << hi) + (1ull << lo));
+  u64 b1 = (tt + 255) / 256;
+  if (b1 > 65536) b1 = 65536;
+  hipLaunchKernelGGL(k_select_tables, dim3((unsigned)b1), dim3(256), 0, st, d_z, d_g, nsel, hi, lo, A, B);
+  u64 n = 1ull << m;
+  u64 blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_select_combine, dim3((unsigned)blocks), dim3(256), 0, st, Wt, n, lo, hi, A, B, nsel);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+// ---------------------------------------------------------------- folds
+// out[x] = sum_s eqrs[s] * f[s * 2^nr + x], s < 2^l
+__global__ void __launch_bounds__(256) k_compress_multi(const u32* __restrict__ f, int nr, int l,
+                                                        const ef* __restrict__ eqrs, ef* __restrict__ out) {
+  const u64 n = 1ull << nr;
+  for (u64 x = (u64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (u64)gridDim.x * blockDim.x) {
+    ef acc = ef_zero();
+    for (int s = 0; s < (1 << l); s++) acc = ef_add(acc, ef_mul_base(eqrs[s], f[((u64)s << nr) + x]));
+    out[x] = acc;
+  }
+}
+
+int compress_multi(const u32* f, int n, int l, const ef* d_eqrs, ef* out, hipStream_t st) {
+  u64 m = 1ull << (n - l);
+  u64 blocks = (m + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_compress_multi, dim3((unsigned)blocks), dim3(256), 0, st, f, n - l, l, d_eqrs, out);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+__global__ void k_base_to_ext(const u32* __restrict__ f, u64 n, ef* __restrict__ out) {
+  for (u64 x = (u64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (u64)gridDim.x * blockDim.x)
+    out[x] = ef_base(f[x]);
+}
+
+int base_to_ext(const u32* f, u64 n, ef* out, hipStream_t st) {
+  u64 blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_base_to_ext, dim3((unsigned)blocks), dim3(256), 0, st, f, n, out);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+// ---------------------------------------------------------------- sumcheck rounds
+// (h(0), h(2)) partial sums over pairs (i, i + h) of e and w (length 2h).
+__global__ void __launch_bounds__(256) k_sc_coeffs(const ef* __restrict__ e, const ef* __restrict__ w, u64 h,
+                                                   ef* __restrict__ partial) {
+  __shared__ ef sh[512];
+  ef acc[2] = {ef_zero(), ef_zero()};
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < h; i += (u64)gridDim.x * blockDim.x) {
+    ef e0 = e[i], e1 = e[i + h], w0 = w[i], w1 = w[i + h];
+    acc[0] = ef_add(acc[0], ef_mul(w0, e0));
+    acc[1] = ef_add(acc[1], ef_mul(ef_sub(ef_dbl(w1), w0), ef_sub(ef_dbl(e1), e0)));
+  }
+  block_reduce_ef<2>(acc, sh);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = acc[0];
+    partial[2 * blockIdx.x + 1] = acc[1];
+  }
+}
+
+// Fold e and w (length 2h) with r into their first halves, and accumulate the
+// next round's (h(0), h(2)) over the folded halves (pairs (j, j + h/2)).
+__global__ void __launch_bounds__(256) k_sc_fold_coeffs(ef* __restrict__ e, ef* __restrict__ w, u64 h, ef r,
+                                                        ef* __restrict__ partial) {
+  __shared__ ef sh[512];
+  ef acc[2] = {ef_zero(), ef_zero()};
+  const u64 q = h >> 1;
+  for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < q; j += (u64)gridDim.x * blockDim.x) {
+    ef a0 = e[j], a1 = e[j + h], b0 = e[j + q], b1 = e[j + q + h];
+    ef f0 = ef_add(a0, ef_mul(r, ef_sub(a1, a0)));
+    ef f1 = ef_add(b0, ef_mul(r, ef_sub(b1, b0)));
+    ef c0 = w[j], c1 = w[j + h], d0 = w[j + q], d1 = w[j + q + h];
+    ef g0 = ef_add(c0, ef_mul(r, ef_sub(c1, c0)));
+    ef g1 = ef_add(d0, ef_mul(r, ef_sub(d1, d0)));
+    e[j] = f0;
+    e[j + q] = f1;
+    w[j] = g0;
+    w[j + q] = g1;
+    acc[0] = ef_add(acc[0], ef_mul(g0, f0));
+    acc[1] = ef_add(acc[1], ef_mul(ef_sub(ef_dbl(g1), g0), ef_sub(ef_dbl(f1), f0)));
+  }
+  block_reduce_ef<2>(acc, sh);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = acc[0];
+    partial[2 * blockIdx.x + 1] = acc[1];
+  }
+}
+
+__global__ void k_sc_fold(ef* __restrict__ e, ef* __restrict__ w, u64 h, ef r) {
+  for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < h; j += (u64)gridDim.x * blockDim.x) {
+    ef a0 = e[j], a1 = e[j + h];
+    e[j] = ef_add(a0, ef_mul(r, ef_sub(a1, a0)));
+    ef c0 = w[j], c1 = w[j + h];
+    w[j] = ef_add(c0, ef_mul(r, ef_sub(c1, c0)));
+  }
+}
+
+static unsigned sc_blocks(u64 work) {
+  u64 b = (work + 255) / 256;
+  if (b > 1024) b = 1024;
+  return b ? (unsigned)b : 1u;
+}
+
+// c0c2[0..2] = (h(0), h(2)) for arrays of length 2h.  Uses partial[2 * 1024].
+int sc_coeffs(const ef* e, const ef* w, u64 h, ef* partial, ef* c0c2, hipStream_t st) {
+  unsigned nb = sc_blocks(h);
+  hipLaunchKernelGGL(k_sc_coeffs, dim3(nb), dim3(256), 0, st, e, w, h, partial);
+  hipLaunchKernelGGL(k_reduce_strided, dim3(2), dim3(256), 0, st, partial, (int)nb, 2, c0c2);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+// Fold arrays of length 2h with r; if h >= 2 also produce the next (h(0), h(2)).
+int sc_fold(ef* e, ef* w, u64 h, const ef& r, ef* partial, ef* c0c2, hipStream_t st) {
+  if (h >= 2) {
+    unsigned nb = sc_blocks(h >> 1);
+    hipLaunchKernelGGL(k_sc_fold_coeffs, dim3(nb), dim3(256), 0, st, e, w, h, r, partial);
+    hipLaunchKernelGGL(k_reduce_strided, dim3(2), dim3(256), 0, st, partial, (int)nb, 2, c0c2);
+  } else {
+    hipLaunchKernelGGL(k_sc_fold, dim3(sc_blocks(h)), dim3(256), 0, st, e, w, h, r);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+}  // namespace kb

@@ -1,0 +1,54 @@
+// Host-side launchers of the device kernels.  Every launcher enqueues on the
+// given stream and returns 0 or a negative code (shape not supported or launch
+// failure); none of them synchronizes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "field.h"
+#include "kcommon.h"
+#include "poseidon2.h"
+
+namespace kb {
+
+// Point of up to 32 extension coordinates, passed by value to kernels.
+struct PointArg {
+  ef z[32];
+};
+
+// Up to four split eq tables: eq(z, x) = hi[k][x >> lo] * lo[k][x & (2^lo - 1)].
+struct EqTables {
+  const ef* hi[4];
+  const ef* lo[4];
+};
+
+// ---- Reed-Solomon encoding (k_lde.hip)
+// src holds 2^m elements; the output is 2^(m-k+R) rows of 2^k elements.
+// tmp (same size as the output) is needed when 2^(m-k) exceeds one LDS pass.
+bool lde_two_pass(int m, int k, bool ext);
+int lde_base(const u32* src, u32* dst, u32* tmp, int m, int k, int R, const Twiddles& tw, hipStream_t st);
+int lde_ext(const ef* src, ef* dst, ef* tmp, int m, int k, int R, const Twiddles& tw, hipStream_t st);
+
+// ---- Merkle tree and openings (k_merkle.hip)
+// tree holds 2 * height - 1 digests of 8 u32; the root is the last one.
+int merkle_commit(const u32* rows, int row_len, u64 height, int leaf_width, const P2C16& c16, const P2C24& c24,
+                  u32* tree, hipStream_t st);
+int merkle_open(const u32* rows, int row_len, const u32* tree, u64 height, const u64* d_idx, int nq, u32* d_out,
+                hipStream_t st);
+// *d_result must hold p before the launch; it receives the smallest witness.
+int grind(const P2C16& c, const u32* state16, int wpos, int bits, u32* d_result, int num_blocks, hipStream_t st);
+
+// ---- multilinear kernels (k_poly.hip)
+int eq_table(const PointArg& z, int first, int t, const ef& scale, ef* out, hipStream_t st);
+int eq_outer(const ef* Ehi, const ef* Elo, int lo, u64 n, ef* out, hipStream_t st);
+int eq_dot_base(const u32* f, int nf, int sb, int lo, const EqTables& tabs, int K, ef* partial, int nblocks, ef* out,
+                hipStream_t st);
+int eq_dot_ext(const ef* f, int nf, int sb, int lo, const EqTables& tabs, int K, ef* partial, int nblocks, ef* out,
+               hipStream_t st);
+int eq_combine(ef* Wt, u64 n, int lo, const EqTables& tabs, int K, int accumulate, hipStream_t st);
+int select_combine(ef* Wt, int m, const u32* d_z, const ef* d_g, int nsel, ef* A, u32* B, hipStream_t st);
+int compress_multi(const u32* f, int n, int l, const ef* d_eqrs, ef* out, hipStream_t st);
+int base_to_ext(const u32* f, u64 n, ef* out, hipStream_t st);
+int sc_coeffs(const ef* e, const ef* w, u64 h, ef* partial, ef* c0c2, hipStream_t st);
+int sc_fold(ef* e, ef* w, u64 h, const ef& r, ef* partial, ef* c0c2, hipStream_t st);
+
+}  // namespace kb

@@ -1,0 +1,1095 @@
+// Device-resident WHIR commit/prove behind the C ABI of include/whir_gpu.h.
+//
+// The host runs the transcript (DuplexChallenger), the parameter schedule and
+// the O(2^folding_factor)-sized steps; every pass over a polynomial, codeword
+// or Merkle tree runs on the device and stays resident in HBM between steps.
+// Per sumcheck round only the two round-polynomial values (32 bytes) come
+// back; the challenge goes down as a kernel argument.
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/whir_gpu.h"
+#include "host.h"
+#include "kernels.h"
+
+using namespace kb;
+using whir::Digest;
+using whir::DuplexChallenger;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Fail {
+  int code;
+};
+
+#define HIPCHK(x)                                                                   \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess) {                                                         \
+      g_err = std::string(#x) + ": " + hipGetErrorString(e_);                       \
+      throw Fail{e_ == hipErrorOutOfMemory ? WHIRGPU_OOM : WHIRGPU_DEVICE};          \
+    }                                                                               \
+  } while (0)
+
+#define KCHK(x)                                                  \
+  do {                                                           \
+    int r_ = (x);                                                \
+    if (r_) {                                                    \
+      g_err = std::string(#x) + " failed: " + std::to_string(r_); \
+      throw Fail{WHIRGPU_DEVICE};                                \
+    }                                                            \
+  } while (0)
+
+void bad(const std::string& msg, int code = WHIRGPU_BAD_PARAMS) {
+  g_err = msg;
+  throw Fail{code};
+}
+
+// Device allocations recycled by exact size, so repeated proofs of one shape
+// allocate nothing after the first.
+struct Pool {
+  std::multimap<size_t, void*> free_;
+  std::vector<void*> all_;
+  void* get(size_t bytes) {
+    bytes = (bytes + 255) & ~size_t(255);
+    auto it = free_.find(bytes);
+    if (it != free_.end()) {
+      void* p = it->second;
+      free_.erase(it);
+      return p;
+    }
+    void* p = nullptr;
+    HIPCHK(hipMalloc(&p, bytes));
+    all_.push_back(p);
+    return p;
+  }
+  void put(void* p, size_t bytes) {
+    if (p) free_.emplace((bytes + 255) & ~size_t(255), p);
+  }
+  void release_all() {
+    for (void* p : all_) (void)hipFree(p);
+    all_.clear();
+    free_.clear();
+  }
+};
+
+template <typename T>
+struct Buf {
+  T* p = nullptr;
+  size_t n = 0;
+};
+
+}  // namespace
+
+struct whirgpu_ctx {
+  int device = 0;
+  hipStream_t st = nullptr;
+  whirgpu_params params{};
+  whir::WhirConfig cfg;
+  P2C16 c16{};
+  P2C24 c24{};
+  u32* d_tw = nullptr;   // 2 * 4096 twiddles
+  Twiddles tw{};
+  Pool pool;
+  void* h_pin = nullptr;
+  size_t h_pin_bytes = 0;
+  ef* d_partial = nullptr;   // reduction scratch
+  size_t partial_cap = 0;
+  ef* d_small = nullptr;     // small results
+  ef* d_tabs = nullptr;      // eq tables: 8 slots of tab_cap
+  size_t tab_cap = 0;
+  u64* d_idx = nullptr;
+  u32* d_open = nullptr;
+  size_t open_cap = 0;
+  u32* d_grind = nullptr;
+  int grind_blocks = 2048;
+  double timings[7] = {0};
+
+  template <typename T>
+  Buf<T> alloc(size_t n) {
+    Buf<T> b;
+    b.n = n;
+    b.p = static_cast<T*>(pool.get(n * sizeof(T)));
+    return b;
+  }
+  template <typename T>
+  void release(Buf<T>& b) {
+    pool.put(b.p, b.n * sizeof(T));
+    b.p = nullptr;
+    b.n = 0;
+  }
+  template <typename T>
+  T* pin(size_t n) {
+    size_t bytes = n * sizeof(T);
+    if (bytes > h_pin_bytes) {
+      if (h_pin) (void)hipHostFree(h_pin);
+      h_pin = nullptr;
+      size_t cap = std::max(bytes, (size_t)4 << 20);
+      HIPCHK(hipHostMalloc(&h_pin, cap, hipHostMallocDefault));
+      h_pin_bytes = cap;
+    }
+    return static_cast<T*>(h_pin);
+  }
+  // Copy device -> host through pinned memory and wait.
+  template <typename T>
+  void d2h(T* dst, const T* src, size_t n) {
+    T* p = pin<T>(n);
+    HIPCHK(hipMemcpyAsync(p, src, n * sizeof(T), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    memcpy(dst, p, n * sizeof(T));
+  }
+  template <typename T>
+  void h2d(T* dst, const T* src, size_t n) {
+    T* p = pin<T>(n);
+    memcpy(p, src, n * sizeof(T));
+    HIPCHK(hipMemcpyAsync(dst, p, n * sizeof(T), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+};
+
+struct whirgpu_statement {
+  whirgpu_ctx* ctx = nullptr;
+  int n = 0;
+  u32* d_poly = nullptr;
+  bool owned = false;
+  size_t poly_bytes = 0;
+  int seg_bits = 0;                       // SVO split: partial sums per 2^seg_bits segments
+  std::vector<std::vector<ef>> points;    // user constraints, then commitment OOD ones
+  std::vector<ef> evals;
+  std::vector<std::vector<ef>> partials;  // per constraint, 2^seg_bits values
+};
+
+struct whirgpu_prover_data {
+  whirgpu_ctx* ctx = nullptr;
+  Buf<u32> rows, tree;
+  u64 height = 0;
+  int row_len = 0;   // base elements per leaf
+  bool ext = false;
+};
+
+struct whirgpu_proof {
+  whirgpu_ctx* ctx = nullptr;
+  whir::WhirProof p;
+};
+
+namespace {
+
+PointArg point_arg(const std::vector<ef>& z, int from = 0) {
+  PointArg a;
+  memset(&a, 0, sizeof a);
+  for (size_t i = from; i < z.size() && i - from < 32; i++) a.z[i - from] = z[i];
+  return a;
+}
+
+// Split eq tables of z (m coordinates) into the context's table slot `slot`:
+// hi covers z[0 .. m-lo), lo covers z[m-lo .. m); the hi table carries `scale`.
+void build_split_eq(whirgpu_ctx* c, const std::vector<ef>& z, int lo, const ef& scale, int slot, EqTables* t,
+                    int k) {
+  int m = (int)z.size();
+  int hi = m - lo;
+  if ((1ull << hi) + (1ull << lo) > c->tab_cap) bad("eq table too large for the context");
+  ef* base = c->d_tabs + (size_t)slot * c->tab_cap;
+  PointArg a = point_arg(z);
+  KCHK(eq_table(a, 0, hi, scale, base, c->st));
+  KCHK(eq_table(a, hi, lo, ef_one(), base + (1ull << hi), c->st));
+  t->hi[k] = base;
+  t->lo[k] = base + (1ull << hi);
+}
+
+int dot_blocks(int rows_log, int seg_bits) {
+  long rows = 1L << rows_log;
+  long want = 2048L >> seg_bits;
+  if (want < 1) want = 1;
+  return (int)std::min(rows, want);
+}
+
+// Segmented eq-weighted sum over the base polynomial for one point (see k_eq_dot).
+std::vector<ef> base_partials(whirgpu_ctx* c, const u32* d_f, int n, int sb, const std::vector<ef>& z_rest) {
+  int nrest = n - sb;
+  int lo = std::min(8, nrest);
+  EqTables t{};
+  build_split_eq(c, z_rest, lo, ef_one(), 0, &t, 0);
+  int nb = dot_blocks(nrest - lo, sb);
+  if ((size_t)nb * (1ull << sb) > c->partial_cap) bad("reduction scratch too small");
+  KCHK(eq_dot_base(d_f, n, sb, lo, t, 1, c->d_partial, nb, c->d_small, c->st));
+  std::vector<ef> out(1ull << sb);
+  c->d2h(out.data(), c->d_small, out.size());
+  return out;
+}
+
+ef ext_mle(whirgpu_ctx* c, const ef* d_f, int m, const std::vector<ef>& z) {
+  int lo = std::min(8, m);
+  EqTables t{};
+  build_split_eq(c, z, lo, ef_one(), 0, &t, 0);
+  int nb = dot_blocks(m - lo, 0);
+  KCHK(eq_dot_ext(d_f, m, 0, lo, t, 1, c->d_partial, nb, c->d_small, c->st));
+  ef out;
+  c->d2h(&out, c->d_small, 1);
+  return out;
+}
+
+ef eq_point(const std::vector<ef>& a, const std::vector<ef>& b) { return whir::eq_at(a, b); }
+
+// Smallest-witness grinding on the device; updates the challenger like
+// GrindingChallenger::grind (observe witness, sample bits).
+u32 gpu_grind(whirgpu_ctx* c, DuplexChallenger& ch, int bits) {
+  u32 s[16];
+  for (int i = 0; i < 16; i++) s[i] = ch.sponge[i];
+  for (int i = 0; i < ch.in_len; i++) s[i] = ch.in[i];
+  u32 init = P;
+  u32* pin = c->pin<u32>(1);
+  *pin = init;
+  HIPCHK(hipMemcpyAsync(c->d_grind, pin, 4, hipMemcpyHostToDevice, c->st));
+  KCHK(grind(c->c16, s, ch.in_len, bits, c->d_grind, c->grind_blocks, c->st));
+  u32 w;
+  c->d2h(&w, c->d_grind, 1);
+  if (w >= P) bad("no proof-of-work witness below p", WHIRGPU_GRIND_FAILED);
+  u32 wm = to_monty(w);
+  if (!ch.check_witness(bits, wm)) bad("grinding witness rejected", WHIRGPU_GRIND_FAILED);
+  return wm;
+}
+
+// SumcheckData::observe_and_sample (src/whir/proof.rs:163-193).
+ef observe_and_sample(whirgpu_ctx* c, whir::SumcheckData& sd, DuplexChallenger& ch, const ef& c0, const ef& c2,
+                      int pow_bits) {
+  sd.polynomial_evaluations.push_back({c0, c2});
+  ch.observe_ef(c0);
+  ch.observe_ef(c2);
+  if (pow_bits > 0) sd.pow_witnesses.push_back(gpu_grind(c, ch, pow_bits));
+  return ch.sample_ef();
+}
+
+// Device-resident product polynomial (ProductPolynomial, src/sumcheck/product_polynomial.rs).
+struct SumcheckState {
+  Buf<ef> e, w;
+  int m = 0;
+  ef sum = ef_zero();
+  bool coeffs_valid = false;   // d_small[0..2] holds (h(0), h(2)) of the current arrays
+};
+
+// ProductPolynomial::round (product_polynomial.rs:307-364).
+ef sumcheck_round(whirgpu_ctx* c, SumcheckState& s, whir::SumcheckData& sd, DuplexChallenger& ch, int pow_bits) {
+  if (s.m < 1) bad("sumcheck round on a constant polynomial", WHIRGPU_BAD_STATE);
+  u64 h = 1ull << (s.m - 1);
+  if (!s.coeffs_valid) KCHK(sc_coeffs(s.e.p, s.w.p, h, c->d_partial, c->d_small, c->st));
+  ef cc[2];
+  c->d2h(cc, c->d_small, 2);
+  ef r = observe_and_sample(c, sd, ch, cc[0], cc[1], pow_bits);
+  KCHK(sc_fold(s.e.p, s.w.p, h, r, c->d_partial, c->d_small, c->st));
+  s.coeffs_valid = h >= 2;
+  s.m -= 1;
+  s.sum = whir::extrapolate_012(cc[0], ef_sub(s.sum, cc[0]), cc[1], r);
+  return r;
+}
+
+// Adds sum_i scale_i * eq(z_i, x) to the weights, four points per launch.
+void add_eq_weights(whirgpu_ctx* c, SumcheckState& s, const std::vector<std::vector<ef>>& pts,
+                    const std::vector<ef>& scales, bool accumulate_first) {
+  int m = s.m;
+  int lo = std::min(8, m);
+  size_t K = pts.size();
+  if (K == 0 && !accumulate_first) {
+    EqTables t{};
+    KCHK(eq_combine(s.w.p, 1ull << m, lo, t, 0, 0, c->st));
+    return;
+  }
+  for (size_t b = 0; b < K; b += 4) {
+    EqTables t{};
+    int kk = (int)std::min<size_t>(4, K - b);
+    for (int k = 0; k < kk; k++) build_split_eq(c, pts[b + k], lo, scales[b + k], k, &t, k);
+    KCHK(eq_combine(s.w.p, 1ull << m, lo, t, kk, (b > 0 || accumulate_first) ? 1 : 0, c->st));
+  }
+}
+
+// Initial sumcheck from the base evaluations (Sumcheck::from_base_evals,
+// src/sumcheck/sumcheck_prover.rs:237-292).  Classic and SVO produce the same
+// transcript (asserted by the reference at src/sumcheck/tests.rs:473-487); the
+// device always takes the split route when n > l0: per constraint, the
+// partial sums P(s) = sum_x eq(z_rest, x) f(s, x) over the l0 leading
+// variables s (computed once at evaluate time) give every round polynomial
+//   h_i(X) = eq(z_<i, r_<i) eq(z_i, X) sum_s' eq(z_>i, s') P~(r_<i, X, s'),
+// and afterwards f is folded by all l0 challenges at once.
+std::vector<ef> initial_sumcheck(whirgpu_ctx* c, const whirgpu_statement* st, DuplexChallenger& ch,
+                                 whir::SumcheckData& sd, SumcheckState& s) {
+  const int n = st->n;
+  const int l0 = c->cfg.ff(0);
+  const int pow = c->cfg.starting_folding_pow_bits;
+  const size_t K = st->points.size();
+  ef alpha = ch.sample_ef();
+  auto ap = whir::powers(alpha, K);
+  s.sum = ef_zero();
+  for (size_t j = 0; j < K; j++) s.sum = ef_add(s.sum, ef_mul(ap[j], st->evals[j]));
+  std::vector<ef> rs;
+  if (st->seg_bits == l0 && n > l0) {
+    std::vector<std::vector<ef>> P = st->partials;
+    for (int i = 0; i < l0; i++) {
+      ef c0 = ef_zero(), c2 = ef_zero();
+      for (size_t j = 0; j < K; j++) {
+        const auto& z = st->points[j];
+        size_t half = P[j].size() / 2;
+        std::vector<ef> z_after(z.begin() + i + 1, z.begin() + l0);
+        auto eqa = whir::eq_table_host(z_after, ef_one());
+        ef A0 = ef_zero(), A1 = ef_zero();
+        for (size_t q = 0; q < half; q++) {
+          A0 = ef_add(A0, ef_mul(eqa[q], P[j][q]));
+          A1 = ef_add(A1, ef_mul(eqa[q], P[j][half + q]));
+        }
+        ef A2 = ef_sub(ef_dbl(A1), A0);
+        std::vector<ef> z_before(z.begin(), z.begin() + i);
+        ef pre = eq_point(z_before, rs);
+        const ef& zi = z[i];
+        ef e0 = ef_sub(ef_one(), zi);
+        ef e2 = ef_sub(ef_add(ef_dbl(zi), zi), ef_one());
+        c0 = ef_add(c0, ef_mul(ap[j], ef_mul(pre, ef_mul(e0, A0))));
+        c2 = ef_add(c2, ef_mul(ap[j], ef_mul(pre, ef_mul(e2, A2))));
+      }
+      ef r = observe_and_sample(c, sd, ch, c0, c2, pow);
+      s.sum = whir::extrapolate_012(c0, ef_sub(s.sum, c0), c2, r);
+      rs.push_back(r);
+      for (auto& Pj : P) {
+        size_t half = Pj.size() / 2;
+        for (size_t q = 0; q < half; q++) Pj[q] = ef_add(Pj[q], ef_mul(r, ef_sub(Pj[half + q], Pj[q])));
+        Pj.resize(half);
+      }
+    }
+    // f(rs, x) for all x, and W(x) = sum_j alpha^j eq(z_j[..l0], rs) eq(z_j[l0..], x)
+    s.m = n - l0;
+    s.e = c->alloc<ef>(1ull << s.m);
+    s.w = c->alloc<ef>(1ull << s.m);
+    auto eqrs = whir::eq_table_host(rs, ef_one());
+    ef* d_eqrs = c->d_tabs + 7 * c->tab_cap;
+    c->h2d(d_eqrs, eqrs.data(), eqrs.size());
+    KCHK(compress_multi(st->d_poly, n, l0, d_eqrs, s.e.p, c->st));
+    std::vector<std::vector<ef>> rests;
+    std::vector<ef> scales;
+    for (size_t j = 0; j < K; j++) {
+      const auto& z = st->points[j];
+      std::vector<ef> zs(z.begin(), z.begin() + l0);
+      rests.emplace_back(z.begin() + l0, z.end());
+      scales.push_back(ef_mul(ap[j], eq_point(zs, rs)));
+    }
+    add_eq_weights(c, s, rests, scales, false);
+    s.coeffs_valid = false;
+    return rs;
+  }
+  // small instances: classic rounds on the extension-embedded evaluations
+  s.m = n;
+  s.e = c->alloc<ef>(1ull << n);
+  s.w = c->alloc<ef>(1ull << n);
+  KCHK(base_to_ext(st->d_poly, 1ull << n, s.e.p, c->st));
+  add_eq_weights(c, s, st->points, ap, false);
+  s.coeffs_valid = false;
+  for (int i = 0; i < l0; i++) rs.push_back(sumcheck_round(c, s, sd, ch, pow));
+  return rs;
+}
+
+int ilog2u(u64 x) {
+  int k = 0;
+  while ((x >> (k + 1)) != 0) k++;
+  return k;
+}
+
+// Opens `idx` on a committed matrix and returns the openings plus each row
+// as extension elements.
+void open_rows(whirgpu_ctx* c, const whirgpu_prover_data* pd, const std::vector<u64>& idx,
+               std::vector<whir::QueryOpening>* out, std::vector<std::vector<ef>>* rows_ef) {
+  int nq = (int)idx.size();
+  if (nq == 0) return;
+  int depth = ilog2u(pd->height);
+  size_t per = (size_t)pd->row_len + 8ull * depth;
+  if (per * nq > c->open_cap || nq > 4096) bad("too many queries for the opening buffer");
+  c->h2d(c->d_idx, idx.data(), idx.size());
+  KCHK(merkle_open(pd->rows.p, pd->row_len, pd->tree.p, pd->height, c->d_idx, nq, c->d_open, c->st));
+  std::vector<u32> h(per * nq);
+  c->d2h(h.data(), c->d_open, h.size());
+  for (int q = 0; q < nq; q++) {
+    const u32* b = h.data() + per * q;
+    whir::QueryOpening o;
+    o.extension = pd->ext;
+    o.values.assign(b, b + pd->row_len);
+    for (int l = 0; l < depth; l++) {
+      Digest d;
+      for (int k = 0; k < 8; k++) d[k] = b[pd->row_len + 8 * l + k];
+      o.proof.push_back(d);
+    }
+    std::vector<ef> r;
+    if (pd->ext) {
+      for (int i = 0; i < pd->row_len; i += 4) r.push_back(ef{{b[i], b[i + 1], b[i + 2], b[i + 3]}});
+    } else {
+      for (int i = 0; i < pd->row_len; i++) r.push_back(ef_base(b[i]));
+    }
+    rows_ef->push_back(std::move(r));
+    out->push_back(std::move(o));
+  }
+}
+
+Digest read_root(whirgpu_ctx* c, const Buf<u32>& tree, u64 height) {
+  Digest d;
+  c->d2h(d.data(), tree.p + (2 * height - 2) * 8, 8);
+  return d;
+}
+
+void observe_digest(DuplexChallenger& ch, const Digest& d) {
+  for (u32 x : d) ch.observe(x);
+}
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void use_device(const whirgpu_ctx* c) {
+  if (c->device < 0 || !c->st) bad("host-only context has no device", WHIRGPU_BAD_STATE);
+  HIPCHK(hipSetDevice(c->device));
+}
+
+DuplexChallenger load_ch(const whirgpu_ctx* c, const whirgpu_challenger* ch) {
+  DuplexChallenger d;
+  d.perm = &c->c16;
+  memcpy(d.sponge, ch->sponge, sizeof d.sponge);
+  memcpy(d.in, ch->input, sizeof d.in);
+  memcpy(d.out, ch->output, sizeof d.out);
+  d.in_len = (int)ch->input_len;
+  d.out_len = (int)ch->output_len;
+  if (d.in_len < 0 || d.in_len > 7 || d.out_len < 0 || d.out_len > 8) bad("invalid challenger state");
+  return d;
+}
+
+void store_ch(const DuplexChallenger& d, whirgpu_challenger* ch) {
+  memcpy(ch->sponge, d.sponge, sizeof d.sponge);
+  memcpy(ch->input, d.in, sizeof d.in);
+  memcpy(ch->output, d.out, sizeof d.out);
+  ch->input_len = (u32)d.in_len;
+  ch->output_len = (u32)d.out_len;
+}
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return WHIRGPU_OK;
+  } catch (const Fail& e) {
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return WHIRGPU_OOM;
+  } catch (...) {
+    g_err = "unexpected internal error";
+    return WHIRGPU_DEVICE;
+  }
+}
+
+void make_twiddles(whirgpu_ctx* c) {
+  std::vector<u32> h(8192);
+  u32 w = two_adic_generator(24);
+  u32 x = ONE;
+  for (int e = 0; e < 4096; e++) {
+    h[e] = x;
+    x = mul(x, w);
+  }
+  u32 w4096 = x;   // w^4096
+  u32 y = ONE;
+  for (int e = 0; e < 4096; e++) {
+    h[4096 + e] = y;
+    y = mul(y, w4096);
+  }
+  HIPCHK(hipMalloc(&c->d_tw, 8192 * sizeof(u32)));
+  HIPCHK(hipMemcpy(c->d_tw, h.data(), 8192 * sizeof(u32), hipMemcpyHostToDevice));
+  c->tw.lo = c->d_tw;
+  c->tw.hi = c->d_tw + 4096;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* whirgpu_last_error(void) { return g_err.c_str(); }
+
+int whirgpu_poseidon2_from_seed(uint64_t seed, whirgpu_poseidon2* out) {
+  return guarded([&] {
+    if (!out) bad("null output");
+    P2C16 a;
+    P2C24 b;
+    whir::poseidon2_constants_from_seed(seed, &a, &b);
+    memcpy(out->w16_external_initial, a.ext_init, sizeof a.ext_init);
+    memcpy(out->w16_external_terminal, a.ext_term, sizeof a.ext_term);
+    memcpy(out->w16_internal, a.internal, sizeof a.internal);
+    memcpy(out->w24_external_initial, b.ext_init, sizeof b.ext_init);
+    memcpy(out->w24_external_terminal, b.ext_term, sizeof b.ext_term);
+    memcpy(out->w24_internal, b.internal, sizeof b.internal);
+  });
+}
+
+int whirgpu_sample_field(uint64_t seed, uint32_t* out, size_t count) {
+  return guarded([&] {
+    if (!out && count) bad("null output");
+    whir::SmallRng rng(seed);
+    for (size_t i = 0; i < count; i++) out[i] = rng.field_monty();
+  });
+}
+
+static void load_perms(const whirgpu_poseidon2* pp, P2C16* a, P2C24* b) {
+  memcpy(a->ext_init, pp->w16_external_initial, sizeof a->ext_init);
+  memcpy(a->ext_term, pp->w16_external_terminal, sizeof a->ext_term);
+  memcpy(a->internal, pp->w16_internal, sizeof a->internal);
+  memcpy(b->ext_init, pp->w24_external_initial, sizeof b->ext_init);
+  memcpy(b->ext_term, pp->w24_external_terminal, sizeof b->ext_term);
+  memcpy(b->internal, pp->w24_internal, sizeof b->internal);
+}
+
+int whirgpu_poseidon2_permute(const whirgpu_poseidon2* perms, uint32_t width, uint32_t* state) {
+  return guarded([&] {
+    if (!perms || !state) bad("null argument");
+    P2C16 a;
+    P2C24 b;
+    load_perms(perms, &a, &b);
+    if (width == 16) permute<16>(state, a);
+    else if (width == 24) permute<24>(state, b);
+    else bad("width must be 16 or 24");
+  });
+}
+
+int whirgpu_create(int device, const whirgpu_params* params, const whirgpu_poseidon2* perms, whirgpu_ctx** out) {
+  whirgpu_ctx* c = nullptr;
+  int rc = guarded([&] {
+    if (!params || !perms || !out) bad("null argument");
+    c = new whirgpu_ctx();
+    c->device = device;
+    c->params = *params;
+    whir::ProtocolParameters p;
+    p.starting_log_inv_rate = (int)params->starting_log_inv_rate;
+    p.rs_domain_initial_reduction_factor = (int)params->rs_domain_initial_reduction_factor;
+    p.folding_factor.first = (int)params->folding_factor_first;
+    p.folding_factor.rest = (int)params->folding_factor;
+    p.soundness_type = (int)params->soundness_type;
+    p.security_level = (int)params->security_level;
+    p.pow_bits = (int)params->pow_bits;
+    if (params->leaf_hash_width != 16 && params->leaf_hash_width != 24) bad("leaf_hash_width must be 16 or 24");
+    if (params->num_variables < 1 || params->num_variables > 28) bad("num_variables must be in [1, 28]");
+    std::string err;
+    if (!whir::WhirConfig::create((int)params->num_variables, p, &c->cfg, &err)) bad(err);
+    load_perms(perms, &c->c16, &c->c24);
+    if (device < 0) return;   // host-only context: schedule and transcript
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    make_twiddles(c);
+    int n = (int)params->num_variables;
+    c->tab_cap = (1ull << std::max(0, n - 8)) + 512;
+    HIPCHK(hipMalloc(&c->d_tabs, 8 * c->tab_cap * sizeof(ef)));
+    c->partial_cap = 1 << 15;
+    HIPCHK(hipMalloc(&c->d_partial, c->partial_cap * sizeof(ef)));
+    HIPCHK(hipMalloc(&c->d_small, 4096 * sizeof(ef)));
+    HIPCHK(hipMalloc(&c->d_idx, 4096 * sizeof(u64)));
+    c->open_cap = 1 << 20;
+    HIPCHK(hipMalloc(&c->d_open, c->open_cap * sizeof(u32)));
+    HIPCHK(hipMalloc(&c->d_grind, 256));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    c->grind_blocks = std::max(64, prop.multiProcessorCount * 8);
+    c->pin<u32>(1);
+  });
+  if (rc) {
+    if (c) whirgpu_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return WHIRGPU_OK;
+}
+
+int whirgpu_destroy(whirgpu_ctx* c) {
+  if (!c) return WHIRGPU_OK;
+  if (c->device >= 0) (void)hipSetDevice(c->device);
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  c->pool.release_all();
+  for (void* p : {(void*)c->d_tw, (void*)c->d_tabs, (void*)c->d_partial, (void*)c->d_small, (void*)c->d_idx,
+                  (void*)c->d_open, (void*)c->d_grind})
+    if (p) (void)hipFree(p);
+  if (c->h_pin) (void)hipHostFree(c->h_pin);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+  return WHIRGPU_OK;
+}
+
+int whirgpu_config(const whirgpu_ctx* c, uint32_t* out, size_t cap, size_t* len) {
+  return guarded([&] {
+    if (!c || !len) bad("null argument");
+    const auto& g = c->cfg;
+    std::vector<u32> v = {(u32)g.commitment_ood_samples, (u32)g.starting_folding_pow_bits, (u32)g.n_rounds(),
+                          (u32)g.final_queries,          (u32)g.final_pow_bits,            (u32)g.final_sumcheck_rounds,
+                          (u32)g.final_folding_pow_bits};
+    for (const auto& r : g.round_parameters) {
+      v.insert(v.end(), {(u32)r.pow_bits, (u32)r.folding_pow_bits, (u32)r.num_queries, (u32)r.ood_samples,
+                         (u32)r.num_variables, (u32)r.folding_factor, (u32)ilog2u(r.domain_size),
+                         from_monty(r.folded_domain_gen)});
+    }
+    *len = v.size();
+    if (cap < v.size() || !out) bad("buffer too small", WHIRGPU_BUFFER_TOO_SMALL);
+    memcpy(out, v.data(), v.size() * 4);
+  });
+}
+
+int whirgpu_domain_separator(const whirgpu_ctx* c, uint32_t* out, size_t cap, size_t* len) {
+  return guarded([&] {
+    if (!c || !len) bad("null argument");
+    auto v = whir::domain_separator_pattern(c->cfg, (int)c->params.digest_elems);
+    *len = v.size();
+    if (cap < v.size() || !out) bad("buffer too small", WHIRGPU_BUFFER_TOO_SMALL);
+    memcpy(out, v.data(), v.size() * 4);
+  });
+}
+
+int whirgpu_challenger_new(whirgpu_challenger* ch) {
+  return guarded([&] {
+    if (!ch) bad("null argument");
+    memset(ch, 0, sizeof *ch);
+  });
+}
+
+int whirgpu_challenger_observe(const whirgpu_ctx* c, whirgpu_challenger* ch, const uint32_t* values, size_t n) {
+  return guarded([&] {
+    if (!c || !ch || (!values && n)) bad("null argument");
+    DuplexChallenger d = load_ch(c, ch);
+    d.observe_slice(values, n);
+    store_ch(d, ch);
+  });
+}
+
+int whirgpu_challenger_observe_domain_separator(const whirgpu_ctx* c, whirgpu_challenger* ch) {
+  return guarded([&] {
+    if (!c || !ch) bad("null argument");
+    DuplexChallenger d = load_ch(c, ch);
+    auto v = whir::domain_separator_pattern(c->cfg, (int)c->params.digest_elems);
+    d.observe_slice(v.data(), v.size());
+    store_ch(d, ch);
+  });
+}
+
+int whirgpu_challenger_sample(const whirgpu_ctx* c, whirgpu_challenger* ch, uint32_t* out, size_t n) {
+  return guarded([&] {
+    if (!c || !ch || (!out && n)) bad("null argument");
+    DuplexChallenger d = load_ch(c, ch);
+    for (size_t i = 0; i < n; i++) out[i] = d.sample();
+    store_ch(d, ch);
+  });
+}
+
+int whirgpu_statement_create(whirgpu_ctx* c, const uint32_t* evals, int on_device, whirgpu_statement** out) {
+  whirgpu_statement* s = nullptr;
+  int rc = guarded([&] {
+    if (!c || !evals || !out) bad("null argument");
+    use_device(c);
+    s = new whirgpu_statement();
+    s->ctx = c;
+    s->n = c->cfg.num_variables;
+    s->poly_bytes = (1ull << s->n) * 4;
+    s->seg_bits = s->n > c->cfg.ff(0) ? c->cfg.ff(0) : 0;
+    if (on_device) {
+      s->d_poly = const_cast<u32*>(evals);
+      s->owned = false;
+    } else {
+      s->d_poly = static_cast<u32*>(c->pool.get(s->poly_bytes));
+      s->owned = true;
+      HIPCHK(hipMemcpyAsync(s->d_poly, evals, s->poly_bytes, hipMemcpyHostToDevice, c->st));
+      HIPCHK(hipStreamSynchronize(c->st));
+    }
+  });
+  if (rc) {
+    if (s) whirgpu_statement_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return WHIRGPU_OK;
+}
+
+static ef statement_evaluate(whirgpu_statement* s, const std::vector<ef>& z) {
+  whirgpu_ctx* c = s->ctx;
+  int sb = s->seg_bits;
+  std::vector<ef> z_rest(z.begin() + sb, z.end());
+  auto part = base_partials(c, s->d_poly, s->n, sb, z_rest);
+  ef eval;
+  if (sb == 0) {
+    eval = part[0];
+  } else {
+    std::vector<ef> zs(z.begin(), z.begin() + sb);
+    auto eqs = whir::eq_table_host(zs, ef_one());
+    eval = ef_zero();
+    for (size_t i = 0; i < part.size(); i++) eval = ef_add(eval, ef_mul(eqs[i], part[i]));
+  }
+  s->points.push_back(z);
+  s->evals.push_back(eval);
+  s->partials.push_back(std::move(part));
+  return eval;
+}
+
+int whirgpu_statement_evaluate(whirgpu_statement* s, const uint32_t* point, uint32_t* eval_out) {
+  return guarded([&] {
+    if (!s || !point || !eval_out) bad("null argument");
+    std::vector<ef> z(s->n);
+    for (int i = 0; i < s->n; i++) memcpy(z[i].c, point + 4 * i, 16);
+    ef e = statement_evaluate(s, z);
+    memcpy(eval_out, e.c, 16);
+  });
+}
+
+int whirgpu_statement_destroy(whirgpu_statement* s) {
+  if (!s) return WHIRGPU_OK;
+  if (s->owned && s->d_poly) s->ctx->pool.put(s->d_poly, s->poly_bytes);
+  delete s;
+  return WHIRGPU_OK;
+}
+
+int whirgpu_proof_create(whirgpu_ctx* c, whirgpu_proof** out) {
+  return guarded([&] {
+    if (!c || !out) bad("null argument");
+    auto* p = new whirgpu_proof();
+    p->ctx = c;
+    p->p.rounds.resize(c->cfg.n_rounds());
+    *out = p;
+  });
+}
+
+int whirgpu_proof_destroy(whirgpu_proof* p) {
+  delete p;
+  return WHIRGPU_OK;
+}
+
+int whirgpu_proof_serialize(const whirgpu_proof* p, uint8_t* out, size_t cap, size_t* len) {
+  return guarded([&] {
+    if (!p || !len) bad("null argument");
+    auto b = whir::serialize(p->p);
+    *len = b.size();
+    if (cap < b.size() || !out) bad("buffer too small", WHIRGPU_BUFFER_TOO_SMALL);
+    memcpy(out, b.data(), b.size());
+  });
+}
+
+int whirgpu_prover_data_destroy(whirgpu_prover_data* pd) {
+  if (!pd) return WHIRGPU_OK;
+  pd->ctx->release(pd->rows);
+  pd->ctx->release(pd->tree);
+  delete pd;
+  return WHIRGPU_OK;
+}
+
+// CommitmentWriter::commit (src/whir/committer/writer.rs:56-109).
+int whirgpu_commit(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs, whirgpu_statement* st,
+                   whirgpu_prover_data** out) {
+  whirgpu_prover_data* pd = nullptr;
+  int rc = guarded([&] {
+    if (!c || !proof || !chs || !st || !out) bad("null argument");
+    use_device(c);
+    double t0 = now_ms();
+    DuplexChallenger ch = load_ch(c, chs);
+    const auto& cfg = c->cfg;
+    const int n = cfg.num_variables, k0 = cfg.ff(0), R = cfg.params.starting_log_inv_rate;
+    const u64 H = 1ull << (n + R - k0), W = 1ull << k0;
+    pd = new whirgpu_prover_data();
+    pd->ctx = c;
+    pd->rows = c->alloc<u32>(H * W);
+    pd->tree = c->alloc<u32>((2 * H - 1) * 8);
+    pd->height = H;
+    pd->row_len = (int)W;
+    pd->ext = false;
+    Buf<u32> tmp;
+    if (lde_two_pass(n, k0, false)) tmp = c->alloc<u32>(H * W);
+    KCHK(lde_base(st->d_poly, pd->rows.p, tmp.p, n, k0, R, c->tw, c->st));
+    KCHK(merkle_commit(pd->rows.p, (int)W, H, (int)c->params.leaf_hash_width, c->c16, c->c24, pd->tree.p, c->st));
+    if (tmp.p) c->release(tmp);
+    Digest root = read_root(c, pd->tree, H);
+    proof->p.has_initial_commitment = true;
+    proof->p.initial_commitment = root;
+    observe_digest(ch, root);
+    for (int i = 0; i < cfg.commitment_ood_samples; i++) {
+      auto pt = whir::expand_from_univariate(ch.sample_ef(), n);
+      ef ev = statement_evaluate(st, pt);
+      proof->p.initial_ood_answers.push_back(ev);
+      ch.observe_ef(ev);
+    }
+    store_ch(ch, chs);
+    c->timings[0] = now_ms() - t0;
+  });
+  if (rc) {
+    if (pd) whirgpu_prover_data_destroy(pd);
+    return rc;
+  }
+  *out = pd;
+  return WHIRGPU_OK;
+}
+
+// Prover::prove (src/whir/prover/mod.rs:97-417).
+int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs, const whirgpu_statement* st,
+                  whirgpu_prover_data* pd0) {
+  SumcheckState s;
+  whirgpu_prover_data* prev = pd0;
+  int rc = guarded([&] {
+    if (!c || !proof || !chs || !st || !pd0) bad("null argument");
+    use_device(c);
+    double t0 = now_ms();
+    DuplexChallenger ch = load_ch(c, chs);
+    const auto& cfg = c->cfg;
+    const int n = cfg.num_variables;
+    auto& P = proof->p;
+    P.rounds.assign(cfg.n_rounds(), whir::WhirRoundProof());
+    std::vector<ef> fold_rand = initial_sumcheck(c, st, ch, P.initial_sumcheck, s);
+    for (int round = 0; round <= cfg.n_rounds(); round++) {
+      const int m = n - cfg.params.folding_factor.total_number(round);
+      if (m != s.m) bad("internal: round size mismatch", WHIRGPU_BAD_STATE);
+      if (round == cfg.n_rounds()) {
+        // final_round (prover/mod.rs:330-417)
+        P.final_poly.resize(1ull << m);
+        c->d2h(P.final_poly.data(), s.e.p, P.final_poly.size());
+        P.has_final_poly = true;
+        for (const ef& e : P.final_poly) ch.observe_ef(e);
+        if (cfg.final_pow_bits > 0) P.final_pow_witness = gpu_grind(c, ch, cfg.final_pow_bits);
+        auto fc = cfg.final_round_config();
+        auto idx = whir::get_challenge_stir_queries(fc.domain_size, cfg.ff(round), cfg.final_queries, ch);
+        std::vector<std::vector<ef>> rows;
+        open_rows(c, prev, idx, &P.final_queries, &rows);
+        if (cfg.final_sumcheck_rounds > 0) {
+          P.has_final_sumcheck = true;
+          for (int i = 0; i < cfg.final_sumcheck_rounds; i++)
+            sumcheck_round(c, s, P.final_sumcheck, ch, cfg.final_folding_pow_bits);
+        }
+        break;
+      }
+      const auto& rp = cfg.round_parameters[round];
+      auto& R = P.rounds[round];
+      const int kn = cfg.ff(round + 1);
+      const int Rl = ilog2u(cfg.inv_rate(round));
+      const u64 H = 1ull << (m - kn + Rl), W = 1ull << kn;
+      // commit to the folded polynomial (prover/mod.rs:158-190)
+      auto* pd = new whirgpu_prover_data();
+      pd->ctx = c;
+      pd->rows = c->alloc<u32>(H * W * 4);
+      pd->tree = c->alloc<u32>((2 * H - 1) * 8);
+      pd->height = H;
+      pd->row_len = (int)(4 * W);
+      pd->ext = true;
+      Buf<ef> tmp;
+      if (lde_two_pass(m, kn, true)) tmp = c->alloc<ef>(H * W);
+      KCHK(lde_ext(s.e.p, reinterpret_cast<ef*>(pd->rows.p), tmp.p, m, kn, Rl, c->tw, c->st));
+      KCHK(merkle_commit(pd->rows.p, pd->row_len, H, (int)c->params.leaf_hash_width, c->c16, c->c24, pd->tree.p,
+                         c->st));
+      if (tmp.p) c->release(tmp);
+      R.commitment = read_root(c, pd->tree, H);
+      observe_digest(ch, R.commitment);
+      // OOD samples (prover/mod.rs:192-205)
+      std::vector<std::vector<ef>> ood_pts;
+      for (int i = 0; i < rp.ood_samples; i++) {
+        auto pt = whir::expand_from_univariate(ch.sample_ef(), m);
+        ef ev = ext_mle(c, s.e.p, m, pt);
+        ch.observe_ef(ev);
+        R.ood_answers.push_back(ev);
+        ood_pts.push_back(pt);
+      }
+      if (rp.pow_bits > 0) R.pow_witness = gpu_grind(c, ch, rp.pow_bits);
+      ch.sample();
+      // STIR queries on the previous commitment (prover/mod.rs:229-303)
+      auto idx = whir::get_challenge_stir_queries(rp.domain_size, cfg.ff(round), rp.num_queries, ch);
+      std::vector<std::vector<ef>> rows;
+      open_rows(c, prev, idx, &R.queries, &rows);
+      std::vector<u32> sel_z;
+      std::vector<ef> sel_v;
+      for (size_t q = 0; q < idx.size(); q++) {
+        sel_z.push_back(kb::pow(rp.folded_domain_gen, idx[q]));
+        sel_v.push_back(whir::mle_host(rows[q], fold_rand));
+      }
+      whirgpu_prover_data_destroy(prev);
+      prev = pd;
+      // Constraint::combine_packed (src/whir/constraints/mod.rs:234-248)
+      ef gamma = ch.sample_ef();
+      auto gp = whir::powers(gamma, ood_pts.size() + sel_z.size());
+      std::vector<ef> eq_scales(gp.begin(), gp.begin() + ood_pts.size());
+      if (!ood_pts.empty()) add_eq_weights(c, s, ood_pts, eq_scales, true);
+      for (size_t i = 0; i < ood_pts.size(); i++) s.sum = ef_add(s.sum, ef_mul(gp[i], R.ood_answers[i]));
+      if (!sel_z.empty()) {
+        size_t ns = sel_z.size();
+        std::vector<ef> g(gp.begin() + ood_pts.size(), gp.end());
+        u32* d_z = reinterpret_cast<u32*>(c->d_tabs + 6 * c->tab_cap);
+        ef* d_g = c->d_tabs + 6 * c->tab_cap + (ns + 3) / 4 + 1;
+        if ((ns + 3) / 4 + 1 + ns > c->tab_cap) bad("too many STIR queries");
+        c->h2d(d_z, sel_z.data(), ns);
+        c->h2d(d_g, g.data(), ns);
+        int lo = m / 2, hi = m - lo;
+        Buf<ef> A = c->alloc<ef>(ns << hi);
+        Buf<u32> B = c->alloc<u32>(ns << lo);
+        KCHK(select_combine(s.w.p, m, d_z, d_g, (int)ns, A.p, B.p, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        c->release(A);
+        c->release(B);
+        for (size_t j = 0; j < ns; j++) s.sum = ef_add(s.sum, ef_mul(g[j], sel_v[j]));
+      }
+      s.coeffs_valid = false;
+      fold_rand.clear();
+      for (int i = 0; i < kn; i++) fold_rand.push_back(sumcheck_round(c, s, R.sumcheck, ch, rp.folding_pow_bits));
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    store_ch(ch, chs);
+    c->timings[6] = now_ms() - t0;
+  });
+  if (prev) whirgpu_prover_data_destroy(prev);
+  if (s.e.p) c->release(s.e);
+  if (s.w.p) c->release(s.w);
+  return rc;
+}
+
+int whirgpu_last_timings(const whirgpu_ctx* c, double* out7) {
+  return guarded([&] {
+    if (!c || !out7) bad("null argument");
+    memcpy(out7, c->timings, sizeof c->timings);
+  });
+}
+
+int whirgpu_device_alloc(int device, size_t bytes, void** out) {
+  return guarded([&] {
+    if (!out) bad("null argument");
+    HIPCHK(hipSetDevice(device));
+    *out = nullptr;
+    HIPCHK(hipMalloc(out, bytes ? bytes : 1));
+  });
+}
+
+int whirgpu_device_free(int device, void* ptr) {
+  return guarded([&] {
+    HIPCHK(hipSetDevice(device));
+    if (ptr) HIPCHK(hipFree(ptr));
+  });
+}
+
+int whirgpu_memcpy(int device, void* dst, const void* src, size_t bytes, int kind) {
+  return guarded([&] {
+    if ((!dst || !src) && bytes) bad("null argument");
+    if (kind < 0 || kind > 2) bad("kind must be 0, 1 or 2");
+    HIPCHK(hipSetDevice(device));
+    hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    HIPCHK(hipMemcpy(dst, src, bytes, k));
+  });
+}
+
+// ---------------------------------------------------------------- kernel-level entry points
+int whirgpu_lde_base(whirgpu_ctx* c, const uint32_t* d_src, uint32_t m, uint32_t k, uint32_t log_blowup,
+                     uint32_t* d_dst) {
+  return guarded([&] {
+    if (!c || !d_src || !d_dst || k > m || m > 28 || m - k + log_blowup > 24) bad("bad shape");
+    use_device(c);
+    u64 out = 1ull << (m + log_blowup);
+    Buf<u32> tmp;
+    if (lde_two_pass((int)m, (int)k, false)) tmp = c->alloc<u32>(out);
+    KCHK(lde_base(d_src, d_dst, tmp.p, (int)m, (int)k, (int)log_blowup, c->tw, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (tmp.p) c->release(tmp);
+  });
+}
+
+int whirgpu_lde_ext(whirgpu_ctx* c, const uint32_t* d_src, uint32_t m, uint32_t k, uint32_t log_blowup,
+                    uint32_t* d_dst) {
+  return guarded([&] {
+    if (!c || !d_src || !d_dst || k > m || m > 28 || m - k + log_blowup > 24) bad("bad shape");
+    use_device(c);
+    u64 out = 1ull << (m + log_blowup);
+    Buf<ef> tmp;
+    if (lde_two_pass((int)m, (int)k, true)) tmp = c->alloc<ef>(out);
+    KCHK(lde_ext(reinterpret_cast<const ef*>(d_src), reinterpret_cast<ef*>(d_dst), tmp.p, (int)m, (int)k,
+                 (int)log_blowup, c->tw, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (tmp.p) c->release(tmp);
+  });
+}
+
+int whirgpu_merkle_commit(whirgpu_ctx* c, const uint32_t* d_rows, uint64_t height, uint32_t row_len, uint32_t* d_tree,
+                          uint32_t* root_out) {
+  return guarded([&] {
+    if (!c || !d_rows || !d_tree || !root_out || row_len == 0) bad("null argument");
+    if (height == 0 || (height & (height - 1))) bad("height must be a power of two");
+    use_device(c);
+    KCHK(merkle_commit(d_rows, (int)row_len, height, (int)c->params.leaf_hash_width, c->c16, c->c24, d_tree, c->st));
+    c->d2h(root_out, d_tree + (2 * height - 2) * 8, 8);
+  });
+}
+
+int whirgpu_grind(whirgpu_ctx* c, whirgpu_challenger* chs, uint32_t bits, uint32_t* witness_out) {
+  return guarded([&] {
+    if (!c || !chs || !witness_out) bad("null argument");
+    if (bits == 0 || bits > 30) bad("bits must be in [1, 30]");
+    use_device(c);
+    DuplexChallenger ch = load_ch(c, chs);
+    *witness_out = gpu_grind(c, ch, (int)bits);
+    store_ch(ch, chs);
+  });
+}
+
+static std::vector<ef> read_point(const uint32_t* p, uint32_t n) {
+  std::vector<ef> z(n);
+  for (uint32_t i = 0; i < n; i++) memcpy(z[i].c, p + 4 * i, 16);
+  return z;
+}
+
+int whirgpu_eq_expand(whirgpu_ctx* c, const uint32_t* point, uint32_t n, const uint32_t* scale, uint32_t* d_out) {
+  return guarded([&] {
+    if (!c || !point || !scale || !d_out || n > 30) bad("bad argument");
+    use_device(c);
+    auto z = read_point(point, n);
+    ef sc;
+    memcpy(sc.c, scale, 16);
+    int lo = (int)n / 2, hi = (int)n - lo;
+    Buf<ef> th = c->alloc<ef>(1ull << hi), tl = c->alloc<ef>(1ull << lo);
+    PointArg a = point_arg(z);
+    KCHK(eq_table(a, 0, hi, sc, th.p, c->st));
+    KCHK(eq_table(a, hi, lo, ef_one(), tl.p, c->st));
+    KCHK(eq_outer(th.p, tl.p, lo, 1ull << n, reinterpret_cast<ef*>(d_out), c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->release(th);
+    c->release(tl);
+  });
+}
+
+int whirgpu_mle_eval_base(whirgpu_ctx* c, const uint32_t* d_f, uint32_t n, const uint32_t* point, uint32_t* out) {
+  return guarded([&] {
+    if (!c || !d_f || !point || !out || n > 30) bad("bad argument");
+    if ((1ull << std::max<int>(0, (int)n - 8)) + 256 > c->tab_cap) bad("n too large for this context");
+    use_device(c);
+    auto part = base_partials(c, d_f, (int)n, 0, read_point(point, n));
+    memcpy(out, part[0].c, 16);
+  });
+}
+
+int whirgpu_mle_eval_ext(whirgpu_ctx* c, const uint32_t* d_f, uint32_t n, const uint32_t* point, uint32_t* out) {
+  return guarded([&] {
+    if (!c || !d_f || !point || !out || n > 30) bad("bad argument");
+    if ((1ull << std::max<int>(0, (int)n - 8)) + 256 > c->tab_cap) bad("n too large for this context");
+    use_device(c);
+    ef e = ext_mle(c, reinterpret_cast<const ef*>(d_f), (int)n, read_point(point, n));
+    memcpy(out, e.c, 16);
+  });
+}
+
+int whirgpu_sumcheck_coeffs(whirgpu_ctx* c, const uint32_t* d_e, const uint32_t* d_w, uint32_t n, uint32_t* out) {
+  return guarded([&] {
+    if (!c || !d_e || !d_w || !out || n < 1 || n > 32) bad("bad argument");
+    use_device(c);
+    KCHK(sc_coeffs(reinterpret_cast<const ef*>(d_e), reinterpret_cast<const ef*>(d_w), 1ull << (n - 1), c->d_partial,
+                   c->d_small, c->st));
+    c->d2h(reinterpret_cast<ef*>(out), c->d_small, 2);
+  });
+}
+
+int whirgpu_sumcheck_fold(whirgpu_ctx* c, uint32_t* d_e, uint32_t* d_w, uint32_t n, const uint32_t* r, uint32_t* out) {
+  return guarded([&] {
+    if (!c || !d_e || !d_w || !r || n < 1 || n > 32) bad("bad argument");
+    use_device(c);
+    ef rr;
+    memcpy(rr.c, r, 16);
+    u64 h = 1ull << (n - 1);
+    KCHK(sc_fold(reinterpret_cast<ef*>(d_e), reinterpret_cast<ef*>(d_w), h, rr, c->d_partial, c->d_small, c->st));
+    if (out && h >= 2) c->d2h(reinterpret_cast<ef*>(out), c->d_small, 2);
+    HIPCHK(hipStreamSynchronize(c->st));
+  });
+}
+
+}  // extern "C"
