@@ -18,6 +18,17 @@ import time
 
 import numpy as np
 
+# MI355X peaks: HBM3E bandwidth, and 32-bit integer VALU lane operations
+# (256 CUs x 64 lanes x 2.4 GHz; 32-bit multiplies issue at a quarter of that).
+HBM_PEAK_GBS = 8000.0
+INT32_PEAK_TOPS = 39.3
+# Integer operations of one width-16 Poseidon2 permutation done the textbook
+# way in Montgomery form: 616 field multiplications (S-boxes, internal
+# diagonal) at 7 int32 ops (4 multiplies, subtract, compare, select) and 1416
+# field additions (round constants, linear layers) at 3 ops (add, subtract, min).
+OPS_PER_PERM = 616 * 7 + 1416 * 3
+VALU_STAGES = ("merkle_leaves", "merkle_tree", "grind")
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -50,6 +61,7 @@ def main():
 
     for _ in range(args.warmup):
         proof = step()
+    cfg.profile(True)
     times = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -58,6 +70,28 @@ def main():
         times.append((time.perf_counter() - s) * 1e3)
     total_ms = (time.perf_counter() - t0) * 1e3
     ms = total_ms / args.steps
+    stages = cfg.profile_read()
+    cfg.profile(False)
+
+    def roof(name):
+        s = stages[name]
+        if not s["launches"] or s["ms"] <= 0:
+            return None
+        avg_s = s["ms"] / s["launches"] * 1e-3
+        if name in VALU_STAGES:
+            per_launch = s["work"] / s["launches"] * OPS_PER_PERM
+            achieved = per_launch / avg_s / 1e12
+            return {"bound": "valu", "kernel": name, "achieved": round(achieved, 3), "peak": INT32_PEAK_TOPS,
+                    "unit": "TOP/s", "frac": round(achieved / INT32_PEAK_TOPS, 4), "traffic": None,
+                    "work_per_launch": round(s["work"] / s["launches"]), "work_unit": "Poseidon2-16 permutations",
+                    "avg_launch_ms": round(s["ms"] / s["launches"], 4)}
+        per_launch = s["work"] / s["launches"]
+        achieved = per_launch / avg_s / 1e9
+        return {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": round(per_launch), "avg_launch_ms": round(s["ms"] / s["launches"], 4)}
+
+    dominant = max(stages, key=lambda k: stages[k]["ms"])
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -91,7 +125,9 @@ def main():
         "proofs_per_s": round(1e3 / ms, 2),
         "step_ms": [round(t, 3) for t in times],
         "proof_bytes": len(proof),
-        "roofline": None,
+        "roofline": roof(dominant),
+        "roofline_by_stage": {k: roof(k) for k in stages if stages[k]["launches"]},
+        "stage_ms_per_step": {k: round(v["ms"] / args.steps, 4) for k, v in stages.items()},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -151,6 +151,17 @@ int whirgpu_prover_data_destroy(whirgpu_prover_data* pd);
  * [lde, merkle, grind, sumcheck, mle, other, total_wall]. */
 int whirgpu_last_timings(const whirgpu_ctx* ctx, double* out7);
 
+/* Live kernel timing with HIP events on the context's stream (no reference
+ * counterpart; the reference's tracing spans are the analogue).  After
+ * enabling, whirgpu_profile_read returns and resets, per stage, the triple
+ * [device ms, launches, work], stages in this order: LDE base (work = bytes),
+ * LDE extension (bytes), Merkle leaves (Poseidon2 permutations), Merkle tree
+ * (permutations), grinding (candidates up to the witness), sumcheck rounds
+ * (bytes), eq-weighted sums (bytes), weight accumulation (bytes), SVO fold
+ * (bytes). */
+int whirgpu_profile_enable(whirgpu_ctx* ctx, int on);
+int whirgpu_profile_read(whirgpu_ctx* ctx, double* out, size_t cap, size_t* len);
+
 /* Device memory for callers that keep inputs resident in HBM (the Rust side
  * would hold such a buffer between proofs).  kind: 0 host->device,
  * 1 device->host, 2 device->device.  No reference counterpart. */

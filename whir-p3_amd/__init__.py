@@ -71,6 +71,9 @@ _SIGS = {
     "whirgpu_prove": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ChallengerState), _VP, _VP]),
     "whirgpu_prover_data_destroy": (ctypes.c_int, [_VP]),
     "whirgpu_last_timings": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double)]),
+    "whirgpu_profile_enable": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "whirgpu_profile_read": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double), ctypes.c_size_t,
+                                            ctypes.POINTER(ctypes.c_size_t)]),
     "whirgpu_device_alloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(_VP)]),
     "whirgpu_device_free": (ctypes.c_int, [ctypes.c_int, _VP]),
     "whirgpu_memcpy": (ctypes.c_int, [ctypes.c_int, _VP, _VP, ctypes.c_size_t, ctypes.c_int]),
@@ -222,6 +225,20 @@ class WhirConfig:
         if observe_domain_separator:
             check(lib().whirgpu_challenger_observe_domain_separator(self.handle, ctypes.byref(ch)))
         return ch
+
+    STAGES = ("lde_base", "lde_ext", "merkle_leaves", "merkle_tree", "grind", "sumcheck", "eq_dot", "combine",
+              "svo_fold")
+
+    def profile(self, on=True):
+        check(lib().whirgpu_profile_enable(self.handle, 1 if on else 0))
+
+    def profile_read(self):
+        out = (ctypes.c_double * 64)()
+        n = ctypes.c_size_t()
+        check(lib().whirgpu_profile_read(self.handle, out, 64, ctypes.byref(n)))
+        v = list(out[:n.value])
+        return {s: {"ms": v[3 * i], "launches": int(v[3 * i + 1]), "work": v[3 * i + 2]}
+                for i, s in enumerate(self.STAGES)}
 
     def timings(self):
         out = (ctypes.c_double * 7)()

@@ -114,12 +114,14 @@ int merkle_leaves_impl(const u32* rows, int row_len, u64 height, u32* tree, cons
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 
-int merkle_commit(const u32* rows, int row_len, u64 height, int leaf_width, const P2C16& c16, const P2C24& c24,
+int merkle_leaves(const u32* rows, int row_len, u64 height, int leaf_width, const P2C16& c16, const P2C24& c24,
                   u32* tree, hipStream_t st) {
   if (height == 0 || (height & (height - 1))) return -1;
-  int rc = leaf_width == 24 ? merkle_leaves_impl<24>(rows, row_len, height, tree, c24, st)
-                            : merkle_leaves_impl<16>(rows, row_len, height, tree, c16, st);
-  if (rc) return rc;
+  return leaf_width == 24 ? merkle_leaves_impl<24>(rows, row_len, height, tree, c24, st)
+                          : merkle_leaves_impl<16>(rows, row_len, height, tree, c16, st);
+}
+
+int merkle_tree(u64 height, const P2C16& c16, u32* tree, hipStream_t st) {
   u64 off = 0, n = height;
   while (n > 1) {
     u64 n_out = n >> 1;
@@ -130,6 +132,12 @@ int merkle_commit(const u32* rows, int row_len, u64 height, int leaf_width, cons
     n = n_out;
   }
   return 0;
+}
+
+int merkle_commit(const u32* rows, int row_len, u64 height, int leaf_width, const P2C16& c16, const P2C24& c24,
+                  u32* tree, hipStream_t st) {
+  int rc = merkle_leaves(rows, row_len, height, leaf_width, c16, c24, tree, st);
+  return rc ? rc : merkle_tree(height, c16, tree, st);
 }
 
 int merkle_open(const u32* rows, int row_len, const u32* tree, u64 height, const u64* d_idx, int nq, u32* d_out,
@@ -150,41 +158,44 @@ int merkle_open(const u32* rows, int row_len, const u32* tree, u64 height, const
 // after observing w, sample_bits(bits) == 0.  Observing w with L < 8 buffered
 // inputs overwrites state[0..L] with the buffer and state[L] with w, and the
 // next sample is state[7] of the permuted state, so each candidate costs one
-// permutation.  Candidates are checked in rounds of gridDim*256; a block stops
-// once the best witness found so far is below its round, so every candidate
-// smaller than the result has been checked.
+// permutation.
+//
+// Candidates are scanned in consecutive chunks, one launch per chunk and one
+// candidate per lane, all on one stream.  A block returns at once when a
+// witness below its chunk is already known, so every candidate smaller than
+// the final result has been checked and the result is the smallest witness.
 struct Sponge16 {
   u32 s[16];
 };
 
-__global__ void __launch_bounds__(256) k_grind(P2C16 c, Sponge16 base, int wpos, u32 mask, u32* result, u32 limit) {
+__global__ void __launch_bounds__(256) k_grind_chunk(P2C16 c, Sponge16 base, int wpos, u32 mask, u32* result,
+                                                     u32 start, u32 limit) {
   __shared__ u32 found_sh;
-  const u64 per_round = (u64)gridDim.x * 256;
-  for (u64 start = 0; start < limit; start += per_round) {
-    if (threadIdx.x == 0) found_sh = __hip_atomic_load(result, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    u32 f = found_sh;
-    __syncthreads();
-    if ((u64)f <= start) break;
-    u64 w = start + (u64)blockIdx.x * 256 + threadIdx.x;
-    if (w < limit) {
-      u32 s[16];
+  if (threadIdx.x == 0) found_sh = __hip_atomic_load(result, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  if (found_sh <= start) return;
+  u64 w = (u64)start + (u64)blockIdx.x * 256 + threadIdx.x;
+  if (w >= limit) return;
+  u32 s[16];
 #pragma unroll
-      for (int k = 0; k < 16; k++) s[k] = base.s[k];
-#pragma unroll
-      for (int k = 0; k < 16; k++)
-        if (k == wpos) s[k] = to_monty((u32)w);
-      permute<16>(s, c);
-      if ((from_monty(s[7]) & mask) == 0) atomicMin(result, (u32)w);
-    }
-  }
+  for (int k = 0; k < 16; k++) s[k] = (k == wpos) ? to_monty((u32)w) : base.s[k];
+  permute<16>(s, c);
+  if ((from_monty(s[7]) & mask) == 0) atomicMin(result, (u32)w);
 }
 
-int grind(const P2C16& c, const u32* state16, int wpos, int bits, u32* d_result, int num_blocks, hipStream_t st) {
+// Enqueues chunks [start, start + nchunks * chunk) (clipped to p).
+int grind_chunks(const P2C16& c, const u32* state16, int wpos, int bits, u32* d_result, u64 start, u64 chunk,
+                 int nchunks, hipStream_t st) {
   Sponge16 b;
   for (int k = 0; k < 16; k++) b.s[k] = state16[k];
   u32 mask = bits >= 32 ? 0xffffffffu : ((1u << bits) - 1);
-  hipLaunchKernelGGL(k_grind, dim3(num_blocks), dim3(256), 0, st, c, b, wpos, mask, d_result, P);
+  for (int i = 0; i < nchunks; i++) {
+    u64 s0 = start + (u64)i * chunk;
+    if (s0 >= P) break;
+    u64 n = std::min<u64>(chunk, P - s0);
+    unsigned blocks = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_grind_chunk, dim3(blocks), dim3(256), 0, st, c, b, wpos, mask, d_result, (u32)s0, P);
+  }
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 

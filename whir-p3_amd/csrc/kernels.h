@@ -30,12 +30,17 @@ int lde_ext(const ef* src, ef* dst, ef* tmp, int m, int k, int R, const Twiddles
 
 // ---- Merkle tree and openings (k_merkle.hip)
 // tree holds 2 * height - 1 digests of 8 u32; the root is the last one.
+int merkle_leaves(const u32* rows, int row_len, u64 height, int leaf_width, const P2C16& c16, const P2C24& c24,
+                  u32* tree, hipStream_t st);
+int merkle_tree(u64 height, const P2C16& c16, u32* tree, hipStream_t st);
 int merkle_commit(const u32* rows, int row_len, u64 height, int leaf_width, const P2C16& c16, const P2C24& c24,
                   u32* tree, hipStream_t st);
 int merkle_open(const u32* rows, int row_len, const u32* tree, u64 height, const u64* d_idx, int nq, u32* d_out,
                 hipStream_t st);
-// *d_result must hold p before the launch; it receives the smallest witness.
-int grind(const P2C16& c, const u32* state16, int wpos, int bits, u32* d_result, int num_blocks, hipStream_t st);
+// Scans [start, start + nchunks * chunk) for the smallest witness.  *d_result
+// must hold p before the first chunk; it receives the smallest witness found.
+int grind_chunks(const P2C16& c, const u32* state16, int wpos, int bits, u32* d_result, u64 start, u64 chunk,
+                 int nchunks, hipStream_t st);
 
 // ---- multilinear kernels (k_poly.hip)
 int eq_table(const PointArg& z, int first, int t, const ef& scale, ef* out, hipStream_t st);

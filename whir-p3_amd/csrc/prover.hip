@@ -111,6 +111,24 @@ struct whirgpu_ctx {
   u32* d_grind = nullptr;
   int grind_blocks = 2048;
   double timings[7] = {0};
+  // live kernel timing: HIP events recorded on `st` around each launch
+  bool prof = false;
+  struct Rec {
+    int stage;
+    hipEvent_t a, b;
+    double work;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> events;
+  size_t next_event = 0;
+  hipEvent_t event() {
+    if (next_event == events.size()) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      events.push_back(e);
+    }
+    return events[next_event++];
+  }
 
   template <typename T>
   Buf<T> alloc(size_t n) {
@@ -181,6 +199,51 @@ struct whirgpu_proof {
 
 namespace {
 
+// Stages reported by whirgpu_profile_read, with the unit of their work count.
+enum Stage {
+  ST_LDE_BASE,   // bytes: 4 * 2^m read + 4 * 2^(m+R) written
+  ST_LDE_EXT,    // bytes: 16 * 2^m read + 16 * 2^(m+R) written
+  ST_LEAVES,     // Poseidon2 permutations
+  ST_TREE,       // Poseidon2 permutations
+  ST_GRIND,      // candidates a sequential search checks (witness + 1)
+  ST_SUMCHECK,   // bytes read and written by coefficient/fold passes
+  ST_EQDOT,      // bytes of the evaluations read
+  ST_COMBINE,    // bytes of weights read and written
+  ST_COMPRESS,   // bytes read and written
+  ST_COUNT
+};
+
+// Times one launch group on the context stream when profiling is enabled.
+struct Span {
+  whirgpu_ctx* c;
+  int stage;
+  double work;
+  hipEvent_t a{};
+  bool open = false;
+  Span(whirgpu_ctx* c_, int stage_, double work_) : c(c_), stage(stage_), work(work_) {
+    if (c->prof) {
+      a = c->event();
+      HIPCHK(hipEventRecord(a, c->st));
+      open = true;
+    }
+  }
+  void close() {
+    if (!open) return;
+    hipEvent_t b = c->event();
+    HIPCHK(hipEventRecord(b, c->st));
+    c->recs.push_back({stage, a, b, work});
+    open = false;
+  }
+  ~Span() {
+    if (open) {
+      try {
+        close();
+      } catch (...) {
+      }
+    }
+  }
+};
+
 PointArg point_arg(const std::vector<ef>& z, int from = 0) {
   PointArg a;
   memset(&a, 0, sizeof a);
@@ -218,7 +281,10 @@ std::vector<ef> base_partials(whirgpu_ctx* c, const u32* d_f, int n, int sb, con
   build_split_eq(c, z_rest, lo, ef_one(), 0, &t, 0);
   int nb = dot_blocks(nrest - lo, sb);
   if ((size_t)nb * (1ull << sb) > c->partial_cap) bad("reduction scratch too small");
-  KCHK(eq_dot_base(d_f, n, sb, lo, t, 1, c->d_partial, nb, c->d_small, c->st));
+  {
+    Span sp(c, ST_EQDOT, 4.0 * (double)(1ull << n));
+    KCHK(eq_dot_base(d_f, n, sb, lo, t, 1, c->d_partial, nb, c->d_small, c->st));
+  }
   std::vector<ef> out(1ull << sb);
   c->d2h(out.data(), c->d_small, out.size());
   return out;
@@ -229,7 +295,10 @@ ef ext_mle(whirgpu_ctx* c, const ef* d_f, int m, const std::vector<ef>& z) {
   EqTables t{};
   build_split_eq(c, z, lo, ef_one(), 0, &t, 0);
   int nb = dot_blocks(m - lo, 0);
-  KCHK(eq_dot_ext(d_f, m, 0, lo, t, 1, c->d_partial, nb, c->d_small, c->st));
+  {
+    Span sp(c, ST_EQDOT, 16.0 * (double)(1ull << m));
+    KCHK(eq_dot_ext(d_f, m, 0, lo, t, 1, c->d_partial, nb, c->d_small, c->st));
+  }
   ef out;
   c->d2h(&out, c->d_small, 1);
   return out;
@@ -247,9 +316,19 @@ u32 gpu_grind(whirgpu_ctx* c, DuplexChallenger& ch, int bits) {
   u32* pin = c->pin<u32>(1);
   *pin = init;
   HIPCHK(hipMemcpyAsync(c->d_grind, pin, 4, hipMemcpyHostToDevice, c->st));
-  KCHK(grind(c->c16, s, ch.in_len, bits, c->d_grind, c->grind_blocks, c->st));
-  u32 w;
-  c->d2h(&w, c->d_grind, 1);
+  // Chunks of about half the expected number of candidates, enqueued in
+  // batches covering about three times that number before each readback.
+  const u64 expect = 1ull << bits;
+  const u64 chunk = std::min<u64>(std::max<u64>(expect / 2, 256), (u64)c->grind_blocks * 256);
+  const int per_batch = (int)std::max<u64>(1, (3 * expect + chunk - 1) / chunk);
+  u32 w = P;
+  Span sp(c, ST_GRIND, 0);
+  for (u64 start = 0; start < P && w >= P; start += chunk * per_batch) {
+    KCHK(grind_chunks(c->c16, s, ch.in_len, bits, c->d_grind, start, chunk, per_batch, c->st));
+    c->d2h(&w, c->d_grind, 1);
+  }
+  sp.close();
+  if (!c->recs.empty() && c->prof) c->recs.back().work = (double)w + 1;
   if (w >= P) bad("no proof-of-work witness below p", WHIRGPU_GRIND_FAILED);
   u32 wm = to_monty(w);
   if (!ch.check_witness(bits, wm)) bad("grinding witness rejected", WHIRGPU_GRIND_FAILED);
@@ -278,11 +357,17 @@ struct SumcheckState {
 ef sumcheck_round(whirgpu_ctx* c, SumcheckState& s, whir::SumcheckData& sd, DuplexChallenger& ch, int pow_bits) {
   if (s.m < 1) bad("sumcheck round on a constant polynomial", WHIRGPU_BAD_STATE);
   u64 h = 1ull << (s.m - 1);
-  if (!s.coeffs_valid) KCHK(sc_coeffs(s.e.p, s.w.p, h, c->d_partial, c->d_small, c->st));
+  if (!s.coeffs_valid) {
+    Span sp(c, ST_SUMCHECK, 64.0 * (double)h);
+    KCHK(sc_coeffs(s.e.p, s.w.p, h, c->d_partial, c->d_small, c->st));
+  }
   ef cc[2];
   c->d2h(cc, c->d_small, 2);
   ef r = observe_and_sample(c, sd, ch, cc[0], cc[1], pow_bits);
-  KCHK(sc_fold(s.e.p, s.w.p, h, r, c->d_partial, c->d_small, c->st));
+  {
+    Span sp(c, ST_SUMCHECK, 96.0 * (double)h);
+    KCHK(sc_fold(s.e.p, s.w.p, h, r, c->d_partial, c->d_small, c->st));
+  }
   s.coeffs_valid = h >= 2;
   s.m -= 1;
   s.sum = whir::extrapolate_012(cc[0], ef_sub(s.sum, cc[0]), cc[1], r);
@@ -297,6 +382,7 @@ void add_eq_weights(whirgpu_ctx* c, SumcheckState& s, const std::vector<std::vec
   size_t K = pts.size();
   if (K == 0 && !accumulate_first) {
     EqTables t{};
+    Span sp(c, ST_COMBINE, 16.0 * (double)(1ull << m));
     KCHK(eq_combine(s.w.p, 1ull << m, lo, t, 0, 0, c->st));
     return;
   }
@@ -304,7 +390,9 @@ void add_eq_weights(whirgpu_ctx* c, SumcheckState& s, const std::vector<std::vec
     EqTables t{};
     int kk = (int)std::min<size_t>(4, K - b);
     for (int k = 0; k < kk; k++) build_split_eq(c, pts[b + k], lo, scales[b + k], k, &t, k);
-    KCHK(eq_combine(s.w.p, 1ull << m, lo, t, kk, (b > 0 || accumulate_first) ? 1 : 0, c->st));
+    bool acc = b > 0 || accumulate_first;
+    Span sp(c, ST_COMBINE, (acc ? 32.0 : 16.0) * (double)(1ull << m));
+    KCHK(eq_combine(s.w.p, 1ull << m, lo, t, kk, acc ? 1 : 0, c->st));
   }
 }
 
@@ -366,7 +454,10 @@ std::vector<ef> initial_sumcheck(whirgpu_ctx* c, const whirgpu_statement* st, Du
     auto eqrs = whir::eq_table_host(rs, ef_one());
     ef* d_eqrs = c->d_tabs + 7 * c->tab_cap;
     c->h2d(d_eqrs, eqrs.data(), eqrs.size());
-    KCHK(compress_multi(st->d_poly, n, l0, d_eqrs, s.e.p, c->st));
+    {
+      Span sp(c, ST_COMPRESS, 4.0 * (double)(1ull << n) + 16.0 * (double)(1ull << (n - l0)));
+      KCHK(compress_multi(st->d_poly, n, l0, d_eqrs, s.e.p, c->st));
+    }
     std::vector<std::vector<ef>> rests;
     std::vector<ef> scales;
     for (size_t j = 0; j < K; j++) {
@@ -428,6 +519,18 @@ void open_rows(whirgpu_ctx* c, const whirgpu_prover_data* pd, const std::vector<
     rows_ef->push_back(std::move(r));
     out->push_back(std::move(o));
   }
+}
+
+// Leaf hashes and the tree above them, timed separately.
+void commit_tree(whirgpu_ctx* c, whirgpu_prover_data* pd) {
+  const int lw = (int)c->params.leaf_hash_width;
+  const int rate = lw - 8;
+  {
+    Span sp(c, ST_LEAVES, (double)pd->height * (double)((pd->row_len + rate - 1) / rate));
+    KCHK(merkle_leaves(pd->rows.p, pd->row_len, pd->height, lw, c->c16, c->c24, pd->tree.p, c->st));
+  }
+  Span sp(c, ST_TREE, (double)(pd->height - 1));
+  KCHK(merkle_tree(pd->height, c->c16, pd->tree.p, c->st));
 }
 
 Digest read_root(whirgpu_ctx* c, const Buf<u32>& tree, u64 height) {
@@ -611,6 +714,7 @@ int whirgpu_destroy(whirgpu_ctx* c) {
                   (void*)c->d_open, (void*)c->d_grind})
     if (p) (void)hipFree(p);
   if (c->h_pin) (void)hipHostFree(c->h_pin);
+  for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
   return WHIRGPU_OK;
@@ -798,8 +902,11 @@ int whirgpu_commit(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs
     pd->ext = false;
     Buf<u32> tmp;
     if (lde_two_pass(n, k0, false)) tmp = c->alloc<u32>(H * W);
-    KCHK(lde_base(st->d_poly, pd->rows.p, tmp.p, n, k0, R, c->tw, c->st));
-    KCHK(merkle_commit(pd->rows.p, (int)W, H, (int)c->params.leaf_hash_width, c->c16, c->c24, pd->tree.p, c->st));
+    {
+      Span sp(c, ST_LDE_BASE, 4.0 * (double)(1ull << n) + 4.0 * (double)(H * W));
+      KCHK(lde_base(st->d_poly, pd->rows.p, tmp.p, n, k0, R, c->tw, c->st));
+    }
+    commit_tree(c, pd);
     if (tmp.p) c->release(tmp);
     Digest root = read_root(c, pd->tree, H);
     proof->p.has_initial_commitment = true;
@@ -873,9 +980,11 @@ int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs,
       pd->ext = true;
       Buf<ef> tmp;
       if (lde_two_pass(m, kn, true)) tmp = c->alloc<ef>(H * W);
-      KCHK(lde_ext(s.e.p, reinterpret_cast<ef*>(pd->rows.p), tmp.p, m, kn, Rl, c->tw, c->st));
-      KCHK(merkle_commit(pd->rows.p, pd->row_len, H, (int)c->params.leaf_hash_width, c->c16, c->c24, pd->tree.p,
-                         c->st));
+      {
+        Span sp(c, ST_LDE_EXT, 16.0 * (double)(1ull << m) + 16.0 * (double)(H * W));
+        KCHK(lde_ext(s.e.p, reinterpret_cast<ef*>(pd->rows.p), tmp.p, m, kn, Rl, c->tw, c->st));
+      }
+      commit_tree(c, pd);
       if (tmp.p) c->release(tmp);
       R.commitment = read_root(c, pd->tree, H);
       observe_digest(ch, R.commitment);
@@ -919,7 +1028,10 @@ int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs,
         int lo = m / 2, hi = m - lo;
         Buf<ef> A = c->alloc<ef>(ns << hi);
         Buf<u32> B = c->alloc<u32>(ns << lo);
-        KCHK(select_combine(s.w.p, m, d_z, d_g, (int)ns, A.p, B.p, c->st));
+        {
+          Span sp(c, ST_COMBINE, 32.0 * (double)(1ull << m));
+          KCHK(select_combine(s.w.p, m, d_z, d_g, (int)ns, A.p, B.p, c->st));
+        }
         HIPCHK(hipStreamSynchronize(c->st));
         c->release(A);
         c->release(B);
@@ -943,6 +1055,37 @@ int whirgpu_last_timings(const whirgpu_ctx* c, double* out7) {
   return guarded([&] {
     if (!c || !out7) bad("null argument");
     memcpy(out7, c->timings, sizeof c->timings);
+  });
+}
+
+int whirgpu_profile_enable(whirgpu_ctx* c, int on) {
+  return guarded([&] {
+    if (!c) bad("null argument");
+    use_device(c);
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->prof = on != 0;
+    c->recs.clear();
+    c->next_event = 0;
+  });
+}
+
+int whirgpu_profile_read(whirgpu_ctx* c, double* out, size_t cap, size_t* len) {
+  return guarded([&] {
+    if (!c || !len) bad("null argument");
+    *len = 3 * ST_COUNT;
+    if (!out || cap < *len) bad("buffer too small", WHIRGPU_BUFFER_TOO_SMALL);
+    use_device(c);
+    HIPCHK(hipStreamSynchronize(c->st));
+    for (size_t i = 0; i < *len; i++) out[i] = 0;
+    for (const auto& r : c->recs) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+      out[3 * r.stage] += ms;
+      out[3 * r.stage + 1] += 1;
+      out[3 * r.stage + 2] += r.work;
+    }
+    c->recs.clear();
+    c->next_event = 0;
   });
 }
 
