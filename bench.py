@@ -33,6 +33,63 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 
+def hbm_roof(stage, name):
+    if not stage["launches"] or stage["ms"] <= 0:
+        return None
+    avg_s = stage["ms"] / stage["launches"] * 1e-3
+    per = stage["work"] / stage["launches"]
+    gbs = per / avg_s / 1e9
+    return {"bound": "hbm", "kernel": name, "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": round(per),
+            "avg_launch_ms": round(stage["ms"] / stage["launches"], 4), "launches": stage["launches"]}
+
+
+def sumcheck_eq_bench(W, args, n=26, l0=3, reps=3):
+    """eq(z, .) expansion over 26 extension variables (2^26 EF4 = 1 GiB) and a
+    26-round sumcheck of f * eq(z, .), f uniform over KoalaBear, the first 3
+    rounds by the split (SVO) route, challenges from a duplex challenger."""
+    import ctypes
+    cfg = W.WhirConfig(W.params(n), device=0)
+    f = W.sample_field(10, 1 << n)
+    z = W.sample_field(11, 4 * n)
+    d_f = W.DeviceBuffer.from_array(f)
+    d_eq = W.DeviceBuffer(16 << n)
+    one = np.array([0x01FFFFFE, 0, 0, 0], np.uint32)   # 1 in Montgomery form
+    polys = np.zeros(8 * n, np.uint32)
+    fin = np.zeros(8, np.uint32)
+
+    def run():
+        W.check(W.lib().whirgpu_eq_expand(cfg.handle, W.u32p(z), n, W.u32p(one), d_eq.ptr))
+        ch = cfg.challenger(observe_domain_separator=False)
+        W.check(W.lib().whirgpu_sumcheck_eq(cfg.handle, d_f.ptr, n, W.u32p(z), l0, None, ctypes.byref(ch),
+                                            W.u32p(polys), W.u32p(fin)))
+
+    run()
+    cfg.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run()
+    wall = (time.perf_counter() - t0) * 1e3 / reps
+    st = cfg.profile_read()
+    cfg.profile(False)
+    sc_ms = sum(st[k]["ms"] for k in ("sumcheck", "eq_dot", "combine", "svo_fold")) / reps
+    out = {
+        "workload": "eq expansion 2^26 EF4 + 26-round sumcheck of f*eq (f base field 2^26), 3 SVO rounds",
+        "eq_expand": hbm_roof(st["eq_expand"], "eq_expand"),
+        "sumcheck_rounds": hbm_roof(st["sumcheck"], "sumcheck fold+coefficients"),
+        "svo_partials": hbm_roof(st["eq_dot"], "SVO partial sums"),
+        "svo_fold": hbm_roof(st["svo_fold"], "SVO 3-variable fold"),
+        "weights": hbm_roof(st["combine"], "eq weights after SVO"),
+        "sumcheck_device_ms": round(sc_ms, 3),
+        "eq_expand_ms": round(st["eq_expand"]["ms"] / reps, 3),
+        "wall_ms_per_rep": round(wall, 3),
+    }
+    d_f.free()
+    d_eq.free()
+    cfg.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -40,6 +97,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--skip-kernel-bench", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -93,6 +151,8 @@ def main():
 
     dominant = max(stages, key=lambda k: stages[k]["ms"])
 
+    kernels = None if args.skip_kernel_bench else sumcheck_eq_bench(W, args)
+
     cpu = None
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -129,6 +189,7 @@ def main():
         "roofline_by_stage": {k: roof(k) for k in stages if stages[k]["launches"]},
         "stage_ms_per_step": {k: round(v["ms"] / args.steps, 4) for k, v in stages.items()},
         "cpu_baseline": cpu,
+        "sumcheck_eq_2^26": kernels,
     }
     print(json.dumps(out))
 

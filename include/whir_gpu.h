@@ -158,7 +158,7 @@ int whirgpu_last_timings(const whirgpu_ctx* ctx, double* out7);
  * LDE extension (bytes), Merkle leaves (Poseidon2 permutations), Merkle tree
  * (permutations), grinding (candidates up to the witness), sumcheck rounds
  * (bytes), eq-weighted sums (bytes), weight accumulation (bytes), SVO fold
- * (bytes). */
+ * (bytes), eq expansion (bytes written). */
 int whirgpu_profile_enable(whirgpu_ctx* ctx, int on);
 int whirgpu_profile_read(whirgpu_ctx* ctx, double* out, size_t cap, size_t* len);
 
@@ -205,6 +205,15 @@ int whirgpu_sumcheck_coeffs(whirgpu_ctx* ctx, const uint32_t* d_evals, const uin
  * `out` (may be NULL) receives the next round's (h(0), h(2)). */
 int whirgpu_sumcheck_fold(whirgpu_ctx* ctx, uint32_t* d_evals, uint32_t* d_weights, uint32_t n, const uint32_t* r,
                           uint32_t* out);
+
+/* Sumcheck of f * eq(point, .) over all n variables: the initial sumcheck of
+ * Sumcheck::from_base_evals (src/sumcheck/sumcheck_prover.rs:164-292) for one
+ * constraint without grinding, the first l0 rounds by the split (SVO) route.
+ * d_f: 2^n base elements on the device.  Challenges: `scripted` (n extension
+ * elements) or, if NULL, *ch (observe h(0) and h(2), then sample).
+ * out_polys: n pairs (h(0), h(2)); out_final: folded f and eq (8 u32). */
+int whirgpu_sumcheck_eq(whirgpu_ctx* ctx, const uint32_t* d_f, uint32_t n, const uint32_t* point, uint32_t l0,
+                        const uint32_t* scripted, whirgpu_challenger* ch, uint32_t* out_polys, uint32_t* out_final);
 
 #ifdef __cplusplus
 }

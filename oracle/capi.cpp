@@ -278,3 +278,37 @@ void orc_xoshiro_outputs(const uint64_t* state4, uint64_t* out, size_t n) {
   for (size_t i = 0; i < n; i++) out[i] = r.next_u64();
 }
 }
+
+extern "C" {
+// Sumcheck of f * eq(z, .) over all n variables by classic rounds on the full
+// tables (ProductPolynomial::round, src/sumcheck/product_polynomial.rs:307-364).
+// Challenges: `scripted` (n EF) or a fresh challenger (Poseidon2 from `seed`)
+// observing (h(0), h(2)) and sampling each round.  Returns 0 when the final
+// claim equals f(r) * eq(z, r).
+int orc_sumcheck_eq(const uint32_t* f, int n, const uint32_t* point, const uint32_t* scripted, uint64_t seed,
+                    uint32_t* out_polys, uint32_t* out_final) {
+  Perms ps(seed);
+  Challenger ch;
+  ch.perm = &ps.p16;
+  std::vector<EF> z = point_in(point, n);
+  std::vector<EF> e(1ull << n), w = eq_table(z, ef_one());
+  for (size_t i = 0; i < e.size(); i++) e[i] = ef_base(from_monty(f[i]));
+  EF sum = ef_zero();
+  for (size_t i = 0; i < e.size(); i++) sum = ef_add(sum, ef_mul(e[i], w[i]));
+  for (int r = 0; r < n; r++) {
+    EF c0, c2;
+    sumcheck_coeffs(e, w, &c0, &c2);
+    ef_out(c0, out_polys + 8 * r);
+    ef_out(c2, out_polys + 8 * r + 4);
+    EF x;
+    if (scripted) x = ef_in(scripted + 4 * r);
+    else { ch.observe_ef(c0); ch.observe_ef(c2); x = ch.sample_ef(); }
+    fold_ext(e, x);
+    fold_ext(w, x);
+    sum = extrapolate_012(c0, ef_sub(sum, c0), c2, x);
+  }
+  ef_out(e[0], out_final);
+  ef_out(w[0], out_final + 4);
+  return ef_eq(ef_mul(e[0], w[0]), sum) ? 0 : 1;
+}
+}

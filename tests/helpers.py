@@ -68,6 +68,7 @@ def oracle():
             "orc_verify": (ctypes.c_int, [ip, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint8), sz, U32P, U32P,
                                           ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),
             "orc_rng_state": (None, [ctypes.c_uint64, U64P]),
+            "orc_sumcheck_eq": (ctypes.c_int, [U32P, ctypes.c_int, U32P, U32P, ctypes.c_uint64, U32P, U32P]),
             "orc_xoshiro_outputs": (None, [U64P, U64P, sz]),
         }
         for name, (res, args) in sigs.items():

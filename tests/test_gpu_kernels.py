@@ -189,3 +189,32 @@ def test_sumcheck_rounds(W, n):
             want2 = np.zeros(8, np.uint32)
             L.orc_sumcheck_coeffs(u32p(cur_e), u32p(cur_w), 1 << (m - 1), u32p(want2))
             assert np.array_equal(nxt, want2)
+
+
+@pytest.mark.parametrize("n,l0", [(1, 0), (3, 0), (6, 2), (12, 3), (12, 4), (16, 3), (20, 3)])
+@pytest.mark.parametrize("mode", ["scripted", "challenger"])
+def test_sumcheck_eq_round_polynomials(W, n, l0, mode):
+    # the kernel-level sumcheck (SVO first rounds, then device folds) against
+    # classic full-table rounds on the oracle
+    cfg = ctx(W, n=max(n, 8))
+    rng = np.random.default_rng(n * 10 + l0)
+    f = rand_monty(rng, 1 << n)
+    z = rand_monty(rng, 4 * n)
+    scripted = rand_monty(rng, 4 * n) if mode == "scripted" else None
+    polys = np.zeros(8 * n, np.uint32)
+    fin = np.zeros(8, np.uint32)
+    d_f = dev(f)
+    if scripted is not None:
+        W.check(W.lib().whirgpu_sumcheck_eq(cfg.handle, d_f.ptr, n, u32p(z), l0, u32p(scripted), None, u32p(polys),
+                                            u32p(fin)))
+    else:
+        ch = cfg.challenger(observe_domain_separator=False)
+        W.check(W.lib().whirgpu_sumcheck_eq(cfg.handle, d_f.ptr, n, u32p(z), l0, None, ctypes.byref(ch), u32p(polys),
+                                            u32p(fin)))
+    want = np.zeros(8 * n, np.uint32)
+    wfin = np.zeros(8, np.uint32)
+    rc = oracle().orc_sumcheck_eq(u32p(f), n, u32p(z), u32p(scripted) if scripted is not None else None, 1,
+                                  u32p(want), u32p(wfin))
+    assert rc == 0
+    assert np.array_equal(polys, want)
+    assert np.array_equal(fin, wfin)

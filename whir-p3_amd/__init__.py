@@ -74,6 +74,8 @@ _SIGS = {
     "whirgpu_profile_enable": (ctypes.c_int, [_VP, ctypes.c_int]),
     "whirgpu_profile_read": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double), ctypes.c_size_t,
                                             ctypes.POINTER(ctypes.c_size_t)]),
+    "whirgpu_sumcheck_eq": (ctypes.c_int, [_VP, _VP, ctypes.c_uint32, U32P, ctypes.c_uint32, U32P,
+                                           ctypes.POINTER(ChallengerState), U32P, U32P]),
     "whirgpu_device_alloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(_VP)]),
     "whirgpu_device_free": (ctypes.c_int, [ctypes.c_int, _VP]),
     "whirgpu_memcpy": (ctypes.c_int, [ctypes.c_int, _VP, _VP, ctypes.c_size_t, ctypes.c_int]),
@@ -227,7 +229,7 @@ class WhirConfig:
         return ch
 
     STAGES = ("lde_base", "lde_ext", "merkle_leaves", "merkle_tree", "grind", "sumcheck", "eq_dot", "combine",
-              "svo_fold")
+              "svo_fold", "eq_expand")
 
     def profile(self, on=True):
         check(lib().whirgpu_profile_enable(self.handle, 1 if on else 0))

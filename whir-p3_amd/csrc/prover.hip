@@ -6,6 +6,7 @@
 // Per sumcheck round only the two round-polynomial values (32 bytes) come
 // back; the challenge goes down as a kernel argument.
 #include <chrono>
+#include <functional>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -210,6 +211,7 @@ enum Stage {
   ST_EQDOT,      // bytes of the evaluations read
   ST_COMBINE,    // bytes of weights read and written
   ST_COMPRESS,   // bytes read and written
+  ST_EQ_EXPAND,  // bytes written
   ST_COUNT
 };
 
@@ -249,6 +251,20 @@ PointArg point_arg(const std::vector<ef>& z, int from = 0) {
   memset(&a, 0, sizeof a);
   for (size_t i = from; i < z.size() && i - from < 32; i++) a.z[i - from] = z[i];
   return a;
+}
+
+void use_device(const whirgpu_ctx* c);
+
+// Grows the eq-table scratch for points of up to n coordinates.
+void ensure_tabs(whirgpu_ctx* c, int n) {
+  size_t need = (1ull << std::max(0, n - 8)) + 512;
+  if (need <= c->tab_cap) return;
+  HIPCHK(hipStreamSynchronize(c->st));
+  ef* t = nullptr;
+  HIPCHK(hipMalloc(&t, 8 * need * sizeof(ef)));
+  (void)hipFree(c->d_tabs);
+  c->d_tabs = t;
+  c->tab_cap = need;
 }
 
 // Split eq tables of z (m coordinates) into the context's table slot `slot`:
@@ -353,8 +369,12 @@ struct SumcheckState {
   bool coeffs_valid = false;   // d_small[0..2] holds (h(0), h(2)) of the current arrays
 };
 
+// Round challenge from the round polynomial (h(0), h(2)): the transcript
+// (observe, grind, sample) in a proof, or a scripted sequence.
+using ChallengeFn = std::function<ef(const ef&, const ef&)>;
+
 // ProductPolynomial::round (product_polynomial.rs:307-364).
-ef sumcheck_round(whirgpu_ctx* c, SumcheckState& s, whir::SumcheckData& sd, DuplexChallenger& ch, int pow_bits) {
+ef sumcheck_round(whirgpu_ctx* c, SumcheckState& s, const ChallengeFn& next) {
   if (s.m < 1) bad("sumcheck round on a constant polynomial", WHIRGPU_BAD_STATE);
   u64 h = 1ull << (s.m - 1);
   if (!s.coeffs_valid) {
@@ -363,7 +383,7 @@ ef sumcheck_round(whirgpu_ctx* c, SumcheckState& s, whir::SumcheckData& sd, Dupl
   }
   ef cc[2];
   c->d2h(cc, c->d_small, 2);
-  ef r = observe_and_sample(c, sd, ch, cc[0], cc[1], pow_bits);
+  ef r = next(cc[0], cc[1]);
   {
     Span sp(c, ST_SUMCHECK, 96.0 * (double)h);
     KCHK(sc_fold(s.e.p, s.w.p, h, r, c->d_partial, c->d_small, c->st));
@@ -404,18 +424,15 @@ void add_eq_weights(whirgpu_ctx* c, SumcheckState& s, const std::vector<std::vec
 // variables s (computed once at evaluate time) give every round polynomial
 //   h_i(X) = eq(z_<i, r_<i) eq(z_i, X) sum_s' eq(z_>i, s') P~(r_<i, X, s'),
 // and afterwards f is folded by all l0 challenges at once.
-std::vector<ef> initial_sumcheck(whirgpu_ctx* c, const whirgpu_statement* st, DuplexChallenger& ch,
-                                 whir::SumcheckData& sd, SumcheckState& s) {
+// `ap` holds the combination weights alpha^j of the constraints.
+std::vector<ef> initial_sumcheck(whirgpu_ctx* c, const whirgpu_statement* st, int l0, const std::vector<ef>& ap,
+                                 const ChallengeFn& next, SumcheckState& s) {
   const int n = st->n;
-  const int l0 = c->cfg.ff(0);
-  const int pow = c->cfg.starting_folding_pow_bits;
   const size_t K = st->points.size();
-  ef alpha = ch.sample_ef();
-  auto ap = whir::powers(alpha, K);
   s.sum = ef_zero();
   for (size_t j = 0; j < K; j++) s.sum = ef_add(s.sum, ef_mul(ap[j], st->evals[j]));
   std::vector<ef> rs;
-  if (st->seg_bits == l0 && n > l0) {
+  if (l0 > 0 && st->seg_bits == l0 && n > l0) {
     std::vector<std::vector<ef>> P = st->partials;
     for (int i = 0; i < l0; i++) {
       ef c0 = ef_zero(), c2 = ef_zero();
@@ -438,7 +455,7 @@ std::vector<ef> initial_sumcheck(whirgpu_ctx* c, const whirgpu_statement* st, Du
         c0 = ef_add(c0, ef_mul(ap[j], ef_mul(pre, ef_mul(e0, A0))));
         c2 = ef_add(c2, ef_mul(ap[j], ef_mul(pre, ef_mul(e2, A2))));
       }
-      ef r = observe_and_sample(c, sd, ch, c0, c2, pow);
+      ef r = next(c0, c2);
       s.sum = whir::extrapolate_012(c0, ef_sub(s.sum, c0), c2, r);
       rs.push_back(r);
       for (auto& Pj : P) {
@@ -477,7 +494,7 @@ std::vector<ef> initial_sumcheck(whirgpu_ctx* c, const whirgpu_statement* st, Du
   KCHK(base_to_ext(st->d_poly, 1ull << n, s.e.p, c->st));
   add_eq_weights(c, s, st->points, ap, false);
   s.coeffs_valid = false;
-  for (int i = 0; i < l0; i++) rs.push_back(sumcheck_round(c, s, sd, ch, pow));
+  for (int i = 0; i < l0; i++) rs.push_back(sumcheck_round(c, s, next));
   return rs;
 }
 
@@ -943,7 +960,14 @@ int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs,
     const int n = cfg.num_variables;
     auto& P = proof->p;
     P.rounds.assign(cfg.n_rounds(), whir::WhirRoundProof());
-    std::vector<ef> fold_rand = initial_sumcheck(c, st, ch, P.initial_sumcheck, s);
+    const int pow0 = cfg.starting_folding_pow_bits;
+    ef alpha = ch.sample_ef();
+    std::vector<ef> fold_rand =
+        initial_sumcheck(c, st, cfg.ff(0), whir::powers(alpha, st->points.size()),
+                         [&](const ef& c0, const ef& c2) {
+                           return observe_and_sample(c, P.initial_sumcheck, ch, c0, c2, pow0);
+                         },
+                         s);
     for (int round = 0; round <= cfg.n_rounds(); round++) {
       const int m = n - cfg.params.folding_factor.total_number(round);
       if (m != s.m) bad("internal: round size mismatch", WHIRGPU_BAD_STATE);
@@ -960,8 +984,11 @@ int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs,
         open_rows(c, prev, idx, &P.final_queries, &rows);
         if (cfg.final_sumcheck_rounds > 0) {
           P.has_final_sumcheck = true;
+          const int fpow = cfg.final_folding_pow_bits;
           for (int i = 0; i < cfg.final_sumcheck_rounds; i++)
-            sumcheck_round(c, s, P.final_sumcheck, ch, cfg.final_folding_pow_bits);
+            sumcheck_round(c, s, [&](const ef& c0, const ef& c2) {
+              return observe_and_sample(c, P.final_sumcheck, ch, c0, c2, fpow);
+            });
         }
         break;
       }
@@ -1039,7 +1066,11 @@ int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs,
       }
       s.coeffs_valid = false;
       fold_rand.clear();
-      for (int i = 0; i < kn; i++) fold_rand.push_back(sumcheck_round(c, s, R.sumcheck, ch, rp.folding_pow_bits));
+      const int fpow = rp.folding_pow_bits;
+      for (int i = 0; i < kn; i++)
+        fold_rand.push_back(sumcheck_round(c, s, [&](const ef& c0, const ef& c2) {
+          return observe_and_sample(c, R.sumcheck, ch, c0, c2, fpow);
+        }));
     }
     HIPCHK(hipStreamSynchronize(c->st));
     store_ch(ch, chs);
@@ -1183,9 +1214,11 @@ int whirgpu_eq_expand(whirgpu_ctx* c, const uint32_t* point, uint32_t n, const u
     int lo = (int)n / 2, hi = (int)n - lo;
     Buf<ef> th = c->alloc<ef>(1ull << hi), tl = c->alloc<ef>(1ull << lo);
     PointArg a = point_arg(z);
+    Span sp(c, ST_EQ_EXPAND, 16.0 * (double)(1ull << n));
     KCHK(eq_table(a, 0, hi, sc, th.p, c->st));
     KCHK(eq_table(a, hi, lo, ef_one(), tl.p, c->st));
     KCHK(eq_outer(th.p, tl.p, lo, 1ull << n, reinterpret_cast<ef*>(d_out), c->st));
+    sp.close();
     HIPCHK(hipStreamSynchronize(c->st));
     c->release(th);
     c->release(tl);
@@ -1233,6 +1266,57 @@ int whirgpu_sumcheck_fold(whirgpu_ctx* c, uint32_t* d_e, uint32_t* d_w, uint32_t
     if (out && h >= 2) c->d2h(reinterpret_cast<ef*>(out), c->d_small, 2);
     HIPCHK(hipStreamSynchronize(c->st));
   });
+}
+
+// Sumcheck of f * eq(z, .) over all n variables (the initial sumcheck of a
+// proof with one constraint and no grinding), l0 leading rounds by the split
+// (SVO) route.  Challenges come from `scripted` (n extension elements) or,
+// when it is NULL, from *ch (observe h(0), h(2); sample).  out_polys receives
+// n pairs (h(0), h(2)); out_final the folded f and eq values (8 u32).
+int whirgpu_sumcheck_eq(whirgpu_ctx* c, const uint32_t* d_f, uint32_t n, const uint32_t* point, uint32_t l0,
+                        const uint32_t* scripted, whirgpu_challenger* chs, uint32_t* out_polys, uint32_t* out_final) {
+  SumcheckState s;
+  int rc = guarded([&] {
+    if (!c || !d_f || !point || !out_polys || !out_final || n < 1 || n > 30 || l0 > n) bad("bad argument");
+    if (!scripted && !chs) bad("need scripted challenges or a challenger");
+    use_device(c);
+    ensure_tabs(c, (int)n);
+    whirgpu_statement st;
+    st.ctx = c;
+    st.n = (int)n;
+    st.d_poly = const_cast<u32*>(d_f);
+    st.seg_bits = (l0 > 0 && n > l0) ? (int)l0 : 0;
+    statement_evaluate(&st, read_point(point, n));
+    DuplexChallenger ch;
+    if (chs) ch = load_ch(c, chs);
+    size_t round = 0;
+    ChallengeFn next = [&](const ef& c0, const ef& c2) {
+      memcpy(out_polys + 8 * round, c0.c, 16);
+      memcpy(out_polys + 8 * round + 4, c2.c, 16);
+      ef r;
+      if (scripted) {
+        memcpy(r.c, scripted + 4 * round, 16);
+      } else {
+        ch.observe_ef(c0);
+        ch.observe_ef(c2);
+        r = ch.sample_ef();
+      }
+      round++;
+      return r;
+    };
+    initial_sumcheck(c, &st, (int)l0, {ef_one()}, next, s);
+    while (s.m > 0) sumcheck_round(c, s, next);
+    ef fin[2];
+    c->d2h(&fin[0], s.e.p, 1);
+    c->d2h(&fin[1], s.w.p, 1);
+    memcpy(out_final, fin[0].c, 16);
+    memcpy(out_final + 4, fin[1].c, 16);
+    if (!ef_eq(ef_mul(fin[0], fin[1]), s.sum)) bad("internal: final sumcheck claim mismatch", WHIRGPU_BAD_STATE);
+    if (chs) store_ch(ch, chs);
+  });
+  if (s.e.p) c->release(s.e);
+  if (s.w.p) c->release(s.w);
+  return rc;
 }
 
 }  // extern "C"
