@@ -90,6 +90,91 @@ def sumcheck_eq_bench(W, args, n=26, l0=3, reps=3):
     return out
 
 
+def batched_bench(W, workers=16, count=64, seed=2024):
+    """64 proofs with num_variables uniform in {16..22}, one constraint each,
+    polynomials resident in HBM: proofs/s one at a time (one context per size,
+    proofs back to back) against proofs/s through the batching server."""
+    rng = np.random.default_rng(seed)
+    ns = [int(x) for x in rng.integers(16, 23, size=count)]
+    bufs = [W.DeviceBuffer.from_array(W.sample_field(2000 + i, 1 << n)) for i, n in enumerate(ns)]
+    pts = [[W.sample_field(3000 + i, 4 * n).reshape(n, 4)] for i, n in enumerate(ns)]
+    cfgs = {n: W.WhirConfig(W.params(n), device=0) for n in sorted(set(ns))}
+    for i, n in enumerate(ns):   # warm every context once
+        if n in cfgs and not getattr(cfgs[n], "_warm", False):
+            W.prove_once(cfgs[n], None, pts[i], device_ptr=bufs[i].ptr)
+            cfgs[n]._warm = True
+    t = time.perf_counter()
+    seq = [W.prove_once(cfgs[n], None, pts[i], device_ptr=bufs[i].ptr)[0] for i, n in enumerate(ns)]
+    seq_s = time.perf_counter() - t
+    for c in cfgs.values():
+        c.close()
+    srv = W.Server(W.params(0), workers=workers)
+
+    def run_batch():
+        reqs = [W.ProofRequest(None, pts[i], device_ptr=bufs[i].ptr, num_variables=n) for i, n in enumerate(ns)]
+        t0 = time.perf_counter()
+        for r in reqs:
+            srv.submit(r)
+        for r in reqs:
+            assert srv.wait(r) == 0, W.lib().whirgpu_last_error()
+        return reqs, time.perf_counter() - t0
+
+    run_batch()                   # warm the workers' contexts
+    reqs, bat_s = run_batch()
+    srv.close()
+    same = all(r.proof() == s for r, s in zip(reqs, seq))
+    lat = sorted(r.job.latency_ms for r in reqs)
+    for b in bufs:
+        b.free()
+    return {"workload": f"{count} proofs, num_variables uniform in {{16..22}} (seed {seed}), 1 constraint each, "
+                        f"folding 4, rate 1/2, rs-reduction 3, lambda 100",
+            "one_at_a_time_proofs_per_s": round(count / seq_s, 2), "batched_proofs_per_s": round(count / bat_s, 2),
+            "speedup": round(seq_s / bat_s, 2), "workers": workers, "bit_identical_to_one_at_a_time": same,
+            "batch_latency_ms_p50": round(lat[len(lat) // 2], 3), "batch_latency_ms_max": round(lat[-1], 3)}
+
+
+def serving_bench(W, workers=16, rate=200.0, count=200, seed=7):
+    """Poisson arrivals at `rate` req/s, num_variables log-uniform in [16, 24],
+    1-4 constraints each, host-resident inputs uploaded per request, admission
+    by free HBM; latency is submit to proof bytes on the host."""
+    rng = np.random.default_rng(seed)
+    ns = [int(x) for x in np.rint(np.exp(rng.uniform(np.log(16), np.log(24), count)))]
+    ks = [int(x) for x in rng.integers(1, 5, count)]
+    gaps = rng.exponential(1.0 / rate, count)
+    polys = [W.sample_field(5000 + i, 1 << n) for i, n in enumerate(ns)]
+    pts = [[W.sample_field(6000 + 10 * i + j, 4 * n).reshape(n, 4) for j in range(k)]
+           for i, (n, k) in enumerate(zip(ns, ks))]
+    srv = W.Server(W.params(0), workers=workers)
+    warm = [W.ProofRequest(polys[i], pts[i]) for i in range(count)]
+    for r in warm:
+        srv.submit(r)
+    for r in warm:
+        assert srv.wait(r) == 0
+    reqs = [W.ProofRequest(polys[i], pts[i]) for i in range(count)]
+    sub = []
+    t0 = time.perf_counter()
+    due = t0
+    for r, g in zip(reqs, gaps):
+        due += g
+        d = due - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        sub.append(time.perf_counter())
+        srv.submit(r)
+    for r in reqs:
+        assert srv.wait(r) == 0
+    srv.close()
+    lat = np.array([r.job.latency_ms for r in reqs])
+    done = max(s + r.job.latency_ms / 1e3 for s, r in zip(sub, reqs))
+    same = all(a.proof() == b.proof() for a, b in zip(warm, reqs))
+    return {"workload": f"Poisson {rate:.0f} req/s, {count} requests, num_variables log-uniform in [16,24] "
+                        f"(seed {seed}), 1-4 constraints, host inputs, admission by free HBM",
+            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "proofs_per_s": round(count / (done - t0), 2), "offered_rate": rate, "workers": workers,
+            "mean_queue_ms": round(float(np.mean([r.job.queue_ms for r in reqs])), 3),
+            "same_alone_and_in_mix": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +183,8 @@ def main():
     ap.add_argument("--n", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-kernel-bench", action="store_true")
+    ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--workers", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -152,6 +239,8 @@ def main():
     dominant = max(stages, key=lambda k: stages[k]["ms"])
 
     kernels = None if args.skip_kernel_bench else sumcheck_eq_bench(W, args)
+    batched = None if args.skip_batch else batched_bench(W, workers=args.workers)
+    serving = None if args.skip_batch else serving_bench(W, workers=args.workers)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -190,6 +279,8 @@ def main():
         "stage_ms_per_step": {k: round(v["ms"] / args.steps, 4) for k, v in stages.items()},
         "cpu_baseline": cpu,
         "sumcheck_eq_2^26": kernels,
+        "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(out))
 

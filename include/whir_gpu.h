@@ -206,6 +206,50 @@ int whirgpu_sumcheck_coeffs(whirgpu_ctx* ctx, const uint32_t* d_evals, const uin
 int whirgpu_sumcheck_fold(whirgpu_ctx* ctx, uint32_t* d_evals, uint32_t* d_weights, uint32_t n, const uint32_t* r,
                           uint32_t* out);
 
+/* ---------------- batched and served proving (no reference counterpart) ----------------
+ * A job is one commit + prove with its own number of variables and
+ * constraints, the transcript starting from a fresh challenger that has
+ * observed the domain separator (benches/whir.rs:139-186).  The job struct is
+ * owned by the caller and must stay valid until the job completes. */
+typedef struct whirgpu_job {
+  uint32_t num_variables;
+  const uint32_t* evals;        /* 2^n base elements (Montgomery) */
+  int evals_on_device;          /* nonzero: evals is a device pointer */
+  const uint32_t* points;       /* npoints * n extension elements */
+  uint32_t npoints;
+  uint32_t* evals_out;          /* npoints extension elements, or NULL */
+  uint8_t* proof_out;           /* caller buffer for the proof bytes */
+  size_t proof_cap;
+  size_t proof_len;             /* out */
+  int status;                   /* out: WHIRGPU_PENDING until done, then a WHIRGPU_* code */
+  double queue_ms;              /* out: submit to start */
+  double latency_ms;            /* out: submit to completion */
+} whirgpu_job;
+
+#define WHIRGPU_PENDING (-1)
+
+typedef struct whirgpu_server whirgpu_server;
+
+/* A pool of `workers` host threads on device `device`, each with its own
+ * contexts and HIP stream, taking jobs in submission order.  A job starts when
+ * a worker is free and its estimated device memory fits in `hbm_budget` bytes
+ * beside the jobs already running (0: 80% of the free HBM at creation).
+ * params->num_variables is ignored (each job has its own). */
+int whirgpu_server_create(int device, const whirgpu_params* params, const whirgpu_poseidon2* perms, int workers,
+                          size_t hbm_budget, whirgpu_server** out);
+int whirgpu_server_submit(whirgpu_server* s, whirgpu_job* job);
+/* Blocks until `job` completes (timeout_ms < 0: no limit); returns its status
+ * or WHIRGPU_PENDING on timeout. */
+int whirgpu_server_wait(whirgpu_server* s, whirgpu_job* job, int timeout_ms);
+/* Jobs queued / running right now, and the device-memory estimate in use. */
+int whirgpu_server_stats(whirgpu_server* s, size_t* queued, size_t* running, size_t* bytes_in_use);
+/* Finishes every submitted job, then stops the workers. */
+int whirgpu_server_destroy(whirgpu_server* s);
+
+/* Submits all jobs to a temporary server and waits for all of them. */
+int whirgpu_prove_batch(int device, const whirgpu_params* params, const whirgpu_poseidon2* perms, whirgpu_job* jobs,
+                        size_t njobs, int workers);
+
 /* Sumcheck of f * eq(point, .) over all n variables: the initial sumcheck of
  * Sumcheck::from_base_evals (src/sumcheck/sumcheck_prover.rs:164-292) for one
  * constraint without grinding, the first l0 rounds by the split (SVO) route.

@@ -42,6 +42,15 @@ class ChallengerState(ctypes.Structure):
                 ("input_len", ctypes.c_uint32), ("output_len", ctypes.c_uint32)]
 
 
+class Job(ctypes.Structure):
+    """whirgpu_job: one commit + prove handled by a server."""
+    _fields_ = [("num_variables", ctypes.c_uint32), ("evals", ctypes.c_void_p), ("evals_on_device", ctypes.c_int),
+                ("points", ctypes.c_void_p), ("npoints", ctypes.c_uint32), ("evals_out", ctypes.c_void_p),
+                ("proof_out", ctypes.c_void_p), ("proof_cap", ctypes.c_size_t), ("proof_len", ctypes.c_size_t),
+                ("status", ctypes.c_int), ("queue_ms", ctypes.c_double), ("latency_ms", ctypes.c_double)]
+
+
+PENDING = -1
 SOUNDNESS = {"UniqueDecoding": 0, "JohnsonBound": 1, "CapacityBound": 2}
 
 # Function signatures of include/whir_gpu.h: name -> (restype, argtypes)
@@ -76,6 +85,15 @@ _SIGS = {
                                             ctypes.POINTER(ctypes.c_size_t)]),
     "whirgpu_sumcheck_eq": (ctypes.c_int, [_VP, _VP, ctypes.c_uint32, U32P, ctypes.c_uint32, U32P,
                                            ctypes.POINTER(ChallengerState), U32P, U32P]),
+    "whirgpu_server_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Params), ctypes.POINTER(Poseidon2Constants),
+                                             ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(_VP)]),
+    "whirgpu_server_submit": (ctypes.c_int, [_VP, ctypes.POINTER(Job)]),
+    "whirgpu_server_wait": (ctypes.c_int, [_VP, ctypes.POINTER(Job), ctypes.c_int]),
+    "whirgpu_server_stats": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t),
+                                            ctypes.POINTER(ctypes.c_size_t)]),
+    "whirgpu_server_destroy": (ctypes.c_int, [_VP]),
+    "whirgpu_prove_batch": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Params), ctypes.POINTER(Poseidon2Constants),
+                                           ctypes.POINTER(Job), ctypes.c_size_t, ctypes.c_int]),
     "whirgpu_device_alloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(_VP)]),
     "whirgpu_device_free": (ctypes.c_int, [ctypes.c_int, _VP]),
     "whirgpu_memcpy": (ctypes.c_int, [ctypes.c_int, _VP, _VP, ctypes.c_size_t, ctypes.c_int]),
@@ -338,3 +356,80 @@ def prove_once(cfg, poly, points=(), device_ptr=None):
     proof.close()
     st.close()
     return out, evals
+
+
+class ProofRequest:
+    """Host-side holder of one job's inputs and output buffer (kept alive while the job runs)."""
+
+    def __init__(self, poly=None, points=(), device_ptr=None, num_variables=None, proof_cap=1 << 22):
+        if device_ptr is None:
+            self.poly = np.ascontiguousarray(poly, dtype=np.uint32)
+            n = int(self.poly.size).bit_length() - 1
+        else:
+            self.poly = None
+            n = num_variables
+        self.n = n
+        pts = [np.ascontiguousarray(p, dtype=np.uint32).reshape(-1) for p in points]
+        self.points = np.ascontiguousarray(np.concatenate(pts) if pts else np.zeros(4, np.uint32))
+        self.evals = np.zeros(4 * max(len(pts), 1), np.uint32)
+        self.buf = (ctypes.c_uint8 * proof_cap)()
+        self.job = Job()
+        j = self.job
+        j.num_variables = n
+        j.evals = device_ptr if device_ptr is not None else self.poly.ctypes.data
+        j.evals_on_device = 1 if device_ptr is not None else 0
+        j.points = self.points.ctypes.data
+        j.npoints = len(pts)
+        j.evals_out = self.evals.ctypes.data
+        j.proof_out = ctypes.addressof(self.buf)
+        j.proof_cap = proof_cap
+        j.status = PENDING
+
+    def proof(self):
+        return bytes(self.buf[: self.job.proof_len])
+
+
+class Server:
+    """whirgpu_server: continuous proving with admission by device memory."""
+
+    def __init__(self, prm, perm_seed=1, device=0, workers=8, hbm_budget=0):
+        self.perms = poseidon2_from_seed(perm_seed)
+        self.prm = prm
+        h = ctypes.c_void_p()
+        check(lib().whirgpu_server_create(device, ctypes.byref(prm), ctypes.byref(self.perms), workers, hbm_budget,
+                                          ctypes.byref(h)))
+        self.handle = h
+
+    def submit(self, req):
+        check(lib().whirgpu_server_submit(self.handle, ctypes.byref(req.job)))
+
+    def wait(self, req, timeout_ms=-1):
+        return lib().whirgpu_server_wait(self.handle, ctypes.byref(req.job), timeout_ms)
+
+    def stats(self):
+        q, r, b = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
+        check(lib().whirgpu_server_stats(self.handle, ctypes.byref(q), ctypes.byref(r), ctypes.byref(b)))
+        return q.value, r.value, b.value
+
+    def close(self):
+        if self.handle:
+            lib().whirgpu_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def prove_batch(prm, requests, workers=8, perm_seed=1, device=0):
+    """whirgpu_prove_batch over ProofRequest objects; returns the first error code (0 if all succeeded)."""
+    perms = poseidon2_from_seed(perm_seed)
+    arr = (Job * len(requests))()
+    for i, r in enumerate(requests):
+        arr[i] = r.job
+    rc = lib().whirgpu_prove_batch(device, ctypes.byref(prm), ctypes.byref(perms), arr, len(requests), workers)
+    for i, r in enumerate(requests):
+        r.job = arr[i]
+    return rc
