@@ -16,6 +16,7 @@
 #include "kcommon.h"
 #include "kernels.h"
 #include "poseidon2.h"
+#include "poseidon2_wide.h"
 
 namespace kb {
 
@@ -89,6 +90,46 @@ __global__ void __launch_bounds__(256) k_tree_level(const u32* __restrict__ in, 
   o[1] = make_uint4(s[4], s[5], s[6], s[7]);
 }
 
+// One tree level with a 16-lane permutation per node (small levels, where
+// latency rather than throughput decides).  Grid: ceil(16 n_out / 256).
+__global__ void __launch_bounds__(256) k_tree_level_wide(const u32* __restrict__ in, u64 n_out, u32* __restrict__ out,
+                                                         P2C16 c) {
+  const u64 t = (u64)blockIdx.x * 256 + threadIdx.x;
+  const u64 p = t >> 4;
+  const int j = threadIdx.x & 15;
+  const WideLane w = wide_lane(c, j);
+  u32 x = p < n_out ? in[16 * p + j] : 0u;
+  x = permute16_wide(x, j, w, c);
+  if (p < n_out && j < 8) out[8 * p + j] = x;
+}
+
+// The top levels of a tree in one launch: n (<= 128) digests at `off` are
+// staged in LDS and reduced to the root, every level written out.  One block
+// of 1024 lanes, 64 nodes per step.
+__global__ void __launch_bounds__(1024) k_tree_top_wide(u32* __restrict__ tree, u64 off, int n, P2C16 c) {
+  __shared__ u32 sh[2][128 * 8];
+  const int t = threadIdx.x;
+  const int j = t & 15;
+  const WideLane w = wide_lane(c, j);
+  for (int e = t; e < n * 8; e += 1024) sh[0][e] = tree[off * 8 + e];
+  __syncthreads();
+  int cur = 0;
+  while (n > 1) {
+    const int n_out = n >> 1;
+    const int p = t >> 4;
+    u32 x = p < n_out ? sh[cur][16 * p + j] : 0u;
+    x = permute16_wide(x, j, w, c);
+    if (p < n_out && j < 8) {
+      sh[cur ^ 1][8 * p + j] = x;
+      tree[(off + n + p) * 8 + j] = x;
+    }
+    __syncthreads();
+    off += n;
+    n = n_out;
+    cur ^= 1;
+  }
+}
+
 // Opening of `nq` rows: out[q] = row (row_len) followed by depth sibling digests.
 __global__ void k_open(const u32* __restrict__ rows, int row_len, const u32* __restrict__ tree, u64 height, int depth,
                        const u64* __restrict__ idx, int nq, u32* __restrict__ out) {
@@ -121,15 +162,28 @@ int merkle_leaves(const u32* rows, int row_len, u64 height, int leaf_width, cons
                           : merkle_leaves_impl<16>(rows, row_len, height, tree, c16, st);
 }
 
+// Levels with more than 2^14 nodes use one lane per node (throughput), smaller
+// ones 16 lanes per node (latency), and the last 7 levels run in one launch.
 int merkle_tree(u64 height, const P2C16& c16, u32* tree, hipStream_t st) {
   u64 off = 0, n = height;
-  while (n > 1) {
+  while (n > 128) {
     u64 n_out = n >> 1;
-    unsigned blocks = (unsigned)((n_out + 255) / 256);
-    hipLaunchKernelGGL(k_tree_level, dim3(blocks), dim3(256), 0, st, tree + off * 8, n_out, tree + (off + n) * 8, c16);
+    if (n_out > (1u << 14)) {
+      unsigned blocks = (unsigned)((n_out + 255) / 256);
+      hipLaunchKernelGGL(k_tree_level, dim3(blocks), dim3(256), 0, st, tree + off * 8, n_out, tree + (off + n) * 8,
+                         c16);
+    } else {
+      unsigned blocks = (unsigned)((n_out * 16 + 255) / 256);
+      hipLaunchKernelGGL(k_tree_level_wide, dim3(blocks), dim3(256), 0, st, tree + off * 8, n_out,
+                         tree + (off + n) * 8, c16);
+    }
     if (hipGetLastError() != hipSuccess) return -5;
     off += n;
     n = n_out;
+  }
+  if (n > 1) {
+    hipLaunchKernelGGL(k_tree_top_wide, dim3(1), dim3(1024), 0, st, tree, off, (int)n, c16);
+    if (hipGetLastError() != hipSuccess) return -5;
   }
   return 0;
 }

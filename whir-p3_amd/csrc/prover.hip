@@ -161,15 +161,33 @@ struct whirgpu_ctx {
   void d2h(T* dst, const T* src, size_t n) {
     T* p = pin<T>(n);
     HIPCHK(hipMemcpyAsync(p, src, n * sizeof(T), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    sync();
     memcpy(dst, p, n * sizeof(T));
+  }
+  // Small host->device uploads are staged in a pinned ring and not waited
+  // for: a slot is reused only after a stream synchronization, by which time
+  // every earlier copy has completed.
+  unsigned char* h_ring = nullptr;
+  size_t ring_cap = 0, ring_pos = 0;
+  void sync() {
+    HIPCHK(hipStreamSynchronize(st));
+    ring_pos = 0;
   }
   template <typename T>
   void h2d(T* dst, const T* src, size_t n) {
-    T* p = pin<T>(n);
-    memcpy(p, src, n * sizeof(T));
-    HIPCHK(hipMemcpyAsync(dst, p, n * sizeof(T), hipMemcpyHostToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));
+    size_t bytes = n * sizeof(T);
+    size_t need = (bytes + 255) & ~size_t(255);
+    if (need > ring_cap) {
+      T* p = pin<T>(n);
+      memcpy(p, src, bytes);
+      HIPCHK(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, st));
+      sync();
+      return;
+    }
+    if (ring_pos + need > ring_cap) sync();
+    memcpy(h_ring + ring_pos, src, bytes);
+    HIPCHK(hipMemcpyAsync(dst, h_ring + ring_pos, bytes, hipMemcpyHostToDevice, st));
+    ring_pos += need;
   }
 };
 
@@ -328,10 +346,8 @@ u32 gpu_grind(whirgpu_ctx* c, DuplexChallenger& ch, int bits) {
   u32 s[16];
   for (int i = 0; i < 16; i++) s[i] = ch.sponge[i];
   for (int i = 0; i < ch.in_len; i++) s[i] = ch.in[i];
-  u32 init = P;
-  u32* pin = c->pin<u32>(1);
-  *pin = init;
-  HIPCHK(hipMemcpyAsync(c->d_grind, pin, 4, hipMemcpyHostToDevice, c->st));
+  const u32 init = P;
+  c->h2d(c->d_grind, &init, 1);
   // Chunks of about half the expected number of candidates, enqueued in
   // batches covering about three times that number before each readback.
   const u64 expect = 1ull << bits;
@@ -713,6 +729,8 @@ int whirgpu_create(int device, const whirgpu_params* params, const whirgpu_posei
     HIPCHK(hipGetDeviceProperties(&prop, device));
     c->grind_blocks = std::max(64, prop.multiProcessorCount * 8);
     c->pin<u32>(1);
+    c->ring_cap = 1 << 20;
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_ring), c->ring_cap, hipHostMallocDefault));
   });
   if (rc) {
     if (c) whirgpu_destroy(c);
@@ -731,6 +749,7 @@ int whirgpu_destroy(whirgpu_ctx* c) {
                   (void*)c->d_open, (void*)c->d_grind})
     if (p) (void)hipFree(p);
   if (c->h_pin) (void)hipHostFree(c->h_pin);
+  if (c->h_ring) (void)hipHostFree(c->h_ring);
   for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
@@ -1059,8 +1078,7 @@ int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs,
           Span sp(c, ST_COMBINE, 32.0 * (double)(1ull << m));
           KCHK(select_combine(s.w.p, m, d_z, d_g, (int)ns, A.p, B.p, c->st));
         }
-        HIPCHK(hipStreamSynchronize(c->st));
-        c->release(A);
+        c->release(A);   // reuse is ordered on the same stream
         c->release(B);
         for (size_t j = 0; j < ns; j++) s.sum = ef_add(s.sum, ef_mul(g[j], sel_v[j]));
       }
