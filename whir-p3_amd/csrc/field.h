@@ -35,21 +35,29 @@ __host__ __device__ __forceinline__ u32 neg(u32 a) { return a ? P - a : 0; }
 __host__ __device__ __forceinline__ u32 dbl(u32 a) { return add(a, a); }
 
 // Montgomery reduction of x < p * 2^32: returns x / 2^32 mod p in [0, p).
+// With t = x * p^-1 mod 2^32, x - t p is divisible by 2^32 and the quotient
+// r = hi(x) - hi(t p) lies in (-p, p); as u32, r + p wraps below r exactly
+// when r < 0, so min(r, r + p) is the reduced value.
 __host__ __device__ __forceinline__ u32 redc(u64 x) {
   u32 xl = (u32)x, xh = (u32)(x >> 32);
   u32 t = xl * MU;
   u32 uh = (u32)(((u64)t * P) >> 32);
   u32 r = xh - uh;
-  return xh < uh ? r + P : r;
+  u32 r2 = r + P;
+  return r2 < r ? r2 : r;
 }
-// Montgomery reduction of x < 4 p^2 (a sum of up to four products).
+// Montgomery reduction of x < 4 p^2 (a sum of up to four products).  Here the
+// quotient lies in (-p, 2p), so the min trick of redc() does not apply to the
+// sign: a borrow (r > xh) marks r < 0, which adding p folds into (0, p);
+// min(r, r - p) then folds [p, 2p) into [0, p).
 __host__ __device__ __forceinline__ u32 redc_wide(u64 x) {
   u32 xl = (u32)x, xh = (u32)(x >> 32);
   u32 t = xl * MU;
   u32 uh = (u32)(((u64)t * P) >> 32);
   u32 r = xh - uh;
-  if (xh < uh) return r + P;
-  return r >= P ? r - P : r;
+  r = r > xh ? r + P : r;
+  u32 r3 = r - P;
+  return r3 < r ? r3 : r;
 }
 __host__ __device__ __forceinline__ u32 mul(u32 a, u32 b) { return redc((u64)a * b); }
 
@@ -69,8 +77,9 @@ __host__ __device__ inline u32 pow(u32 a, u64 e) {
 }
 __host__ __device__ inline u32 inv(u32 a) { return pow(a, P - 2); }
 
-// Halving in Montgomery form: the representative is linear in the value.
-__host__ __device__ __forceinline__ u32 halve(u32 a) { return (a & 1) ? (u32)(((u64)a + P) >> 1) : a >> 1; }
+// Halving in Montgomery form (the representative is linear in the value):
+// a/2 for even a, (a + p)/2 = (a >> 1) + (p + 1)/2 for odd a.
+__host__ __device__ __forceinline__ u32 halve(u32 a) { return (a >> 1) + ((0u - (a & 1)) & ((P + 1) >> 1)); }
 
 // ------------------------------------------------------------------ EF4
 struct alignas(16) ef {

@@ -214,10 +214,11 @@ int merkle_open(const u32* rows, int row_len, const u32* tree, u64 height, const
 // next sample is state[7] of the permuted state, so each candidate costs one
 // permutation.
 //
-// Candidates are scanned in consecutive chunks, one launch per chunk and one
-// candidate per lane, all on one stream.  A block returns at once when a
-// witness below its chunk is already known, so every candidate smaller than
-// the final result has been checked and the result is the smallest witness.
+// One launch scans a range of candidates, one per lane.  A block returns at
+// once when the best witness known is not above its first candidate, which
+// cannot hide a smaller witness: the result only decreases, so every block
+// whose range lies below the final result has run.  The result is therefore
+// the smallest witness in the range, whatever order blocks run in.
 struct Sponge16 {
   u32 s[16];
 };
@@ -225,10 +226,11 @@ struct Sponge16 {
 __global__ void __launch_bounds__(256) k_grind_chunk(P2C16 c, Sponge16 base, int wpos, u32 mask, u32* result,
                                                      u32 start, u32 limit) {
   __shared__ u32 found_sh;
+  const u64 first = (u64)start + (u64)blockIdx.x * 256;
   if (threadIdx.x == 0) found_sh = __hip_atomic_load(result, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
-  if (found_sh <= start) return;
-  u64 w = (u64)start + (u64)blockIdx.x * 256 + threadIdx.x;
+  if ((u64)found_sh <= first) return;
+  u64 w = first + threadIdx.x;
   if (w >= limit) return;
   u32 s[16];
 #pragma unroll
@@ -237,19 +239,16 @@ __global__ void __launch_bounds__(256) k_grind_chunk(P2C16 c, Sponge16 base, int
   if ((from_monty(s[7]) & mask) == 0) atomicMin(result, (u32)w);
 }
 
-// Enqueues chunks [start, start + nchunks * chunk) (clipped to p).
-int grind_chunks(const P2C16& c, const u32* state16, int wpos, int bits, u32* d_result, u64 start, u64 chunk,
-                 int nchunks, hipStream_t st) {
+// Scans [start, start + count) (clipped to p) in one launch.
+int grind_range(const P2C16& c, const u32* state16, int wpos, int bits, u32* d_result, u64 start, u64 count,
+                hipStream_t st) {
   Sponge16 b;
   for (int k = 0; k < 16; k++) b.s[k] = state16[k];
   u32 mask = bits >= 32 ? 0xffffffffu : ((1u << bits) - 1);
-  for (int i = 0; i < nchunks; i++) {
-    u64 s0 = start + (u64)i * chunk;
-    if (s0 >= P) break;
-    u64 n = std::min<u64>(chunk, P - s0);
-    unsigned blocks = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_grind_chunk, dim3(blocks), dim3(256), 0, st, c, b, wpos, mask, d_result, (u32)s0, P);
-  }
+  if (start >= P) return 0;
+  u64 n = std::min<u64>(count, P - start);
+  unsigned blocks = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_grind_chunk, dim3(blocks), dim3(256), 0, st, c, b, wpos, mask, d_result, (u32)start, P);
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 

@@ -103,7 +103,15 @@ struct whirgpu_ctx {
   size_t h_pin_bytes = 0;
   ef* d_partial = nullptr;   // reduction scratch
   size_t partial_cap = 0;
-  ef* d_small = nullptr;     // small results
+  // Small reduction results are written by the kernels straight into
+  // coherent pinned host memory (h_small; d_small is its device address) and
+  // read after a stream synchronization, so no copy is enqueued for them.
+  ef* d_small = nullptr;
+  ef* h_small = nullptr;
+  void read_small(ef* dst, size_t n) {
+    sync();
+    memcpy(dst, h_small, n * sizeof(ef));
+  }
   ef* d_tabs = nullptr;      // eq tables: 8 slots of tab_cap
   size_t tab_cap = 0;
   u64* d_idx = nullptr;
@@ -320,7 +328,7 @@ std::vector<ef> base_partials(whirgpu_ctx* c, const u32* d_f, int n, int sb, con
     KCHK(eq_dot_base(d_f, n, sb, lo, t, 1, c->d_partial, nb, c->d_small, c->st));
   }
   std::vector<ef> out(1ull << sb);
-  c->d2h(out.data(), c->d_small, out.size());
+  c->read_small(out.data(), out.size());
   return out;
 }
 
@@ -334,7 +342,7 @@ ef ext_mle(whirgpu_ctx* c, const ef* d_f, int m, const std::vector<ef>& z) {
     KCHK(eq_dot_ext(d_f, m, 0, lo, t, 1, c->d_partial, nb, c->d_small, c->st));
   }
   ef out;
-  c->d2h(&out, c->d_small, 1);
+  c->read_small(&out, 1);
   return out;
 }
 
@@ -348,15 +356,14 @@ u32 gpu_grind(whirgpu_ctx* c, DuplexChallenger& ch, int bits) {
   for (int i = 0; i < ch.in_len; i++) s[i] = ch.in[i];
   const u32 init = P;
   c->h2d(c->d_grind, &init, 1);
-  // Chunks of about half the expected number of candidates, enqueued in
-  // batches covering about three times that number before each readback.
-  const u64 expect = 1ull << bits;
-  const u64 chunk = std::min<u64>(std::max<u64>(expect / 2, 256), (u64)c->grind_blocks * 256);
-  const int per_batch = (int)std::max<u64>(1, (3 * expect + chunk - 1) / chunk);
+  // Each launch covers about three times the expected number of candidates
+  // (a witness lies inside with probability 1 - e^-3); the range continues
+  // until one is found.
+  const u64 span = std::max<u64>(3ull << bits, 256);
   u32 w = P;
   Span sp(c, ST_GRIND, 0);
-  for (u64 start = 0; start < P && w >= P; start += chunk * per_batch) {
-    KCHK(grind_chunks(c->c16, s, ch.in_len, bits, c->d_grind, start, chunk, per_batch, c->st));
+  for (u64 start = 0; start < P && w >= P; start += span) {
+    KCHK(grind_range(c->c16, s, ch.in_len, bits, c->d_grind, start, span, c->st));
     c->d2h(&w, c->d_grind, 1);
   }
   sp.close();
@@ -398,7 +405,7 @@ ef sumcheck_round(whirgpu_ctx* c, SumcheckState& s, const ChallengeFn& next) {
     KCHK(sc_coeffs(s.e.p, s.w.p, h, c->d_partial, c->d_small, c->st));
   }
   ef cc[2];
-  c->d2h(cc, c->d_small, 2);
+  c->read_small(cc, 2);
   ef r = next(cc[0], cc[1]);
   {
     Span sp(c, ST_SUMCHECK, 96.0 * (double)h);
@@ -720,7 +727,8 @@ int whirgpu_create(int device, const whirgpu_params* params, const whirgpu_posei
     HIPCHK(hipMalloc(&c->d_tabs, 8 * c->tab_cap * sizeof(ef)));
     c->partial_cap = 1 << 15;
     HIPCHK(hipMalloc(&c->d_partial, c->partial_cap * sizeof(ef)));
-    HIPCHK(hipMalloc(&c->d_small, 4096 * sizeof(ef)));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_small), 4096 * sizeof(ef), hipHostMallocCoherent));
+    HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->d_small), c->h_small, 0));
     HIPCHK(hipMalloc(&c->d_idx, 4096 * sizeof(u64)));
     c->open_cap = 1 << 20;
     HIPCHK(hipMalloc(&c->d_open, c->open_cap * sizeof(u32)));
@@ -745,7 +753,8 @@ int whirgpu_destroy(whirgpu_ctx* c) {
   if (c->device >= 0) (void)hipSetDevice(c->device);
   if (c->st) (void)hipStreamSynchronize(c->st);
   c->pool.release_all();
-  for (void* p : {(void*)c->d_tw, (void*)c->d_tabs, (void*)c->d_partial, (void*)c->d_small, (void*)c->d_idx,
+  if (c->h_small) (void)hipHostFree(c->h_small);
+  for (void* p : {(void*)c->d_tw, (void*)c->d_tabs, (void*)c->d_partial, (void*)c->d_idx,
                   (void*)c->d_open, (void*)c->d_grind})
     if (p) (void)hipFree(p);
   if (c->h_pin) (void)hipHostFree(c->h_pin);
@@ -1269,7 +1278,7 @@ int whirgpu_sumcheck_coeffs(whirgpu_ctx* c, const uint32_t* d_e, const uint32_t*
     use_device(c);
     KCHK(sc_coeffs(reinterpret_cast<const ef*>(d_e), reinterpret_cast<const ef*>(d_w), 1ull << (n - 1), c->d_partial,
                    c->d_small, c->st));
-    c->d2h(reinterpret_cast<ef*>(out), c->d_small, 2);
+    c->read_small(reinterpret_cast<ef*>(out), 2);
   });
 }
 
@@ -1281,7 +1290,7 @@ int whirgpu_sumcheck_fold(whirgpu_ctx* c, uint32_t* d_e, uint32_t* d_w, uint32_t
     memcpy(rr.c, r, 16);
     u64 h = 1ull << (n - 1);
     KCHK(sc_fold(reinterpret_cast<ef*>(d_e), reinterpret_cast<ef*>(d_w), h, rr, c->d_partial, c->d_small, c->st));
-    if (out && h >= 2) c->d2h(reinterpret_cast<ef*>(out), c->d_small, 2);
+    if (out && h >= 2) c->read_small(reinterpret_cast<ef*>(out), 2);
     HIPCHK(hipStreamSynchronize(c->st));
   });
 }
