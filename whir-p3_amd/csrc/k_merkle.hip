@@ -214,41 +214,81 @@ int merkle_open(const u32* rows, int row_len, const u32* tree, u64 height, const
 // next sample is state[7] of the permuted state, so each candidate costs one
 // permutation.
 //
-// One launch scans a range of candidates, one per lane.  A block returns at
-// once when the best witness known is not above its first candidate, which
-// cannot hide a smaller witness: the result only decreases, so every block
-// whose range lies below the final result has run.  The result is therefore
-// the smallest witness in the range, whatever order blocks run in.
+// A launch scans a range of candidates with one wave per SIMD: each lane
+// takes one candidate per iteration and the waves step through the range
+// together, so a witness near the start ends the search after about one
+// pass over 64K candidates instead of after every resident wave of a
+// full-occupancy grid.  A wave stops when the best witness known is not above
+// its first candidate in the current iteration; it checks before each
+// iteration and between the three phases of the permutation, with each check's
+// load issued one phase ahead so its latency hides behind arithmetic.  A stop
+// cannot hide a smaller witness: the result only decreases, so every candidate
+// below the final result is checked, whatever order waves run in, and the
+// result is the smallest witness in the range.
 struct Sponge16 {
   u32 s[16];
 };
 
-__global__ void __launch_bounds__(256) k_grind_chunk(P2C16 c, Sponge16 base, int wpos, u32 mask, u32* result,
-                                                     u32 start, u32 limit) {
-  __shared__ u32 found_sh;
-  const u64 first = (u64)start + (u64)blockIdx.x * 256;
-  if (threadIdx.x == 0) found_sh = __hip_atomic_load(result, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __syncthreads();
-  if ((u64)found_sh <= first) return;
-  u64 w = first + threadIdx.x;
-  if (w >= limit) return;
-  u32 s[16];
-#pragma unroll
-  for (int k = 0; k < 16; k++) s[k] = (k == wpos) ? to_monty((u32)w) : base.s[k];
-  permute<16>(s, c);
-  if ((from_monty(s[7]) & mask) == 0) atomicMin(result, (u32)w);
+__device__ __forceinline__ u32 best_known(const u32* result) {
+  return __hip_atomic_load(result, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Scans [start, start + count) (clipped to p) in one launch.
-int grind_range(const P2C16& c, const u32* state16, int wpos, int bits, u32* d_result, u64 start, u64 count,
-                hipStream_t st) {
+// Orders a phase boundary: the state registers pass through an empty asm
+// (so the check after it cannot be moved ahead of the phase's arithmetic) and
+// the memory clobber keeps the next phase's constant loads behind it.
+__device__ __forceinline__ void phase_fence(u32* s) {
+  asm volatile(""
+               : "+v"(s[0]), "+v"(s[1]), "+v"(s[2]), "+v"(s[3]), "+v"(s[4]), "+v"(s[5]), "+v"(s[6]), "+v"(s[7]),
+                 "+v"(s[8]), "+v"(s[9]), "+v"(s[10]), "+v"(s[11]), "+v"(s[12]), "+v"(s[13]), "+v"(s[14]), "+v"(s[15])
+               :
+               : "memory");
+}
+
+__global__ void __launch_bounds__(256) k_grind(const P2C16* __restrict__ cp, Sponge16 base, int wpos, u32 mask,
+                                               u32* result, u32 start, u32 end) {
+  const u32 stride = gridDim.x * 256;
+  const u32 lane = blockIdx.x * 256 + threadIdx.x;
+  const u32 wave0 = lane & ~63u;
+  u32 b0 = best_known(result);
+  for (u64 g = start; g + wave0 < end; g += stride) {
+    const u64 first = g + wave0;
+    if ((u64)b0 <= first) break;
+    // The empty asm makes the constants' offset opaque to the loop, so they
+    // are reloaded per iteration instead of being held in (and spilled from)
+    // scalar registers across it.
+    u32 off = 0;
+    asm volatile("" : "+s"(off));
+    const P2C16& c = cp[off];
+    const u64 w = g + lane;
+    u32 s[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) s[k] = (k == wpos) ? to_monty((u32)w) : base.s[k];
+    const u32 b1 = best_known(result);
+    permute_head<16>(s, c);
+    phase_fence(s);
+    if ((u64)b1 <= first) break;
+    const u32 b2 = best_known(result);
+    permute_internal<16>(s, c);
+    phase_fence(s);
+    if ((u64)b2 <= first) break;
+    b0 = best_known(result);
+    permute_tail<16>(s, c);
+    if (w < end && (from_monty(s[7]) & mask) == 0) atomicMin(result, (u32)w);
+  }
+}
+
+// Scans [start, start + count) (clipped to p) in one launch of `blocks` blocks
+// of 256 lanes (fewer when the range is short).
+int grind_range(const P2C16* d_consts, const u32* state16, int wpos, int bits, u32* d_result, u64 start, u64 count,
+                int blocks, hipStream_t st) {
   Sponge16 b;
   for (int k = 0; k < 16; k++) b.s[k] = state16[k];
   u32 mask = bits >= 32 ? 0xffffffffu : ((1u << bits) - 1);
   if (start >= P) return 0;
-  u64 n = std::min<u64>(count, P - start);
-  unsigned blocks = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(k_grind_chunk, dim3(blocks), dim3(256), 0, st, c, b, wpos, mask, d_result, (u32)start, P);
+  u64 end = std::min<u64>(start + count, P);
+  u64 need = (end - start + 255) / 256;
+  unsigned nb = (unsigned)std::min<u64>(need, (u64)std::max(blocks, 1));
+  hipLaunchKernelGGL(k_grind, dim3(nb), dim3(256), 0, st, d_consts, b, wpos, mask, d_result, (u32)start, (u32)end);
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 

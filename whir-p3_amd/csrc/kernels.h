@@ -39,8 +39,8 @@ int merkle_open(const u32* rows, int row_len, const u32* tree, u64 height, const
                 hipStream_t st);
 // Scans [start, start + count) for the smallest witness.  *d_result must
 // hold p before the first range; it receives the smallest witness found.
-int grind_range(const P2C16& c, const u32* state16, int wpos, int bits, u32* d_result, u64 start, u64 count,
-                hipStream_t st);
+int grind_range(const P2C16* d_consts, const u32* state16, int wpos, int bits, u32* d_result, u64 start, u64 count,
+                int blocks, hipStream_t st);
 
 // ---- multilinear kernels (k_poly.hip)
 int eq_table(const PointArg& z, int first, int t, const ef& scale, ef* out, hipStream_t st);

@@ -126,8 +126,11 @@ __host__ __device__ __forceinline__ void internal_linear(u32* s) {
   else internal_linear24(s);
 }
 
+// The permutation in three phases (initial linear layer and external rounds;
+// internal rounds; terminal external rounds), so a caller can stop between
+// them.
 template <int W>
-__host__ __device__ __forceinline__ void permute(u32* s, const P2Consts<W>& c) {
+__host__ __device__ __forceinline__ void permute_head(u32* s, const P2Consts<W>& c) {
   external_linear<W>(s);
 #pragma unroll
   for (int r = 0; r < 4; r++) {
@@ -135,17 +138,30 @@ __host__ __device__ __forceinline__ void permute(u32* s, const P2Consts<W>& c) {
     for (int i = 0; i < W; i++) s[i] = cube(add(s[i], c.ext_init[r * W + i]));
     external_linear<W>(s);
   }
+}
+template <int W>
+__host__ __device__ __forceinline__ void permute_internal(u32* s, const P2Consts<W>& c) {
 #pragma unroll
   for (int r = 0; r < P2Consts<W>::RP; r++) {
     s[0] = cube(add(s[0], c.internal[r]));
     internal_linear<W>(s);
   }
+}
+template <int W>
+__host__ __device__ __forceinline__ void permute_tail(u32* s, const P2Consts<W>& c) {
 #pragma unroll
   for (int r = 0; r < 4; r++) {
 #pragma unroll
     for (int i = 0; i < W; i++) s[i] = cube(add(s[i], c.ext_term[r * W + i]));
     external_linear<W>(s);
   }
+}
+
+template <int W>
+__host__ __device__ __forceinline__ void permute(u32* s, const P2Consts<W>& c) {
+  permute_head<W>(s, c);
+  permute_internal<W>(s, c);
+  permute_tail<W>(s, c);
 }
 
 }  // namespace kb

@@ -118,7 +118,8 @@ struct whirgpu_ctx {
   u32* d_open = nullptr;
   size_t open_cap = 0;
   u32* d_grind = nullptr;
-  int grind_blocks = 2048;
+  P2C16* d_c16 = nullptr;     // device copy of the width-16 constants (grinding reads them in a loop)
+  int grind_blocks = 256;
   double timings[7] = {0};
   // live kernel timing: HIP events recorded on `st` around each launch
   bool prof = false;
@@ -356,14 +357,15 @@ u32 gpu_grind(whirgpu_ctx* c, DuplexChallenger& ch, int bits) {
   for (int i = 0; i < ch.in_len; i++) s[i] = ch.in[i];
   const u32 init = P;
   c->h2d(c->d_grind, &init, 1);
-  // Each launch covers about three times the expected number of candidates
-  // (a witness lies inside with probability 1 - e^-3); the range continues
-  // until one is found.
-  const u64 span = std::max<u64>(3ull << bits, 256);
+  // Each launch covers sixteen times the expected number of candidates (a
+  // witness lies inside with probability 1 - e^-16; the waves stop as soon as
+  // one is known, so the size costs nothing); the search continues with the
+  // next range until one is found.
+  const u64 span = std::max<u64>(16ull << bits, 256);
   u32 w = P;
   Span sp(c, ST_GRIND, 0);
   for (u64 start = 0; start < P && w >= P; start += span) {
-    KCHK(grind_range(c->c16, s, ch.in_len, bits, c->d_grind, start, span, c->st));
+    KCHK(grind_range(c->d_c16, s, ch.in_len, bits, c->d_grind, start, span, c->grind_blocks, c->st));
     c->d2h(&w, c->d_grind, 1);
   }
   sp.close();
@@ -735,7 +737,12 @@ int whirgpu_create(int device, const whirgpu_params* params, const whirgpu_posei
     HIPCHK(hipMalloc(&c->d_grind, 256));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device));
-    c->grind_blocks = std::max(64, prop.multiProcessorCount * 8);
+    // One 256-lane block per CU: one wave on each of its four SIMDs.  More
+    // waves add no throughput to the search loop (it is issue-bound) and
+    // lengthen each pass over the candidates.
+    c->grind_blocks = std::max(1, prop.multiProcessorCount);
+    HIPCHK(hipMalloc(&c->d_c16, sizeof(P2C16)));
+    HIPCHK(hipMemcpy(c->d_c16, &c->c16, sizeof(P2C16), hipMemcpyHostToDevice));
     c->pin<u32>(1);
     c->ring_cap = 1 << 20;
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_ring), c->ring_cap, hipHostMallocDefault));
@@ -755,7 +762,7 @@ int whirgpu_destroy(whirgpu_ctx* c) {
   c->pool.release_all();
   if (c->h_small) (void)hipHostFree(c->h_small);
   for (void* p : {(void*)c->d_tw, (void*)c->d_tabs, (void*)c->d_partial, (void*)c->d_idx,
-                  (void*)c->d_open, (void*)c->d_grind})
+                  (void*)c->d_open, (void*)c->d_grind, (void*)c->d_c16})
     if (p) (void)hipFree(p);
   if (c->h_pin) (void)hipHostFree(c->h_pin);
   if (c->h_ring) (void)hipHostFree(c->h_ring);
