@@ -23,87 +23,146 @@
 
 namespace kb {
 
-// Radix-2 DIT over `nlines` lines of length 2^lg stored contiguously in LDS
-// (each line already in bit-reversed order, lines `stride` elements apart;
-// the stride is padded by one element to spread LDS banks).  `twl` holds w_{2^lg}^j, j < 2^(lg-1).
-template <typename T>
-__device__ void dit_lines(T* sh, int nlines, int lg, int stride, const u32* twl) {
-  if (lg == 0) return;
-  const int halfL = 1 << (lg - 1);
-  const int total = nlines * halfL;
-  for (int s = 1; s <= lg; s++) {
-    const int half = 1 << (s - 1);
-    for (int t = threadIdx.x; t < total; t += blockDim.x) {
-      int line = t >> (lg - 1);
-      int bt = t & (halfL - 1);
-      int g = bt >> (s - 1);
-      int j = bt & (half - 1);
-      int i1 = line * stride + (g << s) + j;
-      int i2 = i1 + half;
-      u32 w = twl[j << (lg - s)];
-      T u = sh[i1];
-      T v = tmulb(sh[i2], w);
-      sh[i1] = tadd(u, v);
-      sh[i2] = tsub(u, v);
+// ---- line DFTs in LDS
+// A block holds `nlines` lines of length N = 2^LG in LDS, each in bit-reversed
+// order, and runs the LG radix-2 decimation-in-time stages over them.  Stages
+// are taken R at a time (R <= 5 for base-field lines, <= 4 for extension
+// lines): each thread loads a group of 2^R elements into registers, runs the R
+// stages there and stores them back, so a line of 1024 costs two LDS round
+// trips and two barriers instead of ten.
+//
+// Layout: element i of line l sits at l * LS + i + (i >> 5).  The pad after
+// every 32 elements keeps a wave's accesses on distinct banks both when lanes
+// walk groups of one line (stride 33) and when they walk lines at the same
+// position (odd line stride LS).
+//
+// Twiddles: stw[2^(s-1) + e] = w_{2^s}^e for stage s (1..LG) and e < 2^(s-1),
+// so the lanes of a wave read consecutive entries.
+template <int LG>
+struct LineShape {
+  static constexpr int N = 1 << LG;
+  static constexpr int LS = N + (N >> 5) + 1;
+  static __device__ __forceinline__ int pos(int i) { return i + (i >> 5); }
+};
+
+static constexpr int lde_min(int a, int b) { return a < b ? a : b; }
+
+// Stages S0+1 .. S0+R: for a group with base index b (bits S0..S0+R-1 clear)
+// the elements are b + (j << S0), j < 2^R; at local stage g the pair (j, j + h),
+// h = 2^(g-1), uses w_{2^(S0+g)}^((b mod 2^S0) + (j mod h) 2^S0).
+template <typename T, int LG, int R, int S0>
+__device__ __forceinline__ void dit_pass(T* sh, int nlines, const u32* stw) {
+  using L = LineShape<LG>;
+  constexpr int G = 1 << R;
+  constexpr int GPL = L::N >> R;
+  const int ngroups = nlines * GPL;
+  for (int g = threadIdx.x; g < ngroups; g += blockDim.x) {
+    const int line = g / GPL;
+    const int gl = g % GPL;
+    const int base_low = gl & ((1 << S0) - 1);
+    const int base = base_low + ((gl >> S0) << (S0 + R));
+    T* p = sh + line * L::LS;
+    T v[G];
+#pragma unroll
+    for (int j = 0; j < G; j++) v[j] = p[L::pos(base + (j << S0))];
+#pragma unroll
+    for (int sg = 1; sg <= R; sg++) {
+      const int h = 1 << (sg - 1);
+#pragma unroll
+      for (int j = 0; j < G; j++) {
+        if (j & h) continue;
+        const int jl = j & (h - 1);
+        T b = v[j + h];
+        if (S0 != 0 || jl != 0) b = tmulb(b, stw[(1 << (S0 + sg - 1)) + base_low + (jl << S0)]);
+        const T a = v[j];
+        v[j] = tadd(a, b);
+        v[j + h] = tsub(a, b);
+      }
     }
-    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < G; j++) p[L::pos(base + (j << S0))] = v[j];
   }
 }
 
-__device__ void fill_line_twiddles(u32* twl, int lg, const Twiddles& tw) {
-  if (lg == 0) return;
-  for (int j = threadIdx.x; j < (1 << (lg - 1)); j += blockDim.x) twl[j] = root_pow(tw, lg, (u64)j);
-  __syncthreads();
+template <typename T, int LG, int RMAX, int S0 = 0>
+__device__ __forceinline__ void dit_lines(T* sh, int nlines, const u32* stw) {
+  if constexpr (S0 < LG) {
+    constexpr int R = lde_min(RMAX, LG - S0);
+    dit_pass<T, LG, R, S0>(sh, nlines, stw);
+    __syncthreads();
+    dit_lines<T, LG, RMAX, S0 + R>(sh, nlines, stw);
+  }
 }
 
-// Pass 1 of the four-step split.  Grid: (L2 / TB, W, 2^R).
-template <typename T, int TB>
-__global__ void __launch_bounds__(256) k_lde_pass1(const T* __restrict__ src, T* __restrict__ V, int l, int l1, int R,
-                                                   int k, Twiddles tw) {
+template <typename T>
+struct RadixMax {
+  static constexpr int value = sizeof(T) == 4 ? 5 : 4;
+};
+
+template <int LG>
+__device__ void fill_stage_twiddles(u32* stw, const Twiddles& tw) {
+  for (int j = threadIdx.x; j < (1 << LG); j += blockDim.x) {
+    if (j == 0) continue;
+    const int s = 32 - __clz(j);   // j in [2^(s-1), 2^s)
+    stw[j] = root_pow(tw, s, (u64)(j - (1 << (s - 1))));
+  }
+}
+
+// LDS: the stage twiddles (N words, 16-byte aligned) then nlines padded lines.
+template <typename T, int LG>
+__device__ __forceinline__ T* line_storage(unsigned char* smem) {
+  return reinterpret_cast<T*>(smem + (((sizeof(u32) << LG) + 15) & ~size_t(15)));
+}
+
+// Pass 1 of the four-step split.  Grid: (L2 / TB, W, 2^R); LG = l1.
+template <typename T, int TB, int LG>
+__global__ void __launch_bounds__(256) k_lde_pass1(const T* __restrict__ src, T* __restrict__ V, int l, int R, int k,
+                                                   Twiddles tw) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int L1 = 1 << l1, l2 = l - l1;
+  using L = LineShape<LG>;
+  const int l1 = LG, l2 = l - LG;
   const int W = 1 << k;
   const int b2_0 = blockIdx.x * TB;
   const int a = blockIdx.y;
   const int i0 = blockIdx.z;
   const int H_log = l + R;
-  u32* twl = reinterpret_cast<u32*>(smem);
-  T* sh = reinterpret_cast<T*>(smem + sizeof(u32) * (L1 > 1 ? L1 / 2 : 1) + 16);
-  sh = reinterpret_cast<T*>((reinterpret_cast<uintptr_t>(sh) + 15) & ~uintptr_t(15));
-  fill_line_twiddles(twl, l1, tw);
+  u32* stw = reinterpret_cast<u32*>(smem);
+  T* sh = line_storage<T, LG>(smem);
+  fill_stage_twiddles<LG>(stw, tw);
   const u64 col = (u64)a << l;
-  for (int e = threadIdx.x; e < TB * L1; e += blockDim.x) {
+  for (int e = threadIdx.x; e < TB * L::N; e += blockDim.x) {
     int t = e % TB, b1 = e / TB;
     u64 b = ((u64)b1 << l2) + b2_0 + t;
     T x = src[col + b];
     if (i0) x = tmulb(x, root_pow(tw, H_log, (u64)i0 * b));
-    sh[t * (L1 + 1) + bitrev(b1, l1)] = x;
+    sh[t * L::LS + L::pos(bitrev(b1, l1))] = x;
   }
   __syncthreads();
-  dit_lines(sh, TB, l1, L1 + 1, twl);
+  dit_lines<T, LG, RadixMax<T>::value>(sh, TB, stw);
   T* out = V + (((u64)i0 * W + a) << l);
-  for (int e = threadIdx.x; e < TB * L1; e += blockDim.x) {
+  for (int e = threadIdx.x; e < TB * L::N; e += blockDim.x) {
     int t = e % TB, k1 = e / TB;
     int b2 = b2_0 + t;
-    T y = sh[t * (L1 + 1) + k1];
+    T y = sh[t * L::LS + L::pos(k1)];
     if (b2 && k1) y = tmulb(y, root_pow(tw, l, (u64)b2 * k1));
     out[((u64)k1 << l2) + b2] = y;
   }
 }
 
-// Pass 2 (or the only pass when direct != 0).  Grid: (L1, 2^R).
-template <typename T>
-__global__ void __launch_bounds__(256) k_lde_pass2(const T* __restrict__ src, T* __restrict__ dst, int l, int l1, int R,
-                                                   int k, int direct, Twiddles tw) {
+// Pass 2 (or the only pass when direct != 0).  Grid: (L1, 2^R); LG = l2.
+template <typename T, int LG>
+__global__ void __launch_bounds__(256) k_lde_pass2(const T* __restrict__ src, T* __restrict__ dst, int l, int R, int k,
+                                                   int direct, Twiddles tw) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int l2 = l - l1, L2 = 1 << l2;
+  using L = LineShape<LG>;
+  const int l2 = LG, L2 = L::N;
+  const int l1 = l - LG;
   const int W = 1 << k;
   const int k1 = blockIdx.x;
   const int i0 = blockIdx.y;
-  u32* twl = reinterpret_cast<u32*>(smem);
-  T* sh = reinterpret_cast<T*>(smem + sizeof(u32) * (L2 > 1 ? L2 / 2 : 1) + 16);
-  sh = reinterpret_cast<T*>((reinterpret_cast<uintptr_t>(sh) + 15) & ~uintptr_t(15));
-  fill_line_twiddles(twl, l2, tw);
+  u32* stw = reinterpret_cast<u32*>(smem);
+  T* sh = line_storage<T, LG>(smem);
+  fill_stage_twiddles<LG>(stw, tw);
   for (int e = threadIdx.x; e < W * L2; e += blockDim.x) {
     int a = e >> l2, b2 = e & (L2 - 1);
     T x;
@@ -113,15 +172,15 @@ __global__ void __launch_bounds__(256) k_lde_pass2(const T* __restrict__ src, T*
     } else {
       x = src[((((u64)i0 * W + a) << l)) + ((u64)k1 << l2) + b2];
     }
-    sh[a * (L2 + 1) + bitrev(b2, l2)] = x;
+    sh[a * L::LS + L::pos(bitrev(b2, l2))] = x;
   }
   __syncthreads();
-  dit_lines(sh, W, l2, L2 + 1, twl);
+  dit_lines<T, LG, RadixMax<T>::value>(sh, W, stw);
   const int L1 = 1 << l1;
   for (int e = threadIdx.x; e < W * L2; e += blockDim.x) {
     int a = e & (W - 1), k2 = e >> k;
     u64 row = ((u64)(k1 + (u64)L1 * k2) << R) + i0;
-    dst[row * W + a] = sh[a * (L2 + 1) + k2];
+    dst[row * W + a] = sh[a * L::LS + L::pos(k2)];
   }
 }
 
@@ -134,8 +193,45 @@ static void allow_lds(K kernel, size_t bytes) {
 }
 
 static size_t lds_bytes(int lg, int lines, size_t esize) {
-  size_t tw = sizeof(u32) * ((lg > 0 ? (1u << (lg - 1)) : 1)) + 32;
-  return tw + (size_t)lines * ((1u << lg) + 1) * esize;
+  size_t tw = ((sizeof(u32) << lg) + 15) & ~size_t(15);
+  size_t ls = (1u << lg) + ((1u << lg) >> 5) + 1;
+  return tw + (size_t)lines * ls * esize;
+}
+
+// Kernel instances by line length (1 <= LG <= 10).
+template <typename T, int TB>
+static auto pass1_for(int lg) -> void (*)(const T*, T*, int, int, int, Twiddles) {
+  switch (lg) {
+    case 1: return k_lde_pass1<T, TB, 1>;
+    case 2: return k_lde_pass1<T, TB, 2>;
+    case 3: return k_lde_pass1<T, TB, 3>;
+    case 4: return k_lde_pass1<T, TB, 4>;
+    case 5: return k_lde_pass1<T, TB, 5>;
+    case 6: return k_lde_pass1<T, TB, 6>;
+    case 7: return k_lde_pass1<T, TB, 7>;
+    case 8: return k_lde_pass1<T, TB, 8>;
+    case 9: return k_lde_pass1<T, TB, 9>;
+    case 10: return k_lde_pass1<T, TB, 10>;
+    default: return nullptr;
+  }
+}
+
+template <typename T>
+static auto pass2_for(int lg) -> void (*)(const T*, T*, int, int, int, int, Twiddles) {
+  switch (lg) {
+    case 0: return k_lde_pass2<T, 0>;
+    case 1: return k_lde_pass2<T, 1>;
+    case 2: return k_lde_pass2<T, 2>;
+    case 3: return k_lde_pass2<T, 3>;
+    case 4: return k_lde_pass2<T, 4>;
+    case 5: return k_lde_pass2<T, 5>;
+    case 6: return k_lde_pass2<T, 6>;
+    case 7: return k_lde_pass2<T, 7>;
+    case 8: return k_lde_pass2<T, 8>;
+    case 9: return k_lde_pass2<T, 9>;
+    case 10: return k_lde_pass2<T, 10>;
+    default: return nullptr;
+  }
 }
 
 template <typename T, int LP2, int TB>
@@ -146,20 +242,24 @@ static int lde_impl(const T* src, T* dst, T* tmp, int m, int k, int R, const Twi
   if (l <= LP2) {
     size_t sm = lds_bytes(l, W, sizeof(T));
     if (sm > 160 * 1024) return -2;
-    allow_lds(k_lde_pass2<T>, sm);
-    hipLaunchKernelGGL((k_lde_pass2<T>), dim3(1, 1u << R), dim3(256), sm, st, src, dst, l, 0, R, k, 1, tw);
+    auto kern = pass2_for<T>(l);
+    if (!kern) return -6;
+    allow_lds(kern, sm);
+    hipLaunchKernelGGL(kern, dim3(1, 1u << R), dim3(256), sm, st, src, dst, l, R, k, 1, tw);
   } else {
     const int l2 = LP2, l1 = l - LP2;
-    if ((size_t)TB * ((1u << l1) + 1) * sizeof(T) > 80 * 1024) return -3;
-    if (tmp == nullptr) return -4;
     size_t sm1 = lds_bytes(l1, TB, sizeof(T));
-    allow_lds(k_lde_pass1<T, TB>, sm1);
-    hipLaunchKernelGGL((k_lde_pass1<T, TB>), dim3((1u << l2) / TB, W, 1u << R), dim3(256), sm1, st, src, tmp, l, l1, R,
-                       k, tw);
+    if (sm1 > 80 * 1024) return -3;
+    if (tmp == nullptr) return -4;
+    auto k1 = pass1_for<T, TB>(l1);
+    auto k2 = pass2_for<T>(l2);
+    if (!k1 || !k2) return -6;
+    allow_lds(k1, sm1);
+    hipLaunchKernelGGL(k1, dim3((1u << l2) / TB, W, 1u << R), dim3(256), sm1, st, src, tmp, l, R, k, tw);
     size_t sm2 = lds_bytes(l2, W, sizeof(T));
     if (sm2 > 160 * 1024) return -2;
-    allow_lds(k_lde_pass2<T>, sm2);
-    hipLaunchKernelGGL((k_lde_pass2<T>), dim3(1u << l1, 1u << R), dim3(256), sm2, st, tmp, dst, l, l1, R, k, 0, tw);
+    allow_lds(k2, sm2);
+    hipLaunchKernelGGL(k2, dim3(1u << l1, 1u << R), dim3(256), sm2, st, tmp, dst, l, R, k, 0, tw);
   }
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
