@@ -114,38 +114,71 @@ __device__ __forceinline__ T* line_storage(unsigned char* smem) {
   return reinterpret_cast<T*>(smem + (((sizeof(u32) << LG) + 15) & ~size_t(15)));
 }
 
-// Pass 1 of the four-step split.  Grid: (L2 / TB, W, 2^R); LG = l1.
-template <typename T, int TB, int LG>
+// Loads per thread in flight in the gather loops.
+template <typename T>
+struct Batch {
+  static constexpr int value = sizeof(T) == 4 ? 8 : 4;
+};
+
+// Pass 1 of the four-step split.  Grid: (L2 / TB, W, 2^R); LG = l1, TB = 2^tb_log
+// lines per block (TB <= 256).  Thread t of a block always handles line
+// (column offset) t mod TB and rows t / TB + j * (256 / TB), so its coset
+// twist w_H^(i0 b) and four-step twiddle w_L^(b2 k1) advance by one product
+// per row instead of two table lookups.
+template <typename T, int LG>
 __global__ void __launch_bounds__(256) k_lde_pass1(const T* __restrict__ src, T* __restrict__ V, int l, int R, int k,
-                                                   Twiddles tw) {
+                                                   int tb_log, Twiddles tw) {
   extern __shared__ __align__(16) unsigned char smem[];
   using L = LineShape<LG>;
-  const int l1 = LG, l2 = l - LG;
+  constexpr int U = Batch<T>::value;
+  const int l2 = l - LG;
   const int W = 1 << k;
-  const int b2_0 = blockIdx.x * TB;
+  const int TB = 1 << tb_log;
+  const int b2_0 = blockIdx.x << tb_log;
   const int a = blockIdx.y;
   const int i0 = blockIdx.z;
   const int H_log = l + R;
   u32* stw = reinterpret_cast<u32*>(smem);
   T* sh = line_storage<T, LG>(smem);
   fill_stage_twiddles<LG>(stw, tw);
-  const u64 col = (u64)a << l;
-  for (int e = threadIdx.x; e < TB * L::N; e += blockDim.x) {
-    int t = e % TB, b1 = e / TB;
-    u64 b = ((u64)b1 << l2) + b2_0 + t;
-    T x = src[col + b];
-    if (i0) x = tmulb(x, root_pow(tw, H_log, (u64)i0 * b));
-    sh[t * L::LS + L::pos(bitrev(b1, l1))] = x;
+  const int t = threadIdx.x & (TB - 1);
+  const int row0 = threadIdx.x >> tb_log;
+  const int rstep = 256 >> tb_log;
+  const int b2 = b2_0 + t;
+  const T* colp = src + ((u64)a << l) + b2;
+  u32 tcur = ONE, tstep = ONE;
+  if (i0) {
+    tcur = root_pow(tw, H_log, (u64)i0 * (((u64)row0 << l2) + b2));
+    tstep = root_pow(tw, H_log, (u64)i0 * ((u64)rstep << l2));
+  }
+  for (int r = row0; r < L::N; r += U * rstep) {
+    T xs[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int b1 = r + u * rstep;
+      if (b1 < L::N) xs[u] = colp[(u64)b1 << l2];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int b1 = r + u * rstep;
+      if (b1 < L::N) {
+        T x = xs[u];
+        if (i0) {
+          x = tmulb(x, tcur);
+          tcur = mul(tcur, tstep);
+        }
+        sh[t * L::LS + L::pos(bitrev(b1, LG))] = x;
+      }
+    }
   }
   __syncthreads();
   dit_lines<T, LG, RadixMax<T>::value>(sh, TB, stw);
-  T* out = V + (((u64)i0 * W + a) << l);
-  for (int e = threadIdx.x; e < TB * L::N; e += blockDim.x) {
-    int t = e % TB, k1 = e / TB;
-    int b2 = b2_0 + t;
-    T y = sh[t * L::LS + L::pos(k1)];
-    if (b2 && k1) y = tmulb(y, root_pow(tw, l, (u64)b2 * k1));
-    out[((u64)k1 << l2) + b2] = y;
+  T* out = V + (((u64)i0 * W + a) << l) + b2;
+  u32 ocur = root_pow(tw, l, (u64)b2 * row0);
+  const u32 ostep = root_pow(tw, l, (u64)b2 * rstep);
+  for (int k1 = row0; k1 < L::N; k1 += rstep) {
+    out[(u64)k1 << l2] = tmulb(sh[t * L::LS + L::pos(k1)], ocur);
+    ocur = mul(ocur, ostep);
   }
 }
 
@@ -155,6 +188,7 @@ __global__ void __launch_bounds__(256) k_lde_pass2(const T* __restrict__ src, T*
                                                    int direct, Twiddles tw) {
   extern __shared__ __align__(16) unsigned char smem[];
   using L = LineShape<LG>;
+  constexpr int U = Batch<T>::value;
   const int l2 = LG, L2 = L::N;
   const int l1 = l - LG;
   const int W = 1 << k;
@@ -163,21 +197,32 @@ __global__ void __launch_bounds__(256) k_lde_pass2(const T* __restrict__ src, T*
   u32* stw = reinterpret_cast<u32*>(smem);
   T* sh = line_storage<T, LG>(smem);
   fill_stage_twiddles<LG>(stw, tw);
-  for (int e = threadIdx.x; e < W * L2; e += blockDim.x) {
-    int a = e >> l2, b2 = e & (L2 - 1);
-    T x;
-    if (direct) {
-      x = src[((u64)a << l) + b2];
-      if (i0) x = tmulb(x, root_pow(tw, l + R, (u64)i0 * b2));
-    } else {
-      x = src[((((u64)i0 * W + a) << l)) + ((u64)k1 << l2) + b2];
+  const int total = W * L2;
+  for (int e0 = threadIdx.x; e0 < total; e0 += U * 256) {
+    T xs[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int e = e0 + u * 256;
+      if (e < total) {
+        const int a = e >> l2, b2 = e & (L2 - 1);
+        xs[u] = direct ? src[((u64)a << l) + b2] : src[((((u64)i0 * W + a) << l)) + ((u64)k1 << l2) + b2];
+      }
     }
-    sh[a * L::LS + L::pos(bitrev(b2, l2))] = x;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int e = e0 + u * 256;
+      if (e < total) {
+        const int a = e >> l2, b2 = e & (L2 - 1);
+        T x = xs[u];
+        if (direct && i0) x = tmulb(x, root_pow(tw, l + R, (u64)i0 * b2));
+        sh[a * L::LS + L::pos(bitrev(b2, l2))] = x;
+      }
+    }
   }
   __syncthreads();
   dit_lines<T, LG, RadixMax<T>::value>(sh, W, stw);
   const int L1 = 1 << l1;
-  for (int e = threadIdx.x; e < W * L2; e += blockDim.x) {
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
     int a = e & (W - 1), k2 = e >> k;
     u64 row = ((u64)(k1 + (u64)L1 * k2) << R) + i0;
     dst[row * W + a] = sh[a * L::LS + L::pos(k2)];
@@ -199,19 +244,19 @@ static size_t lds_bytes(int lg, int lines, size_t esize) {
 }
 
 // Kernel instances by line length (1 <= LG <= 10).
-template <typename T, int TB>
-static auto pass1_for(int lg) -> void (*)(const T*, T*, int, int, int, Twiddles) {
+template <typename T>
+static auto pass1_for(int lg) -> void (*)(const T*, T*, int, int, int, int, Twiddles) {
   switch (lg) {
-    case 1: return k_lde_pass1<T, TB, 1>;
-    case 2: return k_lde_pass1<T, TB, 2>;
-    case 3: return k_lde_pass1<T, TB, 3>;
-    case 4: return k_lde_pass1<T, TB, 4>;
-    case 5: return k_lde_pass1<T, TB, 5>;
-    case 6: return k_lde_pass1<T, TB, 6>;
-    case 7: return k_lde_pass1<T, TB, 7>;
-    case 8: return k_lde_pass1<T, TB, 8>;
-    case 9: return k_lde_pass1<T, TB, 9>;
-    case 10: return k_lde_pass1<T, TB, 10>;
+    case 1: return k_lde_pass1<T, 1>;
+    case 2: return k_lde_pass1<T, 2>;
+    case 3: return k_lde_pass1<T, 3>;
+    case 4: return k_lde_pass1<T, 4>;
+    case 5: return k_lde_pass1<T, 5>;
+    case 6: return k_lde_pass1<T, 6>;
+    case 7: return k_lde_pass1<T, 7>;
+    case 8: return k_lde_pass1<T, 8>;
+    case 9: return k_lde_pass1<T, 9>;
+    case 10: return k_lde_pass1<T, 10>;
     default: return nullptr;
   }
 }
@@ -234,7 +279,9 @@ static auto pass2_for(int lg) -> void (*)(const T*, T*, int, int, int, int, Twid
   }
 }
 
-template <typename T, int LP2, int TB>
+// Pass 1 takes about 64 KiB of lines per block (16K base or 4K extension
+// elements), at most 256 lines and at most L2.
+template <typename T, int LP2>
 static int lde_impl(const T* src, T* dst, T* tmp, int m, int k, int R, const Twiddles& tw, hipStream_t st) {
   const int l = m - k;
   if (l < 0) return -1;
@@ -248,14 +295,19 @@ static int lde_impl(const T* src, T* dst, T* tmp, int m, int k, int R, const Twi
     hipLaunchKernelGGL(kern, dim3(1, 1u << R), dim3(256), sm, st, src, dst, l, R, k, 1, tw);
   } else {
     const int l2 = LP2, l1 = l - LP2;
-    size_t sm1 = lds_bytes(l1, TB, sizeof(T));
+    const int elem_log = sizeof(T) == 4 ? 14 : 12;
+    int tb_log = elem_log - l1;
+    if (tb_log > 8) tb_log = 8;
+    if (tb_log > l2) tb_log = l2;
+    if (tb_log < 0) tb_log = 0;
+    size_t sm1 = lds_bytes(l1, 1 << tb_log, sizeof(T));
     if (sm1 > 80 * 1024) return -3;
     if (tmp == nullptr) return -4;
-    auto k1 = pass1_for<T, TB>(l1);
+    auto k1 = pass1_for<T>(l1);
     auto k2 = pass2_for<T>(l2);
     if (!k1 || !k2) return -6;
     allow_lds(k1, sm1);
-    hipLaunchKernelGGL(k1, dim3((1u << l2) / TB, W, 1u << R), dim3(256), sm1, st, src, tmp, l, R, k, tw);
+    hipLaunchKernelGGL(k1, dim3(1u << (l2 - tb_log), W, 1u << R), dim3(256), sm1, st, src, tmp, l, R, k, tb_log, tw);
     size_t sm2 = lds_bytes(l2, W, sizeof(T));
     if (sm2 > 160 * 1024) return -2;
     allow_lds(k2, sm2);
@@ -277,11 +329,11 @@ bool lde_two_pass(int m, int k, bool ext) { return m - k > lp2_for(k, ext); }
 int lde_base(const u32* src, u32* dst, u32* tmp, int m, int k, int R, const Twiddles& tw, hipStream_t st) {
   int lp = lp2_for(k, false);
   switch (lp) {
-    case 10: return lde_impl<u32, 10, 16>(src, dst, tmp, m, k, R, tw, st);
-    case 9: return lde_impl<u32, 9, 16>(src, dst, tmp, m, k, R, tw, st);
-    case 8: return lde_impl<u32, 8, 16>(src, dst, tmp, m, k, R, tw, st);
-    case 7: return lde_impl<u32, 7, 16>(src, dst, tmp, m, k, R, tw, st);
-    case 6: return lde_impl<u32, 6, 16>(src, dst, tmp, m, k, R, tw, st);
+    case 10: return lde_impl<u32, 10>(src, dst, tmp, m, k, R, tw, st);
+    case 9: return lde_impl<u32, 9>(src, dst, tmp, m, k, R, tw, st);
+    case 8: return lde_impl<u32, 8>(src, dst, tmp, m, k, R, tw, st);
+    case 7: return lde_impl<u32, 7>(src, dst, tmp, m, k, R, tw, st);
+    case 6: return lde_impl<u32, 6>(src, dst, tmp, m, k, R, tw, st);
     default: return -6;
   }
 }
@@ -289,15 +341,15 @@ int lde_base(const u32* src, u32* dst, u32* tmp, int m, int k, int R, const Twid
 int lde_ext(const ef* src, ef* dst, ef* tmp, int m, int k, int R, const Twiddles& tw, hipStream_t st) {
   int lp = lp2_for(k, true);
   switch (lp) {
-    case 10: return lde_impl<ef, 10, 4>(src, dst, tmp, m, k, R, tw, st);
-    case 9: return lde_impl<ef, 9, 4>(src, dst, tmp, m, k, R, tw, st);
-    case 8: return lde_impl<ef, 8, 4>(src, dst, tmp, m, k, R, tw, st);
-    case 7: return lde_impl<ef, 7, 4>(src, dst, tmp, m, k, R, tw, st);
-    case 6: return lde_impl<ef, 6, 4>(src, dst, tmp, m, k, R, tw, st);
-    case 5: return lde_impl<ef, 5, 4>(src, dst, tmp, m, k, R, tw, st);
-    case 4: return lde_impl<ef, 4, 4>(src, dst, tmp, m, k, R, tw, st);
-    case 3: return lde_impl<ef, 3, 4>(src, dst, tmp, m, k, R, tw, st);
-    case 2: return lde_impl<ef, 2, 4>(src, dst, tmp, m, k, R, tw, st);
+    case 10: return lde_impl<ef, 10>(src, dst, tmp, m, k, R, tw, st);
+    case 9: return lde_impl<ef, 9>(src, dst, tmp, m, k, R, tw, st);
+    case 8: return lde_impl<ef, 8>(src, dst, tmp, m, k, R, tw, st);
+    case 7: return lde_impl<ef, 7>(src, dst, tmp, m, k, R, tw, st);
+    case 6: return lde_impl<ef, 6>(src, dst, tmp, m, k, R, tw, st);
+    case 5: return lde_impl<ef, 5>(src, dst, tmp, m, k, R, tw, st);
+    case 4: return lde_impl<ef, 4>(src, dst, tmp, m, k, R, tw, st);
+    case 3: return lde_impl<ef, 3>(src, dst, tmp, m, k, R, tw, st);
+    case 2: return lde_impl<ef, 2>(src, dst, tmp, m, k, R, tw, st);
     default: return -6;
   }
 }
