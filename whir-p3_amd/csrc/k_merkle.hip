@@ -244,8 +244,14 @@ __device__ __forceinline__ void phase_fence(u32* s) {
                : "memory");
 }
 
-__global__ void __launch_bounds__(256) k_grind(const P2C16* __restrict__ cp, Sponge16 base, int wpos, u32 mask,
-                                               u32* result, u32 start, u32 end) {
+__global__ void __launch_bounds__(256) k_grind(const P2C16* __restrict__ cp, Sponge16 base, int wpos,
+                                               const GrindBase* __restrict__ gb, u32 mask, u32* result, u32 start,
+                                               u32 end) {
+  if (gb) {
+#pragma unroll
+    for (int k = 0; k < 16; k++) base.s[k] = gb->s[k];
+    wpos = (int)gb->wpos;
+  }
   const u32 stride = gridDim.x * 256;
   const u32 lane = blockIdx.x * 256 + threadIdx.x;
   const u32 wave0 = lane & ~63u;
@@ -288,7 +294,21 @@ int grind_range(const P2C16* d_consts, const u32* state16, int wpos, int bits, u
   u64 end = std::min<u64>(start + count, P);
   u64 need = (end - start + 255) / 256;
   unsigned nb = (unsigned)std::min<u64>(need, (u64)std::max(blocks, 1));
-  hipLaunchKernelGGL(k_grind, dim3(nb), dim3(256), 0, st, d_consts, b, wpos, mask, d_result, (u32)start, (u32)end);
+  hipLaunchKernelGGL(k_grind, dim3(nb), dim3(256), 0, st, d_consts, b, wpos, (const GrindBase*)nullptr, mask, d_result,
+                     (u32)start, (u32)end);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+int grind_range_dev(const P2C16* d_consts, const GrindBase* d_base, int bits, u32* d_result, u64 start, u64 count,
+                    int blocks, hipStream_t st) {
+  Sponge16 b{};
+  u32 mask = bits >= 32 ? 0xffffffffu : ((1u << bits) - 1);
+  if (start >= P) return 0;
+  u64 end = std::min<u64>(start + count, P);
+  u64 need = (end - start + 255) / 256;
+  unsigned nb = (unsigned)std::min<u64>(need, (u64)std::max(blocks, 1));
+  hipLaunchKernelGGL(k_grind, dim3(nb), dim3(256), 0, st, d_consts, b, 0, d_base, mask, d_result, (u32)start,
+                     (u32)end);
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 

@@ -252,8 +252,9 @@ __global__ void __launch_bounds__(256) k_sc_coeffs(const ef* __restrict__ e, con
 // Fold e and w (length 2h) with r into their first halves, and accumulate the
 // next round's (h(0), h(2)) over the folded halves (pairs (j, j + h/2)).
 __global__ void __launch_bounds__(256) k_sc_fold_coeffs(ef* __restrict__ e, ef* __restrict__ w, u64 h, ef r,
-                                                        ef* __restrict__ partial) {
+                                                        const ef* __restrict__ rp, ef* __restrict__ partial) {
   __shared__ ef sh[512];
+  if (rp) r = *rp;
   ef acc[2] = {ef_zero(), ef_zero()};
   const u64 q = h >> 1;
   for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < q; j += (u64)gridDim.x * blockDim.x) {
@@ -277,7 +278,8 @@ __global__ void __launch_bounds__(256) k_sc_fold_coeffs(ef* __restrict__ e, ef* 
   }
 }
 
-__global__ void k_sc_fold(ef* __restrict__ e, ef* __restrict__ w, u64 h, ef r) {
+__global__ void k_sc_fold(ef* __restrict__ e, ef* __restrict__ w, u64 h, ef r, const ef* __restrict__ rp) {
+  if (rp) r = *rp;
   for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < h; j += (u64)gridDim.x * blockDim.x) {
     ef a0 = e[j], a1 = e[j + h];
     e[j] = ef_add(a0, ef_mul(r, ef_sub(a1, a0)));
@@ -304,10 +306,27 @@ int sc_coeffs(const ef* e, const ef* w, u64 h, ef* partial, ef* c0c2, hipStream_
 int sc_fold(ef* e, ef* w, u64 h, const ef& r, ef* partial, ef* c0c2, hipStream_t st) {
   if (h >= 2) {
     unsigned nb = sc_blocks(h >> 1);
-    hipLaunchKernelGGL(k_sc_fold_coeffs, dim3(nb), dim3(256), 0, st, e, w, h, r, partial);
+    hipLaunchKernelGGL(k_sc_fold_coeffs, dim3(nb), dim3(256), 0, st, e, w, h, r, (const ef*)nullptr, partial);
     hipLaunchKernelGGL(k_reduce_strided, dim3(2), dim3(256), 0, st, partial, (int)nb, 2, c0c2);
   } else {
-    hipLaunchKernelGGL(k_sc_fold, dim3(sc_blocks(h)), dim3(256), 0, st, e, w, h, r);
+    hipLaunchKernelGGL(k_sc_fold, dim3(sc_blocks(h)), dim3(256), 0, st, e, w, h, r, (const ef*)nullptr);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+int sc_partials(const ef* e, const ef* w, u64 h, ef* partial, unsigned* nb, hipStream_t st) {
+  *nb = sc_blocks(h);
+  hipLaunchKernelGGL(k_sc_coeffs, dim3(*nb), dim3(256), 0, st, e, w, h, partial);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+int sc_fold_dev(ef* e, ef* w, u64 h, const ef* d_r, ef* partial, unsigned* nb, hipStream_t st) {
+  if (h >= 2) {
+    *nb = sc_blocks(h >> 1);
+    hipLaunchKernelGGL(k_sc_fold_coeffs, dim3(*nb), dim3(256), 0, st, e, w, h, ef_zero(), d_r, partial);
+  } else {
+    *nb = 0;
+    hipLaunchKernelGGL(k_sc_fold, dim3(sc_blocks(h)), dim3(256), 0, st, e, w, h, ef_zero(), d_r);
   }
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }

@@ -16,6 +16,28 @@ struct PointArg {
 };
 
 // Up to four split eq tables: eq(z, x) = hi[k][x >> lo] * lo[k][x & (2^lo - 1)].
+// Device copy of a DuplexChallenger<F, Poseidon2-16, 16, 8> (Montgomery form).
+struct DevChallenger {
+  u32 sponge[16];
+  u32 in[8];
+  u32 out[8];
+  u32 in_len, out_len;
+  u32 pad[6];
+};
+// State a grinding candidate is observed into: sponge with the buffered
+// inputs written over its first elements; the candidate goes to s[wpos].
+struct GrindBase {
+  u32 s[16];
+  u32 wpos;
+  u32 pad[15];
+};
+// One device-transcript sumcheck round: round polynomial (h(0), h(2)), the
+// proof-of-work witness (Montgomery form, 0 without grinding), the challenge.
+struct ScLog {
+  ef c0, c2, r;
+  u32 witness, err, pad[2];
+};
+
 struct EqTables {
   const ef* hi[4];
   const ef* lo[4];
@@ -41,6 +63,9 @@ int merkle_open(const u32* rows, int row_len, const u32* tree, u64 height, const
 // hold p before the first range; it receives the smallest witness found.
 int grind_range(const P2C16* d_consts, const u32* state16, int wpos, int bits, u32* d_result, u64 start, u64 count,
                 int blocks, hipStream_t st);
+// Same, with the base state and candidate position read from device memory.
+int grind_range_dev(const P2C16* d_consts, const GrindBase* d_base, int bits, u32* d_result, u64 start, u64 count,
+                    int blocks, hipStream_t st);
 
 // ---- multilinear kernels (k_poly.hip)
 int eq_table(const PointArg& z, int first, int t, const ef& scale, ef* out, hipStream_t st);
@@ -55,5 +80,13 @@ int compress_multi(const u32* f, int n, int l, const ef* d_eqrs, ef* out, hipStr
 int base_to_ext(const u32* f, u64 n, ef* out, hipStream_t st);
 int sc_coeffs(const ef* e, const ef* w, u64 h, ef* partial, ef* c0c2, hipStream_t st);
 int sc_fold(ef* e, ef* w, u64 h, const ef& r, ef* partial, ef* c0c2, hipStream_t st);
+// Device-transcript variants: coefficient partial sums only (*nb blocks, two
+// values each in `partial`), and a fold by the challenge at *d_r that also
+// leaves the next round's partials when h >= 2 (*nb = 0 otherwise).
+int sc_partials(const ef* e, const ef* w, u64 h, ef* partial, unsigned* nb, hipStream_t st);
+int sc_fold_dev(ef* e, ef* w, u64 h, const ef* d_r, ef* partial, unsigned* nb, hipStream_t st);
+// ---- device transcript (k_transcript.hip)
+int sc_transcript(int phase, const ef* partial, unsigned nb, DevChallenger* ch, GrindBase* gb, u32* d_grind,
+                  ScLog* log, int pow_bits, ef* d_r, const P2C16* d_consts, hipStream_t st);
 
 }  // namespace kb

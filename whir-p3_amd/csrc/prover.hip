@@ -119,6 +119,12 @@ struct whirgpu_ctx {
   size_t open_cap = 0;
   u32* d_grind = nullptr;
   P2C16* d_c16 = nullptr;     // device copy of the width-16 constants (grinding reads them in a loop)
+  // device transcript of sumcheck rounds: challenger, grind base, round log, challenge
+  static constexpr int SC_LOG_CAP = 64;
+  DevChallenger* d_chal = nullptr;
+  GrindBase* d_gbase = nullptr;
+  ScLog* d_sclog = nullptr;
+  ef* d_r = nullptr;
   int grind_blocks = 256;
   double timings[7] = {0};
   // live kernel timing: HIP events recorded on `st` around each launch
@@ -392,6 +398,11 @@ struct SumcheckState {
   int m = 0;
   ef sum = ef_zero();
   bool coeffs_valid = false;   // d_small[0..2] holds (h(0), h(2)) of the current arrays
+  unsigned partial_nb = 0;     // d_partial holds their partial sums from this many blocks
+  void invalidate() {
+    coeffs_valid = false;
+    partial_nb = 0;
+  }
 };
 
 // Round challenge from the round polynomial (h(0), h(2)): the transcript
@@ -414,9 +425,80 @@ ef sumcheck_round(whirgpu_ctx* c, SumcheckState& s, const ChallengeFn& next) {
     KCHK(sc_fold(s.e.p, s.w.p, h, r, c->d_partial, c->d_small, c->st));
   }
   s.coeffs_valid = h >= 2;
+  s.partial_nb = 0;
   s.m -= 1;
   s.sum = whir::extrapolate_012(cc[0], ef_sub(s.sum, cc[0]), cc[1], r);
   return r;
+}
+
+// `rounds` sumcheck rounds of a proof with the transcript on the device: per
+// round the coefficient partials, the transcript wave (observe, grind base or
+// challenge), the grind and the transcript wave after it, and the fold by the
+// challenge in device memory; no host round trip until the end.  The host
+// then replays the logged rounds on its own challenger (observe (h(0), h(2)),
+// check the witness, sample) and requires every challenge to equal the
+// device's, so the proof and the challenger state are exactly those of the
+// host-driven rounds (SumcheckData::observe_and_sample, src/whir/proof.rs:163-193;
+// ProductPolynomial::round, src/sumcheck/product_polynomial.rs:307-364).
+std::vector<ef> device_sumcheck(whirgpu_ctx* c, SumcheckState& s, int rounds, int pow_bits, whir::SumcheckData& sd,
+                                DuplexChallenger& ch) {
+  std::vector<ef> rs;
+  if (rounds <= 0) return rs;
+  if (rounds > whirgpu_ctx::SC_LOG_CAP) bad("too many sumcheck rounds in one phase", WHIRGPU_BAD_STATE);
+  if (s.m < rounds) bad("sumcheck round on a constant polynomial", WHIRGPU_BAD_STATE);
+  DevChallenger dc{};
+  for (int i = 0; i < 16; i++) dc.sponge[i] = ch.sponge[i];
+  for (int i = 0; i < ch.in_len; i++) dc.in[i] = ch.in[i];
+  for (int i = 0; i < ch.out_len; i++) dc.out[i] = ch.out[i];
+  dc.in_len = (u32)ch.in_len;
+  dc.out_len = (u32)ch.out_len;
+  c->h2d(c->d_chal, &dc, 1);
+  std::vector<size_t> grind_recs;
+  for (int i = 0; i < rounds; i++) {
+    const u64 h = 1ull << (s.m - 1);
+    if (s.partial_nb == 0) {
+      Span sp(c, ST_SUMCHECK, 64.0 * (double)h);
+      KCHK(sc_partials(s.e.p, s.w.p, h, c->d_partial, &s.partial_nb, c->st));
+    }
+    KCHK(sc_transcript(0, c->d_partial, s.partial_nb, c->d_chal, c->d_gbase, c->d_grind, c->d_sclog + i, pow_bits,
+                       c->d_r, c->d_c16, c->st));
+    if (pow_bits > 0) {
+      {
+        Span sp(c, ST_GRIND, 0);
+        KCHK(grind_range_dev(c->d_c16, c->d_gbase, pow_bits, c->d_grind, 0, P, c->grind_blocks, c->st));
+        sp.close();
+        if (c->prof) grind_recs.push_back(c->recs.size() - 1);
+      }
+      KCHK(sc_transcript(1, c->d_partial, 0, c->d_chal, c->d_gbase, c->d_grind, c->d_sclog + i, pow_bits, c->d_r,
+                         c->d_c16, c->st));
+    }
+    {
+      Span sp(c, ST_SUMCHECK, 96.0 * (double)h);
+      KCHK(sc_fold_dev(s.e.p, s.w.p, h, c->d_r, c->d_partial, &s.partial_nb, c->st));
+    }
+    s.m -= 1;
+  }
+  // d_partial is scratch for other kernels between phases
+  s.invalidate();
+  std::vector<ScLog> log(rounds);
+  c->d2h(log.data(), c->d_sclog, (size_t)rounds);
+  for (int i = 0; i < rounds; i++) {
+    const ScLog& L = log[i];
+    if (L.err) bad("device transcript: no valid proof-of-work witness", WHIRGPU_GRIND_FAILED);
+    sd.polynomial_evaluations.push_back({L.c0, L.c2});
+    ch.observe_ef(L.c0);
+    ch.observe_ef(L.c2);
+    if (pow_bits > 0) {
+      if (!ch.check_witness(pow_bits, L.witness)) bad("device transcript: witness rejected", WHIRGPU_GRIND_FAILED);
+      sd.pow_witnesses.push_back(L.witness);
+      if (i < (int)grind_recs.size()) c->recs[grind_recs[i]].work = (double)from_monty(L.witness) + 1;
+    }
+    ef r = ch.sample_ef();
+    if (!ef_eq(r, L.r)) bad("device transcript diverged from the host challenger", WHIRGPU_BAD_STATE);
+    s.sum = whir::extrapolate_012(L.c0, ef_sub(s.sum, L.c0), L.c2, r);
+    rs.push_back(r);
+  }
+  return rs;
 }
 
 // Adds sum_i scale_i * eq(z_i, x) to the weights, four points per launch.
@@ -509,7 +591,7 @@ std::vector<ef> initial_sumcheck(whirgpu_ctx* c, const whirgpu_statement* st, in
       scales.push_back(ef_mul(ap[j], eq_point(zs, rs)));
     }
     add_eq_weights(c, s, rests, scales, false);
-    s.coeffs_valid = false;
+    s.invalidate();
     return rs;
   }
   // small instances: classic rounds on the extension-embedded evaluations
@@ -518,7 +600,7 @@ std::vector<ef> initial_sumcheck(whirgpu_ctx* c, const whirgpu_statement* st, in
   s.w = c->alloc<ef>(1ull << n);
   KCHK(base_to_ext(st->d_poly, 1ull << n, s.e.p, c->st));
   add_eq_weights(c, s, st->points, ap, false);
-  s.coeffs_valid = false;
+  s.invalidate();
   for (int i = 0; i < l0; i++) rs.push_back(sumcheck_round(c, s, next));
   return rs;
 }
@@ -742,6 +824,10 @@ int whirgpu_create(int device, const whirgpu_params* params, const whirgpu_posei
     // lengthen each pass over the candidates.
     c->grind_blocks = std::max(1, prop.multiProcessorCount);
     HIPCHK(hipMalloc(&c->d_c16, sizeof(P2C16)));
+    HIPCHK(hipMalloc(&c->d_chal, sizeof(DevChallenger)));
+    HIPCHK(hipMalloc(&c->d_gbase, sizeof(GrindBase)));
+    HIPCHK(hipMalloc(&c->d_sclog, whirgpu_ctx::SC_LOG_CAP * sizeof(ScLog)));
+    HIPCHK(hipMalloc(&c->d_r, 256));
     HIPCHK(hipMemcpy(c->d_c16, &c->c16, sizeof(P2C16), hipMemcpyHostToDevice));
     c->pin<u32>(1);
     c->ring_cap = 1 << 20;
@@ -762,7 +848,8 @@ int whirgpu_destroy(whirgpu_ctx* c) {
   c->pool.release_all();
   if (c->h_small) (void)hipHostFree(c->h_small);
   for (void* p : {(void*)c->d_tw, (void*)c->d_tabs, (void*)c->d_partial, (void*)c->d_idx,
-                  (void*)c->d_open, (void*)c->d_grind, (void*)c->d_c16})
+                  (void*)c->d_open, (void*)c->d_grind, (void*)c->d_c16, (void*)c->d_chal,
+                  (void*)c->d_gbase, (void*)c->d_sclog, (void*)c->d_r})
     if (p) (void)hipFree(p);
   if (c->h_pin) (void)hipHostFree(c->h_pin);
   if (c->h_ring) (void)hipHostFree(c->h_ring);
@@ -1019,11 +1106,7 @@ int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs,
         open_rows(c, prev, idx, &P.final_queries, &rows);
         if (cfg.final_sumcheck_rounds > 0) {
           P.has_final_sumcheck = true;
-          const int fpow = cfg.final_folding_pow_bits;
-          for (int i = 0; i < cfg.final_sumcheck_rounds; i++)
-            sumcheck_round(c, s, [&](const ef& c0, const ef& c2) {
-              return observe_and_sample(c, P.final_sumcheck, ch, c0, c2, fpow);
-            });
+          device_sumcheck(c, s, cfg.final_sumcheck_rounds, cfg.final_folding_pow_bits, P.final_sumcheck, ch);
         }
         break;
       }
@@ -1098,13 +1181,8 @@ int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs,
         c->release(B);
         for (size_t j = 0; j < ns; j++) s.sum = ef_add(s.sum, ef_mul(g[j], sel_v[j]));
       }
-      s.coeffs_valid = false;
-      fold_rand.clear();
-      const int fpow = rp.folding_pow_bits;
-      for (int i = 0; i < kn; i++)
-        fold_rand.push_back(sumcheck_round(c, s, [&](const ef& c0, const ef& c2) {
-          return observe_and_sample(c, R.sumcheck, ch, c0, c2, fpow);
-        }));
+      s.invalidate();
+      fold_rand = device_sumcheck(c, s, kn, rp.folding_pow_bits, R.sumcheck, ch);
     }
     HIPCHK(hipStreamSynchronize(c->st));
     store_ch(ch, chs);
