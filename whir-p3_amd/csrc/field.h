@@ -59,7 +59,18 @@ __host__ __device__ __forceinline__ u32 redc_wide(u64 x) {
   u32 r3 = r - P;
   return r3 < r ? r3 : r;
 }
-__host__ __device__ __forceinline__ u32 mul(u32 a, u32 b) { return redc((u64)a * b); }
+// Montgomery product a b / 2^32 mod p for a, b < p: with t = -(a b) p^-1 mod
+// 2^32, y = a b + t p is divisible by 2^32 and y / 2^32 < 2p, so one
+// conditional subtraction finishes it.  Two 32x32+64 multiply-adds and a
+// 32-bit multiply, all full-rate on CDNA.
+constexpr u32 MU_NEG = 0u - MU;   // -p^-1 mod 2^32
+__host__ __device__ __forceinline__ u32 mul(u32 a, u32 b) {
+  const u64 x = (u64)a * b;
+  const u32 t = (u32)x * MU_NEG;
+  const u32 r = (u32)(((u64)t * P + x) >> 32);
+  const u32 s = r - P;
+  return s < r ? s : r;
+}
 
 __host__ __device__ __forceinline__ u32 to_monty(u32 x) { return mul(x, R2); }
 __host__ __device__ __forceinline__ u32 from_monty(u32 m) { return redc((u64)m); }

@@ -20,9 +20,12 @@
 
 namespace kb {
 
+// The constants come from device memory and are re-read for every absorbed
+// chunk (the empty asm makes their offset opaque to the loop), so they are not
+// held in, and spilled from, scalar registers across the loop.
 template <int LW>
 __global__ void __launch_bounds__(256) k_leaf_hash(const u32* __restrict__ rows, int row_len, u64 nrows,
-                                                   u32* __restrict__ dig, P2Consts<LW> c) {
+                                                   u32* __restrict__ dig, const P2Consts<LW>* __restrict__ cp) {
   constexpr int RATE = LW - 8;
   u64 i = (u64)blockIdx.x * 256 + threadIdx.x;
   if (i >= nrows) return;
@@ -41,7 +44,9 @@ __global__ void __launch_bounds__(256) k_leaf_hash(const u32* __restrict__ rows,
         s[4 * q + 2] = x.z;
         s[4 * q + 3] = x.w;
       }
-      permute<LW>(s, c);
+      u32 off = 0;
+      asm volatile("" : "+s"(off));
+      permute<LW>(s, cp[off]);
     }
   } else {
     for (int pos = 0; pos < row_len; pos += RATE) {
@@ -49,7 +54,9 @@ __global__ void __launch_bounds__(256) k_leaf_hash(const u32* __restrict__ rows,
 #pragma unroll
       for (int j = 0; j < RATE; j++)
         if (j < cnt) s[j] = r[pos + j];
-      permute<LW>(s, c);
+      u32 off = 0;
+      asm volatile("" : "+s"(off));
+      permute<LW>(s, cp[off]);
     }
   }
   uint4* o = reinterpret_cast<uint4*>(dig + i * 8);
@@ -149,13 +156,13 @@ __global__ void k_open(const u32* __restrict__ rows, int row_len, const u32* __r
 }
 
 template <int LW>
-int merkle_leaves_impl(const u32* rows, int row_len, u64 height, u32* tree, const P2Consts<LW>& c, hipStream_t st) {
+int merkle_leaves_impl(const u32* rows, int row_len, u64 height, u32* tree, const P2Consts<LW>* c, hipStream_t st) {
   unsigned blocks = (unsigned)((height + 255) / 256);
   hipLaunchKernelGGL((k_leaf_hash<LW>), dim3(blocks), dim3(256), 0, st, rows, row_len, height, tree, c);
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 
-int merkle_leaves(const u32* rows, int row_len, u64 height, int leaf_width, const P2C16& c16, const P2C24& c24,
+int merkle_leaves(const u32* rows, int row_len, u64 height, int leaf_width, const P2C16* c16, const P2C24* c24,
                   u32* tree, hipStream_t st) {
   if (height == 0 || (height & (height - 1))) return -1;
   return leaf_width == 24 ? merkle_leaves_impl<24>(rows, row_len, height, tree, c24, st)
@@ -188,9 +195,10 @@ int merkle_tree(u64 height, const P2C16& c16, u32* tree, hipStream_t st) {
   return 0;
 }
 
-int merkle_commit(const u32* rows, int row_len, u64 height, int leaf_width, const P2C16& c16, const P2C24& c24,
+int merkle_commit(const u32* rows, int row_len, u64 height, int leaf_width, const P2C16& c16, const P2C16* d_c16,
+                  const P2C24* d_c24,
                   u32* tree, hipStream_t st) {
-  int rc = merkle_leaves(rows, row_len, height, leaf_width, c16, c24, tree, st);
+  int rc = merkle_leaves(rows, row_len, height, leaf_width, d_c16, d_c24, tree, st);
   return rc ? rc : merkle_tree(height, c16, tree, st);
 }
 

@@ -52,11 +52,11 @@ int lde_ext(const ef* src, ef* dst, ef* tmp, int m, int k, int R, const Twiddles
 
 // ---- Merkle tree and openings (k_merkle.hip)
 // tree holds 2 * height - 1 digests of 8 u32; the root is the last one.
-int merkle_leaves(const u32* rows, int row_len, u64 height, int leaf_width, const P2C16& c16, const P2C24& c24,
+int merkle_leaves(const u32* rows, int row_len, u64 height, int leaf_width, const P2C16* d_c16, const P2C24* d_c24,
                   u32* tree, hipStream_t st);
 int merkle_tree(u64 height, const P2C16& c16, u32* tree, hipStream_t st);
-int merkle_commit(const u32* rows, int row_len, u64 height, int leaf_width, const P2C16& c16, const P2C24& c24,
-                  u32* tree, hipStream_t st);
+int merkle_commit(const u32* rows, int row_len, u64 height, int leaf_width, const P2C16& c16, const P2C16* d_c16,
+                  const P2C24* d_c24, u32* tree, hipStream_t st);
 int merkle_open(const u32* rows, int row_len, const u32* tree, u64 height, const u64* d_idx, int nq, u32* d_out,
                 hipStream_t st);
 // Scans [start, start + count) for the smallest witness.  *d_result must

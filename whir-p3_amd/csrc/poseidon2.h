@@ -126,34 +126,211 @@ __host__ __device__ __forceinline__ void internal_linear(u32* s) {
   else internal_linear24(s);
 }
 
-// The permutation in three phases (initial linear layer and external rounds;
-// internal rounds; terminal external rounds), so a caller can stop between
-// them.
-template <int W>
-__host__ __device__ __forceinline__ void permute_head(u32* s, const P2Consts<W>& c) {
-  external_linear<W>(s);
+// ---- width 16 with lazy reduction
+// The linear layers add in 64 bits (one full-rate instruction per add on
+// CDNA) and each result is reduced once.  Because 2^31 = 2^24 - 1 (mod p), a
+// value y < 2^37.9 splits as y = hi 2^31 + lo with hi < 126 and
+//   y = lo + hi (2^24 - 1) (mod p),  which is < 2p,
+// so lazy_reduce (three instructions) gives a representative below 2p and
+// full_reduce adds one conditional subtraction.  Between internal rounds
+// elements 1..15 stay below 2p; S-box inputs are always fully reduced.
+//
+// Internal diagonal entries 2^-k use p = 127 2^24 + 1: 2^-k = -127 2^(24-k),
+// so for z = q 2^k + r (r < 2^k)
+//   z 2^-k = q - r 127 2^(24-k)   (mod p),
+// with no multiplication by a full-size constant.  Negative entries -d use
+// 2p - z, which is -z mod p and positive for z < 2p.
+//
+// Every intermediate bound is stated where it is used; the largest 64-bit
+// value reduced is below 71p < 2^37.2.
+constexpr u32 TWO_P = 2u * P;
+
+// 32x32+64 multiply-adds (v_mad_u64_u32 / v_mad_i64_i32, one full-rate
+// instruction each on CDNA).
+__host__ __device__ __forceinline__ u64 mad_u64(u32 a, u32 b, u64 c) { return (u64)a * b + c; }
+__host__ __device__ __forceinline__ u64 mad_i64(int32_t a, int32_t b, u64 c) {
+  return (u64)((int64_t)a * (int64_t)b + (int64_t)c);
+}
+
+// y - (y >> 31) p = (y mod 2^31) + (y >> 31)(2^24 - 1): for y < 2^37.9 the
+// quotient is below 126 and the result below 2p.  Kept 64-bit (its high word
+// is zero) so it feeds 64-bit adds directly.
+// y >> 31 as a 32-bit value.  On the device it comes from v_alignbit_b32,
+// which keeps the compiler from widening the multiply-add below into a 64x64
+// product.
+__host__ __device__ __forceinline__ u32 quot31(u64 y) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_alignbit((u32)(y >> 32), (u32)y, 31);
+#else
+  return (u32)(y >> 31);
+#endif
+}
+__host__ __device__ __forceinline__ u64 lazy_reduce(u64 y) {
+  return mad_i64((int32_t)quot31(y), -(int32_t)P, y);
+}
+__host__ __device__ __forceinline__ u32 full_reduce(u64 y) {   // y < 2^37.9 -> [0, p)
+  const u32 z = (u32)lazy_reduce(y);
+  const u32 t = z - P;
+  return t < z ? t : z;
+}
+
+// sum + z 2^-K and sum - z 2^-K for z < 2p; `sumk` = sum + 2p keeps both
+// nonnegative (r 127 2^(24-K) < 2^31 < 2p, and z >> K < p).
+template <int K>
+__host__ __device__ __forceinline__ u64 add_div2k(u64 sumk, u32 z) {
+  constexpr int32_t CK = 127 << (24 - K);
+  return mad_i64((int32_t)(z & ((1u << K) - 1)), -CK, mad_u64(z >> K, 1u, sumk));
+}
+template <int K>
+__host__ __device__ __forceinline__ u64 sub_div2k(u64 sumk, u32 z) {
+  constexpr u32 CK = 127u << (24 - K);
+  return mad_u64(z & ((1u << K) - 1), CK, sumk + (P - (z >> K)));   // -q = p - q (mod p)
+}
+
+// x^3 / 2^64 mod p in [0, 2p) as a 64-bit value with a zero high word, for
+// x < p: the last Montgomery product skips its conditional subtraction
+// (y / 2^32 < 2p there).
+__host__ __device__ __forceinline__ u64 cube_lazy(u32 x) {
+  const u32 x2 = mul(x, x);
+  const u64 m = (u64)x2 * x;
+  const u32 t = (u32)m * MU_NEG;
+  return ((u64)t * P + m) >> 32;
+}
+
+// External layer on inputs x < 2p: M4 on each quad, then each element plus
+// the sum of its position over the quads.  Outputs < 70p.
+__host__ __device__ __forceinline__ void external_linear16_wide(const u64* x, u64* y) {
+  u64 n[16];
+#pragma unroll
+  for (int c = 0; c < 16; c += 4) {
+    const u64 x0 = x[c], x1 = x[c + 1], x2 = x[c + 2], x3 = x[c + 3];
+    const u64 t01 = x0 + x1;
+    const u64 t23 = x2 + x3;
+    const u64 t0123 = t01 + t23;
+    const u64 t01123 = t0123 + x1;
+    const u64 t01233 = t0123 + x3;
+    n[c + 3] = t01233 + 2 * x0;   // 3x0 + x1 + x2 + 2x3
+    n[c + 1] = t01123 + 2 * x2;   // x0 + 2x1 + 3x2 + x3
+    n[c + 0] = t01123 + t01;      // 2x0 + 3x1 + x2 + x3
+    n[c + 2] = t01233 + t23;      // x0 + x1 + 2x2 + 3x3
+  }
+  u64 s[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) s[k] = n[k] + n[4 + k] + n[8 + k] + n[12 + k];
+#pragma unroll
+  for (int i = 0; i < 16; i++) y[i] = n[i] + s[i & 3];
+}
+
+// Initial linear layer and the four initial external rounds.  In: s < p.
+// Out: s[0] < p, s[1..15] < 2p.
+__host__ __device__ __forceinline__ void permute16_head(u32* s, const P2C16& c) {
+  u64 x[16], y[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) x[i] = s[i];
+  external_linear16_wide(x, y);
 #pragma unroll
   for (int r = 0; r < 4; r++) {
 #pragma unroll
-    for (int i = 0; i < W; i++) s[i] = cube(add(s[i], c.ext_init[r * W + i]));
+    for (int i = 0; i < 16; i++) x[i] = cube_lazy(full_reduce(y[i] + c.ext_init[r * 16 + i]));   // < 71p
+    external_linear16_wide(x, y);
+  }
+  s[0] = full_reduce(y[0]);
+#pragma unroll
+  for (int i = 1; i < 16; i++) s[i] = (u32)lazy_reduce(y[i]);
+}
+
+// The twenty internal rounds: S-box on element 0, then 1 + diag(V) with
+// V = [-2, 1, 2, 1/2, 3, 4, -1/2, -3, -4, 1/2^8, 1/8, 1/2^24, -1/2^8, -1/8,
+// -1/16, -1/2^24].  In and out: s[0] < p, s[1..15] < 2p.
+__host__ __device__ __forceinline__ void permute16_internal(u32* s, const P2C16& c) {
+  u64 z[16];
+#pragma unroll
+  for (int i = 1; i < 16; i++) z[i] = s[i];
+  u64 y0 = s[0];
+#pragma unroll
+  for (int r = 0; r < P2C16::RP; r++) {
+    const u32 x0 = cube(full_reduce(y0 + c.internal[r]));   // y0 + rc < 36p
+    u64 sum = x0;
+#pragma unroll
+    for (int i = 1; i < 16; i++) sum += z[i];   // < 31p
+    const u64 sumk = sum + TWO_P;
+    y0 = mad_i64((int32_t)x0, -2, sum + 2ull * TWO_P);   // sum - 2 x0, < 35p
+    z[1] = lazy_reduce(sum + z[1]);
+    z[2] = lazy_reduce(sum + 2 * z[2]);
+    z[3] = lazy_reduce(add_div2k<1>(sumk, (u32)z[3]));
+    z[4] = lazy_reduce(mad_u64((u32)z[4], 3u, sum));
+    z[5] = lazy_reduce(sum + 4 * z[5]);
+    z[6] = lazy_reduce(sub_div2k<1>(sumk, (u32)z[6]));
+    z[7] = lazy_reduce(mad_u64(TWO_P - (u32)z[7], 3u, sum));
+    z[8] = lazy_reduce(mad_u64(TWO_P - (u32)z[8], 4u, sum));
+    z[9] = lazy_reduce(add_div2k<8>(sumk, (u32)z[9]));
+    z[10] = lazy_reduce(add_div2k<3>(sumk, (u32)z[10]));
+    z[11] = lazy_reduce(add_div2k<24>(sumk, (u32)z[11]));
+    z[12] = lazy_reduce(sub_div2k<8>(sumk, (u32)z[12]));
+    z[13] = lazy_reduce(sub_div2k<3>(sumk, (u32)z[13]));
+    z[14] = lazy_reduce(sub_div2k<4>(sumk, (u32)z[14]));
+    z[15] = lazy_reduce(sub_div2k<24>(sumk, (u32)z[15]));
+  }
+  s[0] = full_reduce(y0);
+#pragma unroll
+  for (int i = 1; i < 16; i++) s[i] = (u32)z[i];
+}
+
+// The four terminal external rounds.  In: s[0] < p, s[1..15] < 2p.  Out: s < p.
+__host__ __device__ __forceinline__ void permute16_tail(u32* s, const P2C16& c) {
+  u64 x[16], y[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) y[i] = s[i];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+#pragma unroll
+    for (int i = 0; i < 16; i++) x[i] = cube_lazy(full_reduce(y[i] + c.ext_term[r * 16 + i]));
+    external_linear16_wide(x, y);
+  }
+#pragma unroll
+  for (int i = 0; i < 16; i++) s[i] = full_reduce(y[i]);
+}
+
+// The permutation in three phases (initial linear layer and external rounds;
+// internal rounds; terminal external rounds), so a caller can stop between
+// them.  Width 16 takes the lazy-reduction path above.
+template <int W>
+__host__ __device__ __forceinline__ void permute_head(u32* s, const P2Consts<W>& c) {
+  if constexpr (W == 16) {
+    permute16_head(s, c);
+  } else {
     external_linear<W>(s);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+#pragma unroll
+      for (int i = 0; i < W; i++) s[i] = cube(add(s[i], c.ext_init[r * W + i]));
+      external_linear<W>(s);
+    }
   }
 }
 template <int W>
 __host__ __device__ __forceinline__ void permute_internal(u32* s, const P2Consts<W>& c) {
+  if constexpr (W == 16) {
+    permute16_internal(s, c);
+  } else {
 #pragma unroll
-  for (int r = 0; r < P2Consts<W>::RP; r++) {
-    s[0] = cube(add(s[0], c.internal[r]));
-    internal_linear<W>(s);
+    for (int r = 0; r < P2Consts<W>::RP; r++) {
+      s[0] = cube(add(s[0], c.internal[r]));
+      internal_linear<W>(s);
+    }
   }
 }
 template <int W>
 __host__ __device__ __forceinline__ void permute_tail(u32* s, const P2Consts<W>& c) {
+  if constexpr (W == 16) {
+    permute16_tail(s, c);
+  } else {
 #pragma unroll
-  for (int r = 0; r < 4; r++) {
+    for (int r = 0; r < 4; r++) {
 #pragma unroll
-    for (int i = 0; i < W; i++) s[i] = cube(add(s[i], c.ext_term[r * W + i]));
-    external_linear<W>(s);
+      for (int i = 0; i < W; i++) s[i] = cube(add(s[i], c.ext_term[r * W + i]));
+      external_linear<W>(s);
+    }
   }
 }
 
@@ -162,6 +339,24 @@ __host__ __device__ __forceinline__ void permute(u32* s, const P2Consts<W>& c) {
   permute_head<W>(s, c);
   permute_internal<W>(s, c);
   permute_tail<W>(s, c);
+}
+
+// The textbook width-16 permutation (one reduction per operation), kept as the
+// reference the lazy path is tested against.
+__host__ __device__ inline void permute16_plain(u32* s, const P2C16& c) {
+  external_linear<16>(s);
+  for (int r = 0; r < 4; r++) {
+    for (int i = 0; i < 16; i++) s[i] = cube(add(s[i], c.ext_init[r * 16 + i]));
+    external_linear<16>(s);
+  }
+  for (int r = 0; r < P2C16::RP; r++) {
+    s[0] = cube(add(s[0], c.internal[r]));
+    internal_linear16(s);
+  }
+  for (int r = 0; r < 4; r++) {
+    for (int i = 0; i < 16; i++) s[i] = cube(add(s[i], c.ext_term[r * 16 + i]));
+    external_linear<16>(s);
+  }
 }
 
 }  // namespace kb

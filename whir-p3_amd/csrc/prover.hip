@@ -118,7 +118,8 @@ struct whirgpu_ctx {
   u32* d_open = nullptr;
   size_t open_cap = 0;
   u32* d_grind = nullptr;
-  P2C16* d_c16 = nullptr;     // device copy of the width-16 constants (grinding reads them in a loop)
+  P2C16* d_c16 = nullptr;     // device copies of the constants (kernels that loop over
+  P2C24* d_c24 = nullptr;     // permutations read them from memory)
   // device transcript of sumcheck rounds: challenger, grind base, round log, challenge
   static constexpr int SC_LOG_CAP = 64;
   DevChallenger* d_chal = nullptr;
@@ -651,7 +652,7 @@ void commit_tree(whirgpu_ctx* c, whirgpu_prover_data* pd) {
   const int rate = lw - 8;
   {
     Span sp(c, ST_LEAVES, (double)pd->height * (double)((pd->row_len + rate - 1) / rate));
-    KCHK(merkle_leaves(pd->rows.p, pd->row_len, pd->height, lw, c->c16, c->c24, pd->tree.p, c->st));
+    KCHK(merkle_leaves(pd->rows.p, pd->row_len, pd->height, lw, c->d_c16, c->d_c24, pd->tree.p, c->st));
   }
   Span sp(c, ST_TREE, (double)(pd->height - 1));
   KCHK(merkle_tree(pd->height, c->c16, pd->tree.p, c->st));
@@ -829,6 +830,8 @@ int whirgpu_create(int device, const whirgpu_params* params, const whirgpu_posei
     HIPCHK(hipMalloc(&c->d_sclog, whirgpu_ctx::SC_LOG_CAP * sizeof(ScLog)));
     HIPCHK(hipMalloc(&c->d_r, 256));
     HIPCHK(hipMemcpy(c->d_c16, &c->c16, sizeof(P2C16), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&c->d_c24, sizeof(P2C24)));
+    HIPCHK(hipMemcpy(c->d_c24, &c->c24, sizeof(P2C24), hipMemcpyHostToDevice));
     c->pin<u32>(1);
     c->ring_cap = 1 << 20;
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_ring), c->ring_cap, hipHostMallocDefault));
@@ -848,7 +851,7 @@ int whirgpu_destroy(whirgpu_ctx* c) {
   c->pool.release_all();
   if (c->h_small) (void)hipHostFree(c->h_small);
   for (void* p : {(void*)c->d_tw, (void*)c->d_tabs, (void*)c->d_partial, (void*)c->d_idx,
-                  (void*)c->d_open, (void*)c->d_grind, (void*)c->d_c16, (void*)c->d_chal,
+                  (void*)c->d_open, (void*)c->d_grind, (void*)c->d_c16, (void*)c->d_c24, (void*)c->d_chal,
                   (void*)c->d_gbase, (void*)c->d_sclog, (void*)c->d_r})
     if (p) (void)hipFree(p);
   if (c->h_pin) (void)hipHostFree(c->h_pin);
@@ -1294,7 +1297,8 @@ int whirgpu_merkle_commit(whirgpu_ctx* c, const uint32_t* d_rows, uint64_t heigh
     if (!c || !d_rows || !d_tree || !root_out || row_len == 0) bad("null argument");
     if (height == 0 || (height & (height - 1))) bad("height must be a power of two");
     use_device(c);
-    KCHK(merkle_commit(d_rows, (int)row_len, height, (int)c->params.leaf_hash_width, c->c16, c->c24, d_tree, c->st));
+    KCHK(merkle_commit(d_rows, (int)row_len, height, (int)c->params.leaf_hash_width, c->c16, c->d_c16, c->d_c24,
+                       d_tree, c->st));
     c->d2h(root_out, d_tree + (2 * height - 2) * 8, 8);
   });
 }
