@@ -113,7 +113,12 @@ __global__ void __launch_bounds__(256) k_tree_level_wide(const u32* __restrict__
 // The top levels of a tree in one launch: n (<= 128) digests at `off` are
 // staged in LDS and reduced to the root, every level written out.  One block
 // of 1024 lanes, 64 nodes per step.
-__global__ void __launch_bounds__(1024) k_tree_top_wide(u32* __restrict__ tree, u64 off, int n, P2C16 c) {
+// Waves whose four nodes are all past the level's end skip the permutation
+// (the lane-parallel permutation only needs its own wave's lanes), so the
+// small top levels run one wave per SIMD instead of four.  The root is also
+// written to `root_out` when given (host-visible memory).
+__global__ void __launch_bounds__(1024) k_tree_top_wide(u32* __restrict__ tree, u64 off, int n, P2C16 c,
+                                                        u32* root_out) {
   __shared__ u32 sh[2][128 * 8];
   const int t = threadIdx.x;
   const int j = t & 15;
@@ -124,35 +129,43 @@ __global__ void __launch_bounds__(1024) k_tree_top_wide(u32* __restrict__ tree, 
   while (n > 1) {
     const int n_out = n >> 1;
     const int p = t >> 4;
-    u32 x = p < n_out ? sh[cur][16 * p + j] : 0u;
-    x = permute16_wide(x, j, w, c);
-    if (p < n_out && j < 8) {
-      sh[cur ^ 1][8 * p + j] = x;
-      tree[(off + n + p) * 8 + j] = x;
+    if (4 * (t >> 6) < n_out) {   // wave-uniform
+      u32 x = p < n_out ? sh[cur][16 * p + j] : 0u;
+      x = permute16_wide(x, j, w, c);
+      if (p < n_out && j < 8) {
+        sh[cur ^ 1][8 * p + j] = x;
+        tree[(off + n + p) * 8 + j] = x;
+      }
     }
     __syncthreads();
     off += n;
     n = n_out;
     cur ^= 1;
   }
+  if (root_out && t < 8) root_out[t] = sh[cur][t];
 }
 
 // Opening of `nq` rows: out[q] = row (row_len) followed by depth sibling digests.
+// Grid: (ceil(per / 256), nq); block (x, q) copies entries x*256.. of query q's
+// opening (row, then the sibling digest of each level).  `idx` and `out` may
+// be host-visible memory.
 __global__ void k_open(const u32* __restrict__ rows, int row_len, const u32* __restrict__ tree, u64 height, int depth,
-                       const u64* __restrict__ idx, int nq, u32* __restrict__ out) {
+                       const u64* __restrict__ idx, u32* __restrict__ out) {
   const u64 per = (u64)row_len + 8ull * depth;
-  for (u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x; e < per * nq; e += (u64)gridDim.x * blockDim.x) {
-    u64 q = e / per, o = e % per;
-    u64 ix = idx[q];
-    if (o < (u64)row_len) {
-      out[e] = rows[ix * row_len + o];
-    } else {
-      u64 l = (o - row_len) / 8, k = (o - row_len) % 8;
-      u64 off = 2 * height - ((2 * height) >> l);
-      u64 node = (ix >> l) ^ 1;
-      out[e] = tree[(off + node) * 8 + k];
-    }
+  const u64 q = blockIdx.y;
+  const u64 o = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= per) return;
+  const u64 ix = idx[q];
+  u32 v;
+  if (o < (u64)row_len) {
+    v = rows[ix * row_len + o];
+  } else {
+    u64 l = (o - row_len) / 8, k = (o - row_len) % 8;
+    u64 off = 2 * height - ((2 * height) >> l);
+    u64 node = (ix >> l) ^ 1;
+    v = tree[(off + node) * 8 + k];
   }
+  out[q * per + o] = v;
 }
 
 template <int LW>
@@ -171,7 +184,7 @@ int merkle_leaves(const u32* rows, int row_len, u64 height, int leaf_width, cons
 
 // Levels with more than 2^14 nodes use one lane per node (throughput), smaller
 // ones 16 lanes per node (latency), and the last 7 levels run in one launch.
-int merkle_tree(u64 height, const P2C16& c16, u32* tree, hipStream_t st) {
+int merkle_tree(u64 height, const P2C16& c16, u32* tree, hipStream_t st, u32* root_out) {
   u64 off = 0, n = height;
   while (n > 128) {
     u64 n_out = n >> 1;
@@ -189,8 +202,10 @@ int merkle_tree(u64 height, const P2C16& c16, u32* tree, hipStream_t st) {
     n = n_out;
   }
   if (n > 1) {
-    hipLaunchKernelGGL(k_tree_top_wide, dim3(1), dim3(1024), 0, st, tree, off, (int)n, c16);
+    hipLaunchKernelGGL(k_tree_top_wide, dim3(1), dim3(1024), 0, st, tree, off, (int)n, c16, root_out);
     if (hipGetLastError() != hipSuccess) return -5;
+  } else if (root_out) {
+    return 1;   // single leaf: the root was not written
   }
   return 0;
 }
@@ -199,18 +214,19 @@ int merkle_commit(const u32* rows, int row_len, u64 height, int leaf_width, cons
                   const P2C24* d_c24,
                   u32* tree, hipStream_t st) {
   int rc = merkle_leaves(rows, row_len, height, leaf_width, d_c16, d_c24, tree, st);
-  return rc ? rc : merkle_tree(height, c16, tree, st);
+  if (rc) return rc;
+  rc = merkle_tree(height, c16, tree, st, nullptr);
+  return rc > 0 ? 0 : rc;
 }
 
 int merkle_open(const u32* rows, int row_len, const u32* tree, u64 height, const u64* d_idx, int nq, u32* d_out,
                 hipStream_t st) {
   int depth = 0;
   while ((1ull << depth) < height) depth++;
-  u64 total = ((u64)row_len + 8ull * depth) * nq;
-  unsigned blocks = (unsigned)((total + 255) / 256);
-  if (blocks == 0) return 0;
-  if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL(k_open, dim3(blocks), dim3(256), 0, st, rows, row_len, tree, height, depth, d_idx, nq, d_out);
+  const u64 per = (u64)row_len + 8ull * depth;
+  if (nq <= 0) return 0;
+  hipLaunchKernelGGL(k_open, dim3((unsigned)((per + 255) / 256), (unsigned)nq), dim3(256), 0, st, rows, row_len, tree,
+                     height, depth, d_idx, d_out);
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 

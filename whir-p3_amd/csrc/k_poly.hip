@@ -31,6 +31,34 @@ __global__ void k_eq_small(PointArg z, int first, int t, ef scale, ef* __restric
   out[x] = acc;
 }
 
+__global__ void k_eq_split(PointArg z, int hi, int lo, ef scale, ef* __restrict__ out_hi, ef* __restrict__ out_lo) {
+  u32 x = blockIdx.x * blockDim.x + threadIdx.x;
+  const u32 nh = 1u << hi;
+  if (x >= nh + (1u << lo)) return;
+  int first = 0, t = hi;
+  ef acc = scale;
+  ef* out = out_hi;
+  if (x >= nh) {
+    x -= nh;
+    first = hi;
+    t = lo;
+    acc = ef_one();
+    out = out_lo;
+  }
+  for (int j = 0; j < t; j++) {
+    const ef& zj = z.z[first + j];
+    int bit = (x >> (t - 1 - j)) & 1;
+    acc = ef_mul(acc, bit ? zj : ef_sub(ef_one(), zj));
+  }
+  out[x] = acc;
+}
+
+int eq_split(const PointArg& z, int hi, int lo, const ef& scale, ef* out_hi, ef* out_lo, hipStream_t st) {
+  u32 n = (1u << hi) + (1u << lo);
+  hipLaunchKernelGGL(k_eq_split, dim3((n + 255) / 256), dim3(256), 0, st, z, hi, lo, scale, out_hi, out_lo);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
 int eq_table(const PointArg& z, int first, int t, const ef& scale, ef* out, hipStream_t st) {
   u32 n = 1u << t;
   hipLaunchKernelGGL(k_eq_small, dim3((n + 255) / 256), dim3(256), 0, st, z, first, t, scale, out);

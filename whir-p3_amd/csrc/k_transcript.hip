@@ -85,18 +85,22 @@ __device__ __forceinline__ ef wave_sum(ef v) {
 // phase 1: read the grind result, observe it, check its bits, sample r.
 // r goes to *d_r and the log; log->err is set when the witness is missing or
 // does not check.
+// The challenger is read from `ch`, or from `init` (a kernel argument) when
+// `use_init` is set, and written back to `ch`.
 __global__ void __launch_bounds__(64) k_sc_transcript(int phase, const ef* __restrict__ partial, int nb,
-                                                      DevChallenger* ch, GrindBase* gb, u32* d_grind, ScLog* log,
-                                                      int pow_bits, ef* d_r, const P2C16* __restrict__ cp) {
+                                                      DevChallenger* ch, DevChallenger init, int use_init,
+                                                      GrindBase* gb, u32* d_grind, ScLog* log, int pow_bits, ef* d_r,
+                                                      const P2C16* __restrict__ cp) {
   __shared__ u32 sp[16], in[8], out[8];
   const int lane = threadIdx.x;
-  if (lane < 16) sp[lane] = ch->sponge[lane];
+  const DevChallenger& src = use_init ? init : *ch;
+  if (lane < 16) sp[lane] = src.sponge[lane];
   if (lane < 8) {
-    in[lane] = ch->in[lane];
-    out[lane] = ch->out[lane];
+    in[lane] = src.in[lane];
+    out[lane] = src.out[lane];
   }
   const WideLane wl = wide_lane(*cp, lane & 15);
-  WaveChallenger wc{sp, in, out, (int)ch->in_len, (int)ch->out_len, &wl, cp};
+  WaveChallenger wc{sp, in, out, (int)src.in_len, (int)src.out_len, &wl, cp};
   __syncthreads();
   u32 err = 0;
   bool sample_r = true;
@@ -156,10 +160,13 @@ __global__ void __launch_bounds__(64) k_sc_transcript(int phase, const ef* __res
   }
 }
 
-int sc_transcript(int phase, const ef* partial, unsigned nb, DevChallenger* ch, GrindBase* gb, u32* d_grind,
-                  ScLog* log, int pow_bits, ef* d_r, const P2C16* d_consts, hipStream_t st) {
-  hipLaunchKernelGGL(k_sc_transcript, dim3(1), dim3(64), 0, st, phase, partial, (int)nb, ch, gb, d_grind, log,
-                     pow_bits, d_r, d_consts);
+int sc_transcript(int phase, const ef* partial, unsigned nb, DevChallenger* ch, const DevChallenger* init,
+                  GrindBase* gb, u32* d_grind, ScLog* log, int pow_bits, ef* d_r, const P2C16* d_consts,
+                  hipStream_t st) {
+  DevChallenger iv{};
+  if (init) iv = *init;
+  hipLaunchKernelGGL(k_sc_transcript, dim3(1), dim3(64), 0, st, phase, partial, (int)nb, ch, iv, init ? 1 : 0, gb,
+                     d_grind, log, pow_bits, d_r, d_consts);
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 

@@ -54,7 +54,8 @@ int lde_ext(const ef* src, ef* dst, ef* tmp, int m, int k, int R, const Twiddles
 // tree holds 2 * height - 1 digests of 8 u32; the root is the last one.
 int merkle_leaves(const u32* rows, int row_len, u64 height, int leaf_width, const P2C16* d_c16, const P2C24* d_c24,
                   u32* tree, hipStream_t st);
-int merkle_tree(u64 height, const P2C16& c16, u32* tree, hipStream_t st);
+// Writes the root to root_out too when given; returns 1 (root not written) for a single leaf.
+int merkle_tree(u64 height, const P2C16& c16, u32* tree, hipStream_t st, u32* root_out);
 int merkle_commit(const u32* rows, int row_len, u64 height, int leaf_width, const P2C16& c16, const P2C16* d_c16,
                   const P2C24* d_c24, u32* tree, hipStream_t st);
 int merkle_open(const u32* rows, int row_len, const u32* tree, u64 height, const u64* d_idx, int nq, u32* d_out,
@@ -69,6 +70,8 @@ int grind_range_dev(const P2C16* d_consts, const GrindBase* d_base, int bits, u3
 
 // ---- multilinear kernels (k_poly.hip)
 int eq_table(const PointArg& z, int first, int t, const ef& scale, ef* out, hipStream_t st);
+// Both halves of a split table in one launch: out_hi = scale * eq(z[0..hi)), out_lo = eq(z[hi..hi+lo)).
+int eq_split(const PointArg& z, int hi, int lo, const ef& scale, ef* out_hi, ef* out_lo, hipStream_t st);
 int eq_outer(const ef* Ehi, const ef* Elo, int lo, u64 n, ef* out, hipStream_t st);
 int eq_dot_base(const u32* f, int nf, int sb, int lo, const EqTables& tabs, int K, ef* partial, int nblocks, ef* out,
                 hipStream_t st);
@@ -86,7 +89,10 @@ int sc_fold(ef* e, ef* w, u64 h, const ef& r, ef* partial, ef* c0c2, hipStream_t
 int sc_partials(const ef* e, const ef* w, u64 h, ef* partial, unsigned* nb, hipStream_t st);
 int sc_fold_dev(ef* e, ef* w, u64 h, const ef* d_r, ef* partial, unsigned* nb, hipStream_t st);
 // ---- device transcript (k_transcript.hip)
-int sc_transcript(int phase, const ef* partial, unsigned nb, DevChallenger* ch, GrindBase* gb, u32* d_grind,
-                  ScLog* log, int pow_bits, ef* d_r, const P2C16* d_consts, hipStream_t st);
+// `init`, when given, is the challenger state to start from (passed by value
+// to the kernel); otherwise the state in `ch` is used.  The result goes to `ch`.
+int sc_transcript(int phase, const ef* partial, unsigned nb, DevChallenger* ch, const DevChallenger* init,
+                  GrindBase* gb, u32* d_grind, ScLog* log, int pow_bits, ef* d_r, const P2C16* d_consts,
+                  hipStream_t st);
 
 }  // namespace kb

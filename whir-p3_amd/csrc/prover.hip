@@ -114,9 +114,20 @@ struct whirgpu_ctx {
   }
   ef* d_tabs = nullptr;      // eq tables: 8 slots of tab_cap
   size_t tab_cap = 0;
-  u64* d_idx = nullptr;
-  u32* d_open = nullptr;
+  // Host-visible coherent memory that kernels read or write directly (no copy
+  // kernels): query indices in, opened rows and Merkle roots out.  Each holds
+  // one call's data and is reused only after the stream has been synchronized.
+  u64* h_idx = nullptr;
+  u64* d_idx = nullptr;   // device address of h_idx
+  u32* h_open = nullptr;
+  u32* d_open = nullptr;  // device address of h_open
   size_t open_cap = 0;
+  u32* h_root = nullptr;
+  u32* d_root = nullptr;  // device address of h_root
+  // Grinding result slots for host-driven grinds, refilled with p in bulk.
+  static constexpr int GRIND_SLOTS = 64;
+  u32* d_gslots = nullptr;
+  int gslot = GRIND_SLOTS;
   u32* d_grind = nullptr;
   P2C16* d_c16 = nullptr;     // device copies of the constants (kernels that loop over
   P2C24* d_c24 = nullptr;     // permutations read them from memory)
@@ -124,7 +135,8 @@ struct whirgpu_ctx {
   static constexpr int SC_LOG_CAP = 64;
   DevChallenger* d_chal = nullptr;
   GrindBase* d_gbase = nullptr;
-  ScLog* d_sclog = nullptr;
+  ScLog* h_sclog = nullptr;   // coherent host memory; the transcript kernel logs into it
+  ScLog* d_sclog = nullptr;   // device address of h_sclog
   ef* d_r = nullptr;
   int grind_blocks = 256;
   double timings[7] = {0};
@@ -225,6 +237,7 @@ struct whirgpu_prover_data {
   u64 height = 0;
   int row_len = 0;   // base elements per leaf
   bool ext = false;
+  bool root_on_host = false;   // the last tree kernel wrote the root to the context's h_root
 };
 
 struct whirgpu_proof {
@@ -310,8 +323,7 @@ void build_split_eq(whirgpu_ctx* c, const std::vector<ef>& z, int lo, const ef& 
   if ((1ull << hi) + (1ull << lo) > c->tab_cap) bad("eq table too large for the context");
   ef* base = c->d_tabs + (size_t)slot * c->tab_cap;
   PointArg a = point_arg(z);
-  KCHK(eq_table(a, 0, hi, scale, base, c->st));
-  KCHK(eq_table(a, hi, lo, ef_one(), base + (1ull << hi), c->st));
+  KCHK(eq_split(a, hi, lo, scale, base, base + (1ull << hi), c->st));
   t->hi[k] = base;
   t->lo[k] = base + (1ull << hi);
 }
@@ -362,8 +374,13 @@ u32 gpu_grind(whirgpu_ctx* c, DuplexChallenger& ch, int bits) {
   u32 s[16];
   for (int i = 0; i < 16; i++) s[i] = ch.sponge[i];
   for (int i = 0; i < ch.in_len; i++) s[i] = ch.in[i];
-  const u32 init = P;
-  c->h2d(c->d_grind, &init, 1);
+  if (c->gslot == whirgpu_ctx::GRIND_SLOTS) {
+    u32 init[whirgpu_ctx::GRIND_SLOTS];
+    for (u32& v : init) v = P;
+    c->h2d(c->d_gslots, init, whirgpu_ctx::GRIND_SLOTS);
+    c->gslot = 0;
+  }
+  u32* d_res = c->d_gslots + c->gslot++;
   // Each launch covers sixteen times the expected number of candidates (a
   // witness lies inside with probability 1 - e^-16; the waves stop as soon as
   // one is known, so the size costs nothing); the search continues with the
@@ -372,8 +389,8 @@ u32 gpu_grind(whirgpu_ctx* c, DuplexChallenger& ch, int bits) {
   u32 w = P;
   Span sp(c, ST_GRIND, 0);
   for (u64 start = 0; start < P && w >= P; start += span) {
-    KCHK(grind_range(c->d_c16, s, ch.in_len, bits, c->d_grind, start, span, c->grind_blocks, c->st));
-    c->d2h(&w, c->d_grind, 1);
+    KCHK(grind_range(c->d_c16, s, ch.in_len, bits, d_res, start, span, c->grind_blocks, c->st));
+    c->d2h(&w, d_res, 1);
   }
   sp.close();
   if (!c->recs.empty() && c->prof) c->recs.back().work = (double)w + 1;
@@ -453,7 +470,7 @@ std::vector<ef> device_sumcheck(whirgpu_ctx* c, SumcheckState& s, int rounds, in
   for (int i = 0; i < ch.out_len; i++) dc.out[i] = ch.out[i];
   dc.in_len = (u32)ch.in_len;
   dc.out_len = (u32)ch.out_len;
-  c->h2d(c->d_chal, &dc, 1);
+  // h_sclog is free: every phase ends with a synchronization before its log is read.
   std::vector<size_t> grind_recs;
   for (int i = 0; i < rounds; i++) {
     const u64 h = 1ull << (s.m - 1);
@@ -461,8 +478,8 @@ std::vector<ef> device_sumcheck(whirgpu_ctx* c, SumcheckState& s, int rounds, in
       Span sp(c, ST_SUMCHECK, 64.0 * (double)h);
       KCHK(sc_partials(s.e.p, s.w.p, h, c->d_partial, &s.partial_nb, c->st));
     }
-    KCHK(sc_transcript(0, c->d_partial, s.partial_nb, c->d_chal, c->d_gbase, c->d_grind, c->d_sclog + i, pow_bits,
-                       c->d_r, c->d_c16, c->st));
+    KCHK(sc_transcript(0, c->d_partial, s.partial_nb, c->d_chal, i == 0 ? &dc : nullptr, c->d_gbase, c->d_grind,
+                       c->d_sclog + i, pow_bits, c->d_r, c->d_c16, c->st));
     if (pow_bits > 0) {
       {
         Span sp(c, ST_GRIND, 0);
@@ -470,8 +487,8 @@ std::vector<ef> device_sumcheck(whirgpu_ctx* c, SumcheckState& s, int rounds, in
         sp.close();
         if (c->prof) grind_recs.push_back(c->recs.size() - 1);
       }
-      KCHK(sc_transcript(1, c->d_partial, 0, c->d_chal, c->d_gbase, c->d_grind, c->d_sclog + i, pow_bits, c->d_r,
-                         c->d_c16, c->st));
+      KCHK(sc_transcript(1, c->d_partial, 0, c->d_chal, nullptr, c->d_gbase, c->d_grind, c->d_sclog + i, pow_bits,
+                         c->d_r, c->d_c16, c->st));
     }
     {
       Span sp(c, ST_SUMCHECK, 96.0 * (double)h);
@@ -481,8 +498,8 @@ std::vector<ef> device_sumcheck(whirgpu_ctx* c, SumcheckState& s, int rounds, in
   }
   // d_partial is scratch for other kernels between phases
   s.invalidate();
-  std::vector<ScLog> log(rounds);
-  c->d2h(log.data(), c->d_sclog, (size_t)rounds);
+  c->sync();
+  std::vector<ScLog> log(c->h_sclog, c->h_sclog + rounds);
   for (int i = 0; i < rounds; i++) {
     const ScLog& L = log[i];
     if (L.err) bad("device transcript: no valid proof-of-work witness", WHIRGPU_GRIND_FAILED);
@@ -621,12 +638,13 @@ void open_rows(whirgpu_ctx* c, const whirgpu_prover_data* pd, const std::vector<
   int depth = ilog2u(pd->height);
   size_t per = (size_t)pd->row_len + 8ull * depth;
   if (per * nq > c->open_cap || nq > 4096) bad("too many queries for the opening buffer");
-  c->h2d(c->d_idx, idx.data(), idx.size());
+  // h_idx / h_open are free: every opening ends with a synchronization.
+  memcpy(c->h_idx, idx.data(), idx.size() * sizeof(u64));
   KCHK(merkle_open(pd->rows.p, pd->row_len, pd->tree.p, pd->height, c->d_idx, nq, c->d_open, c->st));
-  std::vector<u32> h(per * nq);
-  c->d2h(h.data(), c->d_open, h.size());
+  c->sync();
+  const u32* h = c->h_open;
   for (int q = 0; q < nq; q++) {
-    const u32* b = h.data() + per * q;
+    const u32* b = h + per * q;
     whir::QueryOpening o;
     o.extension = pd->ext;
     o.values.assign(b, b + pd->row_len);
@@ -655,12 +673,20 @@ void commit_tree(whirgpu_ctx* c, whirgpu_prover_data* pd) {
     KCHK(merkle_leaves(pd->rows.p, pd->row_len, pd->height, lw, c->d_c16, c->d_c24, pd->tree.p, c->st));
   }
   Span sp(c, ST_TREE, (double)(pd->height - 1));
-  KCHK(merkle_tree(pd->height, c->c16, pd->tree.p, c->st));
+  int rc = merkle_tree(pd->height, c->c16, pd->tree.p, c->st, c->d_root);
+  if (rc < 0) KCHK(rc);
+  pd->root_on_host = rc == 0;
 }
 
-Digest read_root(whirgpu_ctx* c, const Buf<u32>& tree, u64 height) {
+// The root: written to h_root by the tree's last kernel, or (single leaf) copied.
+Digest read_root(whirgpu_ctx* c, const whirgpu_prover_data* pd) {
   Digest d;
-  c->d2h(d.data(), tree.p + (2 * height - 2) * 8, 8);
+  if (pd->root_on_host) {
+    c->sync();
+    memcpy(d.data(), c->h_root, sizeof d);
+  } else {
+    c->d2h(d.data(), pd->tree.p + (2 * pd->height - 2) * 8, 8);
+  }
   return d;
 }
 
@@ -814,10 +840,16 @@ int whirgpu_create(int device, const whirgpu_params* params, const whirgpu_posei
     HIPCHK(hipMalloc(&c->d_partial, c->partial_cap * sizeof(ef)));
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_small), 4096 * sizeof(ef), hipHostMallocCoherent));
     HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->d_small), c->h_small, 0));
-    HIPCHK(hipMalloc(&c->d_idx, 4096 * sizeof(u64)));
+    auto coherent = [&](void** h, void** d, size_t bytes) {
+      HIPCHK(hipHostMalloc(h, bytes, hipHostMallocCoherent));
+      HIPCHK(hipHostGetDevicePointer(d, *h, 0));
+    };
+    coherent(reinterpret_cast<void**>(&c->h_idx), reinterpret_cast<void**>(&c->d_idx), 4096 * sizeof(u64));
     c->open_cap = 1 << 20;
-    HIPCHK(hipMalloc(&c->d_open, c->open_cap * sizeof(u32)));
+    coherent(reinterpret_cast<void**>(&c->h_open), reinterpret_cast<void**>(&c->d_open), c->open_cap * sizeof(u32));
+    coherent(reinterpret_cast<void**>(&c->h_root), reinterpret_cast<void**>(&c->d_root), 64 * sizeof(u32));
     HIPCHK(hipMalloc(&c->d_grind, 256));
+    HIPCHK(hipMalloc(&c->d_gslots, whirgpu_ctx::GRIND_SLOTS * sizeof(u32)));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device));
     // One 256-lane block per CU: one wave on each of its four SIMDs.  More
@@ -827,7 +859,8 @@ int whirgpu_create(int device, const whirgpu_params* params, const whirgpu_posei
     HIPCHK(hipMalloc(&c->d_c16, sizeof(P2C16)));
     HIPCHK(hipMalloc(&c->d_chal, sizeof(DevChallenger)));
     HIPCHK(hipMalloc(&c->d_gbase, sizeof(GrindBase)));
-    HIPCHK(hipMalloc(&c->d_sclog, whirgpu_ctx::SC_LOG_CAP * sizeof(ScLog)));
+    coherent(reinterpret_cast<void**>(&c->h_sclog), reinterpret_cast<void**>(&c->d_sclog),
+             whirgpu_ctx::SC_LOG_CAP * sizeof(ScLog));
     HIPCHK(hipMalloc(&c->d_r, 256));
     HIPCHK(hipMemcpy(c->d_c16, &c->c16, sizeof(P2C16), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&c->d_c24, sizeof(P2C24)));
@@ -849,10 +882,10 @@ int whirgpu_destroy(whirgpu_ctx* c) {
   if (c->device >= 0) (void)hipSetDevice(c->device);
   if (c->st) (void)hipStreamSynchronize(c->st);
   c->pool.release_all();
-  if (c->h_small) (void)hipHostFree(c->h_small);
-  for (void* p : {(void*)c->d_tw, (void*)c->d_tabs, (void*)c->d_partial, (void*)c->d_idx,
-                  (void*)c->d_open, (void*)c->d_grind, (void*)c->d_c16, (void*)c->d_c24, (void*)c->d_chal,
-                  (void*)c->d_gbase, (void*)c->d_sclog, (void*)c->d_r})
+  for (void* h : {(void*)c->h_small, (void*)c->h_idx, (void*)c->h_open, (void*)c->h_root, (void*)c->h_sclog})
+    if (h) (void)hipHostFree(h);
+  for (void* p : {(void*)c->d_tw, (void*)c->d_tabs, (void*)c->d_partial, (void*)c->d_grind, (void*)c->d_gslots,
+                  (void*)c->d_c16, (void*)c->d_c24, (void*)c->d_chal, (void*)c->d_gbase, (void*)c->d_r})
     if (p) (void)hipFree(p);
   if (c->h_pin) (void)hipHostFree(c->h_pin);
   if (c->h_ring) (void)hipHostFree(c->h_ring);
@@ -1050,7 +1083,7 @@ int whirgpu_commit(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs
     }
     commit_tree(c, pd);
     if (tmp.p) c->release(tmp);
-    Digest root = read_root(c, pd->tree, H);
+    Digest root = read_root(c, pd);
     proof->p.has_initial_commitment = true;
     proof->p.initial_commitment = root;
     observe_digest(ch, root);
@@ -1134,7 +1167,7 @@ int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs,
       }
       commit_tree(c, pd);
       if (tmp.p) c->release(tmp);
-      R.commitment = read_root(c, pd->tree, H);
+      R.commitment = read_root(c, pd);
       observe_digest(ch, R.commitment);
       // OOD samples (prover/mod.rs:192-205)
       std::vector<std::vector<ef>> ood_pts;
@@ -1212,6 +1245,13 @@ int whirgpu_profile_enable(whirgpu_ctx* c, int on) {
     c->prof = on != 0;
     c->recs.clear();
     c->next_event = 0;
+    // Create the events up front so that timed proofs record into existing
+    // ones (a proof records a few hundred; the pool grows past this if needed).
+    while (c->prof && c->events.size() < 2048) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      c->events.push_back(e);
+    }
   });
 }
 
