@@ -18,16 +18,28 @@ import time
 
 import numpy as np
 
-# MI355X peaks: HBM3E bandwidth, and 32-bit integer VALU lane operations
-# (256 CUs x 64 lanes x 2.4 GHz; 32-bit multiplies issue at a quarter of that).
+# MI355X peaks: HBM3E bandwidth, and the VALU issue rate: 256 CUs x 4 SIMDs x
+# 16 lanes x 2.4 GHz = 39.3 T lane-instructions/s.  On gfx950 the instructions
+# the hashing kernels are made of (32-bit add/min, v_mul_lo_u32,
+# v_mad_u64_u32, v_mad_i64_i32, v_lshl_add_u64) all issue at that rate
+# (measured with long independent chains of each).
 HBM_PEAK_GBS = 8000.0
-INT32_PEAK_TOPS = 39.3
-# Integer operations of one width-16 Poseidon2 permutation done the textbook
-# way in Montgomery form: 616 field multiplications (S-boxes, internal
-# diagonal) at 7 int32 ops (4 multiplies, subtract, compare, select) and 1416
-# field additions (round constants, linear layers) at 3 ops (add, subtract, min).
-OPS_PER_PERM = 616 * 7 + 1416 * 3
+VALU_PEAK_TOPS = 39.3
+# VALU instructions per width-16 Poseidon2 permutation in the gfx950 build:
+# 5052 in the compiled grinding kernel (one permutation per candidate plus a
+# few instructions of candidate setup), down from 7496 for the textbook form
+# (one modular reduction per operation).  Hashing stages are issue-bound, so
+# permutations x 5052 / time against the issue rate is their roofline.
+VALU_OPS_PER_PERM = 5052
 VALU_STAGES = ("merkle_leaves", "merkle_tree", "grind")
+# Kernels of each stage, for the HBM traffic measured with rocprofv3 --pmc
+# (FETCH_SIZE and WRITE_SIZE per dispatch, calibrated on a copy of known size).
+STAGE_KERNELS = {
+    "grind": ["k_grind"],
+    "merkle_leaves": ["k_leaf_hash<16>"],
+    "lde_base": ["k_lde_pass1<unsigned int, 10>", "k_lde_pass2<unsigned int, 10>"],
+}
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic_n24.json")
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -185,6 +197,10 @@ def main():
     ap.add_argument("--skip-kernel-bench", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--only-kernel-bench", action="store_true",
+                    help="run only the 2^26 eq-expansion + sumcheck workload and print its JSON")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 256 MiB device to device (reference dispatch for rocprofv3 --pmc byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -193,6 +209,18 @@ def main():
     import __graft_entry__ as ge
 
     W = ge._package()
+    if args.copy_calibration:
+        src, dst = W.DeviceBuffer(256 << 20), W.DeviceBuffer(256 << 20)
+        for _ in range(2):
+            W.check(W.lib().whirgpu_memcpy(0, W.ctypes.c_void_p(dst.ptr), W.ctypes.c_void_p(src.ptr), 256 << 20, 2))
+        del src, dst
+    if args.only_kernel_bench:
+        print(json.dumps({"sumcheck_eq_2^26": sumcheck_eq_bench(W, args)}))
+        return
+    traffic = {}
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("kernels", {})
     n = args.n
     prm = W.params(n)
     cfg = W.WhirConfig(prm, device=0)
@@ -225,22 +253,28 @@ def main():
     stages = cfg.profile_read()
     cfg.profile(False)
 
+    def stage_traffic(name):
+        ks = STAGE_KERNELS.get(name)
+        if not ks or any(k not in traffic for k in ks):
+            return None
+        return round(sum(traffic[k]["fetch_bytes"] + traffic[k]["write_bytes"] for k in ks))
+
     def roof(name):
         s = stages[name]
         if not s["launches"] or s["ms"] <= 0:
             return None
         avg_s = s["ms"] / s["launches"] * 1e-3
         if name in VALU_STAGES:
-            per_launch = s["work"] / s["launches"] * OPS_PER_PERM
+            per_launch = s["work"] / s["launches"] * VALU_OPS_PER_PERM
             achieved = per_launch / avg_s / 1e12
-            return {"bound": "valu", "kernel": name, "achieved": round(achieved, 3), "peak": INT32_PEAK_TOPS,
-                    "unit": "TOP/s", "frac": round(achieved / INT32_PEAK_TOPS, 4), "traffic": None,
+            return {"bound": "valu", "kernel": name, "achieved": round(achieved, 3), "peak": VALU_PEAK_TOPS,
+                    "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": stage_traffic(name),
                     "work_per_launch": round(s["work"] / s["launches"]), "work_unit": "Poseidon2-16 permutations",
-                    "avg_launch_ms": round(s["ms"] / s["launches"], 4)}
+                    "ops_per_unit": VALU_OPS_PER_PERM, "avg_launch_ms": round(s["ms"] / s["launches"], 4)}
         per_launch = s["work"] / s["launches"]
         achieved = per_launch / avg_s / 1e9
         return {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": stage_traffic(name),
                 "bytes_per_launch": round(per_launch), "avg_launch_ms": round(s["ms"] / s["launches"], 4)}
 
     dominant = max(stages, key=lambda k: stages[k]["ms"])
