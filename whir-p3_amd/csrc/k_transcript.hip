@@ -85,15 +85,16 @@ __device__ __forceinline__ ef wave_sum(ef v) {
 // phase 1: read the grind result, observe it, check its bits, sample r.
 // r goes to *d_r and the log; log->err is set when the witness is missing or
 // does not check.
-// The challenger is read from `ch`, or from `init` (a kernel argument) when
-// `use_init` is set, and written back to `ch`.
+// The challenger is read from `init` when given (host-visible memory the
+// caller fills before the launch, so a captured graph can be replayed with a
+// new starting state), else from `ch`; it is written back to `ch`.
 __global__ void __launch_bounds__(64) k_sc_transcript(int phase, const ef* __restrict__ partial, int nb,
-                                                      DevChallenger* ch, DevChallenger init, int use_init,
-                                                      GrindBase* gb, u32* d_grind, ScLog* log, int pow_bits, ef* d_r,
+                                                      DevChallenger* ch, const DevChallenger* init, GrindBase* gb,
+                                                      u32* d_grind, ScLog* log, int pow_bits, ef* d_r,
                                                       const P2C16* __restrict__ cp) {
   __shared__ u32 sp[16], in[8], out[8];
   const int lane = threadIdx.x;
-  const DevChallenger& src = use_init ? init : *ch;
+  const DevChallenger& src = init ? *init : *ch;
   if (lane < 16) sp[lane] = src.sponge[lane];
   if (lane < 8) {
     in[lane] = src.in[lane];
@@ -163,10 +164,8 @@ __global__ void __launch_bounds__(64) k_sc_transcript(int phase, const ef* __res
 int sc_transcript(int phase, const ef* partial, unsigned nb, DevChallenger* ch, const DevChallenger* init,
                   GrindBase* gb, u32* d_grind, ScLog* log, int pow_bits, ef* d_r, const P2C16* d_consts,
                   hipStream_t st) {
-  DevChallenger iv{};
-  if (init) iv = *init;
-  hipLaunchKernelGGL(k_sc_transcript, dim3(1), dim3(64), 0, st, phase, partial, (int)nb, ch, iv, init ? 1 : 0, gb,
-                     d_grind, log, pow_bits, d_r, d_consts);
+  hipLaunchKernelGGL(k_sc_transcript, dim3(1), dim3(64), 0, st, phase, partial, (int)nb, ch, init, gb, d_grind, log,
+                     pow_bits, d_r, d_consts);
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 

@@ -89,8 +89,9 @@ int sc_fold(ef* e, ef* w, u64 h, const ef& r, ef* partial, ef* c0c2, hipStream_t
 int sc_partials(const ef* e, const ef* w, u64 h, ef* partial, unsigned* nb, hipStream_t st);
 int sc_fold_dev(ef* e, ef* w, u64 h, const ef* d_r, ef* partial, unsigned* nb, hipStream_t st);
 // ---- device transcript (k_transcript.hip)
-// `init`, when given, is the challenger state to start from (passed by value
-// to the kernel); otherwise the state in `ch` is used.  The result goes to `ch`.
+// `init`, when given, points to the challenger state to start from (memory the
+// device can read, filled before the launch); otherwise the state in `ch` is
+// used.  The result goes to `ch`.
 int sc_transcript(int phase, const ef* partial, unsigned nb, DevChallenger* ch, const DevChallenger* init,
                   GrindBase* gb, u32* d_grind, ScLog* log, int pow_bits, ef* d_r, const P2C16* d_consts,
                   hipStream_t st);

@@ -135,6 +135,8 @@ struct whirgpu_ctx {
   static constexpr int SC_LOG_CAP = 64;
   DevChallenger* d_chal = nullptr;
   GrindBase* d_gbase = nullptr;
+  DevChallenger* h_chal = nullptr;   // coherent host memory: a phase's starting challenger
+  DevChallenger* d_chal_init = nullptr;
   ScLog* h_sclog = nullptr;   // coherent host memory; the transcript kernel logs into it
   ScLog* d_sclog = nullptr;   // device address of h_sclog
   ef* d_r = nullptr;
@@ -470,15 +472,18 @@ std::vector<ef> device_sumcheck(whirgpu_ctx* c, SumcheckState& s, int rounds, in
   for (int i = 0; i < ch.out_len; i++) dc.out[i] = ch.out[i];
   dc.in_len = (u32)ch.in_len;
   dc.out_len = (u32)ch.out_len;
-  // h_sclog is free: every phase ends with a synchronization before its log is read.
+  // h_sclog and h_chal are free: every phase ends with a synchronization
+  // before its log is read.
+  *c->h_chal = dc;
   std::vector<size_t> grind_recs;
+  unsigned nb = 0;
   for (int i = 0; i < rounds; i++) {
     const u64 h = 1ull << (s.m - 1);
-    if (s.partial_nb == 0) {
+    if (i == 0) {
       Span sp(c, ST_SUMCHECK, 64.0 * (double)h);
-      KCHK(sc_partials(s.e.p, s.w.p, h, c->d_partial, &s.partial_nb, c->st));
+      KCHK(sc_partials(s.e.p, s.w.p, h, c->d_partial, &nb, c->st));
     }
-    KCHK(sc_transcript(0, c->d_partial, s.partial_nb, c->d_chal, i == 0 ? &dc : nullptr, c->d_gbase, c->d_grind,
+    KCHK(sc_transcript(0, c->d_partial, nb, c->d_chal, i == 0 ? c->d_chal_init : nullptr, c->d_gbase, c->d_grind,
                        c->d_sclog + i, pow_bits, c->d_r, c->d_c16, c->st));
     if (pow_bits > 0) {
       {
@@ -492,7 +497,7 @@ std::vector<ef> device_sumcheck(whirgpu_ctx* c, SumcheckState& s, int rounds, in
     }
     {
       Span sp(c, ST_SUMCHECK, 96.0 * (double)h);
-      KCHK(sc_fold_dev(s.e.p, s.w.p, h, c->d_r, c->d_partial, &s.partial_nb, c->st));
+      KCHK(sc_fold_dev(s.e.p, s.w.p, h, c->d_r, c->d_partial, &nb, c->st));
     }
     s.m -= 1;
   }
@@ -675,7 +680,7 @@ void commit_tree(whirgpu_ctx* c, whirgpu_prover_data* pd) {
   Span sp(c, ST_TREE, (double)(pd->height - 1));
   int rc = merkle_tree(pd->height, c->c16, pd->tree.p, c->st, c->d_root);
   if (rc < 0) KCHK(rc);
-  pd->root_on_host = rc == 0;
+  pd->root_on_host = rc == 0;   // the tree's last kernel wrote the root to h_root
 }
 
 // The root: written to h_root by the tree's last kernel, or (single leaf) copied.
@@ -861,6 +866,7 @@ int whirgpu_create(int device, const whirgpu_params* params, const whirgpu_posei
     HIPCHK(hipMalloc(&c->d_gbase, sizeof(GrindBase)));
     coherent(reinterpret_cast<void**>(&c->h_sclog), reinterpret_cast<void**>(&c->d_sclog),
              whirgpu_ctx::SC_LOG_CAP * sizeof(ScLog));
+    coherent(reinterpret_cast<void**>(&c->h_chal), reinterpret_cast<void**>(&c->d_chal_init), sizeof(DevChallenger));
     HIPCHK(hipMalloc(&c->d_r, 256));
     HIPCHK(hipMemcpy(c->d_c16, &c->c16, sizeof(P2C16), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&c->d_c24, sizeof(P2C24)));
@@ -882,7 +888,8 @@ int whirgpu_destroy(whirgpu_ctx* c) {
   if (c->device >= 0) (void)hipSetDevice(c->device);
   if (c->st) (void)hipStreamSynchronize(c->st);
   c->pool.release_all();
-  for (void* h : {(void*)c->h_small, (void*)c->h_idx, (void*)c->h_open, (void*)c->h_root, (void*)c->h_sclog})
+  for (void* h : {(void*)c->h_small, (void*)c->h_idx, (void*)c->h_open, (void*)c->h_root, (void*)c->h_sclog,
+                  (void*)c->h_chal})
     if (h) (void)hipHostFree(h);
   for (void* p : {(void*)c->d_tw, (void*)c->d_tabs, (void*)c->d_partial, (void*)c->d_grind, (void*)c->d_gslots,
                   (void*)c->d_c16, (void*)c->d_c24, (void*)c->d_chal, (void*)c->d_gbase, (void*)c->d_r})
