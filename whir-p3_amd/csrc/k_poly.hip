@@ -120,6 +120,108 @@ __global__ void __launch_bounds__(256) k_eq_dot(const T* __restrict__ f, int nf,
   }
 }
 
+// Base-field form of k_eq_dot for lo >= 2: a thread reads four consecutive
+// entries of a row (one 16-byte load) and keeps, per entry and coefficient, an
+// unreduced 64-bit sum of hi[k][r] * f products (each below p^2), reduced once
+// every four rows (a sum of four stays below 4p^2, what redc_wide takes).
+template <int K>
+__global__ void __launch_bounds__(256) k_eq_dot4(const u32* __restrict__ f, int nf, int sb, int lo, EqTables tabs,
+                                                 ef* __restrict__ partial) {
+  __shared__ ef sh[256 * K];
+  const int t = threadIdx.x;
+  const int seg = blockIdx.y;
+  const int nrest = nf - sb;
+  const u64 nrows = 1ull << (nrest - lo);
+  const int lanes = 1 << (lo - 2);   // threads per row
+  const int rows_per_iter = 256 >> (lo - 2);
+  const int xq = t & (lanes - 1);    // entries 4 xq .. 4 xq + 3 of each row
+  const u64 rows_per_block = (nrows + gridDim.x - 1) / gridDim.x;
+  const u64 r0 = (u64)blockIdx.x * rows_per_block;
+  u64 r1 = r0 + rows_per_block;
+  if (r1 > nrows) r1 = nrows;
+  const u32* fs = f + ((u64)seg << nrest) + 4 * xq;
+  ef acc[K][4];
+  u64 lz[K][4][4];
+#pragma unroll
+  for (int k = 0; k < K; k++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      acc[k][i] = ef_zero();
+#pragma unroll
+      for (int c = 0; c < 4; c++) lz[k][i][c] = 0;
+    }
+  auto flush = [&]() {
+#pragma unroll
+    for (int k = 0; k < K; k++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        ef v{{redc_wide(lz[k][i][0]), redc_wide(lz[k][i][1]), redc_wide(lz[k][i][2]), redc_wide(lz[k][i][3])}};
+        // the products carry the Montgomery factor of both operands: one redc
+        // per product, as ef_mul_base does
+        acc[k][i] = ef_add(acc[k][i], v);
+#pragma unroll
+        for (int c = 0; c < 4; c++) lz[k][i][c] = 0;
+      }
+  };
+  // Four rows per step, all loads issued before the arithmetic.
+  u64 r = r0 + (t >> (lo - 2));
+  for (; r + 3ull * rows_per_iter < r1; r += 4ull * rows_per_iter) {
+    uint4 v[4];
+    ef e[K][4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const u64 rj = r + (u64)j * rows_per_iter;
+      v[j] = *reinterpret_cast<const uint4*>(fs + (rj << lo));
+#pragma unroll
+      for (int k = 0; k < K; k++) e[k][j] = tabs.hi[k][rj];
+    }
+    // One wait for all eight loads: the empty asm takes the loaded registers,
+    // so every load is issued before the first is waited for.
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      asm volatile("" : "+v"(v[j].x), "+v"(v[j].y), "+v"(v[j].z), "+v"(v[j].w));
+#pragma unroll
+      for (int k = 0; k < K; k++) asm volatile("" : "+v"(e[k][j].c[0]), "+v"(e[k][j].c[1]), "+v"(e[k][j].c[2]), "+v"(e[k][j].c[3]));
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const u32 vv[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
+#pragma unroll
+      for (int k = 0; k < K; k++)
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int c = 0; c < 4; c++) lz[k][i][c] += (u64)e[k][j].c[c] * vv[i];
+    }
+    flush();
+  }
+  for (; r < r1; r += rows_per_iter) {   // at most three rows left
+    const uint4 v = *reinterpret_cast<const uint4*>(fs + (r << lo));
+    const u32 vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const ef e = tabs.hi[k][r];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) lz[k][i][c] += (u64)e.c[c] * vv[i];
+    }
+  }
+  flush();
+  ef tot[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    tot[k] = ef_zero();
+#pragma unroll
+    for (int i = 0; i < 4; i++) tot[k] = ef_add(tot[k], ef_mul(acc[k][i], tabs.lo[k][4 * xq + i]));
+  }
+  block_reduce_ef<K>(tot, sh);
+  if (t == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) partial[((u64)seg * gridDim.x + blockIdx.x) * K + k] = tot[k];
+  }
+}
+
 // out[seg * K + k] = sum_b partial[(seg * nb + b) * K + k]; one block per (seg, k).
 __global__ void __launch_bounds__(256) k_reduce_strided(const ef* __restrict__ partial, int nb, int K,
                                                         ef* __restrict__ out) {
@@ -135,6 +237,13 @@ template <typename T>
 static int eq_dot_impl(const T* f, int nf, int sb, int lo, const EqTables& tabs, int K, ef* partial, int nblocks,
                        hipStream_t st) {
   dim3 g(nblocks, 1u << sb);
+  if constexpr (sizeof(T) == 4) {
+    if (lo >= 2 && K <= 2) {
+      if (K == 1) hipLaunchKernelGGL((k_eq_dot4<1>), g, dim3(256), 0, st, f, nf, sb, lo, tabs, partial);
+      else hipLaunchKernelGGL((k_eq_dot4<2>), g, dim3(256), 0, st, f, nf, sb, lo, tabs, partial);
+      return hipGetLastError() == hipSuccess ? 0 : -5;
+    }
+  }
   switch (K) {
     case 1: hipLaunchKernelGGL((k_eq_dot<T, 1>), g, dim3(256), 0, st, f, nf, sb, lo, tabs, partial); break;
     case 2: hipLaunchKernelGGL((k_eq_dot<T, 2>), g, dim3(256), 0, st, f, nf, sb, lo, tabs, partial); break;

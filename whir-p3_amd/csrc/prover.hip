@@ -488,7 +488,12 @@ std::vector<ef> device_sumcheck(whirgpu_ctx* c, SumcheckState& s, int rounds, in
     if (pow_bits > 0) {
       {
         Span sp(c, ST_GRIND, 0);
-        KCHK(grind_range_dev(c->d_c16, c->d_gbase, pow_bits, c->d_grind, 0, P, c->grind_blocks, c->st));
+        // The range is all of [0, p), so the search always ends; the grid
+        // is sized to the expected search (16 * 2^bits candidates), one pass
+        // of it at most, so short grinds leave the device to other work.
+        const u64 want = ((16ull << pow_bits) + 255) / 256;
+        const int gblocks = (int)std::max<u64>(1, std::min<u64>((u64)c->grind_blocks, want));
+        KCHK(grind_range_dev(c->d_c16, c->d_gbase, pow_bits, c->d_grind, 0, P, gblocks, c->st));
         sp.close();
         if (c->prof) grind_recs.push_back(c->recs.size() - 1);
       }
