@@ -15,6 +15,7 @@
 // one launch with one compression per lane.
 #include "kcommon.h"
 #include "kernels.h"
+#include "transcript.h"
 #include "poseidon2.h"
 #include "poseidon2_wide.h"
 
@@ -268,13 +269,16 @@ __device__ __forceinline__ void phase_fence(u32* s) {
                : "memory");
 }
 
-__global__ void __launch_bounds__(256) k_grind(const P2C16* __restrict__ cp, Sponge16 base, int wpos,
-                                               const GrindBase* __restrict__ gb, u32 mask, u32* result, u32 start,
-                                               u32 end) {
-  if (gb) {
+// With tr_mode == TR_COEFFS_GRIND every block first runs the transcript step
+// that observes the round's (h(0), h(2)); its state is the candidates' base.
+__global__ void __launch_bounds__(256) k_grind(const P2C16* __restrict__ cp, Sponge16 base, int wpos, int tr_mode,
+                                               ScTr tr, u32 mask, u32* result, u32 start, u32 end) {
+  if (tr_mode == TR_COEFFS_GRIND) {
+    __shared__ GrindBase gbs;
+    tr_prologue(TR_COEFFS_GRIND, tr, *cp, &gbs, nullptr);
 #pragma unroll
-    for (int k = 0; k < 16; k++) base.s[k] = gb->s[k];
-    wpos = (int)gb->wpos;
+    for (int k = 0; k < 16; k++) base.s[k] = gbs.s[k];
+    wpos = (int)gbs.wpos;
   }
   const u32 stride = gridDim.x * 256;
   const u32 lane = blockIdx.x * 256 + threadIdx.x;
@@ -318,21 +322,20 @@ int grind_range(const P2C16* d_consts, const u32* state16, int wpos, int bits, u
   u64 end = std::min<u64>(start + count, P);
   u64 need = (end - start + 255) / 256;
   unsigned nb = (unsigned)std::min<u64>(need, (u64)std::max(blocks, 1));
-  hipLaunchKernelGGL(k_grind, dim3(nb), dim3(256), 0, st, d_consts, b, wpos, (const GrindBase*)nullptr, mask, d_result,
+  hipLaunchKernelGGL(k_grind, dim3(nb), dim3(256), 0, st, d_consts, b, wpos, (int)TR_NONE, ScTr{}, mask, d_result,
                      (u32)start, (u32)end);
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 
-int grind_range_dev(const P2C16* d_consts, const GrindBase* d_base, int bits, u32* d_result, u64 start, u64 count,
-                    int blocks, hipStream_t st) {
+int grind_range_tr(const P2C16* d_consts, const ScTr& tr, int bits, u64 start, u64 count, int blocks, hipStream_t st) {
   Sponge16 b{};
   u32 mask = bits >= 32 ? 0xffffffffu : ((1u << bits) - 1);
   if (start >= P) return 0;
   u64 end = std::min<u64>(start + count, P);
   u64 need = (end - start + 255) / 256;
   unsigned nb = (unsigned)std::min<u64>(need, (u64)std::max(blocks, 1));
-  hipLaunchKernelGGL(k_grind, dim3(nb), dim3(256), 0, st, d_consts, b, 0, d_base, mask, d_result, (u32)start,
-                     (u32)end);
+  hipLaunchKernelGGL(k_grind, dim3(nb), dim3(256), 0, st, d_consts, b, 0, (int)TR_COEFFS_GRIND, tr, mask, tr.grind,
+                     (u32)start, (u32)end);
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 

@@ -14,6 +14,7 @@
 // small and built on the device from the point, which arrives by value.
 #include "kcommon.h"
 #include "kernels.h"
+#include "transcript.h"
 
 namespace kb {
 
@@ -371,8 +372,9 @@ int base_to_ext(const u32* f, u64 n, ef* out, hipStream_t st) {
 // ---------------------------------------------------------------- sumcheck rounds
 // (h(0), h(2)) partial sums over pairs (i, i + h) of e and w (length 2h).
 __global__ void __launch_bounds__(256) k_sc_coeffs(const ef* __restrict__ e, const ef* __restrict__ w, u64 h,
-                                                   ef* __restrict__ partial) {
+                                                   ef* __restrict__ partial, u32* grind_reset) {
   __shared__ ef sh[512];
+  if (grind_reset && blockIdx.x == 0 && threadIdx.x == 0) *grind_reset = P;
   ef acc[2] = {ef_zero(), ef_zero()};
   for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < h; i += (u64)gridDim.x * blockDim.x) {
     ef e0 = e[i], e1 = e[i + h], w0 = w[i], w1 = w[i + h];
@@ -388,10 +390,16 @@ __global__ void __launch_bounds__(256) k_sc_coeffs(const ef* __restrict__ e, con
 
 // Fold e and w (length 2h) with r into their first halves, and accumulate the
 // next round's (h(0), h(2)) over the folded halves (pairs (j, j + h/2)).
+// With a transcript mode, every block first runs that transcript step to get r.
 __global__ void __launch_bounds__(256) k_sc_fold_coeffs(ef* __restrict__ e, ef* __restrict__ w, u64 h, ef r,
-                                                        const ef* __restrict__ rp, ef* __restrict__ partial) {
+                                                        int tr_mode, ScTr tr, const P2C16* __restrict__ cp,
+                                                        ef* __restrict__ partial) {
   __shared__ ef sh[512];
-  if (rp) r = *rp;
+  if (tr_mode != TR_NONE) {
+    __shared__ ef rs;
+    tr_prologue(tr_mode, tr, *cp, nullptr, &rs);
+    r = rs;
+  }
   ef acc[2] = {ef_zero(), ef_zero()};
   const u64 q = h >> 1;
   for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < q; j += (u64)gridDim.x * blockDim.x) {
@@ -415,8 +423,13 @@ __global__ void __launch_bounds__(256) k_sc_fold_coeffs(ef* __restrict__ e, ef* 
   }
 }
 
-__global__ void k_sc_fold(ef* __restrict__ e, ef* __restrict__ w, u64 h, ef r, const ef* __restrict__ rp) {
-  if (rp) r = *rp;
+__global__ void __launch_bounds__(256) k_sc_fold(ef* __restrict__ e, ef* __restrict__ w, u64 h, ef r, int tr_mode,
+                                                 ScTr tr, const P2C16* __restrict__ cp) {
+  if (tr_mode != TR_NONE) {
+    __shared__ ef rs;
+    tr_prologue(tr_mode, tr, *cp, nullptr, &rs);
+    r = rs;
+  }
   for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < h; j += (u64)gridDim.x * blockDim.x) {
     ef a0 = e[j], a1 = e[j + h];
     e[j] = ef_add(a0, ef_mul(r, ef_sub(a1, a0)));
@@ -434,7 +447,7 @@ static unsigned sc_blocks(u64 work) {
 // c0c2[0..2] = (h(0), h(2)) for arrays of length 2h.  Uses partial[2 * 1024].
 int sc_coeffs(const ef* e, const ef* w, u64 h, ef* partial, ef* c0c2, hipStream_t st) {
   unsigned nb = sc_blocks(h);
-  hipLaunchKernelGGL(k_sc_coeffs, dim3(nb), dim3(256), 0, st, e, w, h, partial);
+  hipLaunchKernelGGL(k_sc_coeffs, dim3(nb), dim3(256), 0, st, e, w, h, partial, (u32*)nullptr);
   hipLaunchKernelGGL(k_reduce_strided, dim3(2), dim3(256), 0, st, partial, (int)nb, 2, c0c2);
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
@@ -443,27 +456,31 @@ int sc_coeffs(const ef* e, const ef* w, u64 h, ef* partial, ef* c0c2, hipStream_
 int sc_fold(ef* e, ef* w, u64 h, const ef& r, ef* partial, ef* c0c2, hipStream_t st) {
   if (h >= 2) {
     unsigned nb = sc_blocks(h >> 1);
-    hipLaunchKernelGGL(k_sc_fold_coeffs, dim3(nb), dim3(256), 0, st, e, w, h, r, (const ef*)nullptr, partial);
+    hipLaunchKernelGGL(k_sc_fold_coeffs, dim3(nb), dim3(256), 0, st, e, w, h, r, (int)TR_NONE, ScTr{},
+                       (const P2C16*)nullptr, partial);
     hipLaunchKernelGGL(k_reduce_strided, dim3(2), dim3(256), 0, st, partial, (int)nb, 2, c0c2);
   } else {
-    hipLaunchKernelGGL(k_sc_fold, dim3(sc_blocks(h)), dim3(256), 0, st, e, w, h, r, (const ef*)nullptr);
+    hipLaunchKernelGGL(k_sc_fold, dim3(sc_blocks(h)), dim3(256), 0, st, e, w, h, r, (int)TR_NONE, ScTr{},
+                       (const P2C16*)nullptr);
   }
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 
-int sc_partials(const ef* e, const ef* w, u64 h, ef* partial, unsigned* nb, hipStream_t st) {
+int sc_partials(const ef* e, const ef* w, u64 h, ef* partial, unsigned* nb, u32* grind_reset, hipStream_t st) {
   *nb = sc_blocks(h);
-  hipLaunchKernelGGL(k_sc_coeffs, dim3(*nb), dim3(256), 0, st, e, w, h, partial);
+  hipLaunchKernelGGL(k_sc_coeffs, dim3(*nb), dim3(256), 0, st, e, w, h, partial, grind_reset);
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 
-int sc_fold_dev(ef* e, ef* w, u64 h, const ef* d_r, ef* partial, unsigned* nb, hipStream_t st) {
+int sc_fold_tr(ef* e, ef* w, u64 h, int mode, const ScTr& tr, const P2C16* d_consts, ef* partial_out, unsigned* nb,
+               hipStream_t st) {
   if (h >= 2) {
     *nb = sc_blocks(h >> 1);
-    hipLaunchKernelGGL(k_sc_fold_coeffs, dim3(*nb), dim3(256), 0, st, e, w, h, ef_zero(), d_r, partial);
+    hipLaunchKernelGGL(k_sc_fold_coeffs, dim3(*nb), dim3(256), 0, st, e, w, h, ef_zero(), mode, tr, d_consts,
+                       partial_out);
   } else {
     *nb = 0;
-    hipLaunchKernelGGL(k_sc_fold, dim3(sc_blocks(h)), dim3(256), 0, st, e, w, h, ef_zero(), d_r);
+    hipLaunchKernelGGL(k_sc_fold, dim3(sc_blocks(h)), dim3(256), 0, st, e, w, h, ef_zero(), mode, tr, d_consts);
   }
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }

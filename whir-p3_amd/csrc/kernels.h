@@ -38,6 +38,27 @@ struct ScLog {
   u32 witness, err, pad[2];
 };
 
+// One device transcript step of a sumcheck round, run as a prologue by the
+// kernel that needs its result (transcript.h).
+enum ScTrMode {
+  TR_NONE = 0,
+  TR_COEFFS_GRIND = 1,    // observe (h(0), h(2)); the grind state for the candidates
+  TR_WITNESS_SAMPLE = 2,  // observe the witness, check its bits, sample r
+  TR_COEFFS_SAMPLE = 3,   // observe (h(0), h(2)) and sample r (no proof of work)
+};
+
+// Everything a transcript prologue reads and writes (a kernel argument).
+struct ScTr {
+  const DevChallenger* ch_in;   // state to start from (device or host-visible memory)
+  DevChallenger* ch_out;        // state after this step (block 0 writes it)
+  ScLog* log;                   // this round's entry
+  const ef* partial;            // (h(0), h(2)) partial sums of `nb` blocks
+  u32* grind;                   // this round's grind result
+  u32* grind_next;              // next round's grind result (block 0 resets it to p)
+  int nb;
+  int pow_bits;
+};
+
 struct EqTables {
   const ef* hi[4];
   const ef* lo[4];
@@ -64,9 +85,9 @@ int merkle_open(const u32* rows, int row_len, const u32* tree, u64 height, const
 // hold p before the first range; it receives the smallest witness found.
 int grind_range(const P2C16* d_consts, const u32* state16, int wpos, int bits, u32* d_result, u64 start, u64 count,
                 int blocks, hipStream_t st);
-// Same, with the base state and candidate position read from device memory.
-int grind_range_dev(const P2C16* d_consts, const GrindBase* d_base, int bits, u32* d_result, u64 start, u64 count,
-                    int blocks, hipStream_t st);
+// Same, with the base state and candidate position from the transcript step
+// TR_COEFFS_GRIND run by every block first; the result goes to tr.grind.
+int grind_range_tr(const P2C16* d_consts, const ScTr& tr, int bits, u64 start, u64 count, int blocks, hipStream_t st);
 
 // ---- multilinear kernels (k_poly.hip)
 int eq_table(const PointArg& z, int first, int t, const ef& scale, ef* out, hipStream_t st);
@@ -83,17 +104,14 @@ int compress_multi(const u32* f, int n, int l, const ef* d_eqrs, ef* out, hipStr
 int base_to_ext(const u32* f, u64 n, ef* out, hipStream_t st);
 int sc_coeffs(const ef* e, const ef* w, u64 h, ef* partial, ef* c0c2, hipStream_t st);
 int sc_fold(ef* e, ef* w, u64 h, const ef& r, ef* partial, ef* c0c2, hipStream_t st);
-// Device-transcript variants: coefficient partial sums only (*nb blocks, two
-// values each in `partial`), and a fold by the challenge at *d_r that also
-// leaves the next round's partials when h >= 2 (*nb = 0 otherwise).
-int sc_partials(const ef* e, const ef* w, u64 h, ef* partial, unsigned* nb, hipStream_t st);
-int sc_fold_dev(ef* e, ef* w, u64 h, const ef* d_r, ef* partial, unsigned* nb, hipStream_t st);
-// ---- device transcript (k_transcript.hip)
-// `init`, when given, points to the challenger state to start from (memory the
-// device can read, filled before the launch); otherwise the state in `ch` is
-// used.  The result goes to `ch`.
-int sc_transcript(int phase, const ef* partial, unsigned nb, DevChallenger* ch, const DevChallenger* init,
-                  GrindBase* gb, u32* d_grind, ScLog* log, int pow_bits, ef* d_r, const P2C16* d_consts,
-                  hipStream_t st);
+// Device-transcript variants.  sc_partials: coefficient partial sums only
+// (*nb blocks, two values each in `partial`); it also sets *grind_reset to p
+// (the first round's grind result) when given.  sc_fold_tr: the transcript
+// step `mode` (TR_WITNESS_SAMPLE or TR_COEFFS_SAMPLE) gives r, then the fold by
+// r, leaving the next round's partials in `partial_out` when h >= 2 (*nb = 0
+// otherwise).
+int sc_partials(const ef* e, const ef* w, u64 h, ef* partial, unsigned* nb, u32* grind_reset, hipStream_t st);
+int sc_fold_tr(ef* e, ef* w, u64 h, int mode, const ScTr& tr, const P2C16* d_consts, ef* partial_out, unsigned* nb,
+               hipStream_t st);
 
 }  // namespace kb

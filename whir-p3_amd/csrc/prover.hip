@@ -133,13 +133,11 @@ struct whirgpu_ctx {
   P2C24* d_c24 = nullptr;     // permutations read them from memory)
   // device transcript of sumcheck rounds: challenger, grind base, round log, challenge
   static constexpr int SC_LOG_CAP = 64;
-  DevChallenger* d_chal = nullptr;
-  GrindBase* d_gbase = nullptr;
+  DevChallenger* d_chal = nullptr;   // two states: a launch reads one and writes the other
   DevChallenger* h_chal = nullptr;   // coherent host memory: a phase's starting challenger
   DevChallenger* d_chal_init = nullptr;
   ScLog* h_sclog = nullptr;   // coherent host memory; the transcript kernel logs into it
   ScLog* d_sclog = nullptr;   // device address of h_sclog
-  ef* d_r = nullptr;
   int grind_blocks = 256;
   double timings[7] = {0};
   // live kernel timing: HIP events recorded on `st` around each launch
@@ -476,34 +474,50 @@ std::vector<ef> device_sumcheck(whirgpu_ctx* c, SumcheckState& s, int rounds, in
   // before its log is read.
   *c->h_chal = dc;
   std::vector<size_t> grind_recs;
+  // Each launch reads the challenger from `cin` and writes the next state to
+  // the other buffer; coefficient partials and grind results alternate too,
+  // since every block of a launch reads what block 0 of the same launch
+  // would otherwise overwrite.
+  DevChallenger* chal[2] = {c->d_chal, c->d_chal + 1};
+  ef* part[2] = {c->d_partial, c->d_partial + 4096};
+  u32* gres[2] = {c->d_grind, c->d_grind + 64};
+  const DevChallenger* cin = c->d_chal_init;
+  auto next_state = [&]() { return cin == chal[0] ? chal[1] : chal[0]; };
+  int pp = 0;
   unsigned nb = 0;
+  {
+    const u64 h = 1ull << (s.m - 1);
+    Span sp(c, ST_SUMCHECK, 64.0 * (double)h);
+    KCHK(sc_partials(s.e.p, s.w.p, h, part[0], &nb, gres[0], c->st));
+  }
   for (int i = 0; i < rounds; i++) {
     const u64 h = 1ull << (s.m - 1);
-    if (i == 0) {
-      Span sp(c, ST_SUMCHECK, 64.0 * (double)h);
-      KCHK(sc_partials(s.e.p, s.w.p, h, c->d_partial, &nb, c->st));
-    }
-    KCHK(sc_transcript(0, c->d_partial, nb, c->d_chal, i == 0 ? c->d_chal_init : nullptr, c->d_gbase, c->d_grind,
-                       c->d_sclog + i, pow_bits, c->d_r, c->d_c16, c->st));
+    u32* g = gres[i & 1];
+    u32* g_next = gres[(i + 1) & 1];
+    int fold_mode = TR_COEFFS_SAMPLE;
     if (pow_bits > 0) {
-      {
-        Span sp(c, ST_GRIND, 0);
-        // The range is all of [0, p), so the search always ends; the grid
-        // is sized to the expected search (16 * 2^bits candidates), one pass
-        // of it at most, so short grinds leave the device to other work.
-        const u64 want = ((16ull << pow_bits) + 255) / 256;
-        const int gblocks = (int)std::max<u64>(1, std::min<u64>((u64)c->grind_blocks, want));
-        KCHK(grind_range_dev(c->d_c16, c->d_gbase, pow_bits, c->d_grind, 0, P, gblocks, c->st));
-        sp.close();
-        if (c->prof) grind_recs.push_back(c->recs.size() - 1);
-      }
-      KCHK(sc_transcript(1, c->d_partial, 0, c->d_chal, nullptr, c->d_gbase, c->d_grind, c->d_sclog + i, pow_bits,
-                         c->d_r, c->d_c16, c->st));
+      DevChallenger* cout = next_state();
+      ScTr tr{cin, cout, c->d_sclog + i, part[pp], g, nullptr, (int)nb, pow_bits};
+      // The range is all of [0, p), so the search always ends; the grid is
+      // sized to the expected search (16 * 2^bits candidates), one pass of
+      // it at most, so short grinds leave the device to other work.
+      const u64 want = ((16ull << pow_bits) + 255) / 256;
+      const int gblocks = (int)std::max<u64>(1, std::min<u64>((u64)c->grind_blocks, want));
+      Span sp(c, ST_GRIND, 0);
+      KCHK(grind_range_tr(c->d_c16, tr, pow_bits, 0, P, gblocks, c->st));
+      sp.close();
+      if (c->prof) grind_recs.push_back(c->recs.size() - 1);
+      cin = cout;
+      fold_mode = TR_WITNESS_SAMPLE;
     }
+    DevChallenger* cout = next_state();
+    ScTr tr{cin, cout, c->d_sclog + i, part[pp], g, g_next, (int)nb, pow_bits};
     {
       Span sp(c, ST_SUMCHECK, 96.0 * (double)h);
-      KCHK(sc_fold_dev(s.e.p, s.w.p, h, c->d_r, c->d_partial, &nb, c->st));
+      KCHK(sc_fold_tr(s.e.p, s.w.p, h, fold_mode, tr, c->d_c16, part[pp ^ 1], &nb, c->st));
     }
+    cin = cout;
+    pp ^= 1;
     s.m -= 1;
   }
   // d_partial is scratch for other kernels between phases
@@ -858,7 +872,7 @@ int whirgpu_create(int device, const whirgpu_params* params, const whirgpu_posei
     c->open_cap = 1 << 20;
     coherent(reinterpret_cast<void**>(&c->h_open), reinterpret_cast<void**>(&c->d_open), c->open_cap * sizeof(u32));
     coherent(reinterpret_cast<void**>(&c->h_root), reinterpret_cast<void**>(&c->d_root), 64 * sizeof(u32));
-    HIPCHK(hipMalloc(&c->d_grind, 256));
+    HIPCHK(hipMalloc(&c->d_grind, 2 * 64 * sizeof(u32)));   // two results, 256 bytes apart
     HIPCHK(hipMalloc(&c->d_gslots, whirgpu_ctx::GRIND_SLOTS * sizeof(u32)));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device));
@@ -867,12 +881,10 @@ int whirgpu_create(int device, const whirgpu_params* params, const whirgpu_posei
     // lengthen each pass over the candidates.
     c->grind_blocks = std::max(1, prop.multiProcessorCount);
     HIPCHK(hipMalloc(&c->d_c16, sizeof(P2C16)));
-    HIPCHK(hipMalloc(&c->d_chal, sizeof(DevChallenger)));
-    HIPCHK(hipMalloc(&c->d_gbase, sizeof(GrindBase)));
+    HIPCHK(hipMalloc(&c->d_chal, 2 * sizeof(DevChallenger)));
     coherent(reinterpret_cast<void**>(&c->h_sclog), reinterpret_cast<void**>(&c->d_sclog),
              whirgpu_ctx::SC_LOG_CAP * sizeof(ScLog));
     coherent(reinterpret_cast<void**>(&c->h_chal), reinterpret_cast<void**>(&c->d_chal_init), sizeof(DevChallenger));
-    HIPCHK(hipMalloc(&c->d_r, 256));
     HIPCHK(hipMemcpy(c->d_c16, &c->c16, sizeof(P2C16), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&c->d_c24, sizeof(P2C24)));
     HIPCHK(hipMemcpy(c->d_c24, &c->c24, sizeof(P2C24), hipMemcpyHostToDevice));
@@ -897,7 +909,7 @@ int whirgpu_destroy(whirgpu_ctx* c) {
                   (void*)c->h_chal})
     if (h) (void)hipHostFree(h);
   for (void* p : {(void*)c->d_tw, (void*)c->d_tabs, (void*)c->d_partial, (void*)c->d_grind, (void*)c->d_gslots,
-                  (void*)c->d_c16, (void*)c->d_c24, (void*)c->d_chal, (void*)c->d_gbase, (void*)c->d_r})
+                  (void*)c->d_c16, (void*)c->d_c24, (void*)c->d_chal})
     if (p) (void)hipFree(p);
   if (c->h_pin) (void)hipHostFree(c->h_pin);
   if (c->h_ring) (void)hipHostFree(c->h_ring);

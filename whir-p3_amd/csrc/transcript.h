@@ -1,0 +1,153 @@
+// Fiat-Shamir transcript of sumcheck rounds on the device, as a prologue of
+// the kernels that need its result.
+//
+// A sumcheck round of the prover (SumcheckData::observe_and_sample,
+// src/whir/proof.rs:163-193) observes the round polynomial's (h(0), h(2)),
+// grinds when the round asks for proof of work, and samples the folding
+// challenge r.  Instead of two host round trips per round, every block of the
+// kernel that needs a transcript value computes it itself from the challenger
+// state in memory: the grinding kernel observes (h(0), h(2)) to get the state
+// its candidates are observed into, and the fold kernel observes the witness
+// and samples r.  All blocks compute the same values; block 0 alone writes the
+// new challenger state (to a different buffer than the one read), the round's
+// log entry (h(0), h(2), witness, r) and the reset of the next round's grind
+// result.  The host replays the logged rounds on its own challenger and checks
+// every challenge.
+//
+// The challenger is DuplexChallenger<F, Poseidon2-16, 16, 8> of p3-challenger
+// (used at src/bin/main.rs:104-106): observe clears the output buffer and
+// appends to the input buffer, duplexing at 8 inputs; sample duplexes when
+// the input buffer is non-empty or the output buffer is empty, then pops the
+// last output; a duplex overwrites the first inputs of the sponge state with
+// the buffered inputs, permutes, and refills the output buffer with the first
+// 8 state elements.
+#pragma once
+#include "kcommon.h"
+#include "kernels.h"
+#include "poseidon2_wide.h"
+
+namespace kb {
+
+// The challenger held in LDS by a whole block: every thread runs the same
+// bookkeeping, and every wave computes the lane-parallel permutation (lane
+// j & 15 holds element j & 15), so block barriers order the LDS updates.
+struct BlockChallenger {
+  u32* sp;    // [16] LDS
+  u32* in;    // [8]  LDS
+  u32* out;   // [8]  LDS
+  int in_len;
+  int out_len;
+  const WideLane* wl;
+  const P2C16* c;
+
+  __device__ void load(const DevChallenger& s) {
+    const int t = threadIdx.x;
+    if (t < 16) sp[t] = s.sponge[t];
+    if (t < 8) {
+      in[t] = s.in[t];
+      out[t] = s.out[t];
+    }
+    in_len = (int)s.in_len;
+    out_len = (int)s.out_len;
+    __syncthreads();
+  }
+  __device__ void store(DevChallenger* s) const {   // block 0 only
+    const int t = threadIdx.x;
+    if (t < 16) s->sponge[t] = sp[t];
+    if (t < 8) {
+      s->in[t] = in[t];
+      s->out[t] = out[t];
+    }
+    if (t == 0) {
+      s->in_len = (u32)in_len;
+      s->out_len = (u32)out_len;
+    }
+  }
+  __device__ void duplex() {
+    const int t = threadIdx.x;
+    if (t < in_len) sp[t] = in[t];
+    __syncthreads();
+    u32 x = sp[t & 15];
+    x = permute16_wide(x, t & 15, *wl, *c);
+    __syncthreads();
+    if (t < 16) sp[t] = x;
+    if (t < 8) out[t] = x;
+    __syncthreads();
+    in_len = 0;
+    out_len = 8;
+  }
+  __device__ void observe(u32 v) {
+    out_len = 0;
+    if (threadIdx.x == 0) in[in_len] = v;
+    in_len++;
+    __syncthreads();
+    if (in_len == 8) duplex();
+  }
+  __device__ u32 sample() {
+    if (in_len > 0 || out_len == 0) duplex();
+    out_len--;
+    return out[out_len];
+  }
+};
+
+// Runs one transcript step with the whole block.  TR_COEFFS_GRIND leaves the
+// candidates' base state in gb (LDS); the sampling modes leave r in *r (LDS).
+// Requires blockDim.x == 256.
+__device__ __forceinline__ void tr_prologue(int mode, const ScTr& a, const P2C16& c, GrindBase* gb, ef* r) {
+  __shared__ u32 sp[16], in[8], out[8];
+  __shared__ ef red[2 * 256];
+  const int t = threadIdx.x;
+  const WideLane wl = wide_lane(c, t & 15);
+  BlockChallenger ch{sp, in, out, 0, 0, &wl, &c};
+  ch.load(*a.ch_in);
+  const bool lead = blockIdx.x == 0 && blockIdx.y == 0;
+  u32 err = 0;
+  if (mode == TR_COEFFS_GRIND || mode == TR_COEFFS_SAMPLE) {
+    ef acc[2] = {ef_zero(), ef_zero()};
+    for (int b = t; b < a.nb; b += 256) {
+      acc[0] = ef_add(acc[0], a.partial[2 * b]);
+      acc[1] = ef_add(acc[1], a.partial[2 * b + 1]);
+    }
+    block_reduce_ef<2>(acc, red);
+    if (lead && t == 0) {
+      a.log->c0 = acc[0];
+      a.log->c2 = acc[1];
+      a.log->witness = 0;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) ch.observe(acc[0].c[k]);
+#pragma unroll
+    for (int k = 0; k < 4; k++) ch.observe(acc[1].c[k]);
+    if (mode == TR_COEFFS_GRIND) {
+      // a candidate w is observed into position in_len of this state
+      if (t < 16) gb->s[t] = t < ch.in_len ? in[t] : sp[t];
+      if (t == 0) gb->wpos = (u32)ch.in_len;
+      __syncthreads();
+      if (lead) ch.store(a.ch_out);
+      return;
+    }
+  } else {
+    const u32 w = *a.grind;
+    const u32 wm = w < P ? to_monty(w) : 0u;
+    if (w >= P) err = 1;
+    ch.observe(wm);
+    const u32 mask = a.pow_bits >= 32 ? 0xffffffffu : ((1u << a.pow_bits) - 1);
+    if ((from_monty(ch.sample()) & mask) != 0) err = 1;
+    if (lead && t == 0) a.log->witness = wm;
+  }
+  ef rr;
+#pragma unroll
+  for (int k = 0; k < 4; k++) rr.c[k] = ch.sample();
+  if (t == 0) *r = rr;
+  if (lead) {
+    ch.store(a.ch_out);
+    if (t == 0) {
+      a.log->r = rr;
+      a.log->err = err;
+    }
+    if (a.grind_next && t == 0) *a.grind_next = P;
+  }
+  __syncthreads();
+}
+
+}  // namespace kb
