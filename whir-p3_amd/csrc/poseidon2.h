@@ -187,11 +187,18 @@ __host__ __device__ __forceinline__ u64 sub_div2k(u64 sumk, u32 z) {
   return mad_u64(z & ((1u << K) - 1), CK, sumk + (P - (z >> K)));   // -q = p - q (mod p)
 }
 
-// x^3 / 2^64 mod p in [0, 2p) as a 64-bit value with a zero high word, for
-// x < p: the last Montgomery product skips its conditional subtraction
-// (y / 2^32 < 2p there).
+// Montgomery products without the final conditional subtraction.  For
+// x < p, x^2 / 2^32 mod p comes out below p^2 / 2^32 + p < 1.5p, and then
+// x^3 / 2^64 mod p below 1.5p x / 2^32 + p < 1.75p; neither 64-bit sum
+// overflows (1.5p^2 + 2^32 p < 2^64).  cube_lazy returns a 64-bit value with a
+// zero high word.
+__host__ __device__ __forceinline__ u32 sqr_lazy(u32 x) {
+  const u64 m = (u64)x * x;
+  const u32 t = (u32)m * MU_NEG;
+  return (u32)(((u64)t * P + m) >> 32);
+}
 __host__ __device__ __forceinline__ u64 cube_lazy(u32 x) {
-  const u32 x2 = mul(x, x);
+  const u32 x2 = sqr_lazy(x);
   const u64 m = (u64)x2 * x;
   const u32 t = (u32)m * MU_NEG;
   return ((u64)t * P + m) >> 32;
@@ -249,12 +256,12 @@ __host__ __device__ __forceinline__ void permute16_internal(u32* s, const P2C16&
   u64 y0 = s[0];
 #pragma unroll
   for (int r = 0; r < P2C16::RP; r++) {
-    const u32 x0 = cube(full_reduce(y0 + c.internal[r]));   // y0 + rc < 36p
+    const u32 x0 = (u32)cube_lazy(full_reduce(y0 + c.internal[r]));   // y0 + rc < 37p; x0 < 1.75p
     u64 sum = x0;
 #pragma unroll
-    for (int i = 1; i < 16; i++) sum += z[i];   // < 31p
+    for (int i = 1; i < 16; i++) sum += z[i];   // < 32p
     const u64 sumk = sum + TWO_P;
-    y0 = mad_i64((int32_t)x0, -2, sum + 2ull * TWO_P);   // sum - 2 x0, < 35p
+    y0 = mad_u64(TWO_P - x0, 2u, sum);   // sum - 2 x0 + 4p, < 36p
     z[1] = lazy_reduce(sum + z[1]);
     z[2] = lazy_reduce(sum + 2 * z[2]);
     z[3] = lazy_reduce(add_div2k<1>(sumk, (u32)z[3]));

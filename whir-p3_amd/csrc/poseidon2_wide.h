@@ -1,5 +1,5 @@
 // Lane-parallel Poseidon2-16: one permutation spread over a 16-lane DPP row,
-// lane j holding state element j.  It trades ~2.4x more lane-instructions for
+// lane j holding state element j.  It trades ~3x more lane-instructions for
 // ~5x lower latency than the one-lane form in poseidon2.h, which is what the
 // latency-bound parts need (upper Merkle levels, a handful of candidates).
 //
@@ -8,6 +8,13 @@
 // then adds the sum of the four quads' same-position elements (row rotations
 // by 4, 8, 12); the internal layer adds the 16-lane sum (rotations by 8, 4, 2,
 // 1) to diag_j * s_j.  Every lane of the wave must be active.
+//
+// Arithmetic: every lane holds a value below p between layers.  S-boxes use
+// the lazy Montgomery products of poseidon2.h and one conditional
+// subtraction; the 16-lane and cross-quad sums are 32-bit rotated adds (DPP
+// operands fused into the add), each below 2p and brought below p; the M4
+// step adds in 64 bits and is reduced once, together with the next round
+// constant.
 #pragma once
 #include "poseidon2.h"
 
@@ -24,23 +31,47 @@ constexpr int QROT3 = 3 | (0 << 2) | (1 << 4) | (2 << 6);
 // row_ror:n within a 16-lane row
 constexpr int ROR(int n) { return 0x120 + n; }
 
-__device__ __forceinline__ u32 wide_external_linear(u32 x) {
-  u32 r1 = dpp<QROT1>(x);
-  u32 r2 = dpp<QROT2>(x);
-  u32 r3 = dpp<QROT3>(x);
-  // 2 x + 3 r1 + r2 + r3 = 2 (x + r1) + r1 + (r2 + r3)
-  u32 a = add(x, r1);
-  u32 o = add(add(dbl(a), r1), add(r2, r3));
-  u32 s = add(o, dpp<ROR(4)>(o));
-  s = add(s, dpp<ROR(8)>(s));
-  return add(o, s);
+// Lane exchange with no old value: every source lane of these selectors lies
+// in the row, so the out-of-row fill (zero) is never taken.
+template <int CTRL>
+__device__ __forceinline__ u32 xdpp(u32 x) {
+  return (u32)__builtin_amdgcn_mov_dpp((int)x, CTRL, 0xF, 0xF, true);
+}
+// x - p if x >= p, for x < 2p.
+__device__ __forceinline__ u32 csub(u32 x) {
+  const u32 t = x - P;
+  return t < x ? t : x;
 }
 
+// External layer for x < p.  By linearity, M4 of each quad plus the sum of
+// M4 over the quads is M4 applied to z = x + X, where X is the sum of the
+// four quads' elements at this position (32-bit rotated adds, each sum below
+// 2p brought below p).  Out: 2z + 3 z1 + z2 + z3 < 14p for z < 2p.
+__device__ __forceinline__ u64 wide_external_linear(u32 x) {
+  u32 X = csub(x + dpp<ROR(4)>(x));
+  X = csub(X + dpp<ROR(8)>(X));
+  const u32 z = x + X;
+  const u32 r1 = xdpp<QROT1>(z);
+  const u32 r2 = xdpp<QROT2>(z);
+  const u32 r3 = xdpp<QROT3>(z);
+  return mad_u64(z, 2u, mad_u64(r1, 3u, (u64)r2 + r3));
+}
+
+// Sum of the row's 16 lanes for x < p: < p.
 __device__ __forceinline__ u32 wide_row_sum(u32 x) {
-  u32 s = add(x, dpp<ROR(8)>(x));
-  s = add(s, dpp<ROR(4)>(s));
-  s = add(s, dpp<ROR(2)>(s));
-  return add(s, dpp<ROR(1)>(s));
+  u32 s = csub(x + dpp<ROR(8)>(x));
+  s = csub(s + dpp<ROR(4)>(s));
+  s = csub(s + dpp<ROR(2)>(s));
+  return csub(s + dpp<ROR(1)>(s));
+}
+
+// a b / 2^32 mod p for a < 2p, b < p, without the conditional subtraction:
+// (a b + (2^32 - 1) p) / 2^32 < 2p, and the 64-bit sum stays below
+// 2p^2 + 2^32 p < 2^64.
+__device__ __forceinline__ u32 mul_lazy(u32 a, u32 b) {
+  const u64 m = (u64)a * b;
+  const u32 t = (u32)m * MU_NEG;
+  return (u32)(((u64)t * P + m) >> 32);
 }
 
 // Per-lane constants of a 16-lane permutation: external round constants of
@@ -85,23 +116,32 @@ __device__ __forceinline__ WideLane wide_lane(const P2C16& c, int j) {
     w.ext_term[r] = c.ext_term[r * 16 + j];
   }
   w.diag = internal_diag16(j);
+  // a plain 32-bit register: the switch above otherwise becomes 64-bit
+  // constants and the products with it 64-bit multiplies
+  asm volatile("" : "+v"(w.diag));
   return w;
 }
 
-// x: this lane's state element; j = lane & 15.
+// x: this lane's state element (< p); j = lane & 15.  Returns the lane's
+// element of the permuted state (< p).
 __device__ __forceinline__ u32 permute16_wide(u32 x, int j, const WideLane& w, const P2C16& c) {
-  x = wide_external_linear(x);
+  u64 y = wide_external_linear(x);
 #pragma unroll
-  for (int r = 0; r < 4; r++) x = wide_external_linear(cube(add(x, w.ext_init[r])));
+  for (int r = 0; r < 4; r++)   // y + rc < 15p; the S-box output < 1.75p is brought below p
+    y = wide_external_linear(csub((u32)cube_lazy(full_reduce(mad_u64(w.ext_init[r], 1u, y)))));
+  x = full_reduce(y);
 #pragma unroll
   for (int r = 0; r < P2C16::RP; r++) {
-    u32 y = cube(add(x, c.internal[r]));
-    x = j == 0 ? y : x;
-    x = add(wide_row_sum(x), mul(w.diag, x));
+    const u32 x0 = csub((u32)cube_lazy(add(x, c.internal[r])));
+    x = j == 0 ? x0 : x;
+    const u32 sum = wide_row_sum(x);
+    x = add(sum, csub(mul_lazy(x, w.diag)));   // d x < 2p -> < p
   }
+  y = x;
 #pragma unroll
-  for (int r = 0; r < 4; r++) x = wide_external_linear(cube(add(x, w.ext_term[r])));
-  return x;
+  for (int r = 0; r < 4; r++)
+    y = wide_external_linear(csub((u32)cube_lazy(full_reduce(mad_u64(w.ext_term[r], 1u, y)))));
+  return full_reduce(y);
 }
 
 }  // namespace kb
