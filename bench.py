@@ -30,7 +30,7 @@ VALU_PEAK_TOPS = 39.3
 # few instructions of candidate setup), down from 7496 for the textbook form
 # (one modular reduction per operation).  Hashing stages are issue-bound, so
 # permutations x 5052 / time against the issue rate is their roofline.
-VALU_OPS_PER_PERM = 5052
+VALU_OPS_PER_PERM = 4485
 VALU_STAGES = ("merkle_leaves", "merkle_tree", "grind")
 # Kernels of each stage, for the HBM traffic measured with rocprofv3 --pmc
 # (FETCH_SIZE and WRITE_SIZE per dispatch, calibrated on a copy of known size).

@@ -165,6 +165,20 @@ __host__ __device__ __forceinline__ u32 quot31(u64 y) {
   return (u32)(y >> 31);
 #endif
 }
+// Hides a value's origin from the compiler (no instruction), so it keeps the
+// multiply-add forms below instead of rewriting them into longer sequences
+// (24-bit products, 64-bit subtractions, selects).  A 32-bit value added to a
+// 64-bit one is written mad_u64(x, one, y) with one = opaque(1u): a single
+// v_mad_u64_u32, where a plain add would first widen x with a zeroed high
+// word.
+template <typename T>
+__host__ __device__ __forceinline__ T opaque(T x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm("" : "+v"(x));
+#endif
+  return x;
+}
+
 __host__ __device__ __forceinline__ u64 lazy_reduce(u64 y) {
   return mad_i64((int32_t)quot31(y), -(int32_t)P, y);
 }
@@ -177,14 +191,14 @@ __host__ __device__ __forceinline__ u32 full_reduce(u64 y) {   // y < 2^37.9 -> 
 // sum + z 2^-K and sum - z 2^-K for z < 2p; `sumk` = sum + 2p keeps both
 // nonnegative (r 127 2^(24-K) < 2^31 < 2p, and z >> K < p).
 template <int K>
-__host__ __device__ __forceinline__ u64 add_div2k(u64 sumk, u32 z) {
+__host__ __device__ __forceinline__ u64 add_div2k(u64 sumk, u32 z, u32 one) {
   constexpr int32_t CK = 127 << (24 - K);
-  return mad_i64((int32_t)(z & ((1u << K) - 1)), -CK, mad_u64(z >> K, 1u, sumk));
+  return mad_i64((int32_t)opaque(z & ((1u << K) - 1)), -CK, mad_u64(z >> K, one, sumk));
 }
 template <int K>
-__host__ __device__ __forceinline__ u64 sub_div2k(u64 sumk, u32 z) {
+__host__ __device__ __forceinline__ u64 sub_div2k(u64 sumk, u32 z, u32 one) {
   constexpr u32 CK = 127u << (24 - K);
-  return mad_u64(z & ((1u << K) - 1), CK, sumk + (P - (z >> K)));   // -q = p - q (mod p)
+  return mad_u64(opaque(z & ((1u << K) - 1)), CK, mad_u64(P - (z >> K), one, sumk));   // -q = p - q (mod p)
 }
 
 // Montgomery products without the final conditional subtraction.  For
@@ -206,20 +220,20 @@ __host__ __device__ __forceinline__ u64 cube_lazy(u32 x) {
 
 // External layer on inputs x < 2p: M4 on each quad, then each element plus
 // the sum of its position over the quads.  Outputs < 70p.
-__host__ __device__ __forceinline__ void external_linear16_wide(const u64* x, u64* y) {
+__host__ __device__ __forceinline__ void external_linear16_wide(const u32* x, u64* y, u32 one, u64 zero) {
   u64 n[16];
 #pragma unroll
   for (int c = 0; c < 16; c += 4) {
-    const u64 x0 = x[c], x1 = x[c + 1], x2 = x[c + 2], x3 = x[c + 3];
-    const u64 t01 = x0 + x1;
-    const u64 t23 = x2 + x3;
+    const u32 x0 = x[c], x1 = x[c + 1], x2 = x[c + 2], x3 = x[c + 3];
+    const u64 t01 = mad_u64(x1, one, mad_u64(x0, one, zero));
+    const u64 t23 = mad_u64(x3, one, mad_u64(x2, one, zero));
     const u64 t0123 = t01 + t23;
-    const u64 t01123 = t0123 + x1;
-    const u64 t01233 = t0123 + x3;
-    n[c + 3] = t01233 + 2 * x0;   // 3x0 + x1 + x2 + 2x3
-    n[c + 1] = t01123 + 2 * x2;   // x0 + 2x1 + 3x2 + x3
-    n[c + 0] = t01123 + t01;      // 2x0 + 3x1 + x2 + x3
-    n[c + 2] = t01233 + t23;      // x0 + x1 + 2x2 + 3x3
+    const u64 t01123 = mad_u64(x1, one, t0123);
+    const u64 t01233 = mad_u64(x3, one, t0123);
+    n[c + 3] = mad_u64(x0, 2u, t01233);   // 3x0 + x1 + x2 + 2x3
+    n[c + 1] = mad_u64(x2, 2u, t01123);   // x0 + 2x1 + 3x2 + x3
+    n[c + 0] = t01123 + t01;              // 2x0 + 3x1 + x2 + x3
+    n[c + 2] = t01233 + t23;              // x0 + x1 + 2x2 + 3x3
   }
   u64 s[4];
 #pragma unroll
@@ -231,15 +245,16 @@ __host__ __device__ __forceinline__ void external_linear16_wide(const u64* x, u6
 // Initial linear layer and the four initial external rounds.  In: s < p.
 // Out: s[0] < p, s[1..15] < 2p.
 __host__ __device__ __forceinline__ void permute16_head(u32* s, const P2C16& c) {
-  u64 x[16], y[16];
-#pragma unroll
-  for (int i = 0; i < 16; i++) x[i] = s[i];
-  external_linear16_wide(x, y);
+  const u32 one = opaque(1u);
+  const u64 zero = opaque(0ull);
+  u32 x[16];
+  u64 y[16];
+  external_linear16_wide(s, y, one, zero);
 #pragma unroll
   for (int r = 0; r < 4; r++) {
 #pragma unroll
-    for (int i = 0; i < 16; i++) x[i] = cube_lazy(full_reduce(y[i] + c.ext_init[r * 16 + i]));   // < 71p
-    external_linear16_wide(x, y);
+    for (int i = 0; i < 16; i++) x[i] = (u32)cube_lazy(full_reduce(y[i] + c.ext_init[r * 16 + i]));   // < 71p
+    external_linear16_wide(x, y, one, zero);
   }
   s[0] = full_reduce(y[0]);
 #pragma unroll
@@ -250,33 +265,36 @@ __host__ __device__ __forceinline__ void permute16_head(u32* s, const P2C16& c) 
 // V = [-2, 1, 2, 1/2, 3, 4, -1/2, -3, -4, 1/2^8, 1/8, 1/2^24, -1/2^8, -1/8,
 // -1/16, -1/2^24].  In and out: s[0] < p, s[1..15] < 2p.
 __host__ __device__ __forceinline__ void permute16_internal(u32* s, const P2C16& c) {
+  const u32 one = opaque(1u);
   u64 z[16];
 #pragma unroll
   for (int i = 1; i < 16; i++) z[i] = s[i];
   u64 y0 = s[0];
-#pragma unroll
+#pragma clang loop unroll(full)
   for (int r = 0; r < P2C16::RP; r++) {
-    const u32 x0 = (u32)cube_lazy(full_reduce(y0 + c.internal[r]));   // y0 + rc < 37p; x0 < 1.75p
-    u64 sum = x0;
+    // The sum of z[1..15] does not wait for the S-box: it is added last.
+    u64 sz = z[1];
 #pragma unroll
-    for (int i = 1; i < 16; i++) sum += z[i];   // < 32p
+    for (int i = 2; i < 16; i++) sz += z[i];   // < 30p
+    const u32 x0 = (u32)cube_lazy(full_reduce(y0 + c.internal[r]));   // y0 + rc < 37p; x0 < 1.75p
+    const u64 sum = mad_u64(x0, one, sz);   // < 32p
     const u64 sumk = sum + TWO_P;
-    y0 = mad_u64(TWO_P - x0, 2u, sum);   // sum - 2 x0 + 4p, < 36p
+    y0 = mad_u64(opaque(TWO_P - x0), 2u, sum);   // sum - 2 x0 + 4p, < 36p
     z[1] = lazy_reduce(sum + z[1]);
     z[2] = lazy_reduce(sum + 2 * z[2]);
-    z[3] = lazy_reduce(add_div2k<1>(sumk, (u32)z[3]));
+    z[3] = lazy_reduce(add_div2k<1>(sumk, (u32)z[3], one));
     z[4] = lazy_reduce(mad_u64((u32)z[4], 3u, sum));
     z[5] = lazy_reduce(sum + 4 * z[5]);
-    z[6] = lazy_reduce(sub_div2k<1>(sumk, (u32)z[6]));
+    z[6] = lazy_reduce(sub_div2k<1>(sumk, (u32)z[6], one));
     z[7] = lazy_reduce(mad_u64(TWO_P - (u32)z[7], 3u, sum));
     z[8] = lazy_reduce(mad_u64(TWO_P - (u32)z[8], 4u, sum));
-    z[9] = lazy_reduce(add_div2k<8>(sumk, (u32)z[9]));
-    z[10] = lazy_reduce(add_div2k<3>(sumk, (u32)z[10]));
-    z[11] = lazy_reduce(add_div2k<24>(sumk, (u32)z[11]));
-    z[12] = lazy_reduce(sub_div2k<8>(sumk, (u32)z[12]));
-    z[13] = lazy_reduce(sub_div2k<3>(sumk, (u32)z[13]));
-    z[14] = lazy_reduce(sub_div2k<4>(sumk, (u32)z[14]));
-    z[15] = lazy_reduce(sub_div2k<24>(sumk, (u32)z[15]));
+    z[9] = lazy_reduce(add_div2k<8>(sumk, (u32)z[9], one));
+    z[10] = lazy_reduce(add_div2k<3>(sumk, (u32)z[10], one));
+    z[11] = lazy_reduce(add_div2k<24>(sumk, (u32)z[11], one));
+    z[12] = lazy_reduce(sub_div2k<8>(sumk, (u32)z[12], one));
+    z[13] = lazy_reduce(sub_div2k<3>(sumk, (u32)z[13], one));
+    z[14] = lazy_reduce(sub_div2k<4>(sumk, (u32)z[14], one));
+    z[15] = lazy_reduce(sub_div2k<24>(sumk, (u32)z[15], one));
   }
   s[0] = full_reduce(y0);
 #pragma unroll
@@ -285,14 +303,17 @@ __host__ __device__ __forceinline__ void permute16_internal(u32* s, const P2C16&
 
 // The four terminal external rounds.  In: s[0] < p, s[1..15] < 2p.  Out: s < p.
 __host__ __device__ __forceinline__ void permute16_tail(u32* s, const P2C16& c) {
-  u64 x[16], y[16];
+  const u32 one = opaque(1u);
+  const u64 zero = opaque(0ull);
+  u32 x[16];
+  u64 y[16];
 #pragma unroll
   for (int i = 0; i < 16; i++) y[i] = s[i];
 #pragma unroll
   for (int r = 0; r < 4; r++) {
 #pragma unroll
-    for (int i = 0; i < 16; i++) x[i] = cube_lazy(full_reduce(y[i] + c.ext_term[r * 16 + i]));
-    external_linear16_wide(x, y);
+    for (int i = 0; i < 16; i++) x[i] = (u32)cube_lazy(full_reduce(y[i] + c.ext_term[r * 16 + i]));
+    external_linear16_wide(x, y, one, zero);
   }
 #pragma unroll
   for (int i = 0; i < 16; i++) s[i] = full_reduce(y[i]);
