@@ -26,10 +26,10 @@ import numpy as np
 HBM_PEAK_GBS = 8000.0
 VALU_PEAK_TOPS = 39.3
 # VALU instructions per width-16 Poseidon2 permutation in the gfx950 build:
-# 5052 in the compiled grinding kernel (one permutation per candidate plus a
-# few instructions of candidate setup), down from 7496 for the textbook form
-# (one modular reduction per operation).  Hashing stages are issue-bound, so
-# permutations x 5052 / time against the issue rate is their roofline.
+# 4485 in the compiled grinding kernel's loop (one permutation per candidate
+# plus a few instructions of candidate setup), down from 7496 for the textbook
+# form (one modular reduction per operation).  Hashing stages are issue-bound,
+# so permutations x 4485 / time against the issue rate is their roofline.
 VALU_OPS_PER_PERM = 4485
 VALU_STAGES = ("merkle_leaves", "merkle_tree", "grind")
 # Kernels of each stage, for the HBM traffic measured with rocprofv3 --pmc
