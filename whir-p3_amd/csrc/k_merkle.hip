@@ -273,9 +273,9 @@ __device__ __forceinline__ void phase_fence(u32* s) {
 // that observes the round's (h(0), h(2)); its state is the candidates' base.
 __global__ void __launch_bounds__(256) k_grind(const P2C16* __restrict__ cp, Sponge16 base, int wpos, int tr_mode,
                                                ScTr tr, u32 mask, u32* result, u32 start, u32 end) {
-  if (tr_mode == TR_COEFFS_GRIND) {
+  if (tr_mode == TR_COEFFS_GRIND || tr_mode == TR_OBS_GRIND) {
     __shared__ GrindBase gbs;
-    tr_prologue(TR_COEFFS_GRIND, tr, *cp, &gbs, nullptr);
+    tr_prologue(tr_mode, tr, *cp, &gbs, nullptr);
 #pragma unroll
     for (int k = 0; k < 16; k++) base.s[k] = gbs.s[k];
     wpos = (int)gbs.wpos;
@@ -311,6 +311,53 @@ __global__ void __launch_bounds__(256) k_grind(const P2C16* __restrict__ cp, Spo
   }
 }
 
+// Short searches (at most WIDE_GRIND_BITS bits, a few thousand candidates
+// expected): one candidate per 16-lane row with the lane-parallel
+// permutation, sixteen per block per pass, so a pass takes one permutation's
+// latency (about 2 us) instead of the 8 us of the one-lane form.  Same search
+// order and stop rule as k_grind, with a wave's four candidates as the unit.
+constexpr int WIDE_GRIND_BITS = 12;
+
+__global__ void __launch_bounds__(256) k_grind_wide(const P2C16* __restrict__ cp, Sponge16 base, int wpos,
+                                                    int tr_mode, ScTr tr, u32 mask, u32* result, u32 start,
+                                                    u32 end) {
+  if (tr_mode == TR_COEFFS_GRIND || tr_mode == TR_OBS_GRIND) {
+    __shared__ GrindBase gbs;
+    tr_prologue(tr_mode, tr, *cp, &gbs, nullptr);
+#pragma unroll
+    for (int k = 0; k < 16; k++) base.s[k] = gbs.s[k];
+    wpos = (int)gbs.wpos;
+  }
+  const P2C16& c = *cp;
+  const int j = threadIdx.x & 15;
+  const WideLane wl = wide_lane(c, j);
+  u32 bj = 0;
+#pragma unroll
+  for (int k = 0; k < 16; k++)
+    if (k == j) bj = base.s[k];
+  const u32 stride = gridDim.x * 16;
+  const u32 wave_off = blockIdx.x * 16 + (threadIdx.x >> 6) * 4;
+  const u32 cand_off = blockIdx.x * 16 + (threadIdx.x >> 4);
+  u32 b = best_known(result);
+  for (u64 g = start; g + wave_off < end; g += stride) {
+    const u64 first = g + wave_off;
+    if ((u64)__builtin_amdgcn_readfirstlane(b) <= first) break;
+    b = best_known(result);   // checked before the next pass
+    const u64 w = g + cand_off;
+    u32 x = (j == wpos) ? to_monty((u32)w) : bj;
+    x = permute16_wide(x, j, wl, c);
+    if (j == 7 && w < end && (from_monty(x) & mask) == 0) atomicMin(result, (u32)w);
+  }
+}
+
+// Blocks of a short search: sixteen candidates each, enough for one pass over
+// sixteen times the expected search, at most `blocks` and the range.
+static unsigned wide_grind_blocks(int bits, u64 start, u64 end, int blocks) {
+  const u64 span = (16ull << bits) / 16;
+  return (unsigned)std::max<u64>(1, std::min<u64>(std::min<u64>((end - start + 15) / 16, span),
+                                                  (u64)std::max(blocks, 1)));
+}
+
 // Scans [start, start + count) (clipped to p) in one launch of `blocks` blocks
 // of 256 lanes (fewer when the range is short).
 int grind_range(const P2C16* d_consts, const u32* state16, int wpos, int bits, u32* d_result, u64 start, u64 count,
@@ -320,6 +367,12 @@ int grind_range(const P2C16* d_consts, const u32* state16, int wpos, int bits, u
   u32 mask = bits >= 32 ? 0xffffffffu : ((1u << bits) - 1);
   if (start >= P) return 0;
   u64 end = std::min<u64>(start + count, P);
+  if (bits <= WIDE_GRIND_BITS) {
+    const unsigned nb = wide_grind_blocks(bits, start, end, blocks);
+    hipLaunchKernelGGL(k_grind_wide, dim3(nb), dim3(256), 0, st, d_consts, b, wpos, (int)TR_NONE, ScTr{}, mask,
+                       d_result, (u32)start, (u32)end);
+    return hipGetLastError() == hipSuccess ? 0 : -5;
+  }
   u64 need = (end - start + 255) / 256;
   unsigned nb = (unsigned)std::min<u64>(need, (u64)std::max(blocks, 1));
   hipLaunchKernelGGL(k_grind, dim3(nb), dim3(256), 0, st, d_consts, b, wpos, (int)TR_NONE, ScTr{}, mask, d_result,
@@ -327,15 +380,227 @@ int grind_range(const P2C16* d_consts, const u32* state16, int wpos, int bits, u
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 
-int grind_range_tr(const P2C16* d_consts, const ScTr& tr, int bits, u64 start, u64 count, int blocks, hipStream_t st) {
+int grind_range_tr(const P2C16* d_consts, const ScTr& tr, int bits, u64 start, u64 count, int blocks, hipStream_t st,
+                   int mode) {
   Sponge16 b{};
   u32 mask = bits >= 32 ? 0xffffffffu : ((1u << bits) - 1);
   if (start >= P) return 0;
   u64 end = std::min<u64>(start + count, P);
+  if (bits <= WIDE_GRIND_BITS) {
+    const unsigned nb = wide_grind_blocks(bits, start, end, blocks);
+    hipLaunchKernelGGL(k_grind_wide, dim3(nb), dim3(256), 0, st, d_consts, b, 0, mode, tr, mask, tr.grind,
+                       (u32)start, (u32)end);
+    return hipGetLastError() == hipSuccess ? 0 : -5;
+  }
   u64 need = (end - start + 255) / 256;
   unsigned nb = (unsigned)std::min<u64>(need, (u64)std::max(blocks, 1));
-  hipLaunchKernelGGL(k_grind, dim3(nb), dim3(256), 0, st, d_consts, b, 0, (int)TR_COEFFS_GRIND, tr, mask, tr.grind,
-                     (u32)start, (u32)end);
+  hipLaunchKernelGGL(k_grind, dim3(nb), dim3(256), 0, st, d_consts, b, 0, mode, tr, mask, tr.grind, (u32)start,
+                     (u32)end);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+// ------------------------------------------------------------------ round transcript
+// OOD sample of a round (prover/mod.rs:192-205): every block runs the step
+// (observe st.obs, sample y), expands y to the point z_j = y^(2^(m-1-j))
+// (expand_from_univariate, src/poly/multilinear.rs:73-85), then writes its
+// part of the point's split eq tables, as k_eq_split.  Block 0 logs y,
+// writes the point and the next challenger state.
+__global__ void __launch_bounds__(256) k_ood_eq_tables(StepTr st, ef* log_y, ef* pt_out, int m, int hi, int lo,
+                                                       ef* __restrict__ out_hi, ef* __restrict__ out_lo,
+                                                       const P2C16* __restrict__ cp) {
+  __shared__ u32 sp[16], in[8], out[8];
+  __shared__ ef z[RT_MAX_VARS];
+  const int t = threadIdx.x;
+  const P2C16& c = *cp;
+  const WideLane wl = wide_lane(c, t & 15);
+  BlockChallenger ch{sp, in, out, 0, 0, &wl, &c};
+  ch.load(*st.ch_in);
+  for (int k = 0; k < st.n_obs; k++) ch.observe(st.obs[k]);
+  ef y;
+#pragma unroll
+  for (int k = 0; k < 4; k++) y.c[k] = ch.sample();
+  if (t == 0) {
+    ef cur = y;
+    for (int j = m - 1; j >= 0; j--) {
+      z[j] = cur;
+      cur = ef_mul(cur, cur);
+    }
+  }
+  __syncthreads();
+  if (blockIdx.x == 0) {
+    ch.store(st.ch_out);
+    if (t == 0) {
+      *log_y = y;
+      if (st.n_obs == 4 && st.log_obs) *st.log_obs = ef{{st.obs[0], st.obs[1], st.obs[2], st.obs[3]}};
+    }
+    if (t < m) pt_out[t] = z[t];
+  }
+  u32 x = blockIdx.x * 256 + t;
+  const u32 nh = 1u << hi;
+  if (x >= nh + (1u << lo)) return;
+  int first = 0, cnt = hi;
+  ef acc = ef_one();
+  ef* o = out_hi;
+  if (x >= nh) {
+    x -= nh;
+    first = hi;
+    cnt = lo;
+    o = out_lo;
+  }
+  for (int j = 0; j < cnt; j++) {
+    const ef zj = z[first + j];
+    const int bit = (x >> (cnt - 1 - j)) & 1;
+    acc = ef_mul(acc, bit ? zj : ef_sub(ef_one(), zj));
+  }
+  o[x] = acc;
+}
+
+int ood_eq_tables(const StepTr& st, ef* log_y, ef* pt_out, int m, int hi, int lo, ef* out_hi, ef* out_lo,
+                  const P2C16* d_consts, hipStream_t st_) {
+  if (m < 1 || m > RT_MAX_VARS || hi + lo != m) return -1;
+  const u32 n = (1u << hi) + (1u << lo);
+  hipLaunchKernelGGL(k_ood_eq_tables, dim3((n + 255) / 256), dim3(256), 0, st_, st, log_y, pt_out, m, hi, lo, out_hi,
+                     out_lo, d_consts);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+// The round's steps after its OOD samples, in one block (prover/mod.rs:
+// 206-234): the proof-of-work witness is observed and its bits checked (or,
+// without proof of work, the last OOD answer / the root observed), one sample
+// is discarded, the STIR query indices are sampled (get_challenge_stir_queries,
+// src/whir/utils.rs:24-145: sample_bits calls of up to 20 bits, split into
+// per-query indices), sorted and deduplicated, and gamma is sampled.
+__global__ void __launch_bounds__(256) k_round_queries(StepTr st, const u32* grind, int pow_bits, u64 domain_size,
+                                                       int ff, int num_queries, u32 folded_gen, int n_ood,
+                                                       RoundLog* log, RoundBufs rb, const P2C16* __restrict__ cp) {
+  __shared__ u32 sp[16], in[8], out[8];
+  __shared__ u64 q[RT_MAX_QUERIES];
+  __shared__ u32 firsts[RT_MAX_QUERIES];
+  __shared__ u32 nq_sh;
+  const int t = threadIdx.x;
+  const P2C16& c = *cp;
+  const WideLane wl = wide_lane(c, t & 15);
+  BlockChallenger ch{sp, in, out, 0, 0, &wl, &c};
+  ch.load(*st.ch_in);
+  u32 err = 0, wm = 0;
+  if (pow_bits > 0) {
+    const u32 w = *grind;
+    wm = w < P ? to_monty(w) : 0u;
+    if (w >= P) err = 1;
+    ch.observe(wm);
+    const u64 pmask = pow_bits >= 32 ? 0xffffffffull : ((1ull << pow_bits) - 1);
+    if (((u64)from_monty(ch.sample()) & pmask) != 0) err = 1;
+  } else {
+    for (int k = 0; k < st.n_obs; k++) ch.observe(st.obs[k]);
+  }
+  (void)ch.sample();
+  const u64 folded = domain_size >> ff;
+  int bits = 0;
+  while ((1ull << bits) < folded) bits++;
+  const u64 mask = (1ull << bits) - 1;
+  const u64 total = (u64)num_queries * (u64)bits;
+  if (total <= 20) {
+    u64 all = (u64)from_monty(ch.sample()) & ((1ull << total) - 1);
+    for (int i = 0; i < num_queries; i++, all >>= bits)
+      if (t == 0) q[i] = (all & mask) % folded;
+  } else {
+    const int per_call = bits > 0 ? 20 / bits : 0;
+    if (per_call >= 2) {
+      int i = 0;
+      for (int left = num_queries; left > 0;) {
+        const int batch = left < per_call ? left : per_call;
+        u64 all = (u64)from_monty(ch.sample()) & ((1ull << (batch * bits)) - 1);
+        for (int j = 0; j < batch; j++, all >>= bits, i++)
+          if (t == 0) q[i] = (all & mask) % folded;
+        left -= batch;
+      }
+    } else {
+      for (int i = 0; i < num_queries; i++) {
+        const u64 v = (u64)from_monty(ch.sample()) & mask;
+        if (t == 0) q[i] = v;
+      }
+    }
+  }
+  __syncthreads();
+  // sorted distinct indices: a first occurrence goes to the number of
+  // distinct smaller values
+  u64 v = 0;
+  if (t < num_queries) {
+    v = q[t];
+    u32 first = 1;
+    for (int j = 0; j < t; j++)
+      if (q[j] == v) first = 0;
+    firsts[t] = first;
+  }
+  __syncthreads();
+  if (t == 0) {
+    u32 n = 0;
+    for (int j = 0; j < num_queries; j++) n += firsts[j];
+    nq_sh = n;
+  }
+  __syncthreads();
+  const u32 nq = nq_sh;
+  if (t < num_queries && firsts[t]) {
+    u32 pos = 0;
+    for (int j = 0; j < num_queries; j++)
+      if (firsts[j] && q[j] < v) pos++;
+    rb.idx[pos] = v;
+    rb.selz[pos] = pow(folded_gen, v);
+    log->idx[pos] = (u32)v;
+  }
+  ef gamma;
+#pragma unroll
+  for (int k = 0; k < 4; k++) gamma.c[k] = ch.sample();
+  for (u32 i = t; i < (u32)n_ood + nq; i += 256) rb.gpow[i] = ef_pow(gamma, i);
+  ch.store(st.ch_out);
+  if (t == 0) {
+    *rb.nq = nq;
+    log->nq = nq;
+    log->gamma = gamma;
+    log->witness = wm;
+    log->err = err;
+    if (pow_bits == 0 && st.n_obs == 4 && st.log_obs) *st.log_obs = ef{{st.obs[0], st.obs[1], st.obs[2], st.obs[3]}};
+  }
+}
+
+int round_queries(const StepTr& st, const u32* grind, int pow_bits, u64 domain_size, int folding_factor,
+                  int num_queries, u32 folded_gen, int n_ood, RoundLog* log, const RoundBufs& rb,
+                  const P2C16* d_consts, hipStream_t st_) {
+  if (num_queries < 1 || num_queries > RT_MAX_QUERIES || n_ood < 0 || n_ood > RT_MAX_OOD) return -1;
+  hipLaunchKernelGGL(k_round_queries, dim3(1), dim3(256), 0, st_, st, grind, pow_bits, domain_size, folding_factor,
+                     num_queries, folded_gen, n_ood, log, rb, d_consts);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+// k_open over the first *nq of `idx` (grid sized for max_nq).
+__global__ void k_open_dev(const u32* __restrict__ rows, int row_len, const u32* __restrict__ tree, u64 height,
+                           int depth, const u64* __restrict__ idx, const u32* __restrict__ nq, u32* __restrict__ out) {
+  if (blockIdx.y >= *nq) return;
+  const u64 per = (u64)row_len + 8ull * depth;
+  const u64 q = blockIdx.y;
+  const u64 o = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= per) return;
+  const u64 ix = idx[q];
+  u32 v;
+  if (o < (u64)row_len) {
+    v = rows[ix * row_len + o];
+  } else {
+    u64 l = (o - row_len) / 8, k = (o - row_len) % 8;
+    u64 off = 2 * height - ((2 * height) >> l);
+    u64 node = (ix >> l) ^ 1;
+    v = tree[(off + node) * 8 + k];
+  }
+  out[q * per + o] = v;
+}
+
+int merkle_open_dev(const u32* rows, int row_len, const u32* tree, u64 height, const u64* d_idx, const u32* d_nq,
+                    int max_nq, u32* d_out, hipStream_t st) {
+  int depth = 0;
+  while ((1ull << depth) < height) depth++;
+  const u64 per = (u64)row_len + 8ull * depth;
+  if (max_nq <= 0) return 0;
+  hipLaunchKernelGGL(k_open_dev, dim3((unsigned)((per + 255) / 256), (unsigned)max_nq), dim3(256), 0, st, rows,
+                     row_len, tree, height, depth, d_idx, d_nq, d_out);
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 

@@ -60,6 +60,36 @@ int eq_split(const PointArg& z, int hi, int lo, const ef& scale, ef* out_hi, ef*
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 
+// k_eq_split with the point and the scale read from device memory.
+__global__ void k_eq_split_dev(const ef* __restrict__ z, const ef* __restrict__ scale, int hi, int lo,
+                               ef* __restrict__ out_hi, ef* __restrict__ out_lo) {
+  u32 x = blockIdx.x * blockDim.x + threadIdx.x;
+  const u32 nh = 1u << hi;
+  if (x >= nh + (1u << lo)) return;
+  int first = 0, t = hi;
+  ef acc = *scale;
+  ef* out = out_hi;
+  if (x >= nh) {
+    x -= nh;
+    first = hi;
+    t = lo;
+    acc = ef_one();
+    out = out_lo;
+  }
+  for (int j = 0; j < t; j++) {
+    const ef zj = z[first + j];
+    int bit = (x >> (t - 1 - j)) & 1;
+    acc = ef_mul(acc, bit ? zj : ef_sub(ef_one(), zj));
+  }
+  out[x] = acc;
+}
+
+int eq_split_dev(const ef* z, const ef* scale, int hi, int lo, ef* out_hi, ef* out_lo, hipStream_t st) {
+  u32 n = (1u << hi) + (1u << lo);
+  hipLaunchKernelGGL(k_eq_split_dev, dim3((n + 255) / 256), dim3(256), 0, st, z, scale, hi, lo, out_hi, out_lo);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
 int eq_table(const PointArg& z, int first, int t, const ef& scale, ef* out, hipStream_t st) {
   u32 n = 1u << t;
   hipLaunchKernelGGL(k_eq_small, dim3((n + 255) / 256), dim3(256), 0, st, z, first, t, scale, out);
@@ -294,8 +324,9 @@ int eq_combine(ef* Wt, u64 n, int lo, const EqTables& tabs, int K, int accumulat
 
 // Select tables for z_j in F and weight g_j in EF:
 //   A[j][xh] = g_j * (z_j^(2^lo))^xh  (EF),   B[j][xl] = z_j^xl  (F)
-__global__ void k_select_tables(const u32* __restrict__ z, const ef* __restrict__ g, int nsel, int hi, int lo,
-                                ef* __restrict__ A, u32* __restrict__ B) {
+__global__ void k_select_tables(const u32* __restrict__ z, const ef* __restrict__ g, int nsel, const u32* nsel_dev,
+                                int hi, int lo, ef* __restrict__ A, u32* __restrict__ B) {
+  if (nsel_dev) nsel = (int)*nsel_dev;
   const u32 nh = 1u << hi, nl = 1u << lo;
   u64 total = (u64)nsel * (nh + nl);
   for (u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (u64)gridDim.x * blockDim.x) {
@@ -310,30 +341,58 @@ __global__ void k_select_tables(const u32* __restrict__ z, const ef* __restrict_
 }
 
 // W[x] += sum_j A[j][x >> lo] * B[j][x & mask]
+// Wt[x] += sum_j A[j][x >> lo] * B[j][x & mask].  The products of each EF
+// component are summed unreduced: a 64-bit sum below 2^56.1 plus three
+// products (each below p^2) stays below 2^64, and is folded back below 2^56.1
+// with 2^32 = 2^25 - 2 (mod p); one Montgomery reduction per component ends
+// the sum (a sum of Montgomery products reduces to the Montgomery form of
+// the sum).
+__device__ __forceinline__ u64 fold32(u64 t) {
+  return ((u64)(u32)(t >> 32)) * 33554430u + (u32)t;   // (t >> 32) (2^32 mod p) + (t mod 2^32)
+}
+
 __global__ void __launch_bounds__(256) k_select_combine(ef* __restrict__ Wt, u64 n, int lo, int hi,
                                                         const ef* __restrict__ A, const u32* __restrict__ B,
-                                                        int nsel) {
+                                                        int nsel, const u32* nsel_dev) {
+  if (nsel_dev) nsel = (int)*nsel_dev;
   const u32 mask = (1u << lo) - 1;
   const u64 nh = 1ull << hi, nl = 1ull << lo;
   for (u64 x = (u64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (u64)gridDim.x * blockDim.x) {
-    ef acc = Wt[x];
     const u64 xh = x >> lo, xl = x & mask;
-    for (int j = 0; j < nsel; j++) acc = ef_add(acc, ef_mul_base(A[j * nh + xh], B[j * nl + xl]));
-    Wt[x] = acc;
+    u64 acc[4] = {0, 0, 0, 0};
+    int j = 0;
+    for (; j + 3 <= nsel; j += 3) {
+      const u32 b0 = B[(u64)j * nl + xl], b1 = B[(u64)(j + 1) * nl + xl], b2 = B[(u64)(j + 2) * nl + xl];
+      const ef a0 = A[(u64)j * nh + xh], a1 = A[(u64)(j + 1) * nh + xh], a2 = A[(u64)(j + 2) * nh + xh];
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        acc[k] = fold32((u64)a2.c[k] * b2 + ((u64)a1.c[k] * b1 + ((u64)a0.c[k] * b0 + acc[k])));
+    }
+    for (; j < nsel; j++) {
+      const u32 b0 = B[(u64)j * nl + xl];
+      const ef a0 = A[(u64)j * nh + xh];
+#pragma unroll
+      for (int k = 0; k < 4; k++) acc[k] = fold32((u64)a0.c[k] * b0 + acc[k]);
+    }
+    ef r;
+#pragma unroll
+    for (int k = 0; k < 4; k++) r.c[k] = redc(acc[k]);
+    Wt[x] = ef_add(Wt[x], r);
   }
 }
 
-int select_combine(ef* Wt, int m, const u32* d_z, const ef* d_g, int nsel, ef* A, u32* B, hipStream_t st) {
+int select_combine(ef* Wt, int m, const u32* d_z, const ef* d_g, int nsel, const u32* d_nsel, ef* A, u32* B,
+                   hipStream_t st) {
   if (nsel == 0) return 0;
   int lo = m / 2, hi = m - lo;
   u64 tt = (u64)nsel * ((1ull << hi) + (1ull << lo));
   u64 b1 = (tt + 255) / 256;
   if (b1 > 65536) b1 = 65536;
-  hipLaunchKernelGGL(k_select_tables, dim3((unsigned)b1), dim3(256), 0, st, d_z, d_g, nsel, hi, lo, A, B);
+  hipLaunchKernelGGL(k_select_tables, dim3((unsigned)b1), dim3(256), 0, st, d_z, d_g, nsel, d_nsel, hi, lo, A, B);
   u64 n = 1ull << m;
   u64 blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(k_select_combine, dim3((unsigned)blocks), dim3(256), 0, st, Wt, n, lo, hi, A, B, nsel);
+  hipLaunchKernelGGL(k_select_combine, dim3((unsigned)blocks), dim3(256), 0, st, Wt, n, lo, hi, A, B, nsel, d_nsel);
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 

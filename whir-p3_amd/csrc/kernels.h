@@ -45,9 +45,12 @@ enum ScTrMode {
   TR_COEFFS_GRIND = 1,    // observe (h(0), h(2)); the grind state for the candidates
   TR_WITNESS_SAMPLE = 2,  // observe the witness, check its bits, sample r
   TR_COEFFS_SAMPLE = 3,   // observe (h(0), h(2)) and sample r (no proof of work)
+  TR_OBS_GRIND = 4,       // observe obs; the grind state for the candidates
 };
 
 // Everything a transcript prologue reads and writes (a kernel argument).
+// TR_OBS_GRIND (a round's proof of work) uses ch_in, ch_out and obs/n_obs/
+// log_obs only.
 struct ScTr {
   const DevChallenger* ch_in;   // state to start from (device or host-visible memory)
   DevChallenger* ch_out;        // state after this step (block 0 writes it)
@@ -57,6 +60,48 @@ struct ScTr {
   u32* grind_next;              // next round's grind result (block 0 resets it to p)
   int nb;
   int pow_bits;
+  const u32* obs;               // TR_OBS_GRIND: words to observe first
+  int n_obs;
+  ef* log_obs;
+};
+
+// A WHIR round's transcript steps between its commitment and its sumcheck,
+// run on the device (transcript.h, prover/mod.rs:192-248): observe the root,
+// the OOD samples (challenge, expanded point, answer), the round's proof of
+// work, the STIR query indices and the combination randomness gamma.  The
+// host replays them from this log after the round's sumcheck.
+constexpr int RT_MAX_OOD = 4;
+constexpr int RT_MAX_QUERIES = 256;
+constexpr int RT_MAX_VARS = 32;
+struct RoundLog {
+  ef y[RT_MAX_OOD];    // OOD challenges as sampled (before expansion)
+  ef ev[RT_MAX_OOD];   // OOD answers, as observed
+  ef gamma;
+  u32 witness;         // proof-of-work witness (Montgomery form)
+  u32 nq;              // distinct query indices, sorted, in idx
+  u32 err;             // nonzero: witness rejected
+  u32 pad;
+  u32 idx[RT_MAX_QUERIES];
+};
+// Device buffers of a round's transcript.
+struct RoundBufs {
+  ef* pts;        // RT_MAX_OOD points of RT_MAX_VARS coordinates
+  ef* ev;         // RT_MAX_OOD answers (written by the eq-weighted sums)
+  u64* idx;       // RT_MAX_QUERIES query indices
+  u32* nq;        // their count
+  u32* selz;      // folded-domain point of each query
+  ef* gpow;       // gamma^i, i < RT_MAX_OOD + RT_MAX_QUERIES
+  u32* grind;     // grind slot of the round's proof of work
+};
+// One step's challenger hand-off: read ch_in, observe n_obs words at obs
+// (logging them as an EF answer to log_obs when n_obs == 4), then the step's
+// own samples; block 0 writes ch_out.
+struct StepTr {
+  const DevChallenger* ch_in;
+  DevChallenger* ch_out;
+  const u32* obs;
+  int n_obs;
+  ef* log_obs;
 };
 
 struct EqTables {
@@ -87,19 +132,43 @@ int grind_range(const P2C16* d_consts, const u32* state16, int wpos, int bits, u
                 int blocks, hipStream_t st);
 // Same, with the base state and candidate position from the transcript step
 // TR_COEFFS_GRIND run by every block first; the result goes to tr.grind.
-int grind_range_tr(const P2C16* d_consts, const ScTr& tr, int bits, u64 start, u64 count, int blocks, hipStream_t st);
+int grind_range_tr(const P2C16* d_consts, const ScTr& tr, int bits, u64 start, u64 count, int blocks, hipStream_t st,
+                   int mode = TR_COEFFS_GRIND);
+
+// ---- a round's transcript steps (k_merkle.hip)
+// OOD sample: observe st.obs, sample the challenge y into *log_y, expand it to
+// the m-variable point pt_out (y^(2^(m-1-j)) at j), and write its split eq
+// tables (hi + lo = m variables).
+int ood_eq_tables(const StepTr& st, ef* log_y, ef* pt_out, int m, int hi, int lo, ef* out_hi, ef* out_lo,
+                  const P2C16* d_consts, hipStream_t st_);
+// One block: the round's witness from `grind` (pow_bits > 0) or st.obs, one
+// discarded sample, the STIR query indices over the folded domain (sorted,
+// distinct), their points folded_gen^idx, gamma and its powers (n_ood + nq of
+// them).  Log and buffers as in RoundLog / RoundBufs.
+int round_queries(const StepTr& st, const u32* grind, int pow_bits, u64 domain_size, int folding_factor,
+                  int num_queries, u32 folded_gen, int n_ood, RoundLog* log, const RoundBufs& rb,
+                  const P2C16* d_consts, hipStream_t st_);
+// merkle_open with the query indices and their count in device memory
+// (at most max_nq).
+int merkle_open_dev(const u32* rows, int row_len, const u32* tree, u64 height, const u64* d_idx, const u32* d_nq,
+                    int max_nq, u32* d_out, hipStream_t st);
 
 // ---- multilinear kernels (k_poly.hip)
 int eq_table(const PointArg& z, int first, int t, const ef& scale, ef* out, hipStream_t st);
 // Both halves of a split table in one launch: out_hi = scale * eq(z[0..hi)), out_lo = eq(z[hi..hi+lo)).
 int eq_split(const PointArg& z, int hi, int lo, const ef& scale, ef* out_hi, ef* out_lo, hipStream_t st);
+// Same with the point (hi + lo coordinates) and the scale in device memory.
+int eq_split_dev(const ef* z, const ef* scale, int hi, int lo, ef* out_hi, ef* out_lo, hipStream_t st);
 int eq_outer(const ef* Ehi, const ef* Elo, int lo, u64 n, ef* out, hipStream_t st);
 int eq_dot_base(const u32* f, int nf, int sb, int lo, const EqTables& tabs, int K, ef* partial, int nblocks, ef* out,
                 hipStream_t st);
 int eq_dot_ext(const ef* f, int nf, int sb, int lo, const EqTables& tabs, int K, ef* partial, int nblocks, ef* out,
                hipStream_t st);
 int eq_combine(ef* Wt, u64 n, int lo, const EqTables& tabs, int K, int accumulate, hipStream_t st);
-int select_combine(ef* Wt, int m, const u32* d_z, const ef* d_g, int nsel, ef* A, u32* B, hipStream_t st);
+// With d_nsel given, the count is read from device memory and nsel is its
+// maximum (A and B sized for it).
+int select_combine(ef* Wt, int m, const u32* d_z, const ef* d_g, int nsel, const u32* d_nsel, ef* A, u32* B,
+                   hipStream_t st);
 int compress_multi(const u32* f, int n, int l, const ef* d_eqrs, ef* out, hipStream_t st);
 int base_to_ext(const u32* f, u64 n, ef* out, hipStream_t st);
 int sc_coeffs(const ef* e, const ef* w, u64 h, ef* partial, ef* c0c2, hipStream_t st);

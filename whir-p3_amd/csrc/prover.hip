@@ -138,6 +138,12 @@ struct whirgpu_ctx {
   DevChallenger* d_chal_init = nullptr;
   ScLog* h_sclog = nullptr;   // coherent host memory; the transcript kernel logs into it
   ScLog* d_sclog = nullptr;   // device address of h_sclog
+  // device transcript of a round's steps before its sumcheck
+  RoundLog* h_rlog = nullptr;   // coherent host memory
+  RoundLog* d_rlog = nullptr;   // device address of h_rlog
+  DevChallenger* d_rchal = nullptr;   // two states the round's steps alternate between
+  RoundBufs rb{};
+  void* d_rbuf = nullptr;       // backing memory of rb
   int grind_blocks = 256;
   double timings[7] = {0};
   // live kernel timing: HIP events recorded on `st` around each launch
@@ -458,22 +464,35 @@ ef sumcheck_round(whirgpu_ctx* c, SumcheckState& s, const ChallengeFn& next) {
 // device's, so the proof and the challenger state are exactly those of the
 // host-driven rounds (SumcheckData::observe_and_sample, src/whir/proof.rs:163-193;
 // ProductPolynomial::round, src/sumcheck/product_polynomial.rs:307-364).
-std::vector<ef> device_sumcheck(whirgpu_ctx* c, SumcheckState& s, int rounds, int pow_bits, whir::SumcheckData& sd,
-                                DuplexChallenger& ch) {
-  std::vector<ef> rs;
-  if (rounds <= 0) return rs;
-  if (rounds > whirgpu_ctx::SC_LOG_CAP) bad("too many sumcheck rounds in one phase", WHIRGPU_BAD_STATE);
-  if (s.m < rounds) bad("sumcheck round on a constant polynomial", WHIRGPU_BAD_STATE);
+struct ScLaunch {
+  int rounds = 0;
+  int pow_bits = 0;
+  std::vector<size_t> grind_recs;
+};
+
+// Uploads the host challenger as the device phase's starting state.
+void upload_challenger(whirgpu_ctx* c, const DuplexChallenger& ch) {
   DevChallenger dc{};
   for (int i = 0; i < 16; i++) dc.sponge[i] = ch.sponge[i];
   for (int i = 0; i < ch.in_len; i++) dc.in[i] = ch.in[i];
   for (int i = 0; i < ch.out_len; i++) dc.out[i] = ch.out[i];
   dc.in_len = (u32)ch.in_len;
   dc.out_len = (u32)ch.out_len;
-  // h_sclog and h_chal are free: every phase ends with a synchronization
-  // before its log is read.
+  // h_chal is free: every phase ends with a synchronization before the next
+  // one uploads its state.
   *c->h_chal = dc;
-  std::vector<size_t> grind_recs;
+}
+
+// Launches `rounds` device-transcript sumcheck rounds starting from the
+// challenger at cin (device-visible memory).
+ScLaunch sumcheck_launch(whirgpu_ctx* c, SumcheckState& s, int rounds, int pow_bits, const DevChallenger* cin) {
+  ScLaunch L;
+  L.rounds = rounds;
+  L.pow_bits = pow_bits;
+  if (rounds <= 0) return L;
+  if (rounds > whirgpu_ctx::SC_LOG_CAP) bad("too many sumcheck rounds in one phase", WHIRGPU_BAD_STATE);
+  if (s.m < rounds) bad("sumcheck round on a constant polynomial", WHIRGPU_BAD_STATE);
+  std::vector<size_t>& grind_recs = L.grind_recs;
   // Each launch reads the challenger from `cin` and writes the next state to
   // the other buffer; coefficient partials and grind results alternate too,
   // since every block of a launch reads what block 0 of the same launch
@@ -481,7 +500,6 @@ std::vector<ef> device_sumcheck(whirgpu_ctx* c, SumcheckState& s, int rounds, in
   DevChallenger* chal[2] = {c->d_chal, c->d_chal + 1};
   ef* part[2] = {c->d_partial, c->d_partial + 4096};
   u32* gres[2] = {c->d_grind, c->d_grind + 64};
-  const DevChallenger* cin = c->d_chal_init;
   auto next_state = [&]() { return cin == chal[0] ? chal[1] : chal[0]; };
   int pp = 0;
   unsigned nb = 0;
@@ -502,7 +520,8 @@ std::vector<ef> device_sumcheck(whirgpu_ctx* c, SumcheckState& s, int rounds, in
       // sized to the expected search (16 * 2^bits candidates), one pass of
       // it at most, so short grinds leave the device to other work.
       const u64 want = ((16ull << pow_bits) + 255) / 256;
-      const int gblocks = (int)std::max<u64>(1, std::min<u64>((u64)c->grind_blocks, want));
+      const int gblocks =
+          pow_bits <= 12 ? c->grind_blocks : (int)std::max<u64>(1, std::min<u64>((u64)c->grind_blocks, want));
       Span sp(c, ST_GRIND, 0);
       KCHK(grind_range_tr(c->d_c16, tr, pow_bits, 0, P, gblocks, c->st));
       sp.close();
@@ -522,7 +541,16 @@ std::vector<ef> device_sumcheck(whirgpu_ctx* c, SumcheckState& s, int rounds, in
   }
   // d_partial is scratch for other kernels between phases
   s.invalidate();
-  c->sync();
+  return L;
+}
+
+// After a synchronization: replays the logged rounds on the host challenger
+// and returns the challenges.
+std::vector<ef> sumcheck_replay(whirgpu_ctx* c, SumcheckState& s, const ScLaunch& la, whir::SumcheckData& sd,
+                                DuplexChallenger& ch) {
+  std::vector<ef> rs;
+  const int rounds = la.rounds, pow_bits = la.pow_bits;
+  const std::vector<size_t>& grind_recs = la.grind_recs;
   std::vector<ScLog> log(c->h_sclog, c->h_sclog + rounds);
   for (int i = 0; i < rounds; i++) {
     const ScLog& L = log[i];
@@ -541,6 +569,17 @@ std::vector<ef> device_sumcheck(whirgpu_ctx* c, SumcheckState& s, int rounds, in
     rs.push_back(r);
   }
   return rs;
+}
+
+// `rounds` sumcheck rounds with the transcript on the device, from the host
+// challenger's state.
+std::vector<ef> device_sumcheck(whirgpu_ctx* c, SumcheckState& s, int rounds, int pow_bits, whir::SumcheckData& sd,
+                                DuplexChallenger& ch) {
+  if (rounds <= 0) return {};
+  upload_challenger(c, ch);
+  ScLaunch la = sumcheck_launch(c, s, rounds, pow_bits, c->d_chal_init);
+  c->sync();
+  return sumcheck_replay(c, s, la, sd, ch);
 }
 
 // Adds sum_i scale_i * eq(z_i, x) to the weights, four points per launch.
@@ -655,6 +694,9 @@ int ilog2u(u64 x) {
 
 // Opens `idx` on a committed matrix and returns the openings plus each row
 // as extension elements.
+void parse_openings(whirgpu_ctx* c, const whirgpu_prover_data* pd, int nq, std::vector<whir::QueryOpening>* out,
+                    std::vector<std::vector<ef>>* rows_ef);
+
 void open_rows(whirgpu_ctx* c, const whirgpu_prover_data* pd, const std::vector<u64>& idx,
                std::vector<whir::QueryOpening>* out, std::vector<std::vector<ef>>* rows_ef) {
   int nq = (int)idx.size();
@@ -666,6 +708,15 @@ void open_rows(whirgpu_ctx* c, const whirgpu_prover_data* pd, const std::vector<
   memcpy(c->h_idx, idx.data(), idx.size() * sizeof(u64));
   KCHK(merkle_open(pd->rows.p, pd->row_len, pd->tree.p, pd->height, c->d_idx, nq, c->d_open, c->st));
   c->sync();
+  parse_openings(c, pd, nq, out, rows_ef);
+}
+
+// The openings of nq queries written to h_open by an opening kernel (after a
+// synchronization), as proof entries and rows of extension elements.
+void parse_openings(whirgpu_ctx* c, const whirgpu_prover_data* pd, int nq, std::vector<whir::QueryOpening>* out,
+                    std::vector<std::vector<ef>>* rows_ef) {
+  const int depth = ilog2u(pd->height);
+  const size_t per = (size_t)pd->row_len + 8ull * depth;
   const u32* h = c->h_open;
   for (int q = 0; q < nq; q++) {
     const u32* b = h + per * q;
@@ -716,6 +767,157 @@ Digest read_root(whirgpu_ctx* c, const whirgpu_prover_data* pd) {
 
 void observe_digest(DuplexChallenger& ch, const Digest& d) {
   for (u32 x : d) ch.observe(x);
+}
+
+// Whether a round's steps after its commitment fit the device transcript's
+// buffers (otherwise the host drives them).
+bool device_round_fits(const whirgpu_ctx* c, const whir::RoundConfig& rp, int m, const whirgpu_prover_data* prev) {
+  const int lo = std::min(8, m), hi = m - lo;
+  if (!prev || m < 1 || m > RT_MAX_VARS) return false;
+  if (rp.ood_samples > RT_MAX_OOD || rp.num_queries < 1 || rp.num_queries > RT_MAX_QUERIES) return false;
+  if ((1ull << hi) + (1ull << lo) > c->tab_cap) return false;
+  const size_t per = (size_t)prev->row_len + 8ull * ilog2u(prev->height);
+  return per * (size_t)rp.num_queries <= c->open_cap;
+}
+
+// A round's steps from its commitment's root to the end of its sumcheck
+// (prover/mod.rs:192-303) with the transcript on the device: the OOD samples
+// and their answers, the proof of work, the STIR queries and their openings,
+// gamma, the weight updates and the sumcheck run as one stream of kernels,
+// each reading its challenger from the previous one, with one
+// synchronization at the end.  The host then replays every step on its own
+// challenger and requires each challenge, answer and query index to equal
+// the device's, so the proof and the challenger state are those of the
+// host-driven steps.  Returns the sumcheck's challenges.
+std::vector<ef> device_round(whirgpu_ctx* c, SumcheckState& s, const whir::WhirConfig& cfg, int round, int kn,
+                             const whirgpu_prover_data* pd, const whirgpu_prover_data* prev,
+                             const std::vector<ef>& fold_rand, whir::WhirRoundProof& R, DuplexChallenger& ch) {
+  const auto& rp = cfg.round_parameters[round];
+  const int m = s.m;
+  const int n_ood = rp.ood_samples;
+  const int lo = std::min(8, m), hi = m - lo;
+  upload_challenger(c, ch);
+  DevChallenger* rch[2] = {c->d_rchal, c->d_rchal + 1};
+  int ri = 0;
+  const DevChallenger* cin = c->d_chal_init;
+  const u32* obs = pd->tree.p + (2 * pd->height - 2) * 8;   // the root
+  int n_obs = 8;
+  ef* log_obs = nullptr;
+  for (int i = 0; i < n_ood; i++) {
+    DevChallenger* cout = rch[ri];
+    ri ^= 1;
+    const StepTr st{cin, cout, obs, n_obs, log_obs};
+    ef* tab = c->d_tabs;
+    KCHK(ood_eq_tables(st, &c->d_rlog->y[i], c->rb.pts + (size_t)i * RT_MAX_VARS, m, hi, lo, tab, tab + (1ull << hi),
+                       c->d_c16, c->st));
+    EqTables t{};
+    t.hi[0] = tab;
+    t.lo[0] = tab + (1ull << hi);
+    {
+      Span sp(c, ST_EQDOT, 16.0 * (double)(1ull << m));
+      KCHK(eq_dot_ext(s.e.p, m, 0, lo, t, 1, c->d_partial, dot_blocks(m - lo, 0), c->rb.ev + i, c->st));
+    }
+    cin = cout;
+    obs = reinterpret_cast<const u32*>(c->rb.ev + i);
+    n_obs = 4;
+    log_obs = &c->d_rlog->ev[i];
+  }
+  size_t grind_rec = SIZE_MAX;
+  if (rp.pow_bits > 0) {
+    HIPCHK(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(c->rb.grind), (int)P, 1, c->st));
+    DevChallenger* cout = rch[ri];
+    ri ^= 1;
+    ScTr tr{};
+    tr.ch_in = cin;
+    tr.ch_out = cout;
+    tr.grind = c->rb.grind;
+    tr.pow_bits = rp.pow_bits;
+    tr.obs = obs;
+    tr.n_obs = n_obs;
+    tr.log_obs = log_obs;
+    const u64 want = ((16ull << rp.pow_bits) + 255) / 256;
+    const int gblocks =
+        rp.pow_bits <= 12 ? c->grind_blocks : (int)std::max<u64>(1, std::min<u64>((u64)c->grind_blocks, want));
+    Span sp(c, ST_GRIND, 0);
+    KCHK(grind_range_tr(c->d_c16, tr, rp.pow_bits, 0, P, gblocks, c->st, TR_OBS_GRIND));
+    sp.close();
+    if (c->prof) grind_rec = c->recs.size() - 1;
+    cin = cout;
+    obs = nullptr;
+    n_obs = 0;
+    log_obs = nullptr;
+  }
+  {
+    DevChallenger* cout = rch[ri];
+    ri ^= 1;
+    const StepTr st{cin, cout, obs, n_obs, log_obs};
+    KCHK(round_queries(st, c->rb.grind, rp.pow_bits, rp.domain_size, cfg.ff(round), rp.num_queries,
+                       rp.folded_domain_gen, n_ood, c->d_rlog, c->rb, c->d_c16, c->st));
+    cin = cout;
+  }
+  // openings of the previous commitment into h_open, read after the round
+  KCHK(merkle_open_dev(prev->rows.p, prev->row_len, prev->tree.p, prev->height, c->rb.idx, c->rb.nq,
+                       rp.num_queries, c->d_open, c->st));
+  // Constraint::combine_packed (src/whir/constraints/mod.rs:234-248): the OOD
+  // points' eq weighted by gamma^i, then the queries' select terms
+  for (int b = 0; b < n_ood; b += 4) {
+    EqTables t{};
+    const int kk = std::min(4, n_ood - b);
+    for (int k = 0; k < kk; k++) {
+      ef* base = c->d_tabs + (size_t)k * c->tab_cap;
+      KCHK(eq_split_dev(c->rb.pts + (size_t)(b + k) * RT_MAX_VARS, c->rb.gpow + b + k, hi, lo, base,
+                        base + (1ull << hi), c->st));
+      t.hi[k] = base;
+      t.lo[k] = base + (1ull << hi);
+    }
+    Span sp(c, ST_COMBINE, 32.0 * (double)(1ull << m));
+    KCHK(eq_combine(s.w.p, 1ull << m, lo, t, kk, 1, c->st));
+  }
+  {
+    const int ns = rp.num_queries;
+    Buf<ef> A = c->alloc<ef>((size_t)ns << (m - m / 2));
+    Buf<u32> B = c->alloc<u32>((size_t)ns << (m / 2));
+    {
+      Span sp(c, ST_COMBINE, 32.0 * (double)(1ull << m));
+      KCHK(select_combine(s.w.p, m, c->rb.selz, c->rb.gpow + n_ood, ns, c->rb.nq, A.p, B.p, c->st));
+    }
+    c->release(A);   // reuse is ordered on the same stream
+    c->release(B);
+  }
+  s.invalidate();
+  ScLaunch la = sumcheck_launch(c, s, kn, rp.folding_pow_bits, cin);
+  c->sync();
+
+  // replay on the host challenger
+  R.commitment = read_root(c, pd);
+  observe_digest(ch, R.commitment);
+  const RoundLog& L = *c->h_rlog;
+  for (int i = 0; i < n_ood; i++) {
+    const ef y = ch.sample_ef();
+    if (!ef_eq(y, L.y[i])) bad("device round transcript diverged (OOD challenge)", WHIRGPU_BAD_STATE);
+    ch.observe_ef(L.ev[i]);
+    R.ood_answers.push_back(L.ev[i]);
+  }
+  if (rp.pow_bits > 0) {
+    if (L.err) bad("device round transcript: no valid proof-of-work witness", WHIRGPU_GRIND_FAILED);
+    if (!ch.check_witness(rp.pow_bits, L.witness)) bad("device round transcript: witness rejected", WHIRGPU_GRIND_FAILED);
+    R.pow_witness = L.witness;
+    if (grind_rec != SIZE_MAX) c->recs[grind_rec].work = (double)from_monty(L.witness) + 1;
+  }
+  ch.sample();
+  const auto idx = whir::get_challenge_stir_queries(rp.domain_size, cfg.ff(round), rp.num_queries, ch);
+  if (idx.size() != L.nq) bad("device round transcript diverged (query count)", WHIRGPU_BAD_STATE);
+  for (size_t q = 0; q < idx.size(); q++)
+    if (idx[q] != L.idx[q]) bad("device round transcript diverged (query index)", WHIRGPU_BAD_STATE);
+  std::vector<std::vector<ef>> rows;
+  parse_openings(c, prev, (int)idx.size(), &R.queries, &rows);
+  const ef gamma = ch.sample_ef();
+  if (!ef_eq(gamma, L.gamma)) bad("device round transcript diverged (gamma)", WHIRGPU_BAD_STATE);
+  const auto gp = whir::powers(gamma, (size_t)n_ood + idx.size());
+  for (int i = 0; i < n_ood; i++) s.sum = ef_add(s.sum, ef_mul(gp[i], L.ev[i]));
+  for (size_t j = 0; j < idx.size(); j++)
+    s.sum = ef_add(s.sum, ef_mul(gp[n_ood + j], whir::mle_host(rows[j], fold_rand)));
+  return sumcheck_replay(c, s, la, R.sumcheck, ch);
 }
 
 double now_ms() {
@@ -885,6 +1087,22 @@ int whirgpu_create(int device, const whirgpu_params* params, const whirgpu_posei
     coherent(reinterpret_cast<void**>(&c->h_sclog), reinterpret_cast<void**>(&c->d_sclog),
              whirgpu_ctx::SC_LOG_CAP * sizeof(ScLog));
     coherent(reinterpret_cast<void**>(&c->h_chal), reinterpret_cast<void**>(&c->d_chal_init), sizeof(DevChallenger));
+    coherent(reinterpret_cast<void**>(&c->h_rlog), reinterpret_cast<void**>(&c->d_rlog), sizeof(RoundLog));
+    HIPCHK(hipMalloc(&c->d_rchal, 2 * sizeof(DevChallenger)));
+    {
+      const size_t pts = sizeof(ef) * RT_MAX_OOD * RT_MAX_VARS, ev = sizeof(ef) * RT_MAX_OOD,
+                   idx = sizeof(u64) * RT_MAX_QUERIES, selz = sizeof(u32) * RT_MAX_QUERIES,
+                   gp = sizeof(ef) * (RT_MAX_OOD + RT_MAX_QUERIES), gr = sizeof(u32) * 64, nq = 256;
+      HIPCHK(hipMalloc(&c->d_rbuf, pts + ev + idx + selz + gp + gr + nq));
+      char* b = static_cast<char*>(c->d_rbuf);
+      c->rb.pts = reinterpret_cast<ef*>(b);
+      c->rb.ev = reinterpret_cast<ef*>(b + pts);
+      c->rb.idx = reinterpret_cast<u64*>(b + pts + ev);
+      c->rb.selz = reinterpret_cast<u32*>(b + pts + ev + idx);
+      c->rb.gpow = reinterpret_cast<ef*>(b + pts + ev + idx + selz);
+      c->rb.grind = reinterpret_cast<u32*>(b + pts + ev + idx + selz + gp);
+      c->rb.nq = reinterpret_cast<u32*>(b + pts + ev + idx + selz + gp + gr);
+    }
     HIPCHK(hipMemcpy(c->d_c16, &c->c16, sizeof(P2C16), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&c->d_c24, sizeof(P2C24)));
     HIPCHK(hipMemcpy(c->d_c24, &c->c24, sizeof(P2C24), hipMemcpyHostToDevice));
@@ -906,10 +1124,10 @@ int whirgpu_destroy(whirgpu_ctx* c) {
   if (c->st) (void)hipStreamSynchronize(c->st);
   c->pool.release_all();
   for (void* h : {(void*)c->h_small, (void*)c->h_idx, (void*)c->h_open, (void*)c->h_root, (void*)c->h_sclog,
-                  (void*)c->h_chal})
+                  (void*)c->h_chal, (void*)c->h_rlog})
     if (h) (void)hipHostFree(h);
   for (void* p : {(void*)c->d_tw, (void*)c->d_tabs, (void*)c->d_partial, (void*)c->d_grind, (void*)c->d_gslots,
-                  (void*)c->d_c16, (void*)c->d_c24, (void*)c->d_chal})
+                  (void*)c->d_c16, (void*)c->d_c24, (void*)c->d_chal, (void*)c->d_rchal, c->d_rbuf})
     if (p) (void)hipFree(p);
   if (c->h_pin) (void)hipHostFree(c->h_pin);
   if (c->h_ring) (void)hipHostFree(c->h_ring);
@@ -1191,6 +1409,12 @@ int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs,
       }
       commit_tree(c, pd);
       if (tmp.p) c->release(tmp);
+      if (device_round_fits(c, rp, m, prev)) {
+        fold_rand = device_round(c, s, cfg, round, kn, pd, prev, fold_rand, R, ch);
+        whirgpu_prover_data_destroy(prev);
+        prev = pd;
+        continue;
+      }
       R.commitment = read_root(c, pd);
       observe_digest(ch, R.commitment);
       // OOD samples (prover/mod.rs:192-205)
@@ -1235,7 +1459,7 @@ int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs,
         Buf<u32> B = c->alloc<u32>(ns << lo);
         {
           Span sp(c, ST_COMBINE, 32.0 * (double)(1ull << m));
-          KCHK(select_combine(s.w.p, m, d_z, d_g, (int)ns, A.p, B.p, c->st));
+          KCHK(select_combine(s.w.p, m, d_z, d_g, (int)ns, nullptr, A.p, B.p, c->st));
         }
         c->release(A);   // reuse is ordered on the same stream
         c->release(B);

@@ -102,6 +102,15 @@ __device__ __forceinline__ void tr_prologue(int mode, const ScTr& a, const P2C16
   ch.load(*a.ch_in);
   const bool lead = blockIdx.x == 0 && blockIdx.y == 0;
   u32 err = 0;
+  if (mode == TR_OBS_GRIND) {
+    for (int k = 0; k < a.n_obs; k++) ch.observe(a.obs[k]);
+    if (lead && t == 0 && a.n_obs == 4 && a.log_obs) *a.log_obs = ef{{a.obs[0], a.obs[1], a.obs[2], a.obs[3]}};
+    if (t < 16) gb->s[t] = t < ch.in_len ? in[t] : sp[t];
+    if (t == 0) gb->wpos = (u32)ch.in_len;
+    __syncthreads();
+    if (lead) ch.store(a.ch_out);
+    return;
+  }
   if (mode == TR_COEFFS_GRIND || mode == TR_COEFFS_SAMPLE) {
     ef acc[2] = {ef_zero(), ef_zero()};
     for (int b = t; b < a.nb; b += 256) {
