@@ -105,8 +105,13 @@ def test_poseidon2_constants_and_permutation_match_oracle(width):
     assert list(getattr(perms, pre + "external_terminal")) == list(et)
     assert list(getattr(perms, pre + "internal")) == list(it)
     rng = np.random.default_rng(width)
-    for _ in range(20):
-        st = to_monty(rng.integers(0, 0x7F000001, width, dtype=np.uint64))
+    # random states, then the edges of the lazy reductions' ranges: all p - 1,
+    # all 0, alternating, and values next to p / 2
+    states = [to_monty(rng.integers(0, 0x7F000001, width, dtype=np.uint64)) for _ in range(200)]
+    states += [np.full(width, 0x7F000000, np.uint32), np.zeros(width, np.uint32),
+               np.array([0x7F000000 if i % 2 else 0 for i in range(width)], np.uint32),
+               np.array([0x3F800000 + (i % 3) for i in range(width)], np.uint32)]
+    for st in states:
         a = st.copy()
         b = st.copy()
         W.check(W.lib().whirgpu_poseidon2_permute(ctypes.byref(perms), width, u32p(a)))

@@ -151,10 +151,19 @@ __global__ void __launch_bounds__(256) k_eq_dot(const T* __restrict__ f, int nf,
   }
 }
 
+// Sums of Montgomery products are kept unreduced in 64 bits: fold32 maps a
+// sum t < 2^64 to (t >> 32) (2^25 - 2) + (t mod 2^32) < 1.45e17 (2^32 = 2^25 - 2
+// mod p), and a folded sum plus up to four products (each below p^2 < 4.54e18)
+// stays below 2^64.  One Montgomery reduction ends a sum (the reduction of a
+// sum of products is the Montgomery form of the sum).
+__device__ __forceinline__ u64 fold32(u64 t) {
+  return ((u64)(u32)(t >> 32)) * 33554430u + (u32)t;   // (t >> 32) (2^32 mod p) + (t mod 2^32)
+}
+
 // Base-field form of k_eq_dot for lo >= 2: a thread reads four consecutive
 // entries of a row (one 16-byte load) and keeps, per entry and coefficient, an
-// unreduced 64-bit sum of hi[k][r] * f products (each below p^2), reduced once
-// every four rows (a sum of four stays below 4p^2, what redc_wide takes).
+// unreduced 64-bit sum of hi[k][r] * f products, folded every four rows and
+// reduced once at the end.
 template <int K>
 __global__ void __launch_bounds__(256) k_eq_dot4(const u32* __restrict__ f, int nf, int sb, int lo, EqTables tabs,
                                                  ef* __restrict__ partial) {
@@ -181,18 +190,13 @@ __global__ void __launch_bounds__(256) k_eq_dot4(const u32* __restrict__ f, int 
 #pragma unroll
       for (int c = 0; c < 4; c++) lz[k][i][c] = 0;
     }
-  auto flush = [&]() {
+  auto fold = [&]() {
 #pragma unroll
     for (int k = 0; k < K; k++)
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        ef v{{redc_wide(lz[k][i][0]), redc_wide(lz[k][i][1]), redc_wide(lz[k][i][2]), redc_wide(lz[k][i][3])}};
-        // the products carry the Montgomery factor of both operands: one redc
-        // per product, as ef_mul_base does
-        acc[k][i] = ef_add(acc[k][i], v);
+      for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int c = 0; c < 4; c++) lz[k][i][c] = 0;
-      }
+        for (int c = 0; c < 4; c++) lz[k][i][c] = fold32(lz[k][i][c]);
   };
   // Four rows per step, all loads issued before the arithmetic.
   u64 r = r0 + (t >> (lo - 2));
@@ -224,7 +228,7 @@ __global__ void __launch_bounds__(256) k_eq_dot4(const u32* __restrict__ f, int 
 #pragma unroll
           for (int c = 0; c < 4; c++) lz[k][i][c] += (u64)e[k][j].c[c] * vv[i];
     }
-    flush();
+    fold();
   }
   for (; r < r1; r += rows_per_iter) {   // at most three rows left
     const uint4 v = *reinterpret_cast<const uint4*>(fs + (r << lo));
@@ -238,7 +242,14 @@ __global__ void __launch_bounds__(256) k_eq_dot4(const u32* __restrict__ f, int 
         for (int c = 0; c < 4; c++) lz[k][i][c] += (u64)e.c[c] * vv[i];
     }
   }
-  flush();
+  fold();
+  // the products carry the Montgomery factor of both operands: one redc per
+  // sum, as ef_mul_base does per product
+#pragma unroll
+  for (int k = 0; k < K; k++)
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+      acc[k][i] = ef{{redc(lz[k][i][0]), redc(lz[k][i][1]), redc(lz[k][i][2]), redc(lz[k][i][3])}};
   ef tot[K];
 #pragma unroll
   for (int k = 0; k < K; k++) {
@@ -341,15 +352,8 @@ __global__ void k_select_tables(const u32* __restrict__ z, const ef* __restrict_
 }
 
 // W[x] += sum_j A[j][x >> lo] * B[j][x & mask]
-// Wt[x] += sum_j A[j][x >> lo] * B[j][x & mask].  The products of each EF
-// component are summed unreduced: a 64-bit sum below 2^56.1 plus three
-// products (each below p^2) stays below 2^64, and is folded back below 2^56.1
-// with 2^32 = 2^25 - 2 (mod p); one Montgomery reduction per component ends
-// the sum (a sum of Montgomery products reduces to the Montgomery form of
-// the sum).
-__device__ __forceinline__ u64 fold32(u64 t) {
-  return ((u64)(u32)(t >> 32)) * 33554430u + (u32)t;   // (t >> 32) (2^32 mod p) + (t mod 2^32)
-}
+// Wt[x] += sum_j A[j][x >> lo] * B[j][x & mask], each EF component's
+// products summed unreduced (fold32) three at a time.
 
 __global__ void __launch_bounds__(256) k_select_combine(ef* __restrict__ Wt, u64 n, int lo, int hi,
                                                         const ef* __restrict__ A, const u32* __restrict__ B,

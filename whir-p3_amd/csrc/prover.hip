@@ -336,7 +336,8 @@ void build_split_eq(whirgpu_ctx* c, const std::vector<ef>& z, int lo, const ef& 
 
 int dot_blocks(int rows_log, int seg_bits) {
   long rows = 1L << rows_log;
-  long want = 2048L >> seg_bits;
+  static long tmp_want = getenv("TMP_DOT_WANT") ? atol(getenv("TMP_DOT_WANT")) : 2048;   // TEMPORARY experiment
+  long want = tmp_want >> seg_bits;
   if (want < 1) want = 1;
   return (int)std::min(rows, want);
 }
