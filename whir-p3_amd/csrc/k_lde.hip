@@ -18,6 +18,10 @@
 //           once, written to whole output rows 2^R (k1 + L1 k2) + i0.
 // Each block keeps its lines in LDS in bit-reversed order and runs radix-2
 // decimation-in-time stages there.
+#include <mutex>
+#include <set>
+#include <tuple>
+
 #include "kcommon.h"
 #include "kernels.h"
 
@@ -229,12 +233,22 @@ __global__ void __launch_bounds__(256) k_lde_pass2(const T* __restrict__ src, T*
   }
 }
 
-// Dynamic LDS above 64 KiB must be opted into per kernel.
+// Dynamic LDS above 64 KiB must be opted into per kernel and device; the
+// opt-in is made once (hipFuncSetAttribute takes a runtime-wide lock, which
+// concurrent provers would otherwise contend for on every launch).
 template <typename K>
 static void allow_lds(K kernel, size_t bytes) {
-  if (bytes > 64 * 1024)
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)bytes);
+  if (bytes <= 64 * 1024) return;
+  static std::mutex mu;
+  static std::set<std::tuple<const void*, int, size_t>> done;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  const auto key = std::make_tuple(reinterpret_cast<const void*>(kernel), dev, bytes);
+  std::lock_guard<std::mutex> lock(mu);
+  if (done.count(key)) return;
+  if (hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)bytes) == hipSuccess)
+    done.insert(key);
 }
 
 static size_t lds_bytes(int lg, int lines, size_t esize) {
