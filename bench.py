@@ -34,10 +34,10 @@ VALU_OPS_PER_PERM = 4485
 VALU_STAGES = ("merkle_leaves", "merkle_tree", "grind")
 # Kernels of each stage, for the HBM traffic measured with rocprofv3 --pmc
 # (FETCH_SIZE and WRITE_SIZE per dispatch, calibrated on a copy of known size).
-STAGE_KERNELS = {
-    "grind": ["k_grind"],
-    "merkle_leaves": ["k_leaf_hash<16>"],
-    "lde_base": ["k_lde_pass1<unsigned int, 10>", "k_lde_pass2<unsigned int, 10>"],
+STAGE_KERNELS = {   # per stage: the kernels one launch group runs, each a list of alternatives
+    "grind": [["k_grind", "k_grind_wide"]],
+    "merkle_leaves": [["k_leaf_hash<16>"]],
+    "lde_base": [["k_lde_pass1<unsigned int, 10>"], ["k_lde_pass2<unsigned int, 10>"]],
 }
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic_n24.json")
 
@@ -82,13 +82,27 @@ def sumcheck_eq_bench(W, args, n=26, l0=3, reps=3):
     for _ in range(reps):
         run()
     wall = (time.perf_counter() - t0) * 1e3 / reps
+    recs = cfg.profile_records()
     st = cfg.profile_read()
     cfg.profile(False)
+
+    def large(stage, min_bytes=64 << 20):
+        """The stage's launch groups moving at least min_bytes (the rounds whose
+        time is bandwidth, not launch latency), against the HBM peak."""
+        sel = [(ms, w) for s_, ms, w in recs if s_ == stage and w >= min_bytes]
+        if not sel:
+            return None
+        ms = sum(m for m, _ in sel)
+        by = sum(w for _, w in sel)
+        gbs = by / (ms * 1e-3) / 1e9
+        return {"launches": len(sel), "bytes": round(by), "ms": round(ms, 4), "achieved": round(gbs, 1),
+                "frac": round(gbs / HBM_PEAK_GBS, 4)}
     sc_ms = sum(st[k]["ms"] for k in ("sumcheck", "eq_dot", "combine", "svo_fold")) / reps
     out = {
         "workload": "eq expansion 2^26 EF4 + 26-round sumcheck of f*eq (f base field 2^26), 3 SVO rounds",
         "eq_expand": hbm_roof(st["eq_expand"], "eq_expand"),
         "sumcheck_rounds": hbm_roof(st["sumcheck"], "sumcheck fold+coefficients"),
+        "sumcheck_rounds_64MiB_plus": large("sumcheck"),
         "svo_partials": hbm_roof(st["eq_dot"], "SVO partial sums"),
         "svo_fold": hbm_roof(st["svo_fold"], "SVO 3-variable fold"),
         "weights": hbm_roof(st["combine"], "eq weights after SVO"),
@@ -267,10 +281,21 @@ def main():
     cfg.profile(False)
 
     def stage_traffic(name):
-        ks = STAGE_KERNELS.get(name)
-        if not ks or any(k not in traffic for k in ks):
+        """HBM bytes per launch of the stage: per-dispatch traffic of its kernels
+        (a launch group runs each once; alternatives such as the one-lane and
+        lane-parallel grinds are averaged by their dispatch counts)."""
+        groups = STAGE_KERNELS.get(name)
+        if not groups:
             return None
-        return round(sum(traffic[k]["fetch_bytes"] + traffic[k]["write_bytes"] for k in ks))
+        total = 0.0
+        for alts in groups:
+            alts = [k for k in alts if k in traffic]
+            if not alts:
+                return None
+            n = sum(traffic[k]["dispatches"] for k in alts)
+            total += sum((traffic[k]["fetch_bytes"] + traffic[k]["write_bytes"]) * traffic[k]["dispatches"]
+                         for k in alts) / n
+        return round(total)
 
     def roof(name):
         s = stages[name]

@@ -161,6 +161,9 @@ int whirgpu_last_timings(const whirgpu_ctx* ctx, double* out7);
  * (bytes), eq expansion (bytes written). */
 int whirgpu_profile_enable(whirgpu_ctx* ctx, int on);
 int whirgpu_profile_read(whirgpu_ctx* ctx, double* out, size_t cap, size_t* len);
+/* The same records one launch group at a time, [stage index, device ms, work]
+ * each, without resetting them (call before whirgpu_profile_read). */
+int whirgpu_profile_records(whirgpu_ctx* ctx, double* out, size_t cap, size_t* len);
 
 /* Device memory for callers that keep inputs resident in HBM (the Rust side
  * would hold such a buffer between proofs).  kind: 0 host->device,

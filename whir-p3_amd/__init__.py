@@ -81,6 +81,8 @@ _SIGS = {
     "whirgpu_prover_data_destroy": (ctypes.c_int, [_VP]),
     "whirgpu_last_timings": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double)]),
     "whirgpu_profile_enable": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "whirgpu_profile_records": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double), ctypes.c_size_t,
+                                                ctypes.POINTER(ctypes.c_size_t)]),
     "whirgpu_profile_read": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double), ctypes.c_size_t,
                                             ctypes.POINTER(ctypes.c_size_t)]),
     "whirgpu_sumcheck_eq": (ctypes.c_int, [_VP, _VP, ctypes.c_uint32, U32P, ctypes.c_uint32, U32P,
@@ -259,6 +261,15 @@ class WhirConfig:
         v = list(out[:n.value])
         return {s: {"ms": v[3 * i], "launches": int(v[3 * i + 1]), "work": v[3 * i + 2]}
                 for i, s in enumerate(self.STAGES)}
+
+    def profile_records(self):
+        """Per launch group, without resetting: [(stage, ms, work)]."""
+        n = ctypes.c_size_t()
+        lib().whirgpu_profile_records(self.handle, None, 0, ctypes.byref(n))
+        out = (ctypes.c_double * max(1, n.value))()
+        check(lib().whirgpu_profile_records(self.handle, out, n.value, ctypes.byref(n)))
+        v = list(out[:n.value])
+        return [(self.STAGES[int(v[i])], v[i + 1], v[i + 2]) for i in range(0, len(v), 3)]
 
     def timings(self):
         out = (ctypes.c_double * 7)()

@@ -1524,6 +1524,23 @@ int whirgpu_profile_read(whirgpu_ctx* c, double* out, size_t cap, size_t* len) {
   });
 }
 
+int whirgpu_profile_records(whirgpu_ctx* c, double* out, size_t cap, size_t* len) {
+  return guarded([&] {
+    if (!c || !len) bad("null argument");
+    *len = 3 * c->recs.size();
+    if (!out || cap < *len) bad("buffer too small", WHIRGPU_BUFFER_TOO_SMALL);
+    use_device(c);
+    HIPCHK(hipStreamSynchronize(c->st));
+    for (size_t i = 0; i < c->recs.size(); i++) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, c->recs[i].a, c->recs[i].b));
+      out[3 * i] = c->recs[i].stage;
+      out[3 * i + 1] = ms;
+      out[3 * i + 2] = c->recs[i].work;
+    }
+  });
+}
+
 int whirgpu_device_alloc(int device, size_t bytes, void** out) {
   return guarded([&] {
     if (!out) bad("null argument");
