@@ -111,7 +111,7 @@ def test_merkle_root_and_openings(W, leaf_width, log_h, row_len):
         off += H >> l
 
 
-@pytest.mark.parametrize("bits", [1, 4, 9, 14, 17])
+@pytest.mark.parametrize("bits", [1, 4, 9, 12, 13, 14, 17, 18])
 @pytest.mark.parametrize("n_obs", [0, 3, 7, 8, 13])
 def test_grind_smallest_witness(W, bits, n_obs):
     cfg = ctx(W)

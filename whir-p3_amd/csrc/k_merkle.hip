@@ -350,6 +350,69 @@ __global__ void __launch_bounds__(256) k_grind_wide(const P2C16* __restrict__ cp
   }
 }
 
+// Long searches (at least LOCK_GRIND_BITS bits): two waves per SIMD issue
+// about 15% faster than one, but the older wave gets issue priority, so waves
+// left to themselves split each pass into a fast and a slow half (and a
+// witness in the slow half ends the search late).  Here a block holds both
+// waves of each SIMD of its CU (512 lanes) and passes a barrier after every
+// pass, so they advance together; the stop decision is block-wide: thread 0
+// loads the best witness one pass ahead and shares it through LDS (double
+// buffered, so a pass's write never meets the previous pass's reads).  The
+// candidates' base state comes from k_grind_base (transcript step) or the
+// host.
+constexpr int LOCK_GRIND_BITS = 17;
+#ifndef LOCK_NT_OVERRIDE
+constexpr int LOCK_NT = 512;
+#else
+constexpr int LOCK_NT = LOCK_NT_OVERRIDE;
+#endif
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_grind_lock(const P2C16* __restrict__ cp, Sponge16 base, int wpos,
+                                                   const GrindBase* gb, u32 mask, u32* result, u32 start,
+                                                   u32 end) {
+  __shared__ u32 sb[2];
+  if (gb) {
+#pragma unroll
+    for (int k = 0; k < 16; k++) base.s[k] = gb->s[k];
+    wpos = (int)gb->wpos;
+  }
+  const u32 t = threadIdx.x;
+  const u32 stride = gridDim.x * NT;
+  if (t == 0) sb[0] = best_known(result);
+  __syncthreads();
+  int it = 0;
+  for (u64 g = start;; g += stride, it ^= 1) {
+    const u64 first = g + (u64)blockIdx.x * NT;
+    if (first >= end || (u64)sb[it] <= first) break;   // block-uniform
+    u32 nb = 0;
+    if (t == 0) nb = best_known(result);
+    u32 off = 0;
+    asm volatile("" : "+s"(off));
+    const P2C16& c = cp[off];
+    const u64 w = first + t;
+    u32 s[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) s[k] = (k == wpos) ? to_monty((u32)w) : base.s[k];
+    permute_head<16>(s, c);
+    permute_internal<16>(s, c);
+    permute_tail<16>(s, c);
+    if (t == 0) sb[it ^ 1] = nb;
+    if (w < end && (from_monty(s[7]) & mask) == 0) atomicMin(result, (u32)w);
+    __syncthreads();
+  }
+}
+
+// The transcript step of a long search in one block: writes the candidates'
+// base state to gb_out (and, from block 0, the next challenger state and log).
+__global__ void __launch_bounds__(256) k_grind_base(const P2C16* __restrict__ cp, int tr_mode, ScTr tr,
+                                                    GrindBase* gb_out) {
+  __shared__ GrindBase gbs;
+  tr_prologue(tr_mode, tr, *cp, &gbs, nullptr);
+  if (threadIdx.x < 16) gb_out->s[threadIdx.x] = gbs.s[threadIdx.x];
+  if (threadIdx.x == 0) gb_out->wpos = gbs.wpos;
+}
+
 // Blocks of a short search: sixteen candidates each, enough for one pass over
 // sixteen times the expected search, at most `blocks` and the range.
 static unsigned wide_grind_blocks(int bits, u64 start, u64 end, int blocks) {
@@ -373,6 +436,13 @@ int grind_range(const P2C16* d_consts, const u32* state16, int wpos, int bits, u
                        d_result, (u32)start, (u32)end);
     return hipGetLastError() == hipSuccess ? 0 : -5;
   }
+  if (bits >= LOCK_GRIND_BITS) {
+    const unsigned nb = (unsigned)std::max<u64>(1, std::min<u64>((end - start + LOCK_NT - 1) / LOCK_NT,
+                                                                 (u64)std::max(blocks, 1)));
+    hipLaunchKernelGGL(k_grind_lock<LOCK_NT>, dim3(nb), dim3(LOCK_NT), 0, st, d_consts, b, wpos,
+                       (const GrindBase*)nullptr, mask, d_result, (u32)start, (u32)end);
+    return hipGetLastError() == hipSuccess ? 0 : -5;
+  }
   u64 need = (end - start + 255) / 256;
   unsigned nb = (unsigned)std::min<u64>(need, (u64)std::max(blocks, 1));
   hipLaunchKernelGGL(k_grind, dim3(nb), dim3(256), 0, st, d_consts, b, wpos, (int)TR_NONE, ScTr{}, mask, d_result,
@@ -390,6 +460,14 @@ int grind_range_tr(const P2C16* d_consts, const ScTr& tr, int bits, u64 start, u
     const unsigned nb = wide_grind_blocks(bits, start, end, blocks);
     hipLaunchKernelGGL(k_grind_wide, dim3(nb), dim3(256), 0, st, d_consts, b, 0, mode, tr, mask, tr.grind,
                        (u32)start, (u32)end);
+    return hipGetLastError() == hipSuccess ? 0 : -5;
+  }
+  if (bits >= LOCK_GRIND_BITS && tr.gbase) {
+    hipLaunchKernelGGL(k_grind_base, dim3(1), dim3(256), 0, st, d_consts, mode, tr, tr.gbase);
+    const unsigned nb = (unsigned)std::max<u64>(1, std::min<u64>((end - start + LOCK_NT - 1) / LOCK_NT,
+                                                                 (u64)std::max(blocks, 1)));
+    hipLaunchKernelGGL(k_grind_lock<LOCK_NT>, dim3(nb), dim3(LOCK_NT), 0, st, d_consts, b, 0,
+                       (const GrindBase*)tr.gbase, mask, tr.grind, (u32)start, (u32)end);
     return hipGetLastError() == hipSuccess ? 0 : -5;
   }
   u64 need = (end - start + 255) / 256;

@@ -63,6 +63,7 @@ struct ScTr {
   const u32* obs;               // TR_OBS_GRIND: words to observe first
   int n_obs;
   ef* log_obs;
+  GrindBase* gbase;             // long grinds: device memory for the candidates' base state
 };
 
 // A WHIR round's transcript steps between its commitment and its sumcheck,

@@ -142,6 +142,7 @@ struct whirgpu_ctx {
   RoundLog* h_rlog = nullptr;   // coherent host memory
   RoundLog* d_rlog = nullptr;   // device address of h_rlog
   DevChallenger* d_rchal = nullptr;   // two states the round's steps alternate between
+  GrindBase* d_gbase = nullptr;       // base state of a long grind's candidates
   RoundBufs rb{};
   void* d_rbuf = nullptr;       // backing memory of rb
   int grind_blocks = 256;
@@ -517,6 +518,7 @@ ScLaunch sumcheck_launch(whirgpu_ctx* c, SumcheckState& s, int rounds, int pow_b
     if (pow_bits > 0) {
       DevChallenger* cout = next_state();
       ScTr tr{cin, cout, c->d_sclog + i, part[pp], g, nullptr, (int)nb, pow_bits};
+      tr.gbase = c->d_gbase;
       // The range is all of [0, p), so the search always ends; the grid is
       // sized to the expected search (16 * 2^bits candidates), one pass of
       // it at most, so short grinds leave the device to other work.
@@ -836,6 +838,7 @@ std::vector<ef> device_round(whirgpu_ctx* c, SumcheckState& s, const whir::WhirC
     tr.obs = obs;
     tr.n_obs = n_obs;
     tr.log_obs = log_obs;
+    tr.gbase = c->d_gbase;
     const u64 want = ((16ull << rp.pow_bits) + 255) / 256;
     const int gblocks =
         rp.pow_bits <= 12 ? c->grind_blocks : (int)std::max<u64>(1, std::min<u64>((u64)c->grind_blocks, want));
@@ -1090,6 +1093,7 @@ int whirgpu_create(int device, const whirgpu_params* params, const whirgpu_posei
     coherent(reinterpret_cast<void**>(&c->h_chal), reinterpret_cast<void**>(&c->d_chal_init), sizeof(DevChallenger));
     coherent(reinterpret_cast<void**>(&c->h_rlog), reinterpret_cast<void**>(&c->d_rlog), sizeof(RoundLog));
     HIPCHK(hipMalloc(&c->d_rchal, 2 * sizeof(DevChallenger)));
+    HIPCHK(hipMalloc(&c->d_gbase, sizeof(GrindBase)));
     {
       const size_t pts = sizeof(ef) * RT_MAX_OOD * RT_MAX_VARS, ev = sizeof(ef) * RT_MAX_OOD,
                    idx = sizeof(u64) * RT_MAX_QUERIES, selz = sizeof(u32) * RT_MAX_QUERIES,
@@ -1128,7 +1132,8 @@ int whirgpu_destroy(whirgpu_ctx* c) {
                   (void*)c->h_chal, (void*)c->h_rlog})
     if (h) (void)hipHostFree(h);
   for (void* p : {(void*)c->d_tw, (void*)c->d_tabs, (void*)c->d_partial, (void*)c->d_grind, (void*)c->d_gslots,
-                  (void*)c->d_c16, (void*)c->d_c24, (void*)c->d_chal, (void*)c->d_rchal, c->d_rbuf})
+                  (void*)c->d_c16, (void*)c->d_c24, (void*)c->d_chal, (void*)c->d_rchal, c->d_rbuf,
+                  (void*)c->d_gbase})
     if (p) (void)hipFree(p);
   if (c->h_pin) (void)hipHostFree(c->h_pin);
   if (c->h_ring) (void)hipHostFree(c->h_ring);
