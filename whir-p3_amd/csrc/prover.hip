@@ -122,8 +122,14 @@ struct whirgpu_ctx {
   u32* h_open = nullptr;
   u32* d_open = nullptr;  // device address of h_open
   size_t open_cap = 0;
+  // Eight root slots, used in turn, so the next round's commitment can be
+  // enqueued before this round's root is read.
   u32* h_root = nullptr;
   u32* d_root = nullptr;  // device address of h_root
+  int root_next = 0;
+  // Recorded after a round's transcript steps and sumcheck: the host replays
+  // them once it has passed while the next commitment runs.
+  hipEvent_t ev_round = nullptr;
   // Grinding result slots for host-driven grinds, refilled with p in bulk.
   static constexpr int GRIND_SLOTS = 64;
   u32* d_gslots = nullptr;
@@ -245,6 +251,7 @@ struct whirgpu_prover_data {
   int row_len = 0;   // base elements per leaf
   bool ext = false;
   bool root_on_host = false;   // the last tree kernel wrote the root to the context's h_root
+  int root_slot = 0;           // at h_root + 8 * root_slot
 };
 
 struct whirgpu_proof {
@@ -335,10 +342,10 @@ void build_split_eq(whirgpu_ctx* c, const std::vector<ef>& z, int lo, const ef& 
   t->lo[k] = base + (1ull << hi);
 }
 
+// Blocks of a segmented eq-weighted sum: 2048 >> seg_bits, at most one per row.
 int dot_blocks(int rows_log, int seg_bits) {
   long rows = 1L << rows_log;
-  static long tmp_want = getenv("TMP_DOT_WANT") ? atol(getenv("TMP_DOT_WANT")) : 2048;   // TEMPORARY experiment
-  long want = tmp_want >> seg_bits;
+  long want = 2048L >> seg_bits;
   if (want < 1) want = 1;
   return (int)std::min(rows, want);
 }
@@ -485,6 +492,14 @@ void upload_challenger(whirgpu_ctx* c, const DuplexChallenger& ch) {
   *c->h_chal = dc;
 }
 
+// Grid of a device-transcript grind over all of [0, p) (so the search always
+// ends): sized to the expected search (16 * 2^bits candidates), one pass of
+// it at most, so short grinds leave the device to other work.
+int sc_grind_blocks(const whirgpu_ctx* c, int pow_bits) {
+  const u64 want = ((16ull << pow_bits) + 255) / 256;
+  return pow_bits <= 12 ? c->grind_blocks : (int)std::max<u64>(1, std::min<u64>((u64)c->grind_blocks, want));
+}
+
 // Launches `rounds` device-transcript sumcheck rounds starting from the
 // challenger at cin (device-visible memory).
 ScLaunch sumcheck_launch(whirgpu_ctx* c, SumcheckState& s, int rounds, int pow_bits, const DevChallenger* cin) {
@@ -519,14 +534,8 @@ ScLaunch sumcheck_launch(whirgpu_ctx* c, SumcheckState& s, int rounds, int pow_b
       DevChallenger* cout = next_state();
       ScTr tr{cin, cout, c->d_sclog + i, part[pp], g, nullptr, (int)nb, pow_bits};
       tr.gbase = c->d_gbase;
-      // The range is all of [0, p), so the search always ends; the grid is
-      // sized to the expected search (16 * 2^bits candidates), one pass of
-      // it at most, so short grinds leave the device to other work.
-      const u64 want = ((16ull << pow_bits) + 255) / 256;
-      const int gblocks =
-          pow_bits <= 12 ? c->grind_blocks : (int)std::max<u64>(1, std::min<u64>((u64)c->grind_blocks, want));
       Span sp(c, ST_GRIND, 0);
-      KCHK(grind_range_tr(c->d_c16, tr, pow_bits, 0, P, gblocks, c->st));
+      KCHK(grind_range_tr(c->d_c16, tr, pow_bits, 0, P, sc_grind_blocks(c, pow_bits), c->st));
       sp.close();
       if (c->prof) grind_recs.push_back(c->recs.size() - 1);
       cin = cout;
@@ -751,17 +760,20 @@ void commit_tree(whirgpu_ctx* c, whirgpu_prover_data* pd) {
     KCHK(merkle_leaves(pd->rows.p, pd->row_len, pd->height, lw, c->d_c16, c->d_c24, pd->tree.p, c->st));
   }
   Span sp(c, ST_TREE, (double)(pd->height - 1));
-  int rc = merkle_tree(pd->height, c->c16, pd->tree.p, c->st, c->d_root);
+  pd->root_slot = c->root_next;
+  c->root_next = (c->root_next + 1) & 7;
+  int rc = merkle_tree(pd->height, c->c16, pd->tree.p, c->st, c->d_root + 8 * pd->root_slot);
   if (rc < 0) KCHK(rc);
   pd->root_on_host = rc == 0;   // the tree's last kernel wrote the root to h_root
 }
 
-// The root: written to h_root by the tree's last kernel, or (single leaf) copied.
-Digest read_root(whirgpu_ctx* c, const whirgpu_prover_data* pd) {
+// The root: written to its h_root slot by the tree's last kernel, or (single
+// leaf) copied.  completed: the tree kernels are known to have finished.
+Digest read_root(whirgpu_ctx* c, const whirgpu_prover_data* pd, bool completed = false) {
   Digest d;
   if (pd->root_on_host) {
-    c->sync();
-    memcpy(d.data(), c->h_root, sizeof d);
+    if (!completed) c->sync();
+    memcpy(d.data(), c->h_root + 8 * pd->root_slot, sizeof d);
   } else {
     c->d2h(d.data(), pd->tree.p + (2 * pd->height - 2) * 8, 8);
   }
@@ -787,14 +799,20 @@ bool device_round_fits(const whirgpu_ctx* c, const whir::RoundConfig& rp, int m,
 // (prover/mod.rs:192-303) with the transcript on the device: the OOD samples
 // and their answers, the proof of work, the STIR queries and their openings,
 // gamma, the weight updates and the sumcheck run as one stream of kernels,
-// each reading its challenger from the previous one, with one
-// synchronization at the end.  The host then replays every step on its own
-// challenger and requires each challenge, answer and query index to equal
-// the device's, so the proof and the challenger state are those of the
-// host-driven steps.  Returns the sumcheck's challenges.
-std::vector<ef> device_round(whirgpu_ctx* c, SumcheckState& s, const whir::WhirConfig& cfg, int round, int kn,
-                             const whirgpu_prover_data* pd, const whirgpu_prover_data* prev,
-                             const std::vector<ef>& fold_rand, whir::WhirRoundProof& R, DuplexChallenger& ch) {
+// each reading its challenger from the previous one, and one event at the
+// end.  The caller may enqueue the next round's commitment (which needs none
+// of the host's results) before device_round_replay waits for that event;
+// the host then replays every step on its own challenger and requires each
+// challenge, answer and query index to equal the device's, so the proof and
+// the challenger state are those of the host-driven steps.
+struct RoundLaunch {
+  ScLaunch la;
+  size_t grind_rec = SIZE_MAX;
+};
+
+RoundLaunch device_round_launch(whirgpu_ctx* c, SumcheckState& s, const whir::WhirConfig& cfg, int round, int kn,
+                                const whirgpu_prover_data* pd, const whirgpu_prover_data* prev,
+                                const DuplexChallenger& ch) {
   const auto& rp = cfg.round_parameters[round];
   const int m = s.m;
   const int n_ood = rp.ood_samples;
@@ -825,7 +843,8 @@ std::vector<ef> device_round(whirgpu_ctx* c, SumcheckState& s, const whir::WhirC
     n_obs = 4;
     log_obs = &c->d_rlog->ev[i];
   }
-  size_t grind_rec = SIZE_MAX;
+  RoundLaunch rl;
+  size_t& grind_rec = rl.grind_rec;
   if (rp.pow_bits > 0) {
     HIPCHK(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(c->rb.grind), (int)P, 1, c->st));
     DevChallenger* cout = rch[ri];
@@ -889,11 +908,21 @@ std::vector<ef> device_round(whirgpu_ctx* c, SumcheckState& s, const whir::WhirC
     c->release(B);
   }
   s.invalidate();
-  ScLaunch la = sumcheck_launch(c, s, kn, rp.folding_pow_bits, cin);
-  c->sync();
+  rl.la = sumcheck_launch(c, s, kn, rp.folding_pow_bits, cin);
+  HIPCHK(hipEventRecord(c->ev_round, c->st));
+  return rl;
+}
 
-  // replay on the host challenger
-  R.commitment = read_root(c, pd);
+// Waits for the round launched by device_round_launch (not for what was
+// enqueued after it) and replays its transcript on the host challenger.
+std::vector<ef> device_round_replay(whirgpu_ctx* c, SumcheckState& s, const whir::WhirConfig& cfg, int round,
+                                    const whirgpu_prover_data* pd, const whirgpu_prover_data* prev,
+                                    const std::vector<ef>& fold_rand, whir::WhirRoundProof& R, DuplexChallenger& ch,
+                                    const RoundLaunch& rl) {
+  const auto& rp = cfg.round_parameters[round];
+  const int n_ood = rp.ood_samples;
+  HIPCHK(hipEventSynchronize(c->ev_round));
+  R.commitment = read_root(c, pd, true);
   observe_digest(ch, R.commitment);
   const RoundLog& L = *c->h_rlog;
   for (int i = 0; i < n_ood; i++) {
@@ -906,7 +935,7 @@ std::vector<ef> device_round(whirgpu_ctx* c, SumcheckState& s, const whir::WhirC
     if (L.err) bad("device round transcript: no valid proof-of-work witness", WHIRGPU_GRIND_FAILED);
     if (!ch.check_witness(rp.pow_bits, L.witness)) bad("device round transcript: witness rejected", WHIRGPU_GRIND_FAILED);
     R.pow_witness = L.witness;
-    if (grind_rec != SIZE_MAX) c->recs[grind_rec].work = (double)from_monty(L.witness) + 1;
+    if (rl.grind_rec != SIZE_MAX) c->recs[rl.grind_rec].work = (double)from_monty(L.witness) + 1;
   }
   ch.sample();
   const auto idx = whir::get_challenge_stir_queries(rp.domain_size, cfg.ff(round), rp.num_queries, ch);
@@ -921,7 +950,7 @@ std::vector<ef> device_round(whirgpu_ctx* c, SumcheckState& s, const whir::WhirC
   for (int i = 0; i < n_ood; i++) s.sum = ef_add(s.sum, ef_mul(gp[i], L.ev[i]));
   for (size_t j = 0; j < idx.size(); j++)
     s.sum = ef_add(s.sum, ef_mul(gp[n_ood + j], whir::mle_host(rows[j], fold_rand)));
-  return sumcheck_replay(c, s, la, R.sumcheck, ch);
+  return sumcheck_replay(c, s, rl.la, R.sumcheck, ch);
 }
 
 double now_ms() {
@@ -1062,6 +1091,7 @@ int whirgpu_create(int device, const whirgpu_params* params, const whirgpu_posei
     if (device < 0) return;   // host-only context: schedule and transcript
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_round, hipEventDisableTiming));
     make_twiddles(c);
     int n = (int)params->num_variables;
     c->tab_cap = (1ull << std::max(0, n - 8)) + 512;
@@ -1138,6 +1168,7 @@ int whirgpu_destroy(whirgpu_ctx* c) {
   if (c->h_pin) (void)hipHostFree(c->h_pin);
   if (c->h_ring) (void)hipHostFree(c->h_ring);
   for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+  if (c->ev_round) (void)hipEventDestroy(c->ev_round);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
   return WHIRGPU_OK;
@@ -1357,6 +1388,8 @@ int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs,
                   whirgpu_prover_data* pd0) {
   SumcheckState s;
   whirgpu_prover_data* prev = pd0;
+  whirgpu_prover_data* next_pd = nullptr;   // a commitment enqueued ahead of its round
+  whirgpu_prover_data* cur_pd = nullptr;    // this round's, until it becomes prev
   int rc = guarded([&] {
     if (!c || !proof || !chs || !st || !pd0) bad("null argument");
     use_device(c);
@@ -1367,6 +1400,32 @@ int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs,
     auto& P = proof->p;
     P.rounds.assign(cfg.n_rounds(), whir::WhirRoundProof());
     const int pow0 = cfg.starting_folding_pow_bits;
+    // Commitment of round r's folded polynomial (prover/mod.rs:158-190):
+    // 2^(m - kn + R) rows of 2^kn extension elements and their tree.
+    auto commit_round = [&](int round) {
+      const int m = n - cfg.params.folding_factor.total_number(round);
+      const int kn = cfg.ff(round + 1);
+      const int Rl = ilog2u(cfg.inv_rate(round));
+      const u64 H = 1ull << (m - kn + Rl), W = 1ull << kn;
+      auto* pd = new whirgpu_prover_data();
+      pd->ctx = c;
+      pd->rows = c->alloc<u32>(H * W * 4);
+      pd->tree = c->alloc<u32>((2 * H - 1) * 8);
+      pd->height = H;
+      pd->row_len = (int)(4 * W);
+      pd->ext = true;
+      next_pd = pd;   // destroyed by the caller if anything below throws
+      Buf<ef> tmp;
+      if (lde_two_pass(m, kn, true)) tmp = c->alloc<ef>(H * W);
+      {
+        Span sp(c, ST_LDE_EXT, 16.0 * (double)(1ull << m) + 16.0 * (double)(H * W));
+        KCHK(lde_ext(s.e.p, reinterpret_cast<ef*>(pd->rows.p), tmp.p, m, kn, Rl, c->tw, c->st));
+      }
+      commit_tree(c, pd);
+      if (tmp.p) c->release(tmp);
+      next_pd = nullptr;
+      return pd;
+    };
     ef alpha = ch.sample_ef();
     std::vector<ef> fold_rand =
         initial_sumcheck(c, st, cfg.ff(0), whir::powers(alpha, st->points.size()),
@@ -1397,28 +1456,17 @@ int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs,
       const auto& rp = cfg.round_parameters[round];
       auto& R = P.rounds[round];
       const int kn = cfg.ff(round + 1);
-      const int Rl = ilog2u(cfg.inv_rate(round));
-      const u64 H = 1ull << (m - kn + Rl), W = 1ull << kn;
-      // commit to the folded polynomial (prover/mod.rs:158-190)
-      auto* pd = new whirgpu_prover_data();
-      pd->ctx = c;
-      pd->rows = c->alloc<u32>(H * W * 4);
-      pd->tree = c->alloc<u32>((2 * H - 1) * 8);
-      pd->height = H;
-      pd->row_len = (int)(4 * W);
-      pd->ext = true;
-      Buf<ef> tmp;
-      if (lde_two_pass(m, kn, true)) tmp = c->alloc<ef>(H * W);
-      {
-        Span sp(c, ST_LDE_EXT, 16.0 * (double)(1ull << m) + 16.0 * (double)(H * W));
-        KCHK(lde_ext(s.e.p, reinterpret_cast<ef*>(pd->rows.p), tmp.p, m, kn, Rl, c->tw, c->st));
-      }
-      commit_tree(c, pd);
-      if (tmp.p) c->release(tmp);
+      cur_pd = next_pd ? next_pd : commit_round(round);
+      next_pd = nullptr;
+      whirgpu_prover_data* pd = cur_pd;
       if (device_round_fits(c, rp, m, prev)) {
-        fold_rand = device_round(c, s, cfg, round, kn, pd, prev, fold_rand, R, ch);
+        const RoundLaunch rl = device_round_launch(c, s, cfg, round, kn, pd, prev, ch);
+        // the next commitment depends only on the folded polynomial
+        if (round + 1 < cfg.n_rounds()) next_pd = commit_round(round + 1);
+        fold_rand = device_round_replay(c, s, cfg, round, pd, prev, fold_rand, R, ch, rl);
         whirgpu_prover_data_destroy(prev);
         prev = pd;
+        cur_pd = nullptr;
         continue;
       }
       R.commitment = read_root(c, pd);
@@ -1446,6 +1494,7 @@ int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs,
       }
       whirgpu_prover_data_destroy(prev);
       prev = pd;
+      cur_pd = nullptr;
       // Constraint::combine_packed (src/whir/constraints/mod.rs:234-248)
       ef gamma = ch.sample_ef();
       auto gp = whir::powers(gamma, ood_pts.size() + sel_z.size());
@@ -1478,6 +1527,9 @@ int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs,
     store_ch(ch, chs);
     c->timings[6] = now_ms() - t0;
   });
+  if (rc && c) (void)hipStreamSynchronize(c->st);   // nothing in flight uses what is released below
+  if (next_pd) whirgpu_prover_data_destroy(next_pd);
+  if (cur_pd) whirgpu_prover_data_destroy(cur_pd);
   if (prev) whirgpu_prover_data_destroy(prev);
   if (s.e.p) c->release(s.e);
   if (s.w.p) c->release(s.w);
