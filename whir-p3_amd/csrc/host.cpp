@@ -353,15 +353,20 @@ void poseidon2_constants_from_seed(u64 seed, P2C16* c16, P2C24* c24) {
 
 // ---------------------------------------------------------------- proof bytes
 namespace {
+static_assert(__BYTE_ORDER__ == __ORDER_LITTLE_ENDIAN__, "proof bytes are written as host words");
+// Writes the proof through sink.put(bytes, n): a counting pass sizes the
+// output, a second pass writes it where the caller wants it.
+template <class Sink>
 struct Out {
-  std::vector<uint8_t> b;
-  void u8(uint8_t x) { b.push_back(x); }
-  void le(u64 x, int n) { for (int i = 0; i < n; i++) b.push_back((uint8_t)(x >> (8 * i))); }
+  Sink& k;
+  void u8(uint8_t x) { k.put(&x, 1); }
+  void le(u64 x, int n) { k.put(&x, (size_t)n); }   // little-endian host: the low n bytes
+  void words(const u32* w, size_t n) { k.put(w, 4 * n); }
   void f(u32 x) { le(x, 4); }
   void len(size_t n) { le(n, 8); }
-  void e(const ef& x) { for (u32 c : x.c) f(c); }
-  void digest(const Digest& d) { for (u32 c : d) f(c); }
-  void text(const char* s) { size_t n = strlen(s); len(n); b.insert(b.end(), s, s + n); }
+  void e(const ef& x) { words(x.c, 4); }
+  void digest(const Digest& d) { words(d.data(), d.size()); }
+  void text(const char* s) { size_t n = strlen(s); len(n); k.put(s, n); }
   void sumcheck(const SumcheckData& s) {
     len(s.polynomial_evaluations.size());
     for (auto& pe : s.polynomial_evaluations) { e(pe[0]); e(pe[1]); }
@@ -373,41 +378,66 @@ struct Out {
     for (auto& q : qs) {
       text(q.extension ? "extension" : "base");
       len(q.extension ? q.values.size() / 4 : q.values.size());
-      for (u32 v : q.values) f(v);
+      words(q.values.data(), q.values.size());
       len(q.proof.size());
       for (auto& d : q.proof) digest(d);
     }
   }
+  void proof(const WhirProof& p) {
+    u8(p.has_initial_commitment);
+    if (p.has_initial_commitment) digest(p.initial_commitment);
+    len(p.initial_ood_answers.size());
+    for (auto& x : p.initial_ood_answers) e(x);
+    sumcheck(p.initial_sumcheck);
+    len(p.rounds.size());
+    for (auto& r : p.rounds) {
+      u8(1);
+      digest(r.commitment);
+      len(r.ood_answers.size());
+      for (auto& x : r.ood_answers) e(x);
+      f(r.pow_witness);
+      queries(r.queries);
+      sumcheck(r.sumcheck);
+    }
+    u8(p.has_final_poly);
+    if (p.has_final_poly) {
+      len(p.final_poly.size());
+      for (auto& x : p.final_poly) e(x);
+    }
+    f(p.final_pow_witness);
+    queries(p.final_queries);
+    u8(p.has_final_sumcheck);
+    if (p.has_final_sumcheck) sumcheck(p.final_sumcheck);
+  }
+};
+struct CountSink {
+  size_t n = 0;
+  void put(const void*, size_t k) { n += k; }
+};
+struct BufSink {
+  uint8_t* p;
+  void put(const void* src, size_t k) {
+    memcpy(p, src, k);
+    p += k;
+  }
 };
 }  // namespace
 
+size_t serialized_size(const WhirProof& p) {
+  CountSink k;
+  Out<CountSink>{k}.proof(p);
+  return k.n;
+}
+
+void serialize_into(const WhirProof& p, uint8_t* out) {
+  BufSink k{out};
+  Out<BufSink>{k}.proof(p);
+}
+
 std::vector<uint8_t> serialize(const WhirProof& p) {
-  Out o;
-  o.u8(p.has_initial_commitment);
-  if (p.has_initial_commitment) o.digest(p.initial_commitment);
-  o.len(p.initial_ood_answers.size());
-  for (auto& x : p.initial_ood_answers) o.e(x);
-  o.sumcheck(p.initial_sumcheck);
-  o.len(p.rounds.size());
-  for (auto& r : p.rounds) {
-    o.u8(1);
-    o.digest(r.commitment);
-    o.len(r.ood_answers.size());
-    for (auto& x : r.ood_answers) o.e(x);
-    o.f(r.pow_witness);
-    o.queries(r.queries);
-    o.sumcheck(r.sumcheck);
-  }
-  o.u8(p.has_final_poly);
-  if (p.has_final_poly) {
-    o.len(p.final_poly.size());
-    for (auto& x : p.final_poly) o.e(x);
-  }
-  o.f(p.final_pow_witness);
-  o.queries(p.final_queries);
-  o.u8(p.has_final_sumcheck);
-  if (p.has_final_sumcheck) o.sumcheck(p.final_sumcheck);
-  return o.b;
+  std::vector<uint8_t> b(serialized_size(p));
+  serialize_into(p, b.data());
+  return b;
 }
 
 // ---------------------------------------------------------------- EF helpers

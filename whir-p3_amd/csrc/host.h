@@ -142,6 +142,8 @@ struct WhirProof {
 // Field values as their Montgomery u32, lengths as u64 and Option tags as one
 // byte, all little endian; QueryOpening as {type: "base"|"extension", values, proof}.
 std::vector<uint8_t> serialize(const WhirProof& p);
+size_t serialized_size(const WhirProof& p);
+void serialize_into(const WhirProof& p, uint8_t* out);   // serialized_size(p) bytes
 
 // ---------------------------------------------------------------- EF helpers
 std::vector<ef> eq_table_host(const std::vector<ef>& z, const ef& scale);

@@ -547,10 +547,13 @@ int ood_eq_tables(const StepTr& st, ef* log_y, ef* pt_out, int m, int hi, int lo
 // without proof of work, the last OOD answer / the root observed), one sample
 // is discarded, the STIR query indices are sampled (get_challenge_stir_queries,
 // src/whir/utils.rs:24-145: sample_bits calls of up to 20 bits, split into
-// per-query indices), sorted and deduplicated, and gamma is sampled.
+// per-query indices), sorted and deduplicated, and gamma is sampled.  The
+// final round (prover/mod.rs:343-377) has neither the discarded sample nor
+// gamma (final_step).
 __global__ void __launch_bounds__(256) k_round_queries(StepTr st, const u32* grind, int pow_bits, u64 domain_size,
                                                        int ff, int num_queries, u32 folded_gen, int n_ood,
-                                                       RoundLog* log, RoundBufs rb, const P2C16* __restrict__ cp) {
+                                                       RoundLog* log, RoundBufs rb, const P2C16* __restrict__ cp,
+                                                       int final_step) {
   __shared__ u32 sp[16], in[8], out[8];
   __shared__ u64 q[RT_MAX_QUERIES];
   __shared__ u32 firsts[RT_MAX_QUERIES];
@@ -571,7 +574,7 @@ __global__ void __launch_bounds__(256) k_round_queries(StepTr st, const u32* gri
   } else {
     for (int k = 0; k < st.n_obs; k++) ch.observe(st.obs[k]);
   }
-  (void)ch.sample();
+  if (!final_step) (void)ch.sample();
   const u64 folded = domain_size >> ff;
   int bits = 0;
   while ((1ull << bits) < folded) bits++;
@@ -626,10 +629,12 @@ __global__ void __launch_bounds__(256) k_round_queries(StepTr st, const u32* gri
     rb.selz[pos] = pow(folded_gen, v);
     log->idx[pos] = (u32)v;
   }
-  ef gamma;
+  ef gamma = ef_zero();
+  if (!final_step) {
 #pragma unroll
-  for (int k = 0; k < 4; k++) gamma.c[k] = ch.sample();
-  for (u32 i = t; i < (u32)n_ood + nq; i += 256) rb.gpow[i] = ef_pow(gamma, i);
+    for (int k = 0; k < 4; k++) gamma.c[k] = ch.sample();
+    for (u32 i = t; i < (u32)n_ood + nq; i += 256) rb.gpow[i] = ef_pow(gamma, i);
+  }
   ch.store(st.ch_out);
   if (t == 0) {
     *rb.nq = nq;
@@ -643,10 +648,10 @@ __global__ void __launch_bounds__(256) k_round_queries(StepTr st, const u32* gri
 
 int round_queries(const StepTr& st, const u32* grind, int pow_bits, u64 domain_size, int folding_factor,
                   int num_queries, u32 folded_gen, int n_ood, RoundLog* log, const RoundBufs& rb,
-                  const P2C16* d_consts, hipStream_t st_) {
+                  const P2C16* d_consts, hipStream_t st_, bool final_step) {
   if (num_queries < 1 || num_queries > RT_MAX_QUERIES || n_ood < 0 || n_ood > RT_MAX_OOD) return -1;
   hipLaunchKernelGGL(k_round_queries, dim3(1), dim3(256), 0, st_, st, grind, pow_bits, domain_size, folding_factor,
-                     num_queries, folded_gen, n_ood, log, rb, d_consts);
+                     num_queries, folded_gen, n_ood, log, rb, d_consts, final_step ? 1 : 0);
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 

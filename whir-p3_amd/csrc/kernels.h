@@ -145,10 +145,11 @@ int ood_eq_tables(const StepTr& st, ef* log_y, ef* pt_out, int m, int hi, int lo
 // One block: the round's witness from `grind` (pow_bits > 0) or st.obs, one
 // discarded sample, the STIR query indices over the folded domain (sorted,
 // distinct), their points folded_gen^idx, gamma and its powers (n_ood + nq of
-// them).  Log and buffers as in RoundLog / RoundBufs.
+// them).  Log and buffers as in RoundLog / RoundBufs.  final_step: the final
+// round's queries (no discarded sample, no gamma).
 int round_queries(const StepTr& st, const u32* grind, int pow_bits, u64 domain_size, int folding_factor,
                   int num_queries, u32 folded_gen, int n_ood, RoundLog* log, const RoundBufs& rb,
-                  const P2C16* d_consts, hipStream_t st_);
+                  const P2C16* d_consts, hipStream_t st_, bool final_step = false);
 // merkle_open with the query indices and their count in device memory
 // (at most max_nq).
 int merkle_open_dev(const u32* rows, int row_len, const u32* tree, u64 height, const u64* d_idx, const u32* d_nq,

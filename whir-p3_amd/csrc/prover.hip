@@ -147,6 +147,9 @@ struct whirgpu_ctx {
   // device transcript of a round's steps before its sumcheck
   RoundLog* h_rlog = nullptr;   // coherent host memory
   RoundLog* d_rlog = nullptr;   // device address of h_rlog
+  // pinned host copy of the final polynomial, taken before the final sumcheck folds it
+  static constexpr int FINAL_MAX_LOG = 6;
+  ef* h_final = nullptr;
   DevChallenger* d_rchal = nullptr;   // two states the round's steps alternate between
   GrindBase* d_gbase = nullptr;       // base state of a long grind's candidates
   RoundBufs rb{};
@@ -953,6 +956,92 @@ std::vector<ef> device_round_replay(whirgpu_ctx* c, SumcheckState& s, const whir
   return sumcheck_replay(c, s, rl.la, R.sumcheck, ch);
 }
 
+// Whether the final round's steps fit the device transcript: the device
+// observes the final polynomial word by word, so it must be small.
+bool device_final_fits(const whirgpu_ctx* c, const whir::WhirConfig& cfg, int m, const whirgpu_prover_data* prev) {
+  if (!prev || m < 0 || m > whirgpu_ctx::FINAL_MAX_LOG) return false;
+  if (cfg.final_queries < 1 || cfg.final_queries > RT_MAX_QUERIES) return false;
+  if (cfg.final_sumcheck_rounds > whirgpu_ctx::SC_LOG_CAP) return false;
+  const size_t per = (size_t)prev->row_len + 8ull * ilog2u(prev->height);
+  return per * (size_t)cfg.final_queries <= c->open_cap;
+}
+
+// The final round (Prover::final_round, prover/mod.rs:330-417) with the
+// transcript on the device: observe the final polynomial, the proof of work,
+// the final STIR queries and their openings, and the final sumcheck, as one
+// stream of kernels and one synchronization; then the host replays it.
+void device_final_round(whirgpu_ctx* c, SumcheckState& s, const whir::WhirConfig& cfg, int round,
+                        const whirgpu_prover_data* prev, whir::WhirProof& Pf, DuplexChallenger& ch) {
+  const int m = s.m;
+  const size_t nf = 1ull << m;
+  const int pow = cfg.final_pow_bits;
+  const auto fc = cfg.final_round_config();
+  upload_challenger(c, ch);
+  DevChallenger* rch[2] = {c->d_rchal, c->d_rchal + 1};
+  int ri = 0;
+  const DevChallenger* cin = c->d_chal_init;
+  HIPCHK(hipMemcpyAsync(c->h_final, s.e.p, nf * sizeof(ef), hipMemcpyDeviceToHost, c->st));
+  const u32* obs = reinterpret_cast<const u32*>(s.e.p);
+  int n_obs = (int)(4 * nf);
+  size_t grind_rec = SIZE_MAX;
+  if (pow > 0) {
+    HIPCHK(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(c->rb.grind), (int)P, 1, c->st));
+    DevChallenger* cout = rch[ri];
+    ri ^= 1;
+    ScTr tr{};
+    tr.ch_in = cin;
+    tr.ch_out = cout;
+    tr.grind = c->rb.grind;
+    tr.pow_bits = pow;
+    tr.obs = obs;
+    tr.n_obs = n_obs;
+    tr.gbase = c->d_gbase;
+    Span sp(c, ST_GRIND, 0);
+    KCHK(grind_range_tr(c->d_c16, tr, pow, 0, P, sc_grind_blocks(c, pow), c->st, TR_OBS_GRIND));
+    sp.close();
+    if (c->prof) grind_rec = c->recs.size() - 1;
+    cin = cout;
+    obs = nullptr;
+    n_obs = 0;
+  }
+  {
+    DevChallenger* cout = rch[ri];
+    ri ^= 1;
+    const StepTr st{cin, cout, obs, n_obs, nullptr};
+    KCHK(round_queries(st, c->rb.grind, pow, fc.domain_size, cfg.ff(round), cfg.final_queries, 1u, 0, c->d_rlog,
+                       c->rb, c->d_c16, c->st, true));
+    cin = cout;
+  }
+  KCHK(merkle_open_dev(prev->rows.p, prev->row_len, prev->tree.p, prev->height, c->rb.idx, c->rb.nq,
+                       cfg.final_queries, c->d_open, c->st));
+  ScLaunch la;
+  if (cfg.final_sumcheck_rounds > 0)
+    la = sumcheck_launch(c, s, cfg.final_sumcheck_rounds, cfg.final_folding_pow_bits, cin);
+  c->sync();
+
+  // replay on the host challenger
+  Pf.final_poly.assign(c->h_final, c->h_final + nf);
+  Pf.has_final_poly = true;
+  for (const ef& e : Pf.final_poly) ch.observe_ef(e);
+  const RoundLog& L = *c->h_rlog;
+  if (pow > 0) {
+    if (L.err) bad("device final round: no valid proof-of-work witness", WHIRGPU_GRIND_FAILED);
+    if (!ch.check_witness(pow, L.witness)) bad("device final round: witness rejected", WHIRGPU_GRIND_FAILED);
+    Pf.final_pow_witness = L.witness;
+    if (grind_rec != SIZE_MAX) c->recs[grind_rec].work = (double)from_monty(L.witness) + 1;
+  }
+  const auto idx = whir::get_challenge_stir_queries(fc.domain_size, cfg.ff(round), cfg.final_queries, ch);
+  if (idx.size() != L.nq) bad("device final round diverged (query count)", WHIRGPU_BAD_STATE);
+  for (size_t q = 0; q < idx.size(); q++)
+    if (idx[q] != L.idx[q]) bad("device final round diverged (query index)", WHIRGPU_BAD_STATE);
+  std::vector<std::vector<ef>> rows;
+  parse_openings(c, prev, (int)idx.size(), &Pf.final_queries, &rows);
+  if (cfg.final_sumcheck_rounds > 0) {
+    Pf.has_final_sumcheck = true;
+    sumcheck_replay(c, s, la, Pf.final_sumcheck, ch);
+  }
+}
+
 double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -1122,6 +1211,8 @@ int whirgpu_create(int device, const whirgpu_params* params, const whirgpu_posei
              whirgpu_ctx::SC_LOG_CAP * sizeof(ScLog));
     coherent(reinterpret_cast<void**>(&c->h_chal), reinterpret_cast<void**>(&c->d_chal_init), sizeof(DevChallenger));
     coherent(reinterpret_cast<void**>(&c->h_rlog), reinterpret_cast<void**>(&c->d_rlog), sizeof(RoundLog));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_final), sizeof(ef) << whirgpu_ctx::FINAL_MAX_LOG,
+                         hipHostMallocDefault));
     HIPCHK(hipMalloc(&c->d_rchal, 2 * sizeof(DevChallenger)));
     HIPCHK(hipMalloc(&c->d_gbase, sizeof(GrindBase)));
     {
@@ -1159,7 +1250,7 @@ int whirgpu_destroy(whirgpu_ctx* c) {
   if (c->st) (void)hipStreamSynchronize(c->st);
   c->pool.release_all();
   for (void* h : {(void*)c->h_small, (void*)c->h_idx, (void*)c->h_open, (void*)c->h_root, (void*)c->h_sclog,
-                  (void*)c->h_chal, (void*)c->h_rlog})
+                  (void*)c->h_chal, (void*)c->h_rlog, (void*)c->h_final})
     if (h) (void)hipHostFree(h);
   for (void* p : {(void*)c->d_tw, (void*)c->d_tabs, (void*)c->d_partial, (void*)c->d_grind, (void*)c->d_gslots,
                   (void*)c->d_c16, (void*)c->d_c24, (void*)c->d_chal, (void*)c->d_rchal, c->d_rbuf,
@@ -1320,10 +1411,10 @@ int whirgpu_proof_destroy(whirgpu_proof* p) {
 int whirgpu_proof_serialize(const whirgpu_proof* p, uint8_t* out, size_t cap, size_t* len) {
   return guarded([&] {
     if (!p || !len) bad("null argument");
-    auto b = whir::serialize(p->p);
-    *len = b.size();
-    if (cap < b.size() || !out) bad("buffer too small", WHIRGPU_BUFFER_TOO_SMALL);
-    memcpy(out, b.data(), b.size());
+    const size_t n = whir::serialized_size(p->p);
+    *len = n;
+    if (cap < n || !out) bad("buffer too small", WHIRGPU_BUFFER_TOO_SMALL);
+    whir::serialize_into(p->p, out);
   });
 }
 
@@ -1437,6 +1528,10 @@ int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs,
       const int m = n - cfg.params.folding_factor.total_number(round);
       if (m != s.m) bad("internal: round size mismatch", WHIRGPU_BAD_STATE);
       if (round == cfg.n_rounds()) {
+        if (device_final_fits(c, cfg, m, prev)) {
+          device_final_round(c, s, cfg, round, prev, P, ch);
+          break;
+        }
         // final_round (prover/mod.rs:330-417)
         P.final_poly.resize(1ull << m);
         c->d2h(P.final_poly.data(), s.e.p, P.final_poly.size());
