@@ -98,17 +98,37 @@ __global__ void __launch_bounds__(256) k_tree_level(const u32* __restrict__ in, 
   o[1] = make_uint4(s[4], s[5], s[6], s[7]);
 }
 
-// One tree level with a 16-lane permutation per node (small levels, where
-// latency rather than throughput decides).  Grid: ceil(16 n_out / 256).
-__global__ void __launch_bounds__(256) k_tree_level_wide(const u32* __restrict__ in, u64 n_out, u32* __restrict__ out,
-                                                         P2C16 c) {
-  const u64 t = (u64)blockIdx.x * 256 + threadIdx.x;
-  const u64 p = t >> 4;
-  const int j = threadIdx.x & 15;
+// Seven small levels in one launch, with a 16-lane permutation per node
+// (latency rather than throughput decides there): block b stages the 128
+// digests [128 b, 128 b + 128) of the level at `off` (n digests, a multiple of
+// 128) in LDS and reduces them to one digest seven levels up, writing every
+// level in between; the levels are laid out one after another (the level of
+// size n at off, the next of size n / 2 at off + n, ...).  64 nodes per step.
+__global__ void __launch_bounds__(1024) k_tree_sub_wide(u32* __restrict__ tree, u64 off, u64 n, P2C16 c) {
+  __shared__ u32 sh[2][128 * 8];
+  const int t = threadIdx.x;
+  const int j = t & 15;
   const WideLane w = wide_lane(c, j);
-  u32 x = p < n_out ? in[16 * p + j] : 0u;
-  x = permute16_wide(x, j, w, c);
-  if (p < n_out && j < 8) out[8 * p + j] = x;
+  const u64 b = blockIdx.x;
+  for (int e = t; e < 128 * 8; e += 1024) sh[0][e] = tree[(off + b * 128) * 8 + e];
+  __syncthreads();
+  int cur = 0;
+  for (int m = 128; m > 1; m >>= 1) {
+    const int m_out = m >> 1;
+    const int p = t >> 4;
+    if (4 * (t >> 6) < m_out) {   // wave-uniform; below 4 nodes part of the wave idles
+      u32 x = p < m_out ? sh[cur][16 * p + j] : 0u;
+      x = permute16_wide(x, j, w, c);
+      if (p < m_out && j < 8) {
+        sh[cur ^ 1][8 * p + j] = x;
+        tree[(off + n + b * m_out + p) * 8 + j] = x;
+      }
+    }
+    __syncthreads();
+    off += n;
+    n >>= 1;
+    cur ^= 1;
+  }
 }
 
 // The top levels of a tree in one launch: n (<= 128) digests at `off` are
@@ -183,24 +203,28 @@ int merkle_leaves(const u32* rows, int row_len, u64 height, int leaf_width, cons
                           : merkle_leaves_impl<16>(rows, row_len, height, tree, c16, st);
 }
 
-// Levels with more than 2^14 nodes use one lane per node (throughput), smaller
-// ones 16 lanes per node (latency), and the last 7 levels run in one launch.
+// Levels with more than 2^14 nodes use one lane per node (throughput); below
+// that, seven levels per launch with 16 lanes per node (latency), and the last
+// (at most 7) levels in one block.
 int merkle_tree(u64 height, const P2C16& c16, u32* tree, hipStream_t st, u32* root_out) {
+  if (height == 0 || (height & (height - 1))) return -1;
   u64 off = 0, n = height;
   while (n > 128) {
-    u64 n_out = n >> 1;
+    const u64 n_out = n >> 1;
     if (n_out > (1u << 14)) {
       unsigned blocks = (unsigned)((n_out + 255) / 256);
       hipLaunchKernelGGL(k_tree_level, dim3(blocks), dim3(256), 0, st, tree + off * 8, n_out, tree + (off + n) * 8,
                          c16);
-    } else {
-      unsigned blocks = (unsigned)((n_out * 16 + 255) / 256);
-      hipLaunchKernelGGL(k_tree_level_wide, dim3(blocks), dim3(256), 0, st, tree + off * 8, n_out,
-                         tree + (off + n) * 8, c16);
+      if (hipGetLastError() != hipSuccess) return -5;
+      off += n;
+      n = n_out;
+      continue;
     }
+    // n is a power of two above 128: n / 128 subtrees, levels n .. n / 64 written
+    hipLaunchKernelGGL(k_tree_sub_wide, dim3((unsigned)(n / 128)), dim3(1024), 0, st, tree, off, n, c16);
     if (hipGetLastError() != hipSuccess) return -5;
-    off += n;
-    n = n_out;
+    off += 2 * n - n / 64;
+    n /= 128;
   }
   if (n > 1) {
     hipLaunchKernelGGL(k_tree_top_wide, dim3(1), dim3(1024), 0, st, tree, off, (int)n, c16, root_out);

@@ -7,6 +7,9 @@
 
 namespace kb {
 
+// Four words as one register quad (16-byte loads, including nontemporal ones).
+typedef u32 u32x4 __attribute__((ext_vector_type(4)));
+
 // ---- element-generic arithmetic (base field u32 or extension ef)
 __device__ __forceinline__ u32 tadd(u32 a, u32 b) { return add(a, b); }
 __device__ __forceinline__ ef tadd(const ef& a, const ef& b) { return ef_add(a, b); }
