@@ -518,9 +518,7 @@ __global__ void __launch_bounds__(256) k_ood_eq_tables(StepTr st, ef* log_y, ef*
   BlockChallenger ch{sp, in, out, 0, 0, &wl, &c};
   ch.load(*st.ch_in);
   for (int k = 0; k < st.n_obs; k++) ch.observe(st.obs[k]);
-  ef y;
-#pragma unroll
-  for (int k = 0; k < 4; k++) y.c[k] = ch.sample();
+  const ef y = ch.sample_ef();
   if (t == 0) {
     ef cur = y;
     for (int j = m - 1; j >= 0; j--) {
@@ -655,8 +653,7 @@ __global__ void __launch_bounds__(256) k_round_queries(StepTr st, const u32* gri
   }
   ef gamma = ef_zero();
   if (!final_step) {
-#pragma unroll
-    for (int k = 0; k < 4; k++) gamma.c[k] = ch.sample();
+    gamma = ch.sample_ef();
     for (u32 i = t; i < (u32)n_ood + nq; i += 256) rb.gpow[i] = ef_pow(gamma, i);
   }
   ch.store(st.ch_out);

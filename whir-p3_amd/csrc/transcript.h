@@ -88,6 +88,22 @@ struct BlockChallenger {
     out_len--;
     return out[out_len];
   }
+  // Four samples as an extension element.  The loop is not unrolled (each
+  // sample may duplex), so the samples shift through registers instead of
+  // being stored at a loop index: an indexed private array would be moved to
+  // LDS at a flat work-item index, whose workgroup sizes come from the
+  // dispatch packet in host memory (microseconds per wave).
+  __device__ ef sample_ef() {
+    u32 a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    for (int k = 0; k < 4; k++) {
+      const u32 v = sample();
+      a0 = a1;
+      a1 = a2;
+      a2 = a3;
+      a3 = v;
+    }
+    return ef{{a0, a1, a2, a3}};
+  }
 };
 
 // Runs one transcript step with the whole block.  TR_COEFFS_GRIND leaves the
@@ -117,16 +133,15 @@ __device__ __forceinline__ void tr_prologue(int mode, const ScTr& a, const P2C16
       acc[0] = ef_add(acc[0], a.partial[2 * b]);
       acc[1] = ef_add(acc[1], a.partial[2 * b + 1]);
     }
-    block_reduce_ef<2>(acc, red);
+    block_reduce_ef<2>(acc, red);   // the sums stay in red[0] and red[256]
     if (lead && t == 0) {
       a.log->c0 = acc[0];
       a.log->c2 = acc[1];
       a.log->witness = 0;
     }
-#pragma unroll
-    for (int k = 0; k < 4; k++) ch.observe(acc[0].c[k]);
-#pragma unroll
-    for (int k = 0; k < 4; k++) ch.observe(acc[1].c[k]);
+    // observed from LDS: a loop index into the registers of acc would put
+    // acc in memory (see sample_ef)
+    for (int k = 0; k < 8; k++) ch.observe(red[(k >> 2) * 256].c[k & 3]);
     if (mode == TR_COEFFS_GRIND) {
       // a candidate w is observed into position in_len of this state
       if (t < 16) gb->s[t] = t < ch.in_len ? in[t] : sp[t];
@@ -144,9 +159,7 @@ __device__ __forceinline__ void tr_prologue(int mode, const ScTr& a, const P2C16
     if ((from_monty(ch.sample()) & mask) != 0) err = 1;
     if (lead && t == 0) a.log->witness = wm;
   }
-  ef rr;
-#pragma unroll
-  for (int k = 0; k < 4; k++) rr.c[k] = ch.sample();
+  const ef rr = ch.sample_ef();
   if (t == 0) *r = rr;
   if (lead) {
     ch.store(a.ch_out);
