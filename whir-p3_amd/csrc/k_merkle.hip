@@ -509,7 +509,7 @@ int grind_range_tr(const P2C16* d_consts, const ScTr& tr, int bits, u64 start, u
 // writes the point and the next challenger state.
 __global__ void __launch_bounds__(256) k_ood_eq_tables(StepTr st, ef* log_y, ef* pt_out, int m, int hi, int lo,
                                                        ef* __restrict__ out_hi, ef* __restrict__ out_lo,
-                                                       const P2C16* __restrict__ cp) {
+                                                       const P2C16* __restrict__ cp, u32* grind_reset) {
   __shared__ u32 sp[16], in[8], out[8];
   __shared__ ef z[RT_MAX_VARS];
   const int t = threadIdx.x;
@@ -532,6 +532,7 @@ __global__ void __launch_bounds__(256) k_ood_eq_tables(StepTr st, ef* log_y, ef*
     if (t == 0) {
       *log_y = y;
       if (st.n_obs == 4 && st.log_obs) *st.log_obs = ef{{st.obs[0], st.obs[1], st.obs[2], st.obs[3]}};
+      if (grind_reset) *grind_reset = P;
     }
     if (t < m) pt_out[t] = z[t];
   }
@@ -556,11 +557,11 @@ __global__ void __launch_bounds__(256) k_ood_eq_tables(StepTr st, ef* log_y, ef*
 }
 
 int ood_eq_tables(const StepTr& st, ef* log_y, ef* pt_out, int m, int hi, int lo, ef* out_hi, ef* out_lo,
-                  const P2C16* d_consts, hipStream_t st_) {
+                  const P2C16* d_consts, hipStream_t st_, u32* grind_reset) {
   if (m < 1 || m > RT_MAX_VARS || hi + lo != m) return -1;
   const u32 n = (1u << hi) + (1u << lo);
   hipLaunchKernelGGL(k_ood_eq_tables, dim3((n + 255) / 256), dim3(256), 0, st_, st, log_y, pt_out, m, hi, lo, out_hi,
-                     out_lo, d_consts);
+                     out_lo, d_consts, grind_reset);
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 

@@ -60,36 +60,6 @@ int eq_split(const PointArg& z, int hi, int lo, const ef& scale, ef* out_hi, ef*
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 
-// k_eq_split with the point and the scale read from device memory.
-__global__ void k_eq_split_dev(const ef* __restrict__ z, const ef* __restrict__ scale, int hi, int lo,
-                               ef* __restrict__ out_hi, ef* __restrict__ out_lo) {
-  u32 x = blockIdx.x * blockDim.x + threadIdx.x;
-  const u32 nh = 1u << hi;
-  if (x >= nh + (1u << lo)) return;
-  int first = 0, t = hi;
-  ef acc = *scale;
-  ef* out = out_hi;
-  if (x >= nh) {
-    x -= nh;
-    first = hi;
-    t = lo;
-    acc = ef_one();
-    out = out_lo;
-  }
-  for (int j = 0; j < t; j++) {
-    const ef zj = z[first + j];
-    int bit = (x >> (t - 1 - j)) & 1;
-    acc = ef_mul(acc, bit ? zj : ef_sub(ef_one(), zj));
-  }
-  out[x] = acc;
-}
-
-int eq_split_dev(const ef* z, const ef* scale, int hi, int lo, ef* out_hi, ef* out_lo, hipStream_t st) {
-  u32 n = (1u << hi) + (1u << lo);
-  hipLaunchKernelGGL(k_eq_split_dev, dim3((n + 255) / 256), dim3(256), 0, st, z, scale, hi, lo, out_hi, out_lo);
-  return hipGetLastError() == hipSuccess ? 0 : -5;
-}
-
 int eq_table(const PointArg& z, int first, int t, const ef& scale, ef* out, hipStream_t st) {
   u32 n = 1u << t;
   hipLaunchKernelGGL(k_eq_small, dim3((n + 255) / 256), dim3(256), 0, st, z, first, t, scale, out);
@@ -326,21 +296,27 @@ int eq_dot_ext(const ef* f, int nf, int sb, int lo, const EqTables& tabs, int K,
 }
 
 // ---------------------------------------------------------------- weight accumulation
-// W[x] = (acc ? W[x] : 0) + sum_k Ehi[k][x >> lo] * Elo[k][x & mask]
+// W[x] = (acc ? W[x] : 0) + sum_k s_k Ehi[k][x >> lo] * Elo[k][x & mask], with s_k = scale[k] when scale is
+// given (device memory) and 1 otherwise.
 __global__ void __launch_bounds__(256) k_eq_combine(ef* __restrict__ Wt, u64 n, int lo, EqTables tabs, int K,
-                                                    int accumulate) {
+                                                    int accumulate, const ef* __restrict__ scale) {
   const u32 mask = (1u << lo) - 1;
   for (u64 x = (u64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (u64)gridDim.x * blockDim.x) {
     ef acc = accumulate ? Wt[x] : ef_zero();
-    for (int k = 0; k < K; k++) acc = ef_add(acc, ef_mul(tabs.hi[k][x >> lo], tabs.lo[k][x & mask]));
+    for (int k = 0; k < K; k++) {
+      const ef t = ef_mul(tabs.hi[k][x >> lo], tabs.lo[k][x & mask]);
+      acc = ef_add(acc, scale ? ef_mul(scale[k], t) : t);
+    }
     Wt[x] = acc;
   }
 }
 
-int eq_combine(ef* Wt, u64 n, int lo, const EqTables& tabs, int K, int accumulate, hipStream_t st) {
+int eq_combine(ef* Wt, u64 n, int lo, const EqTables& tabs, int K, int accumulate, hipStream_t st,
+               const ef* d_scale) {
   u64 blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(k_eq_combine, dim3((unsigned)blocks), dim3(256), 0, st, Wt, n, lo, tabs, K, accumulate);
+  hipLaunchKernelGGL(k_eq_combine, dim3((unsigned)blocks), dim3(256), 0, st, Wt, n, lo, tabs, K, accumulate,
+                     d_scale);
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 

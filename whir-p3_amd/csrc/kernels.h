@@ -139,9 +139,10 @@ int grind_range_tr(const P2C16* d_consts, const ScTr& tr, int bits, u64 start, u
 // ---- a round's transcript steps (k_merkle.hip)
 // OOD sample: observe st.obs, sample the challenge y into *log_y, expand it to
 // the m-variable point pt_out (y^(2^(m-1-j)) at j), and write its split eq
-// tables (hi + lo = m variables).
+// tables (hi + lo = m variables).  Block 0 also sets *grind_reset to p when
+// given (the round's grind result slot).
 int ood_eq_tables(const StepTr& st, ef* log_y, ef* pt_out, int m, int hi, int lo, ef* out_hi, ef* out_lo,
-                  const P2C16* d_consts, hipStream_t st_);
+                  const P2C16* d_consts, hipStream_t st_, u32* grind_reset = nullptr);
 // One block: the round's witness from `grind` (pow_bits > 0) or st.obs, one
 // discarded sample, the STIR query indices over the folded domain (sorted,
 // distinct), their points folded_gen^idx, gamma and its powers (n_ood + nq of
@@ -159,14 +160,14 @@ int merkle_open_dev(const u32* rows, int row_len, const u32* tree, u64 height, c
 int eq_table(const PointArg& z, int first, int t, const ef& scale, ef* out, hipStream_t st);
 // Both halves of a split table in one launch: out_hi = scale * eq(z[0..hi)), out_lo = eq(z[hi..hi+lo)).
 int eq_split(const PointArg& z, int hi, int lo, const ef& scale, ef* out_hi, ef* out_lo, hipStream_t st);
-// Same with the point (hi + lo coordinates) and the scale in device memory.
-int eq_split_dev(const ef* z, const ef* scale, int hi, int lo, ef* out_hi, ef* out_lo, hipStream_t st);
 int eq_outer(const ef* Ehi, const ef* Elo, int lo, u64 n, ef* out, hipStream_t st);
 int eq_dot_base(const u32* f, int nf, int sb, int lo, const EqTables& tabs, int K, ef* partial, int nblocks, ef* out,
                 hipStream_t st);
 int eq_dot_ext(const ef* f, int nf, int sb, int lo, const EqTables& tabs, int K, ef* partial, int nblocks, ef* out,
                hipStream_t st);
-int eq_combine(ef* Wt, u64 n, int lo, const EqTables& tabs, int K, int accumulate, hipStream_t st);
+// Term k is scaled by d_scale[k] (device memory) when d_scale is given.
+int eq_combine(ef* Wt, u64 n, int lo, const EqTables& tabs, int K, int accumulate, hipStream_t st,
+               const ef* d_scale = nullptr);
 // With d_nsel given, the count is read from device memory and nsel is its
 // maximum (A and B sized for it).
 int select_combine(ef* Wt, int m, const u32* d_z, const ef* d_g, int nsel, const u32* d_nsel, ef* A, u32* B,

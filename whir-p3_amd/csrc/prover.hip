@@ -831,9 +831,10 @@ RoundLaunch device_round_launch(whirgpu_ctx* c, SumcheckState& s, const whir::Wh
     DevChallenger* cout = rch[ri];
     ri ^= 1;
     const StepTr st{cin, cout, obs, n_obs, log_obs};
-    ef* tab = c->d_tabs;
+    // table slot i: the weight update below uses the same tables, scaled by gamma^i
+    ef* tab = c->d_tabs + (size_t)i * c->tab_cap;
     KCHK(ood_eq_tables(st, &c->d_rlog->y[i], c->rb.pts + (size_t)i * RT_MAX_VARS, m, hi, lo, tab, tab + (1ull << hi),
-                       c->d_c16, c->st));
+                       c->d_c16, c->st, i == 0 && rp.pow_bits > 0 ? c->rb.grind : nullptr));
     EqTables t{};
     t.hi[0] = tab;
     t.lo[0] = tab + (1ull << hi);
@@ -849,7 +850,8 @@ RoundLaunch device_round_launch(whirgpu_ctx* c, SumcheckState& s, const whir::Wh
   RoundLaunch rl;
   size_t& grind_rec = rl.grind_rec;
   if (rp.pow_bits > 0) {
-    HIPCHK(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(c->rb.grind), (int)P, 1, c->st));
+    // the first OOD step reset the grind slot; without OOD samples it is reset here
+    if (n_ood == 0) HIPCHK(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(c->rb.grind), (int)P, 1, c->st));
     DevChallenger* cout = rch[ri];
     ri ^= 1;
     ScTr tr{};
@@ -885,19 +887,18 @@ RoundLaunch device_round_launch(whirgpu_ctx* c, SumcheckState& s, const whir::Wh
   KCHK(merkle_open_dev(prev->rows.p, prev->row_len, prev->tree.p, prev->height, c->rb.idx, c->rb.nq,
                        rp.num_queries, c->d_open, c->st));
   // Constraint::combine_packed (src/whir/constraints/mod.rs:234-248): the OOD
-  // points' eq weighted by gamma^i, then the queries' select terms
+  // points' eq (the tables of the OOD steps) weighted by gamma^i, then the
+  // queries' select terms
   for (int b = 0; b < n_ood; b += 4) {
     EqTables t{};
     const int kk = std::min(4, n_ood - b);
     for (int k = 0; k < kk; k++) {
-      ef* base = c->d_tabs + (size_t)k * c->tab_cap;
-      KCHK(eq_split_dev(c->rb.pts + (size_t)(b + k) * RT_MAX_VARS, c->rb.gpow + b + k, hi, lo, base,
-                        base + (1ull << hi), c->st));
+      ef* base = c->d_tabs + (size_t)(b + k) * c->tab_cap;
       t.hi[k] = base;
       t.lo[k] = base + (1ull << hi);
     }
     Span sp(c, ST_COMBINE, 32.0 * (double)(1ull << m));
-    KCHK(eq_combine(s.w.p, 1ull << m, lo, t, kk, 1, c->st));
+    KCHK(eq_combine(s.w.p, 1ull << m, lo, t, kk, 1, c->st, c->rb.gpow + b));
   }
   {
     const int ns = rp.num_queries;
