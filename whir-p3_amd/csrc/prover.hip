@@ -145,7 +145,7 @@ struct whirgpu_ctx {
   ScLog* h_sclog = nullptr;   // coherent host memory; the transcript kernel logs into it
   ScLog* d_sclog = nullptr;   // device address of h_sclog
   // device transcript of a round's steps before its sumcheck
-  RoundLog* h_rlog = nullptr;   // coherent host memory
+  RoundLog* h_rlog = nullptr;   // coherent host memory: a WHIR round's log, then the final round's
   RoundLog* d_rlog = nullptr;   // device address of h_rlog
   // pinned host copy of the final polynomial, taken before the final sumcheck folds it
   static constexpr int FINAL_MAX_LOG = 6;
@@ -479,6 +479,8 @@ ef sumcheck_round(whirgpu_ctx* c, SumcheckState& s, const ChallengeFn& next) {
 struct ScLaunch {
   int rounds = 0;
   int pow_bits = 0;
+  int log_base = 0;                     // first ScLog entry of these rounds
+  const DevChallenger* out = nullptr;   // device challenger state after the last round
   std::vector<size_t> grind_recs;
 };
 
@@ -505,12 +507,16 @@ int sc_grind_blocks(const whirgpu_ctx* c, int pow_bits) {
 
 // Launches `rounds` device-transcript sumcheck rounds starting from the
 // challenger at cin (device-visible memory).
-ScLaunch sumcheck_launch(whirgpu_ctx* c, SumcheckState& s, int rounds, int pow_bits, const DevChallenger* cin) {
+ScLaunch sumcheck_launch(whirgpu_ctx* c, SumcheckState& s, int rounds, int pow_bits, const DevChallenger* cin,
+                         int log_base = 0) {
   ScLaunch L;
   L.rounds = rounds;
   L.pow_bits = pow_bits;
+  L.log_base = log_base;
+  L.out = cin;
   if (rounds <= 0) return L;
-  if (rounds > whirgpu_ctx::SC_LOG_CAP) bad("too many sumcheck rounds in one phase", WHIRGPU_BAD_STATE);
+  if (log_base < 0 || log_base + rounds > whirgpu_ctx::SC_LOG_CAP)
+    bad("too many sumcheck rounds in one phase", WHIRGPU_BAD_STATE);
   if (s.m < rounds) bad("sumcheck round on a constant polynomial", WHIRGPU_BAD_STATE);
   std::vector<size_t>& grind_recs = L.grind_recs;
   // Each launch reads the challenger from `cin` and writes the next state to
@@ -535,7 +541,7 @@ ScLaunch sumcheck_launch(whirgpu_ctx* c, SumcheckState& s, int rounds, int pow_b
     int fold_mode = TR_COEFFS_SAMPLE;
     if (pow_bits > 0) {
       DevChallenger* cout = next_state();
-      ScTr tr{cin, cout, c->d_sclog + i, part[pp], g, nullptr, (int)nb, pow_bits};
+      ScTr tr{cin, cout, c->d_sclog + log_base + i, part[pp], g, nullptr, (int)nb, pow_bits};
       tr.gbase = c->d_gbase;
       Span sp(c, ST_GRIND, 0);
       KCHK(grind_range_tr(c->d_c16, tr, pow_bits, 0, P, sc_grind_blocks(c, pow_bits), c->st));
@@ -545,7 +551,7 @@ ScLaunch sumcheck_launch(whirgpu_ctx* c, SumcheckState& s, int rounds, int pow_b
       fold_mode = TR_WITNESS_SAMPLE;
     }
     DevChallenger* cout = next_state();
-    ScTr tr{cin, cout, c->d_sclog + i, part[pp], g, g_next, (int)nb, pow_bits};
+    ScTr tr{cin, cout, c->d_sclog + log_base + i, part[pp], g, g_next, (int)nb, pow_bits};
     {
       Span sp(c, ST_SUMCHECK, 96.0 * (double)h);
       KCHK(sc_fold_tr(s.e.p, s.w.p, h, fold_mode, tr, c->d_c16, part[pp ^ 1], &nb, c->st));
@@ -554,6 +560,7 @@ ScLaunch sumcheck_launch(whirgpu_ctx* c, SumcheckState& s, int rounds, int pow_b
     pp ^= 1;
     s.m -= 1;
   }
+  L.out = cin;
   // d_partial is scratch for other kernels between phases
   s.invalidate();
   return L;
@@ -566,7 +573,7 @@ std::vector<ef> sumcheck_replay(whirgpu_ctx* c, SumcheckState& s, const ScLaunch
   std::vector<ef> rs;
   const int rounds = la.rounds, pow_bits = la.pow_bits;
   const std::vector<size_t>& grind_recs = la.grind_recs;
-  std::vector<ScLog> log(c->h_sclog, c->h_sclog + rounds);
+  std::vector<ScLog> log(c->h_sclog + la.log_base, c->h_sclog + la.log_base + rounds);
   for (int i = 0; i < rounds; i++) {
     const ScLog& L = log[i];
     if (L.err) bad("device transcript: no valid proof-of-work witness", WHIRGPU_GRIND_FAILED);
@@ -710,7 +717,7 @@ int ilog2u(u64 x) {
 // Opens `idx` on a committed matrix and returns the openings plus each row
 // as extension elements.
 void parse_openings(whirgpu_ctx* c, const whirgpu_prover_data* pd, int nq, std::vector<whir::QueryOpening>* out,
-                    std::vector<std::vector<ef>>* rows_ef);
+                    std::vector<std::vector<ef>>* rows_ef, const u32* h = nullptr);
 
 void open_rows(whirgpu_ctx* c, const whirgpu_prover_data* pd, const std::vector<u64>& idx,
                std::vector<whir::QueryOpening>* out, std::vector<std::vector<ef>>* rows_ef) {
@@ -726,13 +733,14 @@ void open_rows(whirgpu_ctx* c, const whirgpu_prover_data* pd, const std::vector<
   parse_openings(c, pd, nq, out, rows_ef);
 }
 
-// The openings of nq queries written to h_open by an opening kernel (after a
-// synchronization), as proof entries and rows of extension elements.
+// The openings of nq queries written to h (h_open by default) by an opening
+// kernel (after a synchronization), as proof entries and rows of extension
+// elements.
 void parse_openings(whirgpu_ctx* c, const whirgpu_prover_data* pd, int nq, std::vector<whir::QueryOpening>* out,
-                    std::vector<std::vector<ef>>* rows_ef) {
+                    std::vector<std::vector<ef>>* rows_ef, const u32* h) {
   const int depth = ilog2u(pd->height);
   const size_t per = (size_t)pd->row_len + 8ull * depth;
-  const u32* h = c->h_open;
+  if (!h) h = c->h_open;
   for (int q = 0; q < nq; q++) {
     const u32* b = h + per * q;
     whir::QueryOpening o;
@@ -795,7 +803,7 @@ bool device_round_fits(const whirgpu_ctx* c, const whir::RoundConfig& rp, int m,
   if (rp.ood_samples > RT_MAX_OOD || rp.num_queries < 1 || rp.num_queries > RT_MAX_QUERIES) return false;
   if ((1ull << hi) + (1ull << lo) > c->tab_cap) return false;
   const size_t per = (size_t)prev->row_len + 8ull * ilog2u(prev->height);
-  return per * (size_t)rp.num_queries <= c->open_cap;
+  return per * (size_t)rp.num_queries <= c->open_cap / 2;   // the second half may hold the final round's
 }
 
 // A round's steps from its commitment's root to the end of its sumcheck
@@ -958,33 +966,47 @@ std::vector<ef> device_round_replay(whirgpu_ctx* c, SumcheckState& s, const whir
 }
 
 // Whether the final round's steps fit the device transcript: the device
-// observes the final polynomial word by word, so it must be small.
+// observes the final polynomial word by word, so it must be small.  Its
+// openings use the second half of the opening buffer.
 bool device_final_fits(const whirgpu_ctx* c, const whir::WhirConfig& cfg, int m, const whirgpu_prover_data* prev) {
   if (!prev || m < 0 || m > whirgpu_ctx::FINAL_MAX_LOG) return false;
   if (cfg.final_queries < 1 || cfg.final_queries > RT_MAX_QUERIES) return false;
-  if (cfg.final_sumcheck_rounds > whirgpu_ctx::SC_LOG_CAP) return false;
+  if (cfg.final_sumcheck_rounds > whirgpu_ctx::SC_LOG_CAP / 2) return false;
   const size_t per = (size_t)prev->row_len + 8ull * ilog2u(prev->height);
-  return per * (size_t)cfg.final_queries <= c->open_cap;
+  return per * (size_t)cfg.final_queries <= c->open_cap / 2;
 }
+
+struct FinalLaunch {
+  ScLaunch la;
+  size_t grind_rec = SIZE_MAX;
+  size_t nf = 0;
+};
 
 // The final round (Prover::final_round, prover/mod.rs:330-417) with the
 // transcript on the device: observe the final polynomial, the proof of work,
 // the final STIR queries and their openings, and the final sumcheck, as one
-// stream of kernels and one synchronization; then the host replays it.
-void device_final_round(whirgpu_ctx* c, SumcheckState& s, const whir::WhirConfig& cfg, int round,
-                        const whirgpu_prover_data* prev, whir::WhirProof& Pf, DuplexChallenger& ch) {
+// stream of kernels.  It starts from the device challenger `cin` (the last
+// round's, so it can be enqueued before that round is replayed) or, without
+// one, from the host challenger.  Its log, openings and sumcheck log use
+// their own slots (RoundLog 1, the opening buffer's second half, ScLog
+// entries from SC_LOG_CAP / 2), apart from the last round's.
+FinalLaunch device_final_launch(whirgpu_ctx* c, SumcheckState& s, const whir::WhirConfig& cfg, int round,
+                                const whirgpu_prover_data* prev, const DuplexChallenger& ch,
+                                const DevChallenger* cin) {
+  FinalLaunch fl;
   const int m = s.m;
-  const size_t nf = 1ull << m;
+  fl.nf = 1ull << m;
   const int pow = cfg.final_pow_bits;
   const auto fc = cfg.final_round_config();
-  upload_challenger(c, ch);
+  if (!cin) {
+    upload_challenger(c, ch);
+    cin = c->d_chal_init;
+  }
   DevChallenger* rch[2] = {c->d_rchal, c->d_rchal + 1};
   int ri = 0;
-  const DevChallenger* cin = c->d_chal_init;
-  HIPCHK(hipMemcpyAsync(c->h_final, s.e.p, nf * sizeof(ef), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(c->h_final, s.e.p, fl.nf * sizeof(ef), hipMemcpyDeviceToHost, c->st));
   const u32* obs = reinterpret_cast<const u32*>(s.e.p);
-  int n_obs = (int)(4 * nf);
-  size_t grind_rec = SIZE_MAX;
+  int n_obs = (int)(4 * fl.nf);
   if (pow > 0) {
     HIPCHK(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(c->rb.grind), (int)P, 1, c->st));
     DevChallenger* cout = rch[ri];
@@ -1000,7 +1022,7 @@ void device_final_round(whirgpu_ctx* c, SumcheckState& s, const whir::WhirConfig
     Span sp(c, ST_GRIND, 0);
     KCHK(grind_range_tr(c->d_c16, tr, pow, 0, P, sc_grind_blocks(c, pow), c->st, TR_OBS_GRIND));
     sp.close();
-    if (c->prof) grind_rec = c->recs.size() - 1;
+    if (c->prof) fl.grind_rec = c->recs.size() - 1;
     cin = cout;
     obs = nullptr;
     n_obs = 0;
@@ -1009,37 +1031,44 @@ void device_final_round(whirgpu_ctx* c, SumcheckState& s, const whir::WhirConfig
     DevChallenger* cout = rch[ri];
     ri ^= 1;
     const StepTr st{cin, cout, obs, n_obs, nullptr};
-    KCHK(round_queries(st, c->rb.grind, pow, fc.domain_size, cfg.ff(round), cfg.final_queries, 1u, 0, c->d_rlog,
+    KCHK(round_queries(st, c->rb.grind, pow, fc.domain_size, cfg.ff(round), cfg.final_queries, 1u, 0, c->d_rlog + 1,
                        c->rb, c->d_c16, c->st, true));
     cin = cout;
   }
   KCHK(merkle_open_dev(prev->rows.p, prev->row_len, prev->tree.p, prev->height, c->rb.idx, c->rb.nq,
-                       cfg.final_queries, c->d_open, c->st));
-  ScLaunch la;
+                       cfg.final_queries, c->d_open + c->open_cap / 2, c->st));
   if (cfg.final_sumcheck_rounds > 0)
-    la = sumcheck_launch(c, s, cfg.final_sumcheck_rounds, cfg.final_folding_pow_bits, cin);
-  c->sync();
+    fl.la = sumcheck_launch(c, s, cfg.final_sumcheck_rounds, cfg.final_folding_pow_bits, cin,
+                            whirgpu_ctx::SC_LOG_CAP / 2);
+  return fl;
+}
 
-  // replay on the host challenger
-  Pf.final_poly.assign(c->h_final, c->h_final + nf);
+// Waits for everything enqueued and replays the final round on the host challenger.
+void device_final_replay(whirgpu_ctx* c, SumcheckState& s, const whir::WhirConfig& cfg, int round,
+                         const whirgpu_prover_data* prev, whir::WhirProof& Pf, DuplexChallenger& ch,
+                         const FinalLaunch& fl) {
+  const int pow = cfg.final_pow_bits;
+  const auto fc = cfg.final_round_config();
+  c->sync();
+  Pf.final_poly.assign(c->h_final, c->h_final + fl.nf);
   Pf.has_final_poly = true;
   for (const ef& e : Pf.final_poly) ch.observe_ef(e);
-  const RoundLog& L = *c->h_rlog;
+  const RoundLog& L = c->h_rlog[1];
   if (pow > 0) {
     if (L.err) bad("device final round: no valid proof-of-work witness", WHIRGPU_GRIND_FAILED);
     if (!ch.check_witness(pow, L.witness)) bad("device final round: witness rejected", WHIRGPU_GRIND_FAILED);
     Pf.final_pow_witness = L.witness;
-    if (grind_rec != SIZE_MAX) c->recs[grind_rec].work = (double)from_monty(L.witness) + 1;
+    if (fl.grind_rec != SIZE_MAX) c->recs[fl.grind_rec].work = (double)from_monty(L.witness) + 1;
   }
   const auto idx = whir::get_challenge_stir_queries(fc.domain_size, cfg.ff(round), cfg.final_queries, ch);
   if (idx.size() != L.nq) bad("device final round diverged (query count)", WHIRGPU_BAD_STATE);
   for (size_t q = 0; q < idx.size(); q++)
     if (idx[q] != L.idx[q]) bad("device final round diverged (query index)", WHIRGPU_BAD_STATE);
   std::vector<std::vector<ef>> rows;
-  parse_openings(c, prev, (int)idx.size(), &Pf.final_queries, &rows);
+  parse_openings(c, prev, (int)idx.size(), &Pf.final_queries, &rows, c->h_open + c->open_cap / 2);
   if (cfg.final_sumcheck_rounds > 0) {
     Pf.has_final_sumcheck = true;
-    sumcheck_replay(c, s, la, Pf.final_sumcheck, ch);
+    sumcheck_replay(c, s, fl.la, Pf.final_sumcheck, ch);
   }
 }
 
@@ -1211,7 +1240,7 @@ int whirgpu_create(int device, const whirgpu_params* params, const whirgpu_posei
     coherent(reinterpret_cast<void**>(&c->h_sclog), reinterpret_cast<void**>(&c->d_sclog),
              whirgpu_ctx::SC_LOG_CAP * sizeof(ScLog));
     coherent(reinterpret_cast<void**>(&c->h_chal), reinterpret_cast<void**>(&c->d_chal_init), sizeof(DevChallenger));
-    coherent(reinterpret_cast<void**>(&c->h_rlog), reinterpret_cast<void**>(&c->d_rlog), sizeof(RoundLog));
+    coherent(reinterpret_cast<void**>(&c->h_rlog), reinterpret_cast<void**>(&c->d_rlog), 2 * sizeof(RoundLog));
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_final), sizeof(ef) << whirgpu_ctx::FINAL_MAX_LOG,
                          hipHostMallocDefault));
     HIPCHK(hipMalloc(&c->d_rchal, 2 * sizeof(DevChallenger)));
@@ -1518,6 +1547,8 @@ int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs,
       next_pd = nullptr;
       return pd;
     };
+    FinalLaunch fl;
+    bool final_launched = false;
     ef alpha = ch.sample_ef();
     std::vector<ef> fold_rand =
         initial_sumcheck(c, st, cfg.ff(0), whir::powers(alpha, st->points.size()),
@@ -1526,11 +1557,16 @@ int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs,
                          },
                          s);
     for (int round = 0; round <= cfg.n_rounds(); round++) {
+      if (final_launched) {   // enqueued after the last round (its sumcheck already advanced s)
+        device_final_replay(c, s, cfg, round, prev, P, ch, fl);
+        break;
+      }
       const int m = n - cfg.params.folding_factor.total_number(round);
       if (m != s.m) bad("internal: round size mismatch", WHIRGPU_BAD_STATE);
       if (round == cfg.n_rounds()) {
         if (device_final_fits(c, cfg, m, prev)) {
-          device_final_round(c, s, cfg, round, prev, P, ch);
+          fl = device_final_launch(c, s, cfg, round, prev, ch, nullptr);
+          device_final_replay(c, s, cfg, round, prev, P, ch, fl);
           break;
         }
         // final_round (prover/mod.rs:330-417)
@@ -1557,8 +1593,14 @@ int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs,
       whirgpu_prover_data* pd = cur_pd;
       if (device_round_fits(c, rp, m, prev)) {
         const RoundLaunch rl = device_round_launch(c, s, cfg, round, kn, pd, prev, ch);
-        // the next commitment depends only on the folded polynomial
-        if (round + 1 < cfg.n_rounds()) next_pd = commit_round(round + 1);
+        // the next commitment depends only on the folded polynomial; after
+        // the last round, the final round continues from the device state
+        if (round + 1 < cfg.n_rounds()) {
+          next_pd = commit_round(round + 1);
+        } else if (kn <= whirgpu_ctx::SC_LOG_CAP / 2 && device_final_fits(c, cfg, m - kn, pd)) {
+          fl = device_final_launch(c, s, cfg, round + 1, pd, ch, rl.la.out);
+          final_launched = true;
+        }
         fold_rand = device_round_replay(c, s, cfg, round, pd, prev, fold_rand, R, ch, rl);
         whirgpu_prover_data_destroy(prev);
         prev = pd;
