@@ -128,6 +128,9 @@ int whirgpu_challenger_sample(const whirgpu_ctx* ctx, whirgpu_challenger* ch, ui
 int whirgpu_statement_create(whirgpu_ctx* ctx, const uint32_t* evals, int evals_on_device, whirgpu_statement** out);
 /* InitialStatement::evaluate (src/whir/constraints/statement/initial.rs:143-152). */
 int whirgpu_statement_evaluate(whirgpu_statement* st, const uint32_t* point, uint32_t* eval_out);
+/* Clone of InitialStatement (#[derive(Clone)], src/whir/constraints/statement/initial.rs:31-32): the
+ * constraints and evaluations so far; a statement over a borrowed device buffer borrows the same buffer. */
+int whirgpu_statement_clone(const whirgpu_statement* st, whirgpu_statement** out);
 int whirgpu_statement_destroy(whirgpu_statement* st);
 
 /* WhirProof::from_protocol_parameters (src/whir/proof.rs:209-233). */

@@ -75,6 +75,32 @@ def test_device_resident_input_and_repeat():
     assert a == b == c
 
 
+def test_statement_clone_proves_like_the_original():
+    # InitialStatement::clone: constraints evaluated once, then commit + prove on
+    # clones (as the reference's benchmark does); every clone gives the oracle's
+    # proof, and committing on a clone leaves the original unchanged
+    W = product()
+    n = 14
+    poly, pts = inputs(n, 2, seed=5)
+    cfg = product_cfg(n)
+    want, _, _ = oracle_prove(oracle_params(n), poly, pts)
+    d = W.DeviceBuffer.from_array(poly)
+    for st0 in (W.InitialStatement(cfg, poly), W.InitialStatement(cfg, None, device_ptr=d.ptr)):
+        for p in pts:
+            st0.evaluate(p)
+        for _ in range(2):
+            st = st0.clone()
+            ch = cfg.challenger()
+            proof = W.WhirProof(cfg)
+            pd = W.CommitmentWriter(cfg).commit(proof, ch, st)
+            W.Prover(cfg).prove(proof, ch, st, pd)
+            assert proof.to_bytes() == want
+            proof.close()
+            st.close()
+        st0.close()
+    d.free()
+
+
 def test_config2_n24_matches_oracle():
     # the 2^24-variable benchmark proof: two seeded constraints, Poseidon2-16 Merkle
     W = product()

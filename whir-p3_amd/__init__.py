@@ -71,6 +71,7 @@ _SIGS = {
     "whirgpu_challenger_sample": (ctypes.c_int, [_VP, ctypes.POINTER(ChallengerState), U32P, ctypes.c_size_t]),
     "whirgpu_statement_create": (ctypes.c_int, [_VP, _VP, ctypes.c_int, ctypes.POINTER(_VP)]),
     "whirgpu_statement_evaluate": (ctypes.c_int, [_VP, U32P, U32P]),
+    "whirgpu_statement_clone": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_void_p)]),
     "whirgpu_statement_destroy": (ctypes.c_int, [_VP]),
     "whirgpu_proof_create": (ctypes.c_int, [_VP, ctypes.POINTER(_VP)]),
     "whirgpu_proof_destroy": (ctypes.c_int, [_VP]),
@@ -296,6 +297,17 @@ class InitialStatement:
         out = np.empty(4, dtype=np.uint32)
         check(lib().whirgpu_statement_evaluate(self.handle, u32p(pt), u32p(out)))
         return out
+
+    def clone(self):
+        """InitialStatement::clone: the constraints so far (a borrowed device buffer stays borrowed)."""
+        h = ctypes.c_void_p()
+        check(lib().whirgpu_statement_clone(self.handle, ctypes.byref(h)))
+        st = InitialStatement.__new__(InitialStatement)
+        st.cfg = self.cfg
+        st.handle = h
+        if hasattr(self, "_host"):
+            st._host = self._host
+        return st
 
     def close(self):
         if self.handle:

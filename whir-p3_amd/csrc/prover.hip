@@ -1416,6 +1416,28 @@ int whirgpu_statement_evaluate(whirgpu_statement* s, const uint32_t* point, uint
   });
 }
 
+int whirgpu_statement_clone(const whirgpu_statement* s, whirgpu_statement** out) {
+  whirgpu_statement* t = nullptr;
+  int rc = guarded([&] {
+    if (!s || !out) bad("null argument");
+    whirgpu_ctx* c = s->ctx;
+    use_device(c);
+    t = new whirgpu_statement(*s);   // constraints, evaluations and partial sums
+    if (s->owned) {
+      // the copy gets its own evaluations, as the reference's clone does
+      t->d_poly = nullptr;   // until the copy's buffer exists
+      t->d_poly = static_cast<u32*>(c->pool.get(s->poly_bytes));
+      HIPCHK(hipMemcpyAsync(t->d_poly, s->d_poly, s->poly_bytes, hipMemcpyDeviceToDevice, c->st));
+    }
+  });
+  if (rc) {
+    if (t) whirgpu_statement_destroy(t);
+    return rc;
+  }
+  *out = t;
+  return WHIRGPU_OK;
+}
+
 int whirgpu_statement_destroy(whirgpu_statement* s) {
   if (!s) return WHIRGPU_OK;
   if (s->owned && s->d_poly) s->ctx->pool.put(s->d_poly, s->poly_bytes);
