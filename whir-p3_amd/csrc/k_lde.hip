@@ -293,8 +293,9 @@ static auto pass2_for(int lg) -> void (*)(const T*, T*, int, int, int, int, Twid
   }
 }
 
-// Pass 1 takes about 64 KiB of lines per block (16K base or 4K extension
-// elements), at most 256 lines and at most L2.
+// Pass 1 takes about 32 KiB of lines per block for base elements (8K of them:
+// four blocks per CU; 16K ran 3% slower at 2^24) and 64 KiB for extension
+// elements (4K), at most 256 lines and at most L2.
 template <typename T, int LP2>
 static int lde_impl(const T* src, T* dst, T* tmp, int m, int k, int R, const Twiddles& tw, hipStream_t st) {
   const int l = m - k;
@@ -309,7 +310,7 @@ static int lde_impl(const T* src, T* dst, T* tmp, int m, int k, int R, const Twi
     hipLaunchKernelGGL(kern, dim3(1, 1u << R), dim3(256), sm, st, src, dst, l, R, k, 1, tw);
   } else {
     const int l2 = LP2, l1 = l - LP2;
-    const int elem_log = sizeof(T) == 4 ? 14 : 12;
+    const int elem_log = sizeof(T) == 4 ? 13 : 12;
     int tb_log = elem_log - l1;
     if (tb_log > 8) tb_log = 8;
     if (tb_log > l2) tb_log = l2;
