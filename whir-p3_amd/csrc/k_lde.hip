@@ -186,9 +186,17 @@ __global__ void __launch_bounds__(256) k_lde_pass1(const T* __restrict__ src, T*
   }
 }
 
+// Threads per pass-2 block.  Its W lines take about 64 KiB of LDS, so a CU
+// holds two blocks: base-field blocks of 512 threads keep four waves per SIMD
+// (-5% on the 2^24 base LDE); extension blocks stay at 256 (512 was slower).
+template <typename T>
+struct Pass2Threads {
+  static constexpr int value = sizeof(T) == 4 ? 512 : 256;
+};
+
 // Pass 2 (or the only pass when direct != 0).  Grid: (L1, 2^R); LG = l2.
 template <typename T, int LG>
-__global__ void __launch_bounds__(256) k_lde_pass2(const T* __restrict__ src, T* __restrict__ dst, int l, int R, int k,
+__global__ void __launch_bounds__(Pass2Threads<T>::value) k_lde_pass2(const T* __restrict__ src, T* __restrict__ dst, int l, int R, int k,
                                                    int direct, Twiddles tw) {
   extern __shared__ __align__(16) unsigned char smem[];
   using L = LineShape<LG>;
@@ -202,11 +210,11 @@ __global__ void __launch_bounds__(256) k_lde_pass2(const T* __restrict__ src, T*
   T* sh = line_storage<T, LG>(smem);
   fill_stage_twiddles<LG>(stw, tw);
   const int total = W * L2;
-  for (int e0 = threadIdx.x; e0 < total; e0 += U * 256) {
+  for (int e0 = threadIdx.x; e0 < total; e0 += U * Pass2Threads<T>::value) {
     T xs[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const int e = e0 + u * 256;
+      const int e = e0 + u * Pass2Threads<T>::value;
       if (e < total) {
         const int a = e >> l2, b2 = e & (L2 - 1);
         xs[u] = direct ? src[((u64)a << l) + b2] : src[((((u64)i0 * W + a) << l)) + ((u64)k1 << l2) + b2];
@@ -214,7 +222,7 @@ __global__ void __launch_bounds__(256) k_lde_pass2(const T* __restrict__ src, T*
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const int e = e0 + u * 256;
+      const int e = e0 + u * Pass2Threads<T>::value;
       if (e < total) {
         const int a = e >> l2, b2 = e & (L2 - 1);
         T x = xs[u];
@@ -307,7 +315,7 @@ static int lde_impl(const T* src, T* dst, T* tmp, int m, int k, int R, const Twi
     auto kern = pass2_for<T>(l);
     if (!kern) return -6;
     allow_lds(kern, sm);
-    hipLaunchKernelGGL(kern, dim3(1, 1u << R), dim3(256), sm, st, src, dst, l, R, k, 1, tw);
+    hipLaunchKernelGGL(kern, dim3(1, 1u << R), dim3(Pass2Threads<T>::value), sm, st, src, dst, l, R, k, 1, tw);
   } else {
     const int l2 = LP2, l1 = l - LP2;
     const int elem_log = sizeof(T) == 4 ? 13 : 12;
@@ -326,7 +334,7 @@ static int lde_impl(const T* src, T* dst, T* tmp, int m, int k, int R, const Twi
     size_t sm2 = lds_bytes(l2, W, sizeof(T));
     if (sm2 > 160 * 1024) return -2;
     allow_lds(k2, sm2);
-    hipLaunchKernelGGL(k2, dim3(1u << l1, 1u << R), dim3(256), sm2, st, tmp, dst, l, R, k, 0, tw);
+    hipLaunchKernelGGL(k2, dim3(1u << l1, 1u << R), dim3(Pass2Threads<T>::value), sm2, st, tmp, dst, l, R, k, 0, tw);
   }
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
