@@ -124,13 +124,21 @@ struct Batch {
   static constexpr int value = sizeof(T) == 4 ? 8 : 4;
 };
 
+// Threads per pass-1 block: base-field blocks hold 16K elements (64 KiB of
+// LDS, two blocks per CU) in 512 threads, four waves per SIMD (-10% on the
+// 2^24 base LDE against 256 threads); extension blocks keep 256.
+template <typename T>
+struct Pass1Threads {
+  static constexpr int value = sizeof(T) == 4 ? 512 : 256;
+};
+
 // Pass 1 of the four-step split.  Grid: (L2 / TB, W, 2^R); LG = l1, TB = 2^tb_log
 // lines per block (TB <= 256).  Thread t of a block always handles line
-// (column offset) t mod TB and rows t / TB + j * (256 / TB), so its coset
-// twist w_H^(i0 b) and four-step twiddle w_L^(b2 k1) advance by one product
-// per row instead of two table lookups.
+// (column offset) t mod TB and rows t / TB + j * (NT / TB) for NT threads, so
+// its coset twist w_H^(i0 b) and four-step twiddle w_L^(b2 k1) advance by one
+// product per row instead of two table lookups.
 template <typename T, int LG>
-__global__ void __launch_bounds__(256) k_lde_pass1(const T* __restrict__ src, T* __restrict__ V, int l, int R, int k,
+__global__ void __launch_bounds__(Pass1Threads<T>::value) k_lde_pass1(const T* __restrict__ src, T* __restrict__ V, int l, int R, int k,
                                                    int tb_log, Twiddles tw) {
   extern __shared__ __align__(16) unsigned char smem[];
   using L = LineShape<LG>;
@@ -147,7 +155,7 @@ __global__ void __launch_bounds__(256) k_lde_pass1(const T* __restrict__ src, T*
   fill_stage_twiddles<LG>(stw, tw);
   const int t = threadIdx.x & (TB - 1);
   const int row0 = threadIdx.x >> tb_log;
-  const int rstep = 256 >> tb_log;
+  const int rstep = Pass1Threads<T>::value >> tb_log;
   const int b2 = b2_0 + t;
   const T* colp = src + ((u64)a << l) + b2;
   u32 tcur = ONE, tstep = ONE;
@@ -301,9 +309,8 @@ static auto pass2_for(int lg) -> void (*)(const T*, T*, int, int, int, int, Twid
   }
 }
 
-// Pass 1 takes about 32 KiB of lines per block for base elements (8K of them:
-// four blocks per CU; 16K ran 3% slower at 2^24) and 64 KiB for extension
-// elements (4K), at most 256 lines and at most L2.
+// Pass 1 takes about 64 KiB of lines per block (16K base or 4K extension
+// elements), at most 256 lines and at most L2.
 template <typename T, int LP2>
 static int lde_impl(const T* src, T* dst, T* tmp, int m, int k, int R, const Twiddles& tw, hipStream_t st) {
   const int l = m - k;
@@ -318,7 +325,7 @@ static int lde_impl(const T* src, T* dst, T* tmp, int m, int k, int R, const Twi
     hipLaunchKernelGGL(kern, dim3(1, 1u << R), dim3(Pass2Threads<T>::value), sm, st, src, dst, l, R, k, 1, tw);
   } else {
     const int l2 = LP2, l1 = l - LP2;
-    const int elem_log = sizeof(T) == 4 ? 13 : 12;
+    const int elem_log = sizeof(T) == 4 ? 14 : 12;
     int tb_log = elem_log - l1;
     if (tb_log > 8) tb_log = 8;
     if (tb_log > l2) tb_log = l2;
@@ -330,7 +337,8 @@ static int lde_impl(const T* src, T* dst, T* tmp, int m, int k, int R, const Twi
     auto k2 = pass2_for<T>(l2);
     if (!k1 || !k2) return -6;
     allow_lds(k1, sm1);
-    hipLaunchKernelGGL(k1, dim3(1u << (l2 - tb_log), W, 1u << R), dim3(256), sm1, st, src, tmp, l, R, k, tb_log, tw);
+    hipLaunchKernelGGL(k1, dim3(1u << (l2 - tb_log), W, 1u << R), dim3(Pass1Threads<T>::value), sm1, st, src, tmp, l, R,
+                       k, tb_log, tw);
     size_t sm2 = lds_bytes(l2, W, sizeof(T));
     if (sm2 > 160 * 1024) return -2;
     allow_lds(k2, sm2);
