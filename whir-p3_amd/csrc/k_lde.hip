@@ -124,12 +124,14 @@ struct Batch {
   static constexpr int value = sizeof(T) == 4 ? 8 : 4;
 };
 
-// Threads per pass-1 block: base-field blocks hold 16K elements (64 KiB of
-// LDS, two blocks per CU) in 512 threads, four waves per SIMD (-10% on the
-// 2^24 base LDE against 256 threads); extension blocks keep 256.
+// Threads per pass-1 block: a block holds 16K base or 4K extension elements
+// (64 KiB of LDS, two blocks per CU) in 512 threads, four waves per SIMD
+// (-10% on the 2^24 base LDE, -2% on the extension LDEs, against 256).  A
+// block's lines (TB) must not outnumber its threads: the row step is NT / TB.
 template <typename T>
 struct Pass1Threads {
-  static constexpr int value = sizeof(T) == 4 ? 512 : 256;
+  static constexpr int value = 512;
+  static_assert(value >= 256, "pass 1 blocks take up to 256 lines");
 };
 
 // Pass 1 of the four-step split.  Grid: (L2 / TB, W, 2^R); LG = l1, TB = 2^tb_log
@@ -327,7 +329,7 @@ static int lde_impl(const T* src, T* dst, T* tmp, int m, int k, int R, const Twi
     const int l2 = LP2, l1 = l - LP2;
     const int elem_log = sizeof(T) == 4 ? 14 : 12;
     int tb_log = elem_log - l1;
-    if (tb_log > 8) tb_log = 8;
+    if (tb_log > 8) tb_log = 8;   // TB <= 256 < Pass1Threads: the row step NT / TB is at least 2
     if (tb_log > l2) tb_log = l2;
     if (tb_log < 0) tb_log = 0;
     size_t sm1 = lds_bytes(l1, 1 << tb_log, sizeof(T));
