@@ -580,7 +580,6 @@ __global__ void __launch_bounds__(256) k_round_queries(StepTr st, const u32* gri
   __shared__ u32 sp[16], in[8], out[8];
   __shared__ u64 q[RT_MAX_QUERIES];
   __shared__ u32 firsts[RT_MAX_QUERIES];
-  __shared__ u32 nq_sh;
   const int t = threadIdx.x;
   const P2C16& c = *cp;
   const WideLane wl = wide_lane(c, t & 15);
@@ -627,27 +626,23 @@ __global__ void __launch_bounds__(256) k_round_queries(StepTr st, const u32* gri
   }
   __syncthreads();
   // sorted distinct indices: a first occurrence goes to the number of
-  // distinct smaller values
-  u64 v = 0;
-  if (t < num_queries) {
-    v = q[t];
-    u32 first = 1;
-    for (int j = 0; j < t; j++)
-      if (q[j] == v) first = 0;
-    firsts[t] = first;
-  }
+  // distinct smaller values.  Every thread runs both scans over all entries
+  // (the same bound everywhere, unrolled, so the broadcast LDS reads are
+  // issued back to back) and counts the distinct values itself.
+  const u64 v = t < num_queries ? q[t] : 0;
+  u32 dup = 0;
+#pragma unroll 8
+  for (int j = 0; j < num_queries; j++) dup |= (u32)(j < t) & (u32)(q[j] == v);
+  if (t < num_queries) firsts[t] = dup ^ 1u;
   __syncthreads();
-  if (t == 0) {
-    u32 n = 0;
-    for (int j = 0; j < num_queries; j++) n += firsts[j];
-    nq_sh = n;
+  u32 nq = 0, pos = 0;
+#pragma unroll 8
+  for (int j = 0; j < num_queries; j++) {
+    const u32 f = firsts[j];
+    nq += f;
+    pos += f & (u32)(q[j] < v);
   }
-  __syncthreads();
-  const u32 nq = nq_sh;
-  if (t < num_queries && firsts[t]) {
-    u32 pos = 0;
-    for (int j = 0; j < num_queries; j++)
-      if (firsts[j] && q[j] < v) pos++;
+  if (t < num_queries && !dup) {
     rb.idx[pos] = v;
     rb.selz[pos] = pow(folded_gen, v);
     log->idx[pos] = (u32)v;

@@ -372,6 +372,58 @@ __global__ void __launch_bounds__(256) k_select_combine(ef* __restrict__ Wt, u64
   }
 }
 
+// Same for lo >= 8 with XT points per thread, 256 apart in one row: a
+// block's 256 * XT points (2^lo a multiple of that; the grid stride too)
+// share x >> lo, so the A entries are wave-uniform and come through scalar
+// loads, each used XT times, and a thread has 3 * XT loads of B in flight per
+// step.
+template <int XT>
+__global__ void __launch_bounds__(256) k_select_combine_rows(ef* __restrict__ Wt, u64 n, int lo, int hi,
+                                                             const ef* __restrict__ A, const u32* __restrict__ B,
+                                                             int nsel, const u32* nsel_dev) {
+  if (nsel_dev) nsel = (int)*nsel_dev;
+  const u32 mask = (1u << lo) - 1;
+  const u64 nh = 1ull << hi, nl = 1ull << lo;
+  for (u64 x0 = (u64)blockIdx.x * (256 * XT); x0 < n; x0 += (u64)gridDim.x * (256 * XT)) {
+    const u32 xh = __builtin_amdgcn_readfirstlane((u32)(x0 >> lo));
+    const u32 xl = ((u32)x0 & mask) + threadIdx.x;
+    const ef* __restrict__ Ah = A + xh;
+    const u32* __restrict__ Bl = B + xl;
+    u64 acc[XT][4] = {};
+    int j = 0;
+    for (; j + 3 <= nsel; j += 3) {
+      u32 b[3][XT];
+#pragma unroll
+      for (int q = 0; q < 3; q++)
+#pragma unroll
+        for (int u = 0; u < XT; u++) b[q][u] = Bl[(u64)(j + q) * nl + 256 * u];
+      const ef a0 = Ah[(u64)j * nh], a1 = Ah[(u64)(j + 1) * nh], a2 = Ah[(u64)(j + 2) * nh];
+#pragma unroll
+      for (int u = 0; u < XT; u++)
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+          acc[u][k] = fold32((u64)a2.c[k] * b[2][u] + ((u64)a1.c[k] * b[1][u] + ((u64)a0.c[k] * b[0][u] + acc[u][k])));
+    }
+    for (; j < nsel; j++) {
+      const ef a0 = Ah[(u64)j * nh];
+#pragma unroll
+      for (int u = 0; u < XT; u++) {
+        const u32 b0 = Bl[(u64)j * nl + 256 * u];
+#pragma unroll
+        for (int k = 0; k < 4; k++) acc[u][k] = fold32((u64)a0.c[k] * b0 + acc[u][k]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < XT; u++) {
+      ef r;
+#pragma unroll
+      for (int k = 0; k < 4; k++) r.c[k] = redc(acc[u][k]);
+      const u64 x = x0 + threadIdx.x + 256 * u;
+      Wt[x] = ef_add(Wt[x], r);
+    }
+  }
+}
+
 int select_combine(ef* Wt, int m, const u32* d_z, const ef* d_g, int nsel, const u32* d_nsel, ef* A, u32* B,
                    hipStream_t st) {
   if (nsel == 0) return 0;
@@ -383,7 +435,14 @@ int select_combine(ef* Wt, int m, const u32* d_z, const ef* d_g, int nsel, const
   u64 n = 1ull << m;
   u64 blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(k_select_combine, dim3((unsigned)blocks), dim3(256), 0, st, Wt, n, lo, hi, A, B, nsel, d_nsel);
+  if (lo >= 10)
+    hipLaunchKernelGGL(k_select_combine_rows<4>, dim3((unsigned)((blocks + 3) / 4)), dim3(256), 0, st, Wt, n, lo, hi,
+                       A, B, nsel, d_nsel);
+  else if (lo >= 8)
+    hipLaunchKernelGGL(k_select_combine_rows<1>, dim3((unsigned)blocks), dim3(256), 0, st, Wt, n, lo, hi, A, B, nsel,
+                       d_nsel);
+  else
+    hipLaunchKernelGGL(k_select_combine, dim3((unsigned)blocks), dim3(256), 0, st, Wt, n, lo, hi, A, B, nsel, d_nsel);
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 

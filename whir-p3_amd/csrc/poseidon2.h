@@ -220,7 +220,7 @@ __host__ __device__ __forceinline__ u64 cube_lazy(u32 x) {
 
 // External layer on inputs x < 2p: M4 on each quad, then each element plus
 // the sum of its position over the quads.  Outputs < 70p.
-__host__ __device__ __forceinline__ void external_linear16_wide(const u32* x, u64* y, u32 one, u64 zero) {
+__host__ __device__ __forceinline__ void external_linear16_wide(const u32* x, u64* y, u32 one, u32 two, u64 zero) {
   u64 n[16];
 #pragma unroll
   for (int c = 0; c < 16; c += 4) {
@@ -230,8 +230,8 @@ __host__ __device__ __forceinline__ void external_linear16_wide(const u32* x, u6
     const u64 t0123 = t01 + t23;
     const u64 t01123 = mad_u64(x1, one, t0123);
     const u64 t01233 = mad_u64(x3, one, t0123);
-    n[c + 3] = mad_u64(x0, 2u, t01233);   // 3x0 + x1 + x2 + 2x3
-    n[c + 1] = mad_u64(x2, 2u, t01123);   // x0 + 2x1 + 3x2 + x3
+    n[c + 3] = mad_u64(x0, two, t01233);   // 3x0 + x1 + x2 + 2x3
+    n[c + 1] = mad_u64(x2, two, t01123);   // x0 + 2x1 + 3x2 + x3
     n[c + 0] = t01123 + t01;              // 2x0 + 3x1 + x2 + x3
     n[c + 2] = t01233 + t23;              // x0 + x1 + 2x2 + 3x3
   }
@@ -246,15 +246,16 @@ __host__ __device__ __forceinline__ void external_linear16_wide(const u32* x, u6
 // Out: s[0] < p, s[1..15] < 2p.
 __host__ __device__ __forceinline__ void permute16_head(u32* s, const P2C16& c) {
   const u32 one = opaque(1u);
+  const u32 two = opaque(2u);
   const u64 zero = opaque(0ull);
   u32 x[16];
   u64 y[16];
-  external_linear16_wide(s, y, one, zero);
+  external_linear16_wide(s, y, one, two, zero);
 #pragma unroll
   for (int r = 0; r < 4; r++) {
 #pragma unroll
     for (int i = 0; i < 16; i++) x[i] = (u32)cube_lazy(full_reduce(y[i] + c.ext_init[r * 16 + i]));   // < 71p
-    external_linear16_wide(x, y, one, zero);
+    external_linear16_wide(x, y, one, two, zero);
   }
   s[0] = full_reduce(y[0]);
 #pragma unroll
@@ -266,6 +267,7 @@ __host__ __device__ __forceinline__ void permute16_head(u32* s, const P2C16& c) 
 // -1/16, -1/2^24].  In and out: s[0] < p, s[1..15] < 2p.
 __host__ __device__ __forceinline__ void permute16_internal(u32* s, const P2C16& c) {
   const u32 one = opaque(1u);
+  const u32 two = opaque(2u), four = opaque(4u);
   u64 z[16];
 #pragma unroll
   for (int i = 1; i < 16; i++) z[i] = s[i];
@@ -279,7 +281,7 @@ __host__ __device__ __forceinline__ void permute16_internal(u32* s, const P2C16&
     const u32 x0 = (u32)cube_lazy(full_reduce(y0 + c.internal[r]));   // y0 + rc < 37p; x0 < 1.75p
     const u64 sum = mad_u64(x0, one, sz);   // < 32p
     const u64 sumk = sum + TWO_P;
-    y0 = mad_u64(opaque(TWO_P - x0), 2u, sum);   // sum - 2 x0 + 4p, < 36p
+    y0 = mad_u64(TWO_P - x0, two, sum);   // sum - 2 x0 + 4p, < 36p
     z[1] = lazy_reduce(sum + z[1]);
     z[2] = lazy_reduce(sum + 2 * z[2]);
     z[3] = lazy_reduce(add_div2k<1>(sumk, (u32)z[3], one));
@@ -287,7 +289,7 @@ __host__ __device__ __forceinline__ void permute16_internal(u32* s, const P2C16&
     z[5] = lazy_reduce(sum + 4 * z[5]);
     z[6] = lazy_reduce(sub_div2k<1>(sumk, (u32)z[6], one));
     z[7] = lazy_reduce(mad_u64(TWO_P - (u32)z[7], 3u, sum));
-    z[8] = lazy_reduce(mad_u64(TWO_P - (u32)z[8], 4u, sum));
+    z[8] = lazy_reduce(mad_u64(TWO_P - (u32)z[8], four, sum));
     z[9] = lazy_reduce(add_div2k<8>(sumk, (u32)z[9], one));
     z[10] = lazy_reduce(add_div2k<3>(sumk, (u32)z[10], one));
     z[11] = lazy_reduce(add_div2k<24>(sumk, (u32)z[11], one));
@@ -304,6 +306,7 @@ __host__ __device__ __forceinline__ void permute16_internal(u32* s, const P2C16&
 // The four terminal external rounds.  In: s[0] < p, s[1..15] < 2p.  Out: s < p.
 __host__ __device__ __forceinline__ void permute16_tail(u32* s, const P2C16& c) {
   const u32 one = opaque(1u);
+  const u32 two = opaque(2u);
   const u64 zero = opaque(0ull);
   u32 x[16];
   u64 y[16];
@@ -313,7 +316,7 @@ __host__ __device__ __forceinline__ void permute16_tail(u32* s, const P2C16& c) 
   for (int r = 0; r < 4; r++) {
 #pragma unroll
     for (int i = 0; i < 16; i++) x[i] = (u32)cube_lazy(full_reduce(y[i] + c.ext_term[r * 16 + i]));
-    external_linear16_wide(x, y, one, zero);
+    external_linear16_wide(x, y, one, two, zero);
   }
 #pragma unroll
   for (int i = 0; i < 16; i++) s[i] = full_reduce(y[i]);

@@ -47,14 +47,16 @@ __device__ __forceinline__ u32 csub(u32 x) {
 // M4 over the quads is M4 applied to z = x + X, where X is the sum of the
 // four quads' elements at this position (32-bit rotated adds, each sum below
 // 2p brought below p).  Out: 2z + 3 z1 + z2 + z3 < 14p for z < 2p.
-__device__ __forceinline__ u64 wide_external_linear(u32 x) {
+// one and two are opaque(1u) and opaque(2u): the sums stay multiply-adds of
+// 32-bit operands instead of 64-bit adds of values widened by extra moves.
+__device__ __forceinline__ u64 wide_external_linear(u32 x, u32 one, u32 two) {
   u32 X = csub(x + dpp<ROR(4)>(x));
   X = csub(X + dpp<ROR(8)>(X));
   const u32 z = x + X;
   const u32 r1 = xdpp<QROT1>(z);
   const u32 r2 = xdpp<QROT2>(z);
   const u32 r3 = xdpp<QROT3>(z);
-  return mad_u64(z, 2u, mad_u64(r1, 3u, (u64)r2 + r3));
+  return mad_u64(z, two, mad_u64(r1, 3u, mad_u64(r3, one, (u64)r2)));
 }
 
 // Sum of the row's 16 lanes for x < p: < p.
@@ -80,6 +82,7 @@ struct WideLane {
   u32 ext_init[4];
   u32 ext_term[4];
   u32 diag;
+  u32 one, two;   // opaque(1u), opaque(2u)
 };
 
 // diag(V) of the width-16 internal layer in Montgomery form, element j.
@@ -119,16 +122,18 @@ __device__ __forceinline__ WideLane wide_lane(const P2C16& c, int j) {
   // a plain 32-bit register: the switch above otherwise becomes 64-bit
   // constants and the products with it 64-bit multiplies
   asm volatile("" : "+v"(w.diag));
+  w.one = opaque(1u);
+  w.two = opaque(2u);
   return w;
 }
 
 // x: this lane's state element (< p); j = lane & 15.  Returns the lane's
 // element of the permuted state (< p).
 __device__ __forceinline__ u32 permute16_wide(u32 x, int j, const WideLane& w, const P2C16& c) {
-  u64 y = wide_external_linear(x);
+  u64 y = wide_external_linear(x, w.one, w.two);
 #pragma unroll
   for (int r = 0; r < 4; r++)   // y + rc < 15p; the S-box output < 1.75p is brought below p
-    y = wide_external_linear(csub((u32)cube_lazy(full_reduce(mad_u64(w.ext_init[r], 1u, y)))));
+    y = wide_external_linear(csub((u32)cube_lazy(full_reduce(mad_u64(w.ext_init[r], 1u, y)))), w.one, w.two);
   x = full_reduce(y);
 #pragma unroll
   for (int r = 0; r < P2C16::RP; r++) {
@@ -140,7 +145,7 @@ __device__ __forceinline__ u32 permute16_wide(u32 x, int j, const WideLane& w, c
   y = x;
 #pragma unroll
   for (int r = 0; r < 4; r++)
-    y = wide_external_linear(csub((u32)cube_lazy(full_reduce(mad_u64(w.ext_term[r], 1u, y)))));
+    y = wide_external_linear(csub((u32)cube_lazy(full_reduce(mad_u64(w.ext_term[r], 1u, y)))), w.one, w.two);
   return full_reduce(y);
 }
 
