@@ -98,22 +98,26 @@ __global__ void __launch_bounds__(256) k_tree_level(const u32* __restrict__ in, 
   o[1] = make_uint4(s[4], s[5], s[6], s[7]);
 }
 
-// Seven small levels in one launch, with a 16-lane permutation per node
-// (latency rather than throughput decides there): block b stages the 128
-// digests [128 b, 128 b + 128) of the level at `off` (n digests, a multiple of
-// 128) in LDS and reduces them to one digest seven levels up, writing every
+// log2(S) small levels in one launch, with a 16-lane permutation per node
+// (latency rather than throughput decides there): block b (8 S lanes) stages
+// the S digests [S b, S b + S) of the level at `off` (n digests, a multiple of
+// S) in LDS and reduces them to one digest log2(S) levels up, writing every
 // level in between; the levels are laid out one after another (the level of
-// size n at off, the next of size n / 2 at off + n, ...).  64 nodes per step.
-__global__ void __launch_bounds__(1024) k_tree_sub_wide(u32* __restrict__ tree, u64 off, u64 n, P2C16 c) {
-  __shared__ u32 sh[2][128 * 8];
+// size n at off, the next of size n / 2 at off + n, ...).  S / 2 nodes per
+// step.  S = 32 keeps a block at one wave per SIMD, so a launch of few blocks
+// runs its levels at the permutation's latency (with S = 128 each block holds
+// four waves per SIMD for its first level).
+template <int S>
+__global__ void __launch_bounds__(8 * S) k_tree_sub_wide(u32* __restrict__ tree, u64 off, u64 n, P2C16 c) {
+  __shared__ u32 sh[2][S * 8];
   const int t = threadIdx.x;
   const int j = t & 15;
   const WideLane w = wide_lane(c, j);
   const u64 b = blockIdx.x;
-  for (int e = t; e < 128 * 8; e += 1024) sh[0][e] = tree[(off + b * 128) * 8 + e];
+  for (int e = t; e < S * 8; e += 8 * S) sh[0][e] = tree[(off + b * S) * 8 + e];
   __syncthreads();
   int cur = 0;
-  for (int m = 128; m > 1; m >>= 1) {
+  for (int m = S; m > 1; m >>= 1) {
     const int m_out = m >> 1;
     const int p = t >> 4;
     if (4 * (t >> 6) < m_out) {   // wave-uniform; below 4 nodes part of the wave idles
@@ -131,20 +135,20 @@ __global__ void __launch_bounds__(1024) k_tree_sub_wide(u32* __restrict__ tree, 
   }
 }
 
-// The top levels of a tree in one launch: n (<= 128) digests at `off` are
+// The top levels of a tree in one launch: n (<= 32) digests at `off` are
 // staged in LDS and reduced to the root, every level written out.  One block
-// of 1024 lanes, 64 nodes per step.
+// of 256 lanes, 16 nodes per step.
 // Waves whose four nodes are all past the level's end skip the permutation
-// (the lane-parallel permutation only needs its own wave's lanes), so the
-// small top levels run one wave per SIMD instead of four.  The root is also
-// written to `root_out` when given (host-visible memory).
-__global__ void __launch_bounds__(1024) k_tree_top_wide(u32* __restrict__ tree, u64 off, int n, P2C16 c,
-                                                        u32* root_out) {
-  __shared__ u32 sh[2][128 * 8];
+// (the lane-parallel permutation only needs its own wave's lanes).  The root
+// is also written to `root_out` when given (host-visible memory).
+constexpr int TREE_TOP = 32;
+__global__ void __launch_bounds__(8 * TREE_TOP) k_tree_top_wide(u32* __restrict__ tree, u64 off, int n, P2C16 c,
+                                                                u32* root_out) {
+  __shared__ u32 sh[2][TREE_TOP * 8];
   const int t = threadIdx.x;
   const int j = t & 15;
   const WideLane w = wide_lane(c, j);
-  for (int e = t; e < n * 8; e += 1024) sh[0][e] = tree[off * 8 + e];
+  for (int e = t; e < n * 8; e += 8 * TREE_TOP) sh[0][e] = tree[off * 8 + e];
   __syncthreads();
   int cur = 0;
   while (n > 1) {
@@ -204,12 +208,14 @@ int merkle_leaves(const u32* rows, int row_len, u64 height, int leaf_width, cons
 }
 
 // Levels with more than 2^14 nodes use one lane per node (throughput); below
-// that, seven levels per launch with 16 lanes per node (latency), and the last
-// (at most 7) levels in one block.
+// that, seven or five levels per launch with 16 lanes per node (latency), and
+// the last (at most 5) levels in one block.
+constexpr int TREE_SUB = 32;
+static_assert(TREE_SUB <= TREE_TOP, "the top block takes what the subtree launches leave");
 int merkle_tree(u64 height, const P2C16& c16, u32* tree, hipStream_t st, u32* root_out) {
   if (height == 0 || (height & (height - 1))) return -1;
   u64 off = 0, n = height;
-  while (n > 128) {
+  while (n > TREE_SUB) {
     const u64 n_out = n >> 1;
     if (n_out > (1u << 14)) {
       unsigned blocks = (unsigned)((n_out + 255) / 256);
@@ -220,14 +226,23 @@ int merkle_tree(u64 height, const P2C16& c16, u32* tree, hipStream_t st, u32* ro
       n = n_out;
       continue;
     }
-    // n is a power of two above 128: n / 128 subtrees, levels n .. n / 64 written
-    hipLaunchKernelGGL(k_tree_sub_wide, dim3((unsigned)(n / 128)), dim3(1024), 0, st, tree, off, n, c16);
+    // n is a power of two above S: n / S subtrees, levels n / 2 .. n / S
+    // written.  From 2^15 nodes (256 blocks or more) 128-digest subtrees: the
+    // first level's work then fills the chip and seven levels per launch cost
+    // less than five (22 against 23 us at 2^15); below, 32-digest subtrees at
+    // one wave per SIMD.
+    const u64 S = n >= (1u << 15) ? 128 : TREE_SUB;
+    if (S == 128)
+      hipLaunchKernelGGL(k_tree_sub_wide<128>, dim3((unsigned)(n / 128)), dim3(8 * 128), 0, st, tree, off, n, c16);
+    else
+      hipLaunchKernelGGL(k_tree_sub_wide<TREE_SUB>, dim3((unsigned)(n / TREE_SUB)), dim3(8 * TREE_SUB), 0, st, tree,
+                         off, n, c16);
     if (hipGetLastError() != hipSuccess) return -5;
-    off += 2 * n - n / 64;
-    n /= 128;
+    off += 2 * n - n / (S / 2);
+    n /= S;
   }
   if (n > 1) {
-    hipLaunchKernelGGL(k_tree_top_wide, dim3(1), dim3(1024), 0, st, tree, off, (int)n, c16, root_out);
+    hipLaunchKernelGGL(k_tree_top_wide, dim3(1), dim3(8 * TREE_TOP), 0, st, tree, off, (int)n, c16, root_out);
     if (hipGetLastError() != hipSuccess) return -5;
   } else if (root_out) {
     return 1;   // single leaf: the root was not written
