@@ -38,7 +38,8 @@ def inputs(n, npts, seed=0):
 
 
 @pytest.mark.parametrize("n,kw", CASES)
-@pytest.mark.parametrize("npts", [0, 1, 2])
+# five points: one full four-point weight launch, then one accumulating launch
+@pytest.mark.parametrize("npts", [0, 1, 2, 5])
 def test_proof_bytes_match_oracle(n, kw, npts):
     W = product()
     poly, pts = inputs(n, npts, seed=n)

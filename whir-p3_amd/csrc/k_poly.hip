@@ -298,14 +298,32 @@ int eq_dot_ext(const ef* f, int nf, int sb, int lo, const EqTables& tabs, int K,
 // ---------------------------------------------------------------- weight accumulation
 // W[x] = (acc ? W[x] : 0) + sum_k s_k Ehi[k][x >> lo] * Elo[k][x & mask], with s_k = scale[k] when scale is
 // given (device memory) and 1 otherwise.
+// Without scales the K products are summed unreduced: per component, each
+// term's four 32x32 products (each below p^2) are added to the running sum,
+// which fold32 keeps below 2^57 + 2^32 (4 p^2 + 2^58 < 2^64), and one
+// Montgomery reduction per component follows (its input is below 4 p^2).
+// The result is the canonical sum of the reduced products, as with scales.
 __global__ void __launch_bounds__(256) k_eq_combine(ef* __restrict__ Wt, u64 n, int lo, EqTables tabs, int K,
                                                     int accumulate, const ef* __restrict__ scale) {
   const u32 mask = (1u << lo) - 1;
   for (u64 x = (u64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (u64)gridDim.x * blockDim.x) {
     ef acc = accumulate ? Wt[x] : ef_zero();
-    for (int k = 0; k < K; k++) {
-      const ef t = ef_mul(tabs.hi[k][x >> lo], tabs.lo[k][x & mask]);
-      acc = ef_add(acc, scale ? ef_mul(scale[k], t) : t);
+    if (scale) {
+      for (int k = 0; k < K; k++) {
+        const ef t = ef_mul(tabs.hi[k][x >> lo], tabs.lo[k][x & mask]);
+        acc = ef_add(acc, ef_mul(scale[k], t));
+      }
+    } else if (K > 0) {
+      u64 c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+      for (int k = 0; k < K; k++) {
+        const ef a = tabs.hi[k][x >> lo], b = tabs.lo[k][x & mask];
+        const u32 b1 = add(dbl(b.c[1]), b.c[1]), b2 = add(dbl(b.c[2]), b.c[2]), b3 = add(dbl(b.c[3]), b.c[3]);
+        c0 = fold32(c0 + (u64)a.c[0] * b.c[0] + (u64)a.c[1] * b3 + (u64)a.c[2] * b2 + (u64)a.c[3] * b1);
+        c1 = fold32(c1 + (u64)a.c[0] * b.c[1] + (u64)a.c[1] * b.c[0] + (u64)a.c[2] * b3 + (u64)a.c[3] * b2);
+        c2 = fold32(c2 + (u64)a.c[0] * b.c[2] + (u64)a.c[1] * b.c[1] + (u64)a.c[2] * b.c[0] + (u64)a.c[3] * b3);
+        c3 = fold32(c3 + (u64)a.c[0] * b.c[3] + (u64)a.c[1] * b.c[2] + (u64)a.c[2] * b.c[1] + (u64)a.c[3] * b.c[0]);
+      }
+      acc = ef_add(acc, ef{{redc_wide(c0), redc_wide(c1), redc_wide(c2), redc_wide(c3)}});
     }
     Wt[x] = acc;
   }
