@@ -39,6 +39,9 @@ struct BlockChallenger {
   int out_len;
   const WideLane* wl;
   const P2C16* c;
+  // Output element t & 15 (t & 15 < 8) in this lane's register, so a sample
+  // is a lane read instead of an LDS round trip.
+  u32 lane_out;
 
   __device__ void load(const DevChallenger& s) {
     const int t = threadIdx.x;
@@ -49,6 +52,7 @@ struct BlockChallenger {
     }
     in_len = (int)s.in_len;
     out_len = (int)s.out_len;
+    lane_out = (t & 15) < 8 ? s.out[t & 15] : 0u;
     __syncthreads();
   }
   __device__ void store(DevChallenger* s) const {   // block 0 only
@@ -72,6 +76,7 @@ struct BlockChallenger {
     __syncthreads();
     if (t < 16) sp[t] = x;
     if (t < 8) out[t] = x;
+    lane_out = x;
     __syncthreads();
     in_len = 0;
     out_len = 8;
@@ -86,7 +91,8 @@ struct BlockChallenger {
   __device__ u32 sample() {
     if (in_len > 0 || out_len == 0) duplex();
     out_len--;
-    return out[out_len];
+    // lane out_len of every wave holds element out_len (lanes 0..15 hold 0..15)
+    return (u32)__builtin_amdgcn_readlane((int)lane_out, out_len);
   }
   // Four samples as an extension element.  The loop is not unrolled (each
   // sample may duplex), so the samples shift through registers instead of
