@@ -230,6 +230,9 @@ typedef struct whirgpu_job {
   int status;                   /* out: WHIRGPU_PENDING until done, then a WHIRGPU_* code */
   double queue_ms;              /* out: submit to start */
   double latency_ms;            /* out: submit to completion */
+  char error[160];              /* out: whirgpu_last_error() of the failed call ("" on success); the
+                                   server runs a job on a worker thread, whose last error the
+                                   submitting thread cannot read */
 } whirgpu_job;
 
 #define WHIRGPU_PENDING (-1)
@@ -237,9 +240,13 @@ typedef struct whirgpu_job {
 typedef struct whirgpu_server whirgpu_server;
 
 /* A pool of `workers` host threads on device `device`, each with its own
- * contexts and HIP stream, taking jobs in submission order.  A job starts when
- * a worker is free and its estimated device memory fits in `hbm_budget` bytes
- * beside the jobs already running (0: 80% of the free HBM at creation).
+ * contexts and HIP stream, taking jobs in submission order.  Admission counts
+ * the device memory the workers' contexts really hold (their buffers are kept
+ * between jobs) plus what running jobs may still grow by: a job starts when a
+ * worker is free, that total plus the job's own growth fits in `hbm_budget`
+ * bytes (0: 80% of the free HBM at creation), and its growth fits in the HBM
+ * free at that moment.  A job that does not fit makes idle workers release
+ * contexts it cannot use; with nothing running a job is always admitted.
  * params->num_variables is ignored (each job has its own). */
 int whirgpu_server_create(int device, const whirgpu_params* params, const whirgpu_poseidon2* perms, int workers,
                           size_t hbm_budget, whirgpu_server** out);
@@ -247,7 +254,8 @@ int whirgpu_server_submit(whirgpu_server* s, whirgpu_job* job);
 /* Blocks until `job` completes (timeout_ms < 0: no limit); returns its status
  * or WHIRGPU_PENDING on timeout. */
 int whirgpu_server_wait(whirgpu_server* s, whirgpu_job* job, int timeout_ms);
-/* Jobs queued / running right now, and the device-memory estimate in use. */
+/* Jobs queued / running right now, and the device bytes held by the
+ * workers' contexts plus the growth reserved by running jobs. */
 int whirgpu_server_stats(whirgpu_server* s, size_t* queued, size_t* running, size_t* bytes_in_use);
 /* Finishes every submitted job, then stops the workers. */
 int whirgpu_server_destroy(whirgpu_server* s);

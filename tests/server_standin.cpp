@@ -1,0 +1,384 @@
+// Stand-in for libwhirgpu.so behind the batching server
+// (whir-p3_amd/csrc/server.cpp), for checking the server's threading and its
+// admission by device memory on a host without a GPU.  Test infrastructure
+// only: tests/test_server_threads.py builds it with server.cpp under
+// ThreadSanitizer and runs the scenarios in main().
+//
+// A context models the device memory a real one holds: 1 MiB at creation,
+// then the expected peak of its size after its first proof (the figure the
+// server estimates before it has measured one).  A proof sleeps for a time
+// that grows with its size and writes bytes derived from its inputs, so each
+// result can be checked.  Global counters record the most proofs running at
+// once, the most modelled device memory held, and whether a context was
+// used by two threads at once.
+#include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <random>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "internal.h"
+
+namespace {
+
+thread_local std::string t_err;
+std::atomic<long long> g_live{0}, g_max_live{0};
+std::atomic<int> g_running{0}, g_max_running{0}, g_overlap{0}, g_destroyed{0};
+long long g_total_mem = 64ll << 30;
+
+void bump_max(std::atomic<long long>& m, long long v) {
+  long long c = m.load();
+  while (v > c && !m.compare_exchange_weak(c, v)) {
+  }
+}
+void bump_max(std::atomic<int>& m, int v) {
+  int c = m.load();
+  while (v > c && !m.compare_exchange_weak(c, v)) {
+  }
+}
+
+// The server's own estimate for a context of 2^n variables with host inputs.
+long long peak_bytes(uint32_t n) { return (long long)(1ull << n) * 40 + (16ll << 20); }
+
+uint32_t mix(uint32_t a, uint32_t b) { return (a ^ (b + 0x9e3779b9u + (a << 6) + (a >> 2))) * 0x85ebca6bu; }
+
+}  // namespace
+
+struct whirgpu_ctx {
+  uint32_t n = 0;
+  long long bytes = 0;
+  std::atomic<int> users{0};
+};
+struct whirgpu_statement {
+  whirgpu_ctx* ctx;
+  uint32_t e0;
+  uint32_t acc;
+};
+struct whirgpu_proof {
+  uint32_t n;
+  uint32_t acc;
+};
+struct whirgpu_prover_data {
+  int unused;
+};
+
+extern "C" {
+
+const char* whirgpu_last_error(void) { return t_err.c_str(); }
+
+int whirgpu_create(int, const whirgpu_params* p, const whirgpu_poseidon2*, whirgpu_ctx** out) {
+  if (p->num_variables < 4) {
+    t_err = "num_variables too small for the folding schedule";
+    return WHIRGPU_BAD_PARAMS;
+  }
+  auto* c = new whirgpu_ctx();
+  c->n = p->num_variables;
+  c->bytes = 1 << 20;
+  bump_max(g_max_live, g_live += c->bytes);
+  *out = c;
+  return WHIRGPU_OK;
+}
+
+int whirgpu_destroy(whirgpu_ctx* c) {
+  g_live -= c->bytes;
+  g_destroyed++;
+  delete c;
+  return WHIRGPU_OK;
+}
+
+int whirgpu_statement_create(whirgpu_ctx* c, const uint32_t* evals, int, whirgpu_statement** out) {
+  *out = new whirgpu_statement{c, evals[0], evals[0]};
+  return WHIRGPU_OK;
+}
+
+int whirgpu_statement_evaluate(whirgpu_statement* st, const uint32_t* point, uint32_t* eval_out) {
+  for (int i = 0; i < 4; i++) eval_out[i] = mix(st->e0 + i, point[i]);
+  st->acc = mix(st->acc, eval_out[0]);
+  return WHIRGPU_OK;
+}
+
+int whirgpu_statement_destroy(whirgpu_statement* st) {
+  delete st;
+  return WHIRGPU_OK;
+}
+
+int whirgpu_challenger_new(whirgpu_challenger* ch) {
+  memset(ch, 0, sizeof *ch);
+  return WHIRGPU_OK;
+}
+
+int whirgpu_challenger_observe_domain_separator(const whirgpu_ctx* c, whirgpu_challenger* ch) {
+  ch->sponge[0] = c->n;
+  return WHIRGPU_OK;
+}
+
+int whirgpu_proof_create(whirgpu_ctx* c, whirgpu_proof** out) {
+  *out = new whirgpu_proof{c->n, 0};
+  return WHIRGPU_OK;
+}
+
+int whirgpu_proof_destroy(whirgpu_proof* p) {
+  delete p;
+  return WHIRGPU_OK;
+}
+
+int whirgpu_commit(whirgpu_ctx*, whirgpu_proof* proof, whirgpu_challenger*, whirgpu_statement* st,
+                   whirgpu_prover_data** out) {
+  proof->acc = st->acc;
+  *out = new whirgpu_prover_data{0};
+  return WHIRGPU_OK;
+}
+
+int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* ch, const whirgpu_statement*,
+                  whirgpu_prover_data* pd) {
+  if (c->users.fetch_add(1) != 0) g_overlap++;
+  bump_max(g_max_running, ++g_running);
+  if (c->bytes < peak_bytes(c->n)) {
+    bump_max(g_max_live, g_live += peak_bytes(c->n) - c->bytes);
+    c->bytes = peak_bytes(c->n);
+  }
+  std::this_thread::sleep_for(std::chrono::microseconds(200 + 40 * c->n));
+  proof->acc = mix(proof->acc, ch->sponge[0]);
+  g_running--;
+  c->users--;
+  delete pd;
+  return WHIRGPU_OK;
+}
+
+int whirgpu_prover_data_destroy(whirgpu_prover_data* pd) {
+  delete pd;
+  return WHIRGPU_OK;
+}
+
+int whirgpu_proof_serialize(const whirgpu_proof* p, uint8_t* out, size_t cap, size_t* len) {
+  *len = 8;
+  if (cap < 8) {
+    t_err = "buffer too small";
+    return WHIRGPU_BUFFER_TOO_SMALL;
+  }
+  memcpy(out, &p->n, 4);
+  memcpy(out + 4, &p->acc, 4);
+  return WHIRGPU_OK;
+}
+
+size_t whirgpu_internal_ctx_bytes(const whirgpu_ctx* c) { return (size_t)c->bytes; }
+
+int whirgpu_internal_mem_info(int, size_t* free_b, size_t* total_b) {
+  long long live = g_live.load();
+  *free_b = (size_t)(g_total_mem > live ? g_total_mem - live : 0);
+  *total_b = (size_t)g_total_mem;
+  return WHIRGPU_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- scenarios
+namespace {
+
+struct Req {
+  uint32_t n;
+  std::vector<uint32_t> evals, points;
+  uint32_t npts;
+  std::vector<uint32_t> out_evals;
+  uint8_t proof[64];
+  whirgpu_job job{};
+};
+
+void make(Req& r, uint32_t n, uint32_t npts, uint32_t seed) {
+  r.n = n;
+  r.npts = npts;
+  r.evals = {seed, seed + 1, seed + 2, seed + 3};
+  r.points.assign((size_t)4 * n * (npts ? npts : 1), 0);
+  for (size_t i = 0; i < r.points.size(); i++) r.points[i] = mix(seed, (uint32_t)i);
+  r.out_evals.assign(4 * (npts ? npts : 1), 0);
+  whirgpu_job& j = r.job;
+  j.num_variables = n;
+  j.evals = r.evals.data();
+  j.points = r.points.data();
+  j.npoints = npts;
+  j.evals_out = r.out_evals.data();
+  j.proof_out = r.proof;
+  j.proof_cap = sizeof r.proof;
+}
+
+// The proof bytes the stand-in writes for a request.
+bool check(const Req& r) {
+  if (r.job.status != 0 || r.job.proof_len != 8) return false;
+  uint32_t acc = r.evals[0];
+  for (uint32_t k = 0; k < r.npts; k++) acc = mix(acc, mix(r.evals[0], r.points[(size_t)4 * r.n * k]));
+  acc = mix(acc, r.n);
+  uint32_t n, a;
+  memcpy(&n, r.proof, 4);
+  memcpy(&a, r.proof + 4, 4);
+  return n == r.n && a == acc;
+}
+
+whirgpu_server* start(int workers, size_t budget) {
+  whirgpu_params p{};
+  whirgpu_poseidon2 perms{};
+  whirgpu_server* s = nullptr;
+  if (whirgpu_server_create(0, &p, &perms, workers, budget, &s)) return nullptr;
+  return s;
+}
+
+int fail(const char* what) {
+  fprintf(stderr, "FAILED: %s\n", what);
+  return 1;
+}
+
+void reset() {
+  g_max_live = g_live.load();
+  g_max_running = 0;
+  g_overlap = 0;
+  g_destroyed = 0;
+}
+
+// Three threads submit mixed sizes (and one bad job each) while the main
+// thread waits on every job, with short timeouts when `timed`.
+int scenario_mix(bool timed) {
+  reset();
+  whirgpu_server* s = start(8, 0);
+  if (!s) return fail("server_create");
+  const int per = 80;
+  std::vector<Req> reqs(3 * per);
+  std::vector<std::thread> subs;
+  for (int t = 0; t < 3; t++) {
+    subs.emplace_back([&, t] {
+      std::mt19937 rng(t);
+      for (int i = 0; i < per; i++) {
+        Req& r = reqs[t * per + i];
+        const bool bad = i == per / 2;
+        make(r, bad ? 2 : 4 + rng() % 11, rng() % 5, 1000 * t + i);
+        if (whirgpu_server_submit(s, &r.job)) r.job.status = -2;
+      }
+    });
+  }
+  for (auto& th : subs) th.join();
+  for (auto& r : reqs) {
+    while (whirgpu_server_wait(s, &r.job, timed ? 5 : -1) == WHIRGPU_PENDING) {
+      size_t q, run, b;
+      whirgpu_server_stats(s, &q, &run, &b);
+    }
+  }
+  whirgpu_server_destroy(s);
+  for (auto& r : reqs) {
+    if (r.n == 2) {
+      if (r.job.status != WHIRGPU_BAD_PARAMS) return fail("bad job status");
+      if (!strstr(r.job.error, "num_variables")) return fail("bad job error text not propagated");
+    } else if (!check(r)) {
+      return fail("proof bytes");
+    }
+  }
+  if (g_overlap) return fail("a context was used by two threads at once");
+  if (g_live.load() != 0) return fail("device memory left after destroy");
+  printf("mix ok: %zu jobs, at most %d running\n", reqs.size(), g_max_running.load());
+  return 0;
+}
+
+// A budget of k contexts of one size admits exactly k jobs at once.
+int scenario_budget(int k) {
+  reset();
+  const uint32_t n = 12;
+  const size_t budget = (size_t)k * peak_bytes(n) + peak_bytes(n) / 2;
+  whirgpu_server* s = start(8, budget);
+  if (!s) return fail("server_create");
+  std::vector<Req> reqs(48);
+  for (size_t i = 0; i < reqs.size(); i++) {
+    make(reqs[i], n, 1, (uint32_t)i);
+    whirgpu_server_submit(s, &reqs[i].job);
+  }
+  for (auto& r : reqs) whirgpu_server_wait(s, &r.job, -1);
+  whirgpu_server_destroy(s);
+  for (auto& r : reqs)
+    if (!check(r)) return fail("proof bytes");
+  if (g_max_running.load() != k) {
+    fprintf(stderr, "max running %d, budget fits %d\n", g_max_running.load(), k);
+    return fail("admission count");
+  }
+  if (g_max_live.load() > (long long)budget) return fail("device memory above the budget");
+  printf("budget ok: k=%d, at most %d running, peak %lld of %zu bytes\n", k, g_max_running.load(),
+         g_max_live.load(), budget);
+  return 0;
+}
+
+// Mixed sizes with a budget of about two contexts: idle workers must release
+// contexts of other sizes for the queue to move, without exceeding it.
+int scenario_trim() {
+  reset();
+  const size_t budget = (size_t)(peak_bytes(14) + peak_bytes(13));
+  whirgpu_server* s = start(6, budget);
+  if (!s) return fail("server_create");
+  std::vector<Req> reqs(60);
+  for (size_t i = 0; i < reqs.size(); i++) {
+    make(reqs[i], 12 + (uint32_t)(i % 3), (uint32_t)(i % 4), (uint32_t)(7 * i));
+    whirgpu_server_submit(s, &reqs[i].job);
+  }
+  for (auto& r : reqs) whirgpu_server_wait(s, &r.job, -1);
+  const int destroyed_while_running = g_destroyed.load();
+  whirgpu_server_destroy(s);
+  for (auto& r : reqs)
+    if (!check(r)) return fail("proof bytes");
+  if (g_max_live.load() > (long long)budget) return fail("device memory above the budget");
+  if (destroyed_while_running == 0) return fail("no context was released to admit a job");
+  printf("trim ok: %d contexts released, peak %lld of %zu bytes\n", destroyed_while_running, g_max_live.load(),
+         budget);
+  return 0;
+}
+
+// A budget smaller than any job: one job at a time, all complete.
+int scenario_oversized() {
+  reset();
+  whirgpu_server* s = start(4, 1);
+  if (!s) return fail("server_create");
+  std::vector<Req> reqs(12);
+  for (size_t i = 0; i < reqs.size(); i++) {
+    make(reqs[i], 10 + (uint32_t)(i % 4), 1, (uint32_t)i);
+    whirgpu_server_submit(s, &reqs[i].job);
+  }
+  for (auto& r : reqs) whirgpu_server_wait(s, &r.job, -1);
+  whirgpu_server_destroy(s);
+  for (auto& r : reqs)
+    if (!check(r)) return fail("proof bytes");
+  if (g_max_running.load() != 1) return fail("more than one oversized job at once");
+  printf("oversized ok\n");
+  return 0;
+}
+
+// Destroying the server finishes every job already submitted.
+int scenario_drain() {
+  reset();
+  whirgpu_server* s = start(3, 0);
+  if (!s) return fail("server_create");
+  std::vector<Req> reqs(30);
+  for (size_t i = 0; i < reqs.size(); i++) {
+    make(reqs[i], 8 + (uint32_t)(i % 5), 2, (uint32_t)(3 * i));
+    whirgpu_server_submit(s, &reqs[i].job);
+  }
+  whirgpu_server_destroy(s);
+  for (auto& r : reqs)
+    if (!check(r)) return fail("job not finished by destroy");
+  printf("drain ok\n");
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  const std::string sc = argc > 1 ? argv[1] : "all";
+  int rc = 0;
+  // A timed wait (condition_variable::wait_for) goes through
+  // pthread_cond_clockwait, which this toolchain's ThreadSanitizer does not
+  // intercept (it then misses the wait's unlock and reports a double lock),
+  // so "mix-timed" runs in a plain build and "mix" waits without a timeout.
+  if (sc == "mix" || sc == "all") rc |= scenario_mix(false);
+  if (sc == "mix-timed") rc |= scenario_mix(true);
+  if (sc == "budget" || sc == "all") rc |= scenario_budget(1) | scenario_budget(3);
+  if (sc == "trim" || sc == "all") rc |= scenario_trim();
+  if (sc == "oversized" || sc == "all") rc |= scenario_oversized();
+  if (sc == "drain" || sc == "all") rc |= scenario_drain();
+  return rc;
+}

@@ -192,7 +192,7 @@ def test_sumcheck_rounds(W, n):
             assert np.array_equal(nxt, want2)
 
 
-@pytest.mark.parametrize("n,l0", [(1, 0), (3, 0), (6, 2), (12, 3), (12, 4), (16, 3), (20, 3)])
+@pytest.mark.parametrize("n,l0", [(1, 0), (3, 0), (6, 2), (12, 3), (12, 4), (16, 3), (20, 3), (22, 3)])
 @pytest.mark.parametrize("mode", ["scripted", "challenger"])
 def test_sumcheck_eq_round_polynomials(W, n, l0, mode):
     # the kernel-level sumcheck (SVO first rounds, then device folds) against

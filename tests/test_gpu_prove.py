@@ -53,6 +53,59 @@ def test_proof_bytes_match_oracle(n, kw, npts):
     assert rc == 0, err
 
 
+# Schedules whose query counts exceed the device transcript's 256 (RT_MAX_QUERIES):
+# UniqueDecoding at lambda 128 gives 270 queries in round 0 (host-driven WHIR
+# round) or 270 final queries (host-driven final round).
+FALLBACK_CASES = [
+    ("round", 12, dict(security_level=128, pow_bits=16, ff=4, rs=3, soundness=0)),
+    ("final", 10, dict(security_level=128, pow_bits=16, ff=4, rs=1, soundness=0)),
+    ("final", 6, dict(security_level=128, pow_bits=16, ff=2, rs=1, soundness=0)),
+]
+
+
+@pytest.mark.parametrize("path,n,kw", FALLBACK_CASES)
+@pytest.mark.parametrize("npts", [1, 3])
+def test_host_driven_rounds_match_oracle(path, n, kw, npts):
+    W = product()
+    cfg = product_cfg(n, **kw)
+    sch = cfg.schedule()
+    n_rounds, final_queries = sch[2], sch[3]
+    if path == "round":
+        assert n_rounds >= 1 and sch[7 + 2] > 256, sch   # round 0's queries
+    else:
+        assert final_queries > 256, sch
+    poly, pts = inputs(n, npts, seed=40 + n)
+    got, evals = W.prove_once(cfg, poly, pts)
+    want, want_evals, _ = oracle_prove(oracle_params(n, **kw), poly, pts)
+    assert all(np.array_equal(a, b) for a, b in zip(evals, want_evals))
+    assert got == want
+    rc, err = oracle_verify(oracle_params(n, **kw), got, pts, want_evals)
+    assert rc == 0, err
+    # a second proof on the same context reuses its buffers and gives the same bytes
+    again, _ = W.prove_once(cfg, poly, pts)
+    assert again == want
+
+
+# 2^25 coefficients: the command-line default of the reference (src/bin/main.rs:
+# -d 25 -k 5 -l 90, rate 1, rs-reduction 3) and the benchmark's folding 4; both
+# need pass-1 lines of 2^11 in the four-step encoding
+@pytest.mark.parametrize("kw", [dict(security_level=90, pow_bits=16, ff=5, rs=3),
+                                dict(security_level=100, pow_bits=16, ff=4, rs=3)])
+def test_n25_matches_oracle(kw):
+    W = product()
+    n = 25
+    poly = W.sample_field(0, 1 << n)
+    pts = [W.sample_field(1, 4 * n).reshape(n, 4)]
+    cfg = product_cfg(n, **kw)
+    d = W.DeviceBuffer.from_array(poly)
+    got, evals = W.prove_once(cfg, None, pts, device_ptr=d.ptr)
+    d.free()
+    want, want_evals, _ = oracle_prove(oracle_params(n, **kw), poly, pts)
+    assert np.array_equal(evals[0], want_evals[0])
+    assert got == want
+    assert oracle_verify(oracle_params(n, **kw), got, pts, want_evals)[0] == 0
+
+
 def test_config1_n18_matches_oracle():
     # n = 18, folding 4, rate 1/2, rs-reduction 3, lambda 100, one seeded constraint
     W = product()

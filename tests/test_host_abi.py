@@ -78,6 +78,21 @@ def test_bad_params_are_reported():
         W.WhirConfig(product_params(30, rate=1), device=-1)
 
 
+@pytest.mark.parametrize("n,ff,ok", [(25, 5, True), (25, 4, True), (27, 5, True), (28, 6, True), (22, 8, True),
+                                     (22, 9, False), (20, 10, False)])
+def test_encoding_shapes_are_checked_at_creation(n, ff, ok):
+    # every Reed-Solomon encoding of the schedule must be a shape the LDE
+    # kernels cover (rows of at most 2^8 base elements, columns up to 2^24);
+    # others are refused when the configuration is created, not at commit
+    W = product()
+    prm = product_params(n, ff=ff, security_level=60)
+    if ok:
+        W.WhirConfig(prm, device=-1).close()
+    else:
+        with pytest.raises(W.WhirError, match="Reed-Solomon encoding"):
+            W.WhirConfig(prm, device=-1)
+
+
 def test_host_only_context_refuses_device_work():
     W = product()
     cfg = W.WhirConfig(product_params(10, security_level=32, pow_bits=5, ff=2, rs=1), device=-1)

@@ -47,7 +47,8 @@ class Job(ctypes.Structure):
     _fields_ = [("num_variables", ctypes.c_uint32), ("evals", ctypes.c_void_p), ("evals_on_device", ctypes.c_int),
                 ("points", ctypes.c_void_p), ("npoints", ctypes.c_uint32), ("evals_out", ctypes.c_void_p),
                 ("proof_out", ctypes.c_void_p), ("proof_cap", ctypes.c_size_t), ("proof_len", ctypes.c_size_t),
-                ("status", ctypes.c_int), ("queue_ms", ctypes.c_double), ("latency_ms", ctypes.c_double)]
+                ("status", ctypes.c_int), ("queue_ms", ctypes.c_double), ("latency_ms", ctypes.c_double),
+                ("error", ctypes.c_char * 160)]
 
 
 PENDING = -1

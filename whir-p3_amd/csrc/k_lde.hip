@@ -275,7 +275,9 @@ static size_t lds_bytes(int lg, int lines, size_t esize) {
   return tw + (size_t)lines * ls * esize;
 }
 
-// Kernel instances by line length (1 <= LG <= 10).
+// Kernel instances by line length: pass 1 for 1 <= LG <= 14 (the four-step
+// split of columns up to 2^24, KoalaBear's largest two-adic subgroup; lines of
+// 2^14 fit in LDS for base-field columns only), pass 2 for 0 <= LG <= 10.
 template <typename T>
 static auto pass1_for(int lg) -> void (*)(const T*, T*, int, int, int, int, Twiddles) {
   switch (lg) {
@@ -289,6 +291,10 @@ static auto pass1_for(int lg) -> void (*)(const T*, T*, int, int, int, int, Twid
     case 8: return k_lde_pass1<T, 8>;
     case 9: return k_lde_pass1<T, 9>;
     case 10: return k_lde_pass1<T, 10>;
+    case 11: return k_lde_pass1<T, 11>;
+    case 12: return k_lde_pass1<T, 12>;
+    case 13: return k_lde_pass1<T, 13>;
+    case 14: return k_lde_pass1<T, 14>;
     default: return nullptr;
   }
 }
@@ -311,40 +317,70 @@ static auto pass2_for(int lg) -> void (*)(const T*, T*, int, int, int, int, Twid
   }
 }
 
+// Launch plan of one LDE: a single pass (l <= LP2) or the four-step split.
+struct LdePlan {
+  bool two_pass = false;
+  int l1 = 0, l2 = 0, tb_log = 0;
+  size_t sm1 = 0, sm2 = 0;
+};
+
 // Pass 1 takes about 64 KiB of lines per block (16K base or 4K extension
-// elements), at most 256 lines and at most L2.
-template <typename T, int LP2>
-static int lde_impl(const T* src, T* dst, T* tmp, int m, int k, int R, const Twiddles& tw, hipStream_t st) {
+// elements), at most 256 lines and at most L2; lines of 2^12 and longer take
+// fewer (down to one line per block) and up to 160 KiB of LDS.  Returns 0, or
+// a negative code for a shape no kernel instance covers.
+template <typename T>
+static int lde_plan(int m, int k, int R, int LP2, LdePlan* pl) {
   const int l = m - k;
-  if (l < 0) return -1;
+  if (l < 0 || k < 0 || R < 0 || k > 10) return -1;
+  if (l + R > 24) return -7;   // the codeword domain exceeds KoalaBear's two-adic subgroups
+  if (LP2 < (sizeof(T) == 4 ? 6 : 2)) return -6;   // rows too wide for a single-pass line in LDS
   const int W = 1 << k;
   if (l <= LP2) {
-    size_t sm = lds_bytes(l, W, sizeof(T));
-    if (sm > 160 * 1024) return -2;
+    pl->two_pass = false;
+    pl->sm2 = lds_bytes(l, W, sizeof(T));
+    if (pl->sm2 > 160 * 1024) return -2;
+    if (!pass2_for<T>(l)) return -6;
+    return 0;
+  }
+  pl->two_pass = true;
+  pl->l2 = LP2;
+  pl->l1 = l - LP2;
+  const int elem_log = sizeof(T) == 4 ? 14 : 12;
+  int tb_log = elem_log - pl->l1;
+  if (tb_log > 8) tb_log = 8;   // TB <= 256 < Pass1Threads: the row step NT / TB is at least 2
+  if (tb_log > pl->l2) tb_log = pl->l2;
+  if (tb_log < 0) tb_log = 0;
+  while (tb_log > 0 && lds_bytes(pl->l1, 1 << tb_log, sizeof(T)) > 80 * 1024) tb_log--;
+  pl->tb_log = tb_log;
+  pl->sm1 = lds_bytes(pl->l1, 1 << tb_log, sizeof(T));
+  if (pl->sm1 > 160 * 1024) return -3;
+  pl->sm2 = lds_bytes(pl->l2, W, sizeof(T));
+  if (pl->sm2 > 160 * 1024) return -2;
+  if (!pass1_for<T>(pl->l1) || !pass2_for<T>(pl->l2)) return -6;
+  return 0;
+}
+
+template <typename T>
+static int lde_impl(const T* src, T* dst, T* tmp, int m, int k, int R, int LP2, const Twiddles& tw, hipStream_t st) {
+  LdePlan pl;
+  const int rc = lde_plan<T>(m, k, R, LP2, &pl);
+  if (rc) return rc;
+  const int l = m - k;
+  const int W = 1 << k;
+  if (!pl.two_pass) {
     auto kern = pass2_for<T>(l);
-    if (!kern) return -6;
-    allow_lds(kern, sm);
-    hipLaunchKernelGGL(kern, dim3(1, 1u << R), dim3(Pass2Threads<T>::value), sm, st, src, dst, l, R, k, 1, tw);
+    allow_lds(kern, pl.sm2);
+    hipLaunchKernelGGL(kern, dim3(1, 1u << R), dim3(Pass2Threads<T>::value), pl.sm2, st, src, dst, l, R, k, 1, tw);
   } else {
-    const int l2 = LP2, l1 = l - LP2;
-    const int elem_log = sizeof(T) == 4 ? 14 : 12;
-    int tb_log = elem_log - l1;
-    if (tb_log > 8) tb_log = 8;   // TB <= 256 < Pass1Threads: the row step NT / TB is at least 2
-    if (tb_log > l2) tb_log = l2;
-    if (tb_log < 0) tb_log = 0;
-    size_t sm1 = lds_bytes(l1, 1 << tb_log, sizeof(T));
-    if (sm1 > 80 * 1024) return -3;
     if (tmp == nullptr) return -4;
-    auto k1 = pass1_for<T>(l1);
-    auto k2 = pass2_for<T>(l2);
-    if (!k1 || !k2) return -6;
-    allow_lds(k1, sm1);
-    hipLaunchKernelGGL(k1, dim3(1u << (l2 - tb_log), W, 1u << R), dim3(Pass1Threads<T>::value), sm1, st, src, tmp, l, R,
-                       k, tb_log, tw);
-    size_t sm2 = lds_bytes(l2, W, sizeof(T));
-    if (sm2 > 160 * 1024) return -2;
-    allow_lds(k2, sm2);
-    hipLaunchKernelGGL(k2, dim3(1u << l1, 1u << R), dim3(Pass2Threads<T>::value), sm2, st, tmp, dst, l, R, k, 0, tw);
+    auto k1 = pass1_for<T>(pl.l1);
+    auto k2 = pass2_for<T>(pl.l2);
+    allow_lds(k1, pl.sm1);
+    hipLaunchKernelGGL(k1, dim3(1u << (pl.l2 - pl.tb_log), W, 1u << R), dim3(Pass1Threads<T>::value), pl.sm1, st, src,
+                       tmp, l, R, k, pl.tb_log, tw);
+    allow_lds(k2, pl.sm2);
+    hipLaunchKernelGGL(k2, dim3(1u << pl.l1, 1u << R), dim3(Pass2Threads<T>::value), pl.sm2, st, tmp, dst, l, R, k, 0,
+                       tw);
   }
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
@@ -359,32 +395,18 @@ static int lp2_for(int k, bool ext) {
 
 bool lde_two_pass(int m, int k, bool ext) { return m - k > lp2_for(k, ext); }
 
+int lde_check(int m, int k, int R, bool ext) {
+  LdePlan pl;
+  const int lp = lp2_for(k, ext);
+  return ext ? lde_plan<ef>(m, k, R, lp, &pl) : lde_plan<u32>(m, k, R, lp, &pl);
+}
+
 int lde_base(const u32* src, u32* dst, u32* tmp, int m, int k, int R, const Twiddles& tw, hipStream_t st) {
-  int lp = lp2_for(k, false);
-  switch (lp) {
-    case 10: return lde_impl<u32, 10>(src, dst, tmp, m, k, R, tw, st);
-    case 9: return lde_impl<u32, 9>(src, dst, tmp, m, k, R, tw, st);
-    case 8: return lde_impl<u32, 8>(src, dst, tmp, m, k, R, tw, st);
-    case 7: return lde_impl<u32, 7>(src, dst, tmp, m, k, R, tw, st);
-    case 6: return lde_impl<u32, 6>(src, dst, tmp, m, k, R, tw, st);
-    default: return -6;
-  }
+  return lde_impl<u32>(src, dst, tmp, m, k, R, lp2_for(k, false), tw, st);
 }
 
 int lde_ext(const ef* src, ef* dst, ef* tmp, int m, int k, int R, const Twiddles& tw, hipStream_t st) {
-  int lp = lp2_for(k, true);
-  switch (lp) {
-    case 10: return lde_impl<ef, 10>(src, dst, tmp, m, k, R, tw, st);
-    case 9: return lde_impl<ef, 9>(src, dst, tmp, m, k, R, tw, st);
-    case 8: return lde_impl<ef, 8>(src, dst, tmp, m, k, R, tw, st);
-    case 7: return lde_impl<ef, 7>(src, dst, tmp, m, k, R, tw, st);
-    case 6: return lde_impl<ef, 6>(src, dst, tmp, m, k, R, tw, st);
-    case 5: return lde_impl<ef, 5>(src, dst, tmp, m, k, R, tw, st);
-    case 4: return lde_impl<ef, 4>(src, dst, tmp, m, k, R, tw, st);
-    case 3: return lde_impl<ef, 3>(src, dst, tmp, m, k, R, tw, st);
-    case 2: return lde_impl<ef, 2>(src, dst, tmp, m, k, R, tw, st);
-    default: return -6;
-  }
+  return lde_impl<ef>(src, dst, tmp, m, k, R, lp2_for(k, true), tw, st);
 }
 
 }  // namespace kb

@@ -114,6 +114,8 @@ struct EqTables {
 // src holds 2^m elements; the output is 2^(m-k+R) rows of 2^k elements.
 // tmp (same size as the output) is needed when 2^(m-k) exceeds one LDS pass.
 bool lde_two_pass(int m, int k, bool ext);
+// 0 when lde_base / lde_ext support the shape, else the negative code they would return.
+int lde_check(int m, int k, int R, bool ext);
 int lde_base(const u32* src, u32* dst, u32* tmp, int m, int k, int R, const Twiddles& tw, hipStream_t st);
 int lde_ext(const ef* src, ef* dst, ef* tmp, int m, int k, int R, const Twiddles& tw, hipStream_t st);
 

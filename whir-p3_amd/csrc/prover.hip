@@ -16,6 +16,7 @@
 
 #include "../../include/whir_gpu.h"
 #include "host.h"
+#include "internal.h"
 #include "kernels.h"
 
 using namespace kb;
@@ -58,6 +59,7 @@ void bad(const std::string& msg, int code = WHIRGPU_BAD_PARAMS) {
 struct Pool {
   std::multimap<size_t, void*> free_;
   std::vector<void*> all_;
+  size_t total = 0;   // bytes of every allocation made (in use or free)
   void* get(size_t bytes) {
     bytes = (bytes + 255) & ~size_t(255);
     auto it = free_.find(bytes);
@@ -69,6 +71,7 @@ struct Pool {
     void* p = nullptr;
     HIPCHK(hipMalloc(&p, bytes));
     all_.push_back(p);
+    total += bytes;
     return p;
   }
   void put(void* p, size_t bytes) {
@@ -78,6 +81,7 @@ struct Pool {
     for (void* p : all_) (void)hipFree(p);
     all_.clear();
     free_.clear();
+    total = 0;
   }
 };
 
@@ -99,6 +103,7 @@ struct whirgpu_ctx {
   u32* d_tw = nullptr;   // 2 * 4096 twiddles
   Twiddles tw{};
   Pool pool;
+  size_t fixed_bytes = 0;   // device bytes of the buffers allocated at creation (not pooled)
   void* h_pin = nullptr;
   size_t h_pin_bytes = 0;
   ef* d_partial = nullptr;   // reduction scratch
@@ -328,6 +333,7 @@ void ensure_tabs(whirgpu_ctx* c, int n) {
   HIPCHK(hipMalloc(&t, 8 * need * sizeof(ef)));
   (void)hipFree(c->d_tabs);
   c->d_tabs = t;
+  c->fixed_bytes += 8 * (need - c->tab_cap) * sizeof(ef);
   c->tab_cap = need;
 }
 
@@ -1117,6 +1123,27 @@ int guarded(F&& f) {
   }
 }
 
+// Every Reed-Solomon encoding of the schedule (the initial commitment and each
+// round's, as whirgpu_commit and commit_round size them) must be a shape the
+// LDE kernels cover; otherwise the configuration is rejected up front.
+void check_lde_shapes(const whir::WhirConfig& cfg) {
+  const int n = cfg.num_variables;
+  int rc = lde_check(n, cfg.ff(0), cfg.params.starting_log_inv_rate, false);
+  if (rc)
+    bad("Reed-Solomon encoding of 2^" + std::to_string(n) + " coefficients in rows of 2^" + std::to_string(cfg.ff(0)) +
+        " is not supported (code " + std::to_string(rc) + ")");
+  for (int round = 0; round < cfg.n_rounds(); round++) {
+    const int m = n - cfg.params.folding_factor.total_number(round);
+    const int kn = cfg.ff(round + 1);
+    const int Rl = ilog2u(cfg.inv_rate(round));
+    rc = lde_check(m, kn, Rl, true);
+    if (rc)
+      bad("round " + std::to_string(round) + ": Reed-Solomon encoding of 2^" + std::to_string(m) +
+          " extension elements in rows of 2^" + std::to_string(kn) + " is not supported (code " +
+          std::to_string(rc) + ")");
+  }
+}
+
 void make_twiddles(whirgpu_ctx* c) {
   std::vector<u32> h(8192);
   u32 w = two_adic_generator(24);
@@ -1206,6 +1233,7 @@ int whirgpu_create(int device, const whirgpu_params* params, const whirgpu_posei
     if (params->num_variables < 1 || params->num_variables > 28) bad("num_variables must be in [1, 28]");
     std::string err;
     if (!whir::WhirConfig::create((int)params->num_variables, p, &c->cfg, &err)) bad(err);
+    check_lde_shapes(c->cfg);
     load_perms(perms, &c->c16, &c->c24);
     if (device < 0) return;   // host-only context: schedule and transcript
     HIPCHK(hipSetDevice(device));
@@ -1213,6 +1241,9 @@ int whirgpu_create(int device, const whirgpu_params* params, const whirgpu_posei
     HIPCHK(hipEventCreateWithFlags(&c->ev_round, hipEventDisableTiming));
     make_twiddles(c);
     int n = (int)params->num_variables;
+    c->fixed_bytes = 8192 * sizeof(u32) + 8 * ((1ull << std::max(0, n - 8)) + 512) * sizeof(ef) +
+                     (1 << 15) * sizeof(ef) + 2 * 64 * sizeof(u32) + whirgpu_ctx::GRIND_SLOTS * sizeof(u32) +
+                     sizeof(P2C16) + sizeof(P2C24) + 4 * sizeof(DevChallenger) + sizeof(GrindBase);
     c->tab_cap = (1ull << std::max(0, n - 8)) + 512;
     HIPCHK(hipMalloc(&c->d_tabs, 8 * c->tab_cap * sizeof(ef)));
     c->partial_cap = 1 << 15;
@@ -1250,6 +1281,7 @@ int whirgpu_create(int device, const whirgpu_params* params, const whirgpu_posei
                    idx = sizeof(u64) * RT_MAX_QUERIES, selz = sizeof(u32) * RT_MAX_QUERIES,
                    gp = sizeof(ef) * (RT_MAX_OOD + RT_MAX_QUERIES), gr = sizeof(u32) * 64, nq = 256;
       HIPCHK(hipMalloc(&c->d_rbuf, pts + ev + idx + selz + gp + gr + nq));
+      c->fixed_bytes += pts + ev + idx + selz + gp + gr + nq;
       char* b = static_cast<char*>(c->d_rbuf);
       c->rb.pts = reinterpret_cast<ef*>(b);
       c->rb.ev = reinterpret_cast<ef*>(b + pts);
@@ -1669,9 +1701,12 @@ int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs,
         if ((ns + 3) / 4 + 1 + ns > c->tab_cap) bad("too many STIR queries");
         c->h2d(d_z, sel_z.data(), ns);
         c->h2d(d_g, g.data(), ns);
+        // sized for the round's query count (ns <= num_queries after
+        // deduplication), so every proof of this shape reuses one pool pair
         int lo = m / 2, hi = m - lo;
-        Buf<ef> A = c->alloc<ef>(ns << hi);
-        Buf<u32> B = c->alloc<u32>(ns << lo);
+        const size_t nmax = std::max<size_t>(ns, (size_t)rp.num_queries);
+        Buf<ef> A = c->alloc<ef>(nmax << hi);
+        Buf<u32> B = c->alloc<u32>(nmax << lo);
         {
           Span sp(c, ST_COMBINE, 32.0 * (double)(1ull << m));
           KCHK(select_combine(s.w.p, m, d_z, d_g, (int)ns, nullptr, A.p, B.p, c->st));
@@ -1755,6 +1790,16 @@ int whirgpu_profile_records(whirgpu_ctx* c, double* out, size_t cap, size_t* len
       out[3 * i + 1] = ms;
       out[3 * i + 2] = c->recs[i].work;
     }
+  });
+}
+
+size_t whirgpu_internal_ctx_bytes(const whirgpu_ctx* c) { return c ? c->fixed_bytes + c->pool.total : 0; }
+
+int whirgpu_internal_mem_info(int device, size_t* free_b, size_t* total_b) {
+  return guarded([&] {
+    if (!free_b || !total_b) bad("null argument");
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipMemGetInfo(free_b, total_b));
   });
 }
 
