@@ -46,6 +46,14 @@ TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profile
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the CPU baseline's OpenMP threads stay on their cores (read when the oracle loads)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
+
+def progress(msg):
+    """A progress line on stderr (the JSON result alone goes to stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
 def hbm_roof(stage, name):
@@ -216,6 +224,28 @@ def serving_bench(W, workers=8, rate=200.0, count=200, seed=7):
             "same_alone_and_in_mix": same}
 
 
+def cpu_baseline(n, poly, pts, gpu_proof, runs=3):
+    """The same commit + prove on the CPU oracle (a naive port: canonical 64-bit
+    modular arithmetic, OpenMP over Merkle leaves, DFT columns and grinding
+    candidates), threads pinned to cores, one warm-up run, then the median of
+    `runs` runs, with per-stage medians under the reference's span names."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import ORACLE_SPANS, oracle_params, oracle_prove
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cproof, _, _ = oracle_prove(oracle_params(n), poly, pts)   # warm-up
+    totals, stages = [], []
+    for _ in range(runs):
+        _, _, ms = oracle_prove(oracle_params(n), poly, pts)
+        totals.append(ms[0] + ms[1])
+        stages.append(ms[2:])
+    med = float(np.median(totals))
+    per = {name: round(float(np.median([s_[i] for s_ in stages])), 1) for i, name in enumerate(ORACLE_SPANS)}
+    return {"value": round(med, 1), "unit": "ms", "cores": threads, "kind": "port",
+            "sample": f"full n={n} commit+prove of the same inputs on the CPU oracle (naive port: canonical 64-bit "
+                      f"% arithmetic), OMP threads pinned to cores, 1 warm-up + median of {runs} runs",
+            "runs_ms": [round(t, 1) for t in totals], "stage_ms": per, "bit_exact": cproof == gpu_proof}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -223,6 +253,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-runs", type=int, default=3, help="timed oracle runs after one warm-up (median reported)")
     ap.add_argument("--skip-kernel-bench", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--workers", type=int, default=12, help="batching server workers")
@@ -273,6 +304,7 @@ def main():
         st.close()
         return out
 
+    progress(f"setup done (n={n}); warm-up")
     for _ in range(args.warmup):
         proof = step()
     # Timed steps: no profiling events, so `value` is the plain proof time.
@@ -331,22 +363,21 @@ def main():
 
     dominant = max(stages, key=lambda k: stages[k]["ms"])
 
+    progress(f"n={n}: {ms:.3f} ms per proof; profiled steps done")
     kernels = None if args.skip_kernel_bench else sumcheck_eq_bench(W, args)
+    if kernels:
+        progress("2^26 sumcheck/eq workload done")
     batched = None if args.skip_batch else batched_bench(W, workers=args.workers)
+    if batched:
+        progress(f"batched: {batched['speedup']}x")
     serving = None if args.skip_batch else serving_bench(W, workers=args.serving_workers)
+    if serving:
+        progress(f"serving: p99 {serving['p99_ms']} ms")
 
     cpu = None
     if not args.no_cpu_baseline:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        from helpers import oracle_params, oracle_prove
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        t = time.perf_counter()
-        cproof, _, cms = oracle_prove(oracle_params(n), poly, pts)
-        wall = (time.perf_counter() - t) * 1e3
-        cpu = {"value": round(cms[0] + cms[1], 1), "unit": "ms", "cores": threads, "kind": "port",
-               "sample": f"one full n={n} commit+prove of the same inputs on the CPU oracle ({wall:.0f} ms wall "
-                         f"incl. statement evaluation)",
-               "bit_exact": cproof == proof}
+        progress("CPU baseline")
+        cpu = cpu_baseline(n, poly, pts, proof, runs=args.cpu_runs)
 
     out = {
         "metric": "prove ms @2^24 vars" if n == 24 else f"prove ms @2^{n} vars",

@@ -230,7 +230,11 @@ int orc_prove(const int* params, uint64_t perm_seed, const uint32_t* poly, const
   Proof pr = whir_prove(c, ps.p16, ps.leaf(p.leaf_width), f, pts, ch, &st, &tm);
   auto bytes = serialize_proof(pr);
   *len = bytes.size();
-  if (ms) { ms[0] = tm.commit_ms; ms[1] = tm.prove_ms; }
+  if (ms) {   // 8 values: commit, prove, then the stages of Timing in declaration order
+    const double v[8] = {tm.commit_ms, tm.prove_ms, tm.dft_ms, tm.commit_matrix_ms, tm.ood_ms,
+                         tm.initial_sumcheck_ms, tm.rounds_ms, tm.final_ms};
+    memcpy(ms, v, sizeof v);
+  }
   for (int j = 0; j < npts; j++) ef_out(st.evals[j], evals_out + 4 * j);
   if (bytes.size() > cap) return -2;
   memcpy(out, bytes.data(), bytes.size());

@@ -218,8 +218,14 @@ struct Statement {                      // InitialStatement (normalized view)
   std::vector<std::vector<EF>> points;
   std::vector<EF> evals;
 };
+// Wall time of commit and prove, and of the stages under the reference's
+// tracing span names: "dft" and "commit_matrix" of the commitment
+// (src/whir/committer/writer.rs:85-89), its "ood evaluation" (:102), the
+// initial sumcheck ("svo rounds", src/sumcheck/sumcheck_prover.rs:201), the
+// WHIR rounds (src/whir/prover/mod.rs:130) and the final round (:328).
 struct Timing {
   double commit_ms = 0, prove_ms = 0;
+  double dft_ms = 0, commit_matrix_ms = 0, ood_ms = 0, initial_sumcheck_ms = 0, rounds_ms = 0, final_ms = 0;
 };
 // Full commit + prove.  `user_points` are evaluated first (InitialStatement::evaluate),
 // then commit appends the OOD constraints.  Returns the proof; `stmt_out` receives the

@@ -767,7 +767,10 @@ Proof whir_prove(const Config& cfg, const Poseidon2& p16, const Poseidon2& leaf,
   double t0 = now_ms();
   int k0 = cfg.ff_at(0);
   u64 H0 = 1ull << (n + cfg.p.log_inv_rate - k0);
-  MerkleTree tree0 = merkle_commit(hs, lde_base(poly, n, k0, H0), H0, 1ull << k0);
+  std::vector<u32> codeword = lde_base(poly, n, k0, H0);
+  double td = now_ms();
+  MerkleTree tree0 = merkle_commit(hs, std::move(codeword), H0, 1ull << k0);
+  double tm_ = now_ms();
   pr.initial_commitment = tree0.root();
   observe_digest(ch, pr.initial_commitment);
   for (int i = 0; i < cfg.commitment_ood_samples; i++) {
@@ -785,6 +788,7 @@ Proof whir_prove(const Config& cfg, const Poseidon2& p16, const Poseidon2& leaf,
   std::vector<EF> fold_rand;
   if (cfg.p.svo && n > k0) fold_rand = initial_svo(cfg, poly, st, ch, pr.initial_sumcheck, s);
   else fold_rand = initial_classic(cfg, poly, st, ch, pr.initial_sumcheck, s);
+  double ts = now_ms(), tr = ts;
 
   const MerkleTree* prev = &tree0;
   bool prev_ext = false;
@@ -794,6 +798,7 @@ Proof whir_prove(const Config& cfg, const Poseidon2& p16, const Poseidon2& leaf,
     int m = n - cfg.total_number(round);
     assert(s.f.size() == (1ull << m));
     if (round == cfg.n_rounds()) {
+      tr = now_ms();
       for (auto& e : s.f) ch.observe_ef(e);
       pr.final_poly = s.f;
       if (cfg.final_pow_bits > 0) pr.final_pow_witness = ch.grind(cfg.final_pow_bits);
@@ -860,7 +865,16 @@ Proof whir_prove(const Config& cfg, const Poseidon2& p16, const Poseidon2& leaf,
     prev_ext = true;
   }
   double t2 = now_ms();
-  if (tm) { tm->commit_ms = t1 - t0; tm->prove_ms = t2 - t1; }
+  if (tm) {
+    tm->commit_ms = t1 - t0;
+    tm->prove_ms = t2 - t1;
+    tm->dft_ms = td - t0;
+    tm->commit_matrix_ms = tm_ - td;
+    tm->ood_ms = t1 - tm_;
+    tm->initial_sumcheck_ms = ts - t1;
+    tm->rounds_ms = tr - ts;
+    tm->final_ms = t2 - tr;
+  }
   return pr;
 }
 

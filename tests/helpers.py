@@ -86,7 +86,8 @@ def oracle_params(n, *, security_level=100, pow_bits=16, ff=4, ff0=None, rate=1,
 
 
 def oracle_prove(prm, poly_monty, points_monty, perm_seed=1):
-    """Commit + prove on the oracle; returns (proof bytes, evals (npts x 4), [commit_ms, prove_ms])."""
+    """Commit + prove on the oracle; returns (proof bytes, evals (npts x 4), [commit_ms, prove_ms, *stage ms]),
+    the stages named by ORACLE_SPANS."""
     L = oracle()
     npts = len(points_monty)
     pts = np.ascontiguousarray(np.concatenate([np.asarray(p, dtype=np.uint32).reshape(-1) for p in points_monty])
@@ -96,10 +97,15 @@ def oracle_prove(prm, poly_monty, points_monty, perm_seed=1):
     buf = (ctypes.c_uint8 * cap)()
     ln = ctypes.c_size_t()
     ev = np.zeros(4 * max(npts, 1), dtype=np.uint32)
-    ms = (ctypes.c_double * 2)()
+    ms = (ctypes.c_double * 8)()
     rc = L.orc_prove(prm, perm_seed, u32p(poly), u32p(pts), npts, buf, cap, ctypes.byref(ln), u32p(ev), ms)
     assert rc == 0, f"oracle prove failed: {rc}"
-    return bytes(buf[:ln.value]), ev[:4 * npts].reshape(npts, 4), [ms[0], ms[1]]
+    return bytes(buf[:ln.value]), ev[:4 * npts].reshape(npts, 4), list(ms)
+
+
+# Names of the oracle's stage times (the reference's tracing spans), in the
+# order oracle_prove returns them after commit_ms and prove_ms.
+ORACLE_SPANS = ("dft", "commit_matrix", "ood evaluation", "initial sumcheck (svo rounds)", "rounds", "final_round")
 
 
 def oracle_verify(prm, proof, points_monty, evals_monty, perm_seed=1):
