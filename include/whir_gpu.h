@@ -203,6 +203,20 @@ int whirgpu_eq_expand(whirgpu_ctx* ctx, const uint32_t* point, uint32_t n, const
 int whirgpu_mle_eval_base(whirgpu_ctx* ctx, const uint32_t* d_f, uint32_t n, const uint32_t* point, uint32_t* out);
 int whirgpu_mle_eval_ext(whirgpu_ctx* ctx, const uint32_t* d_f, uint32_t n, const uint32_t* point, uint32_t* out);
 
+/* No counterpart in the reference: whirgpu_mle_eval_base for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It takes no context and allocates
+ * nothing: its eq tables and block partial sums live in a fixed device array
+ * of the library (1,085,440 bytes on each GPU it runs on, held until the process
+ * ends), which limits it to 1 <= n <= 24 (larger n: WHIRGPU_BAD_PARAMS).
+ * d_f: 2^n base elements on the current device; point: n extension elements
+ * (host memory); d_out: 4 uint32_t of device memory that receive the
+ * evaluation.  The kernels are enqueued on `stream` (a hipStream_t; NULL for
+ * the null stream) and not waited for.  Every call on one GPU uses the same
+ * array, so calls that overlap on one GPU (other threads or streams) race. */
+int whirgpu_mle_eval_base_noalloc(const uint32_t* d_f, uint32_t n, const uint32_t* point, uint32_t* d_out,
+                                  void* stream);
+
 /* sumcheck_coefficients (src/poly/evals.rs:493-526): out = (h(0), h(2)) of
  * arrays of 2^n extension elements. */
 int whirgpu_sumcheck_coeffs(whirgpu_ctx* ctx, const uint32_t* d_evals, const uint32_t* d_weights, uint32_t n,

@@ -165,3 +165,15 @@ def test_challenger_matches_oracle():
             W.check(W.lib().whirgpu_challenger_sample(cfg.handle, ctypes.byref(ch), u32p(v), 1))
             got.append(int(v[0]))
     assert got == [int(x) for x in out[:cnt]]
+
+
+def test_noalloc_evaluation_rejects_sizes_beyond_its_scratch():
+    # checked on the host before anything is enqueued (no device needed)
+    W = product()
+    z = np.zeros(4 * 25, np.uint32)
+    dummy = ctypes.c_void_p(256)
+    rc = W.lib().whirgpu_mle_eval_base_noalloc(dummy, 25, u32p(z), dummy, None)
+    assert rc == 1
+    assert b"[1, 24]" in W.lib().whirgpu_last_error()
+    assert W.lib().whirgpu_mle_eval_base_noalloc(dummy, 0, u32p(z), dummy, None) == 1
+    assert W.lib().whirgpu_mle_eval_base_noalloc(None, 8, u32p(z), dummy, None) == 1

@@ -108,6 +108,7 @@ _SIGS = {
     "whirgpu_eq_expand": (ctypes.c_int, [_VP, U32P, ctypes.c_uint32, U32P, _VP]),
     "whirgpu_mle_eval_base": (ctypes.c_int, [_VP, _VP, ctypes.c_uint32, U32P, U32P]),
     "whirgpu_mle_eval_ext": (ctypes.c_int, [_VP, _VP, ctypes.c_uint32, U32P, U32P]),
+    "whirgpu_mle_eval_base_noalloc": (ctypes.c_int, [_VP, ctypes.c_uint32, U32P, _VP, _VP]),
     "whirgpu_sumcheck_coeffs": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_uint32, U32P]),
     "whirgpu_sumcheck_fold": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_uint32, U32P, U32P]),
 }

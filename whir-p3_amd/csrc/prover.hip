@@ -93,6 +93,36 @@ struct Buf {
 
 }  // namespace
 
+// Scratch of whirgpu_mle_eval_base_noalloc, the entry point that allocates
+// nothing: the split eq tables of a point of up to 24 variables (2^16 + 256
+// extension elements) and the block partial sums (2048): 1,085,440 bytes.  Each GPU
+// holds its own copy until the process ends; every call on one GPU uses it,
+// so calls that overlap on one GPU race.  Host code must use the device
+// address from scratch_on_current_device(), never the array's name.
+constexpr int NOALLOC_MAX_VARS = 24;
+constexpr size_t NOALLOC_TAB_HI = size_t(1) << (NOALLOC_MAX_VARS - 8);
+constexpr size_t NOALLOC_TAB_LO = 256;
+constexpr size_t NOALLOC_PARTIALS = 2048;
+static __device__ __align__(256) ef scratch[NOALLOC_TAB_HI + NOALLOC_TAB_LO + NOALLOC_PARTIALS];
+
+static std::mutex g_scratch_mu;
+static ef* g_scratch_ptr[64] = {};
+
+// The address of `scratch` on the current device, looked up once per device
+// (hipGetSymbolAddress, with that device current) and kept.
+static ef* scratch_on_current_device() {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) bad("device index out of range", WHIRGPU_DEVICE);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<ef*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
 struct whirgpu_ctx {
   int device = 0;
   hipStream_t st = nullptr;
@@ -1926,6 +1956,35 @@ int whirgpu_mle_eval_ext(whirgpu_ctx* c, const uint32_t* d_f, uint32_t n, const 
     use_device(c);
     ef e = ext_mle(c, reinterpret_cast<const ef*>(d_f), (int)n, read_point(point, n));
     memcpy(out, e.c, 16);
+  });
+}
+
+// Evaluation of a base-field multilinear polynomial without allocating (no
+// counterpart in the reference: for callers that cannot allocate).  The same
+// kernels as whirgpu_mle_eval_base, with the split eq tables and the block
+// partial sums in the library's fixed device array `scratch` (its address on
+// the current device comes from scratch_on_current_device) and the result
+// written to device memory on the caller's stream.
+int whirgpu_mle_eval_base_noalloc(const uint32_t* d_f, uint32_t n, const uint32_t* point, uint32_t* d_out,
+                                  void* stream) {
+  return guarded([&] {
+    if (!d_f || !point || !d_out) bad("null argument");
+    if (n < 1 || n > (uint32_t)NOALLOC_MAX_VARS)
+      bad("n must be in [1, " + std::to_string(NOALLOC_MAX_VARS) + "] for the evaluation without allocation");
+    ef* s = scratch_on_current_device();
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int lo = std::min(8, (int)n), hi = (int)n - lo;
+    ef* tab_hi = s;
+    ef* tab_lo = s + NOALLOC_TAB_HI;
+    ef* partial = s + NOALLOC_TAB_HI + NOALLOC_TAB_LO;
+    const int nb = dot_blocks(hi, 0);
+    if ((1ull << hi) > NOALLOC_TAB_HI || (size_t)nb > NOALLOC_PARTIALS) bad("internal: scratch too small");
+    const PointArg a = point_arg(read_point(point, n));
+    KCHK(eq_split(a, hi, lo, ef_one(), tab_hi, tab_lo, st));
+    EqTables t{};
+    t.hi[0] = tab_hi;
+    t.lo[0] = tab_lo;
+    KCHK(eq_dot_base(d_f, (int)n, 0, lo, t, 1, partial, nb, reinterpret_cast<ef*>(d_out), st));
   });
 }
 
