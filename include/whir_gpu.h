@@ -150,6 +150,21 @@ int whirgpu_prove(whirgpu_ctx* ctx, whirgpu_proof* proof, whirgpu_challenger* ch
                   whirgpu_prover_data* prover_data);
 int whirgpu_prover_data_destroy(whirgpu_prover_data* pd);
 
+/* Batched commit + prove (benches/whir.rs:157-183 for many polynomials at
+ * once; the reference proves one at a time): `count` proofs (1 to 16) of the
+ * context's configuration, each with its own statement (its constraints
+ * already evaluated, the same number for all), proof object and challenger,
+ * run as one group in lockstep, every device step one launch for all of them.
+ * Each proof and challenger ends exactly as whirgpu_commit + whirgpu_prove
+ * would leave it. */
+int whirgpu_prove_group(whirgpu_ctx* ctx, uint32_t count, whirgpu_statement* const* statements,
+                        whirgpu_proof* const* proofs, whirgpu_challenger* challengers);
+/* InitialStatement::evaluate (initial.rs:143-152) for `count` statements of
+ * one context at once (1 to 16), one point each: points holds count * n
+ * extension elements, evals_out receives count extension elements. */
+int whirgpu_statement_evaluate_group(uint32_t count, whirgpu_statement* const* statements, const uint32_t* points,
+                                     uint32_t* evals_out);
+
 /* Per-stage device time of the last commit/prove on this context, in ms:
  * [lde, merkle, grind, sumcheck, mle, other, total_wall]. */
 int whirgpu_last_timings(const whirgpu_ctx* ctx, double* out7);
@@ -208,7 +223,8 @@ int whirgpu_mle_eval_ext(whirgpu_ctx* ctx, const uint32_t* d_f, uint32_t n, cons
  * loops, harnesses that forbid hipMalloc).  It takes no context and allocates
  * nothing: its eq tables and block partial sums live in a fixed device array
  * of the library (1,085,440 bytes on each GPU it runs on, held until the process
- * ends), which limits it to 1 <= n <= 24 (larger n: WHIRGPU_BAD_PARAMS).
+ * ends), which limits it to 1 <= n <= 24 (larger n: WHIRGPU_BAD_PARAMS).  The
+ * eq-weighted sum kernels are those of whirgpu_mle_eval_base.
  * d_f: 2^n base elements on the current device; point: n extension elements
  * (host memory); d_out: 4 uint32_t of device memory that receive the
  * evaluation.  The kernels are enqueued on `stream` (a hipStream_t; NULL for

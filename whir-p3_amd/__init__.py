@@ -81,6 +81,9 @@ _SIGS = {
     "whirgpu_commit": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ChallengerState), _VP, ctypes.POINTER(_VP)]),
     "whirgpu_prove": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ChallengerState), _VP, _VP]),
     "whirgpu_prover_data_destroy": (ctypes.c_int, [_VP]),
+    "whirgpu_prove_group": (ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.POINTER(_VP), ctypes.POINTER(_VP),
+                                           ctypes.POINTER(ChallengerState)]),
+    "whirgpu_statement_evaluate_group": (ctypes.c_int, [ctypes.c_uint32, ctypes.POINTER(_VP), U32P, U32P]),
     "whirgpu_last_timings": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double)]),
     "whirgpu_profile_enable": (ctypes.c_int, [_VP, ctypes.c_int]),
     "whirgpu_profile_records": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double), ctypes.c_size_t,
@@ -381,6 +384,41 @@ def prove_once(cfg, poly, points=(), device_ptr=None):
     proof.close()
     st.close()
     return out, evals
+
+
+def prove_group(cfg, polys, points_per_proof, device_ptrs=None):
+    """Commit + prove of several polynomials of cfg's size in one group
+    (whirgpu_prove_group), each with its own constraints (the same number for
+    all); returns [(proof bytes, evaluations)] in input order."""
+    count = len(polys) if device_ptrs is None else len(device_ptrs)
+    sts = []
+    for i in range(count):
+        if device_ptrs is None:
+            sts.append(InitialStatement(cfg, polys[i]))
+        else:
+            sts.append(InitialStatement(cfg, None, device_ptr=device_ptrs[i]))
+    handles = (ctypes.c_void_p * count)(*[s.handle.value for s in sts])
+    npts = len(points_per_proof[0]) if count else 0
+    evals = [[] for _ in range(count)]
+    for k in range(npts):
+        pts = np.ascontiguousarray(np.concatenate([np.asarray(points_per_proof[i][k], dtype=np.uint32).reshape(-1)
+                                                   for i in range(count)]))
+        out = np.zeros(4 * count, np.uint32)
+        check(lib().whirgpu_statement_evaluate_group(count, handles, u32p(pts), u32p(out)))
+        for i in range(count):
+            evals[i].append(out[4 * i: 4 * i + 4].copy())
+    proofs = [WhirProof(cfg) for _ in range(count)]
+    phandles = (ctypes.c_void_p * count)(*[p.handle.value for p in proofs])
+    chs = (ChallengerState * count)()
+    for i in range(count):
+        chs[i] = cfg.challenger()
+    check(lib().whirgpu_prove_group(cfg.handle, count, handles, phandles, chs))
+    out = [(p.to_bytes(), ev) for p, ev in zip(proofs, evals)]
+    for p in proofs:
+        p.close()
+    for s in sts:
+        s.close()
+    return out
 
 
 class ProofRequest:

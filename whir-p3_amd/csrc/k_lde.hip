@@ -134,23 +134,27 @@ struct Pass1Threads {
   static_assert(value >= 256, "pass 1 blocks take up to 256 lines");
 };
 
-// Pass 1 of the four-step split.  Grid: (L2 / TB, W, 2^R); LG = l1, TB = 2^tb_log
-// lines per block (TB <= 256).  Thread t of a block always handles line
-// (column offset) t mod TB and rows t / TB + j * (NT / TB) for NT threads, so
-// its coset twist w_H^(i0 b) and four-step twiddle w_L^(b2 k1) advance by one
-// product per row instead of two table lookups.
+// Pass 1 of the four-step split.  Grid: (L2 / TB, W, 2^R * proofs); LG = l1,
+// TB = 2^tb_log lines per block (TB <= 256); blockIdx.z = proof * 2^R + coset.
+// Thread t of a block always handles line (column offset) t mod TB and rows
+// t / TB + j * (NT / TB) for NT threads, so its coset twist w_H^(i0 b) and
+// four-step twiddle w_L^(b2 k1) advance by one product per row instead of two
+// table lookups.
 template <typename T, int LG>
-__global__ void __launch_bounds__(Pass1Threads<T>::value) k_lde_pass1(const T* __restrict__ src, T* __restrict__ V, int l, int R, int k,
-                                                   int tb_log, Twiddles tw) {
+__global__ void __launch_bounds__(Pass1Threads<T>::value) k_lde_pass1(Group<LdeArgs<T>> g, int l, int R, int k,
+                                                                      int tb_log, Twiddles tw) {
   extern __shared__ __align__(16) unsigned char smem[];
   using L = LineShape<LG>;
   constexpr int U = Batch<T>::value;
+  const LdeArgs<T>& io = g.v[blockIdx.z >> R];
+  const T* __restrict__ src = io.src;
+  T* __restrict__ V = io.tmp;
   const int l2 = l - LG;
   const int W = 1 << k;
   const int TB = 1 << tb_log;
   const int b2_0 = blockIdx.x << tb_log;
   const int a = blockIdx.y;
-  const int i0 = blockIdx.z;
+  const int i0 = blockIdx.z & ((1 << R) - 1);
   const int H_log = l + R;
   u32* stw = reinterpret_cast<u32*>(smem);
   T* sh = line_storage<T, LG>(smem);
@@ -204,11 +208,14 @@ struct Pass2Threads {
   static constexpr int value = sizeof(T) == 4 ? 512 : 256;
 };
 
-// Pass 2 (or the only pass when direct != 0).  Grid: (L1, 2^R); LG = l2.
+// Pass 2 (or the only pass when direct != 0).  Grid: (L1, 2^R, proofs); LG = l2.
 template <typename T, int LG>
-__global__ void __launch_bounds__(Pass2Threads<T>::value) k_lde_pass2(const T* __restrict__ src, T* __restrict__ dst, int l, int R, int k,
-                                                   int direct, Twiddles tw) {
+__global__ void __launch_bounds__(Pass2Threads<T>::value) k_lde_pass2(Group<LdeArgs<T>> g, int l, int R, int k,
+                                                                      int direct, Twiddles tw) {
   extern __shared__ __align__(16) unsigned char smem[];
+  const LdeArgs<T>& io = g.v[blockIdx.z];
+  const T* __restrict__ src = direct ? io.src : io.tmp;
+  T* __restrict__ dst = io.dst;
   using L = LineShape<LG>;
   constexpr int U = Batch<T>::value;
   const int l2 = LG, L2 = L::N;
@@ -279,7 +286,7 @@ static size_t lds_bytes(int lg, int lines, size_t esize) {
 // split of columns up to 2^24, KoalaBear's largest two-adic subgroup; lines of
 // 2^14 fit in LDS for base-field columns only), pass 2 for 0 <= LG <= 10.
 template <typename T>
-static auto pass1_for(int lg) -> void (*)(const T*, T*, int, int, int, int, Twiddles) {
+static auto pass1_for(int lg) -> void (*)(Group<LdeArgs<T>>, int, int, int, int, Twiddles) {
   switch (lg) {
     case 1: return k_lde_pass1<T, 1>;
     case 2: return k_lde_pass1<T, 2>;
@@ -300,7 +307,7 @@ static auto pass1_for(int lg) -> void (*)(const T*, T*, int, int, int, int, Twid
 }
 
 template <typename T>
-static auto pass2_for(int lg) -> void (*)(const T*, T*, int, int, int, int, Twiddles) {
+static auto pass2_for(int lg) -> void (*)(Group<LdeArgs<T>>, int, int, int, int, Twiddles) {
   switch (lg) {
     case 0: return k_lde_pass2<T, 0>;
     case 1: return k_lde_pass2<T, 1>;
@@ -361,28 +368,34 @@ static int lde_plan(int m, int k, int R, int LP2, LdePlan* pl) {
 }
 
 template <typename T>
-static int lde_impl(const T* src, T* dst, T* tmp, int m, int k, int R, int LP2, const Twiddles& tw, hipStream_t st) {
+static int lde_impl(const LdeArgs<T>* io, int count, int m, int k, int R, int LP2, const Twiddles& tw,
+                    hipStream_t st) {
   LdePlan pl;
   const int rc = lde_plan<T>(m, k, R, LP2, &pl);
   if (rc) return rc;
+  if (count < 1 || count > GROUP_MAX) return -8;
   const int l = m - k;
   const int W = 1 << k;
+  const unsigned np = (unsigned)count;
   if (!pl.two_pass) {
     auto kern = pass2_for<T>(l);
     allow_lds(kern, pl.sm2);
-    hipLaunchKernelGGL(kern, dim3(1, 1u << R), dim3(Pass2Threads<T>::value), pl.sm2, st, src, dst, l, R, k, 1, tw);
-  } else {
-    if (tmp == nullptr) return -4;
-    auto k1 = pass1_for<T>(pl.l1);
-    auto k2 = pass2_for<T>(pl.l2);
-    allow_lds(k1, pl.sm1);
-    hipLaunchKernelGGL(k1, dim3(1u << (pl.l2 - pl.tb_log), W, 1u << R), dim3(Pass1Threads<T>::value), pl.sm1, st, src,
-                       tmp, l, R, k, pl.tb_log, tw);
-    allow_lds(k2, pl.sm2);
-    hipLaunchKernelGGL(k2, dim3(1u << pl.l1, 1u << R), dim3(Pass2Threads<T>::value), pl.sm2, st, tmp, dst, l, R, k, 0,
-                       tw);
+    return launch_group(io, count, [&](const Group<LdeArgs<T>>& g) {
+      hipLaunchKernelGGL(kern, dim3(1, 1u << R, np), dim3(Pass2Threads<T>::value), pl.sm2, st, g, l, R, k, 1, tw);
+    });
   }
-  return hipGetLastError() == hipSuccess ? 0 : -5;
+  for (int i = 0; i < count; i++)
+    if (io[i].tmp == nullptr) return -4;
+  auto k1 = pass1_for<T>(pl.l1);
+  auto k2 = pass2_for<T>(pl.l2);
+  allow_lds(k1, pl.sm1);
+  allow_lds(k2, pl.sm2);
+  return launch_group(io, count, [&](const Group<LdeArgs<T>>& g) {
+    hipLaunchKernelGGL(k1, dim3(1u << (pl.l2 - pl.tb_log), W, (1u << R) * np), dim3(Pass1Threads<T>::value), pl.sm1,
+                       st, g, l, R, k, pl.tb_log, tw);
+    hipLaunchKernelGGL(k2, dim3(1u << pl.l1, 1u << R, np), dim3(Pass2Threads<T>::value), pl.sm2, st, g, l, R, k, 0,
+                       tw);
+  });
 }
 
 // Largest single-pass length per element type (LDS holds W lines of it).
@@ -401,12 +414,12 @@ int lde_check(int m, int k, int R, bool ext) {
   return ext ? lde_plan<ef>(m, k, R, lp, &pl) : lde_plan<u32>(m, k, R, lp, &pl);
 }
 
-int lde_base(const u32* src, u32* dst, u32* tmp, int m, int k, int R, const Twiddles& tw, hipStream_t st) {
-  return lde_impl<u32>(src, dst, tmp, m, k, R, lp2_for(k, false), tw, st);
+int lde_base(const LdeArgs<u32>* a, int count, int m, int k, int R, const Twiddles& tw, hipStream_t st) {
+  return lde_impl<u32>(a, count, m, k, R, lp2_for(k, false), tw, st);
 }
 
-int lde_ext(const ef* src, ef* dst, ef* tmp, int m, int k, int R, const Twiddles& tw, hipStream_t st) {
-  return lde_impl<ef>(src, dst, tmp, m, k, R, lp2_for(k, true), tw, st);
+int lde_ext(const LdeArgs<ef>* a, int count, int m, int k, int R, const Twiddles& tw, hipStream_t st) {
+  return lde_impl<ef>(a, count, m, k, R, lp2_for(k, true), tw, st);
 }
 
 }  // namespace kb

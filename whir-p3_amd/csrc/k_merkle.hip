@@ -25,9 +25,11 @@ namespace kb {
 // chunk (the empty asm makes their offset opaque to the loop), so they are not
 // held in, and spilled from, scalar registers across the loop.
 template <int LW>
-__global__ void __launch_bounds__(256) k_leaf_hash(const u32* __restrict__ rows, int row_len, u64 nrows,
-                                                   u32* __restrict__ dig, const P2Consts<LW>* __restrict__ cp) {
+__global__ void __launch_bounds__(256) k_leaf_hash(Group<TreeArgs> g, int row_len, u64 nrows,
+                                                   const P2Consts<LW>* __restrict__ cp) {
   constexpr int RATE = LW - 8;
+  const u32* __restrict__ rows = g.v[blockIdx.z].rows;
+  u32* __restrict__ dig = g.v[blockIdx.z].tree;
   u64 i = (u64)blockIdx.x * 256 + threadIdx.x;
   if (i >= nrows) return;
   u32 s[LW];
@@ -77,9 +79,12 @@ __device__ __forceinline__ void compress_pair(const u32* l, const u32* r, u32* o
   for (int k = 0; k < 8; k++) out[k] = s[k];
 }
 
-// One tree level: out[j] = compress(in[2j], in[2j+1]).
-__global__ void __launch_bounds__(256) k_tree_level(const u32* __restrict__ in, u64 n_out, u32* __restrict__ out,
-                                                    P2C16 c) {
+// One tree level: out[j] = compress(in[2j], in[2j+1]), where the level of
+// 2 n_out digests starts at digest `off` of the tree and the next follows it.
+__global__ void __launch_bounds__(256) k_tree_level(Group<TreeArgs> g, u64 off, u64 n_out, P2C16 c) {
+  u32* tree = g.v[blockIdx.z].tree;
+  const u32* __restrict__ in = tree + off * 8;
+  u32* __restrict__ out = tree + (off + 2 * n_out) * 8;
   u64 j = (u64)blockIdx.x * 256 + threadIdx.x;
   if (j >= n_out) return;
   const uint4* v = reinterpret_cast<const uint4*>(in + 16 * j);
@@ -108,8 +113,9 @@ __global__ void __launch_bounds__(256) k_tree_level(const u32* __restrict__ in, 
 // runs its levels at the permutation's latency (with S = 128 each block holds
 // four waves per SIMD for its first level).
 template <int S>
-__global__ void __launch_bounds__(8 * S) k_tree_sub_wide(u32* __restrict__ tree, u64 off, u64 n, P2C16 c) {
+__global__ void __launch_bounds__(8 * S) k_tree_sub_wide(Group<TreeArgs> g, u64 off, u64 n, P2C16 c) {
   __shared__ u32 sh[2][S * 8];
+  u32* __restrict__ tree = g.v[blockIdx.z].tree;
   const int t = threadIdx.x;
   const int j = t & 15;
   const WideLane w = wide_lane(c, j);
@@ -142,9 +148,10 @@ __global__ void __launch_bounds__(8 * S) k_tree_sub_wide(u32* __restrict__ tree,
 // (the lane-parallel permutation only needs its own wave's lanes).  The root
 // is also written to `root_out` when given (host-visible memory).
 constexpr int TREE_TOP = 32;
-__global__ void __launch_bounds__(8 * TREE_TOP) k_tree_top_wide(u32* __restrict__ tree, u64 off, int n, P2C16 c,
-                                                                u32* root_out) {
+__global__ void __launch_bounds__(8 * TREE_TOP) k_tree_top_wide(Group<TreeArgs> g, u64 off, int n, P2C16 c) {
   __shared__ u32 sh[2][TREE_TOP * 8];
+  u32* __restrict__ tree = g.v[blockIdx.z].tree;
+  u32* root_out = g.v[blockIdx.z].root_out;
   const int t = threadIdx.x;
   const int j = t & 15;
   const WideLane w = wide_lane(c, j);
@@ -170,12 +177,18 @@ __global__ void __launch_bounds__(8 * TREE_TOP) k_tree_top_wide(u32* __restrict_
   if (root_out && t < 8) root_out[t] = sh[cur][t];
 }
 
-// Opening of `nq` rows: out[q] = row (row_len) followed by depth sibling digests.
-// Grid: (ceil(per / 256), nq); block (x, q) copies entries x*256.. of query q's
-// opening (row, then the sibling digest of each level).  `idx` and `out` may
-// be host-visible memory.
-__global__ void k_open(const u32* __restrict__ rows, int row_len, const u32* __restrict__ tree, u64 height, int depth,
-                       const u64* __restrict__ idx, u32* __restrict__ out) {
+// Opening of the rows at idx: out[q] = row (row_len) followed by depth
+// sibling digests.  Grid: (ceil(per / 256), max_nq, proofs); block (x, q)
+// copies entries x*256.. of query q's opening (row, then the sibling digest of
+// each level); queries at or past *nq (when given) are skipped.  `idx` and
+// `out` may be host-visible memory.
+__global__ void k_open(Group<OpenArgs> g, int row_len, u64 height, int depth) {
+  const OpenArgs& a = g.v[blockIdx.z];
+  if (a.nq && blockIdx.y >= *a.nq) return;
+  const u32* __restrict__ rows = a.rows;
+  const u32* __restrict__ tree = a.tree;
+  const u64* __restrict__ idx = a.idx;
+  u32* __restrict__ out = a.out;
   const u64 per = (u64)row_len + 8ull * depth;
   const u64 q = blockIdx.y;
   const u64 o = (u64)blockIdx.x * blockDim.x + threadIdx.x;
@@ -194,17 +207,18 @@ __global__ void k_open(const u32* __restrict__ rows, int row_len, const u32* __r
 }
 
 template <int LW>
-int merkle_leaves_impl(const u32* rows, int row_len, u64 height, u32* tree, const P2Consts<LW>* c, hipStream_t st) {
-  unsigned blocks = (unsigned)((height + 255) / 256);
-  hipLaunchKernelGGL((k_leaf_hash<LW>), dim3(blocks), dim3(256), 0, st, rows, row_len, height, tree, c);
-  return hipGetLastError() == hipSuccess ? 0 : -5;
+int merkle_leaves_impl(const TreeArgs* a, int count, int row_len, u64 height, const P2Consts<LW>* c, hipStream_t st) {
+  const unsigned blocks = (unsigned)((height + 255) / 256);
+  return launch_group(a, count, [&](const Group<TreeArgs>& g) {
+    hipLaunchKernelGGL((k_leaf_hash<LW>), dim3(blocks, 1, count), dim3(256), 0, st, g, row_len, height, c);
+  });
 }
 
-int merkle_leaves(const u32* rows, int row_len, u64 height, int leaf_width, const P2C16* c16, const P2C24* c24,
-                  u32* tree, hipStream_t st) {
+int merkle_leaves(const TreeArgs* a, int count, int row_len, u64 height, int leaf_width, const P2C16* c16,
+                  const P2C24* c24, hipStream_t st) {
   if (height == 0 || (height & (height - 1))) return -1;
-  return leaf_width == 24 ? merkle_leaves_impl<24>(rows, row_len, height, tree, c24, st)
-                          : merkle_leaves_impl<16>(rows, row_len, height, tree, c16, st);
+  return leaf_width == 24 ? merkle_leaves_impl<24>(a, count, row_len, height, c24, st)
+                          : merkle_leaves_impl<16>(a, count, row_len, height, c16, st);
 }
 
 // Levels with more than 2^14 nodes use one lane per node (throughput); below
@@ -212,15 +226,18 @@ int merkle_leaves(const u32* rows, int row_len, u64 height, int leaf_width, cons
 // the last (at most 5) levels in one block.
 constexpr int TREE_SUB = 32;
 static_assert(TREE_SUB <= TREE_TOP, "the top block takes what the subtree launches leave");
-int merkle_tree(u64 height, const P2C16& c16, u32* tree, hipStream_t st, u32* root_out) {
+int merkle_tree(const TreeArgs* a, int count, u64 height, const P2C16& c16, hipStream_t st) {
   if (height == 0 || (height & (height - 1))) return -1;
+  if (count < 1 || count > GROUP_MAX) return -8;
+  Group<TreeArgs> g;
+  for (int i = 0; i < count; i++) g.v[i] = a[i];
+  const unsigned np = (unsigned)count;
   u64 off = 0, n = height;
   while (n > TREE_SUB) {
     const u64 n_out = n >> 1;
     if (n_out > (1u << 14)) {
       unsigned blocks = (unsigned)((n_out + 255) / 256);
-      hipLaunchKernelGGL(k_tree_level, dim3(blocks), dim3(256), 0, st, tree + off * 8, n_out, tree + (off + n) * 8,
-                         c16);
+      hipLaunchKernelGGL(k_tree_level, dim3(blocks, 1, np), dim3(256), 0, st, g, off, n_out, c16);
       if (hipGetLastError() != hipSuccess) return -5;
       off += n;
       n = n_out;
@@ -233,41 +250,31 @@ int merkle_tree(u64 height, const P2C16& c16, u32* tree, hipStream_t st, u32* ro
     // one wave per SIMD.
     const u64 S = n >= (1u << 15) ? 128 : TREE_SUB;
     if (S == 128)
-      hipLaunchKernelGGL(k_tree_sub_wide<128>, dim3((unsigned)(n / 128)), dim3(8 * 128), 0, st, tree, off, n, c16);
+      hipLaunchKernelGGL(k_tree_sub_wide<128>, dim3((unsigned)(n / 128), 1, np), dim3(8 * 128), 0, st, g, off, n, c16);
     else
-      hipLaunchKernelGGL(k_tree_sub_wide<TREE_SUB>, dim3((unsigned)(n / TREE_SUB)), dim3(8 * TREE_SUB), 0, st, tree,
-                         off, n, c16);
+      hipLaunchKernelGGL(k_tree_sub_wide<TREE_SUB>, dim3((unsigned)(n / TREE_SUB), 1, np), dim3(8 * TREE_SUB), 0, st,
+                         g, off, n, c16);
     if (hipGetLastError() != hipSuccess) return -5;
     off += 2 * n - n / (S / 2);
     n /= S;
   }
   if (n > 1) {
-    hipLaunchKernelGGL(k_tree_top_wide, dim3(1), dim3(8 * TREE_TOP), 0, st, tree, off, (int)n, c16, root_out);
+    hipLaunchKernelGGL(k_tree_top_wide, dim3(1, 1, np), dim3(8 * TREE_TOP), 0, st, g, off, (int)n, c16);
     if (hipGetLastError() != hipSuccess) return -5;
-  } else if (root_out) {
-    return 1;   // single leaf: the root was not written
+    return 0;
   }
-  return 0;
+  return 1;   // single leaf: the root was not written
 }
 
-int merkle_commit(const u32* rows, int row_len, u64 height, int leaf_width, const P2C16& c16, const P2C16* d_c16,
-                  const P2C24* d_c24,
-                  u32* tree, hipStream_t st) {
-  int rc = merkle_leaves(rows, row_len, height, leaf_width, d_c16, d_c24, tree, st);
-  if (rc) return rc;
-  rc = merkle_tree(height, c16, tree, st, nullptr);
-  return rc > 0 ? 0 : rc;
-}
-
-int merkle_open(const u32* rows, int row_len, const u32* tree, u64 height, const u64* d_idx, int nq, u32* d_out,
-                hipStream_t st) {
+int merkle_open(const OpenArgs* a, int count, int row_len, u64 height, int max_nq, hipStream_t st) {
   int depth = 0;
   while ((1ull << depth) < height) depth++;
   const u64 per = (u64)row_len + 8ull * depth;
-  if (nq <= 0) return 0;
-  hipLaunchKernelGGL(k_open, dim3((unsigned)((per + 255) / 256), (unsigned)nq), dim3(256), 0, st, rows, row_len, tree,
-                     height, depth, d_idx, d_out);
-  return hipGetLastError() == hipSuccess ? 0 : -5;
+  if (max_nq <= 0) return 0;
+  return launch_group(a, count, [&](const Group<OpenArgs>& g) {
+    hipLaunchKernelGGL(k_open, dim3((unsigned)((per + 255) / 256), (unsigned)max_nq, count), dim3(256), 0, st, g,
+                       row_len, height, depth);
+  });
 }
 
 // ------------------------------------------------------------------ grinding
@@ -289,10 +296,6 @@ int merkle_open(const u32* rows, int row_len, const u32* tree, u64 height, const
 // cannot hide a smaller witness: the result only decreases, so every candidate
 // below the final result is checked, whatever order waves run in, and the
 // result is the smallest witness in the range.
-struct Sponge16 {
-  u32 s[16];
-};
-
 __device__ __forceinline__ u32 best_known(const u32* result) {
   return __hip_atomic_load(result, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -310,11 +313,16 @@ __device__ __forceinline__ void phase_fence(u32* s) {
 
 // With tr_mode == TR_COEFFS_GRIND every block first runs the transcript step
 // that observes the round's (h(0), h(2)); its state is the candidates' base.
-__global__ void __launch_bounds__(256) k_grind(const P2C16* __restrict__ cp, Sponge16 base, int wpos, int tr_mode,
-                                               ScTr tr, u32 mask, u32* result, u32 start, u32 end) {
+// Grid: (blocks, 1, proofs), one search per proof.
+__global__ void __launch_bounds__(256) k_grind(const P2C16* __restrict__ cp, Group<GrindArgs> ga, int tr_mode,
+                                               u32 mask, u32 start, u32 end) {
+  const GrindArgs& a = ga.v[blockIdx.z];
+  Sponge16 base = a.base;
+  int wpos = a.wpos;
+  u32* result = a.result;
   if (tr_mode == TR_COEFFS_GRIND || tr_mode == TR_OBS_GRIND) {
     __shared__ GrindBase gbs;
-    tr_prologue(tr_mode, tr, *cp, &gbs, nullptr);
+    tr_prologue(tr_mode, a.tr, *cp, &gbs, nullptr);
 #pragma unroll
     for (int k = 0; k < 16; k++) base.s[k] = gbs.s[k];
     wpos = (int)gbs.wpos;
@@ -357,12 +365,15 @@ __global__ void __launch_bounds__(256) k_grind(const P2C16* __restrict__ cp, Spo
 // order and stop rule as k_grind, with a wave's four candidates as the unit.
 constexpr int WIDE_GRIND_BITS = 12;
 
-__global__ void __launch_bounds__(256) k_grind_wide(const P2C16* __restrict__ cp, Sponge16 base, int wpos,
-                                                    int tr_mode, ScTr tr, u32 mask, u32* result, u32 start,
-                                                    u32 end) {
+__global__ void __launch_bounds__(256) k_grind_wide(const P2C16* __restrict__ cp, Group<GrindArgs> ga, int tr_mode,
+                                                    u32 mask, u32 start, u32 end) {
+  const GrindArgs& a = ga.v[blockIdx.z];
+  Sponge16 base = a.base;
+  int wpos = a.wpos;
+  u32* result = a.result;
   if (tr_mode == TR_COEFFS_GRIND || tr_mode == TR_OBS_GRIND) {
     __shared__ GrindBase gbs;
-    tr_prologue(tr_mode, tr, *cp, &gbs, nullptr);
+    tr_prologue(tr_mode, a.tr, *cp, &gbs, nullptr);
 #pragma unroll
     for (int k = 0; k < 16; k++) base.s[k] = gbs.s[k];
     wpos = (int)gbs.wpos;
@@ -407,11 +418,15 @@ constexpr int LOCK_NT = LOCK_NT_OVERRIDE;
 #endif
 
 template <int NT>
-__global__ void __launch_bounds__(NT) k_grind_lock(const P2C16* __restrict__ cp, Sponge16 base, int wpos,
-                                                   const GrindBase* gb, u32 mask, u32* result, u32 start,
-                                                   u32 end) {
+__global__ void __launch_bounds__(NT) k_grind_lock(const P2C16* __restrict__ cp, Group<GrindArgs> ga, int use_gbase,
+                                                   u32 mask, u32 start, u32 end) {
   __shared__ u32 sb[2];
-  if (gb) {
+  const GrindArgs& a = ga.v[blockIdx.z];
+  Sponge16 base = a.base;
+  int wpos = a.wpos;
+  u32* result = a.result;
+  if (use_gbase) {
+    const GrindBase* gb = a.tr.gbase;
 #pragma unroll
     for (int k = 0; k < 16; k++) base.s[k] = gb->s[k];
     wpos = (int)gb->wpos;
@@ -442,11 +457,12 @@ __global__ void __launch_bounds__(NT) k_grind_lock(const P2C16* __restrict__ cp,
   }
 }
 
-// The transcript step of a long search in one block: writes the candidates'
-// base state to gb_out (and, from block 0, the next challenger state and log).
-__global__ void __launch_bounds__(256) k_grind_base(const P2C16* __restrict__ cp, int tr_mode, ScTr tr,
-                                                    GrindBase* gb_out) {
+// The transcript step of a long search in one block per proof: writes the
+// candidates' base state to tr.gbase (and the next challenger state and log).
+__global__ void __launch_bounds__(256) k_grind_base(const P2C16* __restrict__ cp, int tr_mode, Group<GrindArgs> ga) {
   __shared__ GrindBase gbs;
+  const ScTr& tr = ga.v[blockIdx.z].tr;
+  GrindBase* gb_out = tr.gbase;
   tr_prologue(tr_mode, tr, *cp, &gbs, nullptr);
   if (threadIdx.x < 16) gb_out->s[threadIdx.x] = gbs.s[threadIdx.x];
   if (threadIdx.x == 0) gb_out->wpos = gbs.wpos;
@@ -460,60 +476,39 @@ static unsigned wide_grind_blocks(int bits, u64 start, u64 end, int blocks) {
                                                   (u64)std::max(blocks, 1)));
 }
 
-// Scans [start, start + count) (clipped to p) in one launch of `blocks` blocks
-// of 256 lanes (fewer when the range is short).
-int grind_range(const P2C16* d_consts, const u32* state16, int wpos, int bits, u32* d_result, u64 start, u64 count,
-                int blocks, hipStream_t st) {
-  Sponge16 b;
-  for (int k = 0; k < 16; k++) b.s[k] = state16[k];
-  u32 mask = bits >= 32 ? 0xffffffffu : ((1u << bits) - 1);
-  if (start >= P) return 0;
-  u64 end = std::min<u64>(start + count, P);
-  if (bits <= WIDE_GRIND_BITS) {
-    const unsigned nb = wide_grind_blocks(bits, start, end, blocks);
-    hipLaunchKernelGGL(k_grind_wide, dim3(nb), dim3(256), 0, st, d_consts, b, wpos, (int)TR_NONE, ScTr{}, mask,
-                       d_result, (u32)start, (u32)end);
-    return hipGetLastError() == hipSuccess ? 0 : -5;
-  }
-  if (bits >= LOCK_GRIND_BITS) {
-    const unsigned nb = (unsigned)std::max<u64>(1, std::min<u64>((end - start + LOCK_NT - 1) / LOCK_NT,
-                                                                 (u64)std::max(blocks, 1)));
-    hipLaunchKernelGGL(k_grind_lock<LOCK_NT>, dim3(nb), dim3(LOCK_NT), 0, st, d_consts, b, wpos,
-                       (const GrindBase*)nullptr, mask, d_result, (u32)start, (u32)end);
-    return hipGetLastError() == hipSuccess ? 0 : -5;
-  }
-  u64 need = (end - start + 255) / 256;
-  unsigned nb = (unsigned)std::min<u64>(need, (u64)std::max(blocks, 1));
-  hipLaunchKernelGGL(k_grind, dim3(nb), dim3(256), 0, st, d_consts, b, wpos, (int)TR_NONE, ScTr{}, mask, d_result,
-                     (u32)start, (u32)end);
-  return hipGetLastError() == hipSuccess ? 0 : -5;
-}
+static_assert(sizeof(Group<GrindArgs>) <= 4096, "kernel arguments of a grinding group");
 
-int grind_range_tr(const P2C16* d_consts, const ScTr& tr, int bits, u64 start, u64 count, int blocks, hipStream_t st,
-                   int mode) {
-  Sponge16 b{};
-  u32 mask = bits >= 32 ? 0xffffffffu : ((1u << bits) - 1);
+// Scans [start, start + span) (clipped to p) for every proof of the group in
+// one launch of `blocks` blocks per proof (fewer when the range is short).
+int grind_range(const P2C16* d_consts, const GrindArgs* a, int count, int bits, int mode, u64 start, u64 span,
+                int blocks, hipStream_t st) {
+  const u32 mask = bits >= 32 ? 0xffffffffu : ((1u << bits) - 1);
   if (start >= P) return 0;
-  u64 end = std::min<u64>(start + count, P);
+  const u64 end = std::min<u64>(start + span, P);
+  if (count < 1 || count > GROUP_MAX) return -8;
   if (bits <= WIDE_GRIND_BITS) {
     const unsigned nb = wide_grind_blocks(bits, start, end, blocks);
-    hipLaunchKernelGGL(k_grind_wide, dim3(nb), dim3(256), 0, st, d_consts, b, 0, mode, tr, mask, tr.grind,
-                       (u32)start, (u32)end);
-    return hipGetLastError() == hipSuccess ? 0 : -5;
+    return launch_group(a, count, [&](const Group<GrindArgs>& g) {
+      hipLaunchKernelGGL(k_grind_wide, dim3(nb, 1, count), dim3(256), 0, st, d_consts, g, mode, mask, (u32)start,
+                         (u32)end);
+    });
   }
-  if (bits >= LOCK_GRIND_BITS && tr.gbase) {
-    hipLaunchKernelGGL(k_grind_base, dim3(1), dim3(256), 0, st, d_consts, mode, tr, tr.gbase);
+  bool gbase = mode != TR_NONE;
+  for (int i = 0; i < count; i++) gbase = gbase && a[i].tr.gbase;
+  if (bits >= LOCK_GRIND_BITS && (mode == TR_NONE || gbase)) {
     const unsigned nb = (unsigned)std::max<u64>(1, std::min<u64>((end - start + LOCK_NT - 1) / LOCK_NT,
                                                                  (u64)std::max(blocks, 1)));
-    hipLaunchKernelGGL(k_grind_lock<LOCK_NT>, dim3(nb), dim3(LOCK_NT), 0, st, d_consts, b, 0,
-                       (const GrindBase*)tr.gbase, mask, tr.grind, (u32)start, (u32)end);
-    return hipGetLastError() == hipSuccess ? 0 : -5;
+    return launch_group(a, count, [&](const Group<GrindArgs>& g) {
+      if (gbase) hipLaunchKernelGGL(k_grind_base, dim3(1, 1, count), dim3(256), 0, st, d_consts, mode, g);
+      hipLaunchKernelGGL(k_grind_lock<LOCK_NT>, dim3(nb, 1, count), dim3(LOCK_NT), 0, st, d_consts, g,
+                         gbase ? 1 : 0, mask, (u32)start, (u32)end);
+    });
   }
-  u64 need = (end - start + 255) / 256;
-  unsigned nb = (unsigned)std::min<u64>(need, (u64)std::max(blocks, 1));
-  hipLaunchKernelGGL(k_grind, dim3(nb), dim3(256), 0, st, d_consts, b, 0, mode, tr, mask, tr.grind, (u32)start,
-                     (u32)end);
-  return hipGetLastError() == hipSuccess ? 0 : -5;
+  const u64 need = (end - start + 255) / 256;
+  const unsigned nb = (unsigned)std::min<u64>(need, (u64)std::max(blocks, 1));
+  return launch_group(a, count, [&](const Group<GrindArgs>& g) {
+    hipLaunchKernelGGL(k_grind, dim3(nb, 1, count), dim3(256), 0, st, d_consts, g, mode, mask, (u32)start, (u32)end);
+  });
 }
 
 // ------------------------------------------------------------------ round transcript
@@ -522,10 +517,16 @@ int grind_range_tr(const P2C16* d_consts, const ScTr& tr, int bits, u64 start, u
 // (expand_from_univariate, src/poly/multilinear.rs:73-85), then writes its
 // part of the point's split eq tables, as k_eq_split.  Block 0 logs y,
 // writes the point and the next challenger state.
-__global__ void __launch_bounds__(256) k_ood_eq_tables(StepTr st, ef* log_y, ef* pt_out, int m, int hi, int lo,
-                                                       ef* __restrict__ out_hi, ef* __restrict__ out_lo,
-                                                       const P2C16* __restrict__ cp, u32* grind_reset) {
+__global__ void __launch_bounds__(256) k_ood_eq_tables(Group<OodArgs> g, int m, int hi, int lo,
+                                                       const P2C16* __restrict__ cp) {
   __shared__ u32 sp[16], in[8], out[8];
+  const OodArgs& a = g.v[blockIdx.z];
+  const StepTr& st = a.st;
+  ef* log_y = a.log_y;
+  ef* pt_out = a.pt_out;
+  ef* __restrict__ out_hi = a.out_hi;
+  ef* __restrict__ out_lo = a.out_lo;
+  u32* grind_reset = a.grind_reset;
   __shared__ ef z[RT_MAX_VARS];
   const int t = threadIdx.x;
   const P2C16& c = *cp;
@@ -571,13 +572,12 @@ __global__ void __launch_bounds__(256) k_ood_eq_tables(StepTr st, ef* log_y, ef*
   o[x] = acc;
 }
 
-int ood_eq_tables(const StepTr& st, ef* log_y, ef* pt_out, int m, int hi, int lo, ef* out_hi, ef* out_lo,
-                  const P2C16* d_consts, hipStream_t st_, u32* grind_reset) {
+int ood_eq_tables(const OodArgs* a, int count, int m, int hi, int lo, const P2C16* d_consts, hipStream_t st) {
   if (m < 1 || m > RT_MAX_VARS || hi + lo != m) return -1;
   const u32 n = (1u << hi) + (1u << lo);
-  hipLaunchKernelGGL(k_ood_eq_tables, dim3((n + 255) / 256), dim3(256), 0, st_, st, log_y, pt_out, m, hi, lo, out_hi,
-                     out_lo, d_consts, grind_reset);
-  return hipGetLastError() == hipSuccess ? 0 : -5;
+  return launch_group(a, count, [&](const Group<OodArgs>& g) {
+    hipLaunchKernelGGL(k_ood_eq_tables, dim3((n + 255) / 256, 1, count), dim3(256), 0, st, g, m, hi, lo, d_consts);
+  });
 }
 
 // The round's steps after its OOD samples, in one block (prover/mod.rs:
@@ -588,11 +588,15 @@ int ood_eq_tables(const StepTr& st, ef* log_y, ef* pt_out, int m, int hi, int lo
 // per-query indices), sorted and deduplicated, and gamma is sampled.  The
 // final round (prover/mod.rs:343-377) has neither the discarded sample nor
 // gamma (final_step).
-__global__ void __launch_bounds__(256) k_round_queries(StepTr st, const u32* grind, int pow_bits, u64 domain_size,
-                                                       int ff, int num_queries, u32 folded_gen, int n_ood,
-                                                       RoundLog* log, RoundBufs rb, const P2C16* __restrict__ cp,
-                                                       int final_step) {
+__global__ void __launch_bounds__(256) k_round_queries(Group<QueryArgs> g, int pow_bits, u64 domain_size, int ff,
+                                                       int num_queries, u32 folded_gen, int n_ood,
+                                                       const P2C16* __restrict__ cp, int final_step) {
   __shared__ u32 sp[16], in[8], out[8];
+  const QueryArgs& a = g.v[blockIdx.z];
+  const StepTr& st = a.st;
+  const u32* grind = a.grind;
+  RoundLog* log = a.log;
+  const RoundBufs& rb = a.rb;
   __shared__ u64 q[RT_MAX_QUERIES];
   __shared__ u32 firsts[RT_MAX_QUERIES];
   const int t = threadIdx.x;
@@ -678,45 +682,13 @@ __global__ void __launch_bounds__(256) k_round_queries(StepTr st, const u32* gri
   }
 }
 
-int round_queries(const StepTr& st, const u32* grind, int pow_bits, u64 domain_size, int folding_factor,
-                  int num_queries, u32 folded_gen, int n_ood, RoundLog* log, const RoundBufs& rb,
-                  const P2C16* d_consts, hipStream_t st_, bool final_step) {
+int round_queries(const QueryArgs* a, int count, int pow_bits, u64 domain_size, int folding_factor, int num_queries,
+                  u32 folded_gen, int n_ood, const P2C16* d_consts, hipStream_t st, bool final_step) {
   if (num_queries < 1 || num_queries > RT_MAX_QUERIES || n_ood < 0 || n_ood > RT_MAX_OOD) return -1;
-  hipLaunchKernelGGL(k_round_queries, dim3(1), dim3(256), 0, st_, st, grind, pow_bits, domain_size, folding_factor,
-                     num_queries, folded_gen, n_ood, log, rb, d_consts, final_step ? 1 : 0);
-  return hipGetLastError() == hipSuccess ? 0 : -5;
-}
-
-// k_open over the first *nq of `idx` (grid sized for max_nq).
-__global__ void k_open_dev(const u32* __restrict__ rows, int row_len, const u32* __restrict__ tree, u64 height,
-                           int depth, const u64* __restrict__ idx, const u32* __restrict__ nq, u32* __restrict__ out) {
-  if (blockIdx.y >= *nq) return;
-  const u64 per = (u64)row_len + 8ull * depth;
-  const u64 q = blockIdx.y;
-  const u64 o = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= per) return;
-  const u64 ix = idx[q];
-  u32 v;
-  if (o < (u64)row_len) {
-    v = rows[ix * row_len + o];
-  } else {
-    u64 l = (o - row_len) / 8, k = (o - row_len) % 8;
-    u64 off = 2 * height - ((2 * height) >> l);
-    u64 node = (ix >> l) ^ 1;
-    v = tree[(off + node) * 8 + k];
-  }
-  out[q * per + o] = v;
-}
-
-int merkle_open_dev(const u32* rows, int row_len, const u32* tree, u64 height, const u64* d_idx, const u32* d_nq,
-                    int max_nq, u32* d_out, hipStream_t st) {
-  int depth = 0;
-  while ((1ull << depth) < height) depth++;
-  const u64 per = (u64)row_len + 8ull * depth;
-  if (max_nq <= 0) return 0;
-  hipLaunchKernelGGL(k_open_dev, dim3((unsigned)((per + 255) / 256), (unsigned)max_nq), dim3(256), 0, st, rows,
-                     row_len, tree, height, depth, d_idx, d_nq, d_out);
-  return hipGetLastError() == hipSuccess ? 0 : -5;
+  return launch_group(a, count, [&](const Group<QueryArgs>& g) {
+    hipLaunchKernelGGL(k_round_queries, dim3(1, 1, count), dim3(256), 0, st, g, pow_bits, domain_size,
+                       folding_factor, num_queries, folded_gen, n_ood, d_consts, final_step ? 1 : 0);
+  });
 }
 
 }  // namespace kb

@@ -32,32 +32,40 @@ __global__ void k_eq_small(PointArg z, int first, int t, ef scale, ef* __restric
   out[x] = acc;
 }
 
-__global__ void k_eq_split(PointArg z, int hi, int lo, ef scale, ef* __restrict__ out_hi, ef* __restrict__ out_lo) {
+// Grid: (ceil((2^hi + 2^lo) / 256), 1, proofs).  The point's coordinates are
+// staged in LDS first (they may be in host-visible memory).
+__global__ void k_eq_split(Group<EqSplitArgs> g, int hi, int lo) {
+  __shared__ ef zs[32];
+  const EqSplitArgs& a = g.v[blockIdx.z];
+  if (threadIdx.x < (unsigned)(hi + lo)) zs[threadIdx.x] = a.z[threadIdx.x];
+  __syncthreads();
   u32 x = blockIdx.x * blockDim.x + threadIdx.x;
   const u32 nh = 1u << hi;
   if (x >= nh + (1u << lo)) return;
   int first = 0, t = hi;
-  ef acc = scale;
-  ef* out = out_hi;
+  ef acc = a.scale;
+  ef* out = a.out_hi;
   if (x >= nh) {
     x -= nh;
     first = hi;
     t = lo;
     acc = ef_one();
-    out = out_lo;
+    out = a.out_lo;
   }
   for (int j = 0; j < t; j++) {
-    const ef& zj = z.z[first + j];
+    const ef& zj = zs[first + j];
     int bit = (x >> (t - 1 - j)) & 1;
     acc = ef_mul(acc, bit ? zj : ef_sub(ef_one(), zj));
   }
   out[x] = acc;
 }
 
-int eq_split(const PointArg& z, int hi, int lo, const ef& scale, ef* out_hi, ef* out_lo, hipStream_t st) {
-  u32 n = (1u << hi) + (1u << lo);
-  hipLaunchKernelGGL(k_eq_split, dim3((n + 255) / 256), dim3(256), 0, st, z, hi, lo, scale, out_hi, out_lo);
-  return hipGetLastError() == hipSuccess ? 0 : -5;
+int eq_split(const EqSplitArgs* a, int count, int hi, int lo, hipStream_t st) {
+  if (hi < 0 || lo < 0 || hi + lo > 32) return -1;
+  const u32 n = (1u << hi) + (1u << lo);
+  return launch_group(a, count, [&](const Group<EqSplitArgs>& g) {
+    hipLaunchKernelGGL(k_eq_split, dim3((n + 255) / 256, 1, count), dim3(256), 0, st, g, hi, lo);
+  });
 }
 
 int eq_table(const PointArg& z, int first, int t, const ef& scale, ef* out, hipStream_t st) {
@@ -91,9 +99,12 @@ int eq_outer(const ef* Ehi, const ef* Elo, int lo, u64 n, ef* out, hipStream_t s
 // Thread t owns xl = t mod 2^lo and rows xh = (t >> lo) + i * (256 >> lo).
 // partial[(seg * gridDim.x + block) * K + k] receives the block's sum.
 template <typename T, int K>
-__global__ void __launch_bounds__(256) k_eq_dot(const T* __restrict__ f, int nf, int sb, int lo, EqTables tabs,
-                                                ef* __restrict__ partial) {
+__global__ void __launch_bounds__(256) k_eq_dot(Group<DotArgs> g, int nf, int sb, int lo) {
   __shared__ ef sh[256 * K];
+  const DotArgs& a = g.v[blockIdx.z];
+  const T* __restrict__ f = static_cast<const T*>(a.f);
+  const EqTables& tabs = a.tabs;
+  ef* __restrict__ partial = a.partial;
   const int t = threadIdx.x;
   const int seg = blockIdx.y;
   const int nrest = nf - sb;
@@ -140,9 +151,12 @@ __device__ __forceinline__ u64 fold32(u64 t) {
 // unreduced 64-bit sum of hi[k][r] * f products, folded every four rows and
 // reduced once at the end.
 template <int K>
-__global__ void __launch_bounds__(256) k_eq_dot4(const u32* __restrict__ f, int nf, int sb, int lo, EqTables tabs,
-                                                 ef* __restrict__ partial) {
+__global__ void __launch_bounds__(256) k_eq_dot4(Group<DotArgs> g, int nf, int sb, int lo) {
   __shared__ ef sh[256 * K];
+  const DotArgs& a = g.v[blockIdx.z];
+  const u32* __restrict__ f = static_cast<const u32*>(a.f);
+  const EqTables& tabs = a.tabs;
+  ef* __restrict__ partial = a.partial;
   const int t = threadIdx.x;
   const int seg = blockIdx.y;
   const int nrest = nf - sb;
@@ -245,10 +259,15 @@ __global__ void __launch_bounds__(256) k_eq_dot4(const u32* __restrict__ f, int 
   }
 }
 
-// out[seg * K + k] = sum_b partial[(seg * nb + b) * K + k]; one block per (seg, k).
-__global__ void __launch_bounds__(256) k_reduce_strided(const ef* __restrict__ partial, int nb, int K,
-                                                        ef* __restrict__ out) {
+// out[seg * K + k] = sum_b partial[(seg * nb + b) * K + k]; one block per (seg, k) and proof.
+struct RedArgs {
+  const ef* partial;
+  ef* out;
+};
+__global__ void __launch_bounds__(256) k_reduce_strided(Group<RedArgs> g, int nb, int K) {
   __shared__ ef sh[256];
+  const ef* __restrict__ partial = g.v[blockIdx.z].partial;
+  ef* __restrict__ out = g.v[blockIdx.z].out;
   const int seg = blockIdx.x / K, k = blockIdx.x % K;
   ef acc[1] = {ef_zero()};
   for (int b = threadIdx.x; b < nb; b += 256) acc[0] = ef_add(acc[0], partial[((u64)seg * nb + b) * K + k]);
@@ -257,42 +276,38 @@ __global__ void __launch_bounds__(256) k_reduce_strided(const ef* __restrict__ p
 }
 
 template <typename T>
-static int eq_dot_impl(const T* f, int nf, int sb, int lo, const EqTables& tabs, int K, ef* partial, int nblocks,
-                       hipStream_t st) {
-  dim3 g(nblocks, 1u << sb);
-  if constexpr (sizeof(T) == 4) {
-    if (lo >= 2 && K <= 2) {
-      if (K == 1) hipLaunchKernelGGL((k_eq_dot4<1>), g, dim3(256), 0, st, f, nf, sb, lo, tabs, partial);
-      else hipLaunchKernelGGL((k_eq_dot4<2>), g, dim3(256), 0, st, f, nf, sb, lo, tabs, partial);
-      return hipGetLastError() == hipSuccess ? 0 : -5;
+static int eq_dot_impl(const DotArgs* a, int count, int nf, int sb, int lo, int K, int nblocks, hipStream_t st) {
+  if (count < 1 || count > GROUP_MAX || K < 1 || K > 4) return -1;
+  Group<DotArgs> g;
+  for (int i = 0; i < count; i++) g.v[i] = a[i];
+  const dim3 grid(nblocks, 1u << sb, count);
+  bool four = false;
+  if constexpr (sizeof(T) == 4) four = lo >= 2 && K <= 2;
+  if (four) {
+    if (K == 1) hipLaunchKernelGGL((k_eq_dot4<1>), grid, dim3(256), 0, st, g, nf, sb, lo);
+    else hipLaunchKernelGGL((k_eq_dot4<2>), grid, dim3(256), 0, st, g, nf, sb, lo);
+  } else {
+    switch (K) {
+      case 1: hipLaunchKernelGGL((k_eq_dot<T, 1>), grid, dim3(256), 0, st, g, nf, sb, lo); break;
+      case 2: hipLaunchKernelGGL((k_eq_dot<T, 2>), grid, dim3(256), 0, st, g, nf, sb, lo); break;
+      case 3: hipLaunchKernelGGL((k_eq_dot<T, 3>), grid, dim3(256), 0, st, g, nf, sb, lo); break;
+      default: hipLaunchKernelGGL((k_eq_dot<T, 4>), grid, dim3(256), 0, st, g, nf, sb, lo); break;
     }
   }
-  switch (K) {
-    case 1: hipLaunchKernelGGL((k_eq_dot<T, 1>), g, dim3(256), 0, st, f, nf, sb, lo, tabs, partial); break;
-    case 2: hipLaunchKernelGGL((k_eq_dot<T, 2>), g, dim3(256), 0, st, f, nf, sb, lo, tabs, partial); break;
-    case 3: hipLaunchKernelGGL((k_eq_dot<T, 3>), g, dim3(256), 0, st, f, nf, sb, lo, tabs, partial); break;
-    case 4: hipLaunchKernelGGL((k_eq_dot<T, 4>), g, dim3(256), 0, st, f, nf, sb, lo, tabs, partial); break;
-    default: return -1;
-  }
+  // partial is [seg][block][k]; reduce over blocks for each (seg, k)
+  Group<RedArgs> r;
+  for (int i = 0; i < count; i++) r.v[i] = RedArgs{a[i].partial, a[i].out};
+  hipLaunchKernelGGL(k_reduce_strided, dim3((1u << sb) * K, 1, count), dim3(256), 0, st, r, nblocks, K);
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 
 // Result layout: out[(seg * K) + k].  `partial` needs nblocks * 2^sb * K entries.
-int eq_dot_base(const u32* f, int nf, int sb, int lo, const EqTables& tabs, int K, ef* partial, int nblocks, ef* out,
-                hipStream_t st) {
-  int rc = eq_dot_impl<u32>(f, nf, sb, lo, tabs, K, partial, nblocks, st);
-  if (rc) return rc;
-  // partial is [seg][block][k]; reduce over blocks for each (seg, k)
-  hipLaunchKernelGGL(k_reduce_strided, dim3((1u << sb) * K), dim3(256), 0, st, partial, nblocks, K, out);
-  return hipGetLastError() == hipSuccess ? 0 : -5;
+int eq_dot_base(const DotArgs* a, int count, int nf, int sb, int lo, int K, int nblocks, hipStream_t st) {
+  return eq_dot_impl<u32>(a, count, nf, sb, lo, K, nblocks, st);
 }
 
-int eq_dot_ext(const ef* f, int nf, int sb, int lo, const EqTables& tabs, int K, ef* partial, int nblocks, ef* out,
-               hipStream_t st) {
-  int rc = eq_dot_impl<ef>(f, nf, sb, lo, tabs, K, partial, nblocks, st);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_reduce_strided, dim3((1u << sb) * K), dim3(256), 0, st, partial, nblocks, K, out);
-  return hipGetLastError() == hipSuccess ? 0 : -5;
+int eq_dot_ext(const DotArgs* a, int count, int nf, int sb, int lo, int K, int nblocks, hipStream_t st) {
+  return eq_dot_impl<ef>(a, count, nf, sb, lo, K, nblocks, st);
 }
 
 // ---------------------------------------------------------------- weight accumulation
@@ -303,8 +318,11 @@ int eq_dot_ext(const ef* f, int nf, int sb, int lo, const EqTables& tabs, int K,
 // which fold32 keeps below 2^57 + 2^32 (4 p^2 + 2^58 < 2^64), and one
 // Montgomery reduction per component follows (its input is below 4 p^2).
 // The result is the canonical sum of the reduced products, as with scales.
-__global__ void __launch_bounds__(256) k_eq_combine(ef* __restrict__ Wt, u64 n, int lo, EqTables tabs, int K,
-                                                    int accumulate, const ef* __restrict__ scale) {
+__global__ void __launch_bounds__(256) k_eq_combine(Group<CombineArgs> g, u64 n, int lo, int K, int accumulate) {
+  const CombineArgs& a = g.v[blockIdx.z];
+  ef* __restrict__ Wt = a.Wt;
+  const EqTables& tabs = a.tabs;
+  const ef* __restrict__ scale = a.scale;
   const u32 mask = (1u << lo) - 1;
   for (u64 x = (u64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (u64)gridDim.x * blockDim.x) {
     ef acc = accumulate ? Wt[x] : ef_zero();
@@ -329,20 +347,23 @@ __global__ void __launch_bounds__(256) k_eq_combine(ef* __restrict__ Wt, u64 n, 
   }
 }
 
-int eq_combine(ef* Wt, u64 n, int lo, const EqTables& tabs, int K, int accumulate, hipStream_t st,
-               const ef* d_scale) {
+int eq_combine(const CombineArgs* a, int count, u64 n, int lo, int K, int accumulate, hipStream_t st) {
   u64 blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(k_eq_combine, dim3((unsigned)blocks), dim3(256), 0, st, Wt, n, lo, tabs, K, accumulate,
-                     d_scale);
-  return hipGetLastError() == hipSuccess ? 0 : -5;
+  return launch_group(a, count, [&](const Group<CombineArgs>& g) {
+    hipLaunchKernelGGL(k_eq_combine, dim3((unsigned)blocks, 1, count), dim3(256), 0, st, g, n, lo, K, accumulate);
+  });
 }
 
 // Select tables for z_j in F and weight g_j in EF:
 //   A[j][xh] = g_j * (z_j^(2^lo))^xh  (EF),   B[j][xl] = z_j^xl  (F)
-__global__ void k_select_tables(const u32* __restrict__ z, const ef* __restrict__ g, int nsel, const u32* nsel_dev,
-                                int hi, int lo, ef* __restrict__ A, u32* __restrict__ B) {
-  if (nsel_dev) nsel = (int)*nsel_dev;
+__global__ void k_select_tables(Group<SelectArgs> sg, int nsel, int hi, int lo) {
+  const SelectArgs& a = sg.v[blockIdx.z];
+  const u32* __restrict__ z = a.z;
+  const ef* __restrict__ g = a.g;
+  ef* __restrict__ A = a.A;
+  u32* __restrict__ B = a.B;
+  if (a.nsel_dev) nsel = (int)*a.nsel_dev;
   const u32 nh = 1u << hi, nl = 1u << lo;
   u64 total = (u64)nsel * (nh + nl);
   for (u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (u64)gridDim.x * blockDim.x) {
@@ -360,10 +381,12 @@ __global__ void k_select_tables(const u32* __restrict__ z, const ef* __restrict_
 // Wt[x] += sum_j A[j][x >> lo] * B[j][x & mask], each EF component's
 // products summed unreduced (fold32) three at a time.
 
-__global__ void __launch_bounds__(256) k_select_combine(ef* __restrict__ Wt, u64 n, int lo, int hi,
-                                                        const ef* __restrict__ A, const u32* __restrict__ B,
-                                                        int nsel, const u32* nsel_dev) {
-  if (nsel_dev) nsel = (int)*nsel_dev;
+__global__ void __launch_bounds__(256) k_select_combine(Group<SelectArgs> sg, u64 n, int lo, int hi, int nsel) {
+  const SelectArgs& sa = sg.v[blockIdx.z];
+  ef* __restrict__ Wt = sa.Wt;
+  const ef* __restrict__ A = sa.A;
+  const u32* __restrict__ B = sa.B;
+  if (sa.nsel_dev) nsel = (int)*sa.nsel_dev;
   const u32 mask = (1u << lo) - 1;
   const u64 nh = 1ull << hi, nl = 1ull << lo;
   for (u64 x = (u64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (u64)gridDim.x * blockDim.x) {
@@ -396,10 +419,12 @@ __global__ void __launch_bounds__(256) k_select_combine(ef* __restrict__ Wt, u64
 // loads, each used XT times, and a thread has 3 * XT loads of B in flight per
 // step.
 template <int XT>
-__global__ void __launch_bounds__(256) k_select_combine_rows(ef* __restrict__ Wt, u64 n, int lo, int hi,
-                                                             const ef* __restrict__ A, const u32* __restrict__ B,
-                                                             int nsel, const u32* nsel_dev) {
-  if (nsel_dev) nsel = (int)*nsel_dev;
+__global__ void __launch_bounds__(256) k_select_combine_rows(Group<SelectArgs> sg, u64 n, int lo, int hi, int nsel) {
+  const SelectArgs& sa = sg.v[blockIdx.z];
+  ef* __restrict__ Wt = sa.Wt;
+  const ef* __restrict__ A = sa.A;
+  const u32* __restrict__ B = sa.B;
+  if (sa.nsel_dev) nsel = (int)*sa.nsel_dev;
   const u32 mask = (1u << lo) - 1;
   const u64 nh = 1ull << hi, nl = 1ull << lo;
   for (u64 x0 = (u64)blockIdx.x * (256 * XT); x0 < n; x0 += (u64)gridDim.x * (256 * XT)) {
@@ -442,34 +467,36 @@ __global__ void __launch_bounds__(256) k_select_combine_rows(ef* __restrict__ Wt
   }
 }
 
-int select_combine(ef* Wt, int m, const u32* d_z, const ef* d_g, int nsel, const u32* d_nsel, ef* A, u32* B,
-                   hipStream_t st) {
+int select_combine(const SelectArgs* a, int count, int m, int nsel, hipStream_t st) {
   if (nsel == 0) return 0;
-  int lo = m / 2, hi = m - lo;
-  u64 tt = (u64)nsel * ((1ull << hi) + (1ull << lo));
+  const int lo = m / 2, hi = m - lo;
+  const u64 tt = (u64)nsel * ((1ull << hi) + (1ull << lo));
   u64 b1 = (tt + 255) / 256;
   if (b1 > 65536) b1 = 65536;
-  hipLaunchKernelGGL(k_select_tables, dim3((unsigned)b1), dim3(256), 0, st, d_z, d_g, nsel, d_nsel, hi, lo, A, B);
-  u64 n = 1ull << m;
+  const u64 n = 1ull << m;
   u64 blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  if (lo >= 10)
-    hipLaunchKernelGGL(k_select_combine_rows<4>, dim3((unsigned)((blocks + 3) / 4)), dim3(256), 0, st, Wt, n, lo, hi,
-                       A, B, nsel, d_nsel);
-  else if (lo >= 8)
-    hipLaunchKernelGGL(k_select_combine_rows<1>, dim3((unsigned)blocks), dim3(256), 0, st, Wt, n, lo, hi, A, B, nsel,
-                       d_nsel);
-  else
-    hipLaunchKernelGGL(k_select_combine, dim3((unsigned)blocks), dim3(256), 0, st, Wt, n, lo, hi, A, B, nsel, d_nsel);
-  return hipGetLastError() == hipSuccess ? 0 : -5;
+  return launch_group(a, count, [&](const Group<SelectArgs>& g) {
+    hipLaunchKernelGGL(k_select_tables, dim3((unsigned)b1, 1, count), dim3(256), 0, st, g, nsel, hi, lo);
+    if (lo >= 10)
+      hipLaunchKernelGGL(k_select_combine_rows<4>, dim3((unsigned)((blocks + 3) / 4), 1, count), dim3(256), 0, st, g,
+                         n, lo, hi, nsel);
+    else if (lo >= 8)
+      hipLaunchKernelGGL(k_select_combine_rows<1>, dim3((unsigned)blocks, 1, count), dim3(256), 0, st, g, n, lo, hi,
+                         nsel);
+    else
+      hipLaunchKernelGGL(k_select_combine, dim3((unsigned)blocks, 1, count), dim3(256), 0, st, g, n, lo, hi, nsel);
+  });
 }
 
 // ---------------------------------------------------------------- folds
 // out[x] = sum_s eqrs[s] * f[s * 2^nr + x], s < 2^l
 // (products summed unreduced per component, folded every four).  f is read
 // once: nontemporal loads.
-__global__ void __launch_bounds__(256) k_compress_multi(const u32* __restrict__ f, int nr, int l,
-                                                        const ef* __restrict__ eqrs, ef* __restrict__ out) {
+__global__ void __launch_bounds__(256) k_compress_multi(Group<CompressArgs> g, int nr, int l) {
+  const u32* __restrict__ f = g.v[blockIdx.z].f;
+  const ef* __restrict__ eqrs = g.v[blockIdx.z].eqrs;
+  ef* __restrict__ out = g.v[blockIdx.z].out;
   const u64 n = 1ull << nr;
   const int ns = 1 << l;
   for (u64 x = (u64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (u64)gridDim.x * blockDim.x) {
@@ -494,32 +521,39 @@ __global__ void __launch_bounds__(256) k_compress_multi(const u32* __restrict__ 
   }
 }
 
-int compress_multi(const u32* f, int n, int l, const ef* d_eqrs, ef* out, hipStream_t st) {
+int compress_multi(const CompressArgs* a, int count, int n, int l, hipStream_t st) {
   u64 m = 1ull << (n - l);
   u64 blocks = (m + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(k_compress_multi, dim3((unsigned)blocks), dim3(256), 0, st, f, n - l, l, d_eqrs, out);
-  return hipGetLastError() == hipSuccess ? 0 : -5;
+  return launch_group(a, count, [&](const Group<CompressArgs>& g) {
+    hipLaunchKernelGGL(k_compress_multi, dim3((unsigned)blocks, 1, count), dim3(256), 0, st, g, n - l, l);
+  });
 }
 
-__global__ void k_base_to_ext(const u32* __restrict__ f, u64 n, ef* __restrict__ out) {
+__global__ void k_base_to_ext(Group<ExtArgs> g, u64 n) {
+  const u32* __restrict__ f = g.v[blockIdx.z].f;
+  ef* __restrict__ out = g.v[blockIdx.z].out;
   for (u64 x = (u64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (u64)gridDim.x * blockDim.x)
     out[x] = ef_base(f[x]);
 }
 
-int base_to_ext(const u32* f, u64 n, ef* out, hipStream_t st) {
+int base_to_ext(const ExtArgs* a, int count, u64 n, hipStream_t st) {
   u64 blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(k_base_to_ext, dim3((unsigned)blocks), dim3(256), 0, st, f, n, out);
-  return hipGetLastError() == hipSuccess ? 0 : -5;
+  return launch_group(a, count, [&](const Group<ExtArgs>& g) {
+    hipLaunchKernelGGL(k_base_to_ext, dim3((unsigned)blocks, 1, count), dim3(256), 0, st, g, n);
+  });
 }
 
 // ---------------------------------------------------------------- sumcheck rounds
 // (h(0), h(2)) partial sums over pairs (i, i + h) of e and w (length 2h).
-__global__ void __launch_bounds__(256) k_sc_coeffs(const ef* __restrict__ e, const ef* __restrict__ w, u64 h,
-                                                   ef* __restrict__ partial, u32* grind_reset) {
+__global__ void __launch_bounds__(256) k_sc_coeffs(Group<ScArgs> g, u64 h) {
   __shared__ ef sh[512];
-  if (grind_reset && blockIdx.x == 0 && threadIdx.x == 0) *grind_reset = P;
+  const ScArgs& a = g.v[blockIdx.z];
+  const ef* __restrict__ e = a.e;
+  const ef* __restrict__ w = a.w;
+  ef* __restrict__ partial = a.partial;
+  if (a.grind_reset && blockIdx.x == 0 && threadIdx.x == 0) *a.grind_reset = P;
   ef acc[2] = {ef_zero(), ef_zero()};
   for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < h; i += (u64)gridDim.x * blockDim.x) {
     ef e0 = e[i], e1 = e[i + h], w0 = w[i], w1 = w[i + h];
@@ -536,13 +570,16 @@ __global__ void __launch_bounds__(256) k_sc_coeffs(const ef* __restrict__ e, con
 // Fold e and w (length 2h) with r into their first halves, and accumulate the
 // next round's (h(0), h(2)) over the folded halves (pairs (j, j + h/2)).
 // With a transcript mode, every block first runs that transcript step to get r.
-__global__ void __launch_bounds__(256) k_sc_fold_coeffs(ef* __restrict__ e, ef* __restrict__ w, u64 h, ef r,
-                                                        int tr_mode, ScTr tr, const P2C16* __restrict__ cp,
-                                                        ef* __restrict__ partial) {
+__global__ void __launch_bounds__(256) k_sc_fold_coeffs(Group<ScArgs> g, u64 h, int tr_mode,
+                                                        const P2C16* __restrict__ cp) {
   __shared__ ef sh[512];
+  const ScArgs& a = g.v[blockIdx.z];
+  ef* __restrict__ e = a.e;
+  ef* __restrict__ w = a.w;
+  ef r = a.r;
   if (tr_mode != TR_NONE) {
     __shared__ ef rs;
-    tr_prologue(tr_mode, tr, *cp, nullptr, &rs);
+    tr_prologue(tr_mode, a.tr, *cp, nullptr, &rs);
     r = rs;
   }
   ef acc[2] = {ef_zero(), ef_zero()};
@@ -563,16 +600,19 @@ __global__ void __launch_bounds__(256) k_sc_fold_coeffs(ef* __restrict__ e, ef* 
   }
   block_reduce_ef<2>(acc, sh);
   if (threadIdx.x == 0) {
-    partial[2 * blockIdx.x] = acc[0];
-    partial[2 * blockIdx.x + 1] = acc[1];
+    a.partial[2 * blockIdx.x] = acc[0];
+    a.partial[2 * blockIdx.x + 1] = acc[1];
   }
 }
 
-__global__ void __launch_bounds__(256) k_sc_fold(ef* __restrict__ e, ef* __restrict__ w, u64 h, ef r, int tr_mode,
-                                                 ScTr tr, const P2C16* __restrict__ cp) {
+__global__ void __launch_bounds__(256) k_sc_fold(Group<ScArgs> g, u64 h, int tr_mode, const P2C16* __restrict__ cp) {
+  const ScArgs& a = g.v[blockIdx.z];
+  ef* __restrict__ e = a.e;
+  ef* __restrict__ w = a.w;
+  ef r = a.r;
   if (tr_mode != TR_NONE) {
     __shared__ ef rs;
-    tr_prologue(tr_mode, tr, *cp, nullptr, &rs);
+    tr_prologue(tr_mode, a.tr, *cp, nullptr, &rs);
     r = rs;
   }
   for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < h; j += (u64)gridDim.x * blockDim.x) {
@@ -589,45 +629,56 @@ static unsigned sc_blocks(u64 work) {
   return b ? (unsigned)b : 1u;
 }
 
+static_assert(sizeof(Group<ScArgs>) <= 4096, "kernel arguments of a sumcheck group");
+
+// The nb block partials of each proof reduced into its c0c2 (two values).
+static void reduce_pairs(const ScArgs* a, int count, unsigned nb, hipStream_t st) {
+  Group<RedArgs> r;
+  for (int i = 0; i < count; i++) r.v[i] = RedArgs{a[i].partial, a[i].c0c2};
+  hipLaunchKernelGGL(k_reduce_strided, dim3(2, 1, count), dim3(256), 0, st, r, (int)nb, 2);
+}
+
 // c0c2[0..2] = (h(0), h(2)) for arrays of length 2h.  Uses partial[2 * 1024].
-int sc_coeffs(const ef* e, const ef* w, u64 h, ef* partial, ef* c0c2, hipStream_t st) {
-  unsigned nb = sc_blocks(h);
-  hipLaunchKernelGGL(k_sc_coeffs, dim3(nb), dim3(256), 0, st, e, w, h, partial, (u32*)nullptr);
-  hipLaunchKernelGGL(k_reduce_strided, dim3(2), dim3(256), 0, st, partial, (int)nb, 2, c0c2);
-  return hipGetLastError() == hipSuccess ? 0 : -5;
+int sc_coeffs(const ScArgs* a, int count, u64 h, hipStream_t st) {
+  const unsigned nb = sc_blocks(h);
+  return launch_group(a, count, [&](const Group<ScArgs>& g) {
+    hipLaunchKernelGGL(k_sc_coeffs, dim3(nb, 1, count), dim3(256), 0, st, g, h);
+    reduce_pairs(a, count, nb, st);
+  });
 }
 
 // Fold arrays of length 2h with r; if h >= 2 also produce the next (h(0), h(2)).
-int sc_fold(ef* e, ef* w, u64 h, const ef& r, ef* partial, ef* c0c2, hipStream_t st) {
-  if (h >= 2) {
-    unsigned nb = sc_blocks(h >> 1);
-    hipLaunchKernelGGL(k_sc_fold_coeffs, dim3(nb), dim3(256), 0, st, e, w, h, r, (int)TR_NONE, ScTr{},
-                       (const P2C16*)nullptr, partial);
-    hipLaunchKernelGGL(k_reduce_strided, dim3(2), dim3(256), 0, st, partial, (int)nb, 2, c0c2);
-  } else {
-    hipLaunchKernelGGL(k_sc_fold, dim3(sc_blocks(h)), dim3(256), 0, st, e, w, h, r, (int)TR_NONE, ScTr{},
-                       (const P2C16*)nullptr);
-  }
-  return hipGetLastError() == hipSuccess ? 0 : -5;
+int sc_fold(const ScArgs* a, int count, u64 h, hipStream_t st) {
+  return launch_group(a, count, [&](const Group<ScArgs>& g) {
+    if (h >= 2) {
+      const unsigned nb = sc_blocks(h >> 1);
+      hipLaunchKernelGGL(k_sc_fold_coeffs, dim3(nb, 1, count), dim3(256), 0, st, g, h, (int)TR_NONE,
+                         (const P2C16*)nullptr);
+      reduce_pairs(a, count, nb, st);
+    } else {
+      hipLaunchKernelGGL(k_sc_fold, dim3(sc_blocks(h), 1, count), dim3(256), 0, st, g, h, (int)TR_NONE,
+                         (const P2C16*)nullptr);
+    }
+  });
 }
 
-int sc_partials(const ef* e, const ef* w, u64 h, ef* partial, unsigned* nb, u32* grind_reset, hipStream_t st) {
+int sc_partials(const ScArgs* a, int count, u64 h, unsigned* nb, hipStream_t st) {
   *nb = sc_blocks(h);
-  hipLaunchKernelGGL(k_sc_coeffs, dim3(*nb), dim3(256), 0, st, e, w, h, partial, grind_reset);
-  return hipGetLastError() == hipSuccess ? 0 : -5;
+  const unsigned b = *nb;
+  return launch_group(a, count, [&](const Group<ScArgs>& g) {
+    hipLaunchKernelGGL(k_sc_coeffs, dim3(b, 1, count), dim3(256), 0, st, g, h);
+  });
 }
 
-int sc_fold_tr(ef* e, ef* w, u64 h, int mode, const ScTr& tr, const P2C16* d_consts, ef* partial_out, unsigned* nb,
-               hipStream_t st) {
-  if (h >= 2) {
-    *nb = sc_blocks(h >> 1);
-    hipLaunchKernelGGL(k_sc_fold_coeffs, dim3(*nb), dim3(256), 0, st, e, w, h, ef_zero(), mode, tr, d_consts,
-                       partial_out);
-  } else {
-    *nb = 0;
-    hipLaunchKernelGGL(k_sc_fold, dim3(sc_blocks(h)), dim3(256), 0, st, e, w, h, ef_zero(), mode, tr, d_consts);
-  }
-  return hipGetLastError() == hipSuccess ? 0 : -5;
+int sc_fold_tr(const ScArgs* a, int count, u64 h, int mode, const P2C16* d_consts, unsigned* nb, hipStream_t st) {
+  *nb = h >= 2 ? sc_blocks(h >> 1) : 0;
+  const unsigned b = *nb;
+  return launch_group(a, count, [&](const Group<ScArgs>& g) {
+    if (h >= 2)
+      hipLaunchKernelGGL(k_sc_fold_coeffs, dim3(b, 1, count), dim3(256), 0, st, g, h, mode, d_consts);
+    else
+      hipLaunchKernelGGL(k_sc_fold, dim3(sc_blocks(h), 1, count), dim3(256), 0, st, g, h, mode, d_consts);
+  });
 }
 
 }  // namespace kb

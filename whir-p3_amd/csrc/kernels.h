@@ -110,82 +110,195 @@ struct EqTables {
   const ef* lo[4];
 };
 
+// ---- groups: one launch serving several proofs of one shape
+// Every prover kernel takes the per-proof part of its arguments as a Group
+// passed by value; blockIdx.z (folded with the coset index in LDE pass 1)
+// selects the proof, so a launch for `count` proofs of the same shape runs
+// count times the grid of one.  A single proof is a group of one.
+constexpr int GROUP_MAX = 16;
+template <class A>
+struct Group {
+  A v[GROUP_MAX];
+};
+
+// State a host-driven grind's candidates are observed into (see GrindBase).
+struct Sponge16 {
+  u32 s[16];
+};
+
 // ---- Reed-Solomon encoding (k_lde.hip)
-// src holds 2^m elements; the output is 2^(m-k+R) rows of 2^k elements.
+// src holds 2^m elements; the output (dst) is 2^(m-k+R) rows of 2^k elements.
 // tmp (same size as the output) is needed when 2^(m-k) exceeds one LDS pass.
+template <class T>
+struct LdeArgs {
+  const T* src;
+  T* tmp;
+  T* dst;
+};
 bool lde_two_pass(int m, int k, bool ext);
 // 0 when lde_base / lde_ext support the shape, else the negative code they would return.
 int lde_check(int m, int k, int R, bool ext);
-int lde_base(const u32* src, u32* dst, u32* tmp, int m, int k, int R, const Twiddles& tw, hipStream_t st);
-int lde_ext(const ef* src, ef* dst, ef* tmp, int m, int k, int R, const Twiddles& tw, hipStream_t st);
+int lde_base(const LdeArgs<u32>* a, int count, int m, int k, int R, const Twiddles& tw, hipStream_t st);
+int lde_ext(const LdeArgs<ef>* a, int count, int m, int k, int R, const Twiddles& tw, hipStream_t st);
 
 // ---- Merkle tree and openings (k_merkle.hip)
-// tree holds 2 * height - 1 digests of 8 u32; the root is the last one.
-int merkle_leaves(const u32* rows, int row_len, u64 height, int leaf_width, const P2C16* d_c16, const P2C24* d_c24,
-                  u32* tree, hipStream_t st);
-// Writes the root to root_out too when given; returns 1 (root not written) for a single leaf.
-int merkle_tree(u64 height, const P2C16& c16, u32* tree, hipStream_t st, u32* root_out);
-int merkle_commit(const u32* rows, int row_len, u64 height, int leaf_width, const P2C16& c16, const P2C16* d_c16,
-                  const P2C24* d_c24, u32* tree, hipStream_t st);
-int merkle_open(const u32* rows, int row_len, const u32* tree, u64 height, const u64* d_idx, int nq, u32* d_out,
-                hipStream_t st);
-// Scans [start, start + count) for the smallest witness.  *d_result must
-// hold p before the first range; it receives the smallest witness found.
-int grind_range(const P2C16* d_consts, const u32* state16, int wpos, int bits, u32* d_result, u64 start, u64 count,
+// tree holds 2 * height - 1 digests of 8 u32; the root is the last one and
+// is also written to root_out (host-visible memory) when given.
+struct TreeArgs {
+  const u32* rows;
+  u32* tree;
+  u32* root_out;
+};
+int merkle_leaves(const TreeArgs* a, int count, int row_len, u64 height, int leaf_width, const P2C16* d_c16,
+                  const P2C24* d_c24, hipStream_t st);
+// Returns 1 (root not written to root_out) for a single leaf.
+int merkle_tree(const TreeArgs* a, int count, u64 height, const P2C16& c16, hipStream_t st);
+// Opening of the rows at idx[0 .. nq) (nq from device memory when given,
+// else max_nq): each row followed by its sibling digests, into out.
+struct OpenArgs {
+  const u32* rows;
+  const u32* tree;
+  const u64* idx;
+  const u32* nq;
+  u32* out;
+};
+int merkle_open(const OpenArgs* a, int count, int row_len, u64 height, int max_nq, hipStream_t st);
+
+// Grinding: the smallest witness in [start, start + span) (clipped to p),
+// into *result, which must hold p (or a larger candidate) before the first
+// range.  mode TR_NONE: the candidates' base state and position are base /
+// wpos; otherwise every block first runs the transcript step `mode` of tr
+// (TR_COEFFS_GRIND or TR_OBS_GRIND) and result is tr.grind.
+struct GrindArgs {
+  Sponge16 base;
+  int wpos;
+  u32* result;
+  ScTr tr;
+};
+int grind_range(const P2C16* d_consts, const GrindArgs* a, int count, int bits, int mode, u64 start, u64 span,
                 int blocks, hipStream_t st);
-// Same, with the base state and candidate position from the transcript step
-// TR_COEFFS_GRIND run by every block first; the result goes to tr.grind.
-int grind_range_tr(const P2C16* d_consts, const ScTr& tr, int bits, u64 start, u64 count, int blocks, hipStream_t st,
-                   int mode = TR_COEFFS_GRIND);
 
 // ---- a round's transcript steps (k_merkle.hip)
 // OOD sample: observe st.obs, sample the challenge y into *log_y, expand it to
 // the m-variable point pt_out (y^(2^(m-1-j)) at j), and write its split eq
 // tables (hi + lo = m variables).  Block 0 also sets *grind_reset to p when
 // given (the round's grind result slot).
-int ood_eq_tables(const StepTr& st, ef* log_y, ef* pt_out, int m, int hi, int lo, ef* out_hi, ef* out_lo,
-                  const P2C16* d_consts, hipStream_t st_, u32* grind_reset = nullptr);
-// One block: the round's witness from `grind` (pow_bits > 0) or st.obs, one
-// discarded sample, the STIR query indices over the folded domain (sorted,
-// distinct), their points folded_gen^idx, gamma and its powers (n_ood + nq of
-// them).  Log and buffers as in RoundLog / RoundBufs.  final_step: the final
-// round's queries (no discarded sample, no gamma).
-int round_queries(const StepTr& st, const u32* grind, int pow_bits, u64 domain_size, int folding_factor,
-                  int num_queries, u32 folded_gen, int n_ood, RoundLog* log, const RoundBufs& rb,
-                  const P2C16* d_consts, hipStream_t st_, bool final_step = false);
-// merkle_open with the query indices and their count in device memory
-// (at most max_nq).
-int merkle_open_dev(const u32* rows, int row_len, const u32* tree, u64 height, const u64* d_idx, const u32* d_nq,
-                    int max_nq, u32* d_out, hipStream_t st);
+struct OodArgs {
+  StepTr st;
+  ef* log_y;
+  ef* pt_out;
+  ef* out_hi;
+  ef* out_lo;
+  u32* grind_reset;
+};
+int ood_eq_tables(const OodArgs* a, int count, int m, int hi, int lo, const P2C16* d_consts, hipStream_t st);
+// One block per proof: the round's witness from `grind` (pow_bits > 0) or
+// st.obs, one discarded sample, the STIR query indices over the folded domain
+// (sorted, distinct), their points folded_gen^idx, gamma and its powers
+// (n_ood + nq of them).  Log and buffers as in RoundLog / RoundBufs.
+// final_step: the final round's queries (no discarded sample, no gamma).
+struct QueryArgs {
+  StepTr st;
+  const u32* grind;
+  RoundLog* log;
+  RoundBufs rb;
+};
+int round_queries(const QueryArgs* a, int count, int pow_bits, u64 domain_size, int folding_factor, int num_queries,
+                  u32 folded_gen, int n_ood, const P2C16* d_consts, hipStream_t st, bool final_step = false);
 
 // ---- multilinear kernels (k_poly.hip)
 int eq_table(const PointArg& z, int first, int t, const ef& scale, ef* out, hipStream_t st);
-// Both halves of a split table in one launch: out_hi = scale * eq(z[0..hi)), out_lo = eq(z[hi..hi+lo)).
-int eq_split(const PointArg& z, int hi, int lo, const ef& scale, ef* out_hi, ef* out_lo, hipStream_t st);
 int eq_outer(const ef* Ehi, const ef* Elo, int lo, u64 n, ef* out, hipStream_t st);
-int eq_dot_base(const u32* f, int nf, int sb, int lo, const EqTables& tabs, int K, ef* partial, int nblocks, ef* out,
-                hipStream_t st);
-int eq_dot_ext(const ef* f, int nf, int sb, int lo, const EqTables& tabs, int K, ef* partial, int nblocks, ef* out,
-               hipStream_t st);
-// Term k is scaled by d_scale[k] (device memory) when d_scale is given.
-int eq_combine(ef* Wt, u64 n, int lo, const EqTables& tabs, int K, int accumulate, hipStream_t st,
-               const ef* d_scale = nullptr);
-// With d_nsel given, the count is read from device memory and nsel is its
-// maximum (A and B sized for it).
-int select_combine(ef* Wt, int m, const u32* d_z, const ef* d_g, int nsel, const u32* d_nsel, ef* A, u32* B,
-                   hipStream_t st);
-int compress_multi(const u32* f, int n, int l, const ef* d_eqrs, ef* out, hipStream_t st);
-int base_to_ext(const u32* f, u64 n, ef* out, hipStream_t st);
-int sc_coeffs(const ef* e, const ef* w, u64 h, ef* partial, ef* c0c2, hipStream_t st);
-int sc_fold(ef* e, ef* w, u64 h, const ef& r, ef* partial, ef* c0c2, hipStream_t st);
-// Device-transcript variants.  sc_partials: coefficient partial sums only
-// (*nb blocks, two values each in `partial`); it also sets *grind_reset to p
-// (the first round's grind result) when given.  sc_fold_tr: the transcript
-// step `mode` (TR_WITNESS_SAMPLE or TR_COEFFS_SAMPLE) gives r, then the fold by
-// r, leaving the next round's partials in `partial_out` when h >= 2 (*nb = 0
-// otherwise).
-int sc_partials(const ef* e, const ef* w, u64 h, ef* partial, unsigned* nb, u32* grind_reset, hipStream_t st);
-int sc_fold_tr(ef* e, ef* w, u64 h, int mode, const ScTr& tr, const P2C16* d_consts, ef* partial_out, unsigned* nb,
-               hipStream_t st);
+// Both halves of a split table: out_hi = scale * eq(z[0..hi)), out_lo =
+// eq(z[hi..hi+lo)), the point's hi + lo coordinates read from z (device or
+// host-visible memory).
+struct EqSplitArgs {
+  const ef* z;
+  ef scale;
+  ef* out_hi;
+  ef* out_lo;
+};
+int eq_split(const EqSplitArgs* a, int count, int hi, int lo, hipStream_t st);
+// Segmented eq-weighted sums of f (2^nf entries, 2^sb segments) against K
+// split tables; partial holds nblocks * 2^sb * K block sums, and out receives
+// out[seg * K + k].
+struct DotArgs {
+  const void* f;
+  EqTables tabs;
+  ef* partial;
+  ef* out;
+};
+int eq_dot_base(const DotArgs* a, int count, int nf, int sb, int lo, int K, int nblocks, hipStream_t st);
+int eq_dot_ext(const DotArgs* a, int count, int nf, int sb, int lo, int K, int nblocks, hipStream_t st);
+// Wt[x] = (accumulate ? Wt[x] : 0) + sum_k s_k eq_k(x); s_k = scale[k]
+// (device memory) when scale is given, else 1.
+struct CombineArgs {
+  ef* Wt;
+  EqTables tabs;
+  const ef* scale;
+};
+int eq_combine(const CombineArgs* a, int count, u64 n, int lo, int K, int accumulate, hipStream_t st);
+// Select terms sum_j g_j z_j^x added to Wt; the count comes from nsel_dev when
+// given (nsel is then its maximum, A and B sized for it).
+struct SelectArgs {
+  ef* Wt;
+  const u32* z;
+  const ef* g;
+  const u32* nsel_dev;
+  ef* A;
+  u32* B;
+};
+int select_combine(const SelectArgs* a, int count, int m, int nsel, hipStream_t st);
+struct CompressArgs {
+  const u32* f;
+  const ef* eqrs;
+  ef* out;
+};
+int compress_multi(const CompressArgs* a, int count, int n, int l, hipStream_t st);
+struct ExtArgs {
+  const u32* f;
+  ef* out;
+};
+int base_to_ext(const ExtArgs* a, int count, u64 n, hipStream_t st);
+// A sumcheck round on arrays e, w of length 2h.  sc_coeffs: (h(0), h(2))
+// into c0c2.  sc_fold: fold by r, and when h >= 2 the next round's (h(0),
+// h(2)) into c0c2.  Both use partial (2 * 1024 entries) as scratch.
+// Device-transcript variants: sc_partials leaves the coefficient partial sums
+// of *nb blocks in `partial` (and sets *grind_reset to p when given);
+// sc_fold_tr runs the transcript step `mode` (TR_WITNESS_SAMPLE or
+// TR_COEFFS_SAMPLE, reading tr.partial) to get r, folds, and leaves the next
+// round's partials in `partial` when h >= 2 (*nb = 0 otherwise).
+struct ScArgs {
+  ef* e;
+  ef* w;
+  ef r;
+  ef* partial;
+  ef* c0c2;
+  u32* grind_reset;
+  ScTr tr;
+};
+int sc_coeffs(const ScArgs* a, int count, u64 h, hipStream_t st);
+int sc_fold(const ScArgs* a, int count, u64 h, hipStream_t st);
+int sc_partials(const ScArgs* a, int count, u64 h, unsigned* nb, hipStream_t st);
+int sc_fold_tr(const ScArgs* a, int count, u64 h, int mode, const P2C16* d_consts, unsigned* nb, hipStream_t st);
+
+// Kernel arguments stay within 4 KiB for a full group.
+static_assert(sizeof(Group<GrindArgs>) <= 4096 && sizeof(Group<ScArgs>) <= 4096 &&
+                  sizeof(Group<QueryArgs>) <= 4096 && sizeof(Group<OodArgs>) <= 4096 &&
+                  sizeof(Group<DotArgs>) <= 4096 && sizeof(Group<CombineArgs>) <= 4096 &&
+                  sizeof(Group<SelectArgs>) <= 4096 && sizeof(Group<EqSplitArgs>) <= 4096 &&
+                  sizeof(Group<OpenArgs>) <= 4096 && sizeof(Group<TreeArgs>) <= 4096 &&
+                  sizeof(Group<CompressArgs>) <= 4096 && sizeof(Group<LdeArgs<ef>>) <= 4096,
+              "a group's kernel arguments exceed 4 KiB");
+
+// Launches kern with the first `count` entries of `a` in a Group.
+template <class A, class F>
+int launch_group(const A* a, int count, F&& launch) {
+  if (count < 1 || count > GROUP_MAX) return -8;
+  Group<A> g;
+  for (int i = 0; i < count; i++) g.v[i] = a[i];
+  launch(g);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
 
 }  // namespace kb

@@ -1,15 +1,23 @@
 // Device-resident WHIR commit/prove behind the C ABI of include/whir_gpu.h.
 //
-// The host runs the transcript (DuplexChallenger), the parameter schedule and
-// the O(2^folding_factor)-sized steps; every pass over a polynomial, codeword
-// or Merkle tree runs on the device and stays resident in HBM between steps.
-// Per sumcheck round only the two round-polynomial values (32 bytes) come
-// back; the challenge goes down as a kernel argument.
+// The host runs the parameter schedule and replays the transcript
+// (DuplexChallenger); every pass over a polynomial, codeword or Merkle tree,
+// and the transcript steps between them, run on the device and stay resident
+// in HBM between steps.
+//
+// Proofs run in groups: a group is up to GROUP_MAX proofs of one
+// configuration (same number of variables and parameters, each with its own
+// polynomial, constraints and challenger) that go through the protocol in
+// lockstep, every device step one launch for all of them (the proof index is
+// the grid's z dimension).  The host side loops over the group's proofs for
+// what is per proof (transcript replay, openings, proof objects).  A single
+// proof (whirgpu_commit / whirgpu_prove) is a group of one.
 #include <chrono>
 #include <functional>
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -123,6 +131,53 @@ static ef* scratch_on_current_device() {
   return g_scratch_ptr[dev];
 }
 
+
+// One proof's share of a context: reduction scratch, eq tables, the device
+// transcript's states and logs, and the host-visible buffers kernels read or
+// write directly (small results, query indices, openings, roots, staged
+// points).  A context holds one slot per proof of the largest group it has
+// run; a single proof uses slot 0.
+struct Slot {
+  static constexpr int SC_LOG_CAP = 64;
+  static constexpr size_t SMALL_CAP = 4096;   // extension elements of h_small
+  static constexpr size_t IDX_CAP = 4096;
+  void* d_mem = nullptr;   // device memory carved into the buffers below
+  void* h_mem = nullptr;   // coherent host memory carved into the host-visible buffers
+  size_t d_bytes = 0, h_bytes = 0;
+  ef* d_partial = nullptr;   // reduction scratch: partial_cap extension elements
+  ef* d_tabs = nullptr;      // eq tables: 8 slots of tab_cap (own allocation, may grow)
+  size_t tab_cap = 0;
+  // Small reduction results are written by the kernels straight into
+  // coherent host memory (h_small; d_small is its device address) and read
+  // after a stream synchronization, so no copy is enqueued for them.
+  ef* h_small = nullptr, *d_small = nullptr;
+  // Query indices in, opened rows out; each holds one call's data and is
+  // reused only after the stream has been synchronized.
+  u64* h_idx = nullptr, *d_idx = nullptr;
+  u32* h_open = nullptr, *d_open = nullptr;
+  // Eight root slots, used in turn, so the next round's commitment can be
+  // enqueued before this round's root is read.
+  u32* h_root = nullptr, *d_root = nullptr;
+  int root_next = 0;
+  // Device transcript of sumcheck rounds: two challenger states a launch reads
+  // one of and writes the other, the grind results (two, 256 bytes apart),
+  // the phase's starting challenger (coherent) and the round log (coherent).
+  DevChallenger* d_chal = nullptr;
+  u32* d_grind = nullptr;
+  DevChallenger* h_chal = nullptr, *d_chal_init = nullptr;
+  ScLog* h_sclog = nullptr, *d_sclog = nullptr;
+  // Device transcript of a round's steps: its log (a WHIR round's, then the
+  // final round's), two challenger states, a long grind's base state and the
+  // round buffers.
+  RoundLog* h_rlog = nullptr, *d_rlog = nullptr;
+  DevChallenger* d_rchal = nullptr;
+  GrindBase* d_gbase = nullptr;
+  RoundBufs rb{};
+  // Pinned host copy of the final polynomial, taken before the final sumcheck folds it.
+  static constexpr int FINAL_MAX_LOG = 6;
+  ef* h_final = nullptr;
+};
+
 struct whirgpu_ctx {
   int device = 0;
   hipStream_t st = nullptr;
@@ -133,65 +188,25 @@ struct whirgpu_ctx {
   u32* d_tw = nullptr;   // 2 * 4096 twiddles
   Twiddles tw{};
   Pool pool;
-  size_t fixed_bytes = 0;   // device bytes of the buffers allocated at creation (not pooled)
-  void* h_pin = nullptr;
-  size_t h_pin_bytes = 0;
-  ef* d_partial = nullptr;   // reduction scratch
+  size_t fixed_bytes = 0;   // device bytes of the buffers allocated outside the pool
+  size_t tab_cap = 0;       // eq-table slot size for points of the context's size
   size_t partial_cap = 0;
-  // Small reduction results are written by the kernels straight into
-  // coherent pinned host memory (h_small; d_small is its device address) and
-  // read after a stream synchronization, so no copy is enqueued for them.
-  ef* d_small = nullptr;
-  ef* h_small = nullptr;
-  void read_small(ef* dst, size_t n) {
-    sync();
-    memcpy(dst, h_small, n * sizeof(ef));
-  }
-  ef* d_tabs = nullptr;      // eq tables: 8 slots of tab_cap
-  size_t tab_cap = 0;
-  // Host-visible coherent memory that kernels read or write directly (no copy
-  // kernels): query indices in, opened rows and Merkle roots out.  Each holds
-  // one call's data and is reused only after the stream has been synchronized.
-  u64* h_idx = nullptr;
-  u64* d_idx = nullptr;   // device address of h_idx
-  u32* h_open = nullptr;
-  u32* d_open = nullptr;  // device address of h_open
   size_t open_cap = 0;
-  // Eight root slots, used in turn, so the next round's commitment can be
-  // enqueued before this round's root is read.
-  u32* h_root = nullptr;
-  u32* d_root = nullptr;  // device address of h_root
-  int root_next = 0;
+  std::vector<std::unique_ptr<Slot>> slots;
+  // Host-driven grinds: result slots [GRIND_SLOTS][GROUP_MAX] on the device,
+  // refilled with p in bulk, and a pinned host copy of one row.
+  static constexpr int GRIND_SLOTS = 64;
+  u32* d_hgrind = nullptr;
+  u32* h_hgrind = nullptr;
+  int gslot = GRIND_SLOTS;
+  P2C16* d_c16 = nullptr;     // device copies of the constants (kernels that loop over
+  P2C24* d_c24 = nullptr;     // permutations read them from memory)
   // Recorded after a round's transcript steps and sumcheck: the host replays
   // them once it has passed while the next commitment runs.
   hipEvent_t ev_round = nullptr;
-  // Grinding result slots for host-driven grinds, refilled with p in bulk.
-  static constexpr int GRIND_SLOTS = 64;
-  u32* d_gslots = nullptr;
-  int gslot = GRIND_SLOTS;
-  u32* d_grind = nullptr;
-  P2C16* d_c16 = nullptr;     // device copies of the constants (kernels that loop over
-  P2C24* d_c24 = nullptr;     // permutations read them from memory)
-  // device transcript of sumcheck rounds: challenger, grind base, round log, challenge
-  static constexpr int SC_LOG_CAP = 64;
-  DevChallenger* d_chal = nullptr;   // two states: a launch reads one and writes the other
-  DevChallenger* h_chal = nullptr;   // coherent host memory: a phase's starting challenger
-  DevChallenger* d_chal_init = nullptr;
-  ScLog* h_sclog = nullptr;   // coherent host memory; the transcript kernel logs into it
-  ScLog* d_sclog = nullptr;   // device address of h_sclog
-  // device transcript of a round's steps before its sumcheck
-  RoundLog* h_rlog = nullptr;   // coherent host memory: a WHIR round's log, then the final round's
-  RoundLog* d_rlog = nullptr;   // device address of h_rlog
-  // pinned host copy of the final polynomial, taken before the final sumcheck folds it
-  static constexpr int FINAL_MAX_LOG = 6;
-  ef* h_final = nullptr;
-  DevChallenger* d_rchal = nullptr;   // two states the round's steps alternate between
-  GrindBase* d_gbase = nullptr;       // base state of a long grind's candidates
-  RoundBufs rb{};
-  void* d_rbuf = nullptr;       // backing memory of rb
   int grind_blocks = 256;
   double timings[7] = {0};
-  // live kernel timing: HIP events recorded on `st` around each launch
+  // live kernel timing: HIP events recorded on `st` around each launch group
   bool prof = false;
   struct Rec {
     int stage;
@@ -223,6 +238,8 @@ struct whirgpu_ctx {
     b.p = nullptr;
     b.n = 0;
   }
+  void* h_pin = nullptr;
+  size_t h_pin_bytes = 0;
   template <typename T>
   T* pin(size_t n) {
     size_t bytes = n * sizeof(T);
@@ -244,29 +261,63 @@ struct whirgpu_ctx {
     memcpy(dst, p, n * sizeof(T));
   }
   // Small host->device uploads are staged in a pinned ring and not waited
-  // for: a slot is reused only after a stream synchronization, by which time
-  // every earlier copy has completed.
+  // for: a part of the ring is reused only after a stream synchronization, by
+  // which time every earlier copy has completed.  stage() hands out space in
+  // the ring and the matching space of a device mirror, so several uploads
+  // go down as one copy (flush()).
   unsigned char* h_ring = nullptr;
-  size_t ring_cap = 0, ring_pos = 0;
+  unsigned char* d_ring = nullptr;   // device mirror of h_ring
+  size_t ring_cap = 0, ring_pos = 0, ring_flushed = 0;
+  // Points of eq tables (32 coordinates each), read by the kernels from
+  // coherent host memory; entries are reused only after a synchronization.
+  static constexpr int PTS_CAP = 512;
+  ef* h_pts = nullptr;
+  ef* d_pts = nullptr;   // device address of h_pts
+  int pts_next = 0;
   void sync() {
+    flush();
     HIPCHK(hipStreamSynchronize(st));
-    ring_pos = 0;
+    ring_pos = ring_flushed = 0;
+    pts_next = 0;
   }
+  void flush() {
+    if (ring_pos > ring_flushed)
+      HIPCHK(hipMemcpyAsync(d_ring + ring_flushed, h_ring + ring_flushed, ring_pos - ring_flushed,
+                            hipMemcpyHostToDevice, st));
+    ring_flushed = ring_pos;
+  }
+  // Space for n elements: the host pointer to fill and the device address it
+  // will have after flush().
+  template <typename T>
+  std::pair<T*, T*> stage(size_t n) {
+    const size_t need = (n * sizeof(T) + 255) & ~size_t(255);
+    if (need > ring_cap) bad("staged upload larger than the ring", WHIRGPU_BAD_STATE);
+    if (ring_pos + need > ring_cap) {
+      flush();
+      sync();
+    }
+    T* h = reinterpret_cast<T*>(h_ring + ring_pos);
+    T* d = reinterpret_cast<T*>(d_ring + ring_pos);
+    ring_pos += need;
+    return {h, d};
+  }
+  // An upload straight from the ring to dst (large ones through pinned
+  // memory, waited for).
   template <typename T>
   void h2d(T* dst, const T* src, size_t n) {
-    size_t bytes = n * sizeof(T);
-    size_t need = (bytes + 255) & ~size_t(255);
-    if (need > ring_cap) {
+    const size_t bytes = n * sizeof(T);
+    if (bytes > ring_cap / 4) {
       T* p = pin<T>(n);
       memcpy(p, src, bytes);
       HIPCHK(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, st));
       sync();
       return;
     }
-    if (ring_pos + need > ring_cap) sync();
-    memcpy(h_ring + ring_pos, src, bytes);
-    HIPCHK(hipMemcpyAsync(dst, h_ring + ring_pos, bytes, hipMemcpyHostToDevice, st));
-    ring_pos += need;
+    flush();
+    auto sp = stage<T>(n);
+    memcpy(sp.first, src, bytes);
+    HIPCHK(hipMemcpyAsync(dst, sp.first, bytes, hipMemcpyHostToDevice, st));
+    ring_flushed = ring_pos;   // uploaded directly, not through the mirror
   }
 };
 
@@ -288,8 +339,9 @@ struct whirgpu_prover_data {
   u64 height = 0;
   int row_len = 0;   // base elements per leaf
   bool ext = false;
-  bool root_on_host = false;   // the last tree kernel wrote the root to the context's h_root
-  int root_slot = 0;           // at h_root + 8 * root_slot
+  bool root_on_host = false;   // the last tree kernel wrote the root to the slot's h_root
+  Slot* sl = nullptr;          // the slot of the proof that committed it
+  int root_slot = 0;           // at sl->h_root + 8 * root_slot
 };
 
 struct whirgpu_proof {
@@ -328,12 +380,14 @@ struct Span {
       open = true;
     }
   }
-  void close() {
-    if (!open) return;
+  // Closes the span; returns its record index (SIZE_MAX when not profiling).
+  size_t close() {
+    if (!open) return SIZE_MAX;
     hipEvent_t b = c->event();
     HIPCHK(hipEventRecord(b, c->st));
     c->recs.push_back({stage, a, b, work});
     open = false;
+    return c->recs.size() - 1;
   }
   ~Span() {
     if (open) {
@@ -352,33 +406,134 @@ PointArg point_arg(const std::vector<ef>& z, int from = 0) {
   return a;
 }
 
-void use_device(const whirgpu_ctx* c);
-
-// Grows the eq-table scratch for points of up to n coordinates.
-void ensure_tabs(whirgpu_ctx* c, int n) {
-  size_t need = (1ull << std::max(0, n - 8)) + 512;
-  if (need <= c->tab_cap) return;
-  HIPCHK(hipStreamSynchronize(c->st));
-  ef* t = nullptr;
-  HIPCHK(hipMalloc(&t, 8 * need * sizeof(ef)));
-  (void)hipFree(c->d_tabs);
-  c->d_tabs = t;
-  c->fixed_bytes += 8 * (need - c->tab_cap) * sizeof(ef);
-  c->tab_cap = need;
+int ilog2u(u64 x) {
+  int k = 0;
+  while ((x >> (k + 1)) != 0) k++;
+  return k;
 }
 
-// Split eq tables of z (m coordinates) into the context's table slot `slot`:
-// hi covers z[0 .. m-lo), lo covers z[m-lo .. m); the hi table carries `scale`.
-void build_split_eq(whirgpu_ctx* c, const std::vector<ef>& z, int lo, const ef& scale, int slot, EqTables* t,
-                    int k) {
-  int m = (int)z.size();
-  int hi = m - lo;
-  if ((1ull << hi) + (1ull << lo) > c->tab_cap) bad("eq table too large for the context");
-  ef* base = c->d_tabs + (size_t)slot * c->tab_cap;
-  PointArg a = point_arg(z);
-  KCHK(eq_split(a, hi, lo, scale, base, base + (1ull << hi), c->st));
-  t->hi[k] = base;
-  t->lo[k] = base + (1ull << hi);
+// Carves aligned buffers out of one allocation (first pass: base null, sizes only).
+struct Carve {
+  char* base;
+  size_t off = 0;
+  template <typename T>
+  T* take(size_t n) {
+    off = (off + 255) & ~size_t(255);
+    T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+    off += n * sizeof(T);
+    return p;
+  }
+};
+
+// The slot's buffers, in one device and one coherent host allocation (the eq
+// tables apart, so they can grow).
+void slot_layout(const whirgpu_ctx* c, Slot* sl, char* d, char* h, char* hdev, size_t* d_bytes, size_t* h_bytes) {
+  Carve dv{d};
+  sl->d_partial = dv.take<ef>(c->partial_cap);
+  sl->d_chal = dv.take<DevChallenger>(2);
+  sl->d_grind = dv.take<u32>(2 * 64);   // two results, 256 bytes apart
+  sl->d_rchal = dv.take<DevChallenger>(2);
+  sl->d_gbase = dv.take<GrindBase>(1);
+  sl->rb.pts = dv.take<ef>((size_t)RT_MAX_OOD * RT_MAX_VARS);
+  sl->rb.ev = dv.take<ef>(RT_MAX_OOD);
+  sl->rb.idx = dv.take<u64>(RT_MAX_QUERIES);
+  sl->rb.selz = dv.take<u32>(RT_MAX_QUERIES);
+  sl->rb.gpow = dv.take<ef>(RT_MAX_OOD + RT_MAX_QUERIES);
+  sl->rb.grind = dv.take<u32>(64);
+  sl->rb.nq = dv.take<u32>(64);
+  *d_bytes = dv.off;
+  // host-visible buffers: the same offsets in the host and device views
+  Carve hv{h}, dh{hdev};
+  sl->h_small = hv.take<ef>(Slot::SMALL_CAP);
+  sl->d_small = dh.take<ef>(Slot::SMALL_CAP);
+  sl->h_idx = hv.take<u64>(Slot::IDX_CAP);
+  sl->d_idx = dh.take<u64>(Slot::IDX_CAP);
+  sl->h_open = hv.take<u32>(c->open_cap);
+  sl->d_open = dh.take<u32>(c->open_cap);
+  sl->h_root = hv.take<u32>(64);
+  sl->d_root = dh.take<u32>(64);
+  sl->h_chal = hv.take<DevChallenger>(1);
+  sl->d_chal_init = dh.take<DevChallenger>(1);
+  sl->h_sclog = hv.take<ScLog>(Slot::SC_LOG_CAP);
+  sl->d_sclog = dh.take<ScLog>(Slot::SC_LOG_CAP);
+  sl->h_rlog = hv.take<RoundLog>(2);
+  sl->d_rlog = dh.take<RoundLog>(2);
+  *h_bytes = hv.off;
+}
+
+void slot_free(Slot* sl) {
+  if (sl->d_mem) (void)hipFree(sl->d_mem);
+  if (sl->d_tabs) (void)hipFree(sl->d_tabs);
+  if (sl->h_mem) (void)hipHostFree(sl->h_mem);
+  if (sl->h_final) (void)hipHostFree(sl->h_final);
+  sl->d_mem = sl->h_mem = nullptr;
+  sl->d_tabs = nullptr;
+  sl->h_final = nullptr;
+}
+
+void slot_alloc(whirgpu_ctx* c, Slot* sl) {
+  size_t db = 0, hb = 0;
+  slot_layout(c, sl, nullptr, nullptr, nullptr, &db, &hb);
+  HIPCHK(hipMalloc(&sl->d_mem, db));
+  sl->d_bytes = db;
+  HIPCHK(hipHostMalloc(&sl->h_mem, hb, hipHostMallocCoherent));
+  sl->h_bytes = hb;
+  void* hdev = nullptr;
+  HIPCHK(hipHostGetDevicePointer(&hdev, sl->h_mem, 0));
+  slot_layout(c, sl, static_cast<char*>(sl->d_mem), static_cast<char*>(sl->h_mem), static_cast<char*>(hdev), &db,
+              &hb);
+  sl->tab_cap = c->tab_cap;
+  HIPCHK(hipMalloc(&sl->d_tabs, 8 * sl->tab_cap * sizeof(ef)));
+  HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&sl->h_final), sizeof(ef) << Slot::FINAL_MAX_LOG,
+                       hipHostMallocDefault));
+  c->fixed_bytes += db + 8 * sl->tab_cap * sizeof(ef);
+}
+
+// At least `count` slots (a group of that many proofs).
+void ensure_slots(whirgpu_ctx* c, int count) {
+  if (count < 1 || count > GROUP_MAX) bad("a group holds 1 to " + std::to_string(GROUP_MAX) + " proofs");
+  while ((int)c->slots.size() < count) {
+    c->slots.emplace_back(new Slot());
+    slot_alloc(c, c->slots.back().get());
+  }
+}
+
+// Grows a slot's eq-table scratch for points of up to n coordinates.
+void ensure_tabs(whirgpu_ctx* c, Slot* sl, int n) {
+  const size_t need = (1ull << std::max(0, n - 8)) + 512;
+  if (need <= sl->tab_cap) return;
+  c->sync();
+  ef* t = nullptr;
+  HIPCHK(hipMalloc(&t, 8 * need * sizeof(ef)));
+  (void)hipFree(sl->d_tabs);
+  sl->d_tabs = t;
+  c->fixed_bytes += 8 * (need - sl->tab_cap) * sizeof(ef);
+  sl->tab_cap = need;
+}
+
+// The points of one launch (point b's coordinates zs[b][from..]) in the
+// context's coherent point ring, where the kernels read them; returns their
+// device addresses.  A launch's points are taken together, so the
+// synchronization that recycles a full ring comes before any of them is written.
+std::vector<const ef*> stage_points(whirgpu_ctx* c, const std::vector<const std::vector<ef>*>& zs, size_t from = 0) {
+  const int count = (int)zs.size();
+  if (count > whirgpu_ctx::PTS_CAP) bad("too many points in one launch", WHIRGPU_BAD_STATE);
+  if (c->pts_next + count > whirgpu_ctx::PTS_CAP) c->sync();
+  std::vector<const ef*> out(count);
+  for (int b = 0; b < count; b++) {
+    const std::vector<ef>& z = *zs[b];
+    if (z.size() < from || z.size() - from > 32) bad("point of more than 32 coordinates");
+    ef* h = c->h_pts + (size_t)32 * c->pts_next;
+    out[b] = c->d_pts + (size_t)32 * c->pts_next;
+    c->pts_next++;
+    for (size_t i = from; i < z.size(); i++) h[i - from] = z[i];
+  }
+  return out;
+}
+
+void read_small(whirgpu_ctx* c, const Slot* sl, ef* dst, size_t n) {
+  c->sync();
+  memcpy(dst, sl->h_small, n * sizeof(ef));
 }
 
 // Blocks of a segmented eq-weighted sum: 2048 >> seg_bits, at most one per row.
@@ -389,79 +544,145 @@ int dot_blocks(int rows_log, int seg_bits) {
   return (int)std::min(rows, want);
 }
 
-// Segmented eq-weighted sum over the base polynomial for one point (see k_eq_dot).
-std::vector<ef> base_partials(whirgpu_ctx* c, const u32* d_f, int n, int sb, const std::vector<ef>& z_rest) {
-  int nrest = n - sb;
-  int lo = std::min(8, nrest);
-  EqTables t{};
-  build_split_eq(c, z_rest, lo, ef_one(), 0, &t, 0);
-  int nb = dot_blocks(nrest - lo, sb);
+// Segmented eq-weighted sums of each proof's base polynomial at one point
+// each: sum_x eq(z_rest, x) f(s, x) for the 2^sb leading-variable segments s,
+// into the slot's h_small (read after a synchronization).
+void base_partials(whirgpu_ctx* c, const std::vector<const u32*>& polys, const std::vector<Slot*>& sls, int n,
+                   int sb, const std::vector<std::vector<ef>>& z_rest) {
+  const int B = (int)polys.size();
+  const int nrest = n - sb;
+  const int lo = std::min(8, nrest), hi = nrest - lo;
+  const int nb = dot_blocks(nrest - lo, sb);
   if ((size_t)nb * (1ull << sb) > c->partial_cap) bad("reduction scratch too small");
-  {
-    Span sp(c, ST_EQDOT, 4.0 * (double)(1ull << n));
-    KCHK(eq_dot_base(d_f, n, sb, lo, t, 1, c->d_partial, nb, c->d_small, c->st));
+  if ((size_t)1 << sb > Slot::SMALL_CAP) bad("too many segments");
+  std::vector<EqSplitArgs> ea(B);
+  std::vector<DotArgs> da(B);
+  std::vector<const std::vector<ef>*> zp;
+  for (const auto& z : z_rest) zp.push_back(&z);
+  const auto dz = stage_points(c, zp);
+  for (int b = 0; b < B; b++) {
+    Slot* sl = sls[b];
+    if ((1ull << hi) + (1ull << lo) > sl->tab_cap) bad("eq table too large for the context");
+    ef* tab = sl->d_tabs;
+    ea[b] = EqSplitArgs{dz[b], ef_one(), tab, tab + (1ull << hi)};
+    EqTables t{};
+    t.hi[0] = tab;
+    t.lo[0] = tab + (1ull << hi);
+    da[b] = DotArgs{polys[b], t, sl->d_partial, sl->d_small};
   }
-  std::vector<ef> out(1ull << sb);
-  c->read_small(out.data(), out.size());
+  KCHK(eq_split(ea.data(), B, hi, lo, c->st));
+  Span sp(c, ST_EQDOT, 4.0 * (double)(1ull << n) * B);
+  KCHK(eq_dot_base(da.data(), B, n, sb, lo, 1, nb, c->st));
+}
+
+// InitialStatement::evaluate (src/whir/constraints/statement/initial.rs:143-152)
+// of statement b at point zs[b], for all statements of a context at once (slot
+// b for statement b): the segment partial sums are kept for the SVO rounds.
+std::vector<ef> statement_evaluate(whirgpu_ctx* c, const std::vector<whirgpu_statement*>& sts,
+                                   const std::vector<std::vector<ef>>& zs) {
+  const int B = (int)sts.size();
+  ensure_slots(c, B);
+  const int n = sts[0]->n, sb = sts[0]->seg_bits;
+  std::vector<const u32*> polys(B);
+  std::vector<Slot*> sls(B);
+  std::vector<std::vector<ef>> rest(B);
+  for (int b = 0; b < B; b++) {
+    if (sts[b]->ctx != c || sts[b]->n != n || sts[b]->seg_bits != sb) bad("statements of one group differ");
+    if ((int)zs[b].size() != n) bad("point has the wrong number of coordinates");
+    polys[b] = sts[b]->d_poly;
+    sls[b] = c->slots[b].get();
+    rest[b].assign(zs[b].begin() + sb, zs[b].end());
+  }
+  base_partials(c, polys, sls, n, sb, rest);
+  c->sync();
+  std::vector<ef> out(B);
+  for (int b = 0; b < B; b++) {
+    std::vector<ef> part(sls[b]->h_small, sls[b]->h_small + (1ull << sb));
+    ef eval;
+    if (sb == 0) {
+      eval = part[0];
+    } else {
+      std::vector<ef> zsb(zs[b].begin(), zs[b].begin() + sb);
+      auto eqs = whir::eq_table_host(zsb, ef_one());
+      eval = ef_zero();
+      for (size_t i = 0; i < part.size(); i++) eval = ef_add(eval, ef_mul(eqs[i], part[i]));
+    }
+    sts[b]->points.push_back(zs[b]);
+    sts[b]->evals.push_back(eval);
+    sts[b]->partials.push_back(std::move(part));
+    out[b] = eval;
+  }
   return out;
 }
 
-ef ext_mle(whirgpu_ctx* c, const ef* d_f, int m, const std::vector<ef>& z) {
-  int lo = std::min(8, m);
+// Multilinear evaluation of an extension polynomial on one slot (host-driven rounds).
+ef ext_mle(whirgpu_ctx* c, Slot* sl, const ef* d_f, int m, const std::vector<ef>& z) {
+  const int lo = std::min(8, m), hi = m - lo;
+  if ((1ull << hi) + (1ull << lo) > sl->tab_cap) bad("eq table too large for the context");
+  ef* tab = sl->d_tabs;
+  const EqSplitArgs ea{stage_points(c, {&z})[0], ef_one(), tab, tab + (1ull << hi)};
+  KCHK(eq_split(&ea, 1, hi, lo, c->st));
   EqTables t{};
-  build_split_eq(c, z, lo, ef_one(), 0, &t, 0);
-  int nb = dot_blocks(m - lo, 0);
+  t.hi[0] = tab;
+  t.lo[0] = tab + (1ull << hi);
+  const DotArgs da{d_f, t, sl->d_partial, sl->d_small};
   {
     Span sp(c, ST_EQDOT, 16.0 * (double)(1ull << m));
-    KCHK(eq_dot_ext(d_f, m, 0, lo, t, 1, c->d_partial, nb, c->d_small, c->st));
+    KCHK(eq_dot_ext(&da, 1, m, 0, lo, 1, dot_blocks(m - lo, 0), c->st));
   }
   ef out;
-  c->read_small(&out, 1);
+  read_small(c, sl, &out, 1);
   return out;
 }
 
 ef eq_point(const std::vector<ef>& a, const std::vector<ef>& b) { return whir::eq_at(a, b); }
 
-// Smallest-witness grinding on the device; updates the challenger like
-// GrindingChallenger::grind (observe witness, sample bits).
-u32 gpu_grind(whirgpu_ctx* c, DuplexChallenger& ch, int bits) {
-  u32 s[16];
-  for (int i = 0; i < 16; i++) s[i] = ch.sponge[i];
-  for (int i = 0; i < ch.in_len; i++) s[i] = ch.in[i];
-  if (c->gslot == whirgpu_ctx::GRIND_SLOTS) {
-    u32 init[whirgpu_ctx::GRIND_SLOTS];
-    for (u32& v : init) v = P;
-    c->h2d(c->d_gslots, init, whirgpu_ctx::GRIND_SLOTS);
-    c->gslot = 0;
-  }
-  u32* d_res = c->d_gslots + c->gslot++;
-  // Each launch covers sixteen times the expected number of candidates (a
-  // witness lies inside with probability 1 - e^-16; the waves stop as soon as
-  // one is known, so the size costs nothing); the search continues with the
-  // next range until one is found.
-  const u64 span = std::max<u64>(16ull << bits, 256);
-  u32 w = P;
-  Span sp(c, ST_GRIND, 0);
-  for (u64 start = 0; start < P && w >= P; start += span) {
-    KCHK(grind_range(c->d_c16, s, ch.in_len, bits, d_res, start, span, c->grind_blocks, c->st));
-    c->d2h(&w, d_res, 1);
-  }
-  sp.close();
-  if (!c->recs.empty() && c->prof) c->recs.back().work = (double)w + 1;
-  if (w >= P) bad("no proof-of-work witness below p", WHIRGPU_GRIND_FAILED);
-  u32 wm = to_monty(w);
-  if (!ch.check_witness(bits, wm)) bad("grinding witness rejected", WHIRGPU_GRIND_FAILED);
-  return wm;
+// Blocks per proof of a grind over all of [0, p) (so the search always
+// ends): sized to the expected search (16 * 2^bits candidates), one pass of
+// it at most, so short grinds leave the device to other work.
+int grind_blocks_for(const whirgpu_ctx* c, int bits) {
+  const u64 want = ((16ull << bits) + 255) / 256;
+  return bits <= 12 ? c->grind_blocks : (int)std::max<u64>(1, std::min<u64>((u64)c->grind_blocks, want));
 }
 
-// SumcheckData::observe_and_sample (src/whir/proof.rs:163-193).
-ef observe_and_sample(whirgpu_ctx* c, whir::SumcheckData& sd, DuplexChallenger& ch, const ef& c0, const ef& c2,
-                      int pow_bits) {
-  sd.polynomial_evaluations.push_back({c0, c2});
-  ch.observe_ef(c0);
-  ch.observe_ef(c2);
-  if (pow_bits > 0) sd.pow_witnesses.push_back(gpu_grind(c, ch, pow_bits));
-  return ch.sample_ef();
+// Smallest-witness grinding on the device for each challenger (host state),
+// one launch for all of them; updates each challenger like
+// GrindingChallenger::grind (observe the witness, sample bits) and returns
+// the witnesses (Montgomery form).
+std::vector<u32> gpu_grind(whirgpu_ctx* c, const std::vector<DuplexChallenger*>& chs, int bits) {
+  const int B = (int)chs.size();
+  if (B < 1 || B > GROUP_MAX) bad("grind group size", WHIRGPU_BAD_STATE);
+  if (c->gslot == whirgpu_ctx::GRIND_SLOTS) {
+    std::vector<u32> init((size_t)whirgpu_ctx::GRIND_SLOTS * GROUP_MAX, P);
+    c->h2d(c->d_hgrind, init.data(), init.size());
+    c->gslot = 0;
+  }
+  u32* res = c->d_hgrind + (size_t)c->gslot++ * GROUP_MAX;
+  std::vector<GrindArgs> a(B);
+  for (int b = 0; b < B; b++) {
+    const DuplexChallenger& ch = *chs[b];
+    memset(&a[b], 0, sizeof a[b]);
+    for (int i = 0; i < 16; i++) a[b].base.s[i] = ch.sponge[i];
+    for (int i = 0; i < ch.in_len; i++) a[b].base.s[i] = ch.in[i];
+    a[b].wpos = ch.in_len;
+    a[b].result = res + b;
+  }
+  Span sp(c, ST_GRIND, 0);
+  KCHK(grind_range(c->d_c16, a.data(), B, bits, TR_NONE, 0, P, grind_blocks_for(c, bits), c->st));
+  const size_t rec = sp.close();
+  HIPCHK(hipMemcpyAsync(c->h_hgrind, res, B * sizeof(u32), hipMemcpyDeviceToHost, c->st));
+  c->sync();
+  std::vector<u32> out(B);
+  double work = 0;
+  for (int b = 0; b < B; b++) {
+    const u32 w = c->h_hgrind[b];
+    if (w >= P) bad("no proof-of-work witness below p", WHIRGPU_GRIND_FAILED);
+    work += (double)w + 1;
+    out[b] = to_monty(w);
+    if (!chs[b]->check_witness(bits, out[b])) bad("grinding witness rejected", WHIRGPU_GRIND_FAILED);
+  }
+  if (rec != SIZE_MAX) c->recs[rec].work = work;
+  return out;
 }
 
 // Device-resident product polynomial (ProductPolynomial, src/sumcheck/product_polynomial.rs).
@@ -469,59 +690,70 @@ struct SumcheckState {
   Buf<ef> e, w;
   int m = 0;
   ef sum = ef_zero();
-  bool coeffs_valid = false;   // d_small[0..2] holds (h(0), h(2)) of the current arrays
-  unsigned partial_nb = 0;     // d_partial holds their partial sums from this many blocks
-  void invalidate() {
-    coeffs_valid = false;
-    partial_nb = 0;
-  }
+};
+
+// One proof of a group in flight.
+struct Run {
+  whirgpu_statement* st = nullptr;
+  whir::WhirProof* P = nullptr;
+  DuplexChallenger ch;
+  Slot* sl = nullptr;
+  SumcheckState s;
+  whirgpu_prover_data* prev = nullptr;   // the commitment the next STIR queries open
+  whirgpu_prover_data* cur = nullptr;    // this round's commitment, until it becomes prev
+  whirgpu_prover_data* next = nullptr;   // a commitment enqueued ahead of its round
+  std::vector<ef> fold_rand;
 };
 
 // Round challenge from the round polynomial (h(0), h(2)): the transcript
 // (observe, grind, sample) in a proof, or a scripted sequence.
 using ChallengeFn = std::function<ef(const ef&, const ef&)>;
 
-// ProductPolynomial::round (product_polynomial.rs:307-364).
-ef sumcheck_round(whirgpu_ctx* c, SumcheckState& s, const ChallengeFn& next) {
+// ProductPolynomial::round (product_polynomial.rs:307-364) with host-driven
+// challenges, on one slot.
+ef sumcheck_round(whirgpu_ctx* c, Slot* sl, SumcheckState& s, const ChallengeFn& next) {
   if (s.m < 1) bad("sumcheck round on a constant polynomial", WHIRGPU_BAD_STATE);
-  u64 h = 1ull << (s.m - 1);
-  if (!s.coeffs_valid) {
+  const u64 h = 1ull << (s.m - 1);
+  ScArgs a{};
+  a.e = s.e.p;
+  a.w = s.w.p;
+  a.partial = sl->d_partial;
+  a.c0c2 = sl->d_small;
+  {
     Span sp(c, ST_SUMCHECK, 64.0 * (double)h);
-    KCHK(sc_coeffs(s.e.p, s.w.p, h, c->d_partial, c->d_small, c->st));
+    KCHK(sc_coeffs(&a, 1, h, c->st));
   }
   ef cc[2];
-  c->read_small(cc, 2);
-  ef r = next(cc[0], cc[1]);
+  read_small(c, sl, cc, 2);
+  a.r = next(cc[0], cc[1]);
   {
     Span sp(c, ST_SUMCHECK, 96.0 * (double)h);
-    KCHK(sc_fold(s.e.p, s.w.p, h, r, c->d_partial, c->d_small, c->st));
+    KCHK(sc_fold(&a, 1, h, c->st));
   }
-  s.coeffs_valid = h >= 2;
-  s.partial_nb = 0;
   s.m -= 1;
-  s.sum = whir::extrapolate_012(cc[0], ef_sub(s.sum, cc[0]), cc[1], r);
-  return r;
+  s.sum = whir::extrapolate_012(cc[0], ef_sub(s.sum, cc[0]), cc[1], a.r);
+  return a.r;
 }
 
-// `rounds` sumcheck rounds of a proof with the transcript on the device: per
-// round the coefficient partials, the transcript wave (observe, grind base or
-// challenge), the grind and the transcript wave after it, and the fold by the
-// challenge in device memory; no host round trip until the end.  The host
-// then replays the logged rounds on its own challenger (observe (h(0), h(2)),
-// check the witness, sample) and requires every challenge to equal the
-// device's, so the proof and the challenger state are exactly those of the
-// host-driven rounds (SumcheckData::observe_and_sample, src/whir/proof.rs:163-193;
+// A device-transcript sumcheck phase (launched for a whole group): per round
+// the coefficient partials, the grind with its transcript prologue, and the
+// fold whose prologue observes the witness and samples r; no host round trip
+// until the end.  The host then replays the logged rounds of each proof on
+// its own challenger (observe (h(0), h(2)), check the witness, sample) and
+// requires every challenge to equal the device's, so the proof and the
+// challenger state are exactly those of host-driven rounds
+// (SumcheckData::observe_and_sample, src/whir/proof.rs:163-193;
 // ProductPolynomial::round, src/sumcheck/product_polynomial.rs:307-364).
 struct ScLaunch {
   int rounds = 0;
   int pow_bits = 0;
-  int log_base = 0;                     // first ScLog entry of these rounds
-  const DevChallenger* out = nullptr;   // device challenger state after the last round
-  std::vector<size_t> grind_recs;
+  int log_base = 0;                            // first ScLog entry of these rounds
+  std::vector<const DevChallenger*> out;       // per proof: device challenger after the last round
+  std::vector<size_t> grind_recs;              // profile records of the grinds, by round
 };
 
-// Uploads the host challenger as the device phase's starting state.
-void upload_challenger(whirgpu_ctx* c, const DuplexChallenger& ch) {
+// Uploads a host challenger as the slot's device phase starting state.
+void upload_challenger(Slot* sl, const DuplexChallenger& ch) {
   DevChallenger dc{};
   for (int i = 0; i < 16; i++) dc.sponge[i] = ch.sponge[i];
   for (int i = 0; i < ch.in_len; i++) dc.in[i] = ch.in[i];
@@ -530,253 +762,440 @@ void upload_challenger(whirgpu_ctx* c, const DuplexChallenger& ch) {
   dc.out_len = (u32)ch.out_len;
   // h_chal is free: every phase ends with a synchronization before the next
   // one uploads its state.
-  *c->h_chal = dc;
+  *sl->h_chal = dc;
 }
 
-// Grid of a device-transcript grind over all of [0, p) (so the search always
-// ends): sized to the expected search (16 * 2^bits candidates), one pass of
-// it at most, so short grinds leave the device to other work.
-int sc_grind_blocks(const whirgpu_ctx* c, int pow_bits) {
-  const u64 want = ((16ull << pow_bits) + 255) / 256;
-  return pow_bits <= 12 ? c->grind_blocks : (int)std::max<u64>(1, std::min<u64>((u64)c->grind_blocks, want));
-}
-
-// Launches `rounds` device-transcript sumcheck rounds starting from the
-// challenger at cin (device-visible memory).
-ScLaunch sumcheck_launch(whirgpu_ctx* c, SumcheckState& s, int rounds, int pow_bits, const DevChallenger* cin,
-                         int log_base = 0) {
+// Launches `rounds` device-transcript sumcheck rounds for every proof of the
+// group, proof b starting from the challenger at cin[b] (device-visible memory).
+ScLaunch sumcheck_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int rounds, int pow_bits,
+                         std::vector<const DevChallenger*> cin, int log_base = 0) {
+  const int B = (int)R.size();
   ScLaunch L;
   L.rounds = rounds;
   L.pow_bits = pow_bits;
   L.log_base = log_base;
   L.out = cin;
   if (rounds <= 0) return L;
-  if (log_base < 0 || log_base + rounds > whirgpu_ctx::SC_LOG_CAP)
+  if (log_base < 0 || log_base + rounds > Slot::SC_LOG_CAP)
     bad("too many sumcheck rounds in one phase", WHIRGPU_BAD_STATE);
-  if (s.m < rounds) bad("sumcheck round on a constant polynomial", WHIRGPU_BAD_STATE);
-  std::vector<size_t>& grind_recs = L.grind_recs;
-  // Each launch reads the challenger from `cin` and writes the next state to
+  const int m0 = R[0]->s.m;
+  if (m0 < rounds) bad("sumcheck round on a constant polynomial", WHIRGPU_BAD_STATE);
+  // Each launch reads the challenger from cin and writes the next state to
   // the other buffer; coefficient partials and grind results alternate too,
   // since every block of a launch reads what block 0 of the same launch
   // would otherwise overwrite.
-  DevChallenger* chal[2] = {c->d_chal, c->d_chal + 1};
-  ef* part[2] = {c->d_partial, c->d_partial + 4096};
-  u32* gres[2] = {c->d_grind, c->d_grind + 64};
-  auto next_state = [&]() { return cin == chal[0] ? chal[1] : chal[0]; };
+  auto next_state = [&](int b) {
+    DevChallenger* ch0 = R[b]->sl->d_chal;
+    return cin[b] == ch0 ? ch0 + 1 : ch0;
+  };
+  auto part = [&](int b, int k) { return R[b]->sl->d_partial + (k ? 4096 : 0); };
+  auto gres = [&](int b, int k) { return R[b]->sl->d_grind + (k ? 64 : 0); };
+  std::vector<ScArgs> a(B);
   int pp = 0;
   unsigned nb = 0;
   {
-    const u64 h = 1ull << (s.m - 1);
-    Span sp(c, ST_SUMCHECK, 64.0 * (double)h);
-    KCHK(sc_partials(s.e.p, s.w.p, h, part[0], &nb, gres[0], c->st));
+    const u64 h = 1ull << (m0 - 1);
+    for (int b = 0; b < B; b++) {
+      if (R[b]->s.m != m0) bad("sumcheck sizes of one group differ", WHIRGPU_BAD_STATE);
+      a[b] = ScArgs{};
+      a[b].e = R[b]->s.e.p;
+      a[b].w = R[b]->s.w.p;
+      a[b].partial = part(b, 0);
+      a[b].grind_reset = gres(b, 0);
+    }
+    Span sp(c, ST_SUMCHECK, 64.0 * (double)h * B);
+    KCHK(sc_partials(a.data(), B, h, &nb, c->st));
   }
+  std::vector<GrindArgs> ga(B);
   for (int i = 0; i < rounds; i++) {
-    const u64 h = 1ull << (s.m - 1);
-    u32* g = gres[i & 1];
-    u32* g_next = gres[(i + 1) & 1];
+    const u64 h = 1ull << (m0 - i - 1);
     int fold_mode = TR_COEFFS_SAMPLE;
     if (pow_bits > 0) {
-      DevChallenger* cout = next_state();
-      ScTr tr{cin, cout, c->d_sclog + log_base + i, part[pp], g, nullptr, (int)nb, pow_bits};
-      tr.gbase = c->d_gbase;
+      for (int b = 0; b < B; b++) {
+        DevChallenger* cout = next_state(b);
+        memset(&ga[b], 0, sizeof ga[b]);
+        ScTr& tr = ga[b].tr;
+        tr.ch_in = cin[b];
+        tr.ch_out = cout;
+        tr.log = R[b]->sl->d_sclog + log_base + i;
+        tr.partial = part(b, pp);
+        tr.grind = gres(b, i & 1);
+        tr.nb = (int)nb;
+        tr.pow_bits = pow_bits;
+        tr.gbase = R[b]->sl->d_gbase;
+        ga[b].result = tr.grind;
+        cin[b] = cout;
+      }
       Span sp(c, ST_GRIND, 0);
-      KCHK(grind_range_tr(c->d_c16, tr, pow_bits, 0, P, sc_grind_blocks(c, pow_bits), c->st));
-      sp.close();
-      if (c->prof) grind_recs.push_back(c->recs.size() - 1);
-      cin = cout;
+      KCHK(grind_range(c->d_c16, ga.data(), B, pow_bits, TR_COEFFS_GRIND, 0, P, grind_blocks_for(c, pow_bits),
+                       c->st));
+      L.grind_recs.push_back(sp.close());
       fold_mode = TR_WITNESS_SAMPLE;
     }
-    DevChallenger* cout = next_state();
-    ScTr tr{cin, cout, c->d_sclog + log_base + i, part[pp], g, g_next, (int)nb, pow_bits};
-    {
-      Span sp(c, ST_SUMCHECK, 96.0 * (double)h);
-      KCHK(sc_fold_tr(s.e.p, s.w.p, h, fold_mode, tr, c->d_c16, part[pp ^ 1], &nb, c->st));
+    for (int b = 0; b < B; b++) {
+      DevChallenger* cout = next_state(b);
+      a[b] = ScArgs{};
+      a[b].e = R[b]->s.e.p;
+      a[b].w = R[b]->s.w.p;
+      a[b].partial = part(b, pp ^ 1);
+      ScTr& tr = a[b].tr;
+      tr.ch_in = cin[b];
+      tr.ch_out = cout;
+      tr.log = R[b]->sl->d_sclog + log_base + i;
+      tr.partial = part(b, pp);
+      tr.grind = gres(b, i & 1);
+      tr.grind_next = gres(b, (i + 1) & 1);
+      tr.nb = (int)nb;
+      tr.pow_bits = pow_bits;
+      cin[b] = cout;
     }
-    cin = cout;
+    {
+      Span sp(c, ST_SUMCHECK, 96.0 * (double)h * B);
+      KCHK(sc_fold_tr(a.data(), B, h, fold_mode, c->d_c16, &nb, c->st));
+    }
     pp ^= 1;
-    s.m -= 1;
   }
+  for (Run* r : R) r->s.m -= rounds;
   L.out = cin;
-  // d_partial is scratch for other kernels between phases
-  s.invalidate();
   return L;
 }
 
-// After a synchronization: replays the logged rounds on the host challenger
-// and returns the challenges.
-std::vector<ef> sumcheck_replay(whirgpu_ctx* c, SumcheckState& s, const ScLaunch& la, whir::SumcheckData& sd,
-                                DuplexChallenger& ch) {
+// After the phase has completed: replays proof b's logged rounds on its host
+// challenger and returns the challenges.
+std::vector<ef> sumcheck_replay(whirgpu_ctx* c, Run& r, const ScLaunch& la, whir::SumcheckData& sd) {
   std::vector<ef> rs;
   const int rounds = la.rounds, pow_bits = la.pow_bits;
-  const std::vector<size_t>& grind_recs = la.grind_recs;
-  std::vector<ScLog> log(c->h_sclog + la.log_base, c->h_sclog + la.log_base + rounds);
+  std::vector<ScLog> log(r.sl->h_sclog + la.log_base, r.sl->h_sclog + la.log_base + rounds);
   for (int i = 0; i < rounds; i++) {
     const ScLog& L = log[i];
     if (L.err) bad("device transcript: no valid proof-of-work witness", WHIRGPU_GRIND_FAILED);
     sd.polynomial_evaluations.push_back({L.c0, L.c2});
-    ch.observe_ef(L.c0);
-    ch.observe_ef(L.c2);
+    r.ch.observe_ef(L.c0);
+    r.ch.observe_ef(L.c2);
     if (pow_bits > 0) {
-      if (!ch.check_witness(pow_bits, L.witness)) bad("device transcript: witness rejected", WHIRGPU_GRIND_FAILED);
+      if (!r.ch.check_witness(pow_bits, L.witness)) bad("device transcript: witness rejected", WHIRGPU_GRIND_FAILED);
       sd.pow_witnesses.push_back(L.witness);
-      if (i < (int)grind_recs.size()) c->recs[grind_recs[i]].work = (double)from_monty(L.witness) + 1;
+      if (i < (int)la.grind_recs.size() && la.grind_recs[i] != SIZE_MAX)
+        c->recs[la.grind_recs[i]].work += (double)from_monty(L.witness) + 1;
     }
-    ef r = ch.sample_ef();
-    if (!ef_eq(r, L.r)) bad("device transcript diverged from the host challenger", WHIRGPU_BAD_STATE);
-    s.sum = whir::extrapolate_012(L.c0, ef_sub(s.sum, L.c0), L.c2, r);
-    rs.push_back(r);
+    const ef x = r.ch.sample_ef();
+    if (!ef_eq(x, L.r)) bad("device transcript diverged from the host challenger", WHIRGPU_BAD_STATE);
+    r.s.sum = whir::extrapolate_012(L.c0, ef_sub(r.s.sum, L.c0), L.c2, x);
+    rs.push_back(x);
   }
   return rs;
 }
 
-// `rounds` sumcheck rounds with the transcript on the device, from the host
-// challenger's state.
-std::vector<ef> device_sumcheck(whirgpu_ctx* c, SumcheckState& s, int rounds, int pow_bits, whir::SumcheckData& sd,
-                                DuplexChallenger& ch) {
-  if (rounds <= 0) return {};
-  upload_challenger(c, ch);
-  ScLaunch la = sumcheck_launch(c, s, rounds, pow_bits, c->d_chal_init);
+// `rounds` device-transcript sumcheck rounds of each proof, from its host
+// challenger's state; replayed into each proof's SumcheckData `sd(run)`.
+void device_sumcheck(whirgpu_ctx* c, const std::vector<Run*>& R, int rounds, int pow_bits,
+                     const std::function<whir::SumcheckData&(Run&)>& sd) {
+  if (rounds <= 0) return;
+  std::vector<const DevChallenger*> cin;
+  for (Run* r : R) {
+    upload_challenger(r->sl, r->ch);
+    cin.push_back(r->sl->d_chal_init);
+  }
+  const ScLaunch la = sumcheck_launch(c, R, rounds, pow_bits, cin);
   c->sync();
-  return sumcheck_replay(c, s, la, sd, ch);
+  for (Run* r : R) r->fold_rand = sumcheck_replay(c, *r, la, sd(*r));
 }
 
-// Adds sum_i scale_i * eq(z_i, x) to the weights, four points per launch.
-void add_eq_weights(whirgpu_ctx* c, SumcheckState& s, const std::vector<std::vector<ef>>& pts,
-                    const std::vector<ef>& scales, bool accumulate_first) {
-  int m = s.m;
-  int lo = std::min(8, m);
-  size_t K = pts.size();
+// Adds sum_i scale_i * eq(z_i, x) to each proof's weights (the same number of
+// points for every proof), four points per launch.
+void add_eq_weights(whirgpu_ctx* c, const std::vector<Run*>& R, const std::vector<std::vector<std::vector<ef>>>& pts,
+                    const std::vector<std::vector<ef>>& scales, bool accumulate_first) {
+  const int B = (int)R.size();
+  const int m = R[0]->s.m;
+  const int lo = std::min(8, m), hi = m - lo;
+  const size_t K = pts[0].size();
+  std::vector<CombineArgs> ca(B);
   if (K == 0 && !accumulate_first) {
-    EqTables t{};
-    Span sp(c, ST_COMBINE, 16.0 * (double)(1ull << m));
-    KCHK(eq_combine(s.w.p, 1ull << m, lo, t, 0, 0, c->st));
+    for (int b = 0; b < B; b++) ca[b] = CombineArgs{R[b]->s.w.p, EqTables{}, nullptr};
+    Span sp(c, ST_COMBINE, 16.0 * (double)(1ull << m) * B);
+    KCHK(eq_combine(ca.data(), B, 1ull << m, lo, 0, 0, c->st));
     return;
   }
-  for (size_t b = 0; b < K; b += 4) {
-    EqTables t{};
-    int kk = (int)std::min<size_t>(4, K - b);
-    for (int k = 0; k < kk; k++) build_split_eq(c, pts[b + k], lo, scales[b + k], k, &t, k);
-    bool acc = b > 0 || accumulate_first;
-    Span sp(c, ST_COMBINE, (acc ? 32.0 : 16.0) * (double)(1ull << m));
-    KCHK(eq_combine(s.w.p, 1ull << m, lo, t, kk, acc ? 1 : 0, c->st));
+  std::vector<EqSplitArgs> ea(B);
+  for (size_t b0 = 0; b0 < K; b0 += 4) {
+    const int kk = (int)std::min<size_t>(4, K - b0);
+    for (int b = 0; b < B; b++) {
+      if (pts[b].size() != K) bad("constraint counts of one group differ");
+      ca[b] = CombineArgs{R[b]->s.w.p, EqTables{}, nullptr};
+    }
+    for (int k = 0; k < kk; k++) {
+      std::vector<const std::vector<ef>*> zp;
+      for (int b = 0; b < B; b++) zp.push_back(&pts[b][b0 + k]);
+      const auto dz = stage_points(c, zp);
+      for (int b = 0; b < B; b++) {
+        Slot* sl = R[b]->sl;
+        if ((1ull << hi) + (1ull << lo) > sl->tab_cap) bad("eq table too large for the context");
+        ef* base = sl->d_tabs + (size_t)k * sl->tab_cap;
+        ea[b] = EqSplitArgs{dz[b], scales[b][b0 + k], base, base + (1ull << hi)};
+        ca[b].tabs.hi[k] = base;
+        ca[b].tabs.lo[k] = base + (1ull << hi);
+      }
+      KCHK(eq_split(ea.data(), B, hi, lo, c->st));
+    }
+    const bool acc = b0 > 0 || accumulate_first;
+    Span sp(c, ST_COMBINE, (acc ? 32.0 : 16.0) * (double)(1ull << m) * B);
+    KCHK(eq_combine(ca.data(), B, 1ull << m, lo, kk, acc ? 1 : 0, c->st));
   }
+}
+
+// Round i of the initial sumcheck by the split route (see initial_sumcheck):
+// (h_i(0), h_i(2)) from the partial sums P (already folded by r_<i).
+void svo_round_poly(const whirgpu_statement* st, const std::vector<std::vector<ef>>& P, const std::vector<ef>& ap,
+                    const std::vector<ef>& rs, int i, int l0, ef* c0, ef* c2) {
+  *c0 = ef_zero();
+  *c2 = ef_zero();
+  for (size_t j = 0; j < st->points.size(); j++) {
+    const auto& z = st->points[j];
+    const size_t half = P[j].size() / 2;
+    std::vector<ef> z_after(z.begin() + i + 1, z.begin() + l0);
+    auto eqa = whir::eq_table_host(z_after, ef_one());
+    ef A0 = ef_zero(), A1 = ef_zero();
+    for (size_t q = 0; q < half; q++) {
+      A0 = ef_add(A0, ef_mul(eqa[q], P[j][q]));
+      A1 = ef_add(A1, ef_mul(eqa[q], P[j][half + q]));
+    }
+    const ef A2 = ef_sub(ef_dbl(A1), A0);
+    std::vector<ef> z_before(z.begin(), z.begin() + i);
+    const ef pre = eq_point(z_before, rs);
+    const ef& zi = z[i];
+    const ef e0 = ef_sub(ef_one(), zi);
+    const ef e2 = ef_sub(ef_add(ef_dbl(zi), zi), ef_one());
+    *c0 = ef_add(*c0, ef_mul(ap[j], ef_mul(pre, ef_mul(e0, A0))));
+    *c2 = ef_add(*c2, ef_mul(ap[j], ef_mul(pre, ef_mul(e2, A2))));
+  }
+}
+
+void svo_fold_partials(std::vector<std::vector<ef>>& P, const ef& r) {
+  for (auto& Pj : P) {
+    const size_t half = Pj.size() / 2;
+    for (size_t q = 0; q < half; q++) Pj[q] = ef_add(Pj[q], ef_mul(r, ef_sub(Pj[half + q], Pj[q])));
+    Pj.resize(half);
+  }
+}
+
+// After the SVO rounds: f folded by all l0 challenges at once, and the weights
+// W(x) = sum_j alpha^j eq(z_j[..l0], rs) eq(z_j[l0..], x), for every proof.
+void svo_finish(whirgpu_ctx* c, const std::vector<Run*>& R, int l0, const std::vector<std::vector<ef>>& aps,
+                const std::vector<std::vector<ef>>& rss) {
+  const int B = (int)R.size();
+  const int n = R[0]->st->n;
+  std::vector<CompressArgs> ca(B);
+  // every proof's eq table of rs in one staged upload (one ring allocation,
+  // so a wrap of the ring cannot separate them)
+  const size_t ns = 1ull << l0;
+  auto sp = c->stage<ef>(ns * B);
+  for (int b = 0; b < B; b++) {
+    Run& r = *R[b];
+    r.s.m = n - l0;
+    r.s.e = c->alloc<ef>(1ull << r.s.m);
+    r.s.w = c->alloc<ef>(1ull << r.s.m);
+    const auto eqrs = whir::eq_table_host(rss[b], ef_one());
+    memcpy(sp.first + ns * b, eqrs.data(), ns * sizeof(ef));
+    ca[b] = CompressArgs{r.st->d_poly, sp.second + ns * b, r.s.e.p};
+  }
+  c->flush();
+  {
+    Span sp(c, ST_COMPRESS, (4.0 * (double)(1ull << n) + 16.0 * (double)(1ull << (n - l0))) * B);
+    KCHK(compress_multi(ca.data(), B, n, l0, c->st));
+  }
+  std::vector<std::vector<std::vector<ef>>> rests(B);
+  std::vector<std::vector<ef>> scales(B);
+  for (int b = 0; b < B; b++) {
+    const whirgpu_statement* st = R[b]->st;
+    for (size_t j = 0; j < st->points.size(); j++) {
+      const auto& z = st->points[j];
+      std::vector<ef> zs(z.begin(), z.begin() + l0);
+      rests[b].emplace_back(z.begin() + l0, z.end());
+      scales[b].push_back(ef_mul(aps[b][j], eq_point(zs, rss[b])));
+    }
+  }
+  add_eq_weights(c, R, rests, scales, false);
 }
 
 // Initial sumcheck from the base evaluations (Sumcheck::from_base_evals,
-// src/sumcheck/sumcheck_prover.rs:237-292).  Classic and SVO produce the same
-// transcript (asserted by the reference at src/sumcheck/tests.rs:473-487); the
-// device always takes the split route when n > l0: per constraint, the
-// partial sums P(s) = sum_x eq(z_rest, x) f(s, x) over the l0 leading
-// variables s (computed once at evaluate time) give every round polynomial
+// src/sumcheck/sumcheck_prover.rs:237-292) for every proof of the group.
+// Classic and SVO produce the same transcript (asserted by the reference at
+// src/sumcheck/tests.rs:473-487); the device always takes the split route
+// when n > l0: per constraint, the partial sums P(s) = sum_x eq(z_rest, x)
+// f(s, x) over the l0 leading variables s (computed when the constraint was
+// evaluated) give every round polynomial
 //   h_i(X) = eq(z_<i, r_<i) eq(z_i, X) sum_s' eq(z_>i, s') P~(r_<i, X, s'),
-// and afterwards f is folded by all l0 challenges at once.
-// `ap` holds the combination weights alpha^j of the constraints.
-std::vector<ef> initial_sumcheck(whirgpu_ctx* c, const whirgpu_statement* st, int l0, const std::vector<ef>& ap,
-                                 const ChallengeFn& next, SumcheckState& s) {
-  const int n = st->n;
-  const size_t K = st->points.size();
-  s.sum = ef_zero();
-  for (size_t j = 0; j < K; j++) s.sum = ef_add(s.sum, ef_mul(ap[j], st->evals[j]));
-  std::vector<ef> rs;
-  if (l0 > 0 && st->seg_bits == l0 && n > l0) {
-    std::vector<std::vector<ef>> P = st->partials;
-    for (int i = 0; i < l0; i++) {
-      ef c0 = ef_zero(), c2 = ef_zero();
-      for (size_t j = 0; j < K; j++) {
-        const auto& z = st->points[j];
-        size_t half = P[j].size() / 2;
-        std::vector<ef> z_after(z.begin() + i + 1, z.begin() + l0);
-        auto eqa = whir::eq_table_host(z_after, ef_one());
-        ef A0 = ef_zero(), A1 = ef_zero();
-        for (size_t q = 0; q < half; q++) {
-          A0 = ef_add(A0, ef_mul(eqa[q], P[j][q]));
-          A1 = ef_add(A1, ef_mul(eqa[q], P[j][half + q]));
-        }
-        ef A2 = ef_sub(ef_dbl(A1), A0);
-        std::vector<ef> z_before(z.begin(), z.begin() + i);
-        ef pre = eq_point(z_before, rs);
-        const ef& zi = z[i];
-        ef e0 = ef_sub(ef_one(), zi);
-        ef e2 = ef_sub(ef_add(ef_dbl(zi), zi), ef_one());
-        c0 = ef_add(c0, ef_mul(ap[j], ef_mul(pre, ef_mul(e0, A0))));
-        c2 = ef_add(c2, ef_mul(ap[j], ef_mul(pre, ef_mul(e2, A2))));
-      }
-      ef r = next(c0, c2);
-      s.sum = whir::extrapolate_012(c0, ef_sub(s.sum, c0), c2, r);
-      rs.push_back(r);
-      for (auto& Pj : P) {
-        size_t half = Pj.size() / 2;
-        for (size_t q = 0; q < half; q++) Pj[q] = ef_add(Pj[q], ef_mul(r, ef_sub(Pj[half + q], Pj[q])));
-        Pj.resize(half);
-      }
-    }
-    // f(rs, x) for all x, and W(x) = sum_j alpha^j eq(z_j[..l0], rs) eq(z_j[l0..], x)
-    s.m = n - l0;
-    s.e = c->alloc<ef>(1ull << s.m);
-    s.w = c->alloc<ef>(1ull << s.m);
-    auto eqrs = whir::eq_table_host(rs, ef_one());
-    ef* d_eqrs = c->d_tabs + 7 * c->tab_cap;
-    c->h2d(d_eqrs, eqrs.data(), eqrs.size());
-    {
-      Span sp(c, ST_COMPRESS, 4.0 * (double)(1ull << n) + 16.0 * (double)(1ull << (n - l0)));
-      KCHK(compress_multi(st->d_poly, n, l0, d_eqrs, s.e.p, c->st));
-    }
-    std::vector<std::vector<ef>> rests;
-    std::vector<ef> scales;
-    for (size_t j = 0; j < K; j++) {
-      const auto& z = st->points[j];
-      std::vector<ef> zs(z.begin(), z.begin() + l0);
-      rests.emplace_back(z.begin() + l0, z.end());
-      scales.push_back(ef_mul(ap[j], eq_point(zs, rs)));
-    }
-    add_eq_weights(c, s, rests, scales, false);
-    s.invalidate();
-    return rs;
+// and afterwards f is folded by all l0 challenges at once.  The rounds'
+// proofs of work are one grind launch for the group.
+void initial_sumcheck(whirgpu_ctx* c, const std::vector<Run*>& R, int l0, int pow_bits) {
+  const int B = (int)R.size();
+  std::vector<std::vector<ef>> aps(B), rss(B);
+  std::vector<std::vector<std::vector<ef>>> Ps(B);
+  for (int b = 0; b < B; b++) {
+    Run& r = *R[b];
+    const ef alpha = r.ch.sample_ef();
+    aps[b] = whir::powers(alpha, r.st->points.size());
+    r.s.sum = ef_zero();
+    for (size_t j = 0; j < r.st->points.size(); j++) r.s.sum = ef_add(r.s.sum, ef_mul(aps[b][j], r.st->evals[j]));
+    Ps[b] = r.st->partials;
   }
-  // small instances: classic rounds on the extension-embedded evaluations
-  s.m = n;
-  s.e = c->alloc<ef>(1ull << n);
-  s.w = c->alloc<ef>(1ull << n);
-  KCHK(base_to_ext(st->d_poly, 1ull << n, s.e.p, c->st));
-  add_eq_weights(c, s, st->points, ap, false);
-  s.invalidate();
-  for (int i = 0; i < l0; i++) rs.push_back(sumcheck_round(c, s, next));
-  return rs;
+  const whirgpu_statement* st0 = R[0]->st;
+  if (l0 > 0 && st0->seg_bits == l0 && st0->n > l0) {
+    std::vector<DuplexChallenger*> chs;
+    for (Run* r : R) chs.push_back(&r->ch);
+    for (int i = 0; i < l0; i++) {
+      for (int b = 0; b < B; b++) {
+        Run& r = *R[b];
+        ef c0, c2;
+        svo_round_poly(r.st, Ps[b], aps[b], rss[b], i, l0, &c0, &c2);
+        r.P->initial_sumcheck.polynomial_evaluations.push_back({c0, c2});
+        r.ch.observe_ef(c0);
+        r.ch.observe_ef(c2);
+      }
+      if (pow_bits > 0) {
+        const auto ws = gpu_grind(c, chs, pow_bits);
+        for (int b = 0; b < B; b++) R[b]->P->initial_sumcheck.pow_witnesses.push_back(ws[b]);
+      }
+      for (int b = 0; b < B; b++) {
+        Run& r = *R[b];
+        const auto& pe = r.P->initial_sumcheck.polynomial_evaluations.back();
+        const ef x = r.ch.sample_ef();
+        r.s.sum = whir::extrapolate_012(pe[0], ef_sub(r.s.sum, pe[0]), pe[1], x);
+        rss[b].push_back(x);
+        svo_fold_partials(Ps[b], x);
+      }
+    }
+    svo_finish(c, R, l0, aps, rss);
+    for (int b = 0; b < B; b++) R[b]->fold_rand = rss[b];
+    return;
+  }
+  // small instances (n <= l0): classic rounds on the extension-embedded
+  // evaluations, one proof at a time
+  for (int b = 0; b < B; b++) {
+    Run& r = *R[b];
+    const int n = r.st->n;
+    r.s.m = n;
+    r.s.e = c->alloc<ef>(1ull << n);
+    r.s.w = c->alloc<ef>(1ull << n);
+    const ExtArgs xa{r.st->d_poly, r.s.e.p};
+    KCHK(base_to_ext(&xa, 1, 1ull << n, c->st));
+    add_eq_weights(c, {&r}, {r.st->points}, {aps[b]}, false);
+    r.fold_rand.clear();
+    for (int i = 0; i < l0; i++) {
+      r.fold_rand.push_back(sumcheck_round(c, r.sl, r.s, [&](const ef& c0, const ef& c2) {
+        r.P->initial_sumcheck.polynomial_evaluations.push_back({c0, c2});
+        r.ch.observe_ef(c0);
+        r.ch.observe_ef(c2);
+        if (pow_bits > 0) r.P->initial_sumcheck.pow_witnesses.push_back(gpu_grind(c, {&r.ch}, pow_bits)[0]);
+        return r.ch.sample_ef();
+      }));
+    }
+  }
 }
 
-int ilog2u(u64 x) {
-  int k = 0;
-  while ((x >> (k + 1)) != 0) k++;
-  return k;
+// Leaf hashes and the tree above them for each proof's commitment (one shape),
+// timed separately; the last tree kernel writes each root to its slot's next
+// root entry.
+void commit_trees(whirgpu_ctx* c, const std::vector<whirgpu_prover_data*>& pds) {
+  const int B = (int)pds.size();
+  const whirgpu_prover_data* p0 = pds[0];
+  const int lw = (int)c->params.leaf_hash_width;
+  const int rate = lw - 8;
+  std::vector<TreeArgs> ta(B);
+  for (int b = 0; b < B; b++) {
+    whirgpu_prover_data* pd = pds[b];
+    pd->root_slot = pd->sl->root_next;
+    pd->sl->root_next = (pd->sl->root_next + 1) & 7;
+    ta[b] = TreeArgs{pd->rows.p, pd->tree.p, pd->sl->d_root + 8 * pd->root_slot};
+  }
+  {
+    Span sp(c, ST_LEAVES, (double)p0->height * (double)((p0->row_len + rate - 1) / rate) * B);
+    KCHK(merkle_leaves(ta.data(), B, p0->row_len, p0->height, lw, c->d_c16, c->d_c24, c->st));
+  }
+  Span sp(c, ST_TREE, (double)(p0->height - 1) * B);
+  const int rc = merkle_tree(ta.data(), B, p0->height, c->c16, c->st);
+  if (rc < 0) KCHK(rc);
+  for (whirgpu_prover_data* pd : pds) pd->root_on_host = rc == 0;
 }
 
-// Opens `idx` on a committed matrix and returns the openings plus each row
-// as extension elements.
-void parse_openings(whirgpu_ctx* c, const whirgpu_prover_data* pd, int nq, std::vector<whir::QueryOpening>* out,
-                    std::vector<std::vector<ef>>* rows_ef, const u32* h = nullptr);
+// The root: written to its slot's h_root entry by the tree's last kernel, or
+// (single leaf) copied.  completed: the tree kernels are known to have finished.
+Digest read_root(whirgpu_ctx* c, const whirgpu_prover_data* pd, bool completed = false) {
+  Digest d;
+  if (pd->root_on_host) {
+    if (!completed) c->sync();
+    memcpy(d.data(), pd->sl->h_root + 8 * pd->root_slot, sizeof d);
+  } else {
+    c->d2h(d.data(), pd->tree.p + (2 * pd->height - 2) * 8, 8);
+  }
+  return d;
+}
 
-void open_rows(whirgpu_ctx* c, const whirgpu_prover_data* pd, const std::vector<u64>& idx,
+void observe_digest(DuplexChallenger& ch, const Digest& d) {
+  for (u32 x : d) ch.observe(x);
+}
+
+// Commitment of round r's folded polynomial for every proof (prover/mod.rs:
+// 158-190): 2^(m - kn + R) rows of 2^kn extension elements and their tree.
+// The commitments go to each run's `next`.
+void commit_round(whirgpu_ctx* c, const std::vector<Run*>& R, int round) {
+  const int B = (int)R.size();
+  const auto& cfg = c->cfg;
+  const int m = cfg.num_variables - cfg.params.folding_factor.total_number(round);
+  const int kn = cfg.ff(round + 1);
+  const int Rl = ilog2u(cfg.inv_rate(round));
+  const u64 H = 1ull << (m - kn + Rl), W = 1ull << kn;
+  const bool two = lde_two_pass(m, kn, true);
+  std::vector<Buf<ef>> tmps(B);
+  std::vector<LdeArgs<ef>> la(B);
+  std::vector<whirgpu_prover_data*> pds(B);
+  for (int b = 0; b < B; b++) {
+    auto* pd = new whirgpu_prover_data();
+    R[b]->next = pd;   // destroyed by the caller if anything below throws
+    pd->ctx = c;
+    pd->sl = R[b]->sl;
+    pd->rows = c->alloc<u32>(H * W * 4);
+    pd->tree = c->alloc<u32>((2 * H - 1) * 8);
+    pd->height = H;
+    pd->row_len = (int)(4 * W);
+    pd->ext = true;
+    if (two) tmps[b] = c->alloc<ef>(H * W);
+    la[b] = LdeArgs<ef>{R[b]->s.e.p, tmps[b].p, reinterpret_cast<ef*>(pd->rows.p)};
+    pds[b] = pd;
+  }
+  {
+    Span sp(c, ST_LDE_EXT, (16.0 * (double)(1ull << m) + 16.0 * (double)(H * W)) * B);
+    KCHK(lde_ext(la.data(), B, m, kn, Rl, c->tw, c->st));
+  }
+  commit_trees(c, pds);
+  for (auto& t : tmps)
+    if (t.p) c->release(t);   // reuse is ordered on the same stream
+}
+
+// Opens `idx` on one proof's committed matrix (host-driven rounds).
+void parse_openings(const whirgpu_prover_data* pd, const u32* h, int nq, std::vector<whir::QueryOpening>* out,
+                    std::vector<std::vector<ef>>* rows_ef);
+
+void open_rows(whirgpu_ctx* c, Slot* sl, const whirgpu_prover_data* pd, const std::vector<u64>& idx,
                std::vector<whir::QueryOpening>* out, std::vector<std::vector<ef>>* rows_ef) {
-  int nq = (int)idx.size();
+  const int nq = (int)idx.size();
   if (nq == 0) return;
-  int depth = ilog2u(pd->height);
-  size_t per = (size_t)pd->row_len + 8ull * depth;
-  if (per * nq > c->open_cap || nq > 4096) bad("too many queries for the opening buffer");
+  const size_t per = (size_t)pd->row_len + 8ull * ilog2u(pd->height);
+  if (per * nq > c->open_cap || (size_t)nq > Slot::IDX_CAP) bad("too many queries for the opening buffer");
   // h_idx / h_open are free: every opening ends with a synchronization.
-  memcpy(c->h_idx, idx.data(), idx.size() * sizeof(u64));
-  KCHK(merkle_open(pd->rows.p, pd->row_len, pd->tree.p, pd->height, c->d_idx, nq, c->d_open, c->st));
+  memcpy(sl->h_idx, idx.data(), idx.size() * sizeof(u64));
+  const OpenArgs oa{pd->rows.p, pd->tree.p, sl->d_idx, nullptr, sl->d_open};
+  KCHK(merkle_open(&oa, 1, pd->row_len, pd->height, nq, c->st));
   c->sync();
-  parse_openings(c, pd, nq, out, rows_ef);
+  parse_openings(pd, sl->h_open, nq, out, rows_ef);
 }
 
-// The openings of nq queries written to h (h_open by default) by an opening
-// kernel (after a synchronization), as proof entries and rows of extension
-// elements.
-void parse_openings(whirgpu_ctx* c, const whirgpu_prover_data* pd, int nq, std::vector<whir::QueryOpening>* out,
-                    std::vector<std::vector<ef>>* rows_ef, const u32* h) {
+// The openings of nq queries written to h by an opening kernel (after a
+// synchronization), as proof entries and rows of extension elements.
+void parse_openings(const whirgpu_prover_data* pd, const u32* h, int nq, std::vector<whir::QueryOpening>* out,
+                    std::vector<std::vector<ef>>* rows_ef) {
   const int depth = ilog2u(pd->height);
   const size_t per = (size_t)pd->row_len + 8ull * depth;
-  if (!h) h = c->h_open;
   for (int q = 0; q < nq; q++) {
     const u32* b = h + per * q;
     whir::QueryOpening o;
@@ -798,41 +1217,8 @@ void parse_openings(whirgpu_ctx* c, const whirgpu_prover_data* pd, int nq, std::
   }
 }
 
-// Leaf hashes and the tree above them, timed separately.
-void commit_tree(whirgpu_ctx* c, whirgpu_prover_data* pd) {
-  const int lw = (int)c->params.leaf_hash_width;
-  const int rate = lw - 8;
-  {
-    Span sp(c, ST_LEAVES, (double)pd->height * (double)((pd->row_len + rate - 1) / rate));
-    KCHK(merkle_leaves(pd->rows.p, pd->row_len, pd->height, lw, c->d_c16, c->d_c24, pd->tree.p, c->st));
-  }
-  Span sp(c, ST_TREE, (double)(pd->height - 1));
-  pd->root_slot = c->root_next;
-  c->root_next = (c->root_next + 1) & 7;
-  int rc = merkle_tree(pd->height, c->c16, pd->tree.p, c->st, c->d_root + 8 * pd->root_slot);
-  if (rc < 0) KCHK(rc);
-  pd->root_on_host = rc == 0;   // the tree's last kernel wrote the root to h_root
-}
-
-// The root: written to its h_root slot by the tree's last kernel, or (single
-// leaf) copied.  completed: the tree kernels are known to have finished.
-Digest read_root(whirgpu_ctx* c, const whirgpu_prover_data* pd, bool completed = false) {
-  Digest d;
-  if (pd->root_on_host) {
-    if (!completed) c->sync();
-    memcpy(d.data(), c->h_root + 8 * pd->root_slot, sizeof d);
-  } else {
-    c->d2h(d.data(), pd->tree.p + (2 * pd->height - 2) * 8, 8);
-  }
-  return d;
-}
-
-void observe_digest(DuplexChallenger& ch, const Digest& d) {
-  for (u32 x : d) ch.observe(x);
-}
-
 // Whether a round's steps after its commitment fit the device transcript's
-// buffers (otherwise the host drives them).
+// buffers (otherwise the host drives them, one proof at a time).
 bool device_round_fits(const whirgpu_ctx* c, const whir::RoundConfig& rp, int m, const whirgpu_prover_data* prev) {
   const int lo = std::min(8, m), hi = m - lo;
   if (!prev || m < 1 || m > RT_MAX_VARS) return false;
@@ -843,171 +1229,207 @@ bool device_round_fits(const whirgpu_ctx* c, const whir::RoundConfig& rp, int m,
 }
 
 // A round's steps from its commitment's root to the end of its sumcheck
-// (prover/mod.rs:192-303) with the transcript on the device: the OOD samples
-// and their answers, the proof of work, the STIR queries and their openings,
-// gamma, the weight updates and the sumcheck run as one stream of kernels,
-// each reading its challenger from the previous one, and one event at the
-// end.  The caller may enqueue the next round's commitment (which needs none
-// of the host's results) before device_round_replay waits for that event;
-// the host then replays every step on its own challenger and requires each
-// challenge, answer and query index to equal the device's, so the proof and
-// the challenger state are those of the host-driven steps.
+// (prover/mod.rs:192-303) with the transcript on the device, for every proof
+// of the group: the OOD samples and their answers, the proof of work, the
+// STIR queries and their openings, gamma, the weight updates and the sumcheck
+// run as one stream of launches, each reading each proof's challenger from
+// the previous one, and one event at the end.  The caller may enqueue the
+// next round's commitment (which needs none of the host's results) before
+// device_round_replay waits for that event; the host then replays every step
+// on each proof's challenger and requires each challenge, answer and query
+// index to equal the device's, so the proof and the challenger state are
+// those of the host-driven steps.
 struct RoundLaunch {
   ScLaunch la;
   size_t grind_rec = SIZE_MAX;
 };
 
-RoundLaunch device_round_launch(whirgpu_ctx* c, SumcheckState& s, const whir::WhirConfig& cfg, int round, int kn,
-                                const whirgpu_prover_data* pd, const whirgpu_prover_data* prev,
-                                const DuplexChallenger& ch) {
+RoundLaunch device_round_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int round, int kn) {
+  const int B = (int)R.size();
+  const auto& cfg = c->cfg;
   const auto& rp = cfg.round_parameters[round];
-  const int m = s.m;
+  const int m = R[0]->s.m;
   const int n_ood = rp.ood_samples;
   const int lo = std::min(8, m), hi = m - lo;
-  upload_challenger(c, ch);
-  DevChallenger* rch[2] = {c->d_rchal, c->d_rchal + 1};
-  int ri = 0;
-  const DevChallenger* cin = c->d_chal_init;
-  const u32* obs = pd->tree.p + (2 * pd->height - 2) * 8;   // the root
+  std::vector<const DevChallenger*> cin(B);
+  std::vector<const u32*> obs(B);
+  std::vector<ef*> log_obs(B, nullptr);
   int n_obs = 8;
-  ef* log_obs = nullptr;
+  int ri = 0;
+  auto rch = [&](int b) { return R[b]->sl->d_rchal + ri; };
+  for (int b = 0; b < B; b++) {
+    upload_challenger(R[b]->sl, R[b]->ch);
+    cin[b] = R[b]->sl->d_chal_init;
+    const whirgpu_prover_data* pd = R[b]->cur;
+    obs[b] = pd->tree.p + (2 * pd->height - 2) * 8;   // the root
+  }
+  std::vector<OodArgs> oa(B);
+  std::vector<DotArgs> da(B);
   for (int i = 0; i < n_ood; i++) {
-    DevChallenger* cout = rch[ri];
-    ri ^= 1;
-    const StepTr st{cin, cout, obs, n_obs, log_obs};
-    // table slot i: the weight update below uses the same tables, scaled by gamma^i
-    ef* tab = c->d_tabs + (size_t)i * c->tab_cap;
-    KCHK(ood_eq_tables(st, &c->d_rlog->y[i], c->rb.pts + (size_t)i * RT_MAX_VARS, m, hi, lo, tab, tab + (1ull << hi),
-                       c->d_c16, c->st, i == 0 && rp.pow_bits > 0 ? c->rb.grind : nullptr));
-    EqTables t{};
-    t.hi[0] = tab;
-    t.lo[0] = tab + (1ull << hi);
-    {
-      Span sp(c, ST_EQDOT, 16.0 * (double)(1ull << m));
-      KCHK(eq_dot_ext(s.e.p, m, 0, lo, t, 1, c->d_partial, dot_blocks(m - lo, 0), c->rb.ev + i, c->st));
+    for (int b = 0; b < B; b++) {
+      Slot* sl = R[b]->sl;
+      // table slot i: the weight update below uses the same tables, scaled by gamma^i
+      ef* tab = sl->d_tabs + (size_t)i * sl->tab_cap;
+      oa[b] = OodArgs{StepTr{cin[b], rch(b), obs[b], n_obs, log_obs[b]}, &sl->d_rlog->y[i],
+                      sl->rb.pts + (size_t)i * RT_MAX_VARS, tab, tab + (1ull << hi),
+                      i == 0 && rp.pow_bits > 0 ? sl->rb.grind : nullptr};
+      EqTables t{};
+      t.hi[0] = tab;
+      t.lo[0] = tab + (1ull << hi);
+      da[b] = DotArgs{R[b]->s.e.p, t, sl->d_partial, sl->rb.ev + i};
     }
-    cin = cout;
-    obs = reinterpret_cast<const u32*>(c->rb.ev + i);
+    KCHK(ood_eq_tables(oa.data(), B, m, hi, lo, c->d_c16, c->st));
+    {
+      Span sp(c, ST_EQDOT, 16.0 * (double)(1ull << m) * B);
+      KCHK(eq_dot_ext(da.data(), B, m, 0, lo, 1, dot_blocks(m - lo, 0), c->st));
+    }
+    for (int b = 0; b < B; b++) {
+      cin[b] = rch(b);
+      obs[b] = reinterpret_cast<const u32*>(R[b]->sl->rb.ev + i);
+      log_obs[b] = &R[b]->sl->d_rlog->ev[i];
+    }
     n_obs = 4;
-    log_obs = &c->d_rlog->ev[i];
+    ri ^= 1;
   }
   RoundLaunch rl;
-  size_t& grind_rec = rl.grind_rec;
   if (rp.pow_bits > 0) {
-    // the first OOD step reset the grind slot; without OOD samples it is reset here
-    if (n_ood == 0) HIPCHK(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(c->rb.grind), (int)P, 1, c->st));
-    DevChallenger* cout = rch[ri];
-    ri ^= 1;
-    ScTr tr{};
-    tr.ch_in = cin;
-    tr.ch_out = cout;
-    tr.grind = c->rb.grind;
-    tr.pow_bits = rp.pow_bits;
-    tr.obs = obs;
-    tr.n_obs = n_obs;
-    tr.log_obs = log_obs;
-    tr.gbase = c->d_gbase;
-    const u64 want = ((16ull << rp.pow_bits) + 255) / 256;
-    const int gblocks =
-        rp.pow_bits <= 12 ? c->grind_blocks : (int)std::max<u64>(1, std::min<u64>((u64)c->grind_blocks, want));
+    std::vector<GrindArgs> ga(B);
+    for (int b = 0; b < B; b++) {
+      Slot* sl = R[b]->sl;
+      // the first OOD step reset the grind slot; without OOD samples it is reset here
+      if (n_ood == 0) HIPCHK(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(sl->rb.grind), (int)P, 1, c->st));
+      memset(&ga[b], 0, sizeof ga[b]);
+      ScTr& tr = ga[b].tr;
+      tr.ch_in = cin[b];
+      tr.ch_out = rch(b);
+      tr.grind = sl->rb.grind;
+      tr.pow_bits = rp.pow_bits;
+      tr.obs = obs[b];
+      tr.n_obs = n_obs;
+      tr.log_obs = log_obs[b];
+      tr.gbase = sl->d_gbase;
+      ga[b].result = tr.grind;
+    }
     Span sp(c, ST_GRIND, 0);
-    KCHK(grind_range_tr(c->d_c16, tr, rp.pow_bits, 0, P, gblocks, c->st, TR_OBS_GRIND));
-    sp.close();
-    if (c->prof) grind_rec = c->recs.size() - 1;
-    cin = cout;
-    obs = nullptr;
+    KCHK(grind_range(c->d_c16, ga.data(), B, rp.pow_bits, TR_OBS_GRIND, 0, P, grind_blocks_for(c, rp.pow_bits),
+                     c->st));
+    rl.grind_rec = sp.close();
+    for (int b = 0; b < B; b++) {
+      cin[b] = rch(b);
+      obs[b] = nullptr;
+      log_obs[b] = nullptr;
+    }
     n_obs = 0;
-    log_obs = nullptr;
+    ri ^= 1;
   }
   {
-    DevChallenger* cout = rch[ri];
+    std::vector<QueryArgs> qa(B);
+    for (int b = 0; b < B; b++)
+      qa[b] = QueryArgs{StepTr{cin[b], rch(b), obs[b], n_obs, log_obs[b]}, R[b]->sl->rb.grind, R[b]->sl->d_rlog,
+                        R[b]->sl->rb};
+    KCHK(round_queries(qa.data(), B, rp.pow_bits, rp.domain_size, cfg.ff(round), rp.num_queries,
+                       rp.folded_domain_gen, n_ood, c->d_c16, c->st));
+    for (int b = 0; b < B; b++) cin[b] = rch(b);
     ri ^= 1;
-    const StepTr st{cin, cout, obs, n_obs, log_obs};
-    KCHK(round_queries(st, c->rb.grind, rp.pow_bits, rp.domain_size, cfg.ff(round), rp.num_queries,
-                       rp.folded_domain_gen, n_ood, c->d_rlog, c->rb, c->d_c16, c->st));
-    cin = cout;
   }
-  // openings of the previous commitment into h_open, read after the round
-  KCHK(merkle_open_dev(prev->rows.p, prev->row_len, prev->tree.p, prev->height, c->rb.idx, c->rb.nq,
-                       rp.num_queries, c->d_open, c->st));
+  {
+    // openings of the previous commitment into h_open, read after the round
+    std::vector<OpenArgs> op(B);
+    for (int b = 0; b < B; b++) {
+      const whirgpu_prover_data* prev = R[b]->prev;
+      op[b] = OpenArgs{prev->rows.p, prev->tree.p, R[b]->sl->rb.idx, R[b]->sl->rb.nq, R[b]->sl->d_open};
+    }
+    KCHK(merkle_open(op.data(), B, R[0]->prev->row_len, R[0]->prev->height, rp.num_queries, c->st));
+  }
   // Constraint::combine_packed (src/whir/constraints/mod.rs:234-248): the OOD
   // points' eq (the tables of the OOD steps) weighted by gamma^i, then the
   // queries' select terms
-  for (int b = 0; b < n_ood; b += 4) {
-    EqTables t{};
-    const int kk = std::min(4, n_ood - b);
-    for (int k = 0; k < kk; k++) {
-      ef* base = c->d_tabs + (size_t)(b + k) * c->tab_cap;
-      t.hi[k] = base;
-      t.lo[k] = base + (1ull << hi);
+  std::vector<CombineArgs> ca(B);
+  for (int b0 = 0; b0 < n_ood; b0 += 4) {
+    const int kk = std::min(4, n_ood - b0);
+    for (int b = 0; b < B; b++) {
+      Slot* sl = R[b]->sl;
+      ca[b] = CombineArgs{R[b]->s.w.p, EqTables{}, sl->rb.gpow + b0};
+      for (int k = 0; k < kk; k++) {
+        ef* base = sl->d_tabs + (size_t)(b0 + k) * sl->tab_cap;
+        ca[b].tabs.hi[k] = base;
+        ca[b].tabs.lo[k] = base + (1ull << hi);
+      }
     }
-    Span sp(c, ST_COMBINE, 32.0 * (double)(1ull << m));
-    KCHK(eq_combine(s.w.p, 1ull << m, lo, t, kk, 1, c->st, c->rb.gpow + b));
+    Span sp(c, ST_COMBINE, 32.0 * (double)(1ull << m) * B);
+    KCHK(eq_combine(ca.data(), B, 1ull << m, lo, kk, 1, c->st));
   }
   {
     const int ns = rp.num_queries;
-    Buf<ef> A = c->alloc<ef>((size_t)ns << (m - m / 2));
-    Buf<u32> B = c->alloc<u32>((size_t)ns << (m / 2));
-    {
-      Span sp(c, ST_COMBINE, 32.0 * (double)(1ull << m));
-      KCHK(select_combine(s.w.p, m, c->rb.selz, c->rb.gpow + n_ood, ns, c->rb.nq, A.p, B.p, c->st));
+    std::vector<Buf<ef>> As(B);
+    std::vector<Buf<u32>> Bs(B);
+    std::vector<SelectArgs> sa(B);
+    for (int b = 0; b < B; b++) {
+      Slot* sl = R[b]->sl;
+      As[b] = c->alloc<ef>((size_t)ns << (m - m / 2));
+      Bs[b] = c->alloc<u32>((size_t)ns << (m / 2));
+      sa[b] = SelectArgs{R[b]->s.w.p, sl->rb.selz, sl->rb.gpow + n_ood, sl->rb.nq, As[b].p, Bs[b].p};
     }
-    c->release(A);   // reuse is ordered on the same stream
-    c->release(B);
+    {
+      Span sp(c, ST_COMBINE, 32.0 * (double)(1ull << m) * B);
+      KCHK(select_combine(sa.data(), B, m, ns, c->st));
+    }
+    for (int b = 0; b < B; b++) {   // reuse is ordered on the same stream
+      c->release(As[b]);
+      c->release(Bs[b]);
+    }
   }
-  s.invalidate();
-  rl.la = sumcheck_launch(c, s, kn, rp.folding_pow_bits, cin);
+  rl.la = sumcheck_launch(c, R, kn, rp.folding_pow_bits, cin);
   HIPCHK(hipEventRecord(c->ev_round, c->st));
   return rl;
 }
 
-// Waits for the round launched by device_round_launch (not for what was
-// enqueued after it) and replays its transcript on the host challenger.
-std::vector<ef> device_round_replay(whirgpu_ctx* c, SumcheckState& s, const whir::WhirConfig& cfg, int round,
-                                    const whirgpu_prover_data* pd, const whirgpu_prover_data* prev,
-                                    const std::vector<ef>& fold_rand, whir::WhirRoundProof& R, DuplexChallenger& ch,
-                                    const RoundLaunch& rl) {
+// After the round launched by device_round_launch has completed (not what
+// was enqueued after it): replays one proof's round on its host challenger.
+void device_round_replay(whirgpu_ctx* c, Run& r, int round, const RoundLaunch& rl) {
+  const auto& cfg = c->cfg;
   const auto& rp = cfg.round_parameters[round];
   const int n_ood = rp.ood_samples;
-  HIPCHK(hipEventSynchronize(c->ev_round));
-  R.commitment = read_root(c, pd, true);
-  observe_digest(ch, R.commitment);
-  const RoundLog& L = *c->h_rlog;
+  whir::WhirRoundProof& RP = r.P->rounds[round];
+  RP.commitment = read_root(c, r.cur, true);
+  observe_digest(r.ch, RP.commitment);
+  const RoundLog& L = *r.sl->h_rlog;
   for (int i = 0; i < n_ood; i++) {
-    const ef y = ch.sample_ef();
+    const ef y = r.ch.sample_ef();
     if (!ef_eq(y, L.y[i])) bad("device round transcript diverged (OOD challenge)", WHIRGPU_BAD_STATE);
-    ch.observe_ef(L.ev[i]);
-    R.ood_answers.push_back(L.ev[i]);
+    r.ch.observe_ef(L.ev[i]);
+    RP.ood_answers.push_back(L.ev[i]);
   }
   if (rp.pow_bits > 0) {
     if (L.err) bad("device round transcript: no valid proof-of-work witness", WHIRGPU_GRIND_FAILED);
-    if (!ch.check_witness(rp.pow_bits, L.witness)) bad("device round transcript: witness rejected", WHIRGPU_GRIND_FAILED);
-    R.pow_witness = L.witness;
-    if (rl.grind_rec != SIZE_MAX) c->recs[rl.grind_rec].work = (double)from_monty(L.witness) + 1;
+    if (!r.ch.check_witness(rp.pow_bits, L.witness))
+      bad("device round transcript: witness rejected", WHIRGPU_GRIND_FAILED);
+    RP.pow_witness = L.witness;
+    if (rl.grind_rec != SIZE_MAX) c->recs[rl.grind_rec].work += (double)from_monty(L.witness) + 1;
   }
-  ch.sample();
-  const auto idx = whir::get_challenge_stir_queries(rp.domain_size, cfg.ff(round), rp.num_queries, ch);
+  r.ch.sample();
+  const auto idx = whir::get_challenge_stir_queries(rp.domain_size, cfg.ff(round), rp.num_queries, r.ch);
   if (idx.size() != L.nq) bad("device round transcript diverged (query count)", WHIRGPU_BAD_STATE);
   for (size_t q = 0; q < idx.size(); q++)
     if (idx[q] != L.idx[q]) bad("device round transcript diverged (query index)", WHIRGPU_BAD_STATE);
   std::vector<std::vector<ef>> rows;
-  parse_openings(c, prev, (int)idx.size(), &R.queries, &rows);
-  const ef gamma = ch.sample_ef();
+  parse_openings(r.prev, r.sl->h_open, (int)idx.size(), &RP.queries, &rows);
+  const ef gamma = r.ch.sample_ef();
   if (!ef_eq(gamma, L.gamma)) bad("device round transcript diverged (gamma)", WHIRGPU_BAD_STATE);
   const auto gp = whir::powers(gamma, (size_t)n_ood + idx.size());
-  for (int i = 0; i < n_ood; i++) s.sum = ef_add(s.sum, ef_mul(gp[i], L.ev[i]));
+  for (int i = 0; i < n_ood; i++) r.s.sum = ef_add(r.s.sum, ef_mul(gp[i], L.ev[i]));
   for (size_t j = 0; j < idx.size(); j++)
-    s.sum = ef_add(s.sum, ef_mul(gp[n_ood + j], whir::mle_host(rows[j], fold_rand)));
-  return sumcheck_replay(c, s, rl.la, R.sumcheck, ch);
+    r.s.sum = ef_add(r.s.sum, ef_mul(gp[n_ood + j], whir::mle_host(rows[j], r.fold_rand)));
+  r.fold_rand = sumcheck_replay(c, r, rl.la, RP.sumcheck);
 }
 
 // Whether the final round's steps fit the device transcript: the device
 // observes the final polynomial word by word, so it must be small.  Its
 // openings use the second half of the opening buffer.
 bool device_final_fits(const whirgpu_ctx* c, const whir::WhirConfig& cfg, int m, const whirgpu_prover_data* prev) {
-  if (!prev || m < 0 || m > whirgpu_ctx::FINAL_MAX_LOG) return false;
+  if (!prev || m < 0 || m > Slot::FINAL_MAX_LOG) return false;
   if (cfg.final_queries < 1 || cfg.final_queries > RT_MAX_QUERIES) return false;
-  if (cfg.final_sumcheck_rounds > whirgpu_ctx::SC_LOG_CAP / 2) return false;
+  if (cfg.final_sumcheck_rounds > Slot::SC_LOG_CAP / 2) return false;
   const size_t per = (size_t)prev->row_len + 8ull * ilog2u(prev->height);
   return per * (size_t)cfg.final_queries <= c->open_cap / 2;
 }
@@ -1019,95 +1441,321 @@ struct FinalLaunch {
 };
 
 // The final round (Prover::final_round, prover/mod.rs:330-417) with the
-// transcript on the device: observe the final polynomial, the proof of work,
-// the final STIR queries and their openings, and the final sumcheck, as one
-// stream of kernels.  It starts from the device challenger `cin` (the last
-// round's, so it can be enqueued before that round is replayed) or, without
-// one, from the host challenger.  Its log, openings and sumcheck log use
-// their own slots (RoundLog 1, the opening buffer's second half, ScLog
-// entries from SC_LOG_CAP / 2), apart from the last round's.
-FinalLaunch device_final_launch(whirgpu_ctx* c, SumcheckState& s, const whir::WhirConfig& cfg, int round,
-                                const whirgpu_prover_data* prev, const DuplexChallenger& ch,
-                                const DevChallenger* cin) {
+// transcript on the device, for every proof of the group: observe the final
+// polynomial, the proof of work, the final STIR queries and their openings,
+// and the final sumcheck, as one stream of launches.  Proof b starts from the
+// device challenger cin[b] (the last round's, so it can be enqueued before
+// that round is replayed) or, with cin empty, from its host challenger.  Its
+// log, openings and sumcheck log use their own slots (RoundLog 1, the opening
+// buffer's second half, ScLog entries from SC_LOG_CAP / 2), apart from the
+// last round's.
+FinalLaunch device_final_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int round,
+                                std::vector<const DevChallenger*> cin) {
+  const int B = (int)R.size();
+  const auto& cfg = c->cfg;
   FinalLaunch fl;
-  const int m = s.m;
+  const int m = R[0]->s.m;
   fl.nf = 1ull << m;
   const int pow = cfg.final_pow_bits;
   const auto fc = cfg.final_round_config();
-  if (!cin) {
-    upload_challenger(c, ch);
-    cin = c->d_chal_init;
+  if (cin.empty()) {
+    for (Run* r : R) {
+      upload_challenger(r->sl, r->ch);
+      cin.push_back(r->sl->d_chal_init);
+    }
   }
-  DevChallenger* rch[2] = {c->d_rchal, c->d_rchal + 1};
   int ri = 0;
-  HIPCHK(hipMemcpyAsync(c->h_final, s.e.p, fl.nf * sizeof(ef), hipMemcpyDeviceToHost, c->st));
-  const u32* obs = reinterpret_cast<const u32*>(s.e.p);
+  auto rch = [&](int b) { return R[b]->sl->d_rchal + ri; };
+  for (Run* r : R)
+    HIPCHK(hipMemcpyAsync(r->sl->h_final, r->s.e.p, fl.nf * sizeof(ef), hipMemcpyDeviceToHost, c->st));
   int n_obs = (int)(4 * fl.nf);
+  bool observe_final = true;
   if (pow > 0) {
-    HIPCHK(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(c->rb.grind), (int)P, 1, c->st));
-    DevChallenger* cout = rch[ri];
-    ri ^= 1;
-    ScTr tr{};
-    tr.ch_in = cin;
-    tr.ch_out = cout;
-    tr.grind = c->rb.grind;
-    tr.pow_bits = pow;
-    tr.obs = obs;
-    tr.n_obs = n_obs;
-    tr.gbase = c->d_gbase;
+    std::vector<GrindArgs> ga(B);
+    for (int b = 0; b < B; b++) {
+      Slot* sl = R[b]->sl;
+      HIPCHK(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(sl->rb.grind), (int)P, 1, c->st));
+      memset(&ga[b], 0, sizeof ga[b]);
+      ScTr& tr = ga[b].tr;
+      tr.ch_in = cin[b];
+      tr.ch_out = rch(b);
+      tr.grind = sl->rb.grind;
+      tr.pow_bits = pow;
+      tr.obs = reinterpret_cast<const u32*>(R[b]->s.e.p);
+      tr.n_obs = n_obs;
+      tr.gbase = sl->d_gbase;
+      ga[b].result = tr.grind;
+    }
     Span sp(c, ST_GRIND, 0);
-    KCHK(grind_range_tr(c->d_c16, tr, pow, 0, P, sc_grind_blocks(c, pow), c->st, TR_OBS_GRIND));
-    sp.close();
-    if (c->prof) fl.grind_rec = c->recs.size() - 1;
-    cin = cout;
-    obs = nullptr;
+    KCHK(grind_range(c->d_c16, ga.data(), B, pow, TR_OBS_GRIND, 0, P, grind_blocks_for(c, pow), c->st));
+    fl.grind_rec = sp.close();
+    for (int b = 0; b < B; b++) cin[b] = rch(b);
+    ri ^= 1;
+    observe_final = false;
     n_obs = 0;
   }
   {
-    DevChallenger* cout = rch[ri];
+    std::vector<QueryArgs> qa(B);
+    for (int b = 0; b < B; b++) {
+      const u32* obs = observe_final ? reinterpret_cast<const u32*>(R[b]->s.e.p) : nullptr;
+      qa[b] = QueryArgs{StepTr{cin[b], rch(b), obs, n_obs, nullptr}, R[b]->sl->rb.grind, R[b]->sl->d_rlog + 1,
+                        R[b]->sl->rb};
+    }
+    KCHK(round_queries(qa.data(), B, pow, fc.domain_size, cfg.ff(round), cfg.final_queries, 1u, 0, c->d_c16, c->st,
+                       true));
+    for (int b = 0; b < B; b++) cin[b] = rch(b);
     ri ^= 1;
-    const StepTr st{cin, cout, obs, n_obs, nullptr};
-    KCHK(round_queries(st, c->rb.grind, pow, fc.domain_size, cfg.ff(round), cfg.final_queries, 1u, 0, c->d_rlog + 1,
-                       c->rb, c->d_c16, c->st, true));
-    cin = cout;
   }
-  KCHK(merkle_open_dev(prev->rows.p, prev->row_len, prev->tree.p, prev->height, c->rb.idx, c->rb.nq,
-                       cfg.final_queries, c->d_open + c->open_cap / 2, c->st));
+  {
+    std::vector<OpenArgs> op(B);
+    for (int b = 0; b < B; b++) {
+      const whirgpu_prover_data* prev = R[b]->prev;
+      op[b] = OpenArgs{prev->rows.p, prev->tree.p, R[b]->sl->rb.idx, R[b]->sl->rb.nq,
+                       R[b]->sl->d_open + c->open_cap / 2};
+    }
+    KCHK(merkle_open(op.data(), B, R[0]->prev->row_len, R[0]->prev->height, cfg.final_queries, c->st));
+  }
   if (cfg.final_sumcheck_rounds > 0)
-    fl.la = sumcheck_launch(c, s, cfg.final_sumcheck_rounds, cfg.final_folding_pow_bits, cin,
-                            whirgpu_ctx::SC_LOG_CAP / 2);
+    fl.la = sumcheck_launch(c, R, cfg.final_sumcheck_rounds, cfg.final_folding_pow_bits, cin, Slot::SC_LOG_CAP / 2);
   return fl;
 }
 
-// Waits for everything enqueued and replays the final round on the host challenger.
-void device_final_replay(whirgpu_ctx* c, SumcheckState& s, const whir::WhirConfig& cfg, int round,
-                         const whirgpu_prover_data* prev, whir::WhirProof& Pf, DuplexChallenger& ch,
-                         const FinalLaunch& fl) {
+// After everything enqueued has completed: replays one proof's final round on
+// its host challenger.
+void device_final_replay(whirgpu_ctx* c, Run& r, int round, const FinalLaunch& fl) {
+  const auto& cfg = c->cfg;
   const int pow = cfg.final_pow_bits;
   const auto fc = cfg.final_round_config();
-  c->sync();
-  Pf.final_poly.assign(c->h_final, c->h_final + fl.nf);
+  whir::WhirProof& Pf = *r.P;
+  Pf.final_poly.assign(r.sl->h_final, r.sl->h_final + fl.nf);
   Pf.has_final_poly = true;
-  for (const ef& e : Pf.final_poly) ch.observe_ef(e);
-  const RoundLog& L = c->h_rlog[1];
+  for (const ef& e : Pf.final_poly) r.ch.observe_ef(e);
+  const RoundLog& L = r.sl->h_rlog[1];
   if (pow > 0) {
     if (L.err) bad("device final round: no valid proof-of-work witness", WHIRGPU_GRIND_FAILED);
-    if (!ch.check_witness(pow, L.witness)) bad("device final round: witness rejected", WHIRGPU_GRIND_FAILED);
+    if (!r.ch.check_witness(pow, L.witness)) bad("device final round: witness rejected", WHIRGPU_GRIND_FAILED);
     Pf.final_pow_witness = L.witness;
-    if (fl.grind_rec != SIZE_MAX) c->recs[fl.grind_rec].work = (double)from_monty(L.witness) + 1;
+    if (fl.grind_rec != SIZE_MAX) c->recs[fl.grind_rec].work += (double)from_monty(L.witness) + 1;
   }
-  const auto idx = whir::get_challenge_stir_queries(fc.domain_size, cfg.ff(round), cfg.final_queries, ch);
+  const auto idx = whir::get_challenge_stir_queries(fc.domain_size, cfg.ff(round), cfg.final_queries, r.ch);
   if (idx.size() != L.nq) bad("device final round diverged (query count)", WHIRGPU_BAD_STATE);
   for (size_t q = 0; q < idx.size(); q++)
     if (idx[q] != L.idx[q]) bad("device final round diverged (query index)", WHIRGPU_BAD_STATE);
   std::vector<std::vector<ef>> rows;
-  parse_openings(c, prev, (int)idx.size(), &Pf.final_queries, &rows, c->h_open + c->open_cap / 2);
+  parse_openings(r.prev, r.sl->h_open + c->open_cap / 2, (int)idx.size(), &Pf.final_queries, &rows);
   if (cfg.final_sumcheck_rounds > 0) {
     Pf.has_final_sumcheck = true;
-    sumcheck_replay(c, s, fl.la, Pf.final_sumcheck, ch);
+    r.fold_rand = sumcheck_replay(c, r, fl.la, Pf.final_sumcheck);
   }
 }
 
+// A WHIR round's steps driven by the host, for one proof (shapes beyond the
+// device transcript's buffers): OOD samples, proof of work, STIR queries,
+// gamma, the weight updates and the round's sumcheck.
+void host_round(whirgpu_ctx* c, Run& r, int round) {
+  const auto& cfg = c->cfg;
+  const auto& rp = cfg.round_parameters[round];
+  const int m = r.s.m;
+  const int kn = cfg.ff(round + 1);
+  whir::WhirRoundProof& RP = r.P->rounds[round];
+  RP.commitment = read_root(c, r.cur);
+  observe_digest(r.ch, RP.commitment);
+  // OOD samples (prover/mod.rs:192-205)
+  std::vector<std::vector<ef>> ood_pts;
+  for (int i = 0; i < rp.ood_samples; i++) {
+    auto pt = whir::expand_from_univariate(r.ch.sample_ef(), m);
+    const ef ev = ext_mle(c, r.sl, r.s.e.p, m, pt);
+    r.ch.observe_ef(ev);
+    RP.ood_answers.push_back(ev);
+    ood_pts.push_back(pt);
+  }
+  if (rp.pow_bits > 0) RP.pow_witness = gpu_grind(c, {&r.ch}, rp.pow_bits)[0];
+  r.ch.sample();
+  // STIR queries on the previous commitment (prover/mod.rs:229-303)
+  auto idx = whir::get_challenge_stir_queries(rp.domain_size, cfg.ff(round), rp.num_queries, r.ch);
+  std::vector<std::vector<ef>> rows;
+  open_rows(c, r.sl, r.prev, idx, &RP.queries, &rows);
+  std::vector<u32> sel_z;
+  std::vector<ef> sel_v;
+  for (size_t q = 0; q < idx.size(); q++) {
+    sel_z.push_back(kb::pow(rp.folded_domain_gen, idx[q]));
+    sel_v.push_back(whir::mle_host(rows[q], r.fold_rand));
+  }
+  // Constraint::combine_packed (src/whir/constraints/mod.rs:234-248)
+  const ef gamma = r.ch.sample_ef();
+  const auto gp = whir::powers(gamma, ood_pts.size() + sel_z.size());
+  std::vector<ef> eq_scales(gp.begin(), gp.begin() + ood_pts.size());
+  if (!ood_pts.empty()) add_eq_weights(c, {&r}, {ood_pts}, {eq_scales}, true);
+  for (size_t i = 0; i < ood_pts.size(); i++) r.s.sum = ef_add(r.s.sum, ef_mul(gp[i], RP.ood_answers[i]));
+  if (!sel_z.empty()) {
+    const size_t ns = sel_z.size();
+    std::vector<ef> g(gp.begin() + ood_pts.size(), gp.end());
+    // the points and weights in one staged upload: weights first (16-byte
+    // aligned), then the points
+    auto gs = c->stage<ef>(ns + (ns + 3) / 4);
+    u32* zh = reinterpret_cast<u32*>(gs.first + ns);
+    const u32* zd = reinterpret_cast<const u32*>(gs.second + ns);
+    memcpy(gs.first, g.data(), ns * sizeof(ef));
+    memcpy(zh, sel_z.data(), ns * sizeof(u32));
+    c->flush();
+    const std::pair<const u32*, const ef*> zs_dev{zd, gs.second};
+    // sized for the round's query count (ns <= num_queries after
+    // deduplication), so every proof of this shape reuses one pool pair
+    const int lo = m / 2, hi = m - lo;
+    const size_t nmax = std::max<size_t>(ns, (size_t)rp.num_queries);
+    Buf<ef> A = c->alloc<ef>(nmax << hi);
+    Buf<u32> Bt = c->alloc<u32>(nmax << lo);
+    const SelectArgs sa{r.s.w.p, zs_dev.first, zs_dev.second, nullptr, A.p, Bt.p};
+    {
+      Span sp(c, ST_COMBINE, 32.0 * (double)(1ull << m));
+      KCHK(select_combine(&sa, 1, m, (int)ns, c->st));
+    }
+    c->release(A);   // reuse is ordered on the same stream
+    c->release(Bt);
+    for (size_t j = 0; j < ns; j++) r.s.sum = ef_add(r.s.sum, ef_mul(g[j], sel_v[j]));
+  }
+  device_sumcheck(c, {&r}, kn, rp.folding_pow_bits, [&](Run&) -> whir::SumcheckData& { return RP.sumcheck; });
+}
+
+// The final round driven by the host, for one proof (final_round, prover/mod.rs:330-417).
+void host_final(whirgpu_ctx* c, Run& r, int round) {
+  const auto& cfg = c->cfg;
+  whir::WhirProof& Pf = *r.P;
+  Pf.final_poly.resize(1ull << r.s.m);
+  c->d2h(Pf.final_poly.data(), r.s.e.p, Pf.final_poly.size());
+  Pf.has_final_poly = true;
+  for (const ef& e : Pf.final_poly) r.ch.observe_ef(e);
+  if (cfg.final_pow_bits > 0) Pf.final_pow_witness = gpu_grind(c, {&r.ch}, cfg.final_pow_bits)[0];
+  const auto fc = cfg.final_round_config();
+  const auto idx = whir::get_challenge_stir_queries(fc.domain_size, cfg.ff(round), cfg.final_queries, r.ch);
+  std::vector<std::vector<ef>> rows;
+  open_rows(c, r.sl, r.prev, idx, &Pf.final_queries, &rows);
+  if (cfg.final_sumcheck_rounds > 0) {
+    Pf.has_final_sumcheck = true;
+    device_sumcheck(c, {&r}, cfg.final_sumcheck_rounds, cfg.final_folding_pow_bits,
+                    [&](Run&) -> whir::SumcheckData& { return Pf.final_sumcheck; });
+  }
+}
+
+// CommitmentWriter::commit (src/whir/committer/writer.rs:56-109) for every
+// proof of the group: the base LDE and Merkle tree of all polynomials in one
+// launch set, the roots observed, and the commitment's OOD samples evaluated
+// for all proofs at once.  Each run's commitment goes to run.prev.
+void commit_group(whirgpu_ctx* c, const std::vector<Run*>& R) {
+  const int B = (int)R.size();
+  const auto& cfg = c->cfg;
+  const int n = cfg.num_variables, k0 = cfg.ff(0), Rr = cfg.params.starting_log_inv_rate;
+  const u64 H = 1ull << (n + Rr - k0), W = 1ull << k0;
+  const bool two = lde_two_pass(n, k0, false);
+  std::vector<Buf<u32>> tmps(B);
+  std::vector<LdeArgs<u32>> la(B);
+  std::vector<whirgpu_prover_data*> pds(B);
+  for (int b = 0; b < B; b++) {
+    auto* pd = new whirgpu_prover_data();
+    R[b]->prev = pd;   // destroyed by the caller if anything below throws
+    pd->ctx = c;
+    pd->sl = R[b]->sl;
+    pd->rows = c->alloc<u32>(H * W);
+    pd->tree = c->alloc<u32>((2 * H - 1) * 8);
+    pd->height = H;
+    pd->row_len = (int)W;
+    pd->ext = false;
+    if (two) tmps[b] = c->alloc<u32>(H * W);
+    la[b] = LdeArgs<u32>{R[b]->st->d_poly, tmps[b].p, pd->rows.p};
+    pds[b] = pd;
+  }
+  {
+    Span sp(c, ST_LDE_BASE, (4.0 * (double)(1ull << n) + 4.0 * (double)(H * W)) * B);
+    KCHK(lde_base(la.data(), B, n, k0, Rr, c->tw, c->st));
+  }
+  commit_trees(c, pds);
+  for (auto& t : tmps)
+    if (t.p) c->release(t);
+  if (pds[0]->root_on_host) c->sync();
+  for (int b = 0; b < B; b++) {
+    const Digest root = read_root(c, pds[b], true);
+    R[b]->P->has_initial_commitment = true;
+    R[b]->P->initial_commitment = root;
+    observe_digest(R[b]->ch, root);
+  }
+  std::vector<whirgpu_statement*> sts(B);
+  for (int b = 0; b < B; b++) sts[b] = R[b]->st;
+  for (int i = 0; i < cfg.commitment_ood_samples; i++) {
+    std::vector<std::vector<ef>> pts(B);
+    for (int b = 0; b < B; b++) pts[b] = whir::expand_from_univariate(R[b]->ch.sample_ef(), n);
+    const auto evs = statement_evaluate(c, sts, pts);
+    for (int b = 0; b < B; b++) {
+      R[b]->P->initial_ood_answers.push_back(evs[b]);
+      R[b]->ch.observe_ef(evs[b]);
+    }
+  }
+}
+
+// Prover::prove (src/whir/prover/mod.rs:97-417) for every proof of the group,
+// each starting from its commitment in run.prev (consumed).
+void prove_group(whirgpu_ctx* c, const std::vector<Run*>& R) {
+  const int B = (int)R.size();
+  const auto& cfg = c->cfg;
+  const int n = cfg.num_variables;
+  for (Run* r : R) r->P->rounds.assign(cfg.n_rounds(), whir::WhirRoundProof());
+  initial_sumcheck(c, R, cfg.ff(0), cfg.starting_folding_pow_bits);
+  FinalLaunch fl;
+  bool final_launched = false;
+  for (int round = 0; round <= cfg.n_rounds(); round++) {
+    if (final_launched) {   // enqueued after the last round (its sumcheck already advanced s)
+      c->sync();
+      for (Run* r : R) device_final_replay(c, *r, round, fl);
+      break;
+    }
+    const int m = n - cfg.params.folding_factor.total_number(round);
+    for (Run* r : R)
+      if (m != r->s.m) bad("internal: round size mismatch", WHIRGPU_BAD_STATE);
+    if (round == cfg.n_rounds()) {
+      if (device_final_fits(c, cfg, m, R[0]->prev)) {
+        fl = device_final_launch(c, R, round, {});
+        c->sync();
+        for (Run* r : R) device_final_replay(c, *r, round, fl);
+      } else {
+        for (Run* r : R) host_final(c, *r, round);
+      }
+      break;
+    }
+    const auto& rp = cfg.round_parameters[round];
+    const int kn = cfg.ff(round + 1);
+    if (!R[0]->next) commit_round(c, R, round);
+    for (Run* r : R) {
+      r->cur = r->next;
+      r->next = nullptr;
+    }
+    if (device_round_fits(c, rp, m, R[0]->prev)) {
+      const RoundLaunch rl = device_round_launch(c, R, round, kn);
+      // the next commitment depends only on the folded polynomial; after
+      // the last round, the final round continues from the device state
+      if (round + 1 < cfg.n_rounds()) {
+        commit_round(c, R, round + 1);
+      } else if (kn <= Slot::SC_LOG_CAP / 2 && device_final_fits(c, cfg, m - kn, R[0]->cur)) {
+        std::vector<Run*> RF = R;
+        for (Run* r : RF) {   // the final round opens this round's commitment
+          std::swap(r->prev, r->cur);
+        }
+        fl = device_final_launch(c, RF, round + 1, rl.la.out);
+        for (Run* r : RF) std::swap(r->prev, r->cur);
+        final_launched = true;
+      }
+      HIPCHK(hipEventSynchronize(c->ev_round));
+      for (Run* r : R) device_round_replay(c, *r, round, rl);
+    } else {
+      for (Run* r : R) host_round(c, *r, round);
+    }
+    for (Run* r : R) {
+      whirgpu_prover_data_destroy(r->prev);
+      r->prev = r->cur;
+      r->cur = nullptr;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  (void)B;
+}
 double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -1196,6 +1844,61 @@ void make_twiddles(whirgpu_ctx* c) {
 
 }  // namespace
 
+namespace {
+
+// The initial sumcheck of one constraint with challenges from `next` (no
+// grinding): the split route for the first l0 rounds when n > l0, classic
+// rounds otherwise; used by the kernel-level sumcheck entry point.
+std::vector<ef> initial_sumcheck_scripted(whirgpu_ctx* c, Run& r, int l0, const ChallengeFn& next) {
+  const whirgpu_statement* st = r.st;
+  const std::vector<ef> ap(st->points.size(), ef_one());
+  r.s.sum = ef_zero();
+  for (size_t j = 0; j < st->points.size(); j++) r.s.sum = ef_add(r.s.sum, st->evals[j]);
+  std::vector<ef> rs;
+  if (l0 > 0 && st->seg_bits == l0 && st->n > l0) {
+    auto P = st->partials;
+    for (int i = 0; i < l0; i++) {
+      ef c0, c2;
+      svo_round_poly(st, P, ap, rs, i, l0, &c0, &c2);
+      const ef x = next(c0, c2);
+      r.s.sum = whir::extrapolate_012(c0, ef_sub(r.s.sum, c0), c2, x);
+      rs.push_back(x);
+      svo_fold_partials(P, x);
+    }
+    svo_finish(c, {&r}, l0, {ap}, {rs});
+    return rs;
+  }
+  const int n = st->n;
+  r.s.m = n;
+  r.s.e = c->alloc<ef>(1ull << n);
+  r.s.w = c->alloc<ef>(1ull << n);
+  const ExtArgs xa{st->d_poly, r.s.e.p};
+  KCHK(base_to_ext(&xa, 1, 1ull << n, c->st));
+  add_eq_weights(c, {&r}, {st->points}, {ap}, false);
+  for (int i = 0; i < l0; i++) rs.push_back(sumcheck_round(c, r.sl, r.s, next));
+  return rs;
+}
+
+// Releases what a failed or finished group still holds.
+void release_runs(whirgpu_ctx* c, std::vector<Run>& runs) {
+  for (Run& r : runs) {
+    for (whirgpu_prover_data** p : {&r.next, &r.cur, &r.prev}) {
+      if (*p) whirgpu_prover_data_destroy(*p);
+      *p = nullptr;
+    }
+    if (r.s.e.p) c->release(r.s.e);
+    if (r.s.w.p) c->release(r.s.w);
+  }
+}
+
+std::vector<ef> read_point(const uint32_t* p, uint32_t n) {
+  std::vector<ef> z(n);
+  for (uint32_t i = 0; i < n; i++) memcpy(z[i].c, p + 4 * i, 16);
+  return z;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* whirgpu_last_error(void) { return g_err.c_str(); }
@@ -1244,6 +1947,7 @@ int whirgpu_poseidon2_permute(const whirgpu_poseidon2* perms, uint32_t width, ui
   });
 }
 
+
 int whirgpu_create(int device, const whirgpu_params* params, const whirgpu_poseidon2* perms, whirgpu_ctx** out) {
   whirgpu_ctx* c = nullptr;
   int rc = guarded([&] {
@@ -1270,26 +1974,10 @@ int whirgpu_create(int device, const whirgpu_params* params, const whirgpu_posei
     HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&c->ev_round, hipEventDisableTiming));
     make_twiddles(c);
-    int n = (int)params->num_variables;
-    c->fixed_bytes = 8192 * sizeof(u32) + 8 * ((1ull << std::max(0, n - 8)) + 512) * sizeof(ef) +
-                     (1 << 15) * sizeof(ef) + 2 * 64 * sizeof(u32) + whirgpu_ctx::GRIND_SLOTS * sizeof(u32) +
-                     sizeof(P2C16) + sizeof(P2C24) + 4 * sizeof(DevChallenger) + sizeof(GrindBase);
+    const int n = (int)params->num_variables;
     c->tab_cap = (1ull << std::max(0, n - 8)) + 512;
-    HIPCHK(hipMalloc(&c->d_tabs, 8 * c->tab_cap * sizeof(ef)));
     c->partial_cap = 1 << 15;
-    HIPCHK(hipMalloc(&c->d_partial, c->partial_cap * sizeof(ef)));
-    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_small), 4096 * sizeof(ef), hipHostMallocCoherent));
-    HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->d_small), c->h_small, 0));
-    auto coherent = [&](void** h, void** d, size_t bytes) {
-      HIPCHK(hipHostMalloc(h, bytes, hipHostMallocCoherent));
-      HIPCHK(hipHostGetDevicePointer(d, *h, 0));
-    };
-    coherent(reinterpret_cast<void**>(&c->h_idx), reinterpret_cast<void**>(&c->d_idx), 4096 * sizeof(u64));
     c->open_cap = 1 << 20;
-    coherent(reinterpret_cast<void**>(&c->h_open), reinterpret_cast<void**>(&c->d_open), c->open_cap * sizeof(u32));
-    coherent(reinterpret_cast<void**>(&c->h_root), reinterpret_cast<void**>(&c->d_root), 64 * sizeof(u32));
-    HIPCHK(hipMalloc(&c->d_grind, 2 * 64 * sizeof(u32)));   // two results, 256 bytes apart
-    HIPCHK(hipMalloc(&c->d_gslots, whirgpu_ctx::GRIND_SLOTS * sizeof(u32)));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device));
     // One 256-lane block per CU: one wave on each of its four SIMDs.  More
@@ -1297,36 +1985,21 @@ int whirgpu_create(int device, const whirgpu_params* params, const whirgpu_posei
     // lengthen each pass over the candidates.
     c->grind_blocks = std::max(1, prop.multiProcessorCount);
     HIPCHK(hipMalloc(&c->d_c16, sizeof(P2C16)));
-    HIPCHK(hipMalloc(&c->d_chal, 2 * sizeof(DevChallenger)));
-    coherent(reinterpret_cast<void**>(&c->h_sclog), reinterpret_cast<void**>(&c->d_sclog),
-             whirgpu_ctx::SC_LOG_CAP * sizeof(ScLog));
-    coherent(reinterpret_cast<void**>(&c->h_chal), reinterpret_cast<void**>(&c->d_chal_init), sizeof(DevChallenger));
-    coherent(reinterpret_cast<void**>(&c->h_rlog), reinterpret_cast<void**>(&c->d_rlog), 2 * sizeof(RoundLog));
-    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_final), sizeof(ef) << whirgpu_ctx::FINAL_MAX_LOG,
-                         hipHostMallocDefault));
-    HIPCHK(hipMalloc(&c->d_rchal, 2 * sizeof(DevChallenger)));
-    HIPCHK(hipMalloc(&c->d_gbase, sizeof(GrindBase)));
-    {
-      const size_t pts = sizeof(ef) * RT_MAX_OOD * RT_MAX_VARS, ev = sizeof(ef) * RT_MAX_OOD,
-                   idx = sizeof(u64) * RT_MAX_QUERIES, selz = sizeof(u32) * RT_MAX_QUERIES,
-                   gp = sizeof(ef) * (RT_MAX_OOD + RT_MAX_QUERIES), gr = sizeof(u32) * 64, nq = 256;
-      HIPCHK(hipMalloc(&c->d_rbuf, pts + ev + idx + selz + gp + gr + nq));
-      c->fixed_bytes += pts + ev + idx + selz + gp + gr + nq;
-      char* b = static_cast<char*>(c->d_rbuf);
-      c->rb.pts = reinterpret_cast<ef*>(b);
-      c->rb.ev = reinterpret_cast<ef*>(b + pts);
-      c->rb.idx = reinterpret_cast<u64*>(b + pts + ev);
-      c->rb.selz = reinterpret_cast<u32*>(b + pts + ev + idx);
-      c->rb.gpow = reinterpret_cast<ef*>(b + pts + ev + idx + selz);
-      c->rb.grind = reinterpret_cast<u32*>(b + pts + ev + idx + selz + gp);
-      c->rb.nq = reinterpret_cast<u32*>(b + pts + ev + idx + selz + gp + gr);
-    }
     HIPCHK(hipMemcpy(c->d_c16, &c->c16, sizeof(P2C16), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&c->d_c24, sizeof(P2C24)));
     HIPCHK(hipMemcpy(c->d_c24, &c->c24, sizeof(P2C24), hipMemcpyHostToDevice));
-    c->pin<u32>(1);
+    const size_t gbytes = (size_t)whirgpu_ctx::GRIND_SLOTS * GROUP_MAX * sizeof(u32);
+    HIPCHK(hipMalloc(&c->d_hgrind, gbytes));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_hgrind), GROUP_MAX * sizeof(u32), hipHostMallocDefault));
     c->ring_cap = 1 << 20;
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_ring), c->ring_cap, hipHostMallocDefault));
+    HIPCHK(hipMalloc(reinterpret_cast<void**>(&c->d_ring), c->ring_cap));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_pts), sizeof(ef) * 32 * whirgpu_ctx::PTS_CAP,
+                         hipHostMallocCoherent));
+    HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->d_pts), c->h_pts, 0));
+    c->fixed_bytes = 8192 * sizeof(u32) + sizeof(P2C16) + sizeof(P2C24) + gbytes + c->ring_cap;
+    c->pin<u32>(1);
+    ensure_slots(c, 1);
   });
   if (rc) {
     if (c) whirgpu_destroy(c);
@@ -1341,15 +2014,12 @@ int whirgpu_destroy(whirgpu_ctx* c) {
   if (c->device >= 0) (void)hipSetDevice(c->device);
   if (c->st) (void)hipStreamSynchronize(c->st);
   c->pool.release_all();
-  for (void* h : {(void*)c->h_small, (void*)c->h_idx, (void*)c->h_open, (void*)c->h_root, (void*)c->h_sclog,
-                  (void*)c->h_chal, (void*)c->h_rlog, (void*)c->h_final})
-    if (h) (void)hipHostFree(h);
-  for (void* p : {(void*)c->d_tw, (void*)c->d_tabs, (void*)c->d_partial, (void*)c->d_grind, (void*)c->d_gslots,
-                  (void*)c->d_c16, (void*)c->d_c24, (void*)c->d_chal, (void*)c->d_rchal, c->d_rbuf,
-                  (void*)c->d_gbase})
+  for (auto& s : c->slots) slot_free(s.get());
+  c->slots.clear();
+  for (void* p : {(void*)c->d_tw, (void*)c->d_c16, (void*)c->d_c24, (void*)c->d_hgrind, (void*)c->d_ring})
     if (p) (void)hipFree(p);
-  if (c->h_pin) (void)hipHostFree(c->h_pin);
-  if (c->h_ring) (void)hipHostFree(c->h_ring);
+  for (void* h : {(void*)c->h_pin, (void*)c->h_ring, (void*)c->h_hgrind, (void*)c->h_pts})
+    if (h) (void)hipHostFree(h);
   for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
   if (c->ev_round) (void)hipEventDestroy(c->ev_round);
   if (c->st) (void)hipStreamDestroy(c->st);
@@ -1448,33 +2118,30 @@ int whirgpu_statement_create(whirgpu_ctx* c, const uint32_t* evals, int on_devic
   return WHIRGPU_OK;
 }
 
-static ef statement_evaluate(whirgpu_statement* s, const std::vector<ef>& z) {
-  whirgpu_ctx* c = s->ctx;
-  int sb = s->seg_bits;
-  std::vector<ef> z_rest(z.begin() + sb, z.end());
-  auto part = base_partials(c, s->d_poly, s->n, sb, z_rest);
-  ef eval;
-  if (sb == 0) {
-    eval = part[0];
-  } else {
-    std::vector<ef> zs(z.begin(), z.begin() + sb);
-    auto eqs = whir::eq_table_host(zs, ef_one());
-    eval = ef_zero();
-    for (size_t i = 0; i < part.size(); i++) eval = ef_add(eval, ef_mul(eqs[i], part[i]));
-  }
-  s->points.push_back(z);
-  s->evals.push_back(eval);
-  s->partials.push_back(std::move(part));
-  return eval;
-}
 
 int whirgpu_statement_evaluate(whirgpu_statement* s, const uint32_t* point, uint32_t* eval_out) {
   return guarded([&] {
     if (!s || !point || !eval_out) bad("null argument");
-    std::vector<ef> z(s->n);
-    for (int i = 0; i < s->n; i++) memcpy(z[i].c, point + 4 * i, 16);
-    ef e = statement_evaluate(s, z);
+    use_device(s->ctx);
+    const ef e = statement_evaluate(s->ctx, {s}, {read_point(point, (uint32_t)s->n)})[0];
     memcpy(eval_out, e.c, 16);
+  });
+}
+
+int whirgpu_statement_evaluate_group(uint32_t count, whirgpu_statement* const* statements, const uint32_t* points,
+                                     uint32_t* evals_out) {
+  return guarded([&] {
+    if (!statements || !points || !evals_out || count < 1 || count > GROUP_MAX) bad("bad argument");
+    std::vector<whirgpu_statement*> sts(statements, statements + count);
+    for (auto* s : sts)
+      if (!s) bad("null statement");
+    whirgpu_ctx* c = sts[0]->ctx;
+    use_device(c);
+    const uint32_t n = (uint32_t)sts[0]->n;
+    std::vector<std::vector<ef>> zs;
+    for (uint32_t b = 0; b < count; b++) zs.push_back(read_point(points + (size_t)4 * n * b, n));
+    const auto ev = statement_evaluate(c, sts, zs);
+    for (uint32_t b = 0; b < count; b++) memcpy(evals_out + 4 * b, ev[b].c, 16);
   });
 }
 
@@ -1540,224 +2207,90 @@ int whirgpu_prover_data_destroy(whirgpu_prover_data* pd) {
   return WHIRGPU_OK;
 }
 
-// CommitmentWriter::commit (src/whir/committer/writer.rs:56-109).
+// CommitmentWriter::commit (src/whir/committer/writer.rs:56-109): a group of one.
 int whirgpu_commit(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs, whirgpu_statement* st,
                    whirgpu_prover_data** out) {
-  whirgpu_prover_data* pd = nullptr;
+  std::vector<Run> runs(1);
   int rc = guarded([&] {
     if (!c || !proof || !chs || !st || !out) bad("null argument");
+    if (st->ctx != c) bad("statement of another context");
     use_device(c);
-    double t0 = now_ms();
-    DuplexChallenger ch = load_ch(c, chs);
-    const auto& cfg = c->cfg;
-    const int n = cfg.num_variables, k0 = cfg.ff(0), R = cfg.params.starting_log_inv_rate;
-    const u64 H = 1ull << (n + R - k0), W = 1ull << k0;
-    pd = new whirgpu_prover_data();
-    pd->ctx = c;
-    pd->rows = c->alloc<u32>(H * W);
-    pd->tree = c->alloc<u32>((2 * H - 1) * 8);
-    pd->height = H;
-    pd->row_len = (int)W;
-    pd->ext = false;
-    Buf<u32> tmp;
-    if (lde_two_pass(n, k0, false)) tmp = c->alloc<u32>(H * W);
-    {
-      Span sp(c, ST_LDE_BASE, 4.0 * (double)(1ull << n) + 4.0 * (double)(H * W));
-      KCHK(lde_base(st->d_poly, pd->rows.p, tmp.p, n, k0, R, c->tw, c->st));
-    }
-    commit_tree(c, pd);
-    if (tmp.p) c->release(tmp);
-    Digest root = read_root(c, pd);
-    proof->p.has_initial_commitment = true;
-    proof->p.initial_commitment = root;
-    observe_digest(ch, root);
-    for (int i = 0; i < cfg.commitment_ood_samples; i++) {
-      auto pt = whir::expand_from_univariate(ch.sample_ef(), n);
-      ef ev = statement_evaluate(st, pt);
-      proof->p.initial_ood_answers.push_back(ev);
-      ch.observe_ef(ev);
-    }
-    store_ch(ch, chs);
+    const double t0 = now_ms();
+    Run& r = runs[0];
+    r.st = st;
+    r.P = &proof->p;
+    r.ch = load_ch(c, chs);
+    r.sl = c->slots[0].get();
+    commit_group(c, {&r});
+    store_ch(r.ch, chs);
     c->timings[0] = now_ms() - t0;
   });
   if (rc) {
-    if (pd) whirgpu_prover_data_destroy(pd);
+    if (c) (void)hipStreamSynchronize(c->st);   // nothing in flight uses what is released below
+    if (c) release_runs(c, runs);
     return rc;
   }
-  *out = pd;
+  *out = runs[0].prev;
   return WHIRGPU_OK;
 }
 
-// Prover::prove (src/whir/prover/mod.rs:97-417).
+// Prover::prove (src/whir/prover/mod.rs:97-417): a group of one; consumes the prover data.
 int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs, const whirgpu_statement* st,
                   whirgpu_prover_data* pd0) {
-  SumcheckState s;
-  whirgpu_prover_data* prev = pd0;
-  whirgpu_prover_data* next_pd = nullptr;   // a commitment enqueued ahead of its round
-  whirgpu_prover_data* cur_pd = nullptr;    // this round's, until it becomes prev
+  std::vector<Run> runs(1);
+  runs[0].prev = pd0;
   int rc = guarded([&] {
     if (!c || !proof || !chs || !st || !pd0) bad("null argument");
     use_device(c);
-    double t0 = now_ms();
-    DuplexChallenger ch = load_ch(c, chs);
-    const auto& cfg = c->cfg;
-    const int n = cfg.num_variables;
-    auto& P = proof->p;
-    P.rounds.assign(cfg.n_rounds(), whir::WhirRoundProof());
-    const int pow0 = cfg.starting_folding_pow_bits;
-    // Commitment of round r's folded polynomial (prover/mod.rs:158-190):
-    // 2^(m - kn + R) rows of 2^kn extension elements and their tree.
-    auto commit_round = [&](int round) {
-      const int m = n - cfg.params.folding_factor.total_number(round);
-      const int kn = cfg.ff(round + 1);
-      const int Rl = ilog2u(cfg.inv_rate(round));
-      const u64 H = 1ull << (m - kn + Rl), W = 1ull << kn;
-      auto* pd = new whirgpu_prover_data();
-      pd->ctx = c;
-      pd->rows = c->alloc<u32>(H * W * 4);
-      pd->tree = c->alloc<u32>((2 * H - 1) * 8);
-      pd->height = H;
-      pd->row_len = (int)(4 * W);
-      pd->ext = true;
-      next_pd = pd;   // destroyed by the caller if anything below throws
-      Buf<ef> tmp;
-      if (lde_two_pass(m, kn, true)) tmp = c->alloc<ef>(H * W);
-      {
-        Span sp(c, ST_LDE_EXT, 16.0 * (double)(1ull << m) + 16.0 * (double)(H * W));
-        KCHK(lde_ext(s.e.p, reinterpret_cast<ef*>(pd->rows.p), tmp.p, m, kn, Rl, c->tw, c->st));
-      }
-      commit_tree(c, pd);
-      if (tmp.p) c->release(tmp);
-      next_pd = nullptr;
-      return pd;
-    };
-    FinalLaunch fl;
-    bool final_launched = false;
-    ef alpha = ch.sample_ef();
-    std::vector<ef> fold_rand =
-        initial_sumcheck(c, st, cfg.ff(0), whir::powers(alpha, st->points.size()),
-                         [&](const ef& c0, const ef& c2) {
-                           return observe_and_sample(c, P.initial_sumcheck, ch, c0, c2, pow0);
-                         },
-                         s);
-    for (int round = 0; round <= cfg.n_rounds(); round++) {
-      if (final_launched) {   // enqueued after the last round (its sumcheck already advanced s)
-        device_final_replay(c, s, cfg, round, prev, P, ch, fl);
-        break;
-      }
-      const int m = n - cfg.params.folding_factor.total_number(round);
-      if (m != s.m) bad("internal: round size mismatch", WHIRGPU_BAD_STATE);
-      if (round == cfg.n_rounds()) {
-        if (device_final_fits(c, cfg, m, prev)) {
-          fl = device_final_launch(c, s, cfg, round, prev, ch, nullptr);
-          device_final_replay(c, s, cfg, round, prev, P, ch, fl);
-          break;
-        }
-        // final_round (prover/mod.rs:330-417)
-        P.final_poly.resize(1ull << m);
-        c->d2h(P.final_poly.data(), s.e.p, P.final_poly.size());
-        P.has_final_poly = true;
-        for (const ef& e : P.final_poly) ch.observe_ef(e);
-        if (cfg.final_pow_bits > 0) P.final_pow_witness = gpu_grind(c, ch, cfg.final_pow_bits);
-        auto fc = cfg.final_round_config();
-        auto idx = whir::get_challenge_stir_queries(fc.domain_size, cfg.ff(round), cfg.final_queries, ch);
-        std::vector<std::vector<ef>> rows;
-        open_rows(c, prev, idx, &P.final_queries, &rows);
-        if (cfg.final_sumcheck_rounds > 0) {
-          P.has_final_sumcheck = true;
-          device_sumcheck(c, s, cfg.final_sumcheck_rounds, cfg.final_folding_pow_bits, P.final_sumcheck, ch);
-        }
-        break;
-      }
-      const auto& rp = cfg.round_parameters[round];
-      auto& R = P.rounds[round];
-      const int kn = cfg.ff(round + 1);
-      cur_pd = next_pd ? next_pd : commit_round(round);
-      next_pd = nullptr;
-      whirgpu_prover_data* pd = cur_pd;
-      if (device_round_fits(c, rp, m, prev)) {
-        const RoundLaunch rl = device_round_launch(c, s, cfg, round, kn, pd, prev, ch);
-        // the next commitment depends only on the folded polynomial; after
-        // the last round, the final round continues from the device state
-        if (round + 1 < cfg.n_rounds()) {
-          next_pd = commit_round(round + 1);
-        } else if (kn <= whirgpu_ctx::SC_LOG_CAP / 2 && device_final_fits(c, cfg, m - kn, pd)) {
-          fl = device_final_launch(c, s, cfg, round + 1, pd, ch, rl.la.out);
-          final_launched = true;
-        }
-        fold_rand = device_round_replay(c, s, cfg, round, pd, prev, fold_rand, R, ch, rl);
-        whirgpu_prover_data_destroy(prev);
-        prev = pd;
-        cur_pd = nullptr;
-        continue;
-      }
-      R.commitment = read_root(c, pd);
-      observe_digest(ch, R.commitment);
-      // OOD samples (prover/mod.rs:192-205)
-      std::vector<std::vector<ef>> ood_pts;
-      for (int i = 0; i < rp.ood_samples; i++) {
-        auto pt = whir::expand_from_univariate(ch.sample_ef(), m);
-        ef ev = ext_mle(c, s.e.p, m, pt);
-        ch.observe_ef(ev);
-        R.ood_answers.push_back(ev);
-        ood_pts.push_back(pt);
-      }
-      if (rp.pow_bits > 0) R.pow_witness = gpu_grind(c, ch, rp.pow_bits);
-      ch.sample();
-      // STIR queries on the previous commitment (prover/mod.rs:229-303)
-      auto idx = whir::get_challenge_stir_queries(rp.domain_size, cfg.ff(round), rp.num_queries, ch);
-      std::vector<std::vector<ef>> rows;
-      open_rows(c, prev, idx, &R.queries, &rows);
-      std::vector<u32> sel_z;
-      std::vector<ef> sel_v;
-      for (size_t q = 0; q < idx.size(); q++) {
-        sel_z.push_back(kb::pow(rp.folded_domain_gen, idx[q]));
-        sel_v.push_back(whir::mle_host(rows[q], fold_rand));
-      }
-      whirgpu_prover_data_destroy(prev);
-      prev = pd;
-      cur_pd = nullptr;
-      // Constraint::combine_packed (src/whir/constraints/mod.rs:234-248)
-      ef gamma = ch.sample_ef();
-      auto gp = whir::powers(gamma, ood_pts.size() + sel_z.size());
-      std::vector<ef> eq_scales(gp.begin(), gp.begin() + ood_pts.size());
-      if (!ood_pts.empty()) add_eq_weights(c, s, ood_pts, eq_scales, true);
-      for (size_t i = 0; i < ood_pts.size(); i++) s.sum = ef_add(s.sum, ef_mul(gp[i], R.ood_answers[i]));
-      if (!sel_z.empty()) {
-        size_t ns = sel_z.size();
-        std::vector<ef> g(gp.begin() + ood_pts.size(), gp.end());
-        u32* d_z = reinterpret_cast<u32*>(c->d_tabs + 6 * c->tab_cap);
-        ef* d_g = c->d_tabs + 6 * c->tab_cap + (ns + 3) / 4 + 1;
-        if ((ns + 3) / 4 + 1 + ns > c->tab_cap) bad("too many STIR queries");
-        c->h2d(d_z, sel_z.data(), ns);
-        c->h2d(d_g, g.data(), ns);
-        // sized for the round's query count (ns <= num_queries after
-        // deduplication), so every proof of this shape reuses one pool pair
-        int lo = m / 2, hi = m - lo;
-        const size_t nmax = std::max<size_t>(ns, (size_t)rp.num_queries);
-        Buf<ef> A = c->alloc<ef>(nmax << hi);
-        Buf<u32> B = c->alloc<u32>(nmax << lo);
-        {
-          Span sp(c, ST_COMBINE, 32.0 * (double)(1ull << m));
-          KCHK(select_combine(s.w.p, m, d_z, d_g, (int)ns, nullptr, A.p, B.p, c->st));
-        }
-        c->release(A);   // reuse is ordered on the same stream
-        c->release(B);
-        for (size_t j = 0; j < ns; j++) s.sum = ef_add(s.sum, ef_mul(g[j], sel_v[j]));
-      }
-      s.invalidate();
-      fold_rand = device_sumcheck(c, s, kn, rp.folding_pow_bits, R.sumcheck, ch);
-    }
-    HIPCHK(hipStreamSynchronize(c->st));
-    store_ch(ch, chs);
+    const double t0 = now_ms();
+    Run& r = runs[0];
+    r.st = const_cast<whirgpu_statement*>(st);
+    r.P = &proof->p;
+    r.ch = load_ch(c, chs);
+    r.sl = pd0->sl ? pd0->sl : c->slots[0].get();
+    prove_group(c, {&r});
+    store_ch(r.ch, chs);
     c->timings[6] = now_ms() - t0;
   });
-  if (rc && c) (void)hipStreamSynchronize(c->st);   // nothing in flight uses what is released below
-  if (next_pd) whirgpu_prover_data_destroy(next_pd);
-  if (cur_pd) whirgpu_prover_data_destroy(cur_pd);
-  if (prev) whirgpu_prover_data_destroy(prev);
-  if (s.e.p) c->release(s.e);
-  if (s.w.p) c->release(s.w);
+  if (c) {
+    if (rc) (void)hipStreamSynchronize(c->st);
+    release_runs(c, runs);
+  }
+  return rc;
+}
+
+int whirgpu_prove_group(whirgpu_ctx* c, uint32_t count, whirgpu_statement* const* statements,
+                        whirgpu_proof* const* proofs, whirgpu_challenger* challengers) {
+  std::vector<Run> runs;
+  int rc = guarded([&] {
+    if (!c || !statements || !proofs || !challengers || count < 1 || count > GROUP_MAX) bad("bad argument");
+    use_device(c);
+    ensure_slots(c, (int)count);
+    const double t0 = now_ms();
+    runs.resize(count);
+    std::vector<Run*> R;
+    for (uint32_t b = 0; b < count; b++) {
+      if (!statements[b] || !proofs[b]) bad("null statement or proof");
+      if (statements[b]->ctx != c) bad("statement of another context");
+      if (statements[b]->points.size() != statements[0]->points.size())
+        bad("the statements of a group must have the same number of constraints");
+      Run& r = runs[b];
+      r.st = statements[b];
+      r.P = &proofs[b]->p;
+      r.ch = load_ch(c, &challengers[b]);
+      r.sl = c->slots[b].get();
+      R.push_back(&r);
+    }
+    commit_group(c, R);
+    c->timings[0] = now_ms() - t0;
+    prove_group(c, R);
+    for (uint32_t b = 0; b < count; b++) store_ch(runs[b].ch, &challengers[b]);
+    c->timings[6] = now_ms() - t0;
+  });
+  if (c) {
+    if (rc) (void)hipStreamSynchronize(c->st);
+    release_runs(c, runs);
+  }
   return rc;
 }
 
@@ -1865,10 +2398,11 @@ int whirgpu_lde_base(whirgpu_ctx* c, const uint32_t* d_src, uint32_t m, uint32_t
   return guarded([&] {
     if (!c || !d_src || !d_dst || k > m || m > 28 || m - k + log_blowup > 24) bad("bad shape");
     use_device(c);
-    u64 out = 1ull << (m + log_blowup);
+    const u64 out = 1ull << (m + log_blowup);
     Buf<u32> tmp;
     if (lde_two_pass((int)m, (int)k, false)) tmp = c->alloc<u32>(out);
-    KCHK(lde_base(d_src, d_dst, tmp.p, (int)m, (int)k, (int)log_blowup, c->tw, c->st));
+    const LdeArgs<u32> a{d_src, tmp.p, d_dst};
+    KCHK(lde_base(&a, 1, (int)m, (int)k, (int)log_blowup, c->tw, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     if (tmp.p) c->release(tmp);
   });
@@ -1879,11 +2413,11 @@ int whirgpu_lde_ext(whirgpu_ctx* c, const uint32_t* d_src, uint32_t m, uint32_t 
   return guarded([&] {
     if (!c || !d_src || !d_dst || k > m || m > 28 || m - k + log_blowup > 24) bad("bad shape");
     use_device(c);
-    u64 out = 1ull << (m + log_blowup);
+    const u64 out = 1ull << (m + log_blowup);
     Buf<ef> tmp;
     if (lde_two_pass((int)m, (int)k, true)) tmp = c->alloc<ef>(out);
-    KCHK(lde_ext(reinterpret_cast<const ef*>(d_src), reinterpret_cast<ef*>(d_dst), tmp.p, (int)m, (int)k,
-                 (int)log_blowup, c->tw, c->st));
+    const LdeArgs<ef> a{reinterpret_cast<const ef*>(d_src), tmp.p, reinterpret_cast<ef*>(d_dst)};
+    KCHK(lde_ext(&a, 1, (int)m, (int)k, (int)log_blowup, c->tw, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     if (tmp.p) c->release(tmp);
   });
@@ -1895,8 +2429,10 @@ int whirgpu_merkle_commit(whirgpu_ctx* c, const uint32_t* d_rows, uint64_t heigh
     if (!c || !d_rows || !d_tree || !root_out || row_len == 0) bad("null argument");
     if (height == 0 || (height & (height - 1))) bad("height must be a power of two");
     use_device(c);
-    KCHK(merkle_commit(d_rows, (int)row_len, height, (int)c->params.leaf_hash_width, c->c16, c->d_c16, c->d_c24,
-                       d_tree, c->st));
+    const TreeArgs a{d_rows, d_tree, nullptr};
+    KCHK(merkle_leaves(&a, 1, (int)row_len, height, (int)c->params.leaf_hash_width, c->d_c16, c->d_c24, c->st));
+    const int rc = merkle_tree(&a, 1, height, c->c16, c->st);
+    if (rc < 0) KCHK(rc);
     c->d2h(root_out, d_tree + (2 * height - 2) * 8, 8);
   });
 }
@@ -1907,15 +2443,9 @@ int whirgpu_grind(whirgpu_ctx* c, whirgpu_challenger* chs, uint32_t bits, uint32
     if (bits == 0 || bits > 30) bad("bits must be in [1, 30]");
     use_device(c);
     DuplexChallenger ch = load_ch(c, chs);
-    *witness_out = gpu_grind(c, ch, (int)bits);
+    *witness_out = gpu_grind(c, {&ch}, (int)bits)[0];
     store_ch(ch, chs);
   });
-}
-
-static std::vector<ef> read_point(const uint32_t* p, uint32_t n) {
-  std::vector<ef> z(n);
-  for (uint32_t i = 0; i < n; i++) memcpy(z[i].c, p + 4 * i, 16);
-  return z;
 }
 
 int whirgpu_eq_expand(whirgpu_ctx* c, const uint32_t* point, uint32_t n, const uint32_t* scale, uint32_t* d_out) {
@@ -1944,8 +2474,11 @@ int whirgpu_mle_eval_base(whirgpu_ctx* c, const uint32_t* d_f, uint32_t n, const
     if (!c || !d_f || !point || !out || n > 30) bad("bad argument");
     if ((1ull << std::max<int>(0, (int)n - 8)) + 256 > c->tab_cap) bad("n too large for this context");
     use_device(c);
-    auto part = base_partials(c, d_f, (int)n, 0, read_point(point, n));
-    memcpy(out, part[0].c, 16);
+    Slot* sl = c->slots[0].get();
+    base_partials(c, {d_f}, {sl}, (int)n, 0, {read_point(point, n)});
+    ef e;
+    read_small(c, sl, &e, 1);
+    memcpy(out, e.c, 16);
   });
 }
 
@@ -1954,7 +2487,7 @@ int whirgpu_mle_eval_ext(whirgpu_ctx* c, const uint32_t* d_f, uint32_t n, const 
     if (!c || !d_f || !point || !out || n > 30) bad("bad argument");
     if ((1ull << std::max<int>(0, (int)n - 8)) + 256 > c->tab_cap) bad("n too large for this context");
     use_device(c);
-    ef e = ext_mle(c, reinterpret_cast<const ef*>(d_f), (int)n, read_point(point, n));
+    const ef e = ext_mle(c, c->slots[0].get(), reinterpret_cast<const ef*>(d_f), (int)n, read_point(point, n));
     memcpy(out, e.c, 16);
   });
 }
@@ -1963,8 +2496,9 @@ int whirgpu_mle_eval_ext(whirgpu_ctx* c, const uint32_t* d_f, uint32_t n, const 
 // counterpart in the reference: for callers that cannot allocate).  The same
 // kernels as whirgpu_mle_eval_base, with the split eq tables and the block
 // partial sums in the library's fixed device array `scratch` (its address on
-// the current device comes from scratch_on_current_device) and the result
-// written to device memory on the caller's stream.
+// the current device comes from scratch_on_current_device), the point read
+// from the caller's host memory, and the result written to device memory on
+// the caller's stream.
 int whirgpu_mle_eval_base_noalloc(const uint32_t* d_f, uint32_t n, const uint32_t* point, uint32_t* d_out,
                                   void* stream) {
   return guarded([&] {
@@ -1979,12 +2513,16 @@ int whirgpu_mle_eval_base_noalloc(const uint32_t* d_f, uint32_t n, const uint32_
     ef* partial = s + NOALLOC_TAB_HI + NOALLOC_TAB_LO;
     const int nb = dot_blocks(hi, 0);
     if ((1ull << hi) > NOALLOC_TAB_HI || (size_t)nb > NOALLOC_PARTIALS) bad("internal: scratch too small");
+    // the point goes down by value: a copy of the eq-table kernel's staging
+    // would need memory of the library's own
     const PointArg a = point_arg(read_point(point, n));
-    KCHK(eq_split(a, hi, lo, ef_one(), tab_hi, tab_lo, st));
+    KCHK(eq_table(a, 0, hi, ef_one(), tab_hi, st));
+    KCHK(eq_table(a, hi, lo, ef_one(), tab_lo, st));
     EqTables t{};
     t.hi[0] = tab_hi;
     t.lo[0] = tab_lo;
-    KCHK(eq_dot_base(d_f, (int)n, 0, lo, t, 1, partial, nb, reinterpret_cast<ef*>(d_out), st));
+    const DotArgs da{d_f, t, partial, reinterpret_cast<ef*>(d_out)};
+    KCHK(eq_dot_base(&da, 1, (int)n, 0, lo, 1, nb, st));
   });
 }
 
@@ -1992,9 +2530,14 @@ int whirgpu_sumcheck_coeffs(whirgpu_ctx* c, const uint32_t* d_e, const uint32_t*
   return guarded([&] {
     if (!c || !d_e || !d_w || !out || n < 1 || n > 32) bad("bad argument");
     use_device(c);
-    KCHK(sc_coeffs(reinterpret_cast<const ef*>(d_e), reinterpret_cast<const ef*>(d_w), 1ull << (n - 1), c->d_partial,
-                   c->d_small, c->st));
-    c->read_small(reinterpret_cast<ef*>(out), 2);
+    Slot* sl = c->slots[0].get();
+    ScArgs a{};
+    a.e = const_cast<ef*>(reinterpret_cast<const ef*>(d_e));
+    a.w = const_cast<ef*>(reinterpret_cast<const ef*>(d_w));
+    a.partial = sl->d_partial;
+    a.c0c2 = sl->d_small;
+    KCHK(sc_coeffs(&a, 1, 1ull << (n - 1), c->st));
+    read_small(c, sl, reinterpret_cast<ef*>(out), 2);
   });
 }
 
@@ -2002,11 +2545,16 @@ int whirgpu_sumcheck_fold(whirgpu_ctx* c, uint32_t* d_e, uint32_t* d_w, uint32_t
   return guarded([&] {
     if (!c || !d_e || !d_w || !r || n < 1 || n > 32) bad("bad argument");
     use_device(c);
-    ef rr;
-    memcpy(rr.c, r, 16);
-    u64 h = 1ull << (n - 1);
-    KCHK(sc_fold(reinterpret_cast<ef*>(d_e), reinterpret_cast<ef*>(d_w), h, rr, c->d_partial, c->d_small, c->st));
-    if (out && h >= 2) c->read_small(reinterpret_cast<ef*>(out), 2);
+    Slot* sl = c->slots[0].get();
+    ScArgs a{};
+    a.e = reinterpret_cast<ef*>(d_e);
+    a.w = reinterpret_cast<ef*>(d_w);
+    memcpy(a.r.c, r, 16);
+    a.partial = sl->d_partial;
+    a.c0c2 = sl->d_small;
+    const u64 h = 1ull << (n - 1);
+    KCHK(sc_fold(&a, 1, h, c->st));
+    if (out && h >= 2) read_small(c, sl, reinterpret_cast<ef*>(out), 2);
     HIPCHK(hipStreamSynchronize(c->st));
   });
 }
@@ -2018,47 +2566,50 @@ int whirgpu_sumcheck_fold(whirgpu_ctx* c, uint32_t* d_e, uint32_t* d_w, uint32_t
 // n pairs (h(0), h(2)); out_final the folded f and eq values (8 u32).
 int whirgpu_sumcheck_eq(whirgpu_ctx* c, const uint32_t* d_f, uint32_t n, const uint32_t* point, uint32_t l0,
                         const uint32_t* scripted, whirgpu_challenger* chs, uint32_t* out_polys, uint32_t* out_final) {
-  SumcheckState s;
+  std::vector<Run> runs(1);
   int rc = guarded([&] {
     if (!c || !d_f || !point || !out_polys || !out_final || n < 1 || n > 30 || l0 > n) bad("bad argument");
     if (!scripted && !chs) bad("need scripted challenges or a challenger");
     use_device(c);
-    ensure_tabs(c, (int)n);
+    Slot* sl = c->slots[0].get();
+    ensure_tabs(c, sl, (int)n);
     whirgpu_statement st;
     st.ctx = c;
     st.n = (int)n;
     st.d_poly = const_cast<u32*>(d_f);
     st.seg_bits = (l0 > 0 && n > l0) ? (int)l0 : 0;
-    statement_evaluate(&st, read_point(point, n));
+    statement_evaluate(c, {&st}, {read_point(point, n)});
+    Run& r = runs[0];
+    r.st = &st;
+    r.sl = sl;
     DuplexChallenger ch;
     if (chs) ch = load_ch(c, chs);
     size_t round = 0;
     ChallengeFn next = [&](const ef& c0, const ef& c2) {
       memcpy(out_polys + 8 * round, c0.c, 16);
       memcpy(out_polys + 8 * round + 4, c2.c, 16);
-      ef r;
+      ef x;
       if (scripted) {
-        memcpy(r.c, scripted + 4 * round, 16);
+        memcpy(x.c, scripted + 4 * round, 16);
       } else {
         ch.observe_ef(c0);
         ch.observe_ef(c2);
-        r = ch.sample_ef();
+        x = ch.sample_ef();
       }
       round++;
-      return r;
+      return x;
     };
-    initial_sumcheck(c, &st, (int)l0, {ef_one()}, next, s);
-    while (s.m > 0) sumcheck_round(c, s, next);
+    initial_sumcheck_scripted(c, r, (int)l0, next);
+    while (r.s.m > 0) sumcheck_round(c, sl, r.s, next);
     ef fin[2];
-    c->d2h(&fin[0], s.e.p, 1);
-    c->d2h(&fin[1], s.w.p, 1);
+    c->d2h(&fin[0], r.s.e.p, 1);
+    c->d2h(&fin[1], r.s.w.p, 1);
     memcpy(out_final, fin[0].c, 16);
     memcpy(out_final + 4, fin[1].c, 16);
-    if (!ef_eq(ef_mul(fin[0], fin[1]), s.sum)) bad("internal: final sumcheck claim mismatch", WHIRGPU_BAD_STATE);
+    if (!ef_eq(ef_mul(fin[0], fin[1]), r.s.sum)) bad("internal: final sumcheck claim mismatch", WHIRGPU_BAD_STATE);
     if (chs) store_ch(ch, chs);
   });
-  if (s.e.p) c->release(s.e);
-  if (s.w.p) c->release(s.w);
+  if (c) release_runs(c, runs);
   return rc;
 }
 
