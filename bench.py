@@ -127,11 +127,12 @@ def sumcheck_eq_bench(W, args, n=26, l0=3, reps=3):
     return out
 
 
-def batched_bench(W, workers=12, count=64, seed=2024):
+def batched_bench(W, workers=12, count=64, seed=2024, group_max=16):
     """64 proofs with num_variables uniform in {16..22}, one constraint each,
     polynomials resident in HBM: proofs/s one at a time (one context per size,
     proofs back to back; median of five passes) against proofs/s through the
-    batching server (median of five batches; host CPU time of the process per
+    batching server, all jobs submitted at once so that same-size jobs are
+    proved as groups (median of five batches; host CPU time of the process per
     batched proof)."""
     rng = np.random.default_rng(seed)
     ns = [int(x) for x in rng.integers(16, 23, size=count)]
@@ -150,13 +151,12 @@ def batched_bench(W, workers=12, count=64, seed=2024):
     seq_s = sorted(seq_times)[len(seq_times) // 2]
     for c in cfgs.values():
         c.close()
-    srv = W.Server(W.params(0), workers=workers)
+    srv = W.Server(W.params(0), workers=workers, group_max=group_max)
 
     def run_batch():
         reqs = [W.ProofRequest(None, pts[i], device_ptr=bufs[i].ptr, num_variables=n) for i, n in enumerate(ns)]
         t0 = time.perf_counter()
-        for r in reqs:
-            srv.submit(r)
+        srv.submit_many(reqs)
         for r in reqs:
             assert srv.wait(r) == 0, W.lib().whirgpu_last_error()
         return reqs, time.perf_counter() - t0
@@ -177,7 +177,8 @@ def batched_bench(W, workers=12, count=64, seed=2024):
     return {"workload": f"{count} proofs, num_variables uniform in {{16..22}} (seed {seed}), 1 constraint each, "
                         f"folding 4, rate 1/2, rs-reduction 3, lambda 100",
             "one_at_a_time_proofs_per_s": round(count / seq_s, 2), "batched_proofs_per_s": round(count / bat_s, 2),
-            "speedup": round(seq_s / bat_s, 2), "workers": workers, "bit_identical_to_one_at_a_time": same,
+            "speedup": round(seq_s / bat_s, 2), "workers": workers, "group_max": group_max,
+            "bit_identical_to_one_at_a_time": same,
             "passes_timed": len(seq_times), "batches_timed": len(times), "host_cpu_ms_per_proof": round(cpu_ms, 3),
             "batch_latency_ms_p50": round(lat[len(lat) // 2], 3), "batch_latency_ms_max": round(lat[-1], 3)}
 
@@ -257,6 +258,7 @@ def main():
     ap.add_argument("--skip-kernel-bench", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--workers", type=int, default=12, help="batching server workers")
+    ap.add_argument("--group-max", type=int, default=16, help="largest group a batching server worker forms")
     ap.add_argument("--serving-workers", type=int, default=8)
     ap.add_argument("--only-kernel-bench", action="store_true",
                     help="run only the 2^26 eq-expansion + sumcheck workload and print its JSON")
@@ -367,7 +369,7 @@ def main():
     kernels = None if args.skip_kernel_bench else sumcheck_eq_bench(W, args)
     if kernels:
         progress("2^26 sumcheck/eq workload done")
-    batched = None if args.skip_batch else batched_bench(W, workers=args.workers)
+    batched = None if args.skip_batch else batched_bench(W, workers=args.workers, group_max=args.group_max)
     if batched:
         progress(f"batched: {batched['speedup']}x")
     serving = None if args.skip_batch else serving_bench(W, workers=args.serving_workers)

@@ -281,6 +281,14 @@ typedef struct whirgpu_server whirgpu_server;
 int whirgpu_server_create(int device, const whirgpu_params* params, const whirgpu_poseidon2* perms, int workers,
                           size_t hbm_budget, whirgpu_server** out);
 int whirgpu_server_submit(whirgpu_server* s, whirgpu_job* job);
+/* Submits `count` jobs in one queue update (jobs of one shape submitted
+ * together can share a group). */
+int whirgpu_server_submit_many(whirgpu_server* s, whirgpu_job* const* jobs, size_t count);
+/* Largest group a worker forms (1 to 16, default 16): a free worker takes the
+ * job at the head of the queue and up to group_max - 1 more queued jobs of the
+ * same number of variables, number of constraints and input kind, and proves
+ * them with whirgpu_prove_group. */
+int whirgpu_server_set_group_max(whirgpu_server* s, int group_max);
 /* Blocks until `job` completes (timeout_ms < 0: no limit); returns its status
  * or WHIRGPU_PENDING on timeout. */
 int whirgpu_server_wait(whirgpu_server* s, whirgpu_job* job, int timeout_ms);

@@ -5,8 +5,8 @@
 // ThreadSanitizer and runs the scenarios in main().
 //
 // A context models the device memory a real one holds: 1 MiB at creation,
-// then the expected peak of its size after its first proof (the figure the
-// server estimates before it has measured one).  A proof sleeps for a time
+// then the expected peak of its size times the largest group it has proved
+// (the figure the server estimates before it has measured one).  A proof sleeps for a time
 // that grows with its size and writes bytes derived from its inputs, so each
 // result can be checked.  Global counters record the most proofs running at
 // once, the most modelled device memory held, and whether a context was
@@ -27,7 +27,7 @@ namespace {
 
 thread_local std::string t_err;
 std::atomic<long long> g_live{0}, g_max_live{0};
-std::atomic<int> g_running{0}, g_max_running{0}, g_overlap{0}, g_destroyed{0};
+std::atomic<int> g_running{0}, g_max_running{0}, g_max_group{0}, g_overlap{0}, g_destroyed{0};
 long long g_total_mem = 64ll << 30;
 
 void bump_max(std::atomic<long long>& m, long long v) {
@@ -61,9 +61,6 @@ struct whirgpu_statement {
 struct whirgpu_proof {
   uint32_t n;
   uint32_t acc;
-};
-struct whirgpu_prover_data {
-  int unused;
 };
 
 extern "C" {
@@ -126,31 +123,43 @@ int whirgpu_proof_destroy(whirgpu_proof* p) {
   return WHIRGPU_OK;
 }
 
-int whirgpu_commit(whirgpu_ctx*, whirgpu_proof* proof, whirgpu_challenger*, whirgpu_statement* st,
-                   whirgpu_prover_data** out) {
-  proof->acc = st->acc;
-  *out = new whirgpu_prover_data{0};
-  return WHIRGPU_OK;
-}
-
-int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* ch, const whirgpu_statement*,
-                  whirgpu_prover_data* pd) {
-  if (c->users.fetch_add(1) != 0) g_overlap++;
-  bump_max(g_max_running, ++g_running);
-  if (c->bytes < peak_bytes(c->n)) {
-    bump_max(g_max_live, g_live += peak_bytes(c->n) - c->bytes);
-    c->bytes = peak_bytes(c->n);
+int whirgpu_statement_evaluate_group(uint32_t count, whirgpu_statement* const* sts, const uint32_t* points,
+                                     uint32_t* evals_out) {
+  if (count < 1 || count > 16) {
+    t_err = "group size out of range";
+    return WHIRGPU_BAD_PARAMS;
   }
-  std::this_thread::sleep_for(std::chrono::microseconds(200 + 40 * c->n));
-  proof->acc = mix(proof->acc, ch->sponge[0]);
-  g_running--;
-  c->users--;
-  delete pd;
+  for (uint32_t b = 0; b < count; b++) {
+    int rc = whirgpu_statement_evaluate(sts[b], points + (size_t)4 * sts[b]->ctx->n * b, evals_out + 4 * b);
+    if (rc) return rc;
+  }
   return WHIRGPU_OK;
 }
 
-int whirgpu_prover_data_destroy(whirgpu_prover_data* pd) {
-  delete pd;
+int whirgpu_prove_group(whirgpu_ctx* c, uint32_t count, whirgpu_statement* const* sts, whirgpu_proof* const* proofs,
+                        whirgpu_challenger* chs) {
+  if (count < 1 || count > 16) {
+    t_err = "group size out of range";
+    return WHIRGPU_BAD_PARAMS;
+  }
+  for (uint32_t b = 0; b < count; b++) {
+    if (sts[b]->ctx != c || proofs[b]->n != c->n) {
+      t_err = "statement of another context";
+      return WHIRGPU_BAD_PARAMS;
+    }
+  }
+  if (c->users.fetch_add(1) != 0) g_overlap++;
+  bump_max(g_max_running, g_running += (int)count);
+  bump_max(g_max_group, (int)count);
+  const long long need = peak_bytes(c->n) * count;
+  if (c->bytes < need) {
+    bump_max(g_max_live, g_live += need - c->bytes);
+    c->bytes = need;
+  }
+  std::this_thread::sleep_for(std::chrono::microseconds(200 + 40 * c->n + 20 * count));
+  for (uint32_t b = 0; b < count; b++) proofs[b]->acc = mix(sts[b]->acc, chs[b].sponge[0]);
+  g_running -= (int)count;
+  c->users--;
   return WHIRGPU_OK;
 }
 
@@ -233,6 +242,7 @@ int fail(const char* what) {
 void reset() {
   g_max_live = g_live.load();
   g_max_running = 0;
+  g_max_group = 0;
   g_overlap = 0;
   g_destroyed = 0;
 }
@@ -300,8 +310,9 @@ int scenario_budget(int k) {
     return fail("admission count");
   }
   if (g_max_live.load() > (long long)budget) return fail("device memory above the budget");
-  printf("budget ok: k=%d, at most %d running, peak %lld of %zu bytes\n", k, g_max_running.load(),
-         g_max_live.load(), budget);
+  if (g_max_group.load() > k) return fail("a group larger than the budget admits");
+  printf("budget ok: k=%d, at most %d running, largest group %d, peak %lld of %zu bytes\n", k,
+         g_max_running.load(), g_max_group.load(), g_max_live.load(), budget);
   return 0;
 }
 
@@ -348,6 +359,36 @@ int scenario_oversized() {
   return 0;
 }
 
+// Jobs submitted at once form groups of one shape, at most group_max each;
+// mixed shapes in the queue never share a group (the stand-in refuses a
+// group whose statements belong to another context, and the proof bytes
+// depend on each job's own inputs).
+int scenario_groups() {
+  reset();
+  whirgpu_server* s = start(2, 0);
+  if (!s) return fail("server_create");
+  if (whirgpu_server_set_group_max(s, 17) != WHIRGPU_BAD_PARAMS) return fail("group_max 17 accepted");
+  if (whirgpu_server_set_group_max(s, 6)) return fail("set_group_max");
+  std::vector<Req> reqs(90);
+  std::vector<whirgpu_job*> ptrs;
+  for (size_t i = 0; i < reqs.size(); i++) {
+    make(reqs[i], 10 + (uint32_t)(i % 3 == 0), 1 + (uint32_t)(i % 2), (uint32_t)(11 * i));
+    ptrs.push_back(&reqs[i].job);
+  }
+  if (whirgpu_server_submit_many(s, ptrs.data(), ptrs.size())) return fail("submit_many");
+  for (auto& r : reqs) whirgpu_server_wait(s, &r.job, -1);
+  whirgpu_server_destroy(s);
+  for (auto& r : reqs)
+    if (!check(r)) return fail("proof bytes");
+  if (g_max_group.load() != 6) {
+    fprintf(stderr, "largest group %d\n", g_max_group.load());
+    return fail("groups did not form up to group_max");
+  }
+  if (g_overlap) return fail("a context was used by two threads at once");
+  printf("groups ok: largest group %d, at most %d running\n", g_max_group.load(), g_max_running.load());
+  return 0;
+}
+
 // Destroying the server finishes every job already submitted.
 int scenario_drain() {
   reset();
@@ -380,5 +421,6 @@ int main(int argc, char** argv) {
   if (sc == "trim" || sc == "all") rc |= scenario_trim();
   if (sc == "oversized" || sc == "all") rc |= scenario_oversized();
   if (sc == "drain" || sc == "all") rc |= scenario_drain();
+  if (sc == "groups" || sc == "all") rc |= scenario_groups();
   return rc;
 }

@@ -96,6 +96,8 @@ _SIGS = {
                                              ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(_VP)]),
     "whirgpu_server_submit": (ctypes.c_int, [_VP, ctypes.POINTER(Job)]),
     "whirgpu_server_wait": (ctypes.c_int, [_VP, ctypes.POINTER(Job), ctypes.c_int]),
+    "whirgpu_server_submit_many": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.POINTER(Job)), ctypes.c_size_t]),
+    "whirgpu_server_set_group_max": (ctypes.c_int, [_VP, ctypes.c_int]),
     "whirgpu_server_stats": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t),
                                             ctypes.POINTER(ctypes.c_size_t)]),
     "whirgpu_server_destroy": (ctypes.c_int, [_VP]),
@@ -455,16 +457,26 @@ class ProofRequest:
 class Server:
     """whirgpu_server: continuous proving with admission by device memory."""
 
-    def __init__(self, prm, perm_seed=1, device=0, workers=8, hbm_budget=0):
+    def __init__(self, prm, perm_seed=1, device=0, workers=8, hbm_budget=0, group_max=None):
         self.perms = poseidon2_from_seed(perm_seed)
         self.prm = prm
         h = ctypes.c_void_p()
         check(lib().whirgpu_server_create(device, ctypes.byref(prm), ctypes.byref(self.perms), workers, hbm_budget,
                                           ctypes.byref(h)))
         self.handle = h
+        if group_max is not None:
+            self.set_group_max(group_max)
 
     def submit(self, req):
         check(lib().whirgpu_server_submit(self.handle, ctypes.byref(req.job)))
+
+    def submit_many(self, reqs):
+        """Submits all requests in one queue update (same-shape jobs can share a group)."""
+        arr = (ctypes.POINTER(Job) * len(reqs))(*[ctypes.pointer(r.job) for r in reqs])
+        check(lib().whirgpu_server_submit_many(self.handle, arr, len(reqs)))
+
+    def set_group_max(self, g):
+        check(lib().whirgpu_server_set_group_max(self.handle, g))
 
     def wait(self, req, timeout_ms=-1):
         return lib().whirgpu_server_wait(self.handle, ctypes.byref(req.job), timeout_ms)

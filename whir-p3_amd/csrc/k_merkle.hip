@@ -316,13 +316,17 @@ __device__ __forceinline__ void phase_fence(u32* s) {
 // Grid: (blocks, 1, proofs), one search per proof.
 __global__ void __launch_bounds__(256) k_grind(const P2C16* __restrict__ cp, Group<GrindArgs> ga, int tr_mode,
                                                u32 mask, u32 start, u32 end) {
+  // This proof's arguments are copied out of the group before any asm: an
+  // asm with a memory clobber would otherwise make the compiler keep a
+  // private copy of the whole group (kilobytes of scratch per lane).
   const GrindArgs& a = ga.v[blockIdx.z];
   Sponge16 base = a.base;
   int wpos = a.wpos;
   u32* result = a.result;
+  const ScTr tr = a.tr;
   if (tr_mode == TR_COEFFS_GRIND || tr_mode == TR_OBS_GRIND) {
     __shared__ GrindBase gbs;
-    tr_prologue(tr_mode, a.tr, *cp, &gbs, nullptr);
+    tr_prologue(tr_mode, tr, *cp, &gbs, nullptr);
 #pragma unroll
     for (int k = 0; k < 16; k++) base.s[k] = gbs.s[k];
     wpos = (int)gbs.wpos;
@@ -367,13 +371,15 @@ constexpr int WIDE_GRIND_BITS = 12;
 
 __global__ void __launch_bounds__(256) k_grind_wide(const P2C16* __restrict__ cp, Group<GrindArgs> ga, int tr_mode,
                                                     u32 mask, u32 start, u32 end) {
+  // arguments copied out of the group first, as in k_grind
   const GrindArgs& a = ga.v[blockIdx.z];
   Sponge16 base = a.base;
   int wpos = a.wpos;
   u32* result = a.result;
+  const ScTr tr = a.tr;
   if (tr_mode == TR_COEFFS_GRIND || tr_mode == TR_OBS_GRIND) {
     __shared__ GrindBase gbs;
-    tr_prologue(tr_mode, a.tr, *cp, &gbs, nullptr);
+    tr_prologue(tr_mode, tr, *cp, &gbs, nullptr);
 #pragma unroll
     for (int k = 0; k < 16; k++) base.s[k] = gbs.s[k];
     wpos = (int)gbs.wpos;

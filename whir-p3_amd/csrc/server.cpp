@@ -1,8 +1,13 @@
 // Batched and served proving on one device: a FIFO of jobs and a pool of host
-// worker threads.  Each worker keeps its own contexts (one per number of
-// variables, each with its own HIP stream and buffer pool), so proofs from
+// worker threads.  A free worker takes the job at the head of the queue and,
+// with it, up to group_max - 1 more queued jobs of the same shape (number of
+// variables, number of constraints, input on host or device), and proves
+// them as one group (whirgpu_prove_group: every device step one launch for
+// the whole group).  Each worker keeps its own contexts (one per number of
+// variables, each with its own HIP stream and buffer pool), so groups of
 // different workers overlap on the device: while one worker waits for a
-// round's transcript values another worker's kernels run.
+// round's transcript values another worker's kernels run.  A group that fails
+// is retried job by job, so one bad job does not fail the others.
 //
 // Admission by device memory.  A context keeps its buffers between jobs, so
 // what the server holds on the device is what the workers' contexts hold, not
@@ -52,6 +57,14 @@ size_t estimate_bytes(uint32_t n, int on_device) {
 
 uint32_t size_key(const whirgpu_job* j) { return 2 * j->num_variables + (j->evals_on_device ? 1u : 0u); }
 
+// Jobs that can share a group: same size, constraint count and input kind.
+bool same_shape(const whirgpu_job* a, const whirgpu_job* b) {
+  return a->num_variables == b->num_variables && a->npoints == b->npoints &&
+         (a->evals_on_device != 0) == (b->evals_on_device != 0);
+}
+
+constexpr int GROUP_LIMIT = 16;   // the library's group size limit
+
 struct Pending {
   whirgpu_job* job;
   double submitted;
@@ -75,9 +88,10 @@ struct whirgpu_server {
   size_t held = 0;        // device bytes of every worker's contexts
   size_t reserved = 0;    // growth the running jobs may still add
   size_t releasing = 0;   // bytes of contexts being destroyed (still in `held`)
-  size_t running = 0;
+  size_t running = 0;      // jobs running
+  int group_max = GROUP_LIMIT;
   bool stop = false;
-  std::map<uint32_t, size_t> peak;   // largest context measured per job size (size_key)
+  std::map<uint32_t, size_t> peak;   // largest context bytes per proof measured per job size (size_key)
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<Pending> queue;
@@ -87,52 +101,73 @@ struct whirgpu_server {
 
 namespace {
 
-// Bytes job j is expected to add to worker w's contexts (under the lock).
-size_t growth(const whirgpu_server* s, const Worker& w, const whirgpu_job* j) {
+// Bytes a group of k jobs like j is expected to add to worker w's contexts
+// (under the lock).
+size_t growth(const whirgpu_server* s, const Worker& w, const whirgpu_job* j, size_t k) {
   auto pk = s->peak.find(size_key(j));
-  const size_t expect = pk != s->peak.end() ? pk->second : estimate_bytes(j->num_variables, j->evals_on_device);
+  const size_t per = pk != s->peak.end() ? pk->second : estimate_bytes(j->num_variables, j->evals_on_device);
+  const size_t expect = per * k;
   auto it = w.bytes.find(j->num_variables);
   const size_t have = it != w.bytes.end() ? it->second : 0;
   return expect > have ? expect - have : 0;
 }
 
-int run_job(whirgpu_server* s, Worker* w, whirgpu_job* j, whirgpu_ctx** used) {
-  whirgpu_ctx* ctx;
-  auto it = w->ctxs.find(j->num_variables);
+void set_error(whirgpu_job* j) {
+  strncpy(j->error, whirgpu_last_error(), sizeof j->error - 1);
+  j->error[sizeof j->error - 1] = 0;
+}
+
+// Proves the jobs (one shape) as one group on the worker's context for their
+// size: statements created and their constraints evaluated for all jobs at
+// once, then whirgpu_prove_group.  Returns each job's status (a failed job's
+// error text is in its record); the caller publishes them.
+std::vector<int> run_group(whirgpu_server* s, Worker* w, const std::vector<whirgpu_job*>& js, whirgpu_ctx** used) {
+  const uint32_t B = (uint32_t)js.size();
+  const uint32_t n = js[0]->num_variables;
+  whirgpu_ctx* ctx = nullptr;
+  auto it = w->ctxs.find(n);
+  int rc = WHIRGPU_OK;
   if (it == w->ctxs.end()) {
     whirgpu_params p = s->params;
-    p.num_variables = j->num_variables;
-    int rc = whirgpu_create(s->device, &p, &s->perms, &ctx);
-    if (rc) return rc;
-    w->ctxs[j->num_variables] = ctx;
+    p.num_variables = n;
+    rc = whirgpu_create(s->device, &p, &s->perms, &ctx);
+    if (rc == WHIRGPU_OK) w->ctxs[n] = ctx;
   } else {
     ctx = it->second;
   }
   *used = ctx;
-  whirgpu_statement* st = nullptr;
-  whirgpu_proof* proof = nullptr;
-  whirgpu_prover_data* pd = nullptr;
-  int rc = whirgpu_statement_create(ctx, j->evals, j->evals_on_device, &st);
-  const uint32_t n = j->num_variables;
-  for (uint32_t k = 0; rc == 0 && k < j->npoints; k++) {
-    uint32_t ev[4];
-    rc = whirgpu_statement_evaluate(st, j->points + (size_t)4 * n * k, ev);
-    if (rc == 0 && j->evals_out) memcpy(j->evals_out + 4 * k, ev, sizeof ev);
+  std::vector<whirgpu_statement*> sts(B, nullptr);
+  std::vector<whirgpu_proof*> proofs(B, nullptr);
+  std::vector<whirgpu_challenger> chs(B);
+  for (uint32_t b = 0; rc == WHIRGPU_OK && b < B; b++)
+    rc = whirgpu_statement_create(ctx, js[b]->evals, js[b]->evals_on_device, &sts[b]);
+  std::vector<uint32_t> pts, evs(4 * B);
+  for (uint32_t k = 0; rc == WHIRGPU_OK && k < js[0]->npoints; k++) {
+    pts.assign((size_t)4 * n * B, 0);
+    for (uint32_t b = 0; b < B; b++)
+      memcpy(pts.data() + (size_t)4 * n * b, js[b]->points + (size_t)4 * n * k, (size_t)16 * n);
+    rc = whirgpu_statement_evaluate_group(B, sts.data(), pts.data(), evs.data());
+    for (uint32_t b = 0; rc == WHIRGPU_OK && b < B; b++)
+      if (js[b]->evals_out) memcpy(js[b]->evals_out + 4 * k, evs.data() + 4 * b, 16);
   }
-  whirgpu_challenger ch;
-  if (rc == 0) rc = whirgpu_challenger_new(&ch);
-  if (rc == 0) rc = whirgpu_challenger_observe_domain_separator(ctx, &ch);
-  if (rc == 0) rc = whirgpu_proof_create(ctx, &proof);
-  if (rc == 0) rc = whirgpu_commit(ctx, proof, &ch, st, &pd);
-  if (rc == 0) {
-    rc = whirgpu_prove(ctx, proof, &ch, st, pd);   // consumes pd
-    pd = nullptr;
+  for (uint32_t b = 0; rc == WHIRGPU_OK && b < B; b++) {
+    rc = whirgpu_challenger_new(&chs[b]);
+    if (rc == WHIRGPU_OK) rc = whirgpu_challenger_observe_domain_separator(ctx, &chs[b]);
+    if (rc == WHIRGPU_OK) rc = whirgpu_proof_create(ctx, &proofs[b]);
   }
-  if (rc == 0) rc = whirgpu_proof_serialize(proof, j->proof_out, j->proof_cap, &j->proof_len);
-  if (pd) whirgpu_prover_data_destroy(pd);
-  if (proof) whirgpu_proof_destroy(proof);
-  if (st) whirgpu_statement_destroy(st);
-  return rc;
+  if (rc == WHIRGPU_OK) rc = whirgpu_prove_group(ctx, B, sts.data(), proofs.data(), chs.data());
+  std::vector<int> status(B);
+  for (uint32_t b = 0; b < B; b++) {
+    whirgpu_job* j = js[b];
+    int r = rc;
+    if (r == WHIRGPU_OK) r = whirgpu_proof_serialize(proofs[b], j->proof_out, j->proof_cap, &j->proof_len);
+    if (r != WHIRGPU_OK) set_error(j);
+    else j->error[0] = 0;
+    status[b] = r;
+    if (proofs[b]) whirgpu_proof_destroy(proofs[b]);
+    if (sts[b]) whirgpu_statement_destroy(sts[b]);
+  }
+  return status;
 }
 
 // Bytes held by contexts (of any worker) for sizes other than n (under the lock).
@@ -161,11 +196,11 @@ bool fits_free_memory(const whirgpu_server* s, size_t g) {
 
 void worker(whirgpu_server* s, Worker* w) {
   for (;;) {
-    Pending p{};
+    std::vector<Pending> grp;
     size_t g = 0;
     {
       std::unique_lock<std::mutex> lk(s->mu);
-      bool take = false;
+      bool take = false, alone = false;
       while (!take) {
         if (s->queue.empty()) {
           if (s->stop) break;
@@ -173,7 +208,7 @@ void worker(whirgpu_server* s, Worker* w) {
           continue;
         }
         const whirgpu_job* j = s->queue.front().job;
-        g = growth(s, *w, j);
+        g = growth(s, *w, j, 1);
         if (s->held + s->reserved + g <= s->budget && fits_free_memory(s, g)) {
           take = true;
           break;
@@ -202,7 +237,7 @@ void worker(whirgpu_server* s, Worker* w) {
           const uint32_t n = j->num_variables;
           if (s->running == 0 && s->releasing == 0 && other_size_bytes(s, n) == 0 &&
               (w->bytes.count(n) || !size_held(s, n))) {
-            take = true;
+            take = alone = true;
             break;
           }
           s->cv_work.wait(lk);
@@ -217,36 +252,56 @@ void worker(whirgpu_server* s, Worker* w) {
         s->cv_work.notify_all();   // less is held: others may fit now
       }
       if (!take) break;   // stopped with an empty queue
-      p = s->queue.front();
+      grp.push_back(s->queue.front());
       s->queue.pop_front();
+      // queued jobs of the same shape join the group while it fits the budget
+      for (auto it = s->queue.begin(); !alone && it != s->queue.end() && (int)grp.size() < s->group_max;) {
+        if (!same_shape(it->job, grp[0].job)) {
+          ++it;
+          continue;
+        }
+        const size_t g2 = growth(s, *w, grp[0].job, grp.size() + 1);
+        if (s->held + s->reserved + g2 > s->budget || !fits_free_memory(s, g2)) break;
+        g = g2;
+        grp.push_back(*it);
+        it = s->queue.erase(it);
+      }
       s->reserved += g;
-      s->running++;
+      s->running += grp.size();
     }
-    double t0 = now_ms();
-    p.job->queue_ms = t0 - p.submitted;
+    const double t0 = now_ms();
+    std::vector<whirgpu_job*> js;
+    for (auto& p : grp) {
+      p.job->queue_ms = t0 - p.submitted;
+      js.push_back(p.job);
+    }
     whirgpu_ctx* used = nullptr;
-    int rc = run_job(s, w, p.job, &used);
-    if (rc) {
-      strncpy(p.job->error, whirgpu_last_error(), sizeof p.job->error - 1);
-      p.job->error[sizeof p.job->error - 1] = 0;
-    } else {
-      p.job->error[0] = 0;
+    std::vector<int> status = run_group(s, w, js, &used);
+    if (js.size() > 1) {
+      // a failed group is retried job by job, so one bad job fails alone
+      bool failed = false;
+      for (int r : status) failed = failed || r != WHIRGPU_OK;
+      if (failed)
+        for (size_t b = 0; b < js.size(); b++) status[b] = run_group(s, w, {js[b]}, &used)[0];
     }
     const size_t now_bytes = used ? whirgpu_internal_ctx_bytes(used) : 0;
-    double t1 = now_ms();
+    const double t1 = now_ms();
     {
       std::lock_guard<std::mutex> lk(s->mu);
       s->reserved -= g;
-      s->running--;
+      s->running -= grp.size();
       if (used) {
-        size_t& b = w->bytes[p.job->num_variables];
+        size_t& b = w->bytes[js[0]->num_variables];
         s->held = s->held - b + now_bytes;
         b = now_bytes;
-        size_t& pk = s->peak[size_key(p.job)];
-        if (now_bytes > pk) pk = now_bytes;
+        size_t& pk = s->peak[size_key(js[0])];
+        const size_t per = now_bytes / js.size();
+        if (per > pk) pk = per;
       }
-      p.job->latency_ms = t1 - p.submitted;
-      p.job->status = rc;
+      for (size_t b = 0; b < js.size(); b++) {
+        grp[b].job->latency_ms = t1 - grp[b].submitted;
+        grp[b].job->status = status[b];
+      }
     }
     s->cv_done.notify_all();
     s->cv_work.notify_all();
@@ -298,6 +353,34 @@ int whirgpu_server_submit(whirgpu_server* s, whirgpu_job* job) {
   return WHIRGPU_OK;
 }
 
+int whirgpu_server_submit_many(whirgpu_server* s, whirgpu_job* const* jobs, size_t count) {
+  if (!s || (!jobs && count)) return WHIRGPU_BAD_PARAMS;
+  for (size_t i = 0; i < count; i++) {
+    const whirgpu_job* j = jobs[i];
+    if (!j || !j->evals || (j->npoints && !j->points) || !j->proof_out) return WHIRGPU_BAD_PARAMS;
+  }
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return WHIRGPU_BAD_STATE;
+    const double t = now_ms();
+    for (size_t i = 0; i < count; i++) {
+      jobs[i]->status = WHIRGPU_PENDING;
+      jobs[i]->proof_len = 0;
+      jobs[i]->error[0] = 0;
+      s->queue.push_back({jobs[i], t});
+    }
+  }
+  s->cv_work.notify_all();
+  return WHIRGPU_OK;
+}
+
+int whirgpu_server_set_group_max(whirgpu_server* s, int group_max) {
+  if (!s || group_max < 1 || group_max > GROUP_LIMIT) return WHIRGPU_BAD_PARAMS;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->group_max = group_max;
+  return WHIRGPU_OK;
+}
+
 int whirgpu_server_wait(whirgpu_server* s, whirgpu_job* job, int timeout_ms) {
   if (!s || !job) return WHIRGPU_BAD_PARAMS;
   std::unique_lock<std::mutex> lk(s->mu);
@@ -335,12 +418,20 @@ int whirgpu_prove_batch(int device, const whirgpu_params* params, const whirgpu_
   int rc = whirgpu_server_create(device, params, perms, workers, 0, &s);
   if (rc) return rc;
   int first = WHIRGPU_OK;
+  std::vector<whirgpu_job*> valid;
   for (size_t i = 0; i < njobs; i++) {
-    int r = whirgpu_server_submit(s, &jobs[i]);
-    if (r) {
-      jobs[i].status = r;
-      if (!first) first = r;
+    const whirgpu_job& j = jobs[i];
+    if (!j.evals || (j.npoints && !j.points) || !j.proof_out) {
+      jobs[i].status = WHIRGPU_BAD_PARAMS;
+      if (!first) first = WHIRGPU_BAD_PARAMS;
+    } else {
+      valid.push_back(&jobs[i]);
     }
+  }
+  int r = whirgpu_server_submit_many(s, valid.data(), valid.size());   // one queue update: groups can form
+  if (r) {
+    for (whirgpu_job* j : valid) j->status = r;
+    if (!first) first = r;
   }
   for (size_t i = 0; i < njobs; i++) {
     int r = whirgpu_server_wait(s, &jobs[i], -1);   // returns at once for a job that failed to submit
