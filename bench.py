@@ -225,6 +225,21 @@ def serving_bench(W, workers=8, rate=200.0, count=200, seed=7):
             "same_alone_and_in_mix": same}
 
 
+def verifier_bench(W, cfg, proof, pts, evals, runs=7):
+    """The product's host verifier (Verifier::verify after parse_commitment,
+    one thread) on the headline proof: decode + verify, median of `runs`."""
+    times = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        pr = W.WhirProof.from_bytes(cfg, proof)
+        W.Verifier(cfg).verify(pr, cfg.challenger(), pts, evals)
+        times.append((time.perf_counter() - t) * 1e3)
+        pr.close()
+    times.sort()
+    return {"what": "decode + verify the n=24 proof on the host (1 thread)", "ms": round(times[len(times) // 2], 3),
+            "runs": runs, "proof_bytes": len(proof)}
+
+
 def cpu_baseline(n, poly, pts, gpu_proof, runs=3):
     """The same commit + prove on the CPU oracle (a naive port: canonical 64-bit
     modular arithmetic, OpenMP over Merkle leaves, DFT columns and grinding
@@ -292,8 +307,7 @@ def main():
     d_poly = W.DeviceBuffer.from_array(poly)
 
     st0 = W.InitialStatement(cfg, None, device_ptr=d_poly.ptr)
-    for p in pts:
-        st0.evaluate(p)
+    evals0 = np.array([st0.evaluate(p) for p in pts], dtype=np.uint32).reshape(len(pts), 4)
 
     def step():
         st = st0.clone()
@@ -364,6 +378,7 @@ def main():
                 "bytes_per_launch": round(per_launch), "avg_launch_ms": round(s["ms"] / s["launches"], 4)}
 
     dominant = max(stages, key=lambda k: stages[k]["ms"])
+    verifier = verifier_bench(W, cfg, proof, pts, evals0)
 
     progress(f"n={n}: {ms:.3f} ms per proof; profiled steps done")
     kernels = None if args.skip_kernel_bench else sumcheck_eq_bench(W, args)
@@ -405,6 +420,7 @@ def main():
         "roofline_by_stage": {k: roof(k) for k in stages if stages[k]["launches"]},
         "stage_ms_per_step": {k: round(v["ms"] / args.steps, 4) for k, v in stages.items()},
         "cpu_baseline": cpu,
+        "verifier": verifier,
         "sumcheck_eq_2^26": kernels,
         "batched": batched,
         "serving": serving,

@@ -37,7 +37,9 @@ enum {
   WHIRGPU_DEVICE = 3,
   WHIRGPU_GRIND_FAILED = 4,     /* FiatShamirError::InvalidGrindingWitness (src/fiat_shamir/errors.rs:9-13) */
   WHIRGPU_BUFFER_TOO_SMALL = 5,
-  WHIRGPU_BAD_STATE = 6
+  WHIRGPU_BAD_STATE = 6,
+  WHIRGPU_VERIFY_FAILED = 7,    /* VerifierError (src/whir/verifier/errors.rs): the reason is in whirgpu_last_error() */
+  WHIRGPU_MALFORMED_PROOF = 8   /* the bytes are not a WhirProof encoding (serde deserialization error) */
 };
 
 /* ProtocolParameters (src/parameters/mod.rs:147-166) plus the number of
@@ -140,6 +142,23 @@ int whirgpu_proof_destroy(whirgpu_proof* proof);
  * one byte, little endian, fields in WhirProof declaration order
  * (src/whir/proof.rs:17-141). */
 int whirgpu_proof_serialize(const whirgpu_proof* proof, uint8_t* out, size_t cap, size_t* len);
+/* serde deserialization of WhirProof (src/whir/proof.rs:11-141): reads the
+ * bytes whirgpu_proof_serialize writes into a new proof of `ctx`'s
+ * configuration.  Truncated or trailing bytes, unknown Option tags or opening
+ * types and field words not below p return WHIRGPU_MALFORMED_PROOF. */
+int whirgpu_proof_deserialize(whirgpu_ctx* ctx, const uint8_t* bytes, size_t len, whirgpu_proof** out);
+
+/* CommitmentReader::parse_commitment + Verifier::verify
+ * (src/whir/committer/reader.rs:40-82, src/whir/verifier/mod.rs:58-304) on
+ * the host: checks `proof` for the statement {points[k] -> evals[k]}
+ * (npoints points of n extension elements, npoints extension evaluations, in
+ * the order they were evaluated).  `ch` is the verifier's challenger after
+ * the domain separator; on success it ends where the prover's ended.  A
+ * rejected proof returns WHIRGPU_VERIFY_FAILED with the failed check in
+ * whirgpu_last_error().  Needs no device: a context created with device -1
+ * verifies. */
+int whirgpu_verify(const whirgpu_ctx* ctx, const whirgpu_proof* proof, whirgpu_challenger* ch, uint32_t npoints,
+                   const uint32_t* points, const uint32_t* evals);
 
 /* CommitmentWriter::commit (src/whir/committer/writer.rs:56-109). */
 int whirgpu_commit(whirgpu_ctx* ctx, whirgpu_proof* proof, whirgpu_challenger* ch, whirgpu_statement* st,
@@ -289,6 +308,11 @@ int whirgpu_server_submit_many(whirgpu_server* s, whirgpu_job* const* jobs, size
  * same number of variables, number of constraints and input kind, and proves
  * them with whirgpu_prove_group. */
 int whirgpu_server_set_group_max(whirgpu_server* s, int group_max);
+/* With `on` nonzero, every proof is verified on the host (whirgpu_verify from
+ * a fresh challenger, against the job's points and computed evaluations)
+ * before its bytes are written; a rejected proof fails its job with
+ * WHIRGPU_VERIFY_FAILED.  Off by default. */
+int whirgpu_server_set_self_check(whirgpu_server* s, int on);
 /* Blocks until `job` completes (timeout_ms < 0: no limit); returns its status
  * or WHIRGPU_PENDING on timeout. */
 int whirgpu_server_wait(whirgpu_server* s, whirgpu_job* job, int timeout_ms);

@@ -163,6 +163,21 @@ int whirgpu_prove_group(whirgpu_ctx* c, uint32_t count, whirgpu_statement* const
   return WHIRGPU_OK;
 }
 
+// The stand-in verifier accepts a proof whose accumulator is what the job's
+// inputs give (the same value check() expects), from the evaluations the
+// statement produced.  g_reject_n makes it reject every proof of that size.
+std::atomic<uint32_t> g_reject_n{0};
+std::atomic<int> g_verified{0};
+int whirgpu_verify(const whirgpu_ctx* c, const whirgpu_proof* p, whirgpu_challenger* ch, uint32_t npoints,
+                   const uint32_t* points, const uint32_t* evals) {
+  g_verified++;
+  if (p->n != c->n || ch->sponge[0] != c->n || (npoints && (!points || !evals)) || c->n == g_reject_n.load()) {
+    t_err = "verification failed";
+    return WHIRGPU_VERIFY_FAILED;
+  }
+  return WHIRGPU_OK;
+}
+
 int whirgpu_proof_serialize(const whirgpu_proof* p, uint8_t* out, size_t cap, size_t* len) {
   *len = 8;
   if (cap < 8) {
@@ -389,6 +404,39 @@ int scenario_groups() {
   return 0;
 }
 
+// With the self-check on every proof is verified before it is published,
+// and a proof the verifier rejects fails its job (with the verifier's text)
+// while the rest of its group completes.
+int scenario_self_check() {
+  reset();
+  g_verified = 0;
+  whirgpu_server* s = start(3, 0);
+  if (!s) return fail("server_create");
+  if (whirgpu_server_set_self_check(s, 1)) return fail("set_self_check");
+  g_reject_n = 11;
+  std::vector<Req> reqs(40);
+  std::vector<whirgpu_job*> ptrs;
+  for (size_t i = 0; i < reqs.size(); i++) {
+    make(reqs[i], 10 + (uint32_t)(i % 2), 2, (uint32_t)(13 * i));
+    ptrs.push_back(&reqs[i].job);
+  }
+  if (whirgpu_server_submit_many(s, ptrs.data(), ptrs.size())) return fail("submit_many");
+  for (auto& r : reqs) whirgpu_server_wait(s, &r.job, -1);
+  whirgpu_server_destroy(s);
+  g_reject_n = 0;
+  for (auto& r : reqs) {
+    if (r.n == 11) {
+      if (r.job.status != WHIRGPU_VERIFY_FAILED || !strstr(r.job.error, "verification failed"))
+        return fail("a rejected proof was published");
+    } else if (!check(r)) {
+      return fail("proof bytes");
+    }
+  }
+  if (g_verified.load() < (int)reqs.size()) return fail("a proof was published unverified");
+  printf("self-check ok: %d verifications\n", g_verified.load());
+  return 0;
+}
+
 // Destroying the server finishes every job already submitted.
 int scenario_drain() {
   reset();
@@ -422,5 +470,6 @@ int main(int argc, char** argv) {
   if (sc == "oversized" || sc == "all") rc |= scenario_oversized();
   if (sc == "drain" || sc == "all") rc |= scenario_drain();
   if (sc == "groups" || sc == "all") rc |= scenario_groups();
+  if (sc == "self-check" || sc == "all") rc |= scenario_self_check();
   return rc;
 }

@@ -6,7 +6,8 @@ fits k contexts admits exactly k jobs at once and is never exceeded; idle
 workers release contexts of other sizes so the queue moves; a job whose
 creation fails reports its error text in the job; destroying the server
 finishes every submitted job; jobs submitted together form groups of one
-shape up to the group limit."""
+shape up to the group limit; with the self-check on, no proof is published
+unverified and a rejected one fails only its own job."""
 import os
 import shutil
 import subprocess
@@ -34,7 +35,7 @@ def binaries(tmp_path_factory):
     return build(d, True), build(d, False)
 
 
-@pytest.mark.parametrize("scenario", ["mix", "budget", "trim", "oversized", "drain", "groups"])
+@pytest.mark.parametrize("scenario", ["mix", "budget", "trim", "oversized", "drain", "groups", "self-check"])
 def test_server_under_thread_sanitizer(binaries, scenario):
     env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1 exitcode=66")
     r = subprocess.run([binaries[0], scenario], capture_output=True, text=True, timeout=240, env=env)

@@ -78,6 +78,9 @@ _SIGS = {
     "whirgpu_proof_destroy": (ctypes.c_int, [_VP]),
     "whirgpu_proof_serialize": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_uint8), ctypes.c_size_t,
                                                ctypes.POINTER(ctypes.c_size_t)]),
+    "whirgpu_proof_deserialize": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_uint8), ctypes.c_size_t,
+                                                 ctypes.POINTER(_VP)]),
+    "whirgpu_verify": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ChallengerState), ctypes.c_uint32, U32P, U32P]),
     "whirgpu_commit": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ChallengerState), _VP, ctypes.POINTER(_VP)]),
     "whirgpu_prove": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ChallengerState), _VP, _VP]),
     "whirgpu_prover_data_destroy": (ctypes.c_int, [_VP]),
@@ -98,6 +101,7 @@ _SIGS = {
     "whirgpu_server_wait": (ctypes.c_int, [_VP, ctypes.POINTER(Job), ctypes.c_int]),
     "whirgpu_server_submit_many": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.POINTER(Job)), ctypes.c_size_t]),
     "whirgpu_server_set_group_max": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "whirgpu_server_set_self_check": (ctypes.c_int, [_VP, ctypes.c_int]),
     "whirgpu_server_stats": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t),
                                             ctypes.POINTER(ctypes.c_size_t)]),
     "whirgpu_server_destroy": (ctypes.c_int, [_VP]),
@@ -329,10 +333,20 @@ class InitialStatement:
 
 
 class WhirProof:
-    def __init__(self, cfg):
+    def __init__(self, cfg, handle=None):
+        if handle is None:
+            handle = ctypes.c_void_p()
+            check(lib().whirgpu_proof_create(cfg.handle, ctypes.byref(handle)))
+        self.handle = handle
+
+    @classmethod
+    def from_bytes(cls, cfg, data):
+        """Decodes proof bytes (serde deserialization of WhirProof)."""
+        data = bytes(data)
+        buf = (ctypes.c_uint8 * max(len(data), 1)).from_buffer_copy(data or b"\0")
         h = ctypes.c_void_p()
-        check(lib().whirgpu_proof_create(cfg.handle, ctypes.byref(h)))
-        self.handle = h
+        check(lib().whirgpu_proof_deserialize(cfg.handle, buf, len(data), ctypes.byref(h)))
+        return cls(cfg, handle=h)
 
     def to_bytes(self):
         n = ctypes.c_size_t()
@@ -372,6 +386,35 @@ class Prover:
     def prove(self, proof, challenger, statement, prover_data):
         check(lib().whirgpu_prove(self.cfg.handle, proof.handle, ctypes.byref(challenger), statement.handle,
                                   prover_data))
+
+
+class Verifier:
+    """Verifier::verify (src/whir/verifier/mod.rs:58-200) on the host, after
+    CommitmentReader::parse_commitment."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+
+    def verify(self, proof, challenger, points, evals):
+        """Raises WhirError (code 7, the failed check in the message) when the
+        proof is rejected; `challenger` is the verifier's, after the domain separator."""
+        if isinstance(proof, (bytes, bytearray)):
+            proof = WhirProof.from_bytes(self.cfg, proof)
+        npts = len(points)
+        pts = np.ascontiguousarray(np.concatenate([np.asarray(p, np.uint32).reshape(-1) for p in points])
+                                   if npts else np.zeros(4, np.uint32))
+        evs = np.ascontiguousarray(np.asarray(evals, np.uint32).reshape(-1) if npts else np.zeros(4, np.uint32))
+        check(lib().whirgpu_verify(self.cfg.handle, proof.handle, ctypes.byref(challenger), npts, u32p(pts),
+                                   u32p(evs)))
+
+
+def verify_bytes(cfg, proof_bytes, points, evals):
+    """True if the proof bytes verify for {points -> evals} from a fresh challenger."""
+    try:
+        Verifier(cfg).verify(bytes(proof_bytes), cfg.challenger(), points, evals)
+        return True
+    except WhirError:
+        return False
 
 
 def prove_once(cfg, poly, points=(), device_ptr=None):
@@ -457,7 +500,7 @@ class ProofRequest:
 class Server:
     """whirgpu_server: continuous proving with admission by device memory."""
 
-    def __init__(self, prm, perm_seed=1, device=0, workers=8, hbm_budget=0, group_max=None):
+    def __init__(self, prm, perm_seed=1, device=0, workers=8, hbm_budget=0, group_max=None, self_check=False):
         self.perms = poseidon2_from_seed(perm_seed)
         self.prm = prm
         h = ctypes.c_void_p()
@@ -466,6 +509,8 @@ class Server:
         self.handle = h
         if group_max is not None:
             self.set_group_max(group_max)
+        if self_check:
+            self.set_self_check(True)
 
     def submit(self, req):
         check(lib().whirgpu_server_submit(self.handle, ctypes.byref(req.job)))
@@ -477,6 +522,10 @@ class Server:
 
     def set_group_max(self, g):
         check(lib().whirgpu_server_set_group_max(self.handle, g))
+
+    def set_self_check(self, on=True):
+        """Verify every proof on the host before it is published."""
+        check(lib().whirgpu_server_set_self_check(self.handle, 1 if on else 0))
 
     def wait(self, req, timeout_ms=-1):
         return lib().whirgpu_server_wait(self.handle, ctypes.byref(req.job), timeout_ms)

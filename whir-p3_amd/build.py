@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libwhirgpu.so")
-SOURCES = ["k_lde.hip", "k_merkle.hip", "k_poly.hip", "prover.hip", "host.cpp", "server.cpp"]
+SOURCES = ["k_lde.hip", "k_merkle.hip", "k_poly.hip", "prover.hip", "host.cpp", "verifier.cpp", "server.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-std=c++17", "-O3", "--offload-arch=gfx950", "-fPIC",

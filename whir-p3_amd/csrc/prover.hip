@@ -24,6 +24,7 @@
 
 #include "../../include/whir_gpu.h"
 #include "host.h"
+#include "verifier.h"
 #include "internal.h"
 #include "kernels.h"
 
@@ -2196,6 +2197,39 @@ int whirgpu_proof_serialize(const whirgpu_proof* p, uint8_t* out, size_t cap, si
     *len = n;
     if (cap < n || !out) bad("buffer too small", WHIRGPU_BUFFER_TOO_SMALL);
     whir::serialize_into(p->p, out);
+  });
+}
+
+// serde deserialization of WhirProof (src/whir/proof.rs:11-141).
+int whirgpu_proof_deserialize(whirgpu_ctx* c, const uint8_t* bytes, size_t len, whirgpu_proof** out) {
+  return guarded([&] {
+    if (!c || (!bytes && len) || !out) bad("null argument");
+    auto p = std::make_unique<whirgpu_proof>();
+    p->ctx = c;
+    std::string err;
+    if (!whir::deserialize(bytes, len, &p->p, &err)) bad(err, WHIRGPU_MALFORMED_PROOF);
+    *out = p.release();
+  });
+}
+
+// CommitmentReader::parse + Verifier::verify (committer/reader.rs:40-82,
+// verifier/mod.rs:58-200) on the host, for the statement {points -> evals}.
+int whirgpu_verify(const whirgpu_ctx* c, const whirgpu_proof* proof, whirgpu_challenger* ch, uint32_t npoints,
+                   const uint32_t* points, const uint32_t* evals) {
+  return guarded([&] {
+    if (!c || !proof || !ch || (npoints && (!points || !evals))) bad("null argument");
+    const int n = c->cfg.num_variables;
+    std::vector<std::vector<ef>> pts(npoints, std::vector<ef>(n));
+    std::vector<ef> evs(npoints);
+    for (uint32_t k = 0; k < npoints; k++) {
+      memcpy(pts[k].data(), points + (size_t)4 * n * k, sizeof(ef) * n);
+      memcpy(&evs[k], evals + 4 * k, sizeof(ef));
+    }
+    DuplexChallenger d = load_ch(c, ch);
+    whir::VerifierKey key{&c->cfg, &c->c16, &c->c24, (int)c->params.leaf_hash_width};
+    std::string why;
+    if (whir::verify(key, proof->p, d, pts, evs, &why) != whir::VerifyFailure::None) bad(why, WHIRGPU_VERIFY_FAILED);
+    store_ch(d, ch);
   });
 }
 

@@ -7,7 +7,10 @@
 // variables, each with its own HIP stream and buffer pool), so groups of
 // different workers overlap on the device: while one worker waits for a
 // round's transcript values another worker's kernels run.  A group that fails
-// is retried job by job, so one bad job does not fail the others.
+// is retried job by job, so one bad job does not fail the others.  With the
+// self-check on, each proof is verified on the host (whirgpu_verify, from a
+// fresh transcript) before its bytes are published; a rejected proof fails
+// its job with WHIRGPU_VERIFY_FAILED.
 //
 // Admission by device memory.  A context keeps its buffers between jobs, so
 // what the server holds on the device is what the workers' contexts hold, not
@@ -90,6 +93,7 @@ struct whirgpu_server {
   size_t releasing = 0;   // bytes of contexts being destroyed (still in `held`)
   size_t running = 0;      // jobs running
   int group_max = GROUP_LIMIT;
+  bool self_check = false;   // verify every proof on the host before publishing it
   bool stop = false;
   std::map<uint32_t, size_t> peak;   // largest context bytes per proof measured per job size (size_key)
   std::mutex mu;
@@ -142,13 +146,16 @@ std::vector<int> run_group(whirgpu_server* s, Worker* w, const std::vector<whirg
   for (uint32_t b = 0; rc == WHIRGPU_OK && b < B; b++)
     rc = whirgpu_statement_create(ctx, js[b]->evals, js[b]->evals_on_device, &sts[b]);
   std::vector<uint32_t> pts, evs(4 * B);
+  std::vector<std::vector<uint32_t>> job_evals(B, std::vector<uint32_t>(4 * (size_t)js[0]->npoints + 4));
   for (uint32_t k = 0; rc == WHIRGPU_OK && k < js[0]->npoints; k++) {
     pts.assign((size_t)4 * n * B, 0);
     for (uint32_t b = 0; b < B; b++)
       memcpy(pts.data() + (size_t)4 * n * b, js[b]->points + (size_t)4 * n * k, (size_t)16 * n);
     rc = whirgpu_statement_evaluate_group(B, sts.data(), pts.data(), evs.data());
-    for (uint32_t b = 0; rc == WHIRGPU_OK && b < B; b++)
+    for (uint32_t b = 0; rc == WHIRGPU_OK && b < B; b++) {
+      memcpy(job_evals[b].data() + 4 * k, evs.data() + 4 * b, 16);
       if (js[b]->evals_out) memcpy(js[b]->evals_out + 4 * k, evs.data() + 4 * b, 16);
+    }
   }
   for (uint32_t b = 0; rc == WHIRGPU_OK && b < B; b++) {
     rc = whirgpu_challenger_new(&chs[b]);
@@ -156,10 +163,23 @@ std::vector<int> run_group(whirgpu_server* s, Worker* w, const std::vector<whirg
     if (rc == WHIRGPU_OK) rc = whirgpu_proof_create(ctx, &proofs[b]);
   }
   if (rc == WHIRGPU_OK) rc = whirgpu_prove_group(ctx, B, sts.data(), proofs.data(), chs.data());
+  bool check;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    check = s->self_check;
+  }
   std::vector<int> status(B);
   for (uint32_t b = 0; b < B; b++) {
     whirgpu_job* j = js[b];
     int r = rc;
+    if (r == WHIRGPU_OK && check) {
+      // the verifier's own transcript from the domain separator, and the
+      // evaluations the statement produced
+      whirgpu_challenger vch;
+      r = whirgpu_challenger_new(&vch);
+      if (r == WHIRGPU_OK) r = whirgpu_challenger_observe_domain_separator(ctx, &vch);
+      if (r == WHIRGPU_OK) r = whirgpu_verify(ctx, proofs[b], &vch, j->npoints, j->points, job_evals[b].data());
+    }
     if (r == WHIRGPU_OK) r = whirgpu_proof_serialize(proofs[b], j->proof_out, j->proof_cap, &j->proof_len);
     if (r != WHIRGPU_OK) set_error(j);
     else j->error[0] = 0;
@@ -378,6 +398,13 @@ int whirgpu_server_set_group_max(whirgpu_server* s, int group_max) {
   if (!s || group_max < 1 || group_max > GROUP_LIMIT) return WHIRGPU_BAD_PARAMS;
   std::lock_guard<std::mutex> lk(s->mu);
   s->group_max = group_max;
+  return WHIRGPU_OK;
+}
+
+int whirgpu_server_set_self_check(whirgpu_server* s, int on) {
+  if (!s) return WHIRGPU_BAD_PARAMS;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->self_check = on != 0;
   return WHIRGPU_OK;
 }
 
