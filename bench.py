@@ -139,6 +139,7 @@ def batched_bench(W, workers=12, count=64, seed=2024, group_max=16):
     bufs = [W.DeviceBuffer.from_array(W.sample_field(2000 + i, 1 << n)) for i, n in enumerate(ns)]
     pts = [[W.sample_field(3000 + i, 4 * n).reshape(n, 4)] for i, n in enumerate(ns)]
     cfgs = {n: W.WhirConfig(W.params(n), device=0) for n in sorted(set(ns))}
+    tuned = {n: W.autotune.apply_cached(c) is not None for n, c in cfgs.items()}   # the same shapes both legs
     for i, n in enumerate(ns):   # warm every context once
         if n in cfgs and not getattr(cfgs[n], "_warm", False):
             W.prove_once(cfgs[n], None, pts[i], device_ptr=bufs[i].ptr)
@@ -151,7 +152,7 @@ def batched_bench(W, workers=12, count=64, seed=2024, group_max=16):
     seq_s = sorted(seq_times)[len(seq_times) // 2]
     for c in cfgs.values():
         c.close()
-    srv = W.Server(W.params(0), workers=workers, group_max=group_max)
+    srv = W.Server(W.params(0), workers=workers, group_max=group_max, tuning_cache=W.autotune.DEFAULT_CACHE)
 
     def run_batch():
         reqs = [W.ProofRequest(None, pts[i], device_ptr=bufs[i].ptr, num_variables=n) for i, n in enumerate(ns)]
@@ -178,6 +179,7 @@ def batched_bench(W, workers=12, count=64, seed=2024, group_max=16):
                         f"folding 4, rate 1/2, rs-reduction 3, lambda 100",
             "one_at_a_time_proofs_per_s": round(count / seq_s, 2), "batched_proofs_per_s": round(count / bat_s, 2),
             "speedup": round(seq_s / bat_s, 2), "workers": workers, "group_max": group_max,
+            "tuned_sizes": sorted(n for n, t in tuned.items() if t),
             "bit_identical_to_one_at_a_time": same,
             "passes_timed": len(seq_times), "batches_timed": len(times), "host_cpu_ms_per_proof": round(cpu_ms, 3),
             "batch_latency_ms_p50": round(lat[len(lat) // 2], 3), "batch_latency_ms_max": round(lat[-1], 3)}
@@ -194,7 +196,7 @@ def serving_bench(W, workers=8, rate=200.0, count=200, seed=7):
     polys = [W.sample_field(5000 + i, 1 << n) for i, n in enumerate(ns)]
     pts = [[W.sample_field(6000 + 10 * i + j, 4 * n).reshape(n, 4) for j in range(k)]
            for i, (n, k) in enumerate(zip(ns, ks))]
-    srv = W.Server(W.params(0), workers=workers)
+    srv = W.Server(W.params(0), workers=workers, tuning_cache=W.autotune.DEFAULT_CACHE)
     warm = [W.ProofRequest(polys[i], pts[i]) for i in range(count)]
     for r in warm:
         srv.submit(r)
@@ -223,6 +225,50 @@ def serving_bench(W, workers=8, rate=200.0, count=200, seed=7):
             "proofs_per_s": round(count / (done - t0), 2), "offered_rate": rate, "workers": workers,
             "mean_queue_ms": round(float(np.mean([r.job.queue_ms for r in reqs])), 3),
             "same_alone_and_in_mix": same}
+
+
+def timed_proofs(W, cfg, d_ptr, pts, steps, warmup=1):
+    """ms per commit + prove, as the headline step: the statement's constraints
+    evaluated once, cloned per step, proof bytes returned to the host."""
+    st0 = W.InitialStatement(cfg, None, device_ptr=d_ptr)
+    for p in pts:
+        st0.evaluate(p)
+    times = []
+    for i in range(warmup + steps):
+        t = time.perf_counter()
+        st = st0.clone()
+        ch = cfg.challenger()
+        proof = W.WhirProof(cfg)
+        pd = W.CommitmentWriter(cfg).commit(proof, ch, st)
+        W.Prover(cfg).prove(proof, ch, st, pd)
+        data = proof.to_bytes()
+        proof.close()
+        st.close()
+        if i >= warmup:
+            times.append((time.perf_counter() - t) * 1e3)
+    st0.close()
+    return float(np.median(times)), data
+
+
+def schedule_bench(W, n, d_ptr, pts, benchmark_ms, steps=5):
+    """The autotuner's cached best folding schedule for this n (same security
+    level and PoW bits as the benchmark), proved now beside the benchmark's
+    Constant(4) schedule.  A different schedule is a different proof."""
+    dev = W.autotune.load_cache()["devices"].get(W.device_name(0))
+    key = f"n={n},lambda=100,pow=16,CapacityBound,rate=1"
+    if not dev or key not in dev["schedules"]:
+        return None
+    best = dev["schedules"][key]["best"]
+    prm = W.params(n, folding_factor=best["folding_factor"], folding_factor_first=best["folding_factor_first"],
+                   rs_domain_initial_reduction_factor=best["rs_reduction"])
+    cfg = W.WhirConfig(prm, device=0)
+    W.autotune.apply_cached(cfg)
+    ms, data = timed_proofs(W, cfg, d_ptr, pts, steps)
+    cfg.close()
+    return {"schedule": {k: best[k] for k in ("folding_factor", "folding_factor_first", "rs_reduction", "rounds")},
+            "prove_ms": round(ms, 3), "proof_bytes": len(data), "benchmark_schedule_prove_ms": round(benchmark_ms, 3),
+            "ratio": round(benchmark_ms / ms, 3),
+            "note": "same security level and PoW bits; the benchmark schedule is the reference's Constant(4), rs 3"}
 
 
 def verifier_bench(W, cfg, proof, pts, evals, runs=7):
@@ -302,6 +348,7 @@ def main():
     n = args.n
     prm = W.params(n)
     cfg = W.WhirConfig(prm, device=0)
+    tuning = W.autotune.apply_cached(cfg)   # launch shapes only: the proof bytes do not change
     poly = W.sample_field(0, 1 << n)
     pts = [W.sample_field(1 + j, 4 * n).reshape(n, 4) for j in range(2)]
     d_poly = W.DeviceBuffer.from_array(poly)
@@ -379,6 +426,7 @@ def main():
 
     dominant = max(stages, key=lambda k: stages[k]["ms"])
     verifier = verifier_bench(W, cfg, proof, pts, evals0)
+    schedule = schedule_bench(W, n, d_poly.ptr, pts, ms)
 
     progress(f"n={n}: {ms:.3f} ms per proof; profiled steps done")
     kernels = None if args.skip_kernel_bench else sumcheck_eq_bench(W, args)
@@ -421,6 +469,8 @@ def main():
         "stage_ms_per_step": {k: round(v["ms"] / args.steps, 4) for k, v in stages.items()},
         "cpu_baseline": cpu,
         "verifier": verifier,
+        "launch_shapes": "autotuned (cache)" if tuning is not None else "built-in",
+        "schedule_autotune": schedule,
         "sumcheck_eq_2^26": kernels,
         "batched": batched,
         "serving": serving,

@@ -184,6 +184,32 @@ int whirgpu_prove_group(whirgpu_ctx* ctx, uint32_t count, whirgpu_statement* con
 int whirgpu_statement_evaluate_group(uint32_t count, whirgpu_statement* const* statements, const uint32_t* points,
                                      uint32_t* evals_out);
 
+/* Launch shapes of a context's kernels (no reference counterpart: chosen by
+ * measurement, see whir-p3_amd/autotune.py).  Every value gives the same
+ * proof bytes; 0 or -1 keeps the built-in rule.
+ * - lde_lp2_{base,ext}: longest column (log2, 1..10) one LDE pass transforms
+ *   in LDS; longer columns take the four-step split at it;
+ * - lde_tb_log_{base,ext}: pass-1 lines per block (log2, 0..8);
+ * - tree_lane_log: one lane per node while a Merkle level has more than
+ *   2^tree_lane_log nodes (4..30, default 14); tree_s128_log: 128-digest
+ *   subtrees from 2^tree_s128_log nodes (7..30, default 15);
+ * - grind_wide_bits / grind_lock_bits: lane-parallel searches up to this many
+ *   bits (default 12), lockstep blocks from this many (default 17);
+ *   grind_blocks: blocks per search (0: from the CU count). */
+typedef struct whirgpu_tuning {
+  int32_t lde_lp2_base, lde_lp2_ext, lde_tb_log_base, lde_tb_log_ext;
+  int32_t tree_lane_log, tree_s128_log;
+  int32_t grind_wide_bits, grind_lock_bits, grind_blocks;
+} whirgpu_tuning;
+int whirgpu_get_tuning(const whirgpu_ctx* ctx, whirgpu_tuning* out);
+/* "<name> <gcnArchName> (<CUs> CUs)" of a device (the name may be empty),
+ * the key of a tuning cache. */
+int whirgpu_device_name(int device, char* out, size_t cap);
+/* Applies `t` after checking its ranges and that every Reed-Solomon encoding
+ * of the context's schedule has a launch plan under it (WHIRGPU_BAD_PARAMS
+ * otherwise, the context unchanged).  Not while the context is proving. */
+int whirgpu_set_tuning(whirgpu_ctx* ctx, const whirgpu_tuning* t);
+
 /* Per-stage device time of the last commit/prove on this context, in ms:
  * [lde, merkle, grind, sumcheck, mle, other, total_wall]. */
 int whirgpu_last_timings(const whirgpu_ctx* ctx, double* out7);
@@ -313,6 +339,10 @@ int whirgpu_server_set_group_max(whirgpu_server* s, int group_max);
  * before its bytes are written; a rejected proof fails its job with
  * WHIRGPU_VERIFY_FAILED.  Off by default. */
 int whirgpu_server_set_self_check(whirgpu_server* s, int on);
+/* Launch shapes for the contexts workers create for `num_variables`
+ * (whirgpu_set_tuning); a tuning a context refuses leaves its built-in
+ * shapes.  Shapes never change a proof. */
+int whirgpu_server_set_tuning(whirgpu_server* s, uint32_t num_variables, const whirgpu_tuning* t);
 /* Blocks until `job` completes (timeout_ms < 0: no limit); returns its status
  * or WHIRGPU_PENDING on timeout. */
 int whirgpu_server_wait(whirgpu_server* s, whirgpu_job* job, int timeout_ms);

@@ -27,7 +27,7 @@ namespace {
 
 thread_local std::string t_err;
 std::atomic<long long> g_live{0}, g_max_live{0};
-std::atomic<int> g_running{0}, g_max_running{0}, g_max_group{0}, g_overlap{0}, g_destroyed{0};
+std::atomic<int> g_running{0}, g_max_running{0}, g_max_group{0}, g_overlap{0}, g_destroyed{0}, g_created{0};
 long long g_total_mem = 64ll << 30;
 
 void bump_max(std::atomic<long long>& m, long long v) {
@@ -73,6 +73,7 @@ int whirgpu_create(int, const whirgpu_params* p, const whirgpu_poseidon2*, whirg
     return WHIRGPU_BAD_PARAMS;
   }
   auto* c = new whirgpu_ctx();
+  g_created++;
   c->n = p->num_variables;
   c->bytes = 1 << 20;
   bump_max(g_max_live, g_live += c->bytes);
@@ -178,6 +179,15 @@ int whirgpu_verify(const whirgpu_ctx* c, const whirgpu_proof* p, whirgpu_challen
   return WHIRGPU_OK;
 }
 
+// Launch shapes: recorded, so a scenario can check that workers apply them.
+std::atomic<int> g_tuned{0};
+int whirgpu_set_tuning(whirgpu_ctx* c, const whirgpu_tuning* t) {
+  if (t->tree_lane_log != 13) return WHIRGPU_BAD_PARAMS;
+  (void)c;
+  g_tuned++;
+  return WHIRGPU_OK;
+}
+
 int whirgpu_proof_serialize(const whirgpu_proof* p, uint8_t* out, size_t cap, size_t* len) {
   *len = 8;
   if (cap < 8) {
@@ -255,6 +265,7 @@ int fail(const char* what) {
 }
 
 void reset() {
+  g_created = 0;
   g_max_live = g_live.load();
   g_max_running = 0;
   g_max_group = 0;
@@ -413,6 +424,13 @@ int scenario_self_check() {
   whirgpu_server* s = start(3, 0);
   if (!s) return fail("server_create");
   if (whirgpu_server_set_self_check(s, 1)) return fail("set_self_check");
+  // a tuning for n = 10 the stand-in accepts, one for n = 11 it refuses
+  whirgpu_tuning tu{};
+  tu.tree_lane_log = 13;
+  whirgpu_server_set_tuning(s, 10, &tu);
+  tu.tree_lane_log = 12;
+  whirgpu_server_set_tuning(s, 11, &tu);
+  g_tuned = 0;
   g_reject_n = 11;
   std::vector<Req> reqs(40);
   std::vector<whirgpu_job*> ptrs;
@@ -433,7 +451,40 @@ int scenario_self_check() {
     }
   }
   if (g_verified.load() < (int)reqs.size()) return fail("a proof was published unverified");
+  if (g_tuned.load() < 1) return fail("no context of the tuned size got its tuning");
   printf("self-check ok: %d verifications\n", g_verified.load());
+  return 0;
+}
+
+// Batches of three sizes (one group each) submitted together, again and
+// again: after the first batch every size has a worker holding its context,
+// and later batches create no context (each worker takes its own size; a
+// worker without one leaves the sizes of idle holders to them).
+int scenario_affinity() {
+  reset();
+  whirgpu_server* s = start(3, 0);
+  if (!s) return fail("server_create");
+  std::vector<int> created;
+  for (int batch = 0; batch < 4; batch++) {
+    std::vector<Req> reqs(30);
+    std::vector<whirgpu_job*> ptrs;
+    for (size_t i = 0; i < reqs.size(); i++) {
+      make(reqs[i], 10 + (uint32_t)(i % 3), 1, (uint32_t)(100 * batch + i));
+      ptrs.push_back(&reqs[i].job);
+    }
+    if (whirgpu_server_submit_many(s, ptrs.data(), ptrs.size())) return fail("submit_many");
+    for (auto& r : reqs) whirgpu_server_wait(s, &r.job, -1);
+    for (auto& r : reqs)
+      if (!check(r)) return fail("proof bytes");
+    created.push_back(g_created.load());
+  }
+  whirgpu_server_destroy(s);
+  if (created[3] != created[1]) {
+    fprintf(stderr, "contexts created after each batch: %d %d %d %d\n", created[0], created[1], created[2],
+            created[3]);
+    return fail("later batches created contexts");
+  }
+  printf("affinity ok: %d contexts after the first batch, %d after the last\n", created[0], created[3]);
   return 0;
 }
 
@@ -471,5 +522,6 @@ int main(int argc, char** argv) {
   if (sc == "drain" || sc == "all") rc |= scenario_drain();
   if (sc == "groups" || sc == "all") rc |= scenario_groups();
   if (sc == "self-check" || sc == "all") rc |= scenario_self_check();
+  if (sc == "affinity" || sc == "all") rc |= scenario_affinity();
   return rc;
 }

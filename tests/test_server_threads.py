@@ -7,7 +7,8 @@ workers release contexts of other sizes so the queue moves; a job whose
 creation fails reports its error text in the job; destroying the server
 finishes every submitted job; jobs submitted together form groups of one
 shape up to the group limit; with the self-check on, no proof is published
-unverified and a rejected one fails only its own job."""
+unverified and a rejected one fails only its own job; repeated batches reuse
+the workers' contexts (size affinity) instead of creating new ones."""
 import os
 import shutil
 import subprocess
@@ -35,7 +36,7 @@ def binaries(tmp_path_factory):
     return build(d, True), build(d, False)
 
 
-@pytest.mark.parametrize("scenario", ["mix", "budget", "trim", "oversized", "drain", "groups", "self-check"])
+@pytest.mark.parametrize("scenario", ["mix", "budget", "trim", "oversized", "drain", "groups", "self-check", "affinity"])
 def test_server_under_thread_sanitizer(binaries, scenario):
     env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1 exitcode=66")
     r = subprocess.run([binaries[0], scenario], capture_output=True, text=True, timeout=240, env=env)

@@ -42,6 +42,23 @@ class ChallengerState(ctypes.Structure):
                 ("input_len", ctypes.c_uint32), ("output_len", ctypes.c_uint32)]
 
 
+class Tuning(ctypes.Structure):
+    """whirgpu_tuning: launch shapes of a context's kernels (0 / -1: built-in rule)."""
+    _fields_ = [(f, ctypes.c_int32) for f in ("lde_lp2_base", "lde_lp2_ext", "lde_tb_log_base", "lde_tb_log_ext",
+                                             "tree_lane_log", "tree_s128_log", "grind_wide_bits",
+                                             "grind_lock_bits", "grind_blocks")]
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+    @classmethod
+    def from_dict(cls, d):
+        t = cls()
+        for f, _ in cls._fields_:
+            setattr(t, f, int(d[f]))
+        return t
+
+
 class Job(ctypes.Structure):
     """whirgpu_job: one commit + prove handled by a server."""
     _fields_ = [("num_variables", ctypes.c_uint32), ("evals", ctypes.c_void_p), ("evals_on_device", ctypes.c_int),
@@ -102,6 +119,10 @@ _SIGS = {
     "whirgpu_server_submit_many": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.POINTER(Job)), ctypes.c_size_t]),
     "whirgpu_server_set_group_max": (ctypes.c_int, [_VP, ctypes.c_int]),
     "whirgpu_server_set_self_check": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "whirgpu_server_set_tuning": (ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.POINTER(Tuning)]),
+    "whirgpu_get_tuning": (ctypes.c_int, [_VP, ctypes.POINTER(Tuning)]),
+    "whirgpu_set_tuning": (ctypes.c_int, [_VP, ctypes.POINTER(Tuning)]),
+    "whirgpu_device_name": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]),
     "whirgpu_server_stats": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t),
                                             ctypes.POINTER(ctypes.c_size_t)]),
     "whirgpu_server_destroy": (ctypes.c_int, [_VP]),
@@ -161,6 +182,12 @@ def params(num_variables, *, security_level=100, pow_bits=16, folding_factor=4, 
                   folding_factor if folding_factor_first is None else folding_factor_first, folding_factor,
                   starting_log_inv_rate, rs_domain_initial_reduction_factor,
                   SOUNDNESS.get(soundness, soundness), 1 if svo else 0, leaf_hash_width, digest_elems)
+
+
+def device_name(device=0):
+    buf = ctypes.create_string_buffer(256)
+    check(lib().whirgpu_device_name(device, buf, 256))
+    return buf.value.decode()
 
 
 def poseidon2_from_seed(seed=1):
@@ -262,6 +289,17 @@ class WhirConfig:
 
     STAGES = ("lde_base", "lde_ext", "merkle_leaves", "merkle_tree", "grind", "sumcheck", "eq_dot", "combine",
               "svo_fold", "eq_expand")
+
+    def tuning(self):
+        t = Tuning()
+        check(lib().whirgpu_get_tuning(self.handle, ctypes.byref(t)))
+        return t
+
+    def set_tuning(self, t):
+        """Applies a Tuning (or a dict of its fields); refused if a launch plan does not exist under it."""
+        if isinstance(t, dict):
+            t = Tuning.from_dict(t)
+        check(lib().whirgpu_set_tuning(self.handle, ctypes.byref(t)))
 
     def profile(self, on=True):
         check(lib().whirgpu_profile_enable(self.handle, 1 if on else 0))
@@ -500,7 +538,8 @@ class ProofRequest:
 class Server:
     """whirgpu_server: continuous proving with admission by device memory."""
 
-    def __init__(self, prm, perm_seed=1, device=0, workers=8, hbm_budget=0, group_max=None, self_check=False):
+    def __init__(self, prm, perm_seed=1, device=0, workers=8, hbm_budget=0, group_max=None, self_check=False,
+                 tuning_cache=None):
         self.perms = poseidon2_from_seed(perm_seed)
         self.prm = prm
         h = ctypes.c_void_p()
@@ -511,6 +550,9 @@ class Server:
             self.set_group_max(group_max)
         if self_check:
             self.set_self_check(True)
+        if tuning_cache is not None:
+            for n, t in autotune.cached_tunings_by_size(prm, device, tuning_cache).items():
+                self.set_tuning(n, t)
 
     def submit(self, req):
         check(lib().whirgpu_server_submit(self.handle, ctypes.byref(req.job)))
@@ -522,6 +564,12 @@ class Server:
 
     def set_group_max(self, g):
         check(lib().whirgpu_server_set_group_max(self.handle, g))
+
+    def set_tuning(self, num_variables, t):
+        """Launch shapes for the contexts workers create for `num_variables`."""
+        if isinstance(t, dict):
+            t = Tuning.from_dict(t)
+        check(lib().whirgpu_server_set_tuning(self.handle, num_variables, ctypes.byref(t)))
 
     def set_self_check(self, on=True):
         """Verify every proof on the host before it is published."""
@@ -557,3 +605,6 @@ def prove_batch(prm, requests, workers=8, perm_seed=1, device=0):
     for i, r in enumerate(requests):
         r.job = arr[i]
     return rc
+
+
+from . import autotune  # noqa: E402  (launch-shape and schedule autotuner; uses the bindings above)

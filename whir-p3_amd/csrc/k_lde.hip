@@ -333,10 +333,11 @@ struct LdePlan {
 
 // Pass 1 takes about 64 KiB of lines per block (16K base or 4K extension
 // elements), at most 256 lines and at most L2; lines of 2^12 and longer take
-// fewer (down to one line per block) and up to 160 KiB of LDS.  Returns 0, or
-// a negative code for a shape no kernel instance covers.
+// fewer (down to one line per block) and up to 160 KiB of LDS.  A tuned
+// tb_log (>= 0) replaces that rule, within the same limits.  Returns 0, or a
+// negative code for a shape no kernel instance covers.
 template <typename T>
-static int lde_plan(int m, int k, int R, int LP2, LdePlan* pl) {
+static int lde_plan(int m, int k, int R, int LP2, int tb_tuned, LdePlan* pl) {
   const int l = m - k;
   if (l < 0 || k < 0 || R < 0 || k > 10) return -1;
   if (l + R > 24) return -7;   // the codeword domain exceeds KoalaBear's two-adic subgroups
@@ -353,11 +354,12 @@ static int lde_plan(int m, int k, int R, int LP2, LdePlan* pl) {
   pl->l2 = LP2;
   pl->l1 = l - LP2;
   const int elem_log = sizeof(T) == 4 ? 14 : 12;
-  int tb_log = elem_log - pl->l1;
+  int tb_log = tb_tuned >= 0 ? tb_tuned : elem_log - pl->l1;
   if (tb_log > 8) tb_log = 8;   // TB <= 256 < Pass1Threads: the row step NT / TB is at least 2
   if (tb_log > pl->l2) tb_log = pl->l2;
   if (tb_log < 0) tb_log = 0;
-  while (tb_log > 0 && lds_bytes(pl->l1, 1 << tb_log, sizeof(T)) > 80 * 1024) tb_log--;
+  if (tb_tuned < 0)
+    while (tb_log > 0 && lds_bytes(pl->l1, 1 << tb_log, sizeof(T)) > 80 * 1024) tb_log--;
   pl->tb_log = tb_log;
   pl->sm1 = lds_bytes(pl->l1, 1 << tb_log, sizeof(T));
   if (pl->sm1 > 160 * 1024) return -3;
@@ -368,10 +370,10 @@ static int lde_plan(int m, int k, int R, int LP2, LdePlan* pl) {
 }
 
 template <typename T>
-static int lde_impl(const LdeArgs<T>* io, int count, int m, int k, int R, int LP2, const Twiddles& tw,
+static int lde_impl(const LdeArgs<T>* io, int count, int m, int k, int R, int LP2, int tb_tuned, const Twiddles& tw,
                     hipStream_t st) {
   LdePlan pl;
-  const int rc = lde_plan<T>(m, k, R, LP2, &pl);
+  const int rc = lde_plan<T>(m, k, R, LP2, tb_tuned, &pl);
   if (rc) return rc;
   if (count < 1 || count > GROUP_MAX) return -8;
   const int l = m - k;
@@ -398,28 +400,42 @@ static int lde_impl(const LdeArgs<T>* io, int count, int m, int k, int R, int LP
   });
 }
 
-// Largest single-pass length per element type (LDS holds W lines of it).
-static int lp2_for(int k, bool ext) {
-  // keep W * L2 * sizeof(elem) <= 64 KiB
+// Largest single-pass length per element type: the tuned value, else the
+// longest whose W lines take at most 64 KiB of LDS.
+static int lp2_for(int k, bool ext, const Tuning& t) {
+  if (t.lde_lp2[ext ? 1 : 0] > 0) return t.lde_lp2[ext ? 1 : 0];
   int budget_log = 16 - (ext ? 4 : 2) - k;   // log2(64K / esize / W)
   int lp = budget_log < 10 ? budget_log : 10;
   return lp < 1 ? 1 : lp;
 }
 
-bool lde_two_pass(int m, int k, bool ext) { return m - k > lp2_for(k, ext); }
+bool lde_two_pass(int m, int k, bool ext, const Tuning& t) { return m - k > lp2_for(k, ext, t); }
 
-int lde_check(int m, int k, int R, bool ext) {
+int lde_check(int m, int k, int R, bool ext, const Tuning& t) {
   LdePlan pl;
-  const int lp = lp2_for(k, ext);
-  return ext ? lde_plan<ef>(m, k, R, lp, &pl) : lde_plan<u32>(m, k, R, lp, &pl);
+  const int lp = lp2_for(k, ext, t);
+  const int tb = t.lde_tb_log[ext ? 1 : 0];
+  return ext ? lde_plan<ef>(m, k, R, lp, tb, &pl) : lde_plan<u32>(m, k, R, lp, tb, &pl);
 }
 
-int lde_base(const LdeArgs<u32>* a, int count, int m, int k, int R, const Twiddles& tw, hipStream_t st) {
-  return lde_impl<u32>(a, count, m, k, R, lp2_for(k, false), tw, st);
+int lde_base(const LdeArgs<u32>* a, int count, int m, int k, int R, const Twiddles& tw, const Tuning& t,
+             hipStream_t st) {
+  return lde_impl<u32>(a, count, m, k, R, lp2_for(k, false, t), t.lde_tb_log[0], tw, st);
 }
 
-int lde_ext(const LdeArgs<ef>* a, int count, int m, int k, int R, const Twiddles& tw, hipStream_t st) {
-  return lde_impl<ef>(a, count, m, k, R, lp2_for(k, true), tw, st);
+int lde_ext(const LdeArgs<ef>* a, int count, int m, int k, int R, const Twiddles& tw, const Tuning& t,
+            hipStream_t st) {
+  return lde_impl<ef>(a, count, m, k, R, lp2_for(k, true, t), t.lde_tb_log[1], tw, st);
+}
+
+bool tuning_valid(const Tuning& t) {
+  for (int e = 0; e < 2; e++) {
+    if (t.lde_lp2[e] < 0 || t.lde_lp2[e] > 10) return false;
+    if (t.lde_tb_log[e] < -1 || t.lde_tb_log[e] > 8) return false;
+  }
+  return t.tree_lane_log >= 4 && t.tree_lane_log <= 30 && t.tree_s128_log >= 7 && t.tree_s128_log <= 30 &&
+         t.grind_wide_bits >= 0 && t.grind_wide_bits <= 32 && t.grind_lock_bits >= 1 && t.grind_lock_bits <= 33 &&
+         t.grind_blocks >= 0 && t.grind_blocks <= (1 << 16);
 }
 
 }  // namespace kb

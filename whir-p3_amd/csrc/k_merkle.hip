@@ -221,12 +221,12 @@ int merkle_leaves(const TreeArgs* a, int count, int row_len, u64 height, int lea
                           : merkle_leaves_impl<16>(a, count, row_len, height, c16, st);
 }
 
-// Levels with more than 2^14 nodes use one lane per node (throughput); below
-// that, seven or five levels per launch with 16 lanes per node (latency), and
-// the last (at most 5) levels in one block.
+// Levels with more than 2^tree_lane_log nodes (default 2^14) use one lane per
+// node (throughput); below that, seven or five levels per launch with 16
+// lanes per node (latency), and the last (at most 5) levels in one block.
 constexpr int TREE_SUB = 32;
 static_assert(TREE_SUB <= TREE_TOP, "the top block takes what the subtree launches leave");
-int merkle_tree(const TreeArgs* a, int count, u64 height, const P2C16& c16, hipStream_t st) {
+int merkle_tree(const TreeArgs* a, int count, u64 height, const P2C16& c16, const Tuning& t, hipStream_t st) {
   if (height == 0 || (height & (height - 1))) return -1;
   if (count < 1 || count > GROUP_MAX) return -8;
   Group<TreeArgs> g;
@@ -235,7 +235,7 @@ int merkle_tree(const TreeArgs* a, int count, u64 height, const P2C16& c16, hipS
   u64 off = 0, n = height;
   while (n > TREE_SUB) {
     const u64 n_out = n >> 1;
-    if (n_out > (1u << 14)) {
+    if (n_out > (1ull << t.tree_lane_log)) {
       unsigned blocks = (unsigned)((n_out + 255) / 256);
       hipLaunchKernelGGL(k_tree_level, dim3(blocks, 1, np), dim3(256), 0, st, g, off, n_out, c16);
       if (hipGetLastError() != hipSuccess) return -5;
@@ -244,11 +244,11 @@ int merkle_tree(const TreeArgs* a, int count, u64 height, const P2C16& c16, hipS
       continue;
     }
     // n is a power of two above S: n / S subtrees, levels n / 2 .. n / S
-    // written.  From 2^15 nodes (256 blocks or more) 128-digest subtrees: the
-    // first level's work then fills the chip and seven levels per launch cost
-    // less than five (22 against 23 us at 2^15); below, 32-digest subtrees at
-    // one wave per SIMD.
-    const u64 S = n >= (1u << 15) ? 128 : TREE_SUB;
+    // written.  From 2^15 nodes (256 blocks or more; tree_s128_log) 128-digest
+    // subtrees: the first level's work then fills the chip and seven levels
+    // per launch cost less than five (22 against 23 us at 2^15); below,
+    // 32-digest subtrees at one wave per SIMD.
+    const u64 S = n >= (1ull << t.tree_s128_log) ? 128 : TREE_SUB;
     if (S == 128)
       hipLaunchKernelGGL(k_tree_sub_wide<128>, dim3((unsigned)(n / 128), 1, np), dim3(8 * 128), 0, st, g, off, n, c16);
     else
@@ -362,12 +362,11 @@ __global__ void __launch_bounds__(256) k_grind(const P2C16* __restrict__ cp, Gro
   }
 }
 
-// Short searches (at most WIDE_GRIND_BITS bits, a few thousand candidates
+// Short searches (at most Tuning::grind_wide_bits bits, 12 by default: a few thousand candidates
 // expected): one candidate per 16-lane row with the lane-parallel
 // permutation, sixteen per block per pass, so a pass takes one permutation's
 // latency (about 2 us) instead of the 8 us of the one-lane form.  Same search
 // order and stop rule as k_grind, with a wave's four candidates as the unit.
-constexpr int WIDE_GRIND_BITS = 12;
 
 __global__ void __launch_bounds__(256) k_grind_wide(const P2C16* __restrict__ cp, Group<GrindArgs> ga, int tr_mode,
                                                     u32 mask, u32 start, u32 end) {
@@ -406,7 +405,7 @@ __global__ void __launch_bounds__(256) k_grind_wide(const P2C16* __restrict__ cp
   }
 }
 
-// Long searches (at least LOCK_GRIND_BITS bits): two waves per SIMD issue
+// Long searches (at least Tuning::grind_lock_bits bits, 17 by default): two waves per SIMD issue
 // about 15% faster than one, but the older wave gets issue priority, so waves
 // left to themselves split each pass into a fast and a slow half (and a
 // witness in the slow half ends the search late).  Here a block holds both
@@ -416,7 +415,6 @@ __global__ void __launch_bounds__(256) k_grind_wide(const P2C16* __restrict__ cp
 // buffered, so a pass's write never meets the previous pass's reads).  The
 // candidates' base state comes from k_grind_base (transcript step) or the
 // host.
-constexpr int LOCK_GRIND_BITS = 17;
 #ifndef LOCK_NT_OVERRIDE
 constexpr int LOCK_NT = 512;
 #else
@@ -487,12 +485,12 @@ static_assert(sizeof(Group<GrindArgs>) <= 4096, "kernel arguments of a grinding 
 // Scans [start, start + span) (clipped to p) for every proof of the group in
 // one launch of `blocks` blocks per proof (fewer when the range is short).
 int grind_range(const P2C16* d_consts, const GrindArgs* a, int count, int bits, int mode, u64 start, u64 span,
-                int blocks, hipStream_t st) {
+                int blocks, const Tuning& t, hipStream_t st) {
   const u32 mask = bits >= 32 ? 0xffffffffu : ((1u << bits) - 1);
   if (start >= P) return 0;
   const u64 end = std::min<u64>(start + span, P);
   if (count < 1 || count > GROUP_MAX) return -8;
-  if (bits <= WIDE_GRIND_BITS) {
+  if (bits <= t.grind_wide_bits) {
     const unsigned nb = wide_grind_blocks(bits, start, end, blocks);
     return launch_group(a, count, [&](const Group<GrindArgs>& g) {
       hipLaunchKernelGGL(k_grind_wide, dim3(nb, 1, count), dim3(256), 0, st, d_consts, g, mode, mask, (u32)start,
@@ -501,7 +499,7 @@ int grind_range(const P2C16* d_consts, const GrindArgs* a, int count, int bits, 
   }
   bool gbase = mode != TR_NONE;
   for (int i = 0; i < count; i++) gbase = gbase && a[i].tr.gbase;
-  if (bits >= LOCK_GRIND_BITS && (mode == TR_NONE || gbase)) {
+  if (bits >= t.grind_lock_bits && (mode == TR_NONE || gbase)) {
     const unsigned nb = (unsigned)std::max<u64>(1, std::min<u64>((end - start + LOCK_NT - 1) / LOCK_NT,
                                                                  (u64)std::max(blocks, 1)));
     return launch_group(a, count, [&](const Group<GrindArgs>& g) {

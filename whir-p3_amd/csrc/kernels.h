@@ -126,6 +126,29 @@ struct Sponge16 {
   u32 s[16];
 };
 
+// ---- Launch shapes a context may tune (chosen by measurement; every value
+// gives the same results).  0 or -1 keeps the built-in rule.
+struct Tuning {
+  // LDE, per element type [base, extension]: the longest column one pass
+  // transforms in LDS (log2; longer ones take the four-step split at it), and
+  // pass 1's lines per block (log2).
+  int lde_lp2[2] = {0, 0};
+  int lde_tb_log[2] = {-1, -1};
+  // Merkle: one lane per node while a level has more than 2^tree_lane_log
+  // nodes; below, 128-digest subtrees from 2^tree_s128_log nodes and
+  // 32-digest ones under that.
+  int tree_lane_log = 14;
+  int tree_s128_log = 15;
+  // Grinding: lane-parallel searches up to grind_wide_bits bits, lockstep
+  // blocks from grind_lock_bits, one lane per candidate between; the blocks a
+  // search takes (0: the context's rule from the CU count).
+  int grind_wide_bits = 12;
+  int grind_lock_bits = 17;
+  int grind_blocks = 0;
+};
+// Whether the values are in the ranges the launchers accept.
+bool tuning_valid(const Tuning& t);
+
 // ---- Reed-Solomon encoding (k_lde.hip)
 // src holds 2^m elements; the output (dst) is 2^(m-k+R) rows of 2^k elements.
 // tmp (same size as the output) is needed when 2^(m-k) exceeds one LDS pass.
@@ -135,11 +158,13 @@ struct LdeArgs {
   T* tmp;
   T* dst;
 };
-bool lde_two_pass(int m, int k, bool ext);
+bool lde_two_pass(int m, int k, bool ext, const Tuning& t);
 // 0 when lde_base / lde_ext support the shape, else the negative code they would return.
-int lde_check(int m, int k, int R, bool ext);
-int lde_base(const LdeArgs<u32>* a, int count, int m, int k, int R, const Twiddles& tw, hipStream_t st);
-int lde_ext(const LdeArgs<ef>* a, int count, int m, int k, int R, const Twiddles& tw, hipStream_t st);
+int lde_check(int m, int k, int R, bool ext, const Tuning& t);
+int lde_base(const LdeArgs<u32>* a, int count, int m, int k, int R, const Twiddles& tw, const Tuning& t,
+             hipStream_t st);
+int lde_ext(const LdeArgs<ef>* a, int count, int m, int k, int R, const Twiddles& tw, const Tuning& t,
+            hipStream_t st);
 
 // ---- Merkle tree and openings (k_merkle.hip)
 // tree holds 2 * height - 1 digests of 8 u32; the root is the last one and
@@ -152,7 +177,7 @@ struct TreeArgs {
 int merkle_leaves(const TreeArgs* a, int count, int row_len, u64 height, int leaf_width, const P2C16* d_c16,
                   const P2C24* d_c24, hipStream_t st);
 // Returns 1 (root not written to root_out) for a single leaf.
-int merkle_tree(const TreeArgs* a, int count, u64 height, const P2C16& c16, hipStream_t st);
+int merkle_tree(const TreeArgs* a, int count, u64 height, const P2C16& c16, const Tuning& t, hipStream_t st);
 // Opening of the rows at idx[0 .. nq) (nq from device memory when given,
 // else max_nq): each row followed by its sibling digests, into out.
 struct OpenArgs {
@@ -176,7 +201,7 @@ struct GrindArgs {
   ScTr tr;
 };
 int grind_range(const P2C16* d_consts, const GrindArgs* a, int count, int bits, int mode, u64 start, u64 span,
-                int blocks, hipStream_t st);
+                int blocks, const Tuning& t, hipStream_t st);
 
 // ---- a round's transcript steps (k_merkle.hip)
 // OOD sample: observe st.obs, sample the challenge y into *log_y, expand it to

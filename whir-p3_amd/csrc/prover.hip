@@ -186,6 +186,7 @@ struct whirgpu_ctx {
   whir::WhirConfig cfg;
   P2C16 c16{};
   P2C24 c24{};
+  Tuning tune;           // launch shapes (whirgpu_set_tuning)
   u32* d_tw = nullptr;   // 2 * 4096 twiddles
   Twiddles tw{};
   Pool pool;
@@ -643,7 +644,8 @@ ef eq_point(const std::vector<ef>& a, const std::vector<ef>& b) { return whir::e
 // it at most, so short grinds leave the device to other work.
 int grind_blocks_for(const whirgpu_ctx* c, int bits) {
   const u64 want = ((16ull << bits) + 255) / 256;
-  return bits <= 12 ? c->grind_blocks : (int)std::max<u64>(1, std::min<u64>((u64)c->grind_blocks, want));
+  const int all = c->tune.grind_blocks > 0 ? c->tune.grind_blocks : c->grind_blocks;
+  return bits <= c->tune.grind_wide_bits ? all : (int)std::max<u64>(1, std::min<u64>((u64)all, want));
 }
 
 // Smallest-witness grinding on the device for each challenger (host state),
@@ -669,7 +671,7 @@ std::vector<u32> gpu_grind(whirgpu_ctx* c, const std::vector<DuplexChallenger*>&
     a[b].result = res + b;
   }
   Span sp(c, ST_GRIND, 0);
-  KCHK(grind_range(c->d_c16, a.data(), B, bits, TR_NONE, 0, P, grind_blocks_for(c, bits), c->st));
+  KCHK(grind_range(c->d_c16, a.data(), B, bits, TR_NONE, 0, P, grind_blocks_for(c, bits), c->tune, c->st));
   const size_t rec = sp.close();
   HIPCHK(hipMemcpyAsync(c->h_hgrind, res, B * sizeof(u32), hipMemcpyDeviceToHost, c->st));
   c->sync();
@@ -829,7 +831,7 @@ ScLaunch sumcheck_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int rounds,
       }
       Span sp(c, ST_GRIND, 0);
       KCHK(grind_range(c->d_c16, ga.data(), B, pow_bits, TR_COEFFS_GRIND, 0, P, grind_blocks_for(c, pow_bits),
-                       c->st));
+                       c->tune, c->st));
       L.grind_recs.push_back(sp.close());
       fold_mode = TR_WITNESS_SAMPLE;
     }
@@ -1114,7 +1116,7 @@ void commit_trees(whirgpu_ctx* c, const std::vector<whirgpu_prover_data*>& pds) 
     KCHK(merkle_leaves(ta.data(), B, p0->row_len, p0->height, lw, c->d_c16, c->d_c24, c->st));
   }
   Span sp(c, ST_TREE, (double)(p0->height - 1) * B);
-  const int rc = merkle_tree(ta.data(), B, p0->height, c->c16, c->st);
+  const int rc = merkle_tree(ta.data(), B, p0->height, c->c16, c->tune, c->st);
   if (rc < 0) KCHK(rc);
   for (whirgpu_prover_data* pd : pds) pd->root_on_host = rc == 0;
 }
@@ -1146,7 +1148,7 @@ void commit_round(whirgpu_ctx* c, const std::vector<Run*>& R, int round) {
   const int kn = cfg.ff(round + 1);
   const int Rl = ilog2u(cfg.inv_rate(round));
   const u64 H = 1ull << (m - kn + Rl), W = 1ull << kn;
-  const bool two = lde_two_pass(m, kn, true);
+  const bool two = lde_two_pass(m, kn, true, c->tune);
   std::vector<Buf<ef>> tmps(B);
   std::vector<LdeArgs<ef>> la(B);
   std::vector<whirgpu_prover_data*> pds(B);
@@ -1166,7 +1168,7 @@ void commit_round(whirgpu_ctx* c, const std::vector<Run*>& R, int round) {
   }
   {
     Span sp(c, ST_LDE_EXT, (16.0 * (double)(1ull << m) + 16.0 * (double)(H * W)) * B);
-    KCHK(lde_ext(la.data(), B, m, kn, Rl, c->tw, c->st));
+    KCHK(lde_ext(la.data(), B, m, kn, Rl, c->tw, c->tune, c->st));
   }
   commit_trees(c, pds);
   for (auto& t : tmps)
@@ -1313,7 +1315,7 @@ RoundLaunch device_round_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int 
     }
     Span sp(c, ST_GRIND, 0);
     KCHK(grind_range(c->d_c16, ga.data(), B, rp.pow_bits, TR_OBS_GRIND, 0, P, grind_blocks_for(c, rp.pow_bits),
-                     c->st));
+                     c->tune, c->st));
     rl.grind_rec = sp.close();
     for (int b = 0; b < B; b++) {
       cin[b] = rch(b);
@@ -1488,7 +1490,7 @@ FinalLaunch device_final_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int 
       ga[b].result = tr.grind;
     }
     Span sp(c, ST_GRIND, 0);
-    KCHK(grind_range(c->d_c16, ga.data(), B, pow, TR_OBS_GRIND, 0, P, grind_blocks_for(c, pow), c->st));
+    KCHK(grind_range(c->d_c16, ga.data(), B, pow, TR_OBS_GRIND, 0, P, grind_blocks_for(c, pow), c->tune, c->st));
     fl.grind_rec = sp.close();
     for (int b = 0; b < B; b++) cin[b] = rch(b);
     ri ^= 1;
@@ -1647,7 +1649,7 @@ void commit_group(whirgpu_ctx* c, const std::vector<Run*>& R) {
   const auto& cfg = c->cfg;
   const int n = cfg.num_variables, k0 = cfg.ff(0), Rr = cfg.params.starting_log_inv_rate;
   const u64 H = 1ull << (n + Rr - k0), W = 1ull << k0;
-  const bool two = lde_two_pass(n, k0, false);
+  const bool two = lde_two_pass(n, k0, false, c->tune);
   std::vector<Buf<u32>> tmps(B);
   std::vector<LdeArgs<u32>> la(B);
   std::vector<whirgpu_prover_data*> pds(B);
@@ -1667,7 +1669,7 @@ void commit_group(whirgpu_ctx* c, const std::vector<Run*>& R) {
   }
   {
     Span sp(c, ST_LDE_BASE, (4.0 * (double)(1ull << n) + 4.0 * (double)(H * W)) * B);
-    KCHK(lde_base(la.data(), B, n, k0, Rr, c->tw, c->st));
+    KCHK(lde_base(la.data(), B, n, k0, Rr, c->tw, c->tune, c->st));
   }
   commit_trees(c, pds);
   for (auto& t : tmps)
@@ -1805,9 +1807,9 @@ int guarded(F&& f) {
 // Every Reed-Solomon encoding of the schedule (the initial commitment and each
 // round's, as whirgpu_commit and commit_round size them) must be a shape the
 // LDE kernels cover; otherwise the configuration is rejected up front.
-void check_lde_shapes(const whir::WhirConfig& cfg) {
+void check_lde_shapes(const whir::WhirConfig& cfg, const Tuning& t) {
   const int n = cfg.num_variables;
-  int rc = lde_check(n, cfg.ff(0), cfg.params.starting_log_inv_rate, false);
+  int rc = lde_check(n, cfg.ff(0), cfg.params.starting_log_inv_rate, false, t);
   if (rc)
     bad("Reed-Solomon encoding of 2^" + std::to_string(n) + " coefficients in rows of 2^" + std::to_string(cfg.ff(0)) +
         " is not supported (code " + std::to_string(rc) + ")");
@@ -1815,7 +1817,7 @@ void check_lde_shapes(const whir::WhirConfig& cfg) {
     const int m = n - cfg.params.folding_factor.total_number(round);
     const int kn = cfg.ff(round + 1);
     const int Rl = ilog2u(cfg.inv_rate(round));
-    rc = lde_check(m, kn, Rl, true);
+    rc = lde_check(m, kn, Rl, true, t);
     if (rc)
       bad("round " + std::to_string(round) + ": Reed-Solomon encoding of 2^" + std::to_string(m) +
           " extension elements in rows of 2^" + std::to_string(kn) + " is not supported (code " +
@@ -1968,7 +1970,7 @@ int whirgpu_create(int device, const whirgpu_params* params, const whirgpu_posei
     if (params->num_variables < 1 || params->num_variables > 28) bad("num_variables must be in [1, 28]");
     std::string err;
     if (!whir::WhirConfig::create((int)params->num_variables, p, &c->cfg, &err)) bad(err);
-    check_lde_shapes(c->cfg);
+    check_lde_shapes(c->cfg, c->tune);
     load_perms(perms, &c->c16, &c->c24);
     if (device < 0) return;   // host-only context: schedule and transcript
     HIPCHK(hipSetDevice(device));
@@ -2197,6 +2199,45 @@ int whirgpu_proof_serialize(const whirgpu_proof* p, uint8_t* out, size_t cap, si
     *len = n;
     if (cap < n || !out) bad("buffer too small", WHIRGPU_BUFFER_TOO_SMALL);
     whir::serialize_into(p->p, out);
+  });
+}
+
+// Launch shapes (no reference counterpart).
+int whirgpu_get_tuning(const whirgpu_ctx* c, whirgpu_tuning* out) {
+  return guarded([&] {
+    if (!c || !out) bad("null argument");
+    const Tuning& t = c->tune;
+    *out = whirgpu_tuning{t.lde_lp2[0], t.lde_lp2[1], t.lde_tb_log[0], t.lde_tb_log[1], t.tree_lane_log,
+                          t.tree_s128_log, t.grind_wide_bits, t.grind_lock_bits, t.grind_blocks};
+  });
+}
+
+int whirgpu_device_name(int device, char* out, size_t cap) {
+  return guarded([&] {
+    if (!out || cap == 0) bad("null argument");
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    snprintf(out, cap, "%s%s%s (%d CUs)", prop.name, prop.name[0] ? " " : "", prop.gcnArchName,
+             prop.multiProcessorCount);
+  });
+}
+
+int whirgpu_set_tuning(whirgpu_ctx* c, const whirgpu_tuning* in) {
+  return guarded([&] {
+    if (!c || !in) bad("null argument");
+    Tuning t;
+    t.lde_lp2[0] = in->lde_lp2_base;
+    t.lde_lp2[1] = in->lde_lp2_ext;
+    t.lde_tb_log[0] = in->lde_tb_log_base;
+    t.lde_tb_log[1] = in->lde_tb_log_ext;
+    t.tree_lane_log = in->tree_lane_log;
+    t.tree_s128_log = in->tree_s128_log;
+    t.grind_wide_bits = in->grind_wide_bits;
+    t.grind_lock_bits = in->grind_lock_bits;
+    t.grind_blocks = in->grind_blocks;
+    if (!tuning_valid(t)) bad("tuning value out of range");
+    check_lde_shapes(c->cfg, t);
+    c->tune = t;
   });
 }
 
@@ -2434,9 +2475,9 @@ int whirgpu_lde_base(whirgpu_ctx* c, const uint32_t* d_src, uint32_t m, uint32_t
     use_device(c);
     const u64 out = 1ull << (m + log_blowup);
     Buf<u32> tmp;
-    if (lde_two_pass((int)m, (int)k, false)) tmp = c->alloc<u32>(out);
+    if (lde_two_pass((int)m, (int)k, false, c->tune)) tmp = c->alloc<u32>(out);
     const LdeArgs<u32> a{d_src, tmp.p, d_dst};
-    KCHK(lde_base(&a, 1, (int)m, (int)k, (int)log_blowup, c->tw, c->st));
+    KCHK(lde_base(&a, 1, (int)m, (int)k, (int)log_blowup, c->tw, c->tune, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     if (tmp.p) c->release(tmp);
   });
@@ -2449,9 +2490,9 @@ int whirgpu_lde_ext(whirgpu_ctx* c, const uint32_t* d_src, uint32_t m, uint32_t 
     use_device(c);
     const u64 out = 1ull << (m + log_blowup);
     Buf<ef> tmp;
-    if (lde_two_pass((int)m, (int)k, true)) tmp = c->alloc<ef>(out);
+    if (lde_two_pass((int)m, (int)k, true, c->tune)) tmp = c->alloc<ef>(out);
     const LdeArgs<ef> a{reinterpret_cast<const ef*>(d_src), tmp.p, reinterpret_cast<ef*>(d_dst)};
-    KCHK(lde_ext(&a, 1, (int)m, (int)k, (int)log_blowup, c->tw, c->st));
+    KCHK(lde_ext(&a, 1, (int)m, (int)k, (int)log_blowup, c->tw, c->tune, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     if (tmp.p) c->release(tmp);
   });
@@ -2465,7 +2506,7 @@ int whirgpu_merkle_commit(whirgpu_ctx* c, const uint32_t* d_rows, uint64_t heigh
     use_device(c);
     const TreeArgs a{d_rows, d_tree, nullptr};
     KCHK(merkle_leaves(&a, 1, (int)row_len, height, (int)c->params.leaf_hash_width, c->d_c16, c->d_c24, c->st));
-    const int rc = merkle_tree(&a, 1, height, c->c16, c->st);
+    const int rc = merkle_tree(&a, 1, height, c->c16, c->tune, c->st);
     if (rc < 0) KCHK(rc);
     c->d2h(root_out, d_tree + (2 * height - 2) * 8, 8);
   });

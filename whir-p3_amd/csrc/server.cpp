@@ -3,7 +3,12 @@
 // with it, up to group_max - 1 more queued jobs of the same shape (number of
 // variables, number of constraints, input on host or device), and proves
 // them as one group (whirgpu_prove_group: every device step one launch for
-// the whole group).  Each worker keeps its own contexts (one per number of
+// the whole group).  Jobs submitted together are queued largest first, so
+// the small sizes fill in at the end, and a worker prefers jobs of a size it
+// already holds a context for (pick_job), so repeated batches reuse contexts
+// and pooled buffers instead of allocating.  (Capping a group at its share of the
+// queued work, to spread the largest sizes over idle workers, was measured
+// and was slower: large groups amortize more than a balanced tail gains.)  Each worker keeps its own contexts (one per number of
 // variables, each with its own HIP stream and buffer pool), so groups of
 // different workers overlap on the device: while one worker waits for a
 // round's transcript values another worker's kernels run.  A group that fails
@@ -28,6 +33,7 @@
 // This file uses the prover only through the C ABI and internal.h (no HIP
 // calls of its own), so its threading is checked on the host, under
 // ThreadSanitizer, against a stand-in library (tests/server_standin.cpp).
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -79,6 +85,7 @@ struct Pending {
 struct Worker {
   std::map<uint32_t, whirgpu_ctx*> ctxs;
   std::map<uint32_t, size_t> bytes;
+  bool idle = false;   // waiting with nothing it may take (under the server's lock)
 };
 
 }  // namespace
@@ -96,6 +103,7 @@ struct whirgpu_server {
   bool self_check = false;   // verify every proof on the host before publishing it
   bool stop = false;
   std::map<uint32_t, size_t> peak;   // largest context bytes per proof measured per job size (size_key)
+  std::map<uint32_t, whirgpu_tuning> tuning;   // launch shapes per number of variables
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<Pending> queue;
@@ -135,7 +143,20 @@ std::vector<int> run_group(whirgpu_server* s, Worker* w, const std::vector<whirg
     whirgpu_params p = s->params;
     p.num_variables = n;
     rc = whirgpu_create(s->device, &p, &s->perms, &ctx);
-    if (rc == WHIRGPU_OK) w->ctxs[n] = ctx;
+    if (rc == WHIRGPU_OK) {
+      w->ctxs[n] = ctx;
+      bool tuned = false;
+      whirgpu_tuning t{};
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto ti = s->tuning.find(n);
+        if (ti != s->tuning.end()) {
+          t = ti->second;
+          tuned = true;
+        }
+      }
+      if (tuned) (void)whirgpu_set_tuning(ctx, &t);   // refused: the built-in shapes stay
+    }
   } else {
     ctx = it->second;
   }
@@ -206,6 +227,31 @@ bool size_held(const whirgpu_server* s, uint32_t n) {
   return false;
 }
 
+// Whether a worker other than w that is idle holds a context for size n.
+bool held_by_idle(const whirgpu_server* s, const Worker& w, uint32_t n) {
+  for (const auto& o : s->workers)
+    if (o.get() != &w && o->idle && o->bytes.count(n)) return true;
+  return false;
+}
+
+// Size affinity (under the lock).  Among the jobs submitted within
+// AFFINITY_MS of the head (jobs submitted together, such as a batch), a
+// worker takes the oldest of a size it holds a context for, so its context
+// and pooled buffers are reused; else the oldest whose size no idle worker
+// holds (an idle holder takes it at once, without creating a context or
+// allocating); else nothing (end(): the worker waits and the holders take the
+// jobs).  Only idle workers are deferred to, and an idle worker with a job of
+// its own size in the window never waits, so a job is always taken.
+constexpr double AFFINITY_MS = 5.0;
+std::deque<Pending>::iterator pick_job(whirgpu_server* s, const Worker& w) {
+  const double horizon = s->queue.front().submitted + AFFINITY_MS;
+  for (auto it = s->queue.begin(); it != s->queue.end() && it->submitted <= horizon; ++it)
+    if (w.bytes.count(it->job->num_variables)) return it;
+  for (auto it = s->queue.begin(); it != s->queue.end() && it->submitted <= horizon; ++it)
+    if (!held_by_idle(s, w, it->job->num_variables)) return it;
+  return s->queue.end();
+}
+
 // Reserved growth plus a job's own growth fits in the device memory free now.
 bool fits_free_memory(const whirgpu_server* s, size_t g) {
   if (g == 0) return true;
@@ -218,16 +264,27 @@ void worker(whirgpu_server* s, Worker* w) {
   for (;;) {
     std::vector<Pending> grp;
     size_t g = 0;
+    bool more = false;
     {
       std::unique_lock<std::mutex> lk(s->mu);
       bool take = false, alone = false;
+      std::deque<Pending>::iterator pick;
       while (!take) {
         if (s->queue.empty()) {
           if (s->stop) break;
+          w->idle = true;
           s->cv_work.wait(lk);
+          w->idle = false;
           continue;
         }
-        const whirgpu_job* j = s->queue.front().job;
+        pick = pick_job(s, *w);
+        if (pick == s->queue.end()) {   // left to idle workers holding those sizes
+          w->idle = true;
+          s->cv_work.wait(lk);
+          w->idle = false;
+          continue;
+        }
+        const whirgpu_job* j = pick->job;
         g = growth(s, *w, j, 1);
         if (s->held + s->reserved + g <= s->budget && fits_free_memory(s, g)) {
           take = true;
@@ -272,8 +329,8 @@ void worker(whirgpu_server* s, Worker* w) {
         s->cv_work.notify_all();   // less is held: others may fit now
       }
       if (!take) break;   // stopped with an empty queue
-      grp.push_back(s->queue.front());
-      s->queue.pop_front();
+      grp.push_back(*pick);
+      s->queue.erase(pick);
       // queued jobs of the same shape join the group while it fits the budget
       for (auto it = s->queue.begin(); !alone && it != s->queue.end() && (int)grp.size() < s->group_max;) {
         if (!same_shape(it->job, grp[0].job)) {
@@ -288,7 +345,9 @@ void worker(whirgpu_server* s, Worker* w) {
       }
       s->reserved += g;
       s->running += grp.size();
+      more = !s->queue.empty();
     }
+    if (more) s->cv_work.notify_all();   // jobs left behind may wait for this worker to be busy
     const double t0 = now_ms();
     std::vector<whirgpu_job*> js;
     for (auto& p : grp) {
@@ -383,11 +442,17 @@ int whirgpu_server_submit_many(whirgpu_server* s, whirgpu_job* const* jobs, size
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) return WHIRGPU_BAD_STATE;
     const double t = now_ms();
-    for (size_t i = 0; i < count; i++) {
-      jobs[i]->status = WHIRGPU_PENDING;
-      jobs[i]->proof_len = 0;
-      jobs[i]->error[0] = 0;
-      s->queue.push_back({jobs[i], t});
+    // jobs submitted together are queued largest first (the longest-processing-
+    // time rule: the small ones fill in at the end), in submission order within a size
+    std::vector<whirgpu_job*> order(jobs, jobs + count);
+    std::stable_sort(order.begin(), order.end(), [](const whirgpu_job* a, const whirgpu_job* b) {
+      return a->num_variables > b->num_variables;
+    });
+    for (whirgpu_job* j : order) {
+      j->status = WHIRGPU_PENDING;
+      j->proof_len = 0;
+      j->error[0] = 0;
+      s->queue.push_back({j, t});
     }
   }
   s->cv_work.notify_all();
@@ -398,6 +463,13 @@ int whirgpu_server_set_group_max(whirgpu_server* s, int group_max) {
   if (!s || group_max < 1 || group_max > GROUP_LIMIT) return WHIRGPU_BAD_PARAMS;
   std::lock_guard<std::mutex> lk(s->mu);
   s->group_max = group_max;
+  return WHIRGPU_OK;
+}
+
+int whirgpu_server_set_tuning(whirgpu_server* s, uint32_t num_variables, const whirgpu_tuning* t) {
+  if (!s || !t) return WHIRGPU_BAD_PARAMS;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->tuning[num_variables] = *t;
   return WHIRGPU_OK;
 }
 
