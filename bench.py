@@ -286,6 +286,26 @@ def verifier_bench(W, cfg, proof, pts, evals, runs=7):
             "runs": runs, "proof_bytes": len(proof)}
 
 
+def keccak_bench(W, n, d_ptr, pts, benchmark_ms, steps=5):
+    """The Keccak MMCS variant (src/whir/mod.rs:236-383: Keccak Merkle tree,
+    Keccak-256 byte challenger) at the benchmark's parameters: commit + prove
+    and the host verifier's decode + verify, beside the Poseidon2 benchmark."""
+    cfg = W.WhirConfig(W.params(n, mmcs="keccak"), device=0)
+    ms, data = timed_proofs(W, cfg, d_ptr, pts, steps)
+    st = W.InitialStatement(cfg, None, device_ptr=d_ptr)
+    evals = np.array([st.evaluate(p) for p in pts], dtype=np.uint32).reshape(len(pts), 4)
+    st.close()
+    times = []
+    for _ in range(5):
+        t = time.perf_counter()
+        W.Verifier(cfg).verify(data, cfg.challenger(), pts, evals)
+        times.append((time.perf_counter() - t) * 1e3)
+    cfg.close()
+    return {"prove_ms": round(ms, 3), "verify_ms": round(float(np.median(times)), 3), "proof_bytes": len(data),
+            "poseidon2_prove_ms": round(benchmark_ms, 3),
+            "note": "same parameters as the benchmark, Keccak Merkle tree and byte challenger; transcript host-driven"}
+
+
 def cpu_baseline(n, poly, pts, gpu_proof, runs=3):
     """The same commit + prove on the CPU oracle (a naive port: canonical 64-bit
     modular arithmetic, OpenMP over Merkle leaves, DFT columns and grinding
@@ -427,6 +447,8 @@ def main():
     dominant = max(stages, key=lambda k: stages[k]["ms"])
     verifier = verifier_bench(W, cfg, proof, pts, evals0)
     schedule = schedule_bench(W, n, d_poly.ptr, pts, ms)
+    keccak = keccak_bench(W, n, d_poly.ptr, pts, ms)
+    progress(f"keccak variant: {keccak['prove_ms']} ms per proof")
 
     progress(f"n={n}: {ms:.3f} ms per proof; profiled steps done")
     kernels = None if args.skip_kernel_bench else sumcheck_eq_bench(W, args)
@@ -471,6 +493,7 @@ def main():
         "verifier": verifier,
         "launch_shapes": "autotuned (cache)" if tuning is not None else "built-in",
         "schedule_autotune": schedule,
+        "keccak_mmcs": keccak,
         "sumcheck_eq_2^26": kernels,
         "batched": batched,
         "serving": serving,

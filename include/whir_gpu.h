@@ -59,7 +59,20 @@ typedef struct whirgpu_params {
   uint32_t sumcheck_strategy;   /* SumcheckStrategy (src/whir/parameters.rs:16-29): 0 Classic, 1 Svo */
   uint32_t leaf_hash_width;     /* 16: PaddingFreeSponge<P2-16,16,8,8>; 24: PaddingFreeSponge<P2-24,24,16,8> */
   uint32_t digest_elems;        /* DIGEST_ELEMS of DomainSeparator::commit_statement (domain_separator.rs:95-105) */
+  uint32_t mmcs;                /* WHIRGPU_MMCS_POSEIDON2 (the benchmark's) or WHIRGPU_MMCS_KECCAK */
 } whirgpu_params;
+
+/* The Merkle commitment and challenger a context proves with.
+ * POSEIDON2: MerkleTreeMmcs over PaddingFreeSponge<Poseidon2, W, W-8, 8> and
+ *   TruncatedPermutation<Poseidon2-16, 2, 8, 16>, DuplexChallenger<F,
+ *   Poseidon2-16, 16, 8> (benches/whir.rs:46-86, main.rs:35-36).
+ * KECCAK: the Keccak variant (src/whir/mod.rs:236-383): MerkleTreeMmcs<F, u64,
+ *   SerializingHasher<PaddingFreeSponge<KeccakF, 25, 17, 4>>,
+ *   CompressionFunctionFromHasher<.., 2, 4>, 4> with
+ *   SerializingChallenger32<F, HashChallenger<u8, Keccak256Hash, 32>>;
+ *   proved with the whirgpu_*_keccak entry points (transcript on the host). */
+#define WHIRGPU_MMCS_POSEIDON2 0u
+#define WHIRGPU_MMCS_KECCAK 1u
 
 /* Poseidon2 round constants (Montgomery form) for the width-16 permutation
  * (challenger, node compression, width-16 leaves) and the width-24 one
@@ -82,6 +95,18 @@ typedef struct whirgpu_challenger {
   uint32_t input_len;
   uint32_t output_len;
 } whirgpu_challenger;
+
+/* The Keccak variant's challenger, SerializingChallenger32<F,
+ * HashChallenger<u8, Keccak256Hash, 32>> (src/whir/mod.rs:257): the
+ * HashChallenger's input bytes and the output bytes not yet popped (popped
+ * from the end). */
+#define WHIRGPU_KECCAK_INPUT_CAP 4096
+typedef struct whirgpu_keccak_challenger {
+  uint8_t input[WHIRGPU_KECCAK_INPUT_CAP];
+  uint32_t input_len;
+  uint8_t output[32];
+  uint32_t output_len;
+} whirgpu_keccak_challenger;
 
 typedef struct whirgpu_ctx whirgpu_ctx;                 /* WhirConfig + device + MMCS */
 typedef struct whirgpu_statement whirgpu_statement;     /* InitialStatement */
@@ -159,6 +184,31 @@ int whirgpu_proof_deserialize(whirgpu_ctx* ctx, const uint8_t* bytes, size_t len
  * verifies. */
 int whirgpu_verify(const whirgpu_ctx* ctx, const whirgpu_proof* proof, whirgpu_challenger* ch, uint32_t npoints,
                    const uint32_t* points, const uint32_t* evals);
+/* The same with the Keccak challenger, for a Keccak context. */
+int whirgpu_verify_keccak(const whirgpu_ctx* ctx, const whirgpu_proof* proof, whirgpu_keccak_challenger* ch,
+                          uint32_t npoints, const uint32_t* points, const uint32_t* evals);
+
+int whirgpu_keccak_challenger_new(whirgpu_keccak_challenger* ch);
+/* observe / sample field elements (Montgomery words), and
+ * DomainSeparator::observe_domain_separator, for a Keccak context.  observe
+ * and sample need no constants and accept ctx == NULL. */
+int whirgpu_keccak_challenger_observe(const whirgpu_ctx* ctx, whirgpu_keccak_challenger* ch, const uint32_t* values,
+                                      size_t n);
+int whirgpu_keccak_challenger_sample(const whirgpu_ctx* ctx, whirgpu_keccak_challenger* ch, uint32_t* out, size_t n);
+int whirgpu_keccak_challenger_observe_domain_separator(const whirgpu_ctx* ctx, whirgpu_keccak_challenger* ch);
+/* CanObserve<u8> of the inner HashChallenger (a digest Hash<F, u64, 4> is
+ * observed as its lanes' little-endian bytes), and
+ * CanSampleBits::sample_bits of SerializingChallenger32 (4 sampled bytes as a
+ * little-endian u32, masked to `bits` <= 31).  ctx may be NULL. */
+int whirgpu_keccak_challenger_observe_bytes(const whirgpu_ctx* ctx, whirgpu_keccak_challenger* ch,
+                                            const uint8_t* bytes, size_t n);
+int whirgpu_keccak_challenger_sample_bits(const whirgpu_ctx* ctx, whirgpu_keccak_challenger* ch, uint32_t bits,
+                                          uint32_t* out);
+/* GrindingChallenger::grind of the Keccak challenger (smallest witness, on
+ * the device), for a Keccak context; updates *ch as whirgpu_grind does. */
+int whirgpu_keccak_grind(whirgpu_ctx* ctx, whirgpu_keccak_challenger* ch, uint32_t bits, uint32_t* witness_out);
+/* Keccak-f[1600] on 25 lanes (lane x + 5y), for checking. */
+int whirgpu_keccak_f1600(uint64_t* state);
 
 /* CommitmentWriter::commit (src/whir/committer/writer.rs:56-109). */
 int whirgpu_commit(whirgpu_ctx* ctx, whirgpu_proof* proof, whirgpu_challenger* ch, whirgpu_statement* st,
@@ -175,9 +225,20 @@ int whirgpu_prover_data_destroy(whirgpu_prover_data* pd);
  * already evaluated, the same number for all), proof object and challenger,
  * run as one group in lockstep, every device step one launch for all of them.
  * Each proof and challenger ends exactly as whirgpu_commit + whirgpu_prove
- * would leave it. */
+ * would leave it.  Poseidon2 contexts only (the transcripts run on the
+ * device). */
 int whirgpu_prove_group(whirgpu_ctx* ctx, uint32_t count, whirgpu_statement* const* statements,
                         whirgpu_proof* const* proofs, whirgpu_challenger* challengers);
+/* CommitmentWriter::commit and Prover::prove with the Keccak challenger, for a
+ * context created with mmcs = WHIRGPU_MMCS_KECCAK (the Keccak variant,
+ * src/whir/mod.rs:236-383): the transcript runs on the host, round by round,
+ * and the Merkle trees are Keccak trees; the proof has the same layout, its
+ * digests 4 u64 lanes.  Grinding, when the schedule has proof-of-work bits,
+ * searches on the host. */
+int whirgpu_commit_keccak(whirgpu_ctx* ctx, whirgpu_proof* proof, whirgpu_keccak_challenger* ch,
+                          whirgpu_statement* st, whirgpu_prover_data** out);
+int whirgpu_prove_keccak(whirgpu_ctx* ctx, whirgpu_proof* proof, whirgpu_keccak_challenger* ch,
+                         const whirgpu_statement* st, whirgpu_prover_data* pd);
 /* InitialStatement::evaluate (initial.rs:143-152) for `count` statements of
  * one context at once (1 to 16), one point each: points holds count * n
  * extension elements, evals_out receives count extension elements. */
@@ -322,7 +383,8 @@ typedef struct whirgpu_server whirgpu_server;
  * bytes (0: 80% of the free HBM at creation), and its growth fits in the HBM
  * free at that moment.  A job that does not fit makes idle workers release
  * contexts it cannot use; with nothing running a job is always admitted.
- * params->num_variables is ignored (each job has its own). */
+ * params->num_variables is ignored (each job has its own).  params->mmcs must
+ * be WHIRGPU_MMCS_POSEIDON2 (WHIRGPU_BAD_PARAMS otherwise). */
 int whirgpu_server_create(int device, const whirgpu_params* params, const whirgpu_poseidon2* perms, int workers,
                           size_t hbm_budget, whirgpu_server** out);
 int whirgpu_server_submit(whirgpu_server* s, whirgpu_job* job);

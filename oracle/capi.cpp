@@ -9,7 +9,7 @@ using namespace orc;
 
 namespace {
 // params: [n, security_level, pow_bits, ff0, ff, log_inv_rate, rs_reduction,
-//          soundness, svo, leaf_width, digest_elems_pattern]
+//          soundness, svo, leaf_width, digest_elems_pattern, mmcs]
 Params read_params(const int* a) {
   Params p;
   p.num_variables = a[0];
@@ -23,6 +23,7 @@ Params read_params(const int* a) {
   p.svo = a[8];
   p.leaf_width = a[9];
   p.digest_elems_pattern = a[10];
+  p.mmcs = a[11];
   return p;
 }
 EF ef_in(const uint32_t* m) { return EF{{from_monty(m[0]), from_monty(m[1]), from_monty(m[2]), from_monty(m[3])}}; }
@@ -152,6 +153,26 @@ int orc_merkle(uint64_t seed, int leaf_width, const uint32_t* rows, size_t heigh
   return 0;
 }
 
+// The Keccak MMCS's MerkleTreeMmcs<F, u64, SerializingHasher<..>, CompressionFunctionFromHasher<..>, 4>
+// over Montgomery rows; root and path digests as 8 raw u32 words (4 u64 lanes, low word first).
+int orc_merkle_keccak(const uint32_t* rows, size_t height, size_t width, uint32_t* root, size_t index,
+                      uint32_t* row_out, uint32_t* path_out) {
+  std::vector<u32> v(height * width);
+  for (size_t i = 0; i < v.size(); i++) v[i] = from_monty(rows[i]);
+  MerkleTree t = merkle_commit(MerkleHashers{nullptr, nullptr, true}, std::move(v), height, width);
+  Digest r = t.root();
+  for (int k = 0; k < 8; k++) root[k] = r[k];
+  if (row_out && path_out) {
+    std::vector<u32> row;
+    std::vector<Digest> path;
+    merkle_open(t, index, &row, &path);
+    for (size_t i = 0; i < row.size(); i++) row_out[i] = to_monty(row[i]);
+    for (size_t l = 0; l < path.size(); l++)
+      for (int k = 0; k < 8; k++) path_out[8 * l + k] = path[l][k];
+  }
+  return 0;
+}
+
 void orc_eq_table(const uint32_t* point, int n, const uint32_t* scale, uint32_t* out) {
   auto t = eq_table(point_in(point, n), ef_in(scale));
   for (size_t i = 0; i < t.size(); i++) ef_out(t[i], out + 4 * i);
@@ -224,11 +245,12 @@ int orc_prove(const int* params, uint64_t perm_seed, const uint32_t* poly, const
   for (int j = 0; j < npts; j++) pts.push_back(point_in(points + (size_t)4 * n * j, n));
   Challenger ch;
   ch.perm = &ps.p16;
+  ch.keccak = p.mmcs == 1;
   for (u32 x : domain_separator(c)) ch.observe(x);
   Statement st;
   Timing tm;
   Proof pr = whir_prove(c, ps.p16, ps.leaf(p.leaf_width), f, pts, ch, &st, &tm);
-  auto bytes = serialize_proof(pr);
+  auto bytes = serialize_proof(pr, p.mmcs == 1);
   *len = bytes.size();
   if (ms) {   // 8 values: commit, prove, then the stages of Timing in declaration order
     const double v[8] = {tm.commit_ms, tm.prove_ms, tm.dft_ms, tm.commit_matrix_ms, tm.ood_ms,
@@ -252,7 +274,7 @@ int orc_verify(const int* params, uint64_t perm_seed, const uint8_t* proof, size
   };
   if (!make_config(p, &c, &err)) return fail(-1);
   Proof pr;
-  if (!deserialize_proof(proof, len, &pr)) { err = "malformed proof bytes"; return fail(-3); }
+  if (!deserialize_proof(proof, len, &pr, p.mmcs == 1)) { err = "malformed proof bytes"; return fail(-3); }
   Perms ps(perm_seed);
   int n = p.num_variables;
   Statement st;
@@ -262,6 +284,7 @@ int orc_verify(const int* params, uint64_t perm_seed, const uint8_t* proof, size
   }
   Challenger ch;
   ch.perm = &ps.p16;
+  ch.keccak = p.mmcs == 1;
   for (u32 x : domain_separator(c)) ch.observe(x);
   if (!whir_verify(c, ps.p16, ps.leaf(p.leaf_width), pr, st, ch, &err)) return fail(1);
   return 0;
@@ -314,5 +337,44 @@ int orc_sumcheck_eq(const uint32_t* f, int n, const uint32_t* point, const uint3
   ef_out(e[0], out_final);
   ef_out(w[0], out_final + 4);
   return ef_eq(ef_mul(e[0], w[0]), sum) ? 0 : 1;
+}
+}
+
+extern "C" {
+// Keccak for known-answer checks: the permutation on 25 lanes, and the
+// 256-bit sponge over bytes with domain pad byte `pad` (0x01: Keccak-256,
+// 0x06: SHA3-256).
+void orc_keccak_f1600(uint64_t* lanes) { keccak_permute(lanes); }
+void orc_keccak256(const uint8_t* msg, size_t len, uint8_t pad, uint8_t* out) { keccak256_bytes(msg, len, out, pad); }
+
+// Grinding from a Keccak challenger whose input buffer holds `len` bytes and
+// whose output is empty: returns the witness (Montgomery form) and the next 4
+// field samples (Montgomery form) after it.
+uint32_t orc_keccak_grind_bytes(const uint8_t* in, size_t len, int bits, uint32_t* next4) {
+  Challenger ch;
+  ch.keccak = true;
+  ch.kin.assign(in, in + len);
+  const u32 w = ch.grind(bits);
+  for (int i = 0; i < 4; i++) next4[i] = to_monty(ch.sample());
+  return to_monty(w);
+}
+
+// A script against the Keccak challenger (ops as orc_challenger_script:
+// 0 observe args[i] (Montgomery), 1 sample (Montgomery out), 2 sample_bits
+// args[i], 3 grind args[i] bits).  Returns the outputs written.
+int orc_keccak_challenger_script(const uint32_t* ops, const uint32_t* args, int n, uint32_t* out) {
+  Challenger ch;
+  ch.keccak = true;
+  int k = 0;
+  for (int i = 0; i < n; i++) {
+    switch (ops[i]) {
+      case 0: ch.observe(from_monty(args[i])); break;
+      case 1: out[k++] = to_monty(ch.sample()); break;
+      case 2: out[k++] = (uint32_t)ch.sample_bits((int)args[i]); break;
+      case 3: out[k++] = ch.grind((int)args[i]); break;
+      default: return -1;
+    }
+  }
+  return k;
 }
 }

@@ -181,6 +181,94 @@ void poseidon2_pair_from_seed(u64 seed, Poseidon2* p16, Poseidon2* p24) {
   *p24 = draw_poseidon2(rng, 24);
 }
 
+// ------------------------------------------------------------------ Keccak
+// FIPS 202 written from its definitions: the state as a 5x5 array of lanes,
+// rho's offsets generated by walking (x, y) -> (y, 2x + 3y) from (1, 0) with
+// offset (t + 1)(t + 2) / 2, and iota's round constants from the rc(t) LFSR
+// (Algorithm 5: x^8 + x^6 + x^5 + x^4 + 1).
+namespace {
+bool keccak_rc_bit(int t) {
+  if (t % 255 == 0) return true;
+  // R = 10000000 as bits R[0..7]; each step prepends a 0 (R[8] is the bit
+  // shifted out), feeds R[8] into bits 0, 4, 5, 6 and truncates to 8 bits
+  uint8_t R[9] = {1, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = 1; i <= t % 255; i++) {
+    for (int k = 8; k > 0; k--) R[k] = R[k - 1];
+    R[0] = 0;
+    R[0] ^= R[8];
+    R[4] ^= R[8];
+    R[5] ^= R[8];
+    R[6] ^= R[8];
+  }
+  return R[0] != 0;
+}
+struct KeccakTables {
+  u64 rc[24];
+  int rho[5][5];
+  KeccakTables() {
+    for (int ir = 0; ir < 24; ir++) {
+      u64 v = 0;
+      for (int j = 0; j <= 6; j++)
+        if (keccak_rc_bit(j + 7 * ir)) v |= 1ull << ((1 << j) - 1);
+      rc[ir] = v;
+    }
+    rho[0][0] = 0;
+    int x = 1, y = 0;
+    for (int t = 0; t < 24; t++) {
+      rho[x][y] = ((t + 1) * (t + 2) / 2) % 64;
+      const int nx = y, ny = (2 * x + 3 * y) % 5;
+      x = nx;
+      y = ny;
+    }
+  }
+};
+const KeccakTables& ktab() {
+  static const KeccakTables t;
+  return t;
+}
+u64 rol(u64 v, int r) { return r ? (v << r) | (v >> (64 - r)) : v; }
+}  // namespace
+
+void keccak_permute(u64* lanes) {
+  const KeccakTables& T = ktab();
+  u64 A[5][5];
+  for (int x = 0; x < 5; x++)
+    for (int y = 0; y < 5; y++) A[x][y] = lanes[x + 5 * y];
+  for (int ir = 0; ir < 24; ir++) {
+    u64 C[5], D[5], B[5][5];
+    for (int x = 0; x < 5; x++) C[x] = A[x][0] ^ A[x][1] ^ A[x][2] ^ A[x][3] ^ A[x][4];
+    for (int x = 0; x < 5; x++) D[x] = C[(x + 4) % 5] ^ rol(C[(x + 1) % 5], 1);
+    for (int x = 0; x < 5; x++)
+      for (int y = 0; y < 5; y++) A[x][y] ^= D[x];
+    for (int x = 0; x < 5; x++)
+      for (int y = 0; y < 5; y++) B[y][(2 * x + 3 * y) % 5] = rol(A[x][y], T.rho[x][y]);
+    for (int x = 0; x < 5; x++)
+      for (int y = 0; y < 5; y++) A[x][y] = B[x][y] ^ (~B[(x + 1) % 5][y] & B[(x + 2) % 5][y]);
+    A[0][0] ^= T.rc[ir];
+  }
+  for (int x = 0; x < 5; x++)
+    for (int y = 0; y < 5; y++) lanes[x + 5 * y] = A[x][y];
+}
+
+void keccak256_bytes(const uint8_t* msg, size_t len, uint8_t out[32], uint8_t pad) {
+  // pad10*1 after the domain bits: message || pad || 0* || 0x80 (0x81 when they share
+  // a byte), to a multiple of the 136-byte rate; pad 0x01 is Keccak-256, 0x06 SHA3-256
+  std::vector<uint8_t> m(msg, msg + len);
+  m.push_back(pad);
+  while (m.size() % 136) m.push_back(0);
+  m.back() |= 0x80;
+  u64 st[25] = {0};
+  for (size_t off = 0; off < m.size(); off += 136) {
+    for (int i = 0; i < 17; i++) {
+      u64 w = 0;
+      for (int b = 7; b >= 0; b--) w = (w << 8) | m[off + 8 * i + b];
+      st[i] ^= w;
+    }
+    keccak_permute(st);
+  }
+  for (int i = 0; i < 32; i++) out[i] = (uint8_t)(st[i / 8] >> (8 * (i % 8)));
+}
+
 // ------------------------------------------------------------------ challenger
 void Challenger::duplex() {
   assert(in.size() <= 8);
@@ -190,17 +278,67 @@ void Challenger::duplex() {
   out.assign(state, state + 8);
 }
 
+uint8_t Challenger::sample_byte() {
+  if (kout.empty()) {
+    uint8_t h[32];
+    keccak256_bytes(kin.data(), kin.size(), h);
+    kout.assign(h, h + 32);
+    kin.assign(h, h + 32);
+  }
+  uint8_t b = kout.back();
+  kout.pop_back();
+  return b;
+}
+
 void Challenger::observe(u32 v) {
+  if (keccak) {
+    kout.clear();
+    for (int i = 0; i < 4; i++) kin.push_back((uint8_t)(v >> (8 * i)));
+    return;
+  }
   out.clear();
   in.push_back(v);
   if (in.size() == 8) duplex();
 }
 
+void Challenger::observe_digest(const Digest& d) {
+  if (!keccak) {
+    for (u32 x : d) observe(x);
+    return;
+  }
+  kout.clear();
+  for (u32 x : d)
+    for (int i = 0; i < 4; i++) kin.push_back((uint8_t)(x >> (8 * i)));
+}
+
 u32 Challenger::sample() {
+  if (keccak) {
+    for (;;) {
+      u32 v = 0;
+      for (int i = 0; i < 4; i++) v |= (u32)sample_byte() << (8 * i);
+      v &= 0x7fffffffu;
+      if (v < P) return v;
+    }
+  }
   if (!in.empty() || out.empty()) duplex();
   u32 v = out.back();
   out.pop_back();
   return v;
+}
+
+// DuplexChallenger: a sampled field element's canonical value, masked.
+// SerializingChallenger32: four sampled bytes read as a little-endian u32,
+// masked, with no rejection.
+u64 Challenger::sample_bits(int bits) {
+  u64 r;
+  if (keccak) {
+    u32 v = 0;
+    for (int i = 0; i < 4; i++) v |= (u32)sample_byte() << (8 * i);
+    r = v;
+  } else {
+    r = sample();
+  }
+  return r & ((1ull << bits) - 1);
 }
 
 bool Challenger::check_witness(int bits, u32 w) {
@@ -212,6 +350,25 @@ bool Challenger::check_witness(int bits, u32 w) {
 // permutation: the buffered inputs plus w overwrite the state prefix, and the
 // sampled element is state[7] of the permuted state.
 u32 Challenger::grind(int bits) {
+  if (keccak) {   // each candidate: a copy of the challenger observes it and samples
+    const u64 CH = 1 << 14;
+    u32 found = P;
+    for (u64 base = 0; base < P && found == P; base += CH) {
+      const u64 end = std::min<u64>(base + CH, P);
+      u32 local_min = P;
+#pragma omp parallel for reduction(min : local_min) schedule(static)
+      for (long long w = (long long)base; w < (long long)end; w++) {
+        Challenger probe = *this;
+        if (probe.check_witness(bits, (u32)w) && (u32)w < local_min) local_min = (u32)w;
+      }
+      found = local_min;
+    }
+    assert(found < P);
+    bool ok = check_witness(bits, found);
+    assert(ok);
+    (void)ok;
+    return found;
+  }
   const u64 mask = (1ull << bits) - 1;
   const size_t L = in.size();
   const u64 CH = 1 << 16;
@@ -255,6 +412,40 @@ Digest sponge_hash(const Poseidon2& perm, const u32* vals, size_t len) {
   return d;
 }
 
+// The Keccak MMCS's lane sponge (PaddingFreeSponge<KeccakF, 25, 17, 4>) over
+// u32 words taken two per u64 lane, low word first; 4 lanes out as 8 words.
+Digest keccak_lanes_hash(const std::vector<u32>& words) {
+  std::vector<u64> lanes;
+  for (size_t i = 0; i < words.size(); i += 2)
+    lanes.push_back((u64)words[i] | (i + 1 < words.size() ? (u64)words[i + 1] << 32 : 0));
+  u64 st[25] = {0};
+  size_t pos = 0;
+  while (pos < lanes.size()) {   // no input: no permutation, a zero digest
+    for (size_t k = 0; k < 17 && pos < lanes.size(); k++) st[k] = lanes[pos++];
+    keccak_permute(st);
+  }
+  Digest d;
+  for (int j = 0; j < 4; j++) {
+    d[2 * j] = (u32)st[j];
+    d[2 * j + 1] = (u32)(st[j] >> 32);
+  }
+  return d;
+}
+
+// A Keccak leaf: SerializingHasher over the row's field elements, each as its
+// Montgomery word (the in-memory KoalaBear), packed two per lane.
+Digest keccak_leaf(const u32* vals, size_t len) {
+  std::vector<u32> w(len);
+  for (size_t i = 0; i < len; i++) w[i] = to_monty(vals[i]);
+  return keccak_lanes_hash(w);
+}
+
+Digest keccak_node(const Digest& l, const Digest& r) {
+  std::vector<u32> w(l.begin(), l.end());
+  w.insert(w.end(), r.begin(), r.end());
+  return keccak_lanes_hash(w);
+}
+
 Digest compress(const Poseidon2& p16, const Digest& l, const Digest& r) {
   u32 st[16];
   for (int k = 0; k < 8; k++) { st[k] = l[k]; st[8 + k] = r[k]; }
@@ -273,13 +464,15 @@ MerkleTree merkle_commit(const MerkleHashers& h, std::vector<u32> rows, size_t h
   t.rows = std::move(rows);
   std::vector<Digest> leaves(height);
 #pragma omp parallel for schedule(static)
-  for (long long i = 0; i < (long long)height; i++) leaves[i] = sponge_hash(*h.leaf, &t.rows[i * width], width);
+  for (long long i = 0; i < (long long)height; i++)
+    leaves[i] = h.keccak ? keccak_leaf(&t.rows[i * width], width) : sponge_hash(*h.leaf, &t.rows[i * width], width);
   t.layers.push_back(std::move(leaves));
   while (t.layers.back().size() > 1) {
     const auto& prev = t.layers.back();
     std::vector<Digest> next(prev.size() / 2);
 #pragma omp parallel for schedule(static)
-    for (long long i = 0; i < (long long)next.size(); i++) next[i] = compress(*h.node, prev[2 * i], prev[2 * i + 1]);
+    for (long long i = 0; i < (long long)next.size(); i++)
+      next[i] = h.keccak ? keccak_node(prev[2 * i], prev[2 * i + 1]) : compress(*h.node, prev[2 * i], prev[2 * i + 1]);
     t.layers.push_back(std::move(next));
   }
   return t;
@@ -301,10 +494,12 @@ bool merkle_verify(const MerkleHashers& h, const Digest& root, size_t height, si
   size_t depth = 0;
   while ((1ull << depth) < height) depth++;
   if (path.size() != depth) return false;
-  Digest cur = sponge_hash(*h.leaf, row.data(), row.size());
+  Digest cur = h.keccak ? keccak_leaf(row.data(), row.size()) : sponge_hash(*h.leaf, row.data(), row.size());
   size_t idx = index;
   for (const auto& sib : path) {
-    cur = (idx & 1) ? compress(*h.node, sib, cur) : compress(*h.node, cur, sib);
+    const Digest& l = (idx & 1) ? sib : cur;
+    const Digest& r = (idx & 1) ? cur : sib;
+    cur = h.keccak ? keccak_node(l, r) : compress(*h.node, l, r);
     idx >>= 1;
   }
   return cur == root;

@@ -92,24 +92,41 @@ struct SmallRng {
   EF ext() { EF e; for (int k = 0; k < 4; k++) e.c[k] = field(); return e; }
 };
 
+// ---------------------------------------------------------------- Keccak
+// Keccak-f[1600] on lanes a[x + 5y] and Keccak-256 (original padding), for
+// the Keccak MMCS variant (src/whir/mod.rs:236-383).
+void keccak_permute(u64* lanes25);
+void keccak256_bytes(const uint8_t* msg, size_t len, uint8_t out[32], uint8_t pad = 0x01);
+
+using Digest = std::array<u32, 8>;
+
 // ---------------------------------------------------------------- challenger
-// DuplexChallenger<F, Poseidon2-16, 16, 8>.
+// DuplexChallenger<F, Poseidon2-16, 16, 8>; with `keccak` set,
+// SerializingChallenger32<F, HashChallenger<u8, Keccak256Hash, 32>> instead:
+// a field element is observed as the 4 little-endian bytes of its (canonical)
+// value, a sample with no output bytes left hashes the input bytes (which are
+// then replaced by the hash: chaining) and output bytes are taken from the end;
+// a field sample is 4 bytes as a little-endian u32, masked to 31 bits,
+// rejected until below p.
 struct Challenger {
   const Poseidon2* perm = nullptr;
   u32 state[16] = {0};
   std::vector<u32> in, out;
+  bool keccak = false;
+  std::vector<uint8_t> kin, kout;
   void duplex();
+  uint8_t sample_byte();
   void observe(u32 v);
+  void observe_digest(const Digest& d);   // field elements, or (Keccak) the words' bytes
   void observe_ef(const EF& e) { for (int k = 0; k < 4; k++) observe(e.c[k]); }
   u32 sample();
   EF sample_ef() { EF e; for (int k = 0; k < 4; k++) e.c[k] = sample(); return e; }
-  u64 sample_bits(int bits) { return (u64)sample() & ((1ull << bits) - 1); }
+  u64 sample_bits(int bits);   // Keccak: four raw bytes, no rejection
   bool check_witness(int bits, u32 w);
   u32 grind(int bits);
 };
 
 // ---------------------------------------------------------------- Merkle
-using Digest = std::array<u32, 8>;
 Digest sponge_hash(const Poseidon2& perm, const u32* vals, size_t len);  // PaddingFreeSponge<P,W,W-8... >
 Digest compress(const Poseidon2& p16, const Digest& l, const Digest& r);
 struct MerkleTree {
@@ -121,6 +138,7 @@ struct MerkleTree {
 struct MerkleHashers {
   const Poseidon2* leaf;   // width 16 (rate 8) or 24 (rate 16)
   const Poseidon2* node;   // width 16
+  bool keccak = false;     // the Keccak MMCS: digests are 4 u64 lanes (8 raw words)
 };
 MerkleTree merkle_commit(const MerkleHashers& h, std::vector<u32> rows, size_t height, size_t width);
 void merkle_open(const MerkleTree& t, size_t index, std::vector<u32>* row, std::vector<Digest>* path);
@@ -158,6 +176,7 @@ struct Params {
   int svo = 1;                       // SumcheckStrategy::Svo
   int leaf_width = 16;
   int digest_elems_pattern = 8;      // DIGEST_ELEMS in the domain separator
+  int mmcs = 0;                      // 0 Poseidon2, 1 Keccak (MMCS and challenger)
 };
 struct RoundCfg {
   int pow_bits, folding_pow_bits, num_queries, ood_samples, num_variables, folding_factor;
@@ -210,8 +229,10 @@ struct Proof {
   bool has_final_sumcheck = false;
   SumcheckData final_sumcheck;
 };
-std::vector<uint8_t> serialize_proof(const Proof& p);
-bool deserialize_proof(const uint8_t* buf, size_t len, Proof* p);
+// raw_digests: digest words are written as they are (Keccak lanes) instead of
+// as field elements in Montgomery form.
+std::vector<uint8_t> serialize_proof(const Proof& p, bool raw_digests = false);
+bool deserialize_proof(const uint8_t* buf, size_t len, Proof* p, bool raw_digests = false);
 
 // ---------------------------------------------------------------- protocol
 struct Statement {                      // InitialStatement (normalized view)

@@ -404,12 +404,18 @@ std::vector<u64> stir_queries(Challenger& ch, u64 domain_size, int folding_facto
 namespace {
 struct W {
   std::vector<uint8_t> b;
+  bool raw_digests = false;
   void u8(uint8_t v) { b.push_back(v); }
   void u32v(u32 v) { for (int i = 0; i < 4; i++) b.push_back((v >> (8 * i)) & 0xff); }
   void u64v(u64 v) { for (int i = 0; i < 8; i++) b.push_back((v >> (8 * i)) & 0xff); }
   void f(u32 canon) { u32v(to_monty(canon)); }
   void ef(const EF& e) { for (int k = 0; k < 4; k++) f(e.c[k]); }
-  void dig(const Digest& d) { for (int k = 0; k < 8; k++) f(d[k]); }
+  void dig(const Digest& d) {
+    for (int k = 0; k < 8; k++) {
+      if (raw_digests) u32v(d[k]);
+      else f(d[k]);
+    }
+  }
   void str(const char* s) { u64 n = strlen(s); u64v(n); b.insert(b.end(), s, s + n); }
   void sc(const SumcheckData& s) {
     u64v(s.evals.size());
@@ -438,13 +444,14 @@ struct R {
   const uint8_t* p;
   size_t n, pos = 0;
   bool ok = true;
+  bool raw_digests = false;
   bool need(size_t k) { if (pos + k > n) ok = false; return ok; }
   uint8_t u8() { if (!need(1)) return 0; return p[pos++]; }
   u32 u32v() { if (!need(4)) return 0; u32 v = 0; for (int i = 0; i < 4; i++) v |= (u32)p[pos + i] << (8 * i); pos += 4; return v; }
   u64 u64v() { if (!need(8)) return 0; u64 v = 0; for (int i = 0; i < 8; i++) v |= (u64)p[pos + i] << (8 * i); pos += 8; return v; }
   u32 f() { u32 m = u32v(); if (m >= P) ok = false; return from_monty(m); }
   EF ef() { EF e; for (int k = 0; k < 4; k++) e.c[k] = f(); return e; }
-  Digest dig() { Digest d; for (int k = 0; k < 8; k++) d[k] = f(); return d; }
+  Digest dig() { Digest d; for (int k = 0; k < 8; k++) d[k] = raw_digests ? u32v() : f(); return d; }
   u64 len(size_t unit) { u64 l = u64v(); if (l > (n - pos) / (unit ? unit : 1) + 1) { ok = false; return 0; } return l; }
   std::string str() { u64 l = len(1); if (!need(l)) return ""; std::string s((const char*)p + pos, l); pos += l; return s; }
   SumcheckData sc() {
@@ -480,8 +487,9 @@ struct R {
 };
 }  // namespace
 
-std::vector<uint8_t> serialize_proof(const Proof& pr) {
+std::vector<uint8_t> serialize_proof(const Proof& pr, bool raw_digests) {
   W w;
+  w.raw_digests = raw_digests;
   w.u8(1);
   w.dig(pr.initial_commitment);
   w.u64v(pr.initial_ood.size());
@@ -507,8 +515,9 @@ std::vector<uint8_t> serialize_proof(const Proof& pr) {
   return w.b;
 }
 
-bool deserialize_proof(const uint8_t* buf, size_t len, Proof* out) {
+bool deserialize_proof(const uint8_t* buf, size_t len, Proof* out, bool raw_digests) {
   R r{buf, len};
+  r.raw_digests = raw_digests;
   Proof pr;
   if (r.u8() != 1) return false;
   pr.initial_commitment = r.dig();
@@ -727,7 +736,7 @@ std::vector<u32> lde_ext(const std::vector<EF>& f, int m, int k, u64 height) {
   return mat;
 }
 
-void observe_digest(Challenger& ch, const Digest& d) { for (u32 x : d) ch.observe(x); }
+void observe_digest(Challenger& ch, const Digest& d) { ch.observe_digest(d); }
 
 std::vector<EF> row_as_ef(const std::vector<u32>& row, bool ext) {
   std::vector<EF> v;
@@ -754,7 +763,7 @@ QueryOpening open_query(const MerkleTree& t, bool ext, u64 idx) {
 Proof whir_prove(const Config& cfg, const Poseidon2& p16, const Poseidon2& leaf, const std::vector<u32>& poly,
                  const std::vector<std::vector<EF>>& user_points, Challenger& ch, Statement* stmt_out, Timing* tm) {
   const int n = cfg.p.num_variables;
-  const MerkleHashers hs{&leaf, &p16};
+  const MerkleHashers hs{&leaf, &p16, cfg.p.mmcs == 1};
   Proof pr;
   Statement st;
   for (auto& z : user_points) {
@@ -949,7 +958,7 @@ EF combine_evals(const VConstraint& c) {
 bool whir_verify(const Config& cfg, const Poseidon2& p16, const Poseidon2& leaf, const Proof& pr,
                  const Statement& stmt, Challenger& ch, std::string* err) {
   const int n = cfg.p.num_variables;
-  const MerkleHashers hs{&leaf, &p16};
+  const MerkleHashers hs{&leaf, &p16, cfg.p.mmcs == 1};
   auto parse = [&](const Digest& root, const std::vector<EF>& answers, int nv, int ood, VConstraint* c) -> bool {
     observe_digest(ch, root);
     if ((int)answers.size() < ood) { *err = "missing OOD answers"; return false; }

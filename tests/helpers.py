@@ -70,6 +70,11 @@ def oracle():
             "orc_rng_state": (None, [ctypes.c_uint64, U64P]),
             "orc_sumcheck_eq": (ctypes.c_int, [U32P, ctypes.c_int, U32P, U32P, ctypes.c_uint64, U32P, U32P]),
             "orc_xoshiro_outputs": (None, [U64P, U64P, sz]),
+            "orc_keccak_f1600": (None, [U64P]),
+            "orc_merkle_keccak": (ctypes.c_int, [U32P, sz, sz, U32P, sz, U32P, U32P]),
+            "orc_keccak_grind_bytes": (ctypes.c_uint32, [ctypes.POINTER(ctypes.c_uint8), sz, ctypes.c_int, U32P]),
+            "orc_keccak256": (None, [ctypes.POINTER(ctypes.c_uint8), sz, ctypes.c_uint8, ctypes.POINTER(ctypes.c_uint8)]),
+            "orc_keccak_challenger_script": (ctypes.c_int, [U32P, U32P, ctypes.c_int, U32P]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -80,9 +85,13 @@ def oracle():
 
 
 def oracle_params(n, *, security_level=100, pow_bits=16, ff=4, ff0=None, rate=1, rs=3, soundness=2, svo=1,
-                  leaf_width=16, digest_elems=8):
-    return (ctypes.c_int * 11)(n, security_level, pow_bits, ff if ff0 is None else ff0, ff, rate, rs, soundness, svo,
-                               leaf_width, digest_elems)
+                  leaf_width=16, digest_elems=None, mmcs=0):
+    """mmcs 0: Poseidon2 Merkle tree and duplex challenger; 1: the Keccak tree and byte challenger (whose domain
+    separator counts 4 digest elements unless told otherwise)."""
+    if digest_elems is None:
+        digest_elems = 4 if mmcs == 1 else 8
+    return (ctypes.c_int * 12)(n, security_level, pow_bits, ff if ff0 is None else ff0, ff, rate, rs, soundness, svo,
+                               leaf_width, digest_elems, mmcs)
 
 
 def oracle_prove(prm, poly_monty, points_monty, perm_seed=1):

@@ -501,6 +501,12 @@ int scenario_drain() {
   whirgpu_server_destroy(s);
   for (auto& r : reqs)
     if (!check(r)) return fail("job not finished by destroy");
+  // a Keccak configuration is refused: groups prove with the duplex transcript
+  whirgpu_params kp{};
+  kp.mmcs = WHIRGPU_MMCS_KECCAK;
+  whirgpu_poseidon2 perms{};
+  whirgpu_server* ks = nullptr;
+  if (whirgpu_server_create(0, &kp, &perms, 2, 0, &ks) != WHIRGPU_BAD_PARAMS || ks) return fail("keccak accepted");
   printf("drain ok\n");
   return 0;
 }

@@ -111,6 +111,34 @@ def test_merkle_root_and_openings(W, leaf_width, log_h, row_len):
         off += H >> l
 
 
+# row lengths: odd (a last lane holding one word), one lane, exactly 17 lanes
+# (one absorption, no extra permutation), 18 lanes and 64 words (two
+# absorptions)
+@pytest.mark.parametrize("log_h,row_len", [(0, 16), (1, 3), (5, 16), (8, 1), (9, 64), (4, 34), (6, 35), (3, 36),
+                                           (10, 20), (14, 16), (16, 16)])
+def test_keccak_merkle_root_and_openings(W, log_h, row_len):
+    cfg = ctx(W, mmcs="keccak")
+    rng = np.random.default_rng(log_h * 11 + row_len)
+    H = 1 << log_h
+    rows = rand_monty(rng, H * row_len)
+    d_rows = dev(rows)
+    d_tree = empty((2 * H - 1) * 8)
+    root = np.zeros(8, np.uint32)
+    W.check(W.lib().whirgpu_merkle_commit(cfg.handle, d_rows.ptr, H, row_len, d_tree.ptr, u32p(root)))
+    want = np.zeros(8, np.uint32)
+    idx = int(rng.integers(0, H))
+    row = np.zeros(row_len, np.uint32)
+    path = np.zeros(8 * max(log_h, 1), np.uint32)
+    oracle().orc_merkle_keccak(u32p(rows), H, row_len, u32p(want), idx, u32p(row), u32p(path))
+    assert np.array_equal(root, want)
+    tree = host(d_tree).reshape(-1, 8)
+    off = 0
+    for l in range(log_h):
+        assert np.array_equal(tree[off + ((idx >> l) ^ 1)], path[8 * l: 8 * l + 8])
+        off += H >> l
+    cfg.close()
+
+
 @pytest.mark.parametrize("bits", [1, 4, 9, 12, 13, 14, 17, 18])
 @pytest.mark.parametrize("n_obs", [0, 3, 7, 8, 13])
 def test_grind_smallest_witness(W, bits, n_obs):

@@ -1,4 +1,4 @@
-"""Plonky3 conventions (CONVENTIONS.md C1-C14) against fixtures emitted by the
+"""Plonky3 conventions (CONVENTIONS.md C1-C18) against fixtures emitted by the
 real Plonky3 @ c38eb05 on a machine with a Rust toolchain.
 
 The fixture is tests/golden/plonky3/conventions.json.  This image has no Rust
@@ -20,7 +20,13 @@ need them:
   rs-reduction 1, lambda 32, 5 PoW bits, CapacityBound, SVO, the polynomial of
   SmallRng seed 0 and one constraint at the point of SmallRng seed 1
   (40 draws), Poseidon2 constants of SmallRng seed 1 (grinding witnesses are
-  the smallest ones, CONVENTIONS.md C12).
+  the smallest ones, CONVENTIONS.md C12);
+- keccak_merkle_8x16_iota: the same matrix under the Keccak MMCS, digests as
+  lists of 4 u64: {"root", "open3_row", "open3_path"};
+- keccak_challenger_script: as challenger_script, for a fresh
+  SerializingChallenger32<F, HashChallenger<u8, Keccak256Hash, 32>>;
+- proof_n10_keccak_hex: the proof_n10_hex statement and parameters proved with
+  the Keccak MMCS and challenger (domain separator with 4 digest elements).
 """
 import ctypes
 import json
@@ -151,4 +157,41 @@ def test_c14_proof_bytes(fx):
     poly = oracle_sample(0, 1 << n)
     pts = [oracle_sample(1, 4 * n).reshape(n, 4)]
     got, _, _ = oracle_prove(oracle_params(n, security_level=32, pow_bits=5, ff=2, rs=1), poly, pts)
+    assert got == want
+
+
+def lanes_as_words(lanes):
+    out = []
+    for v in lanes:
+        out += [int(v) & 0xFFFFFFFF, int(v) >> 32]
+    return words(out)
+
+
+def test_c16_keccak_merkle(fx):
+    m = need(fx, "keccak_merkle_8x16_iota")
+    root = np.zeros(8, np.uint32)
+    row = np.zeros(16, np.uint32)
+    path = np.zeros(24, np.uint32)
+    oracle().orc_merkle_keccak(u32p(iota(128)), 8, 16, u32p(root), 3, u32p(row), u32p(path))
+    assert np.array_equal(root, lanes_as_words(m["root"]))
+    assert np.array_equal(row, words(m["open3_row"]))
+    assert np.array_equal(path, np.concatenate([lanes_as_words(d) for d in m["open3_path"]]))
+
+
+def test_c17_keccak_challenger(fx):
+    sc = need(fx, "keccak_challenger_script")
+    ops = words([o[0] for o in sc["ops"]])
+    args = words([o[1] if len(o) > 1 else 0 for o in sc["ops"]])
+    out = np.zeros(len(sc["outputs"]), np.uint32)
+    k = oracle().orc_keccak_challenger_script(u32p(ops), u32p(args), len(ops), u32p(out))
+    assert k == len(sc["outputs"])
+    assert [int(x) for x in out] == [int(x) for x in sc["outputs"]]
+
+
+def test_c18_keccak_proof_bytes(fx):
+    want = bytes.fromhex(need(fx, "proof_n10_keccak_hex"))
+    n = 10
+    poly = oracle_sample(0, 1 << n)
+    pts = [oracle_sample(1, 4 * n).reshape(n, 4)]
+    got, _, _ = oracle_prove(oracle_params(n, security_level=32, pow_bits=5, ff=2, rs=1, mmcs=1), poly, pts)
     assert got == want

@@ -5,7 +5,7 @@ result is the one its inputs give, alone or in a mix; a memory budget that
 fits k contexts admits exactly k jobs at once and is never exceeded; idle
 workers release contexts of other sizes so the queue moves; a job whose
 creation fails reports its error text in the job; destroying the server
-finishes every submitted job; jobs submitted together form groups of one
+finishes every submitted job (and a Keccak configuration is refused); jobs submitted together form groups of one
 shape up to the group limit; with the self-check on, no proof is published
 unverified and a rejected one fails only its own job; repeated batches reuse
 the workers' contexts (size affinity) instead of creating new ones."""

@@ -23,7 +23,7 @@ class Params(ctypes.Structure):
     _fields_ = [(name, ctypes.c_uint32) for name in (
         "num_variables", "security_level", "pow_bits", "folding_factor_first", "folding_factor",
         "starting_log_inv_rate", "rs_domain_initial_reduction_factor", "soundness_type",
-        "sumcheck_strategy", "leaf_hash_width", "digest_elems")]
+        "sumcheck_strategy", "leaf_hash_width", "digest_elems", "mmcs")]
 
 
 class Poseidon2Constants(ctypes.Structure):
@@ -35,6 +35,15 @@ class Poseidon2Constants(ctypes.Structure):
         ("w24_external_terminal", ctypes.c_uint32 * 96),
         ("w24_internal", ctypes.c_uint32 * 23),
     ]
+
+
+class KeccakChallengerState(ctypes.Structure):
+    """whirgpu_keccak_challenger: SerializingChallenger32<F, HashChallenger<u8, Keccak256Hash, 32>>."""
+    _fields_ = [("input", ctypes.c_uint8 * 4096), ("input_len", ctypes.c_uint32), ("output", ctypes.c_uint8 * 32),
+                ("output_len", ctypes.c_uint32)]
+
+
+MMCS = {"poseidon2": 0, "keccak": 1}
 
 
 class ChallengerState(ctypes.Structure):
@@ -99,6 +108,22 @@ _SIGS = {
                                                  ctypes.POINTER(_VP)]),
     "whirgpu_verify": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ChallengerState), ctypes.c_uint32, U32P, U32P]),
     "whirgpu_commit": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ChallengerState), _VP, ctypes.POINTER(_VP)]),
+    "whirgpu_commit_keccak": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(KeccakChallengerState), _VP, ctypes.POINTER(_VP)]),
+    "whirgpu_prove_keccak": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(KeccakChallengerState), _VP, _VP]),
+    "whirgpu_verify_keccak": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(KeccakChallengerState), ctypes.c_uint32, U32P,
+                                             U32P]),
+    "whirgpu_keccak_challenger_new": (ctypes.c_int, [ctypes.POINTER(KeccakChallengerState)]),
+    "whirgpu_keccak_challenger_observe": (ctypes.c_int, [_VP, ctypes.POINTER(KeccakChallengerState), U32P,
+                                                         ctypes.c_size_t]),
+    "whirgpu_keccak_challenger_sample": (ctypes.c_int, [_VP, ctypes.POINTER(KeccakChallengerState), U32P,
+                                                        ctypes.c_size_t]),
+    "whirgpu_keccak_challenger_observe_domain_separator": (ctypes.c_int, [_VP, ctypes.POINTER(KeccakChallengerState)]),
+    "whirgpu_keccak_f1600": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint64)]),
+    "whirgpu_keccak_grind": (ctypes.c_int, [_VP, ctypes.POINTER(KeccakChallengerState), ctypes.c_uint32, U32P]),
+    "whirgpu_keccak_challenger_observe_bytes": (ctypes.c_int, [_VP, ctypes.POINTER(KeccakChallengerState),
+                                                               ctypes.POINTER(ctypes.c_uint8), ctypes.c_size_t]),
+    "whirgpu_keccak_challenger_sample_bits": (ctypes.c_int, [_VP, ctypes.POINTER(KeccakChallengerState),
+                                                             ctypes.c_uint32, U32P]),
     "whirgpu_prove": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ChallengerState), _VP, _VP]),
     "whirgpu_prover_data_destroy": (ctypes.c_int, [_VP]),
     "whirgpu_prove_group": (ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.POINTER(_VP), ctypes.POINTER(_VP),
@@ -176,12 +201,16 @@ def u32p(a):
 
 def params(num_variables, *, security_level=100, pow_bits=16, folding_factor=4, folding_factor_first=None,
            starting_log_inv_rate=1, rs_domain_initial_reduction_factor=3, soundness="CapacityBound",
-           svo=True, leaf_hash_width=16, digest_elems=8):
-    """ProtocolParameters + num_variables, with the 2^24 benchmark's defaults."""
+           svo=True, leaf_hash_width=16, digest_elems=None, mmcs="poseidon2"):
+    """ProtocolParameters + num_variables, with the 2^24 benchmark's defaults.  mmcs="keccak" selects the
+    Keccak variant (src/whir/mod.rs:236-383), whose domain separator counts 4 digest elements."""
+    m = MMCS.get(mmcs, mmcs)
+    if digest_elems is None:
+        digest_elems = 4 if m == 1 else 8
     return Params(num_variables, security_level, pow_bits,
                   folding_factor if folding_factor_first is None else folding_factor_first, folding_factor,
                   starting_log_inv_rate, rs_domain_initial_reduction_factor,
-                  SOUNDNESS.get(soundness, soundness), 1 if svo else 0, leaf_hash_width, digest_elems)
+                  SOUNDNESS.get(soundness, soundness), 1 if svo else 0, leaf_hash_width, digest_elems, m)
 
 
 def device_name(device=0):
@@ -281,6 +310,13 @@ class WhirConfig:
         return np.array(out[:n.value], dtype=np.uint32)
 
     def challenger(self, observe_domain_separator=True):
+        """The context's challenger (duplex Poseidon2, or Keccak for a Keccak context)."""
+        if self.params.mmcs == 1:
+            ch = KeccakChallengerState()
+            check(lib().whirgpu_keccak_challenger_new(ctypes.byref(ch)))
+            if observe_domain_separator:
+                check(lib().whirgpu_keccak_challenger_observe_domain_separator(self.handle, ctypes.byref(ch)))
+            return ch
         ch = ChallengerState()
         check(lib().whirgpu_challenger_new(ctypes.byref(ch)))
         if observe_domain_separator:
@@ -412,8 +448,8 @@ class CommitmentWriter:
 
     def commit(self, proof, challenger, statement):
         pd = ctypes.c_void_p()
-        check(lib().whirgpu_commit(self.cfg.handle, proof.handle, ctypes.byref(challenger), statement.handle,
-                                   ctypes.byref(pd)))
+        fn = lib().whirgpu_commit_keccak if isinstance(challenger, KeccakChallengerState) else lib().whirgpu_commit
+        check(fn(self.cfg.handle, proof.handle, ctypes.byref(challenger), statement.handle, ctypes.byref(pd)))
         return pd
 
 
@@ -422,8 +458,8 @@ class Prover:
         self.cfg = cfg
 
     def prove(self, proof, challenger, statement, prover_data):
-        check(lib().whirgpu_prove(self.cfg.handle, proof.handle, ctypes.byref(challenger), statement.handle,
-                                  prover_data))
+        fn = lib().whirgpu_prove_keccak if isinstance(challenger, KeccakChallengerState) else lib().whirgpu_prove
+        check(fn(self.cfg.handle, proof.handle, ctypes.byref(challenger), statement.handle, prover_data))
 
 
 class Verifier:
@@ -442,8 +478,8 @@ class Verifier:
         pts = np.ascontiguousarray(np.concatenate([np.asarray(p, np.uint32).reshape(-1) for p in points])
                                    if npts else np.zeros(4, np.uint32))
         evs = np.ascontiguousarray(np.asarray(evals, np.uint32).reshape(-1) if npts else np.zeros(4, np.uint32))
-        check(lib().whirgpu_verify(self.cfg.handle, proof.handle, ctypes.byref(challenger), npts, u32p(pts),
-                                   u32p(evs)))
+        fn = lib().whirgpu_verify_keccak if isinstance(challenger, KeccakChallengerState) else lib().whirgpu_verify
+        check(fn(self.cfg.handle, proof.handle, ctypes.byref(challenger), npts, u32p(pts), u32p(evs)))
 
 
 def verify_bytes(cfg, proof_bytes, points, evals):
@@ -473,6 +509,9 @@ def prove_group(cfg, polys, points_per_proof, device_ptrs=None):
     """Commit + prove of several polynomials of cfg's size in one group
     (whirgpu_prove_group), each with its own constraints (the same number for
     all); returns [(proof bytes, evaluations)] in input order."""
+    if cfg.params.mmcs != MMCS["poseidon2"]:
+        raise WhirError("group proving runs the duplex challenger's transcript on the device: "
+                        "a Keccak context proves one proof at a time")
     count = len(polys) if device_ptrs is None else len(device_ptrs)
     sts = []
     for i in range(count):

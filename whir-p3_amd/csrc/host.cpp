@@ -280,8 +280,49 @@ u32 DuplexChallenger::sample() {
   return out[--out_len];
 }
 
+void KeccakChallenger::observe_byte(uint8_t b) {
+  out_len = 0;
+  in.push_back(b);
+}
+
+uint8_t KeccakChallenger::sample_byte() {
+  if (out_len == 0) {
+    keccak256(in.data(), in.size(), out);
+    in.assign(out, out + 32);   // chaining: the output becomes the next input
+    out_len = 32;
+  }
+  return out[--out_len];
+}
+
+void KeccakChallenger::observe_field(u32 monty) {
+  const u32 v = from_monty(monty);
+  for (int b = 0; b < 4; b++) observe_byte((uint8_t)(v >> (8 * b)));
+}
+
+u32 KeccakChallenger::sample_u32() {
+  u32 v = 0;
+  for (int b = 0; b < 4; b++) v |= (u32)sample_byte() << (8 * b);
+  return v;
+}
+
+u32 KeccakChallenger::sample_field() {
+  for (;;) {
+    const u32 v = sample_u32() & 0x7fffffffu;   // 2^ceil(log2 p) - 1
+    if (v < P) return to_monty(v);
+  }
+}
+
+void Challenger::observe_digest(const std::array<u32, 8>& d) {
+  if (!keccak) {
+    dx.observe_slice(d.data(), d.size());
+    return;
+  }
+  for (u32 w : d)
+    for (int b = 0; b < 4; b++) kc.observe_byte((uint8_t)(w >> (8 * b)));
+}
+
 std::vector<u64> get_challenge_stir_queries(u64 domain_size, int folding_factor, int num_queries,
-                                            DuplexChallenger& ch) {
+                                            Challenger& ch) {
   const u64 folded = domain_size >> folding_factor;
   const int bits = ilog2(folded);
   const int max_bits = 20;   // min(F::bits() - 1, 20)

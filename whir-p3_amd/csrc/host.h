@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "field.h"
+#include "keccak.h"
 #include "poseidon2.h"
 
 namespace whir {
@@ -90,8 +91,56 @@ struct DuplexChallenger {
   bool check_witness(int bits, u32 w_monty) { observe(w_monty); return sample_bits(bits) == 0; }
 };
 
+// SerializingChallenger32<F, HashChallenger<u8, Keccak256Hash, 32>>
+// (p3-challenger), the challenger of the Keccak MMCS variant
+// (src/whir/mod.rs:257).  The inner HashChallenger keeps a byte input buffer
+// and a 32-byte output buffer: observing a byte clears the output and appends
+// to the input; sampling a byte with an empty output flushes (Keccak-256 of
+// the whole input, which is then replaced by the 32 output bytes: chaining)
+// and pops the output from its end.  A field element is observed as the four
+// little-endian bytes of its canonical value and sampled by rejection: four
+// sampled bytes as a little-endian u32, masked to 31 bits, kept when below p.
+// sample_bits takes four sampled bytes as a little-endian u32 and masks it to
+// the bits asked for, with no rejection (so PoW checks and query indices read
+// raw bytes).
+struct KeccakChallenger {
+  static constexpr size_t INPUT_CAP = 4096;   // bytes between two flushes (the C ABI's buffer)
+  std::vector<uint8_t> in;
+  uint8_t out[32] = {0};
+  int out_len = 0;
+  void observe_byte(uint8_t b);
+  uint8_t sample_byte();
+  u32 sample_u32();   // four sampled bytes, little endian
+  void observe_field(u32 monty);
+  u32 sample_field();   // Montgomery form
+};
+
+// The transcript of a proof: the Poseidon2 duplex challenger, or the Keccak
+// challenger when the context's MMCS is Keccak.  Field elements are
+// Montgomery-form words either way; a digest is observed as field elements
+// (Poseidon2) or as its bytes (Keccak: four u64 lanes, little endian).
+struct Challenger {
+  bool keccak = false;
+  DuplexChallenger dx;
+  KeccakChallenger kc;
+  void observe(u32 v) {
+    if (keccak) kc.observe_field(v);
+    else dx.observe(v);
+  }
+  void observe_ef(const ef& e) { for (int k = 0; k < 4; k++) observe(e.c[k]); }
+  void observe_slice(const u32* v, size_t n) { for (size_t i = 0; i < n; i++) observe(v[i]); }
+  void observe_digest(const std::array<u32, 8>& d);
+  u32 sample() { return keccak ? kc.sample_field() : dx.sample(); }
+  ef sample_ef() { ef e; for (int k = 0; k < 4; k++) e.c[k] = sample(); return e; }
+  u64 sample_bits(int bits) {
+    const u64 r = keccak ? (u64)kc.sample_u32() : (u64)kb::from_monty(dx.sample());
+    return r & ((1ull << bits) - 1);
+  }
+  bool check_witness(int bits, u32 w_monty) { observe(w_monty); return sample_bits(bits) == 0; }
+};
+
 // get_challenge_stir_queries (src/whir/utils.rs:24-145).
-std::vector<u64> get_challenge_stir_queries(u64 domain_size, int folding_factor, int num_queries, DuplexChallenger& ch);
+std::vector<u64> get_challenge_stir_queries(u64 domain_size, int folding_factor, int num_queries, Challenger& ch);
 
 // ---------------------------------------------------------------- RNG
 // rand's SmallRng on 64-bit targets: xoshiro256++ seeded through SplitMix64.

@@ -18,6 +18,8 @@
 #include "transcript.h"
 #include "poseidon2.h"
 #include "poseidon2_wide.h"
+#include "keccak.h"
+#include "keccak_pair.h"
 
 namespace kb {
 
@@ -277,6 +279,131 @@ int merkle_open(const OpenArgs* a, int count, int row_len, u64 height, int max_n
   });
 }
 
+// ------------------------------------------------------------------ Keccak MMCS
+// MerkleTreeMmcs<F, u64, SerializingHasher<U64Hash>, CompressionFunctionFromHasher<U64Hash, 2, 4>, 4> with
+// U64Hash = PaddingFreeSponge<KeccakF, 25, 17, 4> (src/whir/mod.rs:254-259).  A row's field elements enter
+// the sponge as u64 lanes two at a time, element 2i in the low half and 2i + 1 in the high half, each as
+// its Montgomery word: the row's memory read as little-endian u64 (an odd last element alone).  A node is
+// the sponge over its children's 4 + 4 lanes.  A digest's 4 lanes are 8 u32 words in the tree buffer,
+// low half first, so the tree layout and openings are those of the Poseidon2 tree.  One lane per leaf or
+// node: a Keccak-f[1600] on 64-bit lanes is a long chain of 32-bit pair operations with ample ILP.
+__global__ void __launch_bounds__(256) k_leaf_keccak(Group<TreeArgs> g, int row_len, u64 height) {
+  const TreeArgs& a = g.v[blockIdx.z];
+  const u64 i = (u64)blockIdx.x * 256 + threadIdx.x;
+  if (i >= height) return;
+  const u32* row = a.rows + i * (u64)row_len;
+  // lane j of a chunk = words 2j (low half) and 2j + 1 (high half) of the row
+  u32 lo[25], hi[25];
+#pragma unroll
+  for (int j = 0; j < 25; j++) lo[j] = hi[j] = 0;
+  for (int at = 0; 2 * at < row_len; at += 17) {
+    // unrolled with a guard, so the state stays in registers (a loop bound
+    // known only at run time would index it in scratch memory)
+#pragma unroll
+    for (int j = 0; j < 17; j++) {
+      const int e = 2 * (at + j);
+      if (e < row_len) {
+        lo[j] = row[e];
+        hi[j] = e + 1 < row_len ? row[e + 1] : 0u;
+      }
+    }
+    keccak_f1600_halves(lo, hi);
+  }
+  u32* out = a.tree + i * 8;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    out[2 * j] = lo[j];
+    out[2 * j + 1] = hi[j];
+  }
+}
+
+// CompressionFunctionFromHasher<U64Hash, 2, 4> over tree levels: node =
+// the lane sponge of left || right (8 lanes, one permutation), computed by a
+// lane pair (keccak_pair.h; lane 2p + h handles node p's half h, h = 0: the
+// low words of the digest lanes).  levels + 1 levels per launch: block b
+// computes nodes [b * S, b * S + S) of the first level (S = 2^levels <= 128)
+// from the digests at `off`, then halves them through LDS up to one node,
+// writing every level to the tree (openings read them).  Level widths:
+// n_out, n_out / 2, ...; the last level written has n_out / S nodes.  The
+// root, when reached, also goes to root_out.  The pairs halve the latency of
+// each level against one lane per node, which the upper levels are bound
+// by; measured, they were also no slower on the widest levels.
+__global__ void __launch_bounds__(256) k_tree_keccak(Group<TreeArgs> g, u64 off, u64 n_out, int levels) {
+  const TreeArgs& a = g.v[blockIdx.z];
+  u32* tree = a.tree;
+  u32* root_out = a.root_out;
+  __shared__ u32 sd[128 * 8];
+  const int S = 1 << levels;
+  const int t = threadIdx.x, p = t >> 1, h = t & 1;
+  u64 in_off = off, out_off = off + 2 * n_out, width = n_out;
+  u64 base = (u64)blockIdx.x * S;
+  for (int k = S; k >= 1; k >>= 1) {
+    u32 d[4];
+    if (p < k) {   // pairs are active together
+      const u32* in = k == S ? tree + (in_off + 2 * (base + p)) * 8 : &sd[(2 * p) * 8];
+      u32 st[25];
+#pragma unroll
+      for (int j = 0; j < 25; j++) st[j] = 0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        st[j] = in[2 * j + h];
+        st[4 + j] = in[8 + 2 * j + h];
+      }
+      keccak_f1600_pair(st, h);
+#pragma unroll
+      for (int j = 0; j < 4; j++) d[j] = st[j];
+      u32* out = tree + (out_off + base + p) * 8;
+#pragma unroll
+      for (int j = 0; j < 4; j++) out[2 * j + h] = d[j];
+      if (width == 1 && root_out)
+#pragma unroll
+        for (int j = 0; j < 4; j++) root_out[2 * j + h] = d[j];
+    }
+    __syncthreads();   // every read of sd by this level is done
+    if (p < k)
+#pragma unroll
+      for (int j = 0; j < 4; j++) sd[p * 8 + 2 * j + h] = d[j];
+    __syncthreads();
+    in_off = out_off;
+    out_off += width;
+    width >>= 1;
+    base >>= 1;
+  }
+}
+
+int keccak_leaves(const TreeArgs* a, int count, int row_len, u64 height, hipStream_t st) {
+  if (height == 0 || (height & (height - 1)) || row_len < 1) return -1;
+  const unsigned blocks = (unsigned)((height + 255) / 256);
+  return launch_group(a, count, [&](const Group<TreeArgs>& g) {
+    hipLaunchKernelGGL(k_leaf_keccak, dim3(blocks, 1, count), dim3(256), 0, st, g, row_len, height);
+  });
+}
+
+int keccak_tree(const TreeArgs* a, int count, u64 height, hipStream_t st) {
+  if (height == 0 || (height & (height - 1))) return -1;
+  if (count < 1 || count > GROUP_MAX) return -8;
+  if (height == 1) return 1;   // single leaf: the root was not written to root_out
+  Group<TreeArgs> g;
+  for (int i = 0; i < count; i++) g.v[i] = a[i];
+  u64 off = 0;
+  for (u64 n = height; n > 1;) {
+    const u64 n_out = n >> 1;
+    int levels = 0;   // S = 2^levels first-level nodes per block, at most 128
+    while (levels < 7 && (2ull << levels) <= n_out) levels++;
+    hipLaunchKernelGGL(k_tree_keccak, dim3((unsigned)(n_out >> levels), 1, (unsigned)count), dim3(256), 0, st, g,
+                       off, n_out, levels);
+    if (hipGetLastError() != hipSuccess) return -5;
+    // levels + 1 levels written: widths n_out .. n_out >> levels
+    u64 w = n;
+    for (int l = 0; l <= levels; l++) {
+      off += w;
+      w >>= 1;
+    }
+    n = w;
+  }
+  return 0;
+}
+
 // ------------------------------------------------------------------ grinding
 // GrindingChallenger::grind for DuplexChallenger (used at src/whir/proof.rs:187-189,
 // src/whir/prover/mod.rs:223-225, 362-364): the smallest w in [0, p) such that,
@@ -298,6 +425,84 @@ int merkle_open(const OpenArgs* a, int count, int row_len, u64 height, int max_n
 // result is the smallest witness in the range.
 __device__ __forceinline__ u32 best_known(const u32* result) {
   return __hip_atomic_load(result, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// GrindingChallenger::grind for SerializingChallenger32<F, HashChallenger<u8,
+// Keccak256Hash, 32>> (p3-challenger; the Keccak variant, src/whir/mod.rs:257),
+// smallest witness: check_witness observes w (4 bytes appended to the input,
+// the output cleared) and sample_bits pops 4 bytes from the end of
+// Keccak-256(input), i.e. output bytes 31, 30, 29, 28 as a little-endian u32
+// = the byte-reversed high word of digest lane 3.  Each candidate costs one
+// permutation (two when the padding spills into another block).  Candidates
+// advance in passes over the grid; a wave stops once the best witness known
+// lies at or before its first candidate of the pass, as in k_grind.
+__global__ void __launch_bounds__(256) k_grind_keccak(KeccakGrindArgs a, u32 mask, u32 start, u32 end) {
+  // The constant lanes and, per lane, masks that place the witness go to LDS
+  // once per block: held in scalar registers across the loop (as values or
+  // as lane-compare masks) they would spill.
+  __shared__ uint64_t s0[25], t1[17], m0lo[25], m0hi[25], m1hi[17];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < 25; j++) {
+      s0[j] = a.s0[j];
+      m0lo[j] = j == a.lo_lane ? ~0ull : 0;
+      m0hi[j] = a.hi_blk == 0 && j == a.hi_lane ? ~0ull : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < 17; j++) {
+      t1[j] = a.t1[j];
+      m1hi[j] = a.hi_blk == 1 && j == a.hi_lane ? ~0ull : 0;
+    }
+  }
+  u32* result = a.result;
+  const int nblk = a.nblk;
+  const int shift = a.shift;
+  __syncthreads();
+  const u32 stride = gridDim.x * 256;
+  const u32 lane = blockIdx.x * 256 + threadIdx.x;
+  const u32 wave0 = lane & ~63u;
+  for (u64 g = start; g + wave0 < end; g += stride) {
+    const u64 first = g + wave0;
+    if ((u64)__builtin_amdgcn_readfirstlane(best_known(result)) <= first) break;
+    const u64 w = g + lane;
+    const uint64_t wlo = (uint64_t)(u32)w << shift;
+    const uint64_t whi = shift > 32 ? (uint64_t)(u32)w >> (64 - shift) : 0;
+    // an offset the compiler cannot see through: the LDS reads stay in the
+    // loop instead of being hoisted into registers across it
+    int o = 0;
+    asm volatile("" : "+v"(o));
+    u32 lo[25], hi[25];
+#pragma unroll
+    for (int j = 0; j < 25; j++) {
+      const uint64_t x = s0[o + j] ^ (wlo & m0lo[o + j]) ^ (whi & m0hi[o + j]);
+      lo[j] = (u32)x;
+      hi[j] = (u32)(x >> 32);
+    }
+#pragma unroll 1
+    for (int b = 0; b < nblk; b++) {   // one call site: the permutation's code appears once
+      if (b == 1) {
+#pragma unroll
+        for (int j = 0; j < 17; j++) {
+          const uint64_t x = t1[o + j] ^ (whi & m1hi[o + j]);
+          lo[j] ^= (u32)x;
+          hi[j] ^= (u32)(x >> 32);
+        }
+      }
+      keccak_f1600_halves<24>(lo, hi);
+    }
+    const u32 v = __builtin_bswap32(hi[3]);
+    if (w < end && (v & mask) == 0) atomicMin(result, (u32)w);
+  }
+}
+
+int keccak_grind(const KeccakGrindArgs& a, int bits, int blocks, hipStream_t st) {
+  if (bits < 1 || bits > 30 || a.nblk < 1 || a.nblk > 2 || a.lo_blk != 0 || a.lo_lane < 0 || a.lo_lane > 16 ||
+      a.hi_blk < 0 || a.hi_blk >= a.nblk || a.hi_lane < 0 || a.hi_lane > 16 || a.shift < 0 || a.shift > 56 ||
+      a.shift % 8)
+    return -1;
+  const u32 mask = (1u << bits) - 1;
+  hipLaunchKernelGGL(k_grind_keccak, dim3((unsigned)std::max(blocks, 1)), dim3(256), 0, st, a, mask, 0u, P);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 
 // Orders a phase boundary: the state registers pass through an empty asm
@@ -537,7 +742,8 @@ __global__ void __launch_bounds__(256) k_ood_eq_tables(Group<OodArgs> g, int m, 
   const WideLane wl = wide_lane(c, t & 15);
   BlockChallenger ch{sp, in, out, 0, 0, &wl, &c};
   ch.load(*st.ch_in);
-  for (int k = 0; k < st.n_obs; k++) ch.observe(st.obs[k]);
+  __shared__ u32 stage[OBS_STAGE];
+  ch.observe_words(st.obs, st.n_obs, stage);
   const ef y = ch.sample_ef();
   if (t == 0) {
     ef cur = y;
@@ -617,7 +823,8 @@ __global__ void __launch_bounds__(256) k_round_queries(Group<QueryArgs> g, int p
     const u64 pmask = pow_bits >= 32 ? 0xffffffffull : ((1ull << pow_bits) - 1);
     if (((u64)from_monty(ch.sample()) & pmask) != 0) err = 1;
   } else {
-    for (int k = 0; k < st.n_obs; k++) ch.observe(st.obs[k]);
+    __shared__ u32 stage[OBS_STAGE];
+    ch.observe_words(st.obs, st.n_obs, stage);
   }
   if (!final_step) (void)ch.sample();
   const u64 folded = domain_size >> ff;

@@ -73,6 +73,9 @@ struct ScTr {
 // host replays them from this log after the round's sumcheck.
 constexpr int RT_MAX_OOD = 4;
 constexpr int RT_MAX_QUERIES = 256;
+// Largest run of words a transcript step observes from memory at once (the
+// final polynomial, at most 64 extension elements).
+constexpr int OBS_STAGE = 256;
 constexpr int RT_MAX_VARS = 32;
 struct RoundLog {
   ef y[RT_MAX_OOD];    // OOD challenges as sampled (before expansion)
@@ -178,6 +181,25 @@ int merkle_leaves(const TreeArgs* a, int count, int row_len, u64 height, int lea
                   const P2C24* d_c24, hipStream_t st);
 // Returns 1 (root not written to root_out) for a single leaf.
 int merkle_tree(const TreeArgs* a, int count, u64 height, const P2C16& c16, const Tuning& t, hipStream_t st);
+// The Keccak MMCS (k_merkle.hip): leaves and the tree above them, same layout.
+int keccak_leaves(const TreeArgs* a, int count, int row_len, u64 height, hipStream_t st);
+int keccak_tree(const TreeArgs* a, int count, u64 height, hipStream_t st);
+// Smallest-witness grinding for the Keccak byte challenger.  A candidate w
+// changes only its 4 bytes of the hashed input.  s0 is the sponge state after
+// the blocks before those bytes with the next block (input, padding, witness
+// bytes zero) XORed in; t1 is the block after it when the padding spills over
+// (nblk == 2).  w << shift is XORed into lane lo_lane of block lo_blk (0), and
+// w >> (64 - shift) into lane hi_lane of block hi_blk when shift > 32.  w
+// passes when the byte-reversed high word of lane 3 of the digest has its low
+// `bits` bits clear.
+struct KeccakGrindArgs {
+  uint64_t s0[25];
+  uint64_t t1[17];
+  u32* result;   // smallest passing witness (atomic minimum), P beforehand
+  int nblk;
+  int lo_blk, lo_lane, hi_blk, hi_lane, shift;
+};
+int keccak_grind(const KeccakGrindArgs& a, int bits, int blocks, hipStream_t st);
 // Opening of the rows at idx[0 .. nq) (nq from device memory when given,
 // else max_nq): each row followed by its sibling digests, into out.
 struct OpenArgs {

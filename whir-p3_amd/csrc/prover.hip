@@ -30,6 +30,7 @@
 
 using namespace kb;
 using whir::Digest;
+using whir::Challenger;
 using whir::DuplexChallenger;
 
 namespace {
@@ -176,6 +177,7 @@ struct Slot {
   RoundBufs rb{};
   // Pinned host copy of the final polynomial, taken before the final sumcheck folds it.
   static constexpr int FINAL_MAX_LOG = 6;
+  static_assert((4 << FINAL_MAX_LOG) <= OBS_STAGE, "the final polynomial is observed in one staged run");
   ef* h_final = nullptr;
 };
 
@@ -652,15 +654,21 @@ int grind_blocks_for(const whirgpu_ctx* c, int bits) {
 // one launch for all of them; updates each challenger like
 // GrindingChallenger::grind (observe the witness, sample bits) and returns
 // the witnesses (Montgomery form).
-std::vector<u32> gpu_grind(whirgpu_ctx* c, const std::vector<DuplexChallenger*>& chs, int bits) {
-  const int B = (int)chs.size();
-  if (B < 1 || B > GROUP_MAX) bad("grind group size", WHIRGPU_BAD_STATE);
+// GROUP_MAX result words for one grind, set to p (no witness yet); the slots
+// are reset together once all have been used.
+u32* grind_slot(whirgpu_ctx* c) {
   if (c->gslot == whirgpu_ctx::GRIND_SLOTS) {
     std::vector<u32> init((size_t)whirgpu_ctx::GRIND_SLOTS * GROUP_MAX, P);
     c->h2d(c->d_hgrind, init.data(), init.size());
     c->gslot = 0;
   }
-  u32* res = c->d_hgrind + (size_t)c->gslot++ * GROUP_MAX;
+  return c->d_hgrind + (size_t)c->gslot++ * GROUP_MAX;
+}
+
+std::vector<u32> gpu_grind(whirgpu_ctx* c, const std::vector<DuplexChallenger*>& chs, int bits) {
+  const int B = (int)chs.size();
+  if (B < 1 || B > GROUP_MAX) bad("grind group size", WHIRGPU_BAD_STATE);
+  u32* res = grind_slot(c);
   std::vector<GrindArgs> a(B);
   for (int b = 0; b < B; b++) {
     const DuplexChallenger& ch = *chs[b];
@@ -688,6 +696,73 @@ std::vector<u32> gpu_grind(whirgpu_ctx* c, const std::vector<DuplexChallenger*>&
   return out;
 }
 
+// Smallest-witness grinding for the Keccak byte challenger on the device
+// (k_grind_keccak): the host absorbs the input blocks before the witness's
+// bytes, lays out the rest of the input with its padding, and replays the
+// found witness on its challenger (which also checks it).
+u32 keccak_gpu_grind(whirgpu_ctx* c, Challenger& ch, int bits) {
+  constexpr size_t RATE = 136;
+  const std::vector<uint8_t>& in = ch.kc.in;
+  const size_t L = in.size();
+  const size_t M = L + 4;                     // the witness's bytes appended
+  const size_t padded = (M / RATE + 1) * RATE;  // at least one padding byte
+  const size_t bw = L / RATE;                 // the block holding the witness's first byte
+  KeccakGrindArgs a;
+  memset(&a, 0, sizeof a);
+  auto lane_at = [](const uint8_t* p) {
+    uint64_t x = 0;
+    for (int k = 0; k < 8; k++) x |= (uint64_t)p[k] << (8 * k);
+    return x;
+  };
+  for (size_t b = 0; b < bw; b++) {
+    for (size_t j = 0; j < RATE / 8; j++) a.s0[j] ^= lane_at(&in[b * RATE + 8 * j]);
+    keccak_f1600(a.s0);
+  }
+  a.nblk = (int)(padded / RATE - bw);
+  std::vector<uint8_t> tail((size_t)a.nblk * RATE, 0);
+  for (size_t i = bw * RATE; i < L; i++) tail[i - bw * RATE] = in[i];
+  tail[M - bw * RATE] ^= 0x01;
+  tail.back() ^= 0x80;
+  for (int j = 0; j < 17; j++) a.s0[j] ^= lane_at(&tail[8 * j]);
+  if (a.nblk == 2)
+    for (int j = 0; j < 17; j++) a.t1[j] = lane_at(&tail[RATE + 8 * j]);
+  const size_t rel = L - bw * RATE;
+  a.lo_blk = 0;
+  a.lo_lane = (int)(rel / 8);
+  a.shift = (int)(8 * (rel % 8));
+  const size_t next = (rel / 8 + 1) * 8;   // the lane after, when the bytes run over
+  a.hi_blk = a.shift > 32 ? (int)(next / RATE) : a.lo_blk;
+  a.hi_lane = a.shift > 32 ? (int)((next % RATE) / 8) : a.lo_lane;
+  if (a.hi_blk >= a.nblk) bad("Keccak grind layout", WHIRGPU_BAD_STATE);
+  a.result = grind_slot(c);
+  const int all = c->tune.grind_blocks > 0 ? c->tune.grind_blocks : c->grind_blocks;
+  const int blocks = (int)std::max<u64>(1, std::min<u64>((u64)all, ((4ull << bits) + 255) / 256));
+  Span sp(c, ST_GRIND, 0);
+  KCHK(keccak_grind(a, bits, blocks, c->st));
+  const size_t rec = sp.close();
+  HIPCHK(hipMemcpyAsync(c->h_hgrind, a.result, sizeof(u32), hipMemcpyDeviceToHost, c->st));
+  c->sync();
+  const u32 w = c->h_hgrind[0];
+  if (w >= P) bad("no proof-of-work witness below p", WHIRGPU_GRIND_FAILED);
+  if (rec != SIZE_MAX) c->recs[rec].work = (double)w + 1;
+  const u32 wm = to_monty(w);
+  if (!ch.check_witness(bits, wm)) bad("grinding witness rejected", WHIRGPU_GRIND_FAILED);
+  return wm;
+}
+
+// Grinding for each challenger of a group: on the device for the Poseidon2
+// duplex challenger, on the host for the Keccak one.
+std::vector<u32> grind_all(whirgpu_ctx* c, const std::vector<Challenger*>& chs, int bits) {
+  if (chs[0]->keccak) {
+    std::vector<u32> out;
+    for (Challenger* ch : chs) out.push_back(keccak_gpu_grind(c, *ch, bits));
+    return out;
+  }
+  std::vector<DuplexChallenger*> dx;
+  for (Challenger* ch : chs) dx.push_back(&ch->dx);
+  return gpu_grind(c, dx, bits);
+}
+
 // Device-resident product polynomial (ProductPolynomial, src/sumcheck/product_polynomial.rs).
 struct SumcheckState {
   Buf<ef> e, w;
@@ -699,7 +774,7 @@ struct SumcheckState {
 struct Run {
   whirgpu_statement* st = nullptr;
   whir::WhirProof* P = nullptr;
-  DuplexChallenger ch;
+  Challenger ch;
   Slot* sl = nullptr;
   SumcheckState s;
   whirgpu_prover_data* prev = nullptr;   // the commitment the next STIR queries open
@@ -896,7 +971,7 @@ void device_sumcheck(whirgpu_ctx* c, const std::vector<Run*>& R, int rounds, int
   if (rounds <= 0) return;
   std::vector<const DevChallenger*> cin;
   for (Run* r : R) {
-    upload_challenger(r->sl, r->ch);
+    upload_challenger(r->sl, r->ch.dx);
     cin.push_back(r->sl->d_chal_init);
   }
   const ScLaunch la = sumcheck_launch(c, R, rounds, pow_bits, cin);
@@ -1044,7 +1119,7 @@ void initial_sumcheck(whirgpu_ctx* c, const std::vector<Run*>& R, int l0, int po
   }
   const whirgpu_statement* st0 = R[0]->st;
   if (l0 > 0 && st0->seg_bits == l0 && st0->n > l0) {
-    std::vector<DuplexChallenger*> chs;
+    std::vector<Challenger*> chs;
     for (Run* r : R) chs.push_back(&r->ch);
     for (int i = 0; i < l0; i++) {
       for (int b = 0; b < B; b++) {
@@ -1056,7 +1131,7 @@ void initial_sumcheck(whirgpu_ctx* c, const std::vector<Run*>& R, int l0, int po
         r.ch.observe_ef(c2);
       }
       if (pow_bits > 0) {
-        const auto ws = gpu_grind(c, chs, pow_bits);
+        const auto ws = grind_all(c, chs, pow_bits);
         for (int b = 0; b < B; b++) R[b]->P->initial_sumcheck.pow_witnesses.push_back(ws[b]);
       }
       for (int b = 0; b < B; b++) {
@@ -1089,7 +1164,7 @@ void initial_sumcheck(whirgpu_ctx* c, const std::vector<Run*>& R, int l0, int po
         r.P->initial_sumcheck.polynomial_evaluations.push_back({c0, c2});
         r.ch.observe_ef(c0);
         r.ch.observe_ef(c2);
-        if (pow_bits > 0) r.P->initial_sumcheck.pow_witnesses.push_back(gpu_grind(c, {&r.ch}, pow_bits)[0]);
+        if (pow_bits > 0) r.P->initial_sumcheck.pow_witnesses.push_back(grind_all(c, {&r.ch}, pow_bits)[0]);
         return r.ch.sample_ef();
       }));
     }
@@ -1111,12 +1186,15 @@ void commit_trees(whirgpu_ctx* c, const std::vector<whirgpu_prover_data*>& pds) 
     pd->sl->root_next = (pd->sl->root_next + 1) & 7;
     ta[b] = TreeArgs{pd->rows.p, pd->tree.p, pd->sl->d_root + 8 * pd->root_slot};
   }
+  const bool keccak = c->params.mmcs == WHIRGPU_MMCS_KECCAK;
   {
-    Span sp(c, ST_LEAVES, (double)p0->height * (double)((p0->row_len + rate - 1) / rate) * B);
-    KCHK(merkle_leaves(ta.data(), B, p0->row_len, p0->height, lw, c->d_c16, c->d_c24, c->st));
+    Span sp(c, ST_LEAVES, keccak ? 0.0 : (double)p0->height * (double)((p0->row_len + rate - 1) / rate) * B);
+    KCHK(keccak ? keccak_leaves(ta.data(), B, p0->row_len, p0->height, c->st)
+                : merkle_leaves(ta.data(), B, p0->row_len, p0->height, lw, c->d_c16, c->d_c24, c->st));
   }
-  Span sp(c, ST_TREE, (double)(p0->height - 1) * B);
-  const int rc = merkle_tree(ta.data(), B, p0->height, c->c16, c->tune, c->st);
+  Span sp(c, ST_TREE, keccak ? 0.0 : (double)(p0->height - 1) * B);
+  const int rc = keccak ? keccak_tree(ta.data(), B, p0->height, c->st)
+                        : merkle_tree(ta.data(), B, p0->height, c->c16, c->tune, c->st);
   if (rc < 0) KCHK(rc);
   for (whirgpu_prover_data* pd : pds) pd->root_on_host = rc == 0;
 }
@@ -1134,9 +1212,7 @@ Digest read_root(whirgpu_ctx* c, const whirgpu_prover_data* pd, bool completed =
   return d;
 }
 
-void observe_digest(DuplexChallenger& ch, const Digest& d) {
-  for (u32 x : d) ch.observe(x);
-}
+void observe_digest(Challenger& ch, const Digest& d) { ch.observe_digest(d); }
 
 // Commitment of round r's folded polynomial for every proof (prover/mod.rs:
 // 158-190): 2^(m - kn + R) rows of 2^kn extension elements and their tree.
@@ -1224,6 +1300,7 @@ void parse_openings(const whirgpu_prover_data* pd, const u32* h, int nq, std::ve
 // buffers (otherwise the host drives them, one proof at a time).
 bool device_round_fits(const whirgpu_ctx* c, const whir::RoundConfig& rp, int m, const whirgpu_prover_data* prev) {
   const int lo = std::min(8, m), hi = m - lo;
+  if (c->params.mmcs != WHIRGPU_MMCS_POSEIDON2) return false;   // the device transcript is the duplex challenger's
   if (!prev || m < 1 || m > RT_MAX_VARS) return false;
   if (rp.ood_samples > RT_MAX_OOD || rp.num_queries < 1 || rp.num_queries > RT_MAX_QUERIES) return false;
   if ((1ull << hi) + (1ull << lo) > c->tab_cap) return false;
@@ -1261,7 +1338,7 @@ RoundLaunch device_round_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int 
   int ri = 0;
   auto rch = [&](int b) { return R[b]->sl->d_rchal + ri; };
   for (int b = 0; b < B; b++) {
-    upload_challenger(R[b]->sl, R[b]->ch);
+    upload_challenger(R[b]->sl, R[b]->ch.dx);
     cin[b] = R[b]->sl->d_chal_init;
     const whirgpu_prover_data* pd = R[b]->cur;
     obs[b] = pd->tree.p + (2 * pd->height - 2) * 8;   // the root
@@ -1430,6 +1507,7 @@ void device_round_replay(whirgpu_ctx* c, Run& r, int round, const RoundLaunch& r
 // observes the final polynomial word by word, so it must be small.  Its
 // openings use the second half of the opening buffer.
 bool device_final_fits(const whirgpu_ctx* c, const whir::WhirConfig& cfg, int m, const whirgpu_prover_data* prev) {
+  if (c->params.mmcs != WHIRGPU_MMCS_POSEIDON2) return false;
   if (!prev || m < 0 || m > Slot::FINAL_MAX_LOG) return false;
   if (cfg.final_queries < 1 || cfg.final_queries > RT_MAX_QUERIES) return false;
   if (cfg.final_sumcheck_rounds > Slot::SC_LOG_CAP / 2) return false;
@@ -1463,7 +1541,7 @@ FinalLaunch device_final_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int 
   const auto fc = cfg.final_round_config();
   if (cin.empty()) {
     for (Run* r : R) {
-      upload_challenger(r->sl, r->ch);
+      upload_challenger(r->sl, r->ch.dx);
       cin.push_back(r->sl->d_chal_init);
     }
   }
@@ -1555,6 +1633,27 @@ void device_final_replay(whirgpu_ctx* c, Run& r, int round, const FinalLaunch& f
 // A WHIR round's steps driven by the host, for one proof (shapes beyond the
 // device transcript's buffers): OOD samples, proof of work, STIR queries,
 // gamma, the weight updates and the round's sumcheck.
+// `rounds` sumcheck rounds of one proof into sd: on the device transcript for
+// the Poseidon2 challenger; round by round from the host (coefficients on the
+// device, observe_and_sample on the host, proof/mod.rs:163-193) for the
+// Keccak challenger.
+void run_sumcheck(whirgpu_ctx* c, Run& r, int rounds, int pow_bits, whir::SumcheckData& sd) {
+  if (!r.ch.keccak) {
+    device_sumcheck(c, {&r}, rounds, pow_bits, [&](Run&) -> whir::SumcheckData& { return sd; });
+    return;
+  }
+  r.fold_rand.clear();
+  for (int i = 0; i < rounds; i++) {
+    r.fold_rand.push_back(sumcheck_round(c, r.sl, r.s, [&](const ef& c0, const ef& c2) {
+      sd.polynomial_evaluations.push_back({c0, c2});
+      r.ch.observe_ef(c0);
+      r.ch.observe_ef(c2);
+      if (pow_bits > 0) sd.pow_witnesses.push_back(grind_all(c, {&r.ch}, pow_bits)[0]);
+      return r.ch.sample_ef();
+    }));
+  }
+}
+
 void host_round(whirgpu_ctx* c, Run& r, int round) {
   const auto& cfg = c->cfg;
   const auto& rp = cfg.round_parameters[round];
@@ -1572,7 +1671,7 @@ void host_round(whirgpu_ctx* c, Run& r, int round) {
     RP.ood_answers.push_back(ev);
     ood_pts.push_back(pt);
   }
-  if (rp.pow_bits > 0) RP.pow_witness = gpu_grind(c, {&r.ch}, rp.pow_bits)[0];
+  if (rp.pow_bits > 0) RP.pow_witness = grind_all(c, {&r.ch}, rp.pow_bits)[0];
   r.ch.sample();
   // STIR queries on the previous commitment (prover/mod.rs:229-303)
   auto idx = whir::get_challenge_stir_queries(rp.domain_size, cfg.ff(round), rp.num_queries, r.ch);
@@ -1617,7 +1716,7 @@ void host_round(whirgpu_ctx* c, Run& r, int round) {
     c->release(Bt);
     for (size_t j = 0; j < ns; j++) r.s.sum = ef_add(r.s.sum, ef_mul(g[j], sel_v[j]));
   }
-  device_sumcheck(c, {&r}, kn, rp.folding_pow_bits, [&](Run&) -> whir::SumcheckData& { return RP.sumcheck; });
+  run_sumcheck(c, r, kn, rp.folding_pow_bits, RP.sumcheck);
 }
 
 // The final round driven by the host, for one proof (final_round, prover/mod.rs:330-417).
@@ -1628,15 +1727,14 @@ void host_final(whirgpu_ctx* c, Run& r, int round) {
   c->d2h(Pf.final_poly.data(), r.s.e.p, Pf.final_poly.size());
   Pf.has_final_poly = true;
   for (const ef& e : Pf.final_poly) r.ch.observe_ef(e);
-  if (cfg.final_pow_bits > 0) Pf.final_pow_witness = gpu_grind(c, {&r.ch}, cfg.final_pow_bits)[0];
+  if (cfg.final_pow_bits > 0) Pf.final_pow_witness = grind_all(c, {&r.ch}, cfg.final_pow_bits)[0];
   const auto fc = cfg.final_round_config();
   const auto idx = whir::get_challenge_stir_queries(fc.domain_size, cfg.ff(round), cfg.final_queries, r.ch);
   std::vector<std::vector<ef>> rows;
   open_rows(c, r.sl, r.prev, idx, &Pf.final_queries, &rows);
   if (cfg.final_sumcheck_rounds > 0) {
     Pf.has_final_sumcheck = true;
-    device_sumcheck(c, {&r}, cfg.final_sumcheck_rounds, cfg.final_folding_pow_bits,
-                    [&](Run&) -> whir::SumcheckData& { return Pf.final_sumcheck; });
+    run_sumcheck(c, r, cfg.final_sumcheck_rounds, cfg.final_folding_pow_bits, Pf.final_sumcheck);
   }
 }
 
@@ -1768,8 +1866,10 @@ void use_device(const whirgpu_ctx* c) {
   HIPCHK(hipSetDevice(c->device));
 }
 
-DuplexChallenger load_ch(const whirgpu_ctx* c, const whirgpu_challenger* ch) {
-  DuplexChallenger d;
+Challenger load_ch(const whirgpu_ctx* c, const whirgpu_challenger* ch) {
+  if (c->params.mmcs != WHIRGPU_MMCS_POSEIDON2) bad("this context's MMCS is Keccak: use the Keccak challenger");
+  Challenger out;
+  DuplexChallenger& d = out.dx;
   d.perm = &c->c16;
   memcpy(d.sponge, ch->sponge, sizeof d.sponge);
   memcpy(d.in, ch->input, sizeof d.in);
@@ -1777,10 +1877,33 @@ DuplexChallenger load_ch(const whirgpu_ctx* c, const whirgpu_challenger* ch) {
   d.in_len = (int)ch->input_len;
   d.out_len = (int)ch->output_len;
   if (d.in_len < 0 || d.in_len > 7 || d.out_len < 0 || d.out_len > 8) bad("invalid challenger state");
-  return d;
+  return out;
 }
 
-void store_ch(const DuplexChallenger& d, whirgpu_challenger* ch) {
+// The Keccak challenger's state across the C ABI (its input bytes and the
+// output bytes not yet popped).
+Challenger load_kch(const whirgpu_ctx* c, const whirgpu_keccak_challenger* k) {
+  if (c && c->params.mmcs != WHIRGPU_MMCS_KECCAK) bad("this context's MMCS is Poseidon2: use the duplex challenger");
+  if (k->input_len > WHIRGPU_KECCAK_INPUT_CAP || k->output_len > 32) bad("invalid Keccak challenger state");
+  Challenger ch;
+  ch.keccak = true;
+  ch.kc.in.assign(k->input, k->input + k->input_len);
+  memcpy(ch.kc.out, k->output, sizeof ch.kc.out);
+  ch.kc.out_len = (int)k->output_len;
+  return ch;
+}
+
+void store_kch(const Challenger& ch, whirgpu_keccak_challenger* k) {
+  if (ch.kc.in.size() > WHIRGPU_KECCAK_INPUT_CAP)
+    bad("the Keccak challenger's input exceeds WHIRGPU_KECCAK_INPUT_CAP bytes", WHIRGPU_BAD_STATE);
+  memcpy(k->input, ch.kc.in.data(), ch.kc.in.size());
+  k->input_len = (u32)ch.kc.in.size();
+  memcpy(k->output, ch.kc.out, sizeof k->output);
+  k->output_len = (u32)ch.kc.out_len;
+}
+
+void store_ch(const Challenger& c, whirgpu_challenger* ch) {
+  const DuplexChallenger& d = c.dx;
   memcpy(ch->sponge, d.sponge, sizeof d.sponge);
   memcpy(ch->input, d.in, sizeof d.in);
   memcpy(ch->output, d.out, sizeof d.out);
@@ -1967,6 +2090,8 @@ int whirgpu_create(int device, const whirgpu_params* params, const whirgpu_posei
     p.security_level = (int)params->security_level;
     p.pow_bits = (int)params->pow_bits;
     if (params->leaf_hash_width != 16 && params->leaf_hash_width != 24) bad("leaf_hash_width must be 16 or 24");
+    if (params->mmcs != WHIRGPU_MMCS_POSEIDON2 && params->mmcs != WHIRGPU_MMCS_KECCAK)
+      bad("mmcs must be WHIRGPU_MMCS_POSEIDON2 or WHIRGPU_MMCS_KECCAK");
     if (params->num_variables < 1 || params->num_variables > 28) bad("num_variables must be in [1, 28]");
     std::string err;
     if (!whir::WhirConfig::create((int)params->num_variables, p, &c->cfg, &err)) bad(err);
@@ -2068,7 +2193,7 @@ int whirgpu_challenger_new(whirgpu_challenger* ch) {
 int whirgpu_challenger_observe(const whirgpu_ctx* c, whirgpu_challenger* ch, const uint32_t* values, size_t n) {
   return guarded([&] {
     if (!c || !ch || (!values && n)) bad("null argument");
-    DuplexChallenger d = load_ch(c, ch);
+    Challenger d = load_ch(c, ch);
     d.observe_slice(values, n);
     store_ch(d, ch);
   });
@@ -2077,7 +2202,7 @@ int whirgpu_challenger_observe(const whirgpu_ctx* c, whirgpu_challenger* ch, con
 int whirgpu_challenger_observe_domain_separator(const whirgpu_ctx* c, whirgpu_challenger* ch) {
   return guarded([&] {
     if (!c || !ch) bad("null argument");
-    DuplexChallenger d = load_ch(c, ch);
+    Challenger d = load_ch(c, ch);
     auto v = whir::domain_separator_pattern(c->cfg, (int)c->params.digest_elems);
     d.observe_slice(v.data(), v.size());
     store_ch(d, ch);
@@ -2087,7 +2212,7 @@ int whirgpu_challenger_observe_domain_separator(const whirgpu_ctx* c, whirgpu_ch
 int whirgpu_challenger_sample(const whirgpu_ctx* c, whirgpu_challenger* ch, uint32_t* out, size_t n) {
   return guarded([&] {
     if (!c || !ch || (!out && n)) bad("null argument");
-    DuplexChallenger d = load_ch(c, ch);
+    Challenger d = load_ch(c, ch);
     for (size_t i = 0; i < n; i++) out[i] = d.sample();
     store_ch(d, ch);
   });
@@ -2248,17 +2373,21 @@ int whirgpu_proof_deserialize(whirgpu_ctx* c, const uint8_t* bytes, size_t len, 
     auto p = std::make_unique<whirgpu_proof>();
     p->ctx = c;
     std::string err;
-    if (!whir::deserialize(bytes, len, &p->p, &err)) bad(err, WHIRGPU_MALFORMED_PROOF);
+    if (!whir::deserialize(bytes, len, &p->p, &err, c->params.mmcs == WHIRGPU_MMCS_POSEIDON2))
+      bad(err, WHIRGPU_MALFORMED_PROOF);
     *out = p.release();
   });
 }
 
 // CommitmentReader::parse + Verifier::verify (committer/reader.rs:40-82,
 // verifier/mod.rs:58-200) on the host, for the statement {points -> evals}.
-int whirgpu_verify(const whirgpu_ctx* c, const whirgpu_proof* proof, whirgpu_challenger* ch, uint32_t npoints,
-                   const uint32_t* points, const uint32_t* evals) {
+}  // extern "C"
+
+template <class Load, class Store>
+int verify_one(const whirgpu_ctx* c, const whirgpu_proof* proof, uint32_t npoints, const uint32_t* points,
+               const uint32_t* evals, Load&& load, Store&& store) {
   return guarded([&] {
-    if (!c || !proof || !ch || (npoints && (!points || !evals))) bad("null argument");
+    if (!c || !proof || (npoints && (!points || !evals))) bad("null argument");
     const int n = c->cfg.num_variables;
     std::vector<std::vector<ef>> pts(npoints, std::vector<ef>(n));
     std::vector<ef> evs(npoints);
@@ -2266,12 +2395,112 @@ int whirgpu_verify(const whirgpu_ctx* c, const whirgpu_proof* proof, whirgpu_cha
       memcpy(pts[k].data(), points + (size_t)4 * n * k, sizeof(ef) * n);
       memcpy(&evs[k], evals + 4 * k, sizeof(ef));
     }
-    DuplexChallenger d = load_ch(c, ch);
-    whir::VerifierKey key{&c->cfg, &c->c16, &c->c24, (int)c->params.leaf_hash_width};
+    Challenger d = load();
+    whir::VerifierKey key{&c->cfg, &c->c16, &c->c24, (int)c->params.leaf_hash_width,
+                          c->params.mmcs == WHIRGPU_MMCS_KECCAK};
     std::string why;
     if (whir::verify(key, proof->p, d, pts, evs, &why) != whir::VerifyFailure::None) bad(why, WHIRGPU_VERIFY_FAILED);
-    store_ch(d, ch);
+    store(d);
   });
+}
+
+extern "C" {
+
+int whirgpu_verify(const whirgpu_ctx* c, const whirgpu_proof* proof, whirgpu_challenger* ch, uint32_t npoints,
+                   const uint32_t* points, const uint32_t* evals) {
+  return verify_one(
+      c, proof, npoints, points, evals,
+      [&] {
+        if (!ch) bad("null argument");
+        return load_ch(c, ch);
+      },
+      [&](const Challenger& d) { store_ch(d, ch); });
+}
+
+int whirgpu_verify_keccak(const whirgpu_ctx* c, const whirgpu_proof* proof, whirgpu_keccak_challenger* kch,
+                          uint32_t npoints, const uint32_t* points, const uint32_t* evals) {
+  return verify_one(
+      c, proof, npoints, points, evals,
+      [&] {
+        if (!kch) bad("null argument");
+        return load_kch(c, kch);
+      },
+      [&](const Challenger& d) { store_kch(d, kch); });
+}
+
+// The Keccak challenger (SerializingChallenger32<F, HashChallenger<u8,
+// Keccak256Hash, 32>>) across the C ABI.
+int whirgpu_keccak_challenger_new(whirgpu_keccak_challenger* k) {
+  if (!k) return WHIRGPU_BAD_PARAMS;
+  memset(k, 0, sizeof *k);
+  return WHIRGPU_OK;
+}
+
+int whirgpu_keccak_challenger_observe(const whirgpu_ctx* c, whirgpu_keccak_challenger* k, const uint32_t* values,
+                                      size_t n) {
+  return guarded([&] {
+    if (!k || (n && !values)) bad("null argument");
+    Challenger d = load_kch(c, k);
+    d.observe_slice(values, n);
+    store_kch(d, k);
+  });
+}
+
+int whirgpu_keccak_challenger_sample(const whirgpu_ctx* c, whirgpu_keccak_challenger* k, uint32_t* out, size_t n) {
+  return guarded([&] {
+    if (!k || (n && !out)) bad("null argument");
+    Challenger d = load_kch(c, k);
+    for (size_t i = 0; i < n; i++) out[i] = d.sample();
+    store_kch(d, k);
+  });
+}
+
+int whirgpu_keccak_challenger_observe_bytes(const whirgpu_ctx* c, whirgpu_keccak_challenger* k, const uint8_t* bytes,
+                                            size_t n) {
+  return guarded([&] {
+    if (!k || (n && !bytes)) bad("null argument");
+    Challenger d = load_kch(c, k);
+    for (size_t i = 0; i < n; i++) d.kc.observe_byte(bytes[i]);
+    store_kch(d, k);
+  });
+}
+
+int whirgpu_keccak_challenger_sample_bits(const whirgpu_ctx* c, whirgpu_keccak_challenger* k, uint32_t bits,
+                                          uint32_t* out) {
+  return guarded([&] {
+    if (!k || !out) bad("null argument");
+    if (bits > 31) bad("bits must be at most 31");
+    Challenger d = load_kch(c, k);
+    *out = (uint32_t)d.sample_bits((int)bits);
+    store_kch(d, k);
+  });
+}
+
+int whirgpu_keccak_challenger_observe_domain_separator(const whirgpu_ctx* c, whirgpu_keccak_challenger* k) {
+  return guarded([&] {
+    if (!c || !k) bad("null argument");
+    Challenger d = load_kch(c, k);
+    auto v = whir::domain_separator_pattern(c->cfg, (int)c->params.digest_elems);
+    d.observe_slice(v.data(), v.size());
+    store_kch(d, k);
+  });
+}
+
+int whirgpu_keccak_grind(whirgpu_ctx* c, whirgpu_keccak_challenger* k, uint32_t bits, uint32_t* witness_out) {
+  return guarded([&] {
+    if (!c || !k || !witness_out) bad("null argument");
+    if (bits == 0 || bits > 30) bad("bits must be in [1, 30]");
+    use_device(c);
+    Challenger ch = load_kch(c, k);
+    *witness_out = keccak_gpu_grind(c, ch, (int)bits);
+    store_kch(ch, k);
+  });
+}
+
+int whirgpu_keccak_f1600(uint64_t* state) {
+  if (!state) return WHIRGPU_BAD_PARAMS;
+  keccak_f1600(state);
+  return WHIRGPU_OK;
 }
 
 int whirgpu_prover_data_destroy(whirgpu_prover_data* pd) {
@@ -2282,22 +2511,26 @@ int whirgpu_prover_data_destroy(whirgpu_prover_data* pd) {
   return WHIRGPU_OK;
 }
 
-// CommitmentWriter::commit (src/whir/committer/writer.rs:56-109): a group of one.
-int whirgpu_commit(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs, whirgpu_statement* st,
-                   whirgpu_prover_data** out) {
+}  // extern "C"
+
+// CommitmentWriter::commit (src/whir/committer/writer.rs:56-109): a group of
+// one, from the challenger `load` gives; `store` writes it back on success.
+template <class Load, class Store>
+int commit_one(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_statement* st, whirgpu_prover_data** out, Load&& load,
+               Store&& store) {
   std::vector<Run> runs(1);
   int rc = guarded([&] {
-    if (!c || !proof || !chs || !st || !out) bad("null argument");
+    if (!c || !proof || !st || !out) bad("null argument");
     if (st->ctx != c) bad("statement of another context");
     use_device(c);
     const double t0 = now_ms();
     Run& r = runs[0];
     r.st = st;
     r.P = &proof->p;
-    r.ch = load_ch(c, chs);
+    r.ch = load();
     r.sl = c->slots[0].get();
     commit_group(c, {&r});
-    store_ch(r.ch, chs);
+    store(r.ch);
     c->timings[0] = now_ms() - t0;
   });
   if (rc) {
@@ -2310,21 +2543,22 @@ int whirgpu_commit(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs
 }
 
 // Prover::prove (src/whir/prover/mod.rs:97-417): a group of one; consumes the prover data.
-int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs, const whirgpu_statement* st,
-                  whirgpu_prover_data* pd0) {
+template <class Load, class Store>
+int prove_one(whirgpu_ctx* c, whirgpu_proof* proof, const whirgpu_statement* st, whirgpu_prover_data* pd0,
+              Load&& load, Store&& store) {
   std::vector<Run> runs(1);
   runs[0].prev = pd0;
   int rc = guarded([&] {
-    if (!c || !proof || !chs || !st || !pd0) bad("null argument");
+    if (!c || !proof || !st || !pd0) bad("null argument");
     use_device(c);
     const double t0 = now_ms();
     Run& r = runs[0];
     r.st = const_cast<whirgpu_statement*>(st);
     r.P = &proof->p;
-    r.ch = load_ch(c, chs);
+    r.ch = load();
     r.sl = pd0->sl ? pd0->sl : c->slots[0].get();
     prove_group(c, {&r});
-    store_ch(r.ch, chs);
+    store(r.ch);
     c->timings[6] = now_ms() - t0;
   });
   if (c) {
@@ -2334,11 +2568,60 @@ int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs,
   return rc;
 }
 
+extern "C" {
+
+int whirgpu_commit(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs, whirgpu_statement* st,
+                   whirgpu_prover_data** out) {
+  return commit_one(
+      c, proof, st, out,
+      [&] {
+        if (!chs) bad("null argument");
+        return load_ch(c, chs);
+      },
+      [&](const Challenger& ch) { store_ch(ch, chs); });
+}
+
+int whirgpu_prove(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_challenger* chs, const whirgpu_statement* st,
+                  whirgpu_prover_data* pd0) {
+  return prove_one(
+      c, proof, st, pd0,
+      [&] {
+        if (!chs) bad("null argument");
+        return load_ch(c, chs);
+      },
+      [&](const Challenger& ch) { store_ch(ch, chs); });
+}
+
+// The Keccak variant (src/whir/mod.rs:236-383) of commit and prove.
+int whirgpu_commit_keccak(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_keccak_challenger* kch, whirgpu_statement* st,
+                          whirgpu_prover_data** out) {
+  return commit_one(
+      c, proof, st, out,
+      [&] {
+        if (!kch) bad("null argument");
+        return load_kch(c, kch);
+      },
+      [&](const Challenger& ch) { store_kch(ch, kch); });
+}
+
+int whirgpu_prove_keccak(whirgpu_ctx* c, whirgpu_proof* proof, whirgpu_keccak_challenger* kch,
+                         const whirgpu_statement* st, whirgpu_prover_data* pd0) {
+  return prove_one(
+      c, proof, st, pd0,
+      [&] {
+        if (!kch) bad("null argument");
+        return load_kch(c, kch);
+      },
+      [&](const Challenger& ch) { store_kch(ch, kch); });
+}
+
 int whirgpu_prove_group(whirgpu_ctx* c, uint32_t count, whirgpu_statement* const* statements,
                         whirgpu_proof* const* proofs, whirgpu_challenger* challengers) {
   std::vector<Run> runs;
   int rc = guarded([&] {
     if (!c || !statements || !proofs || !challengers || count < 1 || count > GROUP_MAX) bad("bad argument");
+    if (c->params.mmcs != WHIRGPU_MMCS_POSEIDON2)
+      bad("group proving runs the duplex challenger's transcript on the device; this context's MMCS is Keccak");
     use_device(c);
     ensure_slots(c, (int)count);
     const double t0 = now_ms();
@@ -2505,8 +2788,11 @@ int whirgpu_merkle_commit(whirgpu_ctx* c, const uint32_t* d_rows, uint64_t heigh
     if (height == 0 || (height & (height - 1))) bad("height must be a power of two");
     use_device(c);
     const TreeArgs a{d_rows, d_tree, nullptr};
-    KCHK(merkle_leaves(&a, 1, (int)row_len, height, (int)c->params.leaf_hash_width, c->d_c16, c->d_c24, c->st));
-    const int rc = merkle_tree(&a, 1, height, c->c16, c->tune, c->st);
+    const bool keccak = c->params.mmcs == WHIRGPU_MMCS_KECCAK;
+    KCHK(keccak ? keccak_leaves(&a, 1, (int)row_len, height, c->st)
+                : merkle_leaves(&a, 1, (int)row_len, height, (int)c->params.leaf_hash_width, c->d_c16, c->d_c24,
+                                c->st));
+    const int rc = keccak ? keccak_tree(&a, 1, height, c->st) : merkle_tree(&a, 1, height, c->c16, c->tune, c->st);
     if (rc < 0) KCHK(rc);
     c->d2h(root_out, d_tree + (2 * height - 2) * 8, 8);
   });
@@ -2517,8 +2803,8 @@ int whirgpu_grind(whirgpu_ctx* c, whirgpu_challenger* chs, uint32_t bits, uint32
     if (!c || !chs || !witness_out) bad("null argument");
     if (bits == 0 || bits > 30) bad("bits must be in [1, 30]");
     use_device(c);
-    DuplexChallenger ch = load_ch(c, chs);
-    *witness_out = gpu_grind(c, {&ch}, (int)bits)[0];
+    Challenger ch = load_ch(c, chs);
+    *witness_out = gpu_grind(c, {&ch.dx}, (int)bits)[0];
     store_ch(ch, chs);
   });
 }
@@ -2657,7 +2943,7 @@ int whirgpu_sumcheck_eq(whirgpu_ctx* c, const uint32_t* d_f, uint32_t n, const u
     Run& r = runs[0];
     r.st = &st;
     r.sl = sl;
-    DuplexChallenger ch;
+    Challenger ch;
     if (chs) ch = load_ch(c, chs);
     size_t round = 0;
     ChallengeFn next = [&](const ef& c0, const ef& c2) {

@@ -404,6 +404,10 @@ extern "C" {
 int whirgpu_server_create(int device, const whirgpu_params* params, const whirgpu_poseidon2* perms, int workers,
                           size_t hbm_budget, whirgpu_server** out) {
   if (!params || !perms || !out || workers < 1 || workers > 256) return WHIRGPU_BAD_PARAMS;
+  // jobs are proved in groups, whose transcripts run on the device: the
+  // duplex challenger's, so the Keccak MMCS is proved one call at a time
+  // (whirgpu_commit_keccak / whirgpu_prove_keccak) and not served here
+  if (params->mmcs != WHIRGPU_MMCS_POSEIDON2) return WHIRGPU_BAD_PARAMS;
   size_t free_b = 0, total_b = 0;
   int rc = whirgpu_internal_mem_info(device, &free_b, &total_b);
   if (rc) return rc;

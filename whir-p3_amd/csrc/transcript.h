@@ -88,6 +88,24 @@ struct BlockChallenger {
     __syncthreads();
     if (in_len == 8) duplex();
   }
+  // Observes n words from memory (n <= OBS_STAGE): every word is loaded at
+  // once into `stage` (LDS), then the input buffer is filled up to 8 words
+  // per barrier, duplexing whenever it is full, as n observe() calls would.
+  // One load latency instead of n serialized ones.
+  __device__ void observe_words(const u32* w, int n, u32* stage) {
+    if (n <= 0) return;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) stage[i] = w[i];
+    __syncthreads();
+    out_len = 0;
+    for (int i = 0; i < n;) {
+      const int k = n - i < 8 - in_len ? n - i : 8 - in_len;
+      if ((int)threadIdx.x < k) in[in_len + threadIdx.x] = stage[i + threadIdx.x];
+      __syncthreads();
+      in_len += k;
+      i += k;
+      if (in_len == 8) duplex();
+    }
+  }
   __device__ u32 sample() {
     if (in_len > 0 || out_len == 0) duplex();
     out_len--;
@@ -125,7 +143,8 @@ __device__ __forceinline__ void tr_prologue(int mode, const ScTr& a, const P2C16
   const bool lead = blockIdx.x == 0 && blockIdx.y == 0;
   u32 err = 0;
   if (mode == TR_OBS_GRIND) {
-    for (int k = 0; k < a.n_obs; k++) ch.observe(a.obs[k]);
+    __shared__ u32 stage[OBS_STAGE];
+    ch.observe_words(a.obs, a.n_obs, stage);
     if (lead && t == 0 && a.n_obs == 4 && a.log_obs) *a.log_obs = ef{{a.obs[0], a.obs[1], a.obs[2], a.obs[3]}};
     if (t < 16) gb->s[t] = t < ch.in_len ? in[t] : sp[t];
     if (t == 0) gb->wpos = (u32)ch.in_len;

@@ -31,7 +31,7 @@ namespace {
 // later read, so the decoder's code reads straight through.
 class Cursor {
  public:
-  Cursor(const uint8_t* b, size_t n) : p_(b), end_(b + n) {}
+  Cursor(const uint8_t* b, size_t n, bool digests_are_field) : p_(b), end_(b + n), digest_field_(digests_are_field) {}
   bool failed() const { return !why_.empty(); }
   const std::string& why() const { return why_; }
   bool at_end() const { return p_ == end_; }
@@ -81,9 +81,14 @@ class Cursor {
     fields(e.c, 4, what);
     return e;
   }
+  // A digest: 8 field elements (Poseidon2), or 4 u64 lanes (Keccak) as 8 words.
   Digest digest(const char* what) {
     Digest d{};
-    fields(d.data(), d.size(), what);
+    if (digest_field_) {
+      fields(d.data(), d.size(), what);
+    } else {
+      for (u32& w : d) raw(&w, 4, what);
+    }
     return d;
   }
   std::string text(const char* what) {
@@ -97,6 +102,7 @@ class Cursor {
   size_t left() const { return (size_t)(end_ - p_); }
   const uint8_t* p_;
   const uint8_t* end_;
+  bool digest_field_;
   std::string why_;
 };
 
@@ -151,6 +157,21 @@ Digest sponge(const u32* row, size_t len, const kb::P2Consts<W>& consts) {
   }
   Digest d;
   std::copy(s, s + 8, d.begin());
+  return d;
+}
+
+// The Keccak lane sponge over a run of u32 words read as little-endian u64
+// lanes (an odd last word alone), as 8 digest words.
+Digest keccak_words(const u32* w, size_t n) {
+  std::vector<uint64_t> lanes((n + 1) / 2);
+  for (size_t i = 0; i < n; i++) lanes[i / 2] |= (uint64_t)w[i] << (32 * (i % 2));
+  uint64_t d4[4];
+  kb::keccak_lane_sponge(lanes.data(), lanes.size(), d4);
+  Digest d;
+  for (int j = 0; j < 4; j++) {
+    d[2 * j] = (u32)d4[j];
+    d[2 * j + 1] = (u32)(d4[j] >> 32);
+  }
   return d;
 }
 
@@ -226,7 +247,7 @@ ef horner(const std::vector<ef>& coeffs, u32 var) {
 
 class Check {
  public:
-  Check(const VerifierKey& k, DuplexChallenger& ch, std::string* why) : k_(k), cfg_(*k.cfg), ch_(ch), why_(why) {}
+  Check(const VerifierKey& k, Challenger& ch, std::string* why) : k_(k), cfg_(*k.cfg), ch_(ch), why_(why) {}
 
   VerifyFailure run(const WhirProof& P, const std::vector<std::vector<ef>>& points, const std::vector<ef>& evals);
 
@@ -238,7 +259,7 @@ class Check {
     *why_ = why;
     throw Failed{f};
   }
-  void observe_root(const Digest& d) { ch_.observe_slice(d.data(), d.size()); }
+  void observe_root(const Digest& d) { ch_.observe_digest(d); }
   void shape(const WhirProof& P, const std::vector<std::vector<ef>>& points, const std::vector<ef>& evals);
   std::vector<ef> sumcheck(const SumcheckData& s, int rounds, int pow_bits, const char* where);
   void pow(int bits, u32 witness, const char* where);
@@ -248,7 +269,7 @@ class Check {
 
   const VerifierKey& k_;
   const WhirConfig& cfg_;
-  DuplexChallenger& ch_;
+  Challenger& ch_;
   std::string* why_;
   ef claimed_ = kb::ef_zero();
 };
@@ -313,7 +334,7 @@ void Check::stir(u64 domain_size, int ff, int num_queries, u32 folded_gen, const
     if ((o.extension ? o.values.size() / 4 : o.values.size()) != width)
       fail(VerifyFailure::Shape, std::string("an opened row has the wrong width in ") + where);
     const Digest leaf = leaf_digest(k_, o.values.data(), o.values.size());
-    if (!path_reaches_root(*k_.c16, leaf, idx[q], height, o.proof, root))
+    if (!path_reaches_root(k_, leaf, idx[q], height, o.proof, root))
       fail(VerifyFailure::MerklePath, std::string("Merkle opening at position ") + std::to_string(idx[q]) + " in " +
                                           where + " does not reach the root");
     for (size_t i = 0; i < width; i++) {
@@ -409,8 +430,8 @@ VerifyFailure Check::run(const WhirProof& P, const std::vector<std::vector<ef>>&
 
 }  // namespace
 
-bool deserialize(const uint8_t* bytes, size_t len, WhirProof* out, std::string* err) {
-  Cursor c(bytes, len);
+bool deserialize(const uint8_t* bytes, size_t len, WhirProof* out, std::string* err, bool digests_are_field) {
+  Cursor c(bytes, len, digests_are_field);
   WhirProof p;
   p.has_initial_commitment = c.some("initial commitment");
   if (p.has_initial_commitment) p.initial_commitment = c.digest("initial commitment");
@@ -443,23 +464,33 @@ bool deserialize(const uint8_t* bytes, size_t len, WhirProof* out, std::string* 
 }
 
 Digest leaf_digest(const VerifierKey& key, const u32* row, size_t len) {
+  if (key.keccak) return keccak_words(row, len);
   return key.leaf_width == 24 ? sponge<24>(row, len, *key.c24) : sponge<16>(row, len, *key.c16);
 }
 
-bool path_reaches_root(const kb::P2C16& c16, Digest node, u64 index, u64 height, const std::vector<Digest>& siblings,
+bool path_reaches_root(const VerifierKey& key, Digest node, u64 index, u64 height, const std::vector<Digest>& siblings,
                        const Digest& root) {
   if (index >= height) return false;
   size_t depth = 0;
   while ((1ull << depth) < height) depth++;
   if (siblings.size() != depth) return false;
   for (const Digest& s : siblings) {
-    node = (index & 1) ? compress(s, node, c16) : compress(node, s, c16);
+    const Digest& l = (index & 1) ? s : node;
+    const Digest& r = (index & 1) ? node : s;
+    if (key.keccak) {
+      u32 both[16];
+      std::copy(l.begin(), l.end(), both);
+      std::copy(r.begin(), r.end(), both + 8);
+      node = keccak_words(both, 16);
+    } else {
+      node = compress(l, r, *key.c16);
+    }
     index >>= 1;
   }
   return node == root;
 }
 
-VerifyFailure verify(const VerifierKey& key, const WhirProof& proof, DuplexChallenger& ch,
+VerifyFailure verify(const VerifierKey& key, const WhirProof& proof, Challenger& ch,
                      const std::vector<std::vector<ef>>& points, const std::vector<ef>& evals, std::string* why) {
   return Check(key, ch, why).run(proof, points, evals);
 }

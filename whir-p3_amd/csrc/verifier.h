@@ -14,7 +14,8 @@ namespace whir {
 // bytes serialize() writes.  Rejects truncated or trailing bytes, unknown
 // Option tags or opening types, lengths beyond the input and field words not
 // below p.  Returns false with the reason in *err.
-bool deserialize(const uint8_t* bytes, size_t len, WhirProof* out, std::string* err);
+// With digests_are_field false (the Keccak MMCS) digest words are any u32.
+bool deserialize(const uint8_t* bytes, size_t len, WhirProof* out, std::string* err, bool digests_are_field = true);
 
 // What the verifier checks a proof against.
 struct VerifierKey {
@@ -22,6 +23,7 @@ struct VerifierKey {
   const kb::P2C16* c16 = nullptr;   // challenger, tree compression, width-16 leaves
   const kb::P2C24* c24 = nullptr;   // width-24 leaves
   int leaf_width = 16;
+  bool keccak = false;   // the Keccak MMCS (digests are u64 lanes, not field elements)
 };
 
 // Which check failed (verifier/errors.rs VerifierError, plus the shape checks
@@ -40,14 +42,15 @@ enum class VerifyFailure {
 // {points[i] -> evals[i]} (the order of InitialStatement::normalize).  `ch` is
 // the verifier's challenger after the domain separator; on success it ends
 // where the prover's ended.
-VerifyFailure verify(const VerifierKey& key, const WhirProof& proof, DuplexChallenger& ch,
+VerifyFailure verify(const VerifierKey& key, const WhirProof& proof, Challenger& ch,
                      const std::vector<std::vector<ef>>& points, const std::vector<ef>& evals, std::string* why);
 
-// MerkleTreeMmcs over Poseidon2 on the host: the digest of one row
-// (PaddingFreeSponge of width leaf_width, rate leaf_width - 8) and the
-// check of an opening path from the leaf level up.
+// MerkleTreeMmcs on the host: the digest of one row (Poseidon2:
+// PaddingFreeSponge of width leaf_width, rate leaf_width - 8; Keccak: the
+// lane sponge over the row read as little-endian u64) and the check of an
+// opening path from the leaf level up.
 Digest leaf_digest(const VerifierKey& key, const u32* row, size_t len);
-bool path_reaches_root(const kb::P2C16& c16, Digest node, u64 index, u64 height, const std::vector<Digest>& siblings,
+bool path_reaches_root(const VerifierKey& key, Digest node, u64 index, u64 height, const std::vector<Digest>& siblings,
                        const Digest& root);
 
 }  // namespace whir
