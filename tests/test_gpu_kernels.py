@@ -63,7 +63,7 @@ def oracle_lde_ext(f, m, k, R):
 
 
 @pytest.mark.parametrize("m,k,R", [(2, 2, 1), (6, 2, 1), (10, 4, 1), (14, 4, 1), (16, 4, 1), (18, 4, 1),
-                                   (13, 1, 3), (12, 6, 2), (20, 4, 1)])
+                                   (13, 1, 3), (12, 6, 2), (20, 4, 1), (22, 4, 1), (24, 4, 1)])
 def test_lde_base(W, m, k, R):
     cfg = ctx(W)
     rng = np.random.default_rng(m * 100 + k * 10 + R)
@@ -74,7 +74,8 @@ def test_lde_base(W, m, k, R):
     assert np.array_equal(host(d_out), oracle_lde_base(f, m, k, R))
 
 
-@pytest.mark.parametrize("m,k,R", [(4, 4, 11), (8, 4, 2), (12, 4, 5), (16, 4, 2), (14, 4, 3), (9, 1, 2), (6, 6, 1)])
+@pytest.mark.parametrize("m,k,R", [(4, 4, 11), (8, 4, 2), (12, 4, 5), (16, 4, 2), (14, 4, 3), (9, 1, 2), (6, 6, 1),
+                                   (16, 4, 5), (18, 4, 2), (20, 4, 2)])
 def test_lde_ext(W, m, k, R):
     cfg = ctx(W)
     rng = np.random.default_rng(m * 100 + k * 10 + R + 1)

@@ -72,6 +72,71 @@ __host__ __device__ __forceinline__ u32 mul(u32 a, u32 b) {
   return s < r ? s : r;
 }
 
+// ---- lazy reduction
+// Sums kept in 64 bits and reduced once.  Because 2^31 = 2^24 - 1 (mod p), a
+// value y < 2^37.9 splits as y = hi 2^31 + lo with hi < 126 and
+//   y = lo + hi (2^24 - 1) (mod p),  which is < 2p,
+// so lazy_reduce (two instructions) gives a representative below 2p and
+// full_reduce adds one conditional subtraction.
+constexpr u32 TWO_P = 2u * P;
+
+// 32x32+64 multiply-adds (v_mad_u64_u32 / v_mad_i64_i32, one full-rate
+// instruction each on CDNA).
+__host__ __device__ __forceinline__ u64 mad_u64(u32 a, u32 b, u64 c) { return (u64)a * b + c; }
+__host__ __device__ __forceinline__ u64 mad_i64(int32_t a, int32_t b, u64 c) {
+  return (u64)((int64_t)a * (int64_t)b + (int64_t)c);
+}
+
+// y - (y >> 31) p = (y mod 2^31) + (y >> 31)(2^24 - 1): for y < 2^37.9 the
+// quotient is below 126 and the result below 2p.  Kept 64-bit (its high word
+// is zero) so it feeds 64-bit adds directly.
+// y >> 31 as a 32-bit value.  On the device it comes from v_alignbit_b32,
+// which keeps the compiler from widening the multiply-add below into a 64x64
+// product.
+__host__ __device__ __forceinline__ u32 quot31(u64 y) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_alignbit((u32)(y >> 32), (u32)y, 31);
+#else
+  return (u32)(y >> 31);
+#endif
+}
+// Hides a value's origin from the compiler (no instruction), so it keeps the
+// multiply-add forms below instead of rewriting them into longer sequences
+// (24-bit products, 64-bit subtractions, selects).  A 32-bit value added to a
+// 64-bit one is written mad_u64(x, one, y) with one = opaque(1u): a single
+// v_mad_u64_u32, where a plain add would first widen x with a zeroed high
+// word.
+template <typename T>
+__host__ __device__ __forceinline__ T opaque(T x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm("" : "+v"(x));
+#endif
+  return x;
+}
+
+__host__ __device__ __forceinline__ u64 lazy_reduce(u64 y) {
+  return mad_i64((int32_t)quot31(y), -(int32_t)P, y);
+}
+__host__ __device__ __forceinline__ u32 full_reduce(u64 y) {   // y < 2^37.9 -> [0, p)
+  const u32 z = (u32)lazy_reduce(y);
+  const u32 t = z - P;
+  return t < z ? t : z;
+}
+
+// Montgomery product without the final subtraction, for b w < 2^32 p (for
+// instance b < 2^32 and w < p, or b < p and w < 2p): (b w + t p) / 2^32 is
+// then below 2p, a representative that fits 32 bits.
+__host__ __device__ __forceinline__ u32 mul_lazy(u32 b, u32 w) {
+  const u64 x = (u64)b * w;
+  const u32 t = (u32)x * MU_NEG;
+  return (u32)(((u64)t * P + x) >> 32);
+}
+// [0, 2p) -> [0, p)
+__host__ __device__ __forceinline__ u32 canon(u32 x) {
+  const u32 t = x - P;
+  return t < x ? t : x;
+}
+
 __host__ __device__ __forceinline__ u32 to_monty(u32 x) { return mul(x, R2); }
 __host__ __device__ __forceinline__ u32 from_monty(u32 m) { return redc((u64)m); }
 

@@ -10,7 +10,7 @@
 // length L of the twisted column c_a[b] * w_H^(i0 b), so the padding zeros are
 // never read or multiplied.
 //
-// A length-L DFT with L <= 2^LP runs in one pass (k_lde_pass2 in direct mode);
+// A length-L DFT with L <= 2^LP runs in one pass (k_lde_pass2, DIRECT);
 // longer ones use the four-step split L = L1 * L2:
 //   pass 1: per (column, coset, b2): DFT over b1 of c[L2 b1 + b2] (twisted),
 //           times w_L^(b2 k1), into V[coset][column][k1][b2];
@@ -18,8 +18,9 @@
 //           once, written to whole output rows 2^R (k1 + L1 k2) + i0.
 // Each block keeps its lines in LDS in bit-reversed order and runs radix-2
 // decimation-in-time stages there.
+#include <algorithm>
+#include <map>
 #include <mutex>
-#include <set>
 #include <tuple>
 
 #include "kcommon.h"
@@ -35,6 +36,16 @@ namespace kb {
 // stages there and stores them back, so a line of 1024 costs two LDS round
 // trips and two barriers instead of ten.
 //
+// Values between the first load and pass 2's output are arbitrary 32-bit
+// representatives (any u32 stands for its residue mod p): a butterfly with
+// twiddle w maps (a, b) to (a + t, a - t) with t = mul_lazy(b, w) < 2p, and a
+// carry (borrow) out of 32 bits is worth +2^32 (-2^32) = +-R1 (mod p), which
+// wadd / wsub fold back without overflow because t < 2p and 2p + R1 = 2^32.
+// A butterfly is nine instructions (three for the product, three each for the
+// sum and the difference) against eleven with canonical values, in 32-bit
+// registers; nothing is reduced between stages or passes, and only pass 2's
+// outputs are made canonical.
+//
 // Layout: element i of line l sits at l * LS + i + (i >> 5).  The pad after
 // every 32 elements keeps a wave's accesses on distinct banks both when lanes
 // walk groups of one line (stride 33) and when they walk lines at the same
@@ -46,18 +57,69 @@ template <int LG>
 struct LineShape {
   static constexpr int N = 1 << LG;
   static constexpr int LS = N + (N >> 5) + 1;
-  static __device__ __forceinline__ int pos(int i) { return i + (i >> 5); }
+  static __host__ __device__ constexpr int pos(int i) { return i + (i >> 5); }
 };
 
 static constexpr int lde_min(int a, int b) { return a < b ? a : b; }
 
+// 32-bit components of an element (1 for the base field, 4 for EF4).
+template <typename T>
+struct Comps {
+  static constexpr int value = sizeof(T) / sizeof(u32);
+};
+__device__ __forceinline__ u32 comp_of(const u32& x, int) { return x; }
+__device__ __forceinline__ u32 comp_of(const ef& x, int c) { return x.c[c]; }
+__device__ __forceinline__ void set_comp(u32& x, int, u32 v) { x = v; }
+__device__ __forceinline__ void set_comp(ef& x, int c, u32 v) { x.c[c] = v; }
+
+// a + t and a - t for a < 2^32, t < 2p, as representatives below 2^32.  On a
+// carry a + t - 2^32 < t < 2p, so adding R1 = 2^32 - 2p stays below 2^32; on a
+// borrow a - t + 2^32 > 2^32 - 2p = R1, so taking R1 off stays nonnegative.
+__device__ __forceinline__ u32 wadd(u32 a, u32 t) {
+  u32 s;
+  const bool c = __builtin_add_overflow(a, t, &s);
+  return s + (c ? R1 : 0u);
+}
+__device__ __forceinline__ u32 wsub(u32 a, u32 t) {
+  u32 d;
+  const bool b = __builtin_sub_overflow(a, t, &d);
+  return d - (b ? R1 : 0u);
+}
+// x < 2^32 -> below 2p
+__device__ __forceinline__ u32 csub2p(u32 x) {
+  const u32 t = x - TWO_P;
+  return t < x ? t : x;
+}
+// x < 2^32 (< 3p) -> canonical
+__device__ __forceinline__ u32 canon_any(u32 x) { return canon(csub2p(x)); }
+
+// x * w per component (x < 2^32 and w < p, or x < p and w < 2p): below 2p.
+template <typename T>
+__device__ __forceinline__ T tmul_lazy(T x, u32 w) {
+#pragma unroll
+  for (int c = 0; c < Comps<T>::value; c++) set_comp(x, c, mul_lazy(comp_of(x, c), w));
+  return x;
+}
+
+// One butterfly.  narrow: b is known to be below 2p (a value loaded in the
+// line's first stage); unit: w = 1.
+__device__ __forceinline__ void wbfly(u32& a, u32& b, u32 w, bool narrow, bool unit) {
+  const u32 t = unit ? (narrow ? b : csub2p(b)) : mul_lazy(b, w);
+  const u32 av = a;
+  a = wadd(av, t);
+  b = wsub(av, t);
+}
+
 // Stages S0+1 .. S0+R: for a group with base index b (bits S0..S0+R-1 clear)
 // the elements are b + (j << S0), j < 2^R; at local stage g the pair (j, j + h),
-// h = 2^(g-1), uses w_{2^(S0+g)}^((b mod 2^S0) + (j mod h) 2^S0).
+// h = 2^(g-1), uses w_{2^(S0+g)}^((b mod 2^S0) + (j mod h) 2^S0).  As b and
+// j << S0 have no bit in common, pos(b + (j << S0)) = pos(b) + pos(j << S0):
+// every access of a group is a constant offset from one address.
 template <typename T, int LG, int R, int S0>
 __device__ __forceinline__ void dit_pass(T* sh, int nlines, const u32* stw) {
   using L = LineShape<LG>;
   constexpr int G = 1 << R;
+  constexpr int NC = Comps<T>::value;
   constexpr int GPL = L::N >> R;
   const int ngroups = nlines * GPL;
   for (int g = threadIdx.x; g < ngroups; g += blockDim.x) {
@@ -65,10 +127,15 @@ __device__ __forceinline__ void dit_pass(T* sh, int nlines, const u32* stw) {
     const int gl = g % GPL;
     const int base_low = gl & ((1 << S0) - 1);
     const int base = base_low + ((gl >> S0) << (S0 + R));
-    T* p = sh + line * L::LS;
-    T v[G];
+    T* q = sh + line * L::LS + L::pos(base);
+    const u32* tws = stw + base_low;
+    u32 v[G][NC];
 #pragma unroll
-    for (int j = 0; j < G; j++) v[j] = p[L::pos(base + (j << S0))];
+    for (int j = 0; j < G; j++) {
+      const T x = q[L::pos(j << S0)];
+#pragma unroll
+      for (int c = 0; c < NC; c++) v[j][c] = comp_of(x, c);
+    }
 #pragma unroll
     for (int sg = 1; sg <= R; sg++) {
       const int h = 1 << (sg - 1);
@@ -76,15 +143,19 @@ __device__ __forceinline__ void dit_pass(T* sh, int nlines, const u32* stw) {
       for (int j = 0; j < G; j++) {
         if (j & h) continue;
         const int jl = j & (h - 1);
-        T b = v[j + h];
-        if (S0 != 0 || jl != 0) b = tmulb(b, stw[(1 << (S0 + sg - 1)) + base_low + (jl << S0)]);
-        const T a = v[j];
-        v[j] = tadd(a, b);
-        v[j + h] = tsub(a, b);
+        const bool unit = S0 == 0 && jl == 0;
+        const u32 w = unit ? 0u : tws[(1 << (S0 + sg - 1)) + (jl << S0)];
+#pragma unroll
+        for (int c = 0; c < NC; c++) wbfly(v[j][c], v[j + h][c], w, S0 == 0 && sg == 1, unit);
       }
     }
 #pragma unroll
-    for (int j = 0; j < G; j++) p[L::pos(base + (j << S0))] = v[j];
+    for (int j = 0; j < G; j++) {
+      T x;
+#pragma unroll
+      for (int c = 0; c < NC; c++) set_comp(x, c, v[j][c]);
+      q[L::pos(j << S0)] = x;
+    }
   }
 }
 
@@ -100,7 +171,7 @@ __device__ __forceinline__ void dit_lines(T* sh, int nlines, const u32* stw) {
 
 template <typename T>
 struct RadixMax {
-  static constexpr int value = sizeof(T) == 4 ? 5 : 4;
+  static constexpr int value = sizeof(T) == 4 ? 5 : 3;
 };
 
 template <int LG>
@@ -118,162 +189,319 @@ __device__ __forceinline__ T* line_storage(unsigned char* smem) {
   return reinterpret_cast<T*>(smem + (((sizeof(u32) << LG) + 15) & ~size_t(15)));
 }
 
-// Loads per thread in flight in the gather loops.
+// Launch-order block b of nb -> logical block.  The GPU hands launch-order
+// blocks to its 8 XCDs in turn, and each XCD has its own L2; this order gives
+// every XCD one contiguous range of logical blocks, so neighbouring blocks,
+// which read and write the neighbouring 64-byte pieces of the same 128-byte
+// lines, share an L2.  (The identity when nb is not a multiple of 8.)
+__device__ __forceinline__ u32 xcd_order(u32 b, u32 nb) {
+  return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
+}
+
+// Buffer-resource loads and stores: a 32-bit offset per thread plus a uniform
+// (scalar) offset per row, so a row costs no address arithmetic.  Every range
+// addressed this way is below 2^31 bytes (a column: at most 2^24 extension
+// elements; a block's pass-2 lines: 2^m elements, checked by lde_plan);
+// accesses past num_records read zero and write nothing instead of faulting.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, u32 bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
 template <typename T>
-struct Batch {
-  static constexpr int value = sizeof(T) == 4 ? 8 : 4;
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, u32 voff, u32 soff);
+template <>
+__device__ __forceinline__ u32 buf_load<u32>(__amdgpu_buffer_rsrc_t r, u32 voff, u32 soff) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0);
+}
+template <>
+__device__ __forceinline__ ef buf_load<ef>(__amdgpu_buffer_rsrc_t r, u32 voff, u32 soff) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0);
+  return ef{{v[0], v[1], v[2], v[3]}};
+}
+__device__ __forceinline__ void buf_store(u32 x, __amdgpu_buffer_rsrc_t r, u32 voff, u32 soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(x, r, (int)voff, (int)soff, 0);
+}
+// A 16-byte store keeps a zero scalar offset: with a register soffset the
+// compiler omits the wait state a store of more than 8 bytes needs before a
+// VALU instruction overwrites its data registers, and the store then wrote
+// the overwritten first 8 bytes now and then (measured on MI355X).
+__device__ __forceinline__ void buf_store(const ef& x, __amdgpu_buffer_rsrc_t r, u32 voff, u32 soff) {
+  u32x4 v;
+  v[0] = x.c[0];
+  v[1] = x.c[1];
+  v[2] = x.c[2];
+  v[3] = x.c[3];
+  __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)(voff + soff), 0, 0);
+}
+
+// Pass-1 blocks: 16K base or 4K extension elements (64 KiB of lines, two
+// blocks per CU) in 512 threads, four waves per SIMD.
+template <typename T>
+struct Pass1Shape {
+  static constexpr int threads = 512;
+  static constexpr int elems = sizeof(T) == 4 ? 16384 : 4096;
+  static constexpr int elems_log = sizeof(T) == 4 ? 14 : 12;
+  static constexpr int per_thread = elems / threads;
 };
 
-// Threads per pass-1 block: a block holds 16K base or 4K extension elements
-// (64 KiB of LDS, two blocks per CU) in 512 threads, four waves per SIMD
-// (-10% on the 2^24 base LDE, -2% on the extension LDEs, against 256).  A
-// block's lines (TB) must not outnumber its threads: the row step is NT / TB.
-template <typename T>
-struct Pass1Threads {
-  static constexpr int value = 512;
-  static_assert(value >= 256, "pass 1 blocks take up to 256 lines");
-};
+// Blocks stay resident and take tiles in turn (tile = lt, lt + G, ... for G
+// blocks), the next tile's loads in flight while the current one is
+// transformed: a block reads, transforms and writes in sequence, so without
+// this the memory and arithmetic phases of all resident blocks coincide.
+// Logical tiles are handed out in XCD order (xcd_order of the block index
+// within each round of G tiles), so neighbouring tiles run at the same time
+// on one XCD and share its L2.
 
-// Pass 1 of the four-step split.  Grid: (L2 / TB, W, 2^R * proofs); LG = l1,
-// TB = 2^tb_log lines per block (TB <= 256); blockIdx.z = proof * 2^R + coset.
-// Thread t of a block always handles line (column offset) t mod TB and rows
-// t / TB + j * (NT / TB) for NT threads, so its coset twist w_H^(i0 b) and
-// four-step twiddle w_L^(b2 k1) advance by one product per row instead of two
-// table lookups.
+// Pass 1 of the four-step split.  Grid: (G, 1, proofs); LG = l1; TB =
+// 2^tb_log lines (column offsets b2) per tile, TB * 2^LG at most
+// Pass1Shape::elems.  Tile (a, chunk, i0), coset fastest: the 2^R tiles that
+// read one column segment are neighbours.  Thread t handles line t mod TB and
+// rows b1 = t / TB + j * (NT / TB): its coset twist w_H^(i0 b) and four-step
+// twiddle w_L^(b2 k1) advance by one product per row, and as t / TB and
+// j * (NT / TB) have no bit in common, its LDS positions (bit-reversed,
+// padded) and memory offsets are a per-thread part plus a part uniform
+// across the block.
 template <typename T, int LG>
-__global__ void __launch_bounds__(Pass1Threads<T>::value) k_lde_pass1(Group<LdeArgs<T>> g, int l, int R, int k,
-                                                                      int tb_log, Twiddles tw) {
+__global__ void __launch_bounds__(Pass1Shape<T>::threads) __attribute__((amdgpu_waves_per_eu(4)))
+k_lde_pass1(Group<LdeArgs<T>> g, int l, int R, int k, int tb_log, Twiddles tw) {
   extern __shared__ __align__(16) unsigned char smem[];
   using L = LineShape<LG>;
-  constexpr int U = Batch<T>::value;
-  const LdeArgs<T>& io = g.v[blockIdx.z >> R];
+  constexpr int NT = Pass1Shape<T>::threads;
+  constexpr int NR = Pass1Shape<T>::per_thread;
+  const LdeArgs<T>& io = g.v[blockIdx.z];
   const T* __restrict__ src = io.src;
   T* __restrict__ V = io.tmp;
   const int l2 = l - LG;
   const int W = 1 << k;
-  const int TB = 1 << tb_log;
-  const int b2_0 = blockIdx.x << tb_log;
-  const int a = blockIdx.y;
-  const int i0 = blockIdx.z & ((1 << R) - 1);
-  const int H_log = l + R;
+  const int cl = l2 - tb_log;   // log2 of a column's chunks
+  const u32 ntiles = ((1u << cl) << k) << R;
+  const u32 G = gridDim.x;
+  const u32 me = xcd_order(blockIdx.x, G);
   u32* stw = reinterpret_cast<u32*>(smem);
   T* sh = line_storage<T, LG>(smem);
   fill_stage_twiddles<LG>(stw, tw);
-  const int t = threadIdx.x & (TB - 1);
+  const int t = threadIdx.x & ((1 << tb_log) - 1);
   const int row0 = threadIdx.x >> tb_log;
-  const int rstep = Pass1Threads<T>::value >> tb_log;
-  const int b2 = b2_0 + t;
-  const T* colp = src + ((u64)a << l) + b2;
-  u32 tcur = ONE, tstep = ONE;
-  if (i0) {
-    tcur = root_pow(tw, H_log, (u64)i0 * (((u64)row0 << l2) + b2));
-    tstep = root_pow(tw, H_log, (u64)i0 * ((u64)rstep << l2));
-  }
-  for (int r = row0; r < L::N; r += U * rstep) {
-    T xs[U];
+  const int rstep = NT >> tb_log;
+  const u32 col_bytes = (u32)sizeof(T) << l;
+  const u32 rbytes = (u32)((rstep << l2) * sizeof(T));   // one row step
+  T* lines = sh + t * L::LS;
+  const int lb = L::pos(bitrev(row0, LG));   // LDS position of row0 (bit-reversed)
+  const int lo = L::pos(row0);               // LDS position of k1 = row0 (natural order)
+  // Row u of this thread exists iff u * rstep + row0 < N.  As row0 < rstep and
+  // both rstep and N are powers of two, that is u * rstep < N (uniform) and
+  // row0 < N (the same for every u), so no mask is kept per row.
+  const bool row_ok = row0 < L::N;
+  // tile lt -> coset, column and first column offset
+  auto tile_i0 = [&](u32 lt) { return (int)(lt & ((1u << R) - 1)); };
+  auto tile_a = [&](u32 lt) { return (int)((lt >> R) >> cl); };
+  auto tile_b2 = [&](u32 lt) { return ((int)((lt >> R) & ((1u << cl) - 1)) << tb_log) + t; };
+  T xs[NR];
+  auto load = [&](u32 lt) {
+    const __amdgpu_buffer_rsrc_t col = buf_rsrc(src + ((u64)tile_a(lt) << l), col_bytes);
+    const u32 voff = (u32)(((row0 << l2) + tile_b2(lt)) * sizeof(T));
+    u32 rb = rbytes;
+    int rs = rstep;
+    asm volatile("" : "+s"(rb), "+s"(rs));   // offsets formed at each load, not held across the tile loop
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int b1 = r + u * rstep;
-      if (b1 < L::N) xs[u] = colp[(u64)b1 << l2];
+    for (int u = 0; u < NR; u++)
+      if (u * rs < L::N && row_ok) xs[u] = buf_load<T>(col, voff, (u32)u * rb);
+  };
+  u32 lt = me;
+  if (lt < ntiles) load(lt);
+  while (lt < ntiles) {
+    const int i0 = tile_i0(lt), a = tile_a(lt), b2 = tile_b2(lt);
+    // twist by w_H^(i0 b), b = L2 b1 + b2: x < p and tcur < 2p, so both
+    // products stay below 2p
+    u32 tcur = 0, tstep = 0;
+    if (i0) {
+      tcur = root_pow(tw, l + R, (u64)i0 * (((u64)row0 << l2) + b2));
+      tstep = root_pow(tw, l + R, (u64)i0 * ((u64)rstep << l2));
     }
+    // Uniform offsets pass through an empty asm per tile, so the compiler
+    // forms each where it is used instead of holding all of them (in scalar
+    // registers it does not have) across the tile loop.
+    const int lbc = opaque(lb);
+    int rs = rstep;
+    asm volatile("" : "+s"(rs));
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int b1 = r + u * rstep;
-      if (b1 < L::N) {
+    for (int u = 0; u < NR; u++) {
+      if (u * rs < L::N && row_ok) {
         T x = xs[u];
         if (i0) {
-          x = tmulb(x, tcur);
-          tcur = mul(tcur, tstep);
+          x = tmul_lazy(x, tcur);
+          tcur = mul_lazy(tcur, tstep);
         }
-        sh[t * L::LS + L::pos(bitrev(b1, LG))] = x;
+        lines[lbc + L::pos(bitrev(u * rs, LG))] = x;
       }
     }
-  }
-  __syncthreads();
-  dit_lines<T, LG, RadixMax<T>::value>(sh, TB, stw);
-  T* out = V + (((u64)i0 * W + a) << l) + b2;
-  u32 ocur = root_pow(tw, l, (u64)b2 * row0);
-  const u32 ostep = root_pow(tw, l, (u64)b2 * rstep);
-  for (int k1 = row0; k1 < L::N; k1 += rstep) {
-    out[(u64)k1 << l2] = tmulb(sh[t * L::LS + L::pos(k1)], ocur);
-    ocur = mul(ocur, ostep);
+    __syncthreads();
+    const u32 next = lt + G;
+    if (next < ntiles) load(next);   // in flight while this tile is transformed
+    dit_lines<T, LG, RadixMax<T>::value>(sh, 1 << tb_log, stw);
+    const __amdgpu_buffer_rsrc_t out = buf_rsrc(V + (((u64)i0 * W + a) << l), col_bytes);
+    const u32 voff = (u32)(((row0 << l2) + b2) * sizeof(T));
+    u32 ocur = root_pow(tw, l, (u64)b2 * row0);   // canonical: line values are any 32-bit representatives
+    const u32 ostep = root_pow(tw, l, (u64)b2 * rstep);
+    const int loc = opaque(lo);
+    u32 rb = rbytes;
+    asm volatile("" : "+s"(rb));
+    rs = rstep;
+    asm volatile("" : "+s"(rs));
+#pragma unroll
+    for (int j = 0; j < NR; j++) {
+      if (j * rs < L::N && row_ok) {
+        buf_store(tmul_lazy(lines[loc + L::pos(j * rs)], ocur), out, voff, (u32)j * rb);
+        ocur = mul(ocur, ostep);
+      }
+    }
+    __syncthreads();   // the lines are read before the next tile overwrites them
+    lt = next;
   }
 }
 
 // Threads per pass-2 block.  Its W lines take about 64 KiB of LDS, so a CU
-// holds two blocks: base-field blocks of 512 threads keep four waves per SIMD
-// (-5% on the 2^24 base LDE); extension blocks stay at 256 (512 was slower).
+// holds two blocks: base-field blocks of 512 threads keep four waves per SIMD;
+// extension blocks stay at 256.  A thread holds at most Pass2Shape::per_thread
+// elements of a tile (its prefetch registers); lde_plan keeps W * 2^l2 within.
 template <typename T>
-struct Pass2Threads {
-  static constexpr int value = sizeof(T) == 4 ? 512 : 256;
+struct Pass2Shape {
+  static constexpr int threads = sizeof(T) == 4 ? 512 : 256;
+  static constexpr int per_thread = sizeof(T) == 4 ? 32 : 16;
 };
 
-// Pass 2 (or the only pass when direct != 0).  Grid: (L1, 2^R, proofs); LG = l2.
-template <typename T, int LG>
-__global__ void __launch_bounds__(Pass2Threads<T>::value) k_lde_pass2(Group<LdeArgs<T>> g, int l, int R, int k,
-                                                                      int direct, Twiddles tw) {
+// Pass 2 (or, DIRECT, the only pass).  FULL: the tile fills every thread's
+// registers (W * 2^LG = NE * NT), so no element needs a guard.  Grid: (G, 1,
+// proofs), tiles
+// as in pass 1; LG = l2.  Tile k1 * 2^R + i0 (so the tiles writing the
+// adjacent rows of one k1 are neighbours) transforms the W lines of (i0, k1)
+// and writes whole output rows 2^R (k1 + L1 k2) + i0, canonical.  Element e
+// of a tile is column e >> LG, offset e & (L2 - 1); for e = t + u NT (t < NT)
+// the two terms have no bit in common, so every offset and LDS position
+// below is a per-thread part plus a uniform part.
+template <typename T, int LG, bool DIRECT, bool FULL>
+__global__ void __launch_bounds__(Pass2Shape<T>::threads)
+    __attribute__((amdgpu_waves_per_eu(Pass2Shape<T>::threads / 128)))
+    k_lde_pass2(Group<LdeArgs<T>> g, int l, int R, int k, Twiddles tw) {
   extern __shared__ __align__(16) unsigned char smem[];
   const LdeArgs<T>& io = g.v[blockIdx.z];
-  const T* __restrict__ src = direct ? io.src : io.tmp;
   T* __restrict__ dst = io.dst;
   using L = LineShape<LG>;
-  constexpr int U = Batch<T>::value;
-  const int l2 = LG, L2 = L::N;
+  constexpr int NT = Pass2Shape<T>::threads;
+  constexpr int NE = Pass2Shape<T>::per_thread;
+  constexpr int NC = Comps<T>::value;
+  constexpr int L2 = L::N;
   const int l1 = l - LG;
   const int W = 1 << k;
-  const int k1 = blockIdx.x;
-  const int i0 = blockIdx.y;
+  const u32 ntiles = (1u << l1) << R;
+  const u32 G = gridDim.x;
+  const u32 me = xcd_order(blockIdx.x, G);
   u32* stw = reinterpret_cast<u32*>(smem);
   T* sh = line_storage<T, LG>(smem);
   fill_stage_twiddles<LG>(stw, tw);
   const int total = W * L2;
-  for (int e0 = threadIdx.x; e0 < total; e0 += U * Pass2Threads<T>::value) {
-    T xs[U];
+  const int t = threadIdx.x;
+  // e + t < total for e a multiple of NT: as total and NT are powers of two,
+  // that is e < total (uniform) and t < total (the same for every e).
+  const bool t_ok = FULL || t < total;
+  auto src_off_l = [](int e, int ll) { return (u32)((((e >> LG) << ll) + (e & (L2 - 1))) * (int)sizeof(T)); };
+  auto src_off = [&](int e) { return src_off_l(e, l); };
+  auto lds_pos = [&](int e) { return (e >> LG) * L::LS + L::pos(bitrev(e & (L2 - 1), LG)); };
+  const u32 voff = src_off(t);
+  const int lt_in = lds_pos(t);
+  T xs[NE];
+  auto load = [&](u32 lt) {
+    const int i0 = (int)(lt & ((1u << R) - 1)), k1 = (int)(lt >> R);
+    // the tile's W lines: column a at a * L, offset b2
+    const T* lsrc = DIRECT ? io.src : io.tmp + ((((u64)i0 * W) << l) + ((u64)k1 << LG));
+    const __amdgpu_buffer_rsrc_t in = buf_rsrc(lsrc, (u32)sizeof(T) << (l + k));
+    int ll = l;
+    asm volatile("" : "+s"(ll));   // offsets formed at each load, not held across the tile loop
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int e = e0 + u * Pass2Threads<T>::value;
-      if (e < total) {
-        const int a = e >> l2, b2 = e & (L2 - 1);
-        xs[u] = direct ? src[((u64)a << l) + b2] : src[((((u64)i0 * W + a) << l)) + ((u64)k1 << l2) + b2];
-      }
-    }
+    for (int u = 0; u < NE; u++)
+      if (FULL || (u * NT < total && t_ok)) xs[u] = buf_load<T>(in, voff, src_off_l(u * NT, ll));
+  };
+  // e = k2 * W + a -> dst row ((k1 + L1 k2) << R) + i0, column a
+  const int rk = l1 + R + k;   // log2 of one k2 step in elements
+  auto dst_off_k = [](int e, int kk, int rkk) { return ((u64)(e >> kk) << rkk) + (u64)(e & ((1 << kk) - 1)); };
+  auto lds_out_k = [](int e, int kk) { return (e & ((1 << kk) - 1)) * L::LS + L::pos(e >> kk); };
+  auto dst_off = [&](int e) { return dst_off_k(e, k, rk); };
+  auto lds_out = [&](int e) { return lds_out_k(e, k); };
+  const u64 dt_off = dst_off(t);
+  const int lt_out = lds_out(t);
+  u32 lt = me;
+  if (lt < ntiles) load(lt);
+  while (lt < ntiles) {
+    const int i0 = (int)(lt & ((1u << R) - 1)), k1 = (int)(lt >> R);
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int e = e0 + u * Pass2Threads<T>::value;
-      if (e < total) {
-        const int a = e >> l2, b2 = e & (L2 - 1);
+    for (int u = 0; u < NE; u++) {
+      const int e = u * NT;
+      if (FULL || (e < total && t_ok)) {
         T x = xs[u];
-        if (direct && i0) x = tmulb(x, root_pow(tw, l + R, (u64)i0 * b2));
-        sh[a * L::LS + L::pos(bitrev(b2, l2))] = x;
+        // DIRECT: canonical input times a canonical twist, below 2p
+        if (DIRECT && i0) x = tmul_lazy(x, root_pow(tw, l + R, (u64)i0 * ((e + t) & (L2 - 1))));
+        sh[lt_in + lds_pos(e)] = x;
       }
     }
-  }
-  __syncthreads();
-  dit_lines<T, LG, RadixMax<T>::value>(sh, W, stw);
-  const int L1 = 1 << l1;
-  for (int e = threadIdx.x; e < total; e += blockDim.x) {
-    int a = e & (W - 1), k2 = e >> k;
-    u64 row = ((u64)(k1 + (u64)L1 * k2) << R) + i0;
-    dst[row * W + a] = sh[a * L::LS + L::pos(k2)];
+    __syncthreads();
+    const u32 next = lt + G;
+    if (next < ntiles) load(next);   // in flight while this tile is transformed
+    dit_lines<T, LG, RadixMax<T>::value>(sh, W, stw);
+    T* dt = dst + ((((u64)k1 << R) + i0) << k) + dt_off;
+    const int lo = opaque(lt_out);
+    int kk = k, rkk = rk;
+    asm volatile("" : "+s"(kk), "+s"(rkk));
+    if (t_ok) {
+#pragma unroll
+      for (int u = 0; u < NE; u++) {
+        const int e = u * NT;
+        if (FULL || e < total) {
+          T x = sh[lo + lds_out_k(e, kk)];
+#pragma unroll
+          for (int c = 0; c < NC; c++) set_comp(x, c, canon_any(comp_of(x, c)));
+          dt[dst_off_k(e, kk, rkk)] = x;
+        }
+      }
+    }
+    __syncthreads();   // the lines are read before the next tile overwrites them
+    lt = next;
   }
 }
 
-// Dynamic LDS above 64 KiB must be opted into per kernel and device; the
-// opt-in is made once (hipFuncSetAttribute takes a runtime-wide lock, which
-// concurrent provers would otherwise contend for on every launch).
+// Dynamic LDS above 64 KiB must be opted into per kernel and device, and a
+// launch takes as many blocks as the device holds at once.  Both are settled
+// once per kernel, device and LDS size (hipFuncSetAttribute takes a
+// runtime-wide lock, which concurrent provers would otherwise contend for on
+// every launch).  Returns the resident blocks (at least 1).
 template <typename K>
-static void allow_lds(K kernel, size_t bytes) {
-  if (bytes <= 64 * 1024) return;
+static int prepare_kernel(K kernel, int threads, size_t bytes) {
   static std::mutex mu;
-  static std::set<std::tuple<const void*, int, size_t>> done;
+  static std::map<std::tuple<const void*, int, size_t>, int> done;
   int dev = 0;
   (void)hipGetDevice(&dev);
   const auto key = std::make_tuple(reinterpret_cast<const void*>(kernel), dev, bytes);
   std::lock_guard<std::mutex> lock(mu);
-  if (done.count(key)) return;
-  if (hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)bytes) == hipSuccess)
-    done.insert(key);
+  auto it = done.find(key);
+  if (it != done.end()) return it->second;
+  if (bytes > 64 * 1024 && hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
+    return 1;
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kernel), threads, bytes) !=
+          hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 1;
+  const int r = std::max(1, per_cu * cus);
+  done.emplace(key, r);
+  return r;
+}
+
+// Blocks of a persistent launch per proof: the resident blocks shared by the
+// group's proofs, a multiple of 8 (one share per XCD) where possible, and no
+// more than the tiles.
+static unsigned persistent_grid(int resident, int proofs, u64 tiles) {
+  u64 g = (u64)std::max(1, resident / std::max(1, proofs));
+  if (g >= 8) g &= ~u64(7);
+  return (unsigned)std::max<u64>(1, std::min(g, tiles));
 }
 
 static size_t lds_bytes(int lg, int lines, size_t esize) {
@@ -306,22 +534,28 @@ static auto pass1_for(int lg) -> void (*)(Group<LdeArgs<T>>, int, int, int, int,
   }
 }
 
-template <typename T>
-static auto pass2_for(int lg) -> void (*)(Group<LdeArgs<T>>, int, int, int, int, Twiddles) {
+template <typename T, bool DIRECT, bool FULL>
+static auto pass2_lg(int lg) -> void (*)(Group<LdeArgs<T>>, int, int, int, Twiddles) {
   switch (lg) {
-    case 0: return k_lde_pass2<T, 0>;
-    case 1: return k_lde_pass2<T, 1>;
-    case 2: return k_lde_pass2<T, 2>;
-    case 3: return k_lde_pass2<T, 3>;
-    case 4: return k_lde_pass2<T, 4>;
-    case 5: return k_lde_pass2<T, 5>;
-    case 6: return k_lde_pass2<T, 6>;
-    case 7: return k_lde_pass2<T, 7>;
-    case 8: return k_lde_pass2<T, 8>;
-    case 9: return k_lde_pass2<T, 9>;
-    case 10: return k_lde_pass2<T, 10>;
+    case 0: return k_lde_pass2<T, 0, DIRECT, FULL>;
+    case 1: return k_lde_pass2<T, 1, DIRECT, FULL>;
+    case 2: return k_lde_pass2<T, 2, DIRECT, FULL>;
+    case 3: return k_lde_pass2<T, 3, DIRECT, FULL>;
+    case 4: return k_lde_pass2<T, 4, DIRECT, FULL>;
+    case 5: return k_lde_pass2<T, 5, DIRECT, FULL>;
+    case 6: return k_lde_pass2<T, 6, DIRECT, FULL>;
+    case 7: return k_lde_pass2<T, 7, DIRECT, FULL>;
+    case 8: return k_lde_pass2<T, 8, DIRECT, FULL>;
+    case 9: return k_lde_pass2<T, 9, DIRECT, FULL>;
+    case 10: return k_lde_pass2<T, 10, DIRECT, FULL>;
     default: return nullptr;
   }
+}
+// Pass 2 for lines of 2^lg and W = 2^k of them.
+template <typename T, bool DIRECT>
+static auto pass2_for(int lg, int k) -> void (*)(Group<LdeArgs<T>>, int, int, int, Twiddles) {
+  const bool full = (1 << (lg + k)) == Pass2Shape<T>::per_thread * Pass2Shape<T>::threads;
+  return full ? pass2_lg<T, DIRECT, true>(lg) : pass2_lg<T, DIRECT, false>(lg);
 }
 
 // Launch plan of one LDE: a single pass (l <= LP2) or the four-step split.
@@ -331,41 +565,48 @@ struct LdePlan {
   size_t sm1 = 0, sm2 = 0;
 };
 
-// Pass 1 takes about 64 KiB of lines per block (16K base or 4K extension
-// elements), at most 256 lines and at most L2; lines of 2^12 and longer take
-// fewer (down to one line per block) and up to 160 KiB of LDS.  A tuned
-// tb_log (>= 0) replaces that rule, within the same limits.  Returns 0, or a
-// negative code for a shape no kernel instance covers.
+// Pass 1 takes Pass1Shape::elems elements per block (16K base or 4K
+// extension: 64 KiB of lines), at most 256 lines and at most L2; lines of
+// 2^12 and longer take fewer (down to one line per block) and up to 160 KiB of
+// LDS.  A tuned tb_log (>= 0) replaces that rule, within the same limits (a
+// block never holds more than Pass1Shape::elems elements: each thread keeps
+// its share in registers).  Returns 0, or a negative code for a shape no
+// kernel instance covers.
 template <typename T>
 static int lde_plan(int m, int k, int R, int LP2, int tb_tuned, LdePlan* pl) {
   const int l = m - k;
   if (l < 0 || k < 0 || R < 0 || k > 10) return -1;
   if (l + R > 24) return -7;   // the codeword domain exceeds KoalaBear's two-adic subgroups
+  if (((u64)sizeof(T) << m) >= (1ull << 31)) return -7;   // beyond the buffer loads' 32-bit offsets
   if (LP2 < (sizeof(T) == 4 ? 6 : 2)) return -6;   // rows too wide for a single-pass line in LDS
   const int W = 1 << k;
+  constexpr int pass2_max = Pass2Shape<T>::per_thread * Pass2Shape<T>::threads;
   if (l <= LP2) {
     pl->two_pass = false;
+    if (W << l > pass2_max) return -2;   // more than a block's registers hold
     pl->sm2 = lds_bytes(l, W, sizeof(T));
     if (pl->sm2 > 160 * 1024) return -2;
-    if (!pass2_for<T>(l)) return -6;
+    if (!pass2_for<T, true>(l, k)) return -6;
     return 0;
   }
   pl->two_pass = true;
   pl->l2 = LP2;
   pl->l1 = l - LP2;
-  const int elem_log = sizeof(T) == 4 ? 14 : 12;
+  const int elem_log = Pass1Shape<T>::elems_log;
   int tb_log = tb_tuned >= 0 ? tb_tuned : elem_log - pl->l1;
-  if (tb_log > 8) tb_log = 8;   // TB <= 256 < Pass1Threads: the row step NT / TB is at least 2
+  if (tb_log > 8) tb_log = 8;   // TB <= 256 < Pass1Shape::threads: the row step NT / TB is at least 2
   if (tb_log > pl->l2) tb_log = pl->l2;
   if (tb_log < 0) tb_log = 0;
   if (tb_tuned < 0)
     while (tb_log > 0 && lds_bytes(pl->l1, 1 << tb_log, sizeof(T)) > 80 * 1024) tb_log--;
+  if (pl->l1 + tb_log > elem_log) return -3;   // more than a block's registers hold
   pl->tb_log = tb_log;
   pl->sm1 = lds_bytes(pl->l1, 1 << tb_log, sizeof(T));
   if (pl->sm1 > 160 * 1024) return -3;
+  if (W << pl->l2 > pass2_max) return -2;
   pl->sm2 = lds_bytes(pl->l2, W, sizeof(T));
   if (pl->sm2 > 160 * 1024) return -2;
-  if (!pass1_for<T>(pl->l1) || !pass2_for<T>(pl->l2)) return -6;
+  if (!pass1_for<T>(pl->l1) || !pass2_for<T, false>(pl->l2, k)) return -6;
   return 0;
 }
 
@@ -380,23 +621,22 @@ static int lde_impl(const LdeArgs<T>* io, int count, int m, int k, int R, int LP
   const int W = 1 << k;
   const unsigned np = (unsigned)count;
   if (!pl.two_pass) {
-    auto kern = pass2_for<T>(l);
-    allow_lds(kern, pl.sm2);
+    auto kern = pass2_for<T, true>(l, k);
+    const unsigned G = persistent_grid(prepare_kernel(kern, Pass2Shape<T>::threads, pl.sm2), count, 1ull << R);
     return launch_group(io, count, [&](const Group<LdeArgs<T>>& g) {
-      hipLaunchKernelGGL(kern, dim3(1, 1u << R, np), dim3(Pass2Threads<T>::value), pl.sm2, st, g, l, R, k, 1, tw);
+      hipLaunchKernelGGL(kern, dim3(G, 1, np), dim3(Pass2Shape<T>::threads), pl.sm2, st, g, l, R, k, tw);
     });
   }
   for (int i = 0; i < count; i++)
     if (io[i].tmp == nullptr) return -4;
   auto k1 = pass1_for<T>(pl.l1);
-  auto k2 = pass2_for<T>(pl.l2);
-  allow_lds(k1, pl.sm1);
-  allow_lds(k2, pl.sm2);
+  auto k2 = pass2_for<T, false>(pl.l2, k);
+  const unsigned G1 = persistent_grid(prepare_kernel(k1, Pass1Shape<T>::threads, pl.sm1), count,
+                                      ((1ull << (pl.l2 - pl.tb_log)) << k) << R);
+  const unsigned G2 = persistent_grid(prepare_kernel(k2, Pass2Shape<T>::threads, pl.sm2), count, (1ull << pl.l1) << R);
   return launch_group(io, count, [&](const Group<LdeArgs<T>>& g) {
-    hipLaunchKernelGGL(k1, dim3(1u << (pl.l2 - pl.tb_log), W, (1u << R) * np), dim3(Pass1Threads<T>::value), pl.sm1,
-                       st, g, l, R, k, pl.tb_log, tw);
-    hipLaunchKernelGGL(k2, dim3(1u << pl.l1, 1u << R, np), dim3(Pass2Threads<T>::value), pl.sm2, st, g, l, R, k, 0,
-                       tw);
+    hipLaunchKernelGGL(k1, dim3(G1, 1, np), dim3(Pass1Shape<T>::threads), pl.sm1, st, g, l, R, k, pl.tb_log, tw);
+    hipLaunchKernelGGL(k2, dim3(G2, 1, np), dim3(Pass2Shape<T>::threads), pl.sm2, st, g, l, R, k, tw);
   });
 }
 

@@ -143,51 +143,6 @@ __host__ __device__ __forceinline__ void internal_linear(u32* s) {
 //
 // Every intermediate bound is stated where it is used; the largest 64-bit
 // value reduced is below 71p < 2^37.2.
-constexpr u32 TWO_P = 2u * P;
-
-// 32x32+64 multiply-adds (v_mad_u64_u32 / v_mad_i64_i32, one full-rate
-// instruction each on CDNA).
-__host__ __device__ __forceinline__ u64 mad_u64(u32 a, u32 b, u64 c) { return (u64)a * b + c; }
-__host__ __device__ __forceinline__ u64 mad_i64(int32_t a, int32_t b, u64 c) {
-  return (u64)((int64_t)a * (int64_t)b + (int64_t)c);
-}
-
-// y - (y >> 31) p = (y mod 2^31) + (y >> 31)(2^24 - 1): for y < 2^37.9 the
-// quotient is below 126 and the result below 2p.  Kept 64-bit (its high word
-// is zero) so it feeds 64-bit adds directly.
-// y >> 31 as a 32-bit value.  On the device it comes from v_alignbit_b32,
-// which keeps the compiler from widening the multiply-add below into a 64x64
-// product.
-__host__ __device__ __forceinline__ u32 quot31(u64 y) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  return __builtin_amdgcn_alignbit((u32)(y >> 32), (u32)y, 31);
-#else
-  return (u32)(y >> 31);
-#endif
-}
-// Hides a value's origin from the compiler (no instruction), so it keeps the
-// multiply-add forms below instead of rewriting them into longer sequences
-// (24-bit products, 64-bit subtractions, selects).  A 32-bit value added to a
-// 64-bit one is written mad_u64(x, one, y) with one = opaque(1u): a single
-// v_mad_u64_u32, where a plain add would first widen x with a zeroed high
-// word.
-template <typename T>
-__host__ __device__ __forceinline__ T opaque(T x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  asm("" : "+v"(x));
-#endif
-  return x;
-}
-
-__host__ __device__ __forceinline__ u64 lazy_reduce(u64 y) {
-  return mad_i64((int32_t)quot31(y), -(int32_t)P, y);
-}
-__host__ __device__ __forceinline__ u32 full_reduce(u64 y) {   // y < 2^37.9 -> [0, p)
-  const u32 z = (u32)lazy_reduce(y);
-  const u32 t = z - P;
-  return t < z ? t : z;
-}
-
 // sum + z 2^-K and sum - z 2^-K for z < 2p; `sumk` = sum + 2p keeps both
 // nonnegative (r 127 2^(24-K) < 2^31 < 2p, and z >> K < p).
 template <int K>

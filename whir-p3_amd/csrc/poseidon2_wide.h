@@ -67,15 +67,6 @@ __device__ __forceinline__ u32 wide_row_sum(u32 x) {
   return csub(s + dpp<ROR(1)>(s));
 }
 
-// a b / 2^32 mod p for a < 2p, b < p, without the conditional subtraction:
-// (a b + (2^32 - 1) p) / 2^32 < 2p, and the 64-bit sum stays below
-// 2p^2 + 2^32 p < 2^64.
-__device__ __forceinline__ u32 mul_lazy(u32 a, u32 b) {
-  const u64 m = (u64)a * b;
-  const u32 t = (u32)m * MU_NEG;
-  return (u32)(((u64)t * P + m) >> 32);
-}
-
 // Per-lane constants of a 16-lane permutation: external round constants of
 // this lane's element and its internal diagonal entry (Montgomery form).
 struct WideLane {
