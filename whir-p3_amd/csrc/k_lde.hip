@@ -93,14 +93,6 @@ __device__ __forceinline__ u32 csub2p(u32 x) {
 // x < 2^32 (< 3p) -> canonical
 __device__ __forceinline__ u32 canon_any(u32 x) { return canon(csub2p(x)); }
 
-// x * w per component (x < 2^32 and w < p, or x < p and w < 2p): below 2p.
-template <typename T>
-__device__ __forceinline__ T tmul_lazy(T x, u32 w) {
-#pragma unroll
-  for (int c = 0; c < Comps<T>::value; c++) set_comp(x, c, mul_lazy(comp_of(x, c), w));
-  return x;
-}
-
 // One butterfly.  narrow: b is known to be below 2p (a value loaded in the
 // line's first stage); unit: w = 1.
 __device__ __forceinline__ void wbfly(u32& a, u32& b, u32 w, bool narrow, bool unit) {
@@ -115,7 +107,7 @@ __device__ __forceinline__ void wbfly(u32& a, u32& b, u32 w, bool narrow, bool u
 // h = 2^(g-1), uses w_{2^(S0+g)}^((b mod 2^S0) + (j mod h) 2^S0).  As b and
 // j << S0 have no bit in common, pos(b + (j << S0)) = pos(b) + pos(j << S0):
 // every access of a group is a constant offset from one address.
-template <typename T, int LG, int R, int S0>
+template <typename T, int LG, int R, int S0, bool UNIT>
 __device__ __forceinline__ void dit_pass(T* sh, int nlines, const u32* stw) {
   using L = LineShape<LG>;
   constexpr int G = 1 << R;
@@ -143,7 +135,7 @@ __device__ __forceinline__ void dit_pass(T* sh, int nlines, const u32* stw) {
       for (int j = 0; j < G; j++) {
         if (j & h) continue;
         const int jl = j & (h - 1);
-        const bool unit = S0 == 0 && jl == 0;
+        const bool unit = UNIT && S0 == 0 && jl == 0;
         const u32 w = unit ? 0u : tws[(1 << (S0 + sg - 1)) + (jl << S0)];
 #pragma unroll
         for (int c = 0; c < NC; c++) wbfly(v[j][c], v[j + h][c], w, S0 == 0 && sg == 1, unit);
@@ -159,13 +151,15 @@ __device__ __forceinline__ void dit_pass(T* sh, int nlines, const u32* stw) {
   }
 }
 
-template <typename T, int LG, int RMAX, int S0 = 0>
+// UNIT: the table's first entry of every stage is 1 (an untwisted table), so
+// those butterflies skip their product.
+template <typename T, int LG, int RMAX, bool UNIT, int S0 = 0>
 __device__ __forceinline__ void dit_lines(T* sh, int nlines, const u32* stw) {
   if constexpr (S0 < LG) {
     constexpr int R = lde_min(RMAX, LG - S0);
-    dit_pass<T, LG, R, S0>(sh, nlines, stw);
+    dit_pass<T, LG, R, S0, UNIT>(sh, nlines, stw);
     __syncthreads();
-    dit_lines<T, LG, RMAX, S0 + R>(sh, nlines, stw);
+    dit_lines<T, LG, RMAX, UNIT, S0 + R>(sh, nlines, stw);
   }
 }
 
@@ -174,19 +168,57 @@ struct RadixMax {
   static constexpr int value = sizeof(T) == 4 ? 5 : 3;
 };
 
+// Stage twiddles of a twisted line DFT Z[k] = sum_n x[n] theta^n w_N^(k n),
+// N = 2^LG, theta = w_H^kappa (H = 2^H_log): decimation in time with
+// stage-s twiddles theta^(N / 2^s) w_{2^s}^e, which is
+//   stw[2^(s-1) + e] = w_H^(kappa 2^(LG - s) + e 2^(H_log - s)).
+// kappa = 0 gives the plain table w_{2^s}^e.  A per-element twist or a
+// four-step twiddle that depends on the line position only through theta^n
+// thus costs one table of N entries per tile instead of a product per element.
+// A thread's entries of such a table (j = threadIdx.x + r NT) as the two
+// table words of each root_pow, fetched ahead (the loads in flight while
+// other work runs) and multiplied when the table is written.
+template <int LG, int NT>
+struct TwPend {
+  static constexpr int n = ((1 << LG) + NT - 1) / NT;
+  u32 hi[n], lo[n];
+};
+template <int LG, int NT>
+__device__ __forceinline__ void tw_fetch(TwPend<LG, NT>& p, const Twiddles& tw, int H_log, u64 kappa) {
+#pragma unroll
+  for (int r = 0; r < TwPend<LG, NT>::n; r++) {
+    const int j = (int)threadIdx.x + r * NT;
+    if (j < (1 << LG) && j) {
+      const int s = 32 - __clz(j);   // j in [2^(s-1), 2^s)
+      const u64 x = (kappa << (LG - s)) + ((u64)(j - (1 << (s - 1))) << (H_log - s));
+      const u32 e = (u32)((x << (24 - H_log)) & 0xffffffu);
+      p.hi[r] = tw.hi[e >> 12];
+      p.lo[r] = tw.lo[e & 4095u];
+    }
+  }
+}
+template <int LG, int NT>
+__device__ __forceinline__ void tw_write(const TwPend<LG, NT>& p, u32* stw) {
+#pragma unroll
+  for (int r = 0; r < TwPend<LG, NT>::n; r++) {
+    const int j = (int)threadIdx.x + r * NT;
+    if (j < (1 << LG) && j) stw[j] = mul(p.hi[r], p.lo[r]);
+  }
+}
 template <int LG>
-__device__ void fill_stage_twiddles(u32* stw, const Twiddles& tw) {
+__device__ __forceinline__ void fill_stage_twiddles(u32* stw, const Twiddles& tw, int H_log, u64 kappa) {
   for (int j = threadIdx.x; j < (1 << LG); j += blockDim.x) {
     if (j == 0) continue;
     const int s = 32 - __clz(j);   // j in [2^(s-1), 2^s)
-    stw[j] = root_pow(tw, s, (u64)(j - (1 << (s - 1))));
+    stw[j] = root_pow(tw, H_log, (kappa << (LG - s)) + ((u64)(j - (1 << (s - 1))) << (H_log - s)));
   }
 }
 
-// LDS: the stage twiddles (N words, 16-byte aligned) then nlines padded lines.
-template <typename T, int LG>
+// LDS: NTAB stage-twiddle tables (N words each, 16-byte aligned) then the
+// padded lines.
+template <typename T, int LG, int NTAB = 1>
 __device__ __forceinline__ T* line_storage(unsigned char* smem) {
-  return reinterpret_cast<T*>(smem + (((sizeof(u32) << LG) + 15) & ~size_t(15)));
+  return reinterpret_cast<T*>(smem + NTAB * (((sizeof(u32) << LG) + 15) & ~size_t(15)));
 }
 
 // Launch-order block b of nb -> logical block.  The GPU hands launch-order
@@ -233,6 +265,13 @@ __device__ __forceinline__ void buf_store(const ef& x, __amdgpu_buffer_rsrc_t r,
   __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)(voff + soff), 0, 0);
 }
 
+// Twiddle tables a pass-1 block keeps (one per coset of its pair, while
+// they fit beside the lines).
+template <int LG>
+struct Pass1Tables {
+  static constexpr int value = LG <= 12 ? 2 : 1;
+};
+
 // Pass-1 blocks: 16K base or 4K extension elements (64 KiB of lines, two
 // blocks per CU) in 512 threads, four waves per SIMD.
 template <typename T>
@@ -253,13 +292,19 @@ struct Pass1Shape {
 
 // Pass 1 of the four-step split.  Grid: (G, 1, proofs); LG = l1; TB =
 // 2^tb_log lines (column offsets b2) per tile, TB * 2^LG at most
-// Pass1Shape::elems.  Tile (a, chunk, i0), coset fastest: the 2^R tiles that
-// read one column segment are neighbours.  Thread t handles line t mod TB and
-// rows b1 = t / TB + j * (NT / TB): its coset twist w_H^(i0 b) and four-step
-// twiddle w_L^(b2 k1) advance by one product per row, and as t / TB and
+// Pass1Shape::elems.  With b = L2 b1 + b2 and output rows i = 2^R (k1 + L1 k2)
+// + i0, w_H^(i b) splits into w_L1^(k1 b1) w_H^(i0 L2 b1) (pass 1, a line DFT
+// over b1 twisted by theta = w_H^(i0 L2)) times w_H^(kappa b2) w_L2^(k2 b2)
+// with kappa = 2^R k1 + i0 (pass 2), so pass 1 writes
+//   V[i0][a][k1][b2] = sum_b1 c_a[L2 b1 + b2] w_H^(i0 L2 b1) w_L1^(k1 b1)
+// with no product per element: the coset twist lives in the stage twiddle
+// table.  Tile (a, chunk, coset pair), pair fastest: a tile loads its segment
+// of one column once and transforms it for 2^CB cosets (CB = min(R, 1)); the
+// next tile's loads go out once the last coset's lines are in LDS.  Thread t
+// handles line t mod TB and rows b1 = t / TB + j * (NT / TB); as t / TB and
 // j * (NT / TB) have no bit in common, its LDS positions (bit-reversed,
-// padded) and memory offsets are a per-thread part plus a part uniform
-// across the block.
+// padded) and memory offsets are a per-thread part plus a part uniform across
+// the block.
 template <typename T, int LG>
 __global__ void __launch_bounds__(Pass1Shape<T>::threads) __attribute__((amdgpu_waves_per_eu(4)))
 k_lde_pass1(Group<LdeArgs<T>> g, int l, int R, int k, int tb_log, Twiddles tw) {
@@ -272,13 +317,20 @@ k_lde_pass1(Group<LdeArgs<T>> g, int l, int R, int k, int tb_log, Twiddles tw) {
   T* __restrict__ V = io.tmp;
   const int l2 = l - LG;
   const int W = 1 << k;
-  const int cl = l2 - tb_log;   // log2 of a column's chunks
-  const u32 ntiles = ((1u << cl) << k) << R;
+  const int cl = l2 - tb_log;         // log2 of a column's chunks
+  const int CB = R < 1 ? R : 1;       // log2 of the cosets per tile
+  const int RG = R - CB;              // log2 of the coset groups
+  const u32 ntiles = ((1u << cl) << k) << RG;
   const u32 G = gridDim.x;
   const u32 me = xcd_order(blockIdx.x, G);
+  // One twiddle table per coset of the pair, rewritten when a tile's pair
+  // differs from the previous tile's (never for R <= 1); lines longer than
+  // 2^12 leave room for one table only, rewritten for every coset.
+  constexpr int NTAB = Pass1Tables<LG>::value;
+  constexpr u32 TAB = ((sizeof(u32) << LG) + 15) & ~size_t(15);
   u32* stw = reinterpret_cast<u32*>(smem);
-  T* sh = line_storage<T, LG>(smem);
-  fill_stage_twiddles<LG>(stw, tw);
+  T* sh = line_storage<T, LG, NTAB>(smem);
+  int tab_ig = -1;
   const int t = threadIdx.x & ((1 << tb_log) - 1);
   const int row0 = threadIdx.x >> tb_log;
   const int rstep = NT >> tb_log;
@@ -291,10 +343,10 @@ k_lde_pass1(Group<LdeArgs<T>> g, int l, int R, int k, int tb_log, Twiddles tw) {
   // both rstep and N are powers of two, that is u * rstep < N (uniform) and
   // row0 < N (the same for every u), so no mask is kept per row.
   const bool row_ok = row0 < L::N;
-  // tile lt -> coset, column and first column offset
-  auto tile_i0 = [&](u32 lt) { return (int)(lt & ((1u << R) - 1)); };
-  auto tile_a = [&](u32 lt) { return (int)((lt >> R) >> cl); };
-  auto tile_b2 = [&](u32 lt) { return ((int)((lt >> R) & ((1u << cl) - 1)) << tb_log) + t; };
+  // tile lt -> first coset, column and column offset
+  auto tile_ig = [&](u32 lt) { return (int)(lt & ((1u << RG) - 1)) << CB; };
+  auto tile_a = [&](u32 lt) { return (int)((lt >> RG) >> cl); };
+  auto tile_b2 = [&](u32 lt) { return ((int)((lt >> RG) & ((1u << cl) - 1)) << tb_log) + t; };
   T xs[NR];
   auto load = [&](u32 lt) {
     const __amdgpu_buffer_rsrc_t col = buf_rsrc(src + ((u64)tile_a(lt) << l), col_bytes);
@@ -309,52 +361,43 @@ k_lde_pass1(Group<LdeArgs<T>> g, int l, int R, int k, int tb_log, Twiddles tw) {
   u32 lt = me;
   if (lt < ntiles) load(lt);
   while (lt < ntiles) {
-    const int i0 = tile_i0(lt), a = tile_a(lt), b2 = tile_b2(lt);
-    // twist by w_H^(i0 b), b = L2 b1 + b2: x < p and tcur < 2p, so both
-    // products stay below 2p
-    u32 tcur = 0, tstep = 0;
-    if (i0) {
-      tcur = root_pow(tw, l + R, (u64)i0 * (((u64)row0 << l2) + b2));
-      tstep = root_pow(tw, l + R, (u64)i0 * ((u64)rstep << l2));
-    }
-    // Uniform offsets pass through an empty asm per tile, so the compiler
-    // forms each where it is used instead of holding all of them (in scalar
-    // registers it does not have) across the tile loop.
-    const int lbc = opaque(lb);
-    int rs = rstep;
-    asm volatile("" : "+s"(rs));
-#pragma unroll
-    for (int u = 0; u < NR; u++) {
-      if (u * rs < L::N && row_ok) {
-        T x = xs[u];
-        if (i0) {
-          x = tmul_lazy(x, tcur);
-          tcur = mul_lazy(tcur, tstep);
-        }
-        lines[lbc + L::pos(bitrev(u * rs, LG))] = x;
-      }
-    }
-    __syncthreads();
+    const int ig = tile_ig(lt), a = tile_a(lt), b2 = tile_b2(lt);
     const u32 next = lt + G;
-    if (next < ntiles) load(next);   // in flight while this tile is transformed
-    dit_lines<T, LG, RadixMax<T>::value>(sh, 1 << tb_log, stw);
-    const __amdgpu_buffer_rsrc_t out = buf_rsrc(V + (((u64)i0 * W + a) << l), col_bytes);
     const u32 voff = (u32)(((row0 << l2) + b2) * sizeof(T));
-    u32 ocur = root_pow(tw, l, (u64)b2 * row0);   // canonical: line values are any 32-bit representatives
-    const u32 ostep = root_pow(tw, l, (u64)b2 * rstep);
-    const int loc = opaque(lo);
-    u32 rb = rbytes;
-    asm volatile("" : "+s"(rb));
-    rs = rstep;
-    asm volatile("" : "+s"(rs));
-#pragma unroll
-    for (int j = 0; j < NR; j++) {
-      if (j * rs < L::N && row_ok) {
-        buf_store(tmul_lazy(lines[loc + L::pos(j * rs)], ocur), out, voff, (u32)j * rb);
-        ocur = mul(ocur, ostep);
-      }
+    if (NTAB == 2 && ig != tab_ig) {   // uniform; the previous tile ended with a barrier
+      for (int ci = 0; ci < (1 << CB); ci++)
+        fill_stage_twiddles<LG>(stw + ci * (TAB / 4), tw, l + R, (u64)(ig + ci) << l2);
+      tab_ig = ig;
     }
-    __syncthreads();   // the lines are read before the next tile overwrites them
+    for (int ci = 0; ci < (1 << CB); ci++) {
+      const int i0 = ig + ci;
+      if (NTAB == 1) fill_stage_twiddles<LG>(stw, tw, l + R, (u64)i0 << l2);
+      const u32* tab = stw + (NTAB == 2 ? ci * (TAB / 4) : 0);
+      // Uniform offsets pass through an empty asm per coset, so the compiler
+      // forms each where it is used instead of holding all of them (in scalar
+      // registers it does not have) across the loops.
+      const int lbc = opaque(lb);
+      int rs = rstep;
+      asm volatile("" : "+s"(rs));
+#pragma unroll
+      for (int u = 0; u < NR; u++)
+        if (u * rs < L::N && row_ok) lines[lbc + L::pos(bitrev(u * rs, LG))] = xs[u];
+      __syncthreads();
+      if (ci == (1 << CB) - 1 && next < ntiles) load(next);   // in flight while this coset is transformed
+      if (i0 == 0)
+        dit_lines<T, LG, RadixMax<T>::value, true>(sh, 1 << tb_log, tab);
+      else
+        dit_lines<T, LG, RadixMax<T>::value, false>(sh, 1 << tb_log, tab);
+      const __amdgpu_buffer_rsrc_t out = buf_rsrc(V + (((u64)i0 * W + a) << l), col_bytes);
+      const int loc = opaque(lo);
+      u32 rb = rbytes;
+      rs = rstep;
+      asm volatile("" : "+s"(rb), "+s"(rs));
+#pragma unroll
+      for (int j = 0; j < NR; j++)
+        if (j * rs < L::N && row_ok) buf_store(lines[loc + L::pos(j * rs)], out, voff, (u32)j * rb);
+      __syncthreads();   // lines and tables are read before the next coset or tile overwrites them
+    }
     lt = next;
   }
 }
@@ -369,7 +412,8 @@ struct Pass2Shape {
   static constexpr int per_thread = sizeof(T) == 4 ? 32 : 16;
 };
 
-// Pass 2 (or, DIRECT, the only pass).  FULL: the tile fills every thread's
+// Pass 2 (or, DIRECT, the only pass; then the twist is the coset's, w_H^(i0 b)).
+// FULL: the tile fills every thread's
 // registers (W * 2^LG = NE * NT), so no element needs a guard.  Grid: (G, 1,
 // proofs), tiles
 // as in pass 1; LG = l2.  Tile k1 * 2^R + i0 (so the tiles writing the
@@ -397,7 +441,6 @@ __global__ void __launch_bounds__(Pass2Shape<T>::threads)
   const u32 me = xcd_order(blockIdx.x, G);
   u32* stw = reinterpret_cast<u32*>(smem);
   T* sh = line_storage<T, LG>(smem);
-  fill_stage_twiddles<LG>(stw, tw);
   const int total = W * L2;
   const int t = threadIdx.x;
   // e + t < total for e a multiple of NT: as total and NT are powers of two,
@@ -409,8 +452,12 @@ __global__ void __launch_bounds__(Pass2Shape<T>::threads)
   const u32 voff = src_off(t);
   const int lt_in = lds_pos(t);
   T xs[NE];
+  TwPend<LG, NT> twp;
   auto load = [&](u32 lt) {
     const int i0 = (int)(lt & ((1u << R) - 1)), k1 = (int)(lt >> R);
+    // the tile's lines are twisted by w_H^(kappa b2), kappa = 2^R k1 + i0
+    // (DIRECT: the coset twist w_H^(i0 b), k1 = 0)
+    tw_fetch(twp, tw, l + R, ((u64)k1 << R) + i0);
     // the tile's W lines: column a at a * L, offset b2
     const T* lsrc = DIRECT ? io.src : io.tmp + ((((u64)i0 * W) << l) + ((u64)k1 << LG));
     const __amdgpu_buffer_rsrc_t in = buf_rsrc(lsrc, (u32)sizeof(T) << (l + k));
@@ -432,20 +479,16 @@ __global__ void __launch_bounds__(Pass2Shape<T>::threads)
   if (lt < ntiles) load(lt);
   while (lt < ntiles) {
     const int i0 = (int)(lt & ((1u << R) - 1)), k1 = (int)(lt >> R);
+    tw_write(twp, stw);
 #pragma unroll
     for (int u = 0; u < NE; u++) {
       const int e = u * NT;
-      if (FULL || (e < total && t_ok)) {
-        T x = xs[u];
-        // DIRECT: canonical input times a canonical twist, below 2p
-        if (DIRECT && i0) x = tmul_lazy(x, root_pow(tw, l + R, (u64)i0 * ((e + t) & (L2 - 1))));
-        sh[lt_in + lds_pos(e)] = x;
-      }
+      if (FULL || (e < total && t_ok)) sh[lt_in + lds_pos(e)] = xs[u];
     }
     __syncthreads();
     const u32 next = lt + G;
     if (next < ntiles) load(next);   // in flight while this tile is transformed
-    dit_lines<T, LG, RadixMax<T>::value>(sh, W, stw);
+    dit_lines<T, LG, RadixMax<T>::value, false>(sh, W, stw);
     T* dt = dst + ((((u64)k1 << R) + i0) << k) + dt_off;
     const int lo = opaque(lt_out);
     int kk = k, rkk = rk;
@@ -462,7 +505,7 @@ __global__ void __launch_bounds__(Pass2Shape<T>::threads)
         }
       }
     }
-    __syncthreads();   // the lines are read before the next tile overwrites them
+    __syncthreads();   // lines and twiddles are read before the next tile overwrites them
     lt = next;
   }
 }
@@ -504,8 +547,8 @@ static unsigned persistent_grid(int resident, int proofs, u64 tiles) {
   return (unsigned)std::max<u64>(1, std::min(g, tiles));
 }
 
-static size_t lds_bytes(int lg, int lines, size_t esize) {
-  size_t tw = ((sizeof(u32) << lg) + 15) & ~size_t(15);
+static size_t lds_bytes(int lg, int lines, size_t esize, int tables = 1) {
+  size_t tw = tables * (((sizeof(u32) << lg) + 15) & ~size_t(15));
   size_t ls = (1u << lg) + ((1u << lg) >> 5) + 1;
   return tw + (size_t)lines * ls * esize;
 }
@@ -598,10 +641,10 @@ static int lde_plan(int m, int k, int R, int LP2, int tb_tuned, LdePlan* pl) {
   if (tb_log > pl->l2) tb_log = pl->l2;
   if (tb_log < 0) tb_log = 0;
   if (tb_tuned < 0)
-    while (tb_log > 0 && lds_bytes(pl->l1, 1 << tb_log, sizeof(T)) > 80 * 1024) tb_log--;
+    while (tb_log > 0 && lds_bytes(pl->l1, 1 << tb_log, sizeof(T), pl->l1 <= 12 ? 2 : 1) > 80 * 1024) tb_log--;
   if (pl->l1 + tb_log > elem_log) return -3;   // more than a block's registers hold
   pl->tb_log = tb_log;
-  pl->sm1 = lds_bytes(pl->l1, 1 << tb_log, sizeof(T));
+  pl->sm1 = lds_bytes(pl->l1, 1 << tb_log, sizeof(T), pl->l1 <= 12 ? 2 : 1);
   if (pl->sm1 > 160 * 1024) return -3;
   if (W << pl->l2 > pass2_max) return -2;
   pl->sm2 = lds_bytes(pl->l2, W, sizeof(T));
@@ -632,7 +675,7 @@ static int lde_impl(const LdeArgs<T>* io, int count, int m, int k, int R, int LP
   auto k1 = pass1_for<T>(pl.l1);
   auto k2 = pass2_for<T, false>(pl.l2, k);
   const unsigned G1 = persistent_grid(prepare_kernel(k1, Pass1Shape<T>::threads, pl.sm1), count,
-                                      ((1ull << (pl.l2 - pl.tb_log)) << k) << R);
+                                      ((1ull << (pl.l2 - pl.tb_log)) << k) << (R - std::min(R, 1)));
   const unsigned G2 = persistent_grid(prepare_kernel(k2, Pass2Shape<T>::threads, pl.sm2), count, (1ull << pl.l1) << R);
   return launch_group(io, count, [&](const Group<LdeArgs<T>>& g) {
     hipLaunchKernelGGL(k1, dim3(G1, 1, np), dim3(Pass1Shape<T>::threads), pl.sm1, st, g, l, R, k, pl.tb_log, tw);
