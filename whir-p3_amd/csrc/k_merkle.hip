@@ -536,6 +536,8 @@ __global__ void __launch_bounds__(256) k_grind(const P2C16* __restrict__ cp, Gro
     for (int k = 0; k < 16; k++) base.s[k] = gbs.s[k];
     wpos = (int)gbs.wpos;
   }
+  GrindPre gp;
+  grind_pre(gp, base.s, wpos);
   const u32 stride = gridDim.x * 256;
   const u32 lane = blockIdx.x * 256 + threadIdx.x;
   const u32 wave0 = lane & ~63u;
@@ -551,10 +553,10 @@ __global__ void __launch_bounds__(256) k_grind(const P2C16* __restrict__ cp, Gro
     const P2C16& c = cp[off];
     const u64 w = g + lane;
     u32 s[16];
-#pragma unroll
-    for (int k = 0; k < 16; k++) s[k] = (k == wpos) ? to_monty((u32)w) : base.s[k];
+    u64 y[16];
+    grind_layer(gp, to_monty((u32)w), y);
     const u32 b1 = best_known(result);
-    permute_head<16>(s, c);
+    permute16_head_rounds(y, s, c);
     phase_fence(s);
     if ((u64)b1 <= first) break;
     const u32 b2 = best_known(result);
@@ -562,8 +564,8 @@ __global__ void __launch_bounds__(256) k_grind(const P2C16* __restrict__ cp, Gro
     phase_fence(s);
     if ((u64)b2 <= first) break;
     b0 = best_known(result);
-    permute_tail<16>(s, c);
-    if (w < end && (from_monty(s[7]) & mask) == 0) atomicMin(result, (u32)w);
+    const u32 s7 = permute16_tail_s7(s, c);
+    if (w < end && (from_monty(s7) & mask) == 0) atomicMin(result, (u32)w);
   }
 }
 
@@ -640,6 +642,8 @@ __global__ void __launch_bounds__(NT) k_grind_lock(const P2C16* __restrict__ cp,
     for (int k = 0; k < 16; k++) base.s[k] = gb->s[k];
     wpos = (int)gb->wpos;
   }
+  GrindPre gp;
+  grind_pre(gp, base.s, wpos);
   const u32 t = threadIdx.x;
   const u32 stride = gridDim.x * NT;
   if (t == 0) sb[0] = best_known(result);
@@ -655,13 +659,13 @@ __global__ void __launch_bounds__(NT) k_grind_lock(const P2C16* __restrict__ cp,
     const P2C16& c = cp[off];
     const u64 w = first + t;
     u32 s[16];
-#pragma unroll
-    for (int k = 0; k < 16; k++) s[k] = (k == wpos) ? to_monty((u32)w) : base.s[k];
-    permute_head<16>(s, c);
+    u64 y[16];
+    grind_layer(gp, to_monty((u32)w), y);
+    permute16_head_rounds(y, s, c);
     permute_internal<16>(s, c);
-    permute_tail<16>(s, c);
+    const u32 s7 = permute16_tail_s7(s, c);
     if (t == 0) sb[it ^ 1] = nb;
-    if (w < end && (from_monty(s[7]) & mask) == 0) atomicMin(result, (u32)w);
+    if (w < end && (from_monty(s7) & mask) == 0) atomicMin(result, (u32)w);
     __syncthreads();
   }
 }

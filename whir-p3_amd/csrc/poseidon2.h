@@ -197,15 +197,13 @@ __host__ __device__ __forceinline__ void external_linear16_wide(const u32* x, u6
   for (int i = 0; i < 16; i++) y[i] = n[i] + s[i & 3];
 }
 
-// Initial linear layer and the four initial external rounds.  In: s < p.
-// Out: s[0] < p, s[1..15] < 2p.
-__host__ __device__ __forceinline__ void permute16_head(u32* s, const P2C16& c) {
+// The four initial external rounds from the initial linear layer's output y
+// (each y[i] < 77p).  Out: s[0] < p, s[1..15] < 2p.
+__host__ __device__ __forceinline__ void permute16_head_rounds(u64* y, u32* s, const P2C16& c) {
   const u32 one = opaque(1u);
   const u32 two = opaque(2u);
   const u64 zero = opaque(0ull);
   u32 x[16];
-  u64 y[16];
-  external_linear16_wide(s, y, one, two, zero);
 #pragma unroll
   for (int r = 0; r < 4; r++) {
 #pragma unroll
@@ -215,6 +213,14 @@ __host__ __device__ __forceinline__ void permute16_head(u32* s, const P2C16& c) 
   s[0] = full_reduce(y[0]);
 #pragma unroll
   for (int i = 1; i < 16; i++) s[i] = (u32)lazy_reduce(y[i]);
+}
+
+// Initial linear layer and the four initial external rounds.  In: s < p.
+// Out: s[0] < p, s[1..15] < 2p.
+__host__ __device__ __forceinline__ void permute16_head(u32* s, const P2C16& c) {
+  u64 y[16];
+  external_linear16_wide(s, y, opaque(1u), opaque(2u), opaque(0ull));   // < 35p
+  permute16_head_rounds(y, s, c);
 }
 
 // The twenty internal rounds: S-box on element 0, then 1 + diag(V) with
@@ -275,6 +281,69 @@ __host__ __device__ __forceinline__ void permute16_tail(u32* s, const P2C16& c) 
   }
 #pragma unroll
   for (int i = 0; i < 16; i++) s[i] = full_reduce(y[i]);
+}
+
+// ---- grinding candidates
+// A candidate's input state is a fixed state with the word at position q
+// replaced by the witness w, so the initial linear layer M_E x is
+// M_E x' + w M_E e_q: M_E x' (x' the fixed state with 0 at q) is computed once
+// per search, and column q of M_E has the small entries
+// M4[i mod 4][q mod 4], doubled in q's own quad (M4 = 2 3 1 1 / 1 2 3 1 /
+// 1 1 2 3 / 3 1 1 2).  A candidate's layer is then 16 multiply-adds.
+struct GrindPre {
+  u64 y0[16];    // M_E x', each below 35p
+  u32 col[16];   // column q of M_E, 1..6
+};
+__host__ __device__ inline void grind_pre(GrindPre& g, const u32* base, int q) {
+  u32 x[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) x[i] = i == q ? 0u : base[i];
+  external_linear16_wide(x, g.y0, opaque(1u), opaque(2u), opaque(0ull));
+  constexpr u32 M4[4][4] = {{2, 3, 1, 1}, {1, 2, 3, 1}, {1, 1, 2, 3}, {3, 1, 1, 2}};
+#pragma unroll
+  for (int i = 0; i < 16; i++) g.col[i] = M4[i & 3][q & 3] * ((i >> 2) == (q >> 2) ? 2u : 1u);
+}
+// The initial linear layer's output for witness wm (Montgomery form, < p):
+// below 35p + 6p.
+__host__ __device__ __forceinline__ void grind_layer(const GrindPre& g, u32 wm, u64* y) {
+#pragma unroll
+  for (int i = 0; i < 16; i++) y[i] = mad_u64(wm, g.col[i], g.y0[i]);
+}
+
+// The four terminal external rounds, returning only element 7 of the output
+// (canonical; the sample a grinding candidate is judged by).  The last
+// linear layer's element 7 is n_7 + sum_q n_(4q+3) with n_(4q+3) = 3 x_4q +
+// x_(4q+1) + x_(4q+2) + 2 x_(4q+3), so quad 1 counts twice: 16 multiply-adds
+// (x < 1.75p: below 62p) and one reduction instead of the whole layer and
+// sixteen reductions.  In: s[0] < p, s[1..15] < 2p.
+__host__ __device__ __forceinline__ u32 permute16_tail_s7(u32* s, const P2C16& c) {
+  const u32 one = opaque(1u);
+  const u32 two = opaque(2u);
+  const u64 zero = opaque(0ull);
+  u32 x[16];
+  u64 y[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) y[i] = s[i];
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+#pragma unroll
+    for (int i = 0; i < 16; i++) x[i] = (u32)cube_lazy(full_reduce(y[i] + c.ext_term[r * 16 + i]));
+    external_linear16_wide(x, y, one, two, zero);
+  }
+#pragma unroll
+  for (int i = 0; i < 16; i++) x[i] = (u32)cube_lazy(full_reduce(y[i] + c.ext_term[48 + i]));
+  const u32 w1 = opaque(1u), w2 = opaque(2u), w3 = opaque(3u), w4 = opaque(4u), w6 = opaque(6u);
+  u64 acc[2] = {zero, zero};
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const bool d = q == 1;   // quad 1 counts twice
+    u64& a = acc[q & 1];
+    a = mad_u64(x[4 * q], d ? w6 : w3, a);
+    a = mad_u64(x[4 * q + 1], d ? w2 : w1, a);
+    a = mad_u64(x[4 * q + 2], d ? w2 : w1, a);
+    a = mad_u64(x[4 * q + 3], d ? w4 : w2, a);
+  }
+  return full_reduce(acc[0] + acc[1]);
 }
 
 // The permutation in three phases (initial linear layer and external rounds;
