@@ -44,9 +44,9 @@ VALU_STAGES = tuple(VALU_OPS)
 # Kernels of each stage, for the HBM traffic measured with rocprofv3 --pmc
 # (FETCH_SIZE and WRITE_SIZE per dispatch, calibrated on a copy of known size).
 STAGE_KERNELS = {   # per stage: the kernels one launch group runs, each a list of alternatives
-    "grind": [["k_grind", "k_grind_wide"]],
+    "grind": [["k_grind", "k_grind_wide", "k_grind_lock<512>"]],
     "merkle_leaves": [["k_leaf_hash<16>"]],
-    "lde_base": [["k_lde_pass1<unsigned int, 10>"], ["k_lde_pass2<unsigned int, 10>"]],
+    "lde_base": [["k_lde_pass1<unsigned int, 10>"], ["k_lde_pass2<unsigned int, 10, false, true>"]],
 }
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic_n24.json")
 
@@ -60,6 +60,48 @@ os.environ.setdefault("OMP_PLACES", "cores")
 def progress(msg):
     """A progress line on stderr (the JSON result alone goes to stdout)."""
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def traffic_summary(fetch_csv, write_csv):
+    """Per-kernel HBM bytes per dispatch from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE; KiB per
+    dispatch), scaled by what each counter reads for the 256 MiB calibration copy, into TRAFFIC_FILE."""
+    import csv
+
+    def load(path, counter):
+        per = {}
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] != counter:
+                    continue
+                name = r["Kernel_Name"].split("(")[0].replace("void ", "").replace("kb::", "").strip()
+                per.setdefault(name, []).append(float(r["Counter_Value"]))
+        return per
+
+    fetch, write = load(fetch_csv, "FETCH_SIZE"), load(write_csv, "WRITE_SIZE")
+    cal = "__amd_rocclr_copyBuffer"
+    moved_kib = 256 * 1024
+    f_cal, w_cal = max(fetch[cal]), max(write[cal])   # the calibration copy is the largest copy dispatch
+    ff, wf = moved_kib / f_cal, moved_kib / w_cal
+    kernels = {}
+    for name in fetch:
+        if name == cal or name not in write:
+            continue
+        fk = sum(fetch[name]) / len(fetch[name])
+        wk = sum(write[name]) / len(write[name])
+        kernels[name] = {"dispatches": len(fetch[name]), "fetch_kib_raw": round(fk, 1), "write_kib_raw": round(wk, 1),
+                         "fetch_bytes": round(fk * 1024 * ff), "write_bytes": round(wk * 1024 * wf)}
+    out = {"workload": "WHIR commit+prove n=24 (bench.py --steps 1 --warmup 1 --copy-calibration)",
+           "calibration": {"kernel": cal + ", 256 MiB device to device", "bytes": 256 << 20,
+                           "fetch_kib_read": f_cal, "write_kib_read": w_cal, "fetch_factor": ff, "write_factor": wf},
+           "kernels": dict(sorted(kernels.items(), key=lambda kv: -(kv[1]["fetch_bytes"] + kv[1]["write_bytes"])))}
+    with open(TRAFFIC_FILE, "w") as f:
+        json.dump(out, f, indent=1)
+    print(f"fetch factor {ff:.3f}, write factor {wf:.3f}")
+    print("| kernel | dispatches | FETCH raw KiB | WRITE raw KiB | fetch MiB (scaled) | write MiB (scaled) |")
+    print("|---|---:|---:|---:|---:|---:|")
+    for name, k in out["kernels"].items():
+        print(f"| `{name}` | {k['dispatches']} | {k['fetch_kib_raw']} | {k['write_kib_raw']} | "
+              f"{k['fetch_bytes'] / 2**20:.2f} | {k['write_bytes'] / 2**20:.2f} |")
 
 
 def hbm_roof(stage, name):
@@ -344,6 +386,9 @@ def main():
     ap.add_argument("--cpu-runs", type=int, default=3, help="timed oracle runs after one warm-up (median reported)")
     ap.add_argument("--skip-kernel-bench", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the n-variable commit+prove steps (no verifier, schedule, Keccak, 2^26, batched or "
+                         "serving workloads): the run the committed kernel profiles cover")
     ap.add_argument("--workers", type=int, default=12, help="batching server workers")
     ap.add_argument("--group-max", type=int, default=16, help="largest group a batching server worker forms")
     ap.add_argument("--serving-workers", type=int, default=8)
@@ -351,7 +396,13 @@ def main():
                     help="run only the 2^26 eq-expansion + sumcheck workload and print its JSON")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB device to device (reference dispatch for rocprofv3 --pmc byte counters)")
+    ap.add_argument("--traffic-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write the per-dispatch HBM traffic file from the counter_collection.csv of a FETCH_SIZE "
+                         "and a WRITE_SIZE pass over --copy-calibration runs, print its table, and exit")
     args = ap.parse_args()
+    if args.traffic_summary:
+        traffic_summary(*args.traffic_summary)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
@@ -451,19 +502,22 @@ def main():
                 "bytes_per_launch": round(per_launch), "avg_launch_ms": round(s["ms"] / s["launches"], 4)}
 
     dominant = max(stages, key=lambda k: stages[k]["ms"])
-    verifier = verifier_bench(W, cfg, proof, pts, evals0)
-    schedule = schedule_bench(W, n, d_poly.ptr, pts, ms)
-    keccak = keccak_bench(W, n, d_poly.ptr, pts, ms)
-    progress(f"keccak variant: {keccak['prove_ms']} ms per proof")
+    verifier = schedule = keccak = None
+    if not args.headline_only:
+        verifier = verifier_bench(W, cfg, proof, pts, evals0)
+        schedule = schedule_bench(W, n, d_poly.ptr, pts, ms)
+        keccak = keccak_bench(W, n, d_poly.ptr, pts, ms)
+        progress(f"keccak variant: {keccak['prove_ms']} ms per proof")
 
     progress(f"n={n}: {ms:.3f} ms per proof; profiled steps done")
-    kernels = None if args.skip_kernel_bench else sumcheck_eq_bench(W, args)
+    kernels = None if args.skip_kernel_bench or args.headline_only else sumcheck_eq_bench(W, args)
     if kernels:
         progress("2^26 sumcheck/eq workload done")
-    batched = None if args.skip_batch else batched_bench(W, workers=args.workers, group_max=args.group_max)
+    batched = None if args.skip_batch or args.headline_only else batched_bench(W, workers=args.workers,
+                                                                                group_max=args.group_max)
     if batched:
         progress(f"batched: {batched['speedup']}x")
-    serving = None if args.skip_batch else serving_bench(W, workers=args.serving_workers)
+    serving = None if args.skip_batch or args.headline_only else serving_bench(W, workers=args.serving_workers)
     if serving:
         progress(f"serving: p99 {serving['p99_ms']} ms")
 
