@@ -661,7 +661,6 @@ static int lde_impl(const LdeArgs<T>* io, int count, int m, int k, int R, int LP
   if (rc) return rc;
   if (count < 1 || count > GROUP_MAX) return -8;
   const int l = m - k;
-  const int W = 1 << k;
   const unsigned np = (unsigned)count;
   if (!pl.two_pass) {
     auto kern = pass2_for<T, true>(l, k);

@@ -732,7 +732,6 @@ int grind_range(const P2C16* d_consts, const GrindArgs* a, int count, int bits, 
 // writes the point and the next challenger state.
 __global__ void __launch_bounds__(256) k_ood_eq_tables(Group<OodArgs> g, int m, int hi, int lo,
                                                        const P2C16* __restrict__ cp) {
-  __shared__ u32 sp[16], in[8], out[8];
   const OodArgs& a = g.v[blockIdx.z];
   const StepTr& st = a.st;
   ef* log_y = a.log_y;
@@ -744,7 +743,7 @@ __global__ void __launch_bounds__(256) k_ood_eq_tables(Group<OodArgs> g, int m, 
   const int t = threadIdx.x;
   const P2C16& c = *cp;
   const WideLane wl = wide_lane(c, t & 15);
-  BlockChallenger ch{sp, in, out, 0, 0, &wl, &c};
+  BlockChallenger ch{0, 0, 0, 0, 0, &wl, &c};
   ch.load(*st.ch_in);
   __shared__ u32 stage[OBS_STAGE];
   ch.observe_words(st.obs, st.n_obs, stage);
@@ -805,7 +804,6 @@ int ood_eq_tables(const OodArgs* a, int count, int m, int hi, int lo, const P2C1
 __global__ void __launch_bounds__(256) k_round_queries(Group<QueryArgs> g, int pow_bits, u64 domain_size, int ff,
                                                        int num_queries, u32 folded_gen, int n_ood,
                                                        const P2C16* __restrict__ cp, int final_step) {
-  __shared__ u32 sp[16], in[8], out[8];
   const QueryArgs& a = g.v[blockIdx.z];
   const StepTr& st = a.st;
   const u32* grind = a.grind;
@@ -816,7 +814,7 @@ __global__ void __launch_bounds__(256) k_round_queries(Group<QueryArgs> g, int p
   const int t = threadIdx.x;
   const P2C16& c = *cp;
   const WideLane wl = wide_lane(c, t & 15);
-  BlockChallenger ch{sp, in, out, 0, 0, &wl, &c};
+  BlockChallenger ch{0, 0, 0, 0, 0, &wl, &c};
   ch.load(*st.ch_in);
   u32 err = 0, wm = 0;
   if (pow_bits > 0) {

@@ -100,7 +100,7 @@ int eq_outer(const ef* Ehi, const ef* Elo, int lo, u64 n, ef* out, hipStream_t s
 // partial[(seg * gridDim.x + block) * K + k] receives the block's sum.
 template <typename T, int K>
 __global__ void __launch_bounds__(256) k_eq_dot(Group<DotArgs> g, int nf, int sb, int lo) {
-  __shared__ ef sh[256 * K];
+  __shared__ ef sh[16 * K];
   const DotArgs& a = g.v[blockIdx.z];
   const T* __restrict__ f = static_cast<const T*>(a.f);
   const EqTables& tabs = a.tabs;
@@ -152,7 +152,7 @@ __device__ __forceinline__ u64 fold32(u64 t) {
 // reduced once at the end.
 template <int K>
 __global__ void __launch_bounds__(256) k_eq_dot4(Group<DotArgs> g, int nf, int sb, int lo) {
-  __shared__ ef sh[256 * K];
+  __shared__ ef sh[16 * K];
   const DotArgs& a = g.v[blockIdx.z];
   const u32* __restrict__ f = static_cast<const u32*>(a.f);
   const EqTables& tabs = a.tabs;
@@ -265,7 +265,7 @@ struct RedArgs {
   ef* out;
 };
 __global__ void __launch_bounds__(256) k_reduce_strided(Group<RedArgs> g, int nb, int K) {
-  __shared__ ef sh[256];
+  __shared__ ef sh[16];
   const ef* __restrict__ partial = g.v[blockIdx.z].partial;
   ef* __restrict__ out = g.v[blockIdx.z].out;
   const int seg = blockIdx.x / K, k = blockIdx.x % K;
@@ -548,7 +548,7 @@ int base_to_ext(const ExtArgs* a, int count, u64 n, hipStream_t st) {
 // ---------------------------------------------------------------- sumcheck rounds
 // (h(0), h(2)) partial sums over pairs (i, i + h) of e and w (length 2h).
 __global__ void __launch_bounds__(256) k_sc_coeffs(Group<ScArgs> g, u64 h) {
-  __shared__ ef sh[512];
+  __shared__ ef sh[32];
   const ScArgs& a = g.v[blockIdx.z];
   const ef* __restrict__ e = a.e;
   const ef* __restrict__ w = a.w;
@@ -569,13 +569,23 @@ __global__ void __launch_bounds__(256) k_sc_coeffs(Group<ScArgs> g, u64 h) {
 
 // Fold e and w (length 2h) with r into their first halves, and accumulate the
 // next round's (h(0), h(2)) over the folded halves (pairs (j, j + h/2)).
-// With a transcript mode, every block first runs that transcript step to get r.
+// With a transcript mode, every block first runs that transcript step to get
+// r; a thread's first eight entries are loaded before it (they do not depend
+// on r), so their latency overlaps the step's own loads and permutation.
 __global__ void __launch_bounds__(256) k_sc_fold_coeffs(Group<ScArgs> g, u64 h, int tr_mode,
                                                         const P2C16* __restrict__ cp) {
-  __shared__ ef sh[512];
+  __shared__ ef sh[32];
   const ScArgs& a = g.v[blockIdx.z];
   ef* __restrict__ e = a.e;
   ef* __restrict__ w = a.w;
+  const u64 q = h >> 1;
+  const u64 j0 = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  const u64 stride = (u64)gridDim.x * blockDim.x;
+  ef a0, a1, b0, b1, c0, c1, d0, d1;
+  if (j0 < q) {
+    a0 = e[j0], a1 = e[j0 + h], b0 = e[j0 + q], b1 = e[j0 + q + h];
+    c0 = w[j0], c1 = w[j0 + h], d0 = w[j0 + q], d1 = w[j0 + q + h];
+  }
   ef r = a.r;
   if (tr_mode != TR_NONE) {
     __shared__ ef rs;
@@ -583,14 +593,15 @@ __global__ void __launch_bounds__(256) k_sc_fold_coeffs(Group<ScArgs> g, u64 h, 
     r = rs;
   }
   ef acc[2] = {ef_zero(), ef_zero()};
-  const u64 q = h >> 1;
-  for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < q; j += (u64)gridDim.x * blockDim.x) {
-    ef a0 = e[j], a1 = e[j + h], b0 = e[j + q], b1 = e[j + q + h];
-    ef f0 = ef_add(a0, ef_mul(r, ef_sub(a1, a0)));
-    ef f1 = ef_add(b0, ef_mul(r, ef_sub(b1, b0)));
-    ef c0 = w[j], c1 = w[j + h], d0 = w[j + q], d1 = w[j + q + h];
-    ef g0 = ef_add(c0, ef_mul(r, ef_sub(c1, c0)));
-    ef g1 = ef_add(d0, ef_mul(r, ef_sub(d1, d0)));
+  for (u64 j = j0; j < q; j += stride) {
+    if (j != j0) {
+      a0 = e[j], a1 = e[j + h], b0 = e[j + q], b1 = e[j + q + h];
+      c0 = w[j], c1 = w[j + h], d0 = w[j + q], d1 = w[j + q + h];
+    }
+    const ef f0 = ef_add(a0, ef_mul(r, ef_sub(a1, a0)));
+    const ef f1 = ef_add(b0, ef_mul(r, ef_sub(b1, b0)));
+    const ef g0 = ef_add(c0, ef_mul(r, ef_sub(c1, c0)));
+    const ef g1 = ef_add(d0, ef_mul(r, ef_sub(d1, d0)));
     e[j] = f0;
     e[j + q] = f1;
     w[j] = g0;
@@ -609,17 +620,81 @@ __global__ void __launch_bounds__(256) k_sc_fold(Group<ScArgs> g, u64 h, int tr_
   const ScArgs& a = g.v[blockIdx.z];
   ef* __restrict__ e = a.e;
   ef* __restrict__ w = a.w;
+  const u64 j0 = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  const u64 stride = (u64)gridDim.x * blockDim.x;
+  ef a0, a1, c0, c1;
+  if (j0 < h) a0 = e[j0], a1 = e[j0 + h], c0 = w[j0], c1 = w[j0 + h];
   ef r = a.r;
   if (tr_mode != TR_NONE) {
     __shared__ ef rs;
     tr_prologue(tr_mode, a.tr, *cp, nullptr, &rs);
     r = rs;
   }
-  for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < h; j += (u64)gridDim.x * blockDim.x) {
-    ef a0 = e[j], a1 = e[j + h];
+  for (u64 j = j0; j < h; j += stride) {
+    if (j != j0) a0 = e[j], a1 = e[j + h], c0 = w[j], c1 = w[j + h];
     e[j] = ef_add(a0, ef_mul(r, ef_sub(a1, a0)));
-    ef c0 = w[j], c1 = w[j + h];
     w[j] = ef_add(c0, ef_mul(r, ef_sub(c1, c0)));
+  }
+}
+
+// Every round of a sumcheck phase without proof of work, on arrays of at most
+// 2^SC_SMALL_LOG entries, in one block per proof: e and w are held in LDS, and
+// each round's (h(0), h(2)), transcript step (observe both, sample r) and fold
+// are separated by block barriers instead of launches.  The log entries, the
+// challenger state left in tr.ch_out and the folded arrays written back are
+// those of the per-round kernels (k_sc_coeffs, then k_sc_fold_coeffs in mode
+// TR_COEFFS_SAMPLE).  Grid: (1, 1, proofs).
+__global__ void __launch_bounds__(256) k_sc_small(Group<ScArgs> g, int m0, int rounds, const P2C16* __restrict__ cp) {
+  __shared__ ef se[1 << SC_SMALL_LOG], sw[1 << SC_SMALL_LOG];
+  __shared__ ef sh[32];
+  const ScArgs& a = g.v[blockIdx.z];
+  const int t = threadIdx.x;
+  const int n = 1 << m0;
+  const WideLane wl = wide_lane(*cp, t & 15);
+  BlockChallenger ch{0, 0, 0, 0, 0, &wl, cp};
+  ch.load(*a.tr.ch_in);
+  for (int i = t; i < n; i += 256) {
+    se[i] = a.e[i];
+    sw[i] = a.w[i];
+  }
+  __syncthreads();
+  for (int i = 0; i < rounds; i++) {
+    const int h = n >> (i + 1);
+    ef acc[2] = {ef_zero(), ef_zero()};
+    for (int j = t; j < h; j += 256) {
+      const ef e0 = se[j], e1 = se[j + h], w0 = sw[j], w1 = sw[j + h];
+      acc[0] = ef_add(acc[0], ef_mul(w0, e0));
+      acc[1] = ef_add(acc[1], ef_mul(ef_sub(ef_dbl(w1), w0), ef_sub(ef_dbl(e1), e0)));
+    }
+    block_reduce_ef<2>(acc, sh);   // the same sums in every thread
+    const u32 v[8] = {acc[0].c[0], acc[0].c[1], acc[0].c[2], acc[0].c[3],
+                      acc[1].c[0], acc[1].c[1], acc[1].c[2], acc[1].c[3]};
+    ch.observe8(v);
+    const ef r = ch.sample_ef();
+    if (t == 0) {
+      ScLog* L = a.tr.log + i;
+      L->c0 = acc[0];
+      L->c2 = acc[1];
+      L->r = r;
+      L->witness = 0;
+      L->err = 0;
+    }
+    // a thread folds the pairs it summed (j < h reads j + h, written by none)
+    for (int j = t; j < h; j += 256) {
+      const ef e0 = se[j], w0 = sw[j];
+      se[j] = ef_add(e0, ef_mul(r, ef_sub(se[j + h], e0)));
+      sw[j] = ef_add(w0, ef_mul(r, ef_sub(sw[j + h], w0)));
+    }
+    __syncthreads();   // folded entries before the next round's reads; sh reads before its next writes
+  }
+  for (int i = t; i < (n >> rounds); i += 256) {
+    a.e[i] = se[i];
+    a.w[i] = sw[i];
+  }
+  ch.store(a.tr.ch_out);
+  if (t == 0) {
+    if (a.grind_reset) *a.grind_reset = P;
+    if (a.tr.grind_next) *a.tr.grind_next = P;
   }
 }
 
@@ -678,6 +753,13 @@ int sc_fold_tr(const ScArgs* a, int count, u64 h, int mode, const P2C16* d_const
       hipLaunchKernelGGL(k_sc_fold_coeffs, dim3(b, 1, count), dim3(256), 0, st, g, h, mode, d_consts);
     else
       hipLaunchKernelGGL(k_sc_fold, dim3(sc_blocks(h), 1, count), dim3(256), 0, st, g, h, mode, d_consts);
+  });
+}
+
+int sc_small(const ScArgs* a, int count, int m0, int rounds, const P2C16* d_consts, hipStream_t st) {
+  if (m0 < 1 || m0 > SC_SMALL_LOG || rounds < 1 || rounds > m0) return -1;
+  return launch_group(a, count, [&](const Group<ScArgs>& g) {
+    hipLaunchKernelGGL(k_sc_small, dim3(1, 1, count), dim3(256), 0, st, g, m0, rounds, d_consts);
   });
 }
 

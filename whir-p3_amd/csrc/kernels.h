@@ -328,6 +328,11 @@ int sc_coeffs(const ScArgs* a, int count, u64 h, hipStream_t st);
 int sc_fold(const ScArgs* a, int count, u64 h, hipStream_t st);
 int sc_partials(const ScArgs* a, int count, u64 h, unsigned* nb, hipStream_t st);
 int sc_fold_tr(const ScArgs* a, int count, u64 h, int mode, const P2C16* d_consts, unsigned* nb, hipStream_t st);
+// Every round of a phase without proof of work on 2^m0 <= 2^SC_SMALL_LOG
+// entries in one block per proof (tr: ch_in, ch_out, log of `rounds` entries,
+// grind_next; grind_reset as in sc_partials).
+constexpr int SC_SMALL_LOG = 10;
+int sc_small(const ScArgs* a, int count, int m0, int rounds, const P2C16* d_consts, hipStream_t st);
 
 // Kernel arguments stay within 4 KiB for a full group.
 static_assert(sizeof(Group<GrindArgs>) <= 4096 && sizeof(Group<ScArgs>) <= 4096 &&

@@ -869,6 +869,29 @@ ScLaunch sumcheck_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int rounds,
   auto part = [&](int b, int k) { return R[b]->sl->d_partial + (k ? 4096 : 0); };
   auto gres = [&](int b, int k) { return R[b]->sl->d_grind + (k ? 64 : 0); };
   std::vector<ScArgs> a(B);
+  if (pow_bits == 0 && m0 <= SC_SMALL_LOG) {
+    // the whole phase in one block per proof
+    for (int b = 0; b < B; b++) {
+      if (R[b]->s.m != m0) bad("sumcheck sizes of one group differ", WHIRGPU_BAD_STATE);
+      a[b] = ScArgs{};
+      a[b].e = R[b]->s.e.p;
+      a[b].w = R[b]->s.w.p;
+      a[b].grind_reset = gres(b, 0);
+      ScTr& tr = a[b].tr;
+      tr.ch_in = cin[b];
+      tr.ch_out = next_state(b);
+      tr.log = R[b]->sl->d_sclog + log_base;
+      tr.grind_next = gres(b, rounds & 1);
+      cin[b] = tr.ch_out;
+    }
+    {
+      Span sp(c, ST_SUMCHECK, 64.0 * (double)(1ull << m0) * B);
+      KCHK(sc_small(a.data(), B, m0, rounds, c->d_c16, c->st));
+    }
+    for (Run* r : R) r->s.m -= rounds;
+    L.out = cin;
+    return L;
+  }
   int pp = 0;
   unsigned nb = 0;
   {

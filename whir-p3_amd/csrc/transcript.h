@@ -28,79 +28,90 @@
 
 namespace kb {
 
-// The challenger held in LDS by a whole block: every thread runs the same
-// bookkeeping, and every wave computes the lane-parallel permutation (lane
-// j & 15 holds element j & 15), so block barriers order the LDS updates.
+// The challenger held in the lanes of every wave of a block: lane j = t & 15
+// of each 16-lane row holds sponge element j, input buffer element j and
+// output buffer element j (j < 8), and every thread keeps the same buffer
+// lengths.  Observed values are block-uniform (every thread passes the same
+// value), so an observation is a register write in the lane it lands in, and
+// every wave computes the lane-parallel permutation itself: no LDS and no
+// barrier in the challenger.
 struct BlockChallenger {
-  u32* sp;    // [16] LDS
-  u32* in;    // [8]  LDS
-  u32* out;   // [8]  LDS
+  u32 sp;    // sponge element t & 15
+  u32 in;    // input buffer element t & 15 (t & 15 < 8)
+  u32 out;   // output buffer element t & 15 (t & 15 < 8)
   int in_len;
   int out_len;
   const WideLane* wl;
   const P2C16* c;
-  // Output element t & 15 (t & 15 < 8) in this lane's register, so a sample
-  // is a lane read instead of an LDS round trip.
-  u32 lane_out;
 
   __device__ void load(const DevChallenger& s) {
-    const int t = threadIdx.x;
-    if (t < 16) sp[t] = s.sponge[t];
-    if (t < 8) {
-      in[t] = s.in[t];
-      out[t] = s.out[t];
-    }
+    const int j = threadIdx.x & 15;
+    sp = s.sponge[j];
+    in = s.in[j & 7];
+    out = s.out[j & 7];
     in_len = (int)s.in_len;
     out_len = (int)s.out_len;
-    lane_out = (t & 15) < 8 ? s.out[t & 15] : 0u;
-    __syncthreads();
   }
   __device__ void store(DevChallenger* s) const {   // block 0 only
     const int t = threadIdx.x;
-    if (t < 16) s->sponge[t] = sp[t];
+    if (t < 16) s->sponge[t] = sp;
     if (t < 8) {
-      s->in[t] = in[t];
-      s->out[t] = out[t];
+      s->in[t] = in;
+      s->out[t] = out;
     }
     if (t == 0) {
       s->in_len = (u32)in_len;
       s->out_len = (u32)out_len;
     }
   }
+  // Sponge element t & 15 with the buffered inputs written over the first
+  // in_len elements (what a duplex would permute).
+  __device__ u32 absorbed() const { return (int)(threadIdx.x & 15) < in_len ? in : sp; }
   __device__ void duplex() {
-    const int t = threadIdx.x;
-    if (t < in_len) sp[t] = in[t];
-    __syncthreads();
-    u32 x = sp[t & 15];
-    x = permute16_wide(x, t & 15, *wl, *c);
-    __syncthreads();
-    if (t < 16) sp[t] = x;
-    if (t < 8) out[t] = x;
-    lane_out = x;
-    __syncthreads();
+    const u32 x = permute16_wide(absorbed(), threadIdx.x & 15, *wl, *c);
+    sp = x;
+    out = x;
     in_len = 0;
     out_len = 8;
   }
   __device__ void observe(u32 v) {
     out_len = 0;
-    if (threadIdx.x == 0) in[in_len] = v;
+    if ((int)(threadIdx.x & 15) == in_len) in = v;
     in_len++;
-    __syncthreads();
     if (in_len == 8) duplex();
   }
+  // Eight block-uniform words, as eight observe() calls: the first 8 - in_len
+  // fill the input buffer, which then duplexes exactly once, and the other
+  // in_len start the next input buffer.
+  __device__ void observe8(const u32 (&v)[8]) {
+    const int j = threadIdx.x & 15, n0 = in_len;
+    u32 v1 = 0, v2 = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      if (j - n0 == i) v1 = v[i];
+      if (j + 8 - n0 == i) v2 = v[i];
+    }
+    if (j >= n0 && j < 8) in = v1;
+    in_len = 8;
+    duplex();
+    if (j < n0) in = v2;
+    in_len = n0;
+    out_len = n0 > 0 ? 0 : 8;
+  }
   // Observes n words from memory (n <= OBS_STAGE): every word is loaded at
-  // once into `stage` (LDS), then the input buffer is filled up to 8 words
-  // per barrier, duplexing whenever it is full, as n observe() calls would.
-  // One load latency instead of n serialized ones.
+  // once into `stage` (LDS, one barrier), then each lane takes the words that
+  // land in its input position, duplexing whenever the buffer is full, as n
+  // observe() calls would.  One load latency instead of n serialized ones.
+  // A second call with the same stage needs a barrier in between.
   __device__ void observe_words(const u32* w, int n, u32* stage) {
     if (n <= 0) return;
     for (int i = threadIdx.x; i < n; i += blockDim.x) stage[i] = w[i];
     __syncthreads();
     out_len = 0;
+    const int j = threadIdx.x & 15;
     for (int i = 0; i < n;) {
       const int k = n - i < 8 - in_len ? n - i : 8 - in_len;
-      if ((int)threadIdx.x < k) in[in_len + threadIdx.x] = stage[i + threadIdx.x];
-      __syncthreads();
+      if (j >= in_len && j < in_len + k) in = stage[i + j - in_len];
       in_len += k;
       i += k;
       if (in_len == 8) duplex();
@@ -109,8 +120,8 @@ struct BlockChallenger {
   __device__ u32 sample() {
     if (in_len > 0 || out_len == 0) duplex();
     out_len--;
-    // lane out_len of every wave holds element out_len (lanes 0..15 hold 0..15)
-    return (u32)__builtin_amdgcn_readlane((int)lane_out, out_len);
+    // lane out_len of every wave holds output element out_len
+    return (u32)__builtin_amdgcn_readlane((int)out, out_len);
   }
   // Four samples as an extension element.  The loop is not unrolled (each
   // sample may duplex), so the samples shift through registers instead of
@@ -132,13 +143,14 @@ struct BlockChallenger {
 
 // Runs one transcript step with the whole block.  TR_COEFFS_GRIND leaves the
 // candidates' base state in gb (LDS); the sampling modes leave r in *r (LDS).
-// Requires blockDim.x == 256.
+// Every global load of the step (constants, challenger, partial sums or
+// witness) is issued before the first value is used, so they share one
+// memory latency.  Requires blockDim.x == 256.
 __device__ __forceinline__ void tr_prologue(int mode, const ScTr& a, const P2C16& c, GrindBase* gb, ef* r) {
-  __shared__ u32 sp[16], in[8], out[8];
-  __shared__ ef red[2 * 256];
+  __shared__ ef red[2 * 16];
   const int t = threadIdx.x;
   const WideLane wl = wide_lane(c, t & 15);
-  BlockChallenger ch{sp, in, out, 0, 0, &wl, &c};
+  BlockChallenger ch{0, 0, 0, 0, 0, &wl, &c};
   ch.load(*a.ch_in);
   const bool lead = blockIdx.x == 0 && blockIdx.y == 0;
   u32 err = 0;
@@ -146,7 +158,7 @@ __device__ __forceinline__ void tr_prologue(int mode, const ScTr& a, const P2C16
     __shared__ u32 stage[OBS_STAGE];
     ch.observe_words(a.obs, a.n_obs, stage);
     if (lead && t == 0 && a.n_obs == 4 && a.log_obs) *a.log_obs = ef{{a.obs[0], a.obs[1], a.obs[2], a.obs[3]}};
-    if (t < 16) gb->s[t] = t < ch.in_len ? in[t] : sp[t];
+    if (t < 16) gb->s[t] = ch.absorbed();
     if (t == 0) gb->wpos = (u32)ch.in_len;
     __syncthreads();
     if (lead) ch.store(a.ch_out);
@@ -158,18 +170,18 @@ __device__ __forceinline__ void tr_prologue(int mode, const ScTr& a, const P2C16
       acc[0] = ef_add(acc[0], a.partial[2 * b]);
       acc[1] = ef_add(acc[1], a.partial[2 * b + 1]);
     }
-    block_reduce_ef<2>(acc, red);   // the sums stay in red[0] and red[256]
+    block_reduce_ef<2>(acc, red);   // the same sums in every thread
     if (lead && t == 0) {
       a.log->c0 = acc[0];
       a.log->c2 = acc[1];
       a.log->witness = 0;
     }
-    // observed from LDS: a loop index into the registers of acc would put
-    // acc in memory (see sample_ef)
-    for (int k = 0; k < 8; k++) ch.observe(red[(k >> 2) * 256].c[k & 3]);
+    const u32 v[8] = {acc[0].c[0], acc[0].c[1], acc[0].c[2], acc[0].c[3],
+                      acc[1].c[0], acc[1].c[1], acc[1].c[2], acc[1].c[3]};
+    ch.observe8(v);
     if (mode == TR_COEFFS_GRIND) {
       // a candidate w is observed into position in_len of this state
-      if (t < 16) gb->s[t] = t < ch.in_len ? in[t] : sp[t];
+      if (t < 16) gb->s[t] = ch.absorbed();
       if (t == 0) gb->wpos = (u32)ch.in_len;
       __syncthreads();
       if (lead) ch.store(a.ch_out);
