@@ -858,6 +858,15 @@ ScLaunch sumcheck_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int rounds,
     bad("too many sumcheck rounds in one phase", WHIRGPU_BAD_STATE);
   const int m0 = R[0]->s.m;
   if (m0 < rounds) bad("sumcheck round on a constant polynomial", WHIRGPU_BAD_STATE);
+  if (pow_bits == 0 && m0 > SC_SMALL_LOG && m0 - rounds < SC_SMALL_LOG) {
+    // a launch per round down to 2^SC_SMALL_LOG entries, the rest in one block
+    const int k1 = m0 - SC_SMALL_LOG;
+    const ScLaunch first = sumcheck_launch(c, R, k1, 0, cin, log_base);
+    ScLaunch rest = sumcheck_launch(c, R, rounds - k1, 0, first.out, log_base + k1);
+    rest.rounds = rounds;
+    rest.log_base = log_base;
+    return rest;
+  }
   // Each launch reads the challenger from cin and writes the next state to
   // the other buffer; coefficient partials and grind results alternate too,
   // since every block of a launch reads what block 0 of the same launch
@@ -2984,6 +2993,19 @@ int whirgpu_sumcheck_eq(whirgpu_ctx* c, const uint32_t* d_f, uint32_t n, const u
       return x;
     };
     initial_sumcheck_scripted(c, r, (int)l0, next);
+    if (!scripted && r.s.m > 0) {
+      // the remaining rounds on the device transcript (one synchronization),
+      // replayed on the host challenger
+      r.ch = ch;
+      whir::SumcheckData sd;
+      device_sumcheck(c, {&r}, r.s.m, 0, [&](Run&) -> whir::SumcheckData& { return sd; });
+      for (const auto& pe : sd.polynomial_evaluations) {
+        memcpy(out_polys + 8 * round, pe[0].c, 16);
+        memcpy(out_polys + 8 * round + 4, pe[1].c, 16);
+        round++;
+      }
+      ch = r.ch;
+    }
     while (r.s.m > 0) sumcheck_round(c, sl, r.s, next);
     ef fin[2];
     c->d2h(&fin[0], r.s.e.p, 1);
