@@ -518,6 +518,7 @@ __device__ __forceinline__ void phase_fence(u32* s) {
 
 // With tr_mode == TR_COEFFS_GRIND every block first runs the transcript step
 // that observes the round's (h(0), h(2)); its state is the candidates' base.
+// With TR_GBASE that state was written to tr.gbase by an earlier kernel.
 // Grid: (blocks, 1, proofs), one search per proof.
 __global__ void __launch_bounds__(256) k_grind(const P2C16* __restrict__ cp, Group<GrindArgs> ga, int tr_mode,
                                                u32 mask, u32 start, u32 end) {
@@ -535,6 +536,10 @@ __global__ void __launch_bounds__(256) k_grind(const P2C16* __restrict__ cp, Gro
 #pragma unroll
     for (int k = 0; k < 16; k++) base.s[k] = gbs.s[k];
     wpos = (int)gbs.wpos;
+  } else if (tr_mode == TR_GBASE) {
+#pragma unroll
+    for (int k = 0; k < 16; k++) base.s[k] = tr.gbase->s[k];
+    wpos = (int)tr.gbase->wpos;
   }
   GrindPre gp;
   grind_pre(gp, base.s, wpos);
@@ -589,6 +594,10 @@ __global__ void __launch_bounds__(256) k_grind_wide(const P2C16* __restrict__ cp
 #pragma unroll
     for (int k = 0; k < 16; k++) base.s[k] = gbs.s[k];
     wpos = (int)gbs.wpos;
+  } else if (tr_mode == TR_GBASE) {
+#pragma unroll
+    for (int k = 0; k < 16; k++) base.s[k] = tr.gbase->s[k];
+    wpos = (int)tr.gbase->wpos;
   }
   const P2C16& c = *cp;
   const int j = threadIdx.x & 15;
@@ -708,11 +717,13 @@ int grind_range(const P2C16* d_consts, const GrindArgs* a, int count, int bits, 
   }
   bool gbase = mode != TR_NONE;
   for (int i = 0; i < count; i++) gbase = gbase && a[i].tr.gbase;
+  if (mode == TR_GBASE && !gbase) return -1;
   if (bits >= t.grind_lock_bits && (mode == TR_NONE || gbase)) {
     const unsigned nb = (unsigned)std::max<u64>(1, std::min<u64>((end - start + LOCK_NT - 1) / LOCK_NT,
                                                                  (u64)std::max(blocks, 1)));
     return launch_group(a, count, [&](const Group<GrindArgs>& g) {
-      if (gbase) hipLaunchKernelGGL(k_grind_base, dim3(1, 1, count), dim3(256), 0, st, d_consts, mode, g);
+      if (gbase && mode != TR_GBASE)
+        hipLaunchKernelGGL(k_grind_base, dim3(1, 1, count), dim3(256), 0, st, d_consts, mode, g);
       hipLaunchKernelGGL(k_grind_lock<LOCK_NT>, dim3(nb, 1, count), dim3(LOCK_NT), 0, st, d_consts, g,
                          gbase ? 1 : 0, mask, (u32)start, (u32)end);
     });

@@ -698,6 +698,77 @@ __global__ void __launch_bounds__(256) k_sc_small(Group<ScArgs> g, int m0, int r
   }
 }
 
+// One round with proof of work on small arrays in one block per proof (what
+// the fold kernel of the previous round and the transcript prologue of this
+// round's grinding kernel do, without their launches): e and w in LDS.
+// Grid: (1, 1, proofs).
+__global__ void __launch_bounds__(256) k_sc_step(Group<ScArgs> g, int m, int fold_first, const P2C16* __restrict__ cp) {
+  __shared__ ef se[1 << SC_SMALL_LOG], sw[1 << SC_SMALL_LOG];
+  __shared__ ef sh[32];
+  const ScArgs& a = g.v[blockIdx.z];
+  const ScTr& tr = a.tr;
+  const int t = threadIdx.x;
+  int n = 1 << m;
+  const WideLane wl = wide_lane(*cp, t & 15);
+  BlockChallenger ch{0, 0, 0, 0, 0, &wl, cp};
+  ch.load(*tr.ch_in);
+  const u32 w = fold_first ? *tr.grind : 0u;
+  for (int i = t; i < n; i += 256) {
+    se[i] = a.e[i];
+    sw[i] = a.w[i];
+  }
+  __syncthreads();
+  if (fold_first) {
+    u32 err = 0;
+    const u32 wm = w < P ? to_monty(w) : 0u;
+    if (w >= P) err = 1;
+    ch.observe(wm);
+    const u32 mask = tr.pow_bits >= 32 ? 0xffffffffu : ((1u << tr.pow_bits) - 1);
+    if ((from_monty(ch.sample()) & mask) != 0) err = 1;
+    const ef r = ch.sample_ef();
+    if (t == 0) {
+      ScLog* L = tr.log - 1;
+      L->witness = wm;
+      L->r = r;
+      L->err = err;
+    }
+    n >>= 1;
+    for (int j = t; j < n; j += 256) {
+      const ef e0 = se[j], w0 = sw[j];
+      const ef e1 = ef_add(e0, ef_mul(r, ef_sub(se[j + n], e0)));
+      const ef w1 = ef_add(w0, ef_mul(r, ef_sub(sw[j + n], w0)));
+      se[j] = e1;
+      sw[j] = w1;
+      a.e[j] = e1;
+      a.w[j] = w1;
+    }
+    __syncthreads();
+  }
+  const int h = n >> 1;
+  ef acc[2] = {ef_zero(), ef_zero()};
+  for (int j = t; j < h; j += 256) {
+    const ef e0 = se[j], e1 = se[j + h], w0 = sw[j], w1 = sw[j + h];
+    acc[0] = ef_add(acc[0], ef_mul(w0, e0));
+    acc[1] = ef_add(acc[1], ef_mul(ef_sub(ef_dbl(w1), w0), ef_sub(ef_dbl(e1), e0)));
+  }
+  block_reduce_ef<2>(acc, sh);   // the same sums in every thread
+  if (t == 0) {
+    tr.log->c0 = acc[0];
+    tr.log->c2 = acc[1];
+    tr.log->witness = 0;
+  }
+  const u32 v[8] = {acc[0].c[0], acc[0].c[1], acc[0].c[2], acc[0].c[3],
+                    acc[1].c[0], acc[1].c[1], acc[1].c[2], acc[1].c[3]};
+  ch.observe8(v);
+  // a candidate is observed into position in_len of this state
+  if (t < 16) tr.gbase->s[t] = ch.absorbed();
+  if (t == 0) {
+    tr.gbase->wpos = (u32)ch.in_len;
+    if (tr.grind_next) *tr.grind_next = P;
+  }
+  ch.store(tr.ch_out);
+}
+
 static unsigned sc_blocks(u64 work) {
   u64 b = (work + 255) / 256;
   if (b > 1024) b = 1024;
@@ -760,6 +831,13 @@ int sc_small(const ScArgs* a, int count, int m0, int rounds, const P2C16* d_cons
   if (m0 < 1 || m0 > SC_SMALL_LOG || rounds < 1 || rounds > m0) return -1;
   return launch_group(a, count, [&](const Group<ScArgs>& g) {
     hipLaunchKernelGGL(k_sc_small, dim3(1, 1, count), dim3(256), 0, st, g, m0, rounds, d_consts);
+  });
+}
+
+int sc_step(const ScArgs* a, int count, int m, int fold_first, const P2C16* d_consts, hipStream_t st) {
+  if (m < 1 || m > SC_SMALL_LOG || (fold_first && m < 2)) return -1;
+  return launch_group(a, count, [&](const Group<ScArgs>& g) {
+    hipLaunchKernelGGL(k_sc_step, dim3(1, 1, count), dim3(256), 0, st, g, m, fold_first, d_consts);
   });
 }
 

@@ -46,6 +46,7 @@ enum ScTrMode {
   TR_WITNESS_SAMPLE = 2,  // observe the witness, check its bits, sample r
   TR_COEFFS_SAMPLE = 3,   // observe (h(0), h(2)) and sample r (no proof of work)
   TR_OBS_GRIND = 4,       // observe obs; the grind state for the candidates
+  TR_GBASE = 5,           // grinding only: the candidates' base state is in tr.gbase already
 };
 
 // Everything a transcript prologue reads and writes (a kernel argument).
@@ -333,6 +334,13 @@ int sc_fold_tr(const ScArgs* a, int count, u64 h, int mode, const P2C16* d_const
 // grind_next; grind_reset as in sc_partials).
 constexpr int SC_SMALL_LOG = 10;
 int sc_small(const ScArgs* a, int count, int m0, int rounds, const P2C16* d_consts, hipStream_t st);
+// A round with proof of work on arrays of 2^m <= 2^SC_SMALL_LOG entries, in
+// one block per proof: with fold_first, the previous round's transcript step
+// (observe the witness in *tr.grind, check its bits, sample r; logged in
+// tr.log[-1]) and the fold of e and w by r (written back); then this round's
+// (h(0), h(2)) observed (logged in tr.log[0]) and the candidates' base state
+// written to tr.gbase, the challenger to tr.ch_out, *tr.grind_next set to p.
+int sc_step(const ScArgs* a, int count, int m, int fold_first, const P2C16* d_consts, hipStream_t st);
 
 // Kernel arguments stay within 4 KiB for a full group.
 static_assert(sizeof(Group<GrindArgs>) <= 4096 && sizeof(Group<ScArgs>) <= 4096 &&

@@ -901,6 +901,67 @@ ScLaunch sumcheck_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int rounds,
     L.out = cin;
     return L;
   }
+  if (pow_bits > 0 && m0 <= SC_SMALL_LOG) {
+    // Per round one block per proof (the previous round's transcript step and
+    // fold, this round's coefficients and grinding state), then the search
+    // from that state; the last round's step and fold close the phase.
+    std::vector<GrindArgs> ga(B);
+    for (int i = 0; i < rounds; i++) {
+      for (int b = 0; b < B; b++) {
+        if (R[b]->s.m != m0) bad("sumcheck sizes of one group differ", WHIRGPU_BAD_STATE);
+        DevChallenger* cout = next_state(b);
+        a[b] = ScArgs{};
+        a[b].e = R[b]->s.e.p;
+        a[b].w = R[b]->s.w.p;
+        ScTr& tr = a[b].tr;
+        tr.ch_in = cin[b];
+        tr.ch_out = cout;
+        tr.log = R[b]->sl->d_sclog + log_base + i;
+        tr.grind = i > 0 ? gres(b, (i - 1) & 1) : nullptr;
+        tr.grind_next = gres(b, i & 1);
+        tr.pow_bits = pow_bits;
+        tr.gbase = R[b]->sl->d_gbase;
+        cin[b] = cout;
+        memset(&ga[b], 0, sizeof ga[b]);
+        ga[b].result = gres(b, i & 1);
+        ga[b].tr.gbase = R[b]->sl->d_gbase;
+        ga[b].tr.pow_bits = pow_bits;
+      }
+      {
+        Span sp(c, ST_SUMCHECK, 64.0 * (double)(1ull << (m0 - i)) * B);
+        KCHK(sc_step(a.data(), B, i > 0 ? m0 - i + 1 : m0, i > 0 ? 1 : 0, c->d_c16, c->st));
+      }
+      Span sp(c, ST_GRIND, 0);
+      KCHK(grind_range(c->d_c16, ga.data(), B, pow_bits, TR_GBASE, 0, P, grind_blocks_for(c, pow_bits), c->tune,
+                       c->st));
+      L.grind_recs.push_back(sp.close());
+    }
+    for (int b = 0; b < B; b++) {
+      DevChallenger* cout = next_state(b);
+      a[b] = ScArgs{};
+      a[b].e = R[b]->s.e.p;
+      a[b].w = R[b]->s.w.p;
+      a[b].partial = part(b, 1);
+      ScTr& tr = a[b].tr;
+      tr.ch_in = cin[b];
+      tr.ch_out = cout;
+      tr.log = R[b]->sl->d_sclog + log_base + rounds - 1;
+      tr.partial = part(b, 0);
+      tr.grind = gres(b, (rounds - 1) & 1);
+      tr.grind_next = gres(b, rounds & 1);
+      tr.pow_bits = pow_bits;
+      cin[b] = cout;
+    }
+    {
+      const u64 h = 1ull << (m0 - rounds);
+      Span sp(c, ST_SUMCHECK, 96.0 * (double)h * B);
+      unsigned nb_last = 0;
+      KCHK(sc_fold_tr(a.data(), B, h, TR_WITNESS_SAMPLE, c->d_c16, &nb_last, c->st));
+    }
+    for (Run* r : R) r->s.m -= rounds;
+    L.out = cin;
+    return L;
+  }
   int pp = 0;
   unsigned nb = 0;
   {
