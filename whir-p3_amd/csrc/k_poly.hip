@@ -769,6 +769,100 @@ __global__ void __launch_bounds__(256) k_sc_step(Group<ScArgs> g, int m, int fol
   ch.store(tr.ch_out);
 }
 
+// Step i of the initial sumcheck's split-route rounds (see svo_step).  Round
+// i's polynomial (Sumcheck::from_base_evals with the SVO partial sums,
+// sumcheck_prover.rs:196-227): with P_j folded by r_<i,
+//   h_i(X) = sum_j ap_j eq(z_j[<i], r_<i) eq(z_j[i], X) sum_s' eq(z_j[>i], s') P_j(r_<i, X, s'),
+// and P_j(r_<i, X, s') = sum_b eq(r_<i, b) P_j(b, X, s'), so every thread takes
+// terms (j, b, s') of the unfolded P and the block sums them.
+__global__ void __launch_bounds__(256) k_svo_step(Group<SvoArgs> g, int J, int l0, int i, int pow_bits,
+                                                  const P2C16* __restrict__ cp) {
+  __shared__ ef sh[32];
+  const SvoArgs& a = g.v[blockIdx.z];
+  const ScTr& tr = a.tr;
+  const int t = threadIdx.x;
+  const WideLane wl = wide_lane(*cp, t & 15);
+  BlockChallenger ch{0, 0, 0, 0, 0, &wl, cp};
+  ch.load(*tr.ch_in);
+  const bool wit = i > 0 && pow_bits > 0;
+  const u32 w = wit ? *tr.grind : 0u;
+  ef r_last = ef_zero();
+  if (wit) {
+    u32 err = 0;
+    const u32 wm = w < P ? to_monty(w) : 0u;
+    if (w >= P) err = 1;
+    ch.observe(wm);
+    const u32 mask = pow_bits >= 32 ? 0xffffffffu : ((1u << pow_bits) - 1);
+    if ((from_monty(ch.sample()) & mask) != 0) err = 1;
+    r_last = ch.sample_ef();
+    if (t == 0) {
+      ScLog* L = tr.log + (i - 1);
+      L->witness = wm;
+      L->r = r_last;
+      L->err = err;
+      a.rs[i - 1] = r_last;
+    }
+  }
+  if (i < l0) {
+    const int half = 1 << (l0 - 1 - i);
+    const int per = half << i;   // terms (b, s') per constraint
+    ef acc[2] = {ef_zero(), ef_zero()};
+    for (int u = t; u < J * per; u += 256) {
+      const int j = u / per, v = u % per;
+      const int b = v / half, sp = v % half;
+      const ef* z = a.z + (size_t)j * l0;
+      ef coef = a.ap[j];
+      for (int k = 0; k < i; k++) {   // eq(z_k, r_k) eq(r_k, b_k), b's top bit for r_0
+        const ef rk = (wit && k == i - 1) ? r_last : a.rs[k];
+        const ef zk = z[k];
+        const ef one = ef_one();
+        const ef zr = ef_mul(zk, rk);
+        const ef eqz = ef_add(ef_sub(ef_sub(one, zk), rk), ef_dbl(zr));   // z r + (1 - z)(1 - r)
+        const bool bit = (b >> (i - 1 - k)) & 1;
+        coef = ef_mul(coef, ef_mul(eqz, bit ? rk : ef_sub(one, rk)));
+      }
+      for (int k = 0; k < l0 - 1 - i; k++) {   // eq(z_{i+1+k}, s'_k), s' top bit first
+        const ef zk = z[i + 1 + k];
+        const bool bit = (sp >> (l0 - 2 - i - k)) & 1;
+        coef = ef_mul(coef, bit ? zk : ef_sub(ef_one(), zk));
+      }
+      const ef zi = z[i];
+      const ef e0 = ef_sub(ef_one(), zi);
+      const ef e2 = ef_sub(ef_add(ef_dbl(zi), zi), ef_one());
+      const ef* Pj = a.P + ((size_t)j << l0);
+      const int x0 = (b << (l0 - i)) | sp;
+      const ef p0 = Pj[x0], p1 = Pj[x0 | half];
+      acc[0] = ef_add(acc[0], ef_mul(ef_mul(coef, e0), p0));
+      acc[1] = ef_add(acc[1], ef_mul(ef_mul(coef, e2), ef_sub(ef_dbl(p1), p0)));
+    }
+    block_reduce_ef<2>(acc, sh);   // the same sums in every thread
+    const u32 v[8] = {acc[0].c[0], acc[0].c[1], acc[0].c[2], acc[0].c[3],
+                      acc[1].c[0], acc[1].c[1], acc[1].c[2], acc[1].c[3]};
+    ch.observe8(v);
+    ScLog* L = tr.log + i;
+    if (t == 0) {
+      L->c0 = acc[0];
+      L->c2 = acc[1];
+      L->witness = 0;
+    }
+    if (pow_bits > 0) {
+      if (t < 16) tr.gbase->s[t] = ch.absorbed();
+      if (t == 0) {
+        tr.gbase->wpos = (u32)ch.in_len;
+        if (tr.grind_next) *tr.grind_next = P;
+      }
+    } else {
+      const ef r = ch.sample_ef();
+      if (t == 0) {
+        L->r = r;
+        L->err = 0;
+        a.rs[i] = r;
+      }
+    }
+  }
+  ch.store(tr.ch_out);
+}
+
 static unsigned sc_blocks(u64 work) {
   u64 b = (work + 255) / 256;
   if (b > 1024) b = 1024;
@@ -838,6 +932,13 @@ int sc_step(const ScArgs* a, int count, int m, int fold_first, const P2C16* d_co
   if (m < 1 || m > SC_SMALL_LOG || (fold_first && m < 2)) return -1;
   return launch_group(a, count, [&](const Group<ScArgs>& g) {
     hipLaunchKernelGGL(k_sc_step, dim3(1, 1, count), dim3(256), 0, st, g, m, fold_first, d_consts);
+  });
+}
+
+int svo_step(const SvoArgs* a, int count, int J, int l0, int i, int pow_bits, const P2C16* d_consts, hipStream_t st) {
+  if (J < 0 || l0 < 1 || l0 > SVO_DEV_MAX || i < 0 || i > l0) return -1;
+  return launch_group(a, count, [&](const Group<SvoArgs>& g) {
+    hipLaunchKernelGGL(k_svo_step, dim3(1, 1, count), dim3(256), 0, st, g, J, l0, i, pow_bits, d_consts);
   });
 }
 

@@ -342,11 +342,30 @@ int sc_small(const ScArgs* a, int count, int m0, int rounds, const P2C16* d_cons
 // written to tr.gbase, the challenger to tr.ch_out, *tr.grind_next set to p.
 int sc_step(const ScArgs* a, int count, int m, int fold_first, const P2C16* d_consts, hipStream_t st);
 
+// The initial sumcheck's rounds by the split route (svo.rs:465-559) on the
+// device transcript.  Step i (0 <= i <= l0) of one proof, one block:
+// with i > 0 and proof of work, round i-1's transcript step (observe the
+// witness in *tr.grind, check its bits, sample r_{i-1} into rs[i-1]; logged in
+// tr.log[i-1]); with i < l0, round i's (h(0), h(2)) from the constraints'
+// partial sums P (J blocks of 2^l0, variable 0 the top index bit) folded by
+// r_<i, observed and logged in tr.log[i], then the grinding state written to
+// tr.gbase (*tr.grind_next set to p) or, without proof of work, r_i sampled
+// into rs[i] and the log.
+constexpr int SVO_DEV_MAX = 10;   // l0
+struct SvoArgs {
+  const ef* P;    // J * 2^l0 partial sums
+  const ef* z;    // J * l0: each constraint point's first l0 coordinates
+  const ef* ap;   // J: powers of the combination challenge
+  ef* rs;         // l0 challenges (device memory)
+  ScTr tr;
+};
+int svo_step(const SvoArgs* a, int count, int J, int l0, int i, int pow_bits, const P2C16* d_consts, hipStream_t st);
+
 // Kernel arguments stay within 4 KiB for a full group.
 static_assert(sizeof(Group<GrindArgs>) <= 4096 && sizeof(Group<ScArgs>) <= 4096 &&
                   sizeof(Group<QueryArgs>) <= 4096 && sizeof(Group<OodArgs>) <= 4096 &&
                   sizeof(Group<DotArgs>) <= 4096 && sizeof(Group<CombineArgs>) <= 4096 &&
-                  sizeof(Group<SelectArgs>) <= 4096 && sizeof(Group<EqSplitArgs>) <= 4096 &&
+                  sizeof(Group<SelectArgs>) <= 4096 && sizeof(Group<EqSplitArgs>) <= 4096 && sizeof(Group<SvoArgs>) <= 4096 &&
                   sizeof(Group<OpenArgs>) <= 4096 && sizeof(Group<TreeArgs>) <= 4096 &&
                   sizeof(Group<CompressArgs>) <= 4096 && sizeof(Group<LdeArgs<ef>>) <= 4096,
               "a group's kernel arguments exceed 4 KiB");

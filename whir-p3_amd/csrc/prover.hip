@@ -1211,6 +1211,70 @@ void initial_sumcheck(whirgpu_ctx* c, const std::vector<Run*>& R, int l0, int po
     Ps[b] = r.st->partials;
   }
   const whirgpu_statement* st0 = R[0]->st;
+  if (l0 > 0 && st0->seg_bits == l0 && st0->n > l0 && !R[0]->ch.keccak && l0 <= SVO_DEV_MAX) {
+    // the rounds on the device transcript: per round one block per proof
+    // (the previous round's witness step, this round's polynomial and
+    // grinding state), then the search; one synchronization, then the host
+    // replays the log on its challengers
+    const int J = (int)st0->points.size();
+    const size_t per = ((size_t)J << l0) + (size_t)J * l0 + (size_t)J;
+    auto sp = c->stage<ef>(per * B);
+    std::vector<const DevChallenger*> cin(B);
+    std::vector<SvoArgs> sa(B);
+    for (int b = 0; b < B; b++) {
+      Run& r = *R[b];
+      if ((int)r.st->points.size() != J || r.st->seg_bits != l0) bad("constraint counts of one group differ");
+      ef* h = sp.first + per * b;
+      for (int j = 0; j < J; j++) {
+        if (r.st->partials[j].size() != (1ull << l0)) bad("internal: split partial sums", WHIRGPU_BAD_STATE);
+        memcpy(h + ((size_t)j << l0), r.st->partials[j].data(), sizeof(ef) << l0);
+        memcpy(h + ((size_t)J << l0) + (size_t)j * l0, r.st->points[j].data(), sizeof(ef) * l0);
+      }
+      memcpy(h + ((size_t)J << l0) + (size_t)J * l0, aps[b].data(), sizeof(ef) * J);
+      const ef* d = sp.second + per * b;
+      sa[b] = SvoArgs{d, d + ((size_t)J << l0), d + ((size_t)J << l0) + (size_t)J * l0, r.sl->d_partial, ScTr{}};
+      upload_challenger(r.sl, r.ch.dx);
+      cin[b] = r.sl->d_chal_init;
+    }
+    c->flush();
+    ScLaunch la;
+    la.rounds = l0;
+    la.pow_bits = pow_bits;
+    std::vector<GrindArgs> ga(B);
+    for (int i = 0; i <= l0; i++) {
+      if (i == l0 && pow_bits == 0) break;
+      for (int b = 0; b < B; b++) {
+        Slot* sl = R[b]->sl;
+        DevChallenger* cout = cin[b] == sl->d_chal ? sl->d_chal + 1 : sl->d_chal;
+        ScTr& tr = sa[b].tr;
+        tr = ScTr{};
+        tr.ch_in = cin[b];
+        tr.ch_out = cout;
+        tr.log = sl->d_sclog;
+        tr.grind = i > 0 ? sl->d_grind + ((i - 1) & 1) * 64 : nullptr;
+        tr.grind_next = i < l0 ? sl->d_grind + (i & 1) * 64 : nullptr;
+        tr.pow_bits = pow_bits;
+        tr.gbase = sl->d_gbase;
+        cin[b] = cout;
+        memset(&ga[b], 0, sizeof ga[b]);
+        ga[b].result = sl->d_grind + (i & 1) * 64;
+        ga[b].tr.gbase = sl->d_gbase;
+        ga[b].tr.pow_bits = pow_bits;
+      }
+      KCHK(svo_step(sa.data(), B, J, l0, i, pow_bits, c->d_c16, c->st));
+      if (i < l0 && pow_bits > 0) {
+        Span gs(c, ST_GRIND, 0);
+        KCHK(grind_range(c->d_c16, ga.data(), B, pow_bits, TR_GBASE, 0, P, grind_blocks_for(c, pow_bits), c->tune,
+                         c->st));
+        la.grind_recs.push_back(gs.close());
+      }
+    }
+    c->sync();
+    for (int b = 0; b < B; b++) rss[b] = sumcheck_replay(c, *R[b], la, R[b]->P->initial_sumcheck);
+    svo_finish(c, R, l0, aps, rss);
+    for (int b = 0; b < B; b++) R[b]->fold_rand = rss[b];
+    return;
+  }
   if (l0 > 0 && st0->seg_bits == l0 && st0->n > l0) {
     std::vector<Challenger*> chs;
     for (Run* r : R) chs.push_back(&r->ch);
