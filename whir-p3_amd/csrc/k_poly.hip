@@ -778,6 +778,7 @@ __global__ void __launch_bounds__(256) k_sc_step(Group<ScArgs> g, int m, int fol
 __global__ void __launch_bounds__(256) k_svo_step(Group<SvoArgs> g, int J, int l0, int i, int pow_bits,
                                                   const P2C16* __restrict__ cp) {
   __shared__ ef sh[32];
+  __shared__ ef zs[SVO_MAX_Z], rsh[SVO_DEV_MAX], aps[SVO_MAX_J];
   const SvoArgs& a = g.v[blockIdx.z];
   const ScTr& tr = a.tr;
   const int t = threadIdx.x;
@@ -786,6 +787,12 @@ __global__ void __launch_bounds__(256) k_svo_step(Group<SvoArgs> g, int J, int l
   ch.load(*tr.ch_in);
   const bool wit = i > 0 && pow_bits > 0;
   const u32 w = wit ? *tr.grind : 0u;
+  // the points' coordinates, the challenges so far and the combination
+  // powers in LDS: one load latency instead of one per loop step below
+  for (int u = t; u < J * l0; u += 256) zs[u] = a.z[u];
+  if (t < i) rsh[t] = a.rs[t];
+  if (t < J) aps[t] = a.ap[t];
+  __syncthreads();
   ef r_last = ef_zero();
   if (wit) {
     u32 err = 0;
@@ -810,10 +817,10 @@ __global__ void __launch_bounds__(256) k_svo_step(Group<SvoArgs> g, int J, int l
     for (int u = t; u < J * per; u += 256) {
       const int j = u / per, v = u % per;
       const int b = v / half, sp = v % half;
-      const ef* z = a.z + (size_t)j * l0;
-      ef coef = a.ap[j];
+      const ef* z = zs + j * l0;
+      ef coef = aps[j];
       for (int k = 0; k < i; k++) {   // eq(z_k, r_k) eq(r_k, b_k), b's top bit for r_0
-        const ef rk = (wit && k == i - 1) ? r_last : a.rs[k];
+        const ef rk = (wit && k == i - 1) ? r_last : rsh[k];
         const ef zk = z[k];
         const ef one = ef_one();
         const ef zr = ef_mul(zk, rk);
@@ -936,7 +943,7 @@ int sc_step(const ScArgs* a, int count, int m, int fold_first, const P2C16* d_co
 }
 
 int svo_step(const SvoArgs* a, int count, int J, int l0, int i, int pow_bits, const P2C16* d_consts, hipStream_t st) {
-  if (J < 0 || l0 < 1 || l0 > SVO_DEV_MAX || i < 0 || i > l0) return -1;
+  if (J < 0 || J > SVO_MAX_J || J * l0 > SVO_MAX_Z || l0 < 1 || l0 > SVO_DEV_MAX || i < 0 || i > l0) return -1;
   return launch_group(a, count, [&](const Group<SvoArgs>& g) {
     hipLaunchKernelGGL(k_svo_step, dim3(1, 1, count), dim3(256), 0, st, g, J, l0, i, pow_bits, d_consts);
   });

@@ -352,6 +352,8 @@ int sc_step(const ScArgs* a, int count, int m, int fold_first, const P2C16* d_co
 // tr.gbase (*tr.grind_next set to p) or, without proof of work, r_i sampled
 // into rs[i] and the log.
 constexpr int SVO_DEV_MAX = 10;   // l0
+constexpr int SVO_MAX_J = 64;      // constraints
+constexpr int SVO_MAX_Z = 512;     // J * l0
 struct SvoArgs {
   const ef* P;    // J * 2^l0 partial sums
   const ef* z;    // J * l0: each constraint point's first l0 coordinates

@@ -1211,7 +1211,8 @@ void initial_sumcheck(whirgpu_ctx* c, const std::vector<Run*>& R, int l0, int po
     Ps[b] = r.st->partials;
   }
   const whirgpu_statement* st0 = R[0]->st;
-  if (l0 > 0 && st0->seg_bits == l0 && st0->n > l0 && !R[0]->ch.keccak && l0 <= SVO_DEV_MAX) {
+  if (l0 > 0 && st0->seg_bits == l0 && st0->n > l0 && !R[0]->ch.keccak && l0 <= SVO_DEV_MAX &&
+      (int)st0->points.size() <= SVO_MAX_J && (int)st0->points.size() * l0 <= SVO_MAX_Z) {
     // the rounds on the device transcript: per round one block per proof
     // (the previous round's witness step, this round's polynomial and
     // grinding state), then the search; one synchronization, then the host
