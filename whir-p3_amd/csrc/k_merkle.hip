@@ -804,6 +804,98 @@ int ood_eq_tables(const OodArgs* a, int count, int m, int hi, int lo, const P2C1
   });
 }
 
+// Every OOD sample of a round on a polynomial of at most 2^OOD_SMALL_LOG
+// evaluations in one block per proof (prover/mod.rs:192-205 with the answer
+// of evaluate_at_extension): sample y, expand it to the point, write its split
+// eq tables, sum e against them and observe the answer before the next
+// sample.  The per-sample launches (k_ood_eq_tables, the eq-weighted sum and
+// its reduction) write the same tables, points, answers and log entries.
+__global__ void __launch_bounds__(256) k_ood_small(Group<OodSmallArgs> g, int m, int hi, int lo, int n_ood,
+                                                   const P2C16* __restrict__ cp) {
+  __shared__ ef z[RT_MAX_VARS];
+  __shared__ ef th[1 << (OOD_SMALL_LOG - 8)], tl[1 << 8];
+  __shared__ ef sh[16];
+  __shared__ u32 stage[OBS_STAGE];
+  const OodSmallArgs& a = g.v[blockIdx.z];
+  const int t = threadIdx.x;
+  const P2C16& c = *cp;
+  const WideLane wl = wide_lane(c, t & 15);
+  BlockChallenger ch{0, 0, 0, 0, 0, &wl, &c};
+  ch.load(*a.st.ch_in);
+  // this thread's entries of e (at most four) before the first sample
+  const int n = 1 << m;
+  ef ex[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) ex[k] = t + 256 * k < n ? a.e[t + 256 * k] : ef_zero();
+  ch.observe_words(a.st.obs, a.st.n_obs, stage);
+  if (t == 0 && a.grind_reset) *a.grind_reset = P;
+  const int nh = 1 << hi, nl = 1 << lo;
+  for (int i = 0; i < n_ood; i++) {
+    const ef y = ch.sample_ef();
+    if (t == 0) {
+      a.log_y[i] = y;
+      ef cur = y;
+      for (int j = m - 1; j >= 0; j--) {
+        z[j] = cur;
+        cur = ef_mul(cur, cur);
+      }
+    }
+    __syncthreads();
+    ef* pt = a.pts + (size_t)i * RT_MAX_VARS;
+    if (t < m) pt[t] = z[t];
+    ef* out_hi = a.tabs + (size_t)i * a.tab_cap;
+    ef* out_lo = out_hi + nh;
+    for (int x = t; x < nh + nl; x += 256) {
+      int first = 0, cnt = hi, xx = x;
+      if (x >= nh) {
+        xx = x - nh;
+        first = hi;
+        cnt = lo;
+      }
+      ef acc = ef_one();
+      for (int j = 0; j < cnt; j++) {
+        const ef zj = z[first + j];
+        const int bit = (xx >> (cnt - 1 - j)) & 1;
+        acc = ef_mul(acc, bit ? zj : ef_sub(ef_one(), zj));
+      }
+      if (x < nh) {
+        th[xx] = acc;
+        out_hi[xx] = acc;
+      } else {
+        tl[xx] = acc;
+        out_lo[xx] = acc;
+      }
+    }
+    __syncthreads();
+    ef acc[1] = {ef_zero()};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int x = t + 256 * k;
+      if (x < n) acc[0] = ef_add(acc[0], ef_mul(ex[k], ef_mul(th[x >> lo], tl[x & (nl - 1)])));
+    }
+    block_reduce_ef<1>(acc, sh);   // the answer in every thread
+    if (t == 0) a.ev[i] = acc[0];
+    if (i + 1 < n_ood) {
+      if (t == 0) a.log_ev[i] = acc[0];
+      const u32 v[4] = {acc[0].c[0], acc[0].c[1], acc[0].c[2], acc[0].c[3]};
+#pragma unroll
+      for (int k = 0; k < 4; k++) ch.observe(v[k]);
+    }
+    __syncthreads();   // z, th, tl and sh are rewritten by the next sample
+  }
+  ch.store(a.st.ch_out);
+  if (t == 0 && a.st.n_obs == 4 && a.st.log_obs)
+    *a.st.log_obs = ef{{a.st.obs[0], a.st.obs[1], a.st.obs[2], a.st.obs[3]}};
+}
+
+int ood_small(const OodSmallArgs* a, int count, int m, int hi, int lo, int n_ood, const P2C16* d_consts,
+              hipStream_t st) {
+  if (m < 1 || m > OOD_SMALL_LOG || hi + lo != m || lo > 8 || n_ood < 1 || n_ood > RT_MAX_OOD) return -1;
+  return launch_group(a, count, [&](const Group<OodSmallArgs>& g) {
+    hipLaunchKernelGGL(k_ood_small, dim3(1, 1, count), dim3(256), 0, st, g, m, hi, lo, n_ood, d_consts);
+  });
+}
+
 // The round's steps after its OOD samples, in one block (prover/mod.rs:
 // 206-234): the proof-of-work witness is observed and its bits checked (or,
 // without proof of work, the last OOD answer / the root observed), one sample

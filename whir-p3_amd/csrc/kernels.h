@@ -240,6 +240,26 @@ struct OodArgs {
   u32* grind_reset;
 };
 int ood_eq_tables(const OodArgs* a, int count, int m, int hi, int lo, const P2C16* d_consts, hipStream_t st);
+// All n_ood OOD samples of a round on a small polynomial (m <= OOD_SMALL_LOG) in
+// one block per proof: for each sample what ood_eq_tables and the eq-weighted
+// sum of e do (tables of sample i at tabs + i * tab_cap, its answer to ev[i]),
+// the answers but the last also observed (and logged in log_ev) before the
+// next sample; the last is left to the next step, as with the per-sample
+// launches.
+constexpr int OOD_SMALL_LOG = 10;
+struct OodSmallArgs {
+  StepTr st;        // observe st.obs (the root), then the samples
+  ef* log_y;        // n_ood entries
+  ef* log_ev;       // n_ood entries (all but the last written here)
+  ef* pts;          // n_ood points of RT_MAX_VARS coordinates
+  ef* tabs;         // eq tables: sample i's hi table at tabs + i * tab_cap, its lo table 2^hi after
+  size_t tab_cap;
+  const ef* e;      // the polynomial (2^m evaluations)
+  ef* ev;           // n_ood answers
+  u32* grind_reset;
+};
+int ood_small(const OodSmallArgs* a, int count, int m, int hi, int lo, int n_ood, const P2C16* d_consts,
+              hipStream_t st);
 // One block per proof: the round's witness from `grind` (pow_bits > 0) or
 // st.obs, one discarded sample, the STIR query indices over the folded domain
 // (sorted, distinct), their points folded_gen^idx, gamma and its powers
@@ -367,7 +387,7 @@ int svo_step(const SvoArgs* a, int count, int J, int l0, int i, int pow_bits, co
 static_assert(sizeof(Group<GrindArgs>) <= 4096 && sizeof(Group<ScArgs>) <= 4096 &&
                   sizeof(Group<QueryArgs>) <= 4096 && sizeof(Group<OodArgs>) <= 4096 &&
                   sizeof(Group<DotArgs>) <= 4096 && sizeof(Group<CombineArgs>) <= 4096 &&
-                  sizeof(Group<SelectArgs>) <= 4096 && sizeof(Group<EqSplitArgs>) <= 4096 && sizeof(Group<SvoArgs>) <= 4096 &&
+                  sizeof(Group<SelectArgs>) <= 4096 && sizeof(Group<EqSplitArgs>) <= 4096 && sizeof(Group<SvoArgs>) <= 4096 && sizeof(Group<OodSmallArgs>) <= 4096 &&
                   sizeof(Group<OpenArgs>) <= 4096 && sizeof(Group<TreeArgs>) <= 4096 &&
                   sizeof(Group<CompressArgs>) <= 4096 && sizeof(Group<LdeArgs<ef>>) <= 4096,
               "a group's kernel arguments exceed 4 KiB");

@@ -1503,7 +1503,25 @@ RoundLaunch device_round_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int 
   }
   std::vector<OodArgs> oa(B);
   std::vector<DotArgs> da(B);
-  for (int i = 0; i < n_ood; i++) {
+  if (n_ood > 0 && m <= OOD_SMALL_LOG && lo <= 8) {
+    // every sample in one block per proof
+    std::vector<OodSmallArgs> os(B);
+    for (int b = 0; b < B; b++) {
+      Slot* sl = R[b]->sl;
+      os[b] = OodSmallArgs{StepTr{cin[b], rch(b), obs[b], n_obs, nullptr}, sl->d_rlog->y, sl->d_rlog->ev, sl->rb.pts,
+                           sl->d_tabs, sl->tab_cap, R[b]->s.e.p, sl->rb.ev,
+                           rp.pow_bits > 0 ? sl->rb.grind : nullptr};
+    }
+    KCHK(ood_small(os.data(), B, m, hi, lo, n_ood, c->d_c16, c->st));
+    for (int b = 0; b < B; b++) {
+      cin[b] = rch(b);
+      obs[b] = reinterpret_cast<const u32*>(R[b]->sl->rb.ev + n_ood - 1);
+      log_obs[b] = &R[b]->sl->d_rlog->ev[n_ood - 1];
+    }
+    n_obs = 4;
+    ri ^= 1;
+  }
+  for (int i = 0; i < n_ood && !(m <= OOD_SMALL_LOG && lo <= 8); i++) {
     for (int b = 0; b < B; b++) {
       Slot* sl = R[b]->sl;
       // table slot i: the weight update below uses the same tables, scaled by gamma^i
