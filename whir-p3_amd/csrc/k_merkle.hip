@@ -757,7 +757,8 @@ __global__ void __launch_bounds__(256) k_ood_eq_tables(Group<OodArgs> g, int m, 
   BlockChallenger ch{0, 0, 0, 0, 0, &wl, &c};
   ch.load(*st.ch_in);
   __shared__ u32 stage[OBS_STAGE];
-  ch.observe_words(st.obs, st.n_obs, stage);
+  __shared__ ef red[16];
+  observe_input(ch, st.obs, st.n_obs, st.obs_sum, st.obs_nb, st.log_obs, blockIdx.x == 0, stage, red);
   const ef y = ch.sample_ef();
   if (t == 0) {
     ef cur = y;
@@ -771,7 +772,6 @@ __global__ void __launch_bounds__(256) k_ood_eq_tables(Group<OodArgs> g, int m, 
     ch.store(st.ch_out);
     if (t == 0) {
       *log_y = y;
-      if (st.n_obs == 4 && st.log_obs) *st.log_obs = ef{{st.obs[0], st.obs[1], st.obs[2], st.obs[3]}};
       if (grind_reset) *grind_reset = P;
     }
     if (t < m) pt_out[t] = z[t];
@@ -884,8 +884,6 @@ __global__ void __launch_bounds__(256) k_ood_small(Group<OodSmallArgs> g, int m,
     __syncthreads();   // z, th, tl and sh are rewritten by the next sample
   }
   ch.store(a.st.ch_out);
-  if (t == 0 && a.st.n_obs == 4 && a.st.log_obs)
-    *a.st.log_obs = ef{{a.st.obs[0], a.st.obs[1], a.st.obs[2], a.st.obs[3]}};
 }
 
 int ood_small(const OodSmallArgs* a, int count, int m, int hi, int lo, int n_ood, const P2C16* d_consts,
@@ -929,7 +927,8 @@ __global__ void __launch_bounds__(256) k_round_queries(Group<QueryArgs> g, int p
     if (((u64)from_monty(ch.sample()) & pmask) != 0) err = 1;
   } else {
     __shared__ u32 stage[OBS_STAGE];
-    ch.observe_words(st.obs, st.n_obs, stage);
+    __shared__ ef red[16];
+    observe_input(ch, st.obs, st.n_obs, st.obs_sum, st.obs_nb, st.log_obs, true, stage, red);
   }
   if (!final_step) (void)ch.sample();
   const u64 folded = domain_size >> ff;
@@ -994,7 +993,6 @@ __global__ void __launch_bounds__(256) k_round_queries(Group<QueryArgs> g, int p
     log->gamma = gamma;
     log->witness = wm;
     log->err = err;
-    if (pow_bits == 0 && st.n_obs == 4 && st.log_obs) *st.log_obs = ef{{st.obs[0], st.obs[1], st.obs[2], st.obs[3]}};
   }
 }
 

@@ -295,9 +295,15 @@ static int eq_dot_impl(const DotArgs* a, int count, int nf, int sb, int lo, int 
     }
   }
   // partial is [seg][block][k]; reduce over blocks for each (seg, k)
-  Group<RedArgs> r;
-  for (int i = 0; i < count; i++) r.v[i] = RedArgs{a[i].partial, a[i].out};
-  hipLaunchKernelGGL(k_reduce_strided, dim3((1u << sb) * K, 1, count), dim3(256), 0, st, r, nblocks, K);
+  bool reduce = false;
+  for (int i = 0; i < count; i++) reduce = reduce || a[i].out;
+  if (reduce) {
+    for (int i = 0; i < count; i++)
+      if (!a[i].out) return -1;
+    Group<RedArgs> r;
+    for (int i = 0; i < count; i++) r.v[i] = RedArgs{a[i].partial, a[i].out};
+    hipLaunchKernelGGL(k_reduce_strided, dim3((1u << sb) * K, 1, count), dim3(256), 0, st, r, nblocks, K);
+  }
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 

@@ -64,6 +64,8 @@ struct ScTr {
   const u32* obs;               // TR_OBS_GRIND: words to observe first
   int n_obs;
   ef* log_obs;
+  const ef* obs_sum;            // TR_OBS_GRIND: instead of obs, the sum of obs_nb partial sums (as in StepTr)
+  int obs_nb;
   GrindBase* gbase;             // long grinds: device memory for the candidates' base state
 };
 
@@ -107,6 +109,10 @@ struct StepTr {
   const u32* obs;
   int n_obs;
   ef* log_obs;
+  // Instead of obs: observe the sum of obs_nb partial sums at obs_sum (one
+  // extension element, the eq-weighted sum's blocks; n_obs is 4).
+  const ef* obs_sum;
+  int obs_nb;
 };
 
 struct EqTables {
@@ -294,7 +300,7 @@ struct DotArgs {
   const void* f;
   EqTables tabs;
   ef* partial;
-  ef* out;
+  ef* out;   // nullptr: only the block partial sums (a transcript step reduces them)
 };
 int eq_dot_base(const DotArgs* a, int count, int nf, int sb, int lo, int K, int nblocks, hipStream_t st);
 int eq_dot_ext(const DotArgs* a, int count, int nf, int sb, int lo, int K, int nblocks, hipStream_t st);

@@ -1492,6 +1492,8 @@ RoundLaunch device_round_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int 
   std::vector<const DevChallenger*> cin(B);
   std::vector<const u32*> obs(B);
   std::vector<ef*> log_obs(B, nullptr);
+  std::vector<const ef*> obs_sum(B, nullptr);   // the last answer as its eq-weighted sum's block partials
+  int obs_nb = 0;
   int n_obs = 8;
   int ri = 0;
   auto rch = [&](int b) { return R[b]->sl->d_rchal + ri; };
@@ -1508,7 +1510,8 @@ RoundLaunch device_round_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int 
     std::vector<OodSmallArgs> os(B);
     for (int b = 0; b < B; b++) {
       Slot* sl = R[b]->sl;
-      os[b] = OodSmallArgs{StepTr{cin[b], rch(b), obs[b], n_obs, nullptr}, sl->d_rlog->y, sl->d_rlog->ev, sl->rb.pts,
+      os[b] = OodSmallArgs{StepTr{cin[b], rch(b), obs[b], n_obs, nullptr, nullptr, 0}, sl->d_rlog->y, sl->d_rlog->ev,
+                           sl->rb.pts,
                            sl->d_tabs, sl->tab_cap, R[b]->s.e.p, sl->rb.ev,
                            rp.pow_bits > 0 ? sl->rb.grind : nullptr};
     }
@@ -1526,13 +1529,13 @@ RoundLaunch device_round_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int 
       Slot* sl = R[b]->sl;
       // table slot i: the weight update below uses the same tables, scaled by gamma^i
       ef* tab = sl->d_tabs + (size_t)i * sl->tab_cap;
-      oa[b] = OodArgs{StepTr{cin[b], rch(b), obs[b], n_obs, log_obs[b]}, &sl->d_rlog->y[i],
+      oa[b] = OodArgs{StepTr{cin[b], rch(b), obs[b], n_obs, log_obs[b], obs_sum[b], obs_nb}, &sl->d_rlog->y[i],
                       sl->rb.pts + (size_t)i * RT_MAX_VARS, tab, tab + (1ull << hi),
                       i == 0 && rp.pow_bits > 0 ? sl->rb.grind : nullptr};
       EqTables t{};
       t.hi[0] = tab;
       t.lo[0] = tab + (1ull << hi);
-      da[b] = DotArgs{R[b]->s.e.p, t, sl->d_partial, sl->rb.ev + i};
+      da[b] = DotArgs{R[b]->s.e.p, t, sl->d_partial, nullptr};   // the next step reduces the partials
     }
     KCHK(ood_eq_tables(oa.data(), B, m, hi, lo, c->d_c16, c->st));
     {
@@ -1541,9 +1544,11 @@ RoundLaunch device_round_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int 
     }
     for (int b = 0; b < B; b++) {
       cin[b] = rch(b);
-      obs[b] = reinterpret_cast<const u32*>(R[b]->sl->rb.ev + i);
+      obs[b] = nullptr;
+      obs_sum[b] = R[b]->sl->d_partial;
       log_obs[b] = &R[b]->sl->d_rlog->ev[i];
     }
+    obs_nb = dot_blocks(m - lo, 0);
     n_obs = 4;
     ri ^= 1;
   }
@@ -1563,6 +1568,8 @@ RoundLaunch device_round_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int 
       tr.obs = obs[b];
       tr.n_obs = n_obs;
       tr.log_obs = log_obs[b];
+      tr.obs_sum = obs_sum[b];
+      tr.obs_nb = obs_nb;
       tr.gbase = sl->d_gbase;
       ga[b].result = tr.grind;
     }
@@ -1573,6 +1580,7 @@ RoundLaunch device_round_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int 
     for (int b = 0; b < B; b++) {
       cin[b] = rch(b);
       obs[b] = nullptr;
+      obs_sum[b] = nullptr;
       log_obs[b] = nullptr;
     }
     n_obs = 0;
@@ -1581,7 +1589,8 @@ RoundLaunch device_round_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int 
   {
     std::vector<QueryArgs> qa(B);
     for (int b = 0; b < B; b++)
-      qa[b] = QueryArgs{StepTr{cin[b], rch(b), obs[b], n_obs, log_obs[b]}, R[b]->sl->rb.grind, R[b]->sl->d_rlog,
+      qa[b] = QueryArgs{StepTr{cin[b], rch(b), obs[b], n_obs, log_obs[b], obs_sum[b], obs_nb}, R[b]->sl->rb.grind,
+                        R[b]->sl->d_rlog,
                         R[b]->sl->rb};
     KCHK(round_queries(qa.data(), B, rp.pow_bits, rp.domain_size, cfg.ff(round), rp.num_queries,
                        rp.folded_domain_gen, n_ood, c->d_c16, c->st));
@@ -1755,7 +1764,8 @@ FinalLaunch device_final_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int 
     std::vector<QueryArgs> qa(B);
     for (int b = 0; b < B; b++) {
       const u32* obs = observe_final ? reinterpret_cast<const u32*>(R[b]->s.e.p) : nullptr;
-      qa[b] = QueryArgs{StepTr{cin[b], rch(b), obs, n_obs, nullptr}, R[b]->sl->rb.grind, R[b]->sl->d_rlog + 1,
+      qa[b] = QueryArgs{StepTr{cin[b], rch(b), obs, n_obs, nullptr, nullptr, 0}, R[b]->sl->rb.grind,
+                        R[b]->sl->d_rlog + 1,
                         R[b]->sl->rb};
     }
     KCHK(round_queries(qa.data(), B, pow, fc.domain_size, cfg.ff(round), cfg.final_queries, 1u, 0, c->d_c16, c->st,

@@ -141,6 +141,27 @@ struct BlockChallenger {
   }
 };
 
+// Observes a step's input with the whole block: n_obs words at obs, or (when
+// obs_sum is given) the sum of obs_nb partial sums as one extension element,
+// reduced here instead of by a launch of its own.  The lead thread logs an
+// extension-element input (four words) to *log when given.  stage: OBS_STAGE
+// words of LDS; red: 16 extension elements of LDS.
+__device__ __forceinline__ void observe_input(BlockChallenger& ch, const u32* obs, int n_obs, const ef* obs_sum,
+                                              int obs_nb, ef* log, bool lead, u32* stage, ef* red) {
+  if (obs_sum) {
+    ef acc[1] = {ef_zero()};
+    for (int b = threadIdx.x; b < obs_nb; b += 256) acc[0] = ef_add(acc[0], obs_sum[b]);
+    block_reduce_ef<1>(acc, red);   // the same sum in every thread
+    if (lead && threadIdx.x == 0 && log) *log = acc[0];
+    const u32 v[4] = {acc[0].c[0], acc[0].c[1], acc[0].c[2], acc[0].c[3]};
+#pragma unroll
+    for (int k = 0; k < 4; k++) ch.observe(v[k]);
+    return;
+  }
+  ch.observe_words(obs, n_obs, stage);
+  if (lead && threadIdx.x == 0 && n_obs == 4 && log) *log = ef{{obs[0], obs[1], obs[2], obs[3]}};
+}
+
 // Runs one transcript step with the whole block.  TR_COEFFS_GRIND leaves the
 // candidates' base state in gb (LDS); the sampling modes leave r in *r (LDS).
 // Every global load of the step (constants, challenger, partial sums or
@@ -156,8 +177,7 @@ __device__ __forceinline__ void tr_prologue(int mode, const ScTr& a, const P2C16
   u32 err = 0;
   if (mode == TR_OBS_GRIND) {
     __shared__ u32 stage[OBS_STAGE];
-    ch.observe_words(a.obs, a.n_obs, stage);
-    if (lead && t == 0 && a.n_obs == 4 && a.log_obs) *a.log_obs = ef{{a.obs[0], a.obs[1], a.obs[2], a.obs[3]}};
+    observe_input(ch, a.obs, a.n_obs, a.obs_sum, a.obs_nb, a.log_obs, lead, stage, red);
     if (t < 16) gb->s[t] = ch.absorbed();
     if (t == 0) gb->wpos = (u32)ch.in_len;
     __syncthreads();
