@@ -804,6 +804,99 @@ int ood_eq_tables(const OodArgs* a, int count, int m, int hi, int lo, const P2C1
   });
 }
 
+// One step of the commitment's OOD samples (see commit_ood).  The answer of a
+// sample is InitialStatement::evaluate's (constraints/statement/initial.rs:
+// 143-152) from the segment sums: sum_s eq(z[..sb], s) P(s).
+__global__ void __launch_bounds__(256) k_commit_ood(Group<CoodArgs> g, int n, int sb, int hi, int lo, int sample,
+                                                    const P2C16* __restrict__ cp) {
+  __shared__ ef z[RT_MAX_VARS];
+  __shared__ ef Psh[1 << COOD_MAX_SB];
+  __shared__ ef red[16];
+  __shared__ u32 stage[OBS_STAGE];
+  const CoodArgs& a = g.v[blockIdx.z];
+  const int t = threadIdx.x;
+  const bool lead = blockIdx.x == 0;
+  const P2C16& c = *cp;
+  const WideLane wl = wide_lane(c, t & 15);
+  BlockChallenger ch{0, 0, 0, 0, 0, &wl, &c};
+  ch.load(*a.st.ch_in);
+  const int S = 1 << sb;
+  if (a.prev_part) {
+    // segment s's sum by row (t >> 4) of the block, lanes over its blocks
+    for (int s0 = 0; s0 < S; s0 += 16) {
+      const int s = s0 + (t >> 4);
+      ef acc = ef_zero();
+      if (s < S)
+        for (int b = t & 15; b < a.prev_nb; b += 16) acc = ef_add(acc, a.prev_part[(size_t)s * a.prev_nb + b]);
+      acc = row_sum16(acc);
+      if (s < S && (t & 15) == 0) Psh[s] = acc;
+    }
+    __syncthreads();
+    ef term[1] = {ef_zero()};
+    if (t < S) {
+      ef w = Psh[t];
+      for (int j = 0; j < sb; j++) {   // eq(z_j, bit j of s), bit 0 the top one
+        const ef zj = a.prev_z[j];
+        w = ef_mul(w, ((t >> (sb - 1 - j)) & 1) ? zj : ef_sub(ef_one(), zj));
+      }
+      term[0] = w;
+      if (lead) a.prev_P[t] = Psh[t];
+    }
+    block_reduce_ef<1>(term, red);   // the answer in every thread
+    if (lead && t == 0) *a.prev_eval = term[0];
+    const u32 v[4] = {term[0].c[0], term[0].c[1], term[0].c[2], term[0].c[3]};
+#pragma unroll
+    for (int k = 0; k < 4; k++) ch.observe(v[k]);
+  } else {
+    ch.observe_words(a.st.obs, a.st.n_obs, stage);
+  }
+  if (sample) {
+    const ef y = ch.sample_ef();
+    if (t == 0) {
+      ef cur = y;
+      for (int j = n - 1; j >= 0; j--) {
+        z[j] = cur;
+        cur = ef_mul(cur, cur);
+      }
+    }
+    __syncthreads();
+    if (lead) {
+      if (t == 0) *a.log_y = y;
+      if (t < n) a.z_out[t] = z[t];
+    }
+    // this block's entries of the split tables of z[sb..n)
+    u32 x = blockIdx.x * 256 + t;
+    const u32 nh = 1u << hi;
+    if (x < nh + (1u << lo)) {
+      int first = sb, cnt = hi;
+      ef* o = a.out_hi;
+      if (x >= nh) {
+        x -= nh;
+        first = sb + hi;
+        cnt = lo;
+        o = a.out_lo;
+      }
+      ef acc = ef_one();
+      for (int j = 0; j < cnt; j++) {
+        const ef zj = z[first + j];
+        acc = ef_mul(acc, ((x >> (cnt - 1 - j)) & 1) ? zj : ef_sub(ef_one(), zj));
+      }
+      o[x] = acc;
+    }
+  }
+  if (lead) ch.store(a.st.ch_out);
+}
+
+int commit_ood(const CoodArgs* a, int count, int n, int sb, int hi, int lo, int sample, const P2C16* d_consts,
+               hipStream_t st) {
+  if (n < 1 || n > RT_MAX_VARS || sb < 0 || sb > COOD_MAX_SB || hi < 0 || lo < 0 || sb + hi + lo != n) return -1;
+  const u32 tabs = (1u << hi) + (1u << lo);
+  const unsigned blocks = sample ? (tabs + 255) / 256 : 1;
+  return launch_group(a, count, [&](const Group<CoodArgs>& g) {
+    hipLaunchKernelGGL(k_commit_ood, dim3(blocks, 1, count), dim3(256), 0, st, g, n, sb, hi, lo, sample, d_consts);
+  });
+}
+
 // Every OOD sample of a round on a polynomial of at most 2^OOD_SMALL_LOG
 // evaluations in one block per proof (prover/mod.rs:192-205 with the answer
 // of evaluate_at_extension): sample y, expand it to the point, write its split

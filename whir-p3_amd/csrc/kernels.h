@@ -252,6 +252,30 @@ int ood_eq_tables(const OodArgs* a, int count, int m, int hi, int lo, const P2C1
 // the answers but the last also observed (and logged in log_ev) before the
 // next sample; the last is left to the next step, as with the per-sample
 // launches.
+// The commitment's OOD samples (CommitmentWriter::commit, committer/writer.rs:
+// 88-106) on the device transcript, one step per sample and a closing step.
+// Each step: with prev_part, reduce the previous sample's eq-weighted block
+// sums ([segment][block], prev_nb blocks) into its 2^sb segment sums (to
+// prev_P) and its answer sum_s eq(prev_z[..sb], s) P(s) (to prev_eval), and
+// observe the answer; else observe st.obs (the root, n_obs words).  Then,
+// unless closing, sample y (to log_y), expand it to the point (to z_out) and
+// write the split eq tables of its coordinates sb..n.  Grid: (blocks covering
+// the tables, or 1 when closing, 1, proofs); every block runs the transcript.
+constexpr int COOD_MAX_SB = 8;
+struct CoodArgs {
+  StepTr st;
+  const ef* prev_part;
+  int prev_nb;
+  const ef* prev_z;
+  ef* prev_P;
+  ef* prev_eval;
+  ef* log_y;
+  ef* z_out;
+  ef* out_hi;
+  ef* out_lo;
+};
+int commit_ood(const CoodArgs* a, int count, int n, int sb, int hi, int lo, int sample, const P2C16* d_consts,
+               hipStream_t st);
 constexpr int OOD_SMALL_LOG = 10;
 struct OodSmallArgs {
   StepTr st;        // observe st.obs (the root), then the samples
@@ -393,7 +417,7 @@ int svo_step(const SvoArgs* a, int count, int J, int l0, int i, int pow_bits, co
 static_assert(sizeof(Group<GrindArgs>) <= 4096 && sizeof(Group<ScArgs>) <= 4096 &&
                   sizeof(Group<QueryArgs>) <= 4096 && sizeof(Group<OodArgs>) <= 4096 &&
                   sizeof(Group<DotArgs>) <= 4096 && sizeof(Group<CombineArgs>) <= 4096 &&
-                  sizeof(Group<SelectArgs>) <= 4096 && sizeof(Group<EqSplitArgs>) <= 4096 && sizeof(Group<SvoArgs>) <= 4096 && sizeof(Group<OodSmallArgs>) <= 4096 &&
+                  sizeof(Group<SelectArgs>) <= 4096 && sizeof(Group<EqSplitArgs>) <= 4096 && sizeof(Group<SvoArgs>) <= 4096 && sizeof(Group<OodSmallArgs>) <= 4096 && sizeof(Group<CoodArgs>) <= 4096 &&
                   sizeof(Group<OpenArgs>) <= 4096 && sizeof(Group<TreeArgs>) <= 4096 &&
                   sizeof(Group<CompressArgs>) <= 4096 && sizeof(Group<LdeArgs<ef>>) <= 4096,
               "a group's kernel arguments exceed 4 KiB");

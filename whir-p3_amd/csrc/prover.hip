@@ -1924,6 +1924,96 @@ void host_final(whirgpu_ctx* c, Run& r, int round) {
   }
 }
 
+// The commitment's root observation and OOD samples on the device transcript
+// (commit_ood): per sample one step and the eq-weighted sums, a closing step,
+// and one synchronization instead of one per sample; the host then replays
+// the samples on its challengers (each y must equal the device's) and appends
+// the points, answers and segment sums to the statements, as
+// statement_evaluate would.  Returns false, having launched nothing, when the
+// shapes do not fit; the caller then takes the host-driven path.
+bool commit_ood_device(whirgpu_ctx* c, const std::vector<Run*>& R, const std::vector<whirgpu_prover_data*>& pds) {
+  const auto& cfg = c->cfg;
+  const int B = (int)R.size();
+  const int n = cfg.num_variables;
+  const int n_ood = cfg.commitment_ood_samples;
+  if (n_ood < 1 || n_ood > RT_MAX_OOD || R[0]->ch.keccak || !pds[0]->root_on_host) return false;
+  const int sb = R[0]->st->seg_bits;
+  const int nrest = n - sb;
+  if (sb > COOD_MAX_SB || n > RT_MAX_VARS || nrest < 1) return false;
+  const int lo = std::min(8, nrest), hi = nrest - lo;
+  const int nb = dot_blocks(nrest - lo, sb);
+  const size_t S = 1ull << sb;
+  if ((size_t)nb * S > c->partial_cap || S * n_ood + 2 * n_ood > Slot::SMALL_CAP) return false;
+  for (int b = 0; b < B; b++)
+    if (R[b]->st->seg_bits != sb || R[b]->st->n != n || (1ull << hi) + (1ull << lo) > R[b]->sl->tab_cap) return false;
+  // host-visible layout in each slot's small buffer: the samples' segment
+  // sums, then their answers, then their challenges y
+  const size_t at_ev = S * n_ood, at_y = at_ev + n_ood;
+  std::vector<const DevChallenger*> cin(B);
+  for (int b = 0; b < B; b++) {
+    upload_challenger(R[b]->sl, R[b]->ch.dx);
+    cin[b] = R[b]->sl->d_chal_init;
+  }
+  int ri = 0;
+  std::vector<CoodArgs> ca(B);
+  std::vector<DotArgs> da(B);
+  for (int i = 0; i <= n_ood; i++) {
+    for (int b = 0; b < B; b++) {
+      Slot* sl = R[b]->sl;
+      const whirgpu_prover_data* pd = pds[b];
+      DevChallenger* cout = sl->d_rchal + ri;
+      CoodArgs& a = ca[b];
+      a = CoodArgs{};
+      a.st = StepTr{cin[b], cout, i == 0 ? pd->tree.p + (2 * pd->height - 2) * 8 : nullptr, i == 0 ? 8 : 0, nullptr,
+                    nullptr, 0};
+      if (i > 0) {
+        a.prev_part = sl->d_partial;
+        a.prev_nb = nb;
+        a.prev_z = sl->rb.pts + (size_t)(i - 1) * RT_MAX_VARS;
+        a.prev_P = sl->d_small + S * (i - 1);
+        a.prev_eval = sl->d_small + at_ev + (i - 1);
+      }
+      if (i < n_ood) {
+        a.log_y = sl->d_small + at_y + i;
+        a.z_out = sl->rb.pts + (size_t)i * RT_MAX_VARS;
+        a.out_hi = sl->d_tabs;
+        a.out_lo = sl->d_tabs + (1ull << hi);
+      }
+      cin[b] = cout;
+      EqTables t{};
+      t.hi[0] = sl->d_tabs;
+      t.lo[0] = sl->d_tabs + (1ull << hi);
+      da[b] = DotArgs{R[b]->st->d_poly, t, sl->d_partial, nullptr};   // reduced by the next step
+    }
+    KCHK(commit_ood(ca.data(), B, n, sb, hi, lo, i < n_ood ? 1 : 0, c->d_c16, c->st));
+    ri ^= 1;
+    if (i < n_ood) {
+      Span sp(c, ST_EQDOT, 4.0 * (double)(1ull << n) * B);
+      KCHK(eq_dot_base(da.data(), B, n, sb, lo, 1, nb, c->st));
+    }
+  }
+  c->sync();
+  for (int b = 0; b < B; b++) {
+    Run& r = *R[b];
+    const Digest root = read_root(c, pds[b], true);
+    r.P->has_initial_commitment = true;
+    r.P->initial_commitment = root;
+    observe_digest(r.ch, root);
+    const ef* small = r.sl->h_small;
+    for (int i = 0; i < n_ood; i++) {
+      const ef y = r.ch.sample_ef();
+      if (!ef_eq(y, small[at_y + i])) bad("device transcript diverged from the host challenger", WHIRGPU_BAD_STATE);
+      const ef ev = small[at_ev + i];
+      r.st->points.push_back(whir::expand_from_univariate(y, n));
+      r.st->evals.push_back(ev);
+      r.st->partials.emplace_back(small + S * i, small + S * (i + 1));
+      r.P->initial_ood_answers.push_back(ev);
+      r.ch.observe_ef(ev);
+    }
+  }
+  return true;
+}
+
 // CommitmentWriter::commit (src/whir/committer/writer.rs:56-109) for every
 // proof of the group: the base LDE and Merkle tree of all polynomials in one
 // launch set, the roots observed, and the commitment's OOD samples evaluated
@@ -1958,6 +2048,7 @@ void commit_group(whirgpu_ctx* c, const std::vector<Run*>& R) {
   commit_trees(c, pds);
   for (auto& t : tmps)
     if (t.p) c->release(t);
+  if (commit_ood_device(c, R, pds)) return;
   if (pds[0]->root_on_host) c->sync();
   for (int b = 0; b < B; b++) {
     const Digest root = read_root(c, pds[b], true);
