@@ -1271,9 +1271,18 @@ void initial_sumcheck(whirgpu_ctx* c, const std::vector<Run*>& R, int l0, int po
       }
     }
     c->sync();
-    for (int b = 0; b < B; b++) rss[b] = sumcheck_replay(c, *R[b], la, R[b]->P->initial_sumcheck);
+    // the fold and weights are enqueued with the logged challenges first, so
+    // the device works while the host replays (which requires every logged
+    // challenge to equal its own and fails the proof otherwise)
+    for (int b = 0; b < B; b++)
+      for (int i = 0; i < l0; i++) rss[b].push_back(R[b]->sl->h_sclog[i].r);
     svo_finish(c, R, l0, aps, rss);
-    for (int b = 0; b < B; b++) R[b]->fold_rand = rss[b];
+    for (int b = 0; b < B; b++) {
+      const std::vector<ef> rs = sumcheck_replay(c, *R[b], la, R[b]->P->initial_sumcheck);
+      for (int i = 0; i < l0; i++)
+        if (!ef_eq(rs[i], rss[b][i])) bad("device transcript diverged from the host challenger", WHIRGPU_BAD_STATE);
+      R[b]->fold_rand = rss[b];
+    }
     return;
   }
   if (l0 > 0 && st0->seg_bits == l0 && st0->n > l0) {
