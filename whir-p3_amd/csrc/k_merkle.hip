@@ -758,7 +758,7 @@ __global__ void __launch_bounds__(256) k_ood_eq_tables(Group<OodArgs> g, int m, 
   ch.load(*st.ch_in);
   __shared__ u32 stage[OBS_STAGE];
   __shared__ ef red[16];
-  observe_input(ch, st.obs, st.n_obs, st.obs_sum, st.obs_nb, st.log_obs, blockIdx.x == 0, stage, red);
+  observe_input(ch, st.obs, st.n_obs, st.obs_sum, st.obs_nb, st.log_obs, blockIdx.x == 0, stage, red, st.obs_copy);
   const ef y = ch.sample_ef();
   if (t == 0) {
     ef cur = y;
@@ -1000,7 +1000,7 @@ __global__ void __launch_bounds__(256) k_round_queries(Group<QueryArgs> g, int p
                                                        const P2C16* __restrict__ cp, int final_step) {
   const QueryArgs& a = g.v[blockIdx.z];
   const StepTr& st = a.st;
-  const u32* grind = a.grind;
+  u32* grind = a.grind;
   RoundLog* log = a.log;
   const RoundBufs& rb = a.rb;
   __shared__ u64 q[RT_MAX_QUERIES];
@@ -1021,7 +1021,7 @@ __global__ void __launch_bounds__(256) k_round_queries(Group<QueryArgs> g, int p
   } else {
     __shared__ u32 stage[OBS_STAGE];
     __shared__ ef red[16];
-    observe_input(ch, st.obs, st.n_obs, st.obs_sum, st.obs_nb, st.log_obs, true, stage, red);
+    observe_input(ch, st.obs, st.n_obs, st.obs_sum, st.obs_nb, st.log_obs, true, stage, red, st.obs_copy);
   }
   if (!final_step) (void)ch.sample();
   const u64 folded = domain_size >> ff;
@@ -1086,6 +1086,8 @@ __global__ void __launch_bounds__(256) k_round_queries(Group<QueryArgs> g, int p
     log->gamma = gamma;
     log->witness = wm;
     log->err = err;
+    // every thread read the witness before the barriers above
+    if (grind) *grind = P;
   }
 }
 

@@ -66,6 +66,7 @@ struct ScTr {
   ef* log_obs;
   const ef* obs_sum;            // TR_OBS_GRIND: instead of obs, the sum of obs_nb partial sums (as in StepTr)
   int obs_nb;
+  u32* obs_copy;                // TR_OBS_GRIND: block 0 also copies the n_obs words at obs here (as in StepTr)
   GrindBase* gbase;             // long grinds: device memory for the candidates' base state
 };
 
@@ -113,6 +114,9 @@ struct StepTr {
   // extension element, the eq-weighted sum's blocks; n_obs is 4).
   const ef* obs_sum;
   int obs_nb;
+  // When given, block 0 also copies the n_obs words observed from obs here
+  // (host-visible memory: the final polynomial for the proof).
+  u32* obs_copy;
 };
 
 struct EqTables {
@@ -297,7 +301,7 @@ int ood_small(const OodSmallArgs* a, int count, int m, int hi, int lo, int n_ood
 // final_step: the final round's queries (no discarded sample, no gamma).
 struct QueryArgs {
   StepTr st;
-  const u32* grind;
+  u32* grind;   // the round's witness (pow_bits > 0); set back to p at the end, ready for the next search
   RoundLog* log;
   RoundBufs rb;
 };

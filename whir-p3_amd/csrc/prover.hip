@@ -175,10 +175,11 @@ struct Slot {
   DevChallenger* d_rchal = nullptr;
   GrindBase* d_gbase = nullptr;
   RoundBufs rb{};
-  // Pinned host copy of the final polynomial, taken before the final sumcheck folds it.
+  // The final polynomial (at most 2^FINAL_MAX_LOG entries) is observed in
+  // one staged run and copied to h_small by the step that observes it.
   static constexpr int FINAL_MAX_LOG = 6;
   static_assert((4 << FINAL_MAX_LOG) <= OBS_STAGE, "the final polynomial is observed in one staged run");
-  ef* h_final = nullptr;
+  static_assert((1 << FINAL_MAX_LOG) <= SMALL_CAP, "the final polynomial fits the small buffer");
 };
 
 struct whirgpu_ctx {
@@ -469,10 +470,8 @@ void slot_free(Slot* sl) {
   if (sl->d_mem) (void)hipFree(sl->d_mem);
   if (sl->d_tabs) (void)hipFree(sl->d_tabs);
   if (sl->h_mem) (void)hipHostFree(sl->h_mem);
-  if (sl->h_final) (void)hipHostFree(sl->h_final);
   sl->d_mem = sl->h_mem = nullptr;
   sl->d_tabs = nullptr;
-  sl->h_final = nullptr;
 }
 
 void slot_alloc(whirgpu_ctx* c, Slot* sl) {
@@ -488,8 +487,6 @@ void slot_alloc(whirgpu_ctx* c, Slot* sl) {
               &hb);
   sl->tab_cap = c->tab_cap;
   HIPCHK(hipMalloc(&sl->d_tabs, 8 * sl->tab_cap * sizeof(ef)));
-  HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&sl->h_final), sizeof(ef) << Slot::FINAL_MAX_LOG,
-                       hipHostMallocDefault));
   c->fixed_bytes += db + 8 * sl->tab_cap * sizeof(ef);
 }
 
@@ -1519,7 +1516,7 @@ RoundLaunch device_round_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int 
     std::vector<OodSmallArgs> os(B);
     for (int b = 0; b < B; b++) {
       Slot* sl = R[b]->sl;
-      os[b] = OodSmallArgs{StepTr{cin[b], rch(b), obs[b], n_obs, nullptr, nullptr, 0}, sl->d_rlog->y, sl->d_rlog->ev,
+      os[b] = OodSmallArgs{StepTr{cin[b], rch(b), obs[b], n_obs, nullptr, nullptr, 0, nullptr}, sl->d_rlog->y, sl->d_rlog->ev,
                            sl->rb.pts,
                            sl->d_tabs, sl->tab_cap, R[b]->s.e.p, sl->rb.ev,
                            rp.pow_bits > 0 ? sl->rb.grind : nullptr};
@@ -1538,7 +1535,7 @@ RoundLaunch device_round_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int 
       Slot* sl = R[b]->sl;
       // table slot i: the weight update below uses the same tables, scaled by gamma^i
       ef* tab = sl->d_tabs + (size_t)i * sl->tab_cap;
-      oa[b] = OodArgs{StepTr{cin[b], rch(b), obs[b], n_obs, log_obs[b], obs_sum[b], obs_nb}, &sl->d_rlog->y[i],
+      oa[b] = OodArgs{StepTr{cin[b], rch(b), obs[b], n_obs, log_obs[b], obs_sum[b], obs_nb, nullptr}, &sl->d_rlog->y[i],
                       sl->rb.pts + (size_t)i * RT_MAX_VARS, tab, tab + (1ull << hi),
                       i == 0 && rp.pow_bits > 0 ? sl->rb.grind : nullptr};
       EqTables t{};
@@ -1598,7 +1595,7 @@ RoundLaunch device_round_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int 
   {
     std::vector<QueryArgs> qa(B);
     for (int b = 0; b < B; b++)
-      qa[b] = QueryArgs{StepTr{cin[b], rch(b), obs[b], n_obs, log_obs[b], obs_sum[b], obs_nb}, R[b]->sl->rb.grind,
+      qa[b] = QueryArgs{StepTr{cin[b], rch(b), obs[b], n_obs, log_obs[b], obs_sum[b], obs_nb, nullptr}, R[b]->sl->rb.grind,
                         R[b]->sl->d_rlog,
                         R[b]->sl->rb};
     KCHK(round_queries(qa.data(), B, rp.pow_bits, rp.domain_size, cfg.ff(round), rp.num_queries,
@@ -1724,8 +1721,10 @@ struct FinalLaunch {
 // log, openings and sumcheck log use their own slots (RoundLog 1, the opening
 // buffer's second half, ScLog entries from SC_LOG_CAP / 2), apart from the
 // last round's.
+// grind_clear: the round's grind slot is known to hold p (a device round's
+// query step, which sets it back after reading its witness, ran last).
 FinalLaunch device_final_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int round,
-                                std::vector<const DevChallenger*> cin) {
+                                std::vector<const DevChallenger*> cin, bool grind_clear) {
   const int B = (int)R.size();
   const auto& cfg = c->cfg;
   FinalLaunch fl;
@@ -1741,15 +1740,16 @@ FinalLaunch device_final_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int 
   }
   int ri = 0;
   auto rch = [&](int b) { return R[b]->sl->d_rchal + ri; };
-  for (Run* r : R)
-    HIPCHK(hipMemcpyAsync(r->sl->h_final, r->s.e.p, fl.nf * sizeof(ef), hipMemcpyDeviceToHost, c->st));
+  // the step that observes the final polynomial also copies it to the slot's
+  // small buffer (host-visible), before the final sumcheck folds it
+  if (fl.nf > Slot::SMALL_CAP) bad("final polynomial larger than the small buffer", WHIRGPU_BAD_STATE);
   int n_obs = (int)(4 * fl.nf);
   bool observe_final = true;
   if (pow > 0) {
     std::vector<GrindArgs> ga(B);
     for (int b = 0; b < B; b++) {
       Slot* sl = R[b]->sl;
-      HIPCHK(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(sl->rb.grind), (int)P, 1, c->st));
+      if (!grind_clear) HIPCHK(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(sl->rb.grind), (int)P, 1, c->st));
       memset(&ga[b], 0, sizeof ga[b]);
       ScTr& tr = ga[b].tr;
       tr.ch_in = cin[b];
@@ -1758,6 +1758,7 @@ FinalLaunch device_final_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int 
       tr.pow_bits = pow;
       tr.obs = reinterpret_cast<const u32*>(R[b]->s.e.p);
       tr.n_obs = n_obs;
+      tr.obs_copy = reinterpret_cast<u32*>(sl->d_small);
       tr.gbase = sl->d_gbase;
       ga[b].result = tr.grind;
     }
@@ -1773,7 +1774,9 @@ FinalLaunch device_final_launch(whirgpu_ctx* c, const std::vector<Run*>& R, int 
     std::vector<QueryArgs> qa(B);
     for (int b = 0; b < B; b++) {
       const u32* obs = observe_final ? reinterpret_cast<const u32*>(R[b]->s.e.p) : nullptr;
-      qa[b] = QueryArgs{StepTr{cin[b], rch(b), obs, n_obs, nullptr, nullptr, 0}, R[b]->sl->rb.grind,
+      qa[b] = QueryArgs{StepTr{cin[b], rch(b), obs, n_obs, nullptr, nullptr, 0,
+                               observe_final ? reinterpret_cast<u32*>(R[b]->sl->d_small) : nullptr},
+                        R[b]->sl->rb.grind,
                         R[b]->sl->d_rlog + 1,
                         R[b]->sl->rb};
     }
@@ -1803,7 +1806,7 @@ void device_final_replay(whirgpu_ctx* c, Run& r, int round, const FinalLaunch& f
   const int pow = cfg.final_pow_bits;
   const auto fc = cfg.final_round_config();
   whir::WhirProof& Pf = *r.P;
-  Pf.final_poly.assign(r.sl->h_final, r.sl->h_final + fl.nf);
+  Pf.final_poly.assign(r.sl->h_small, r.sl->h_small + fl.nf);
   Pf.has_final_poly = true;
   for (const ef& e : Pf.final_poly) r.ch.observe_ef(e);
   const RoundLog& L = r.sl->h_rlog[1];
@@ -1974,7 +1977,7 @@ bool commit_ood_device(whirgpu_ctx* c, const std::vector<Run*>& R, const std::ve
       CoodArgs& a = ca[b];
       a = CoodArgs{};
       a.st = StepTr{cin[b], cout, i == 0 ? pd->tree.p + (2 * pd->height - 2) * 8 : nullptr, i == 0 ? 8 : 0, nullptr,
-                    nullptr, 0};
+                    nullptr, 0, nullptr};
       if (i > 0) {
         a.prev_part = sl->d_partial;
         a.prev_nb = nb;
@@ -2088,6 +2091,7 @@ void prove_group(whirgpu_ctx* c, const std::vector<Run*>& R) {
   initial_sumcheck(c, R, cfg.ff(0), cfg.starting_folding_pow_bits);
   FinalLaunch fl;
   bool final_launched = false;
+  bool last_device = false;   // the last WHIR round ran on the device transcript
   for (int round = 0; round <= cfg.n_rounds(); round++) {
     if (final_launched) {   // enqueued after the last round (its sumcheck already advanced s)
       c->sync();
@@ -2099,7 +2103,7 @@ void prove_group(whirgpu_ctx* c, const std::vector<Run*>& R) {
       if (m != r->s.m) bad("internal: round size mismatch", WHIRGPU_BAD_STATE);
     if (round == cfg.n_rounds()) {
       if (device_final_fits(c, cfg, m, R[0]->prev)) {
-        fl = device_final_launch(c, R, round, {});
+        fl = device_final_launch(c, R, round, {}, last_device);
         c->sync();
         for (Run* r : R) device_final_replay(c, *r, round, fl);
       } else {
@@ -2116,6 +2120,7 @@ void prove_group(whirgpu_ctx* c, const std::vector<Run*>& R) {
     }
     if (device_round_fits(c, rp, m, R[0]->prev)) {
       const RoundLaunch rl = device_round_launch(c, R, round, kn);
+      last_device = true;
       // the next commitment depends only on the folded polynomial; after
       // the last round, the final round continues from the device state
       if (round + 1 < cfg.n_rounds()) {
@@ -2125,7 +2130,7 @@ void prove_group(whirgpu_ctx* c, const std::vector<Run*>& R) {
         for (Run* r : RF) {   // the final round opens this round's commitment
           std::swap(r->prev, r->cur);
         }
-        fl = device_final_launch(c, RF, round + 1, rl.la.out);
+        fl = device_final_launch(c, RF, round + 1, rl.la.out, true);
         for (Run* r : RF) std::swap(r->prev, r->cur);
         final_launched = true;
       }
@@ -2133,6 +2138,7 @@ void prove_group(whirgpu_ctx* c, const std::vector<Run*>& R) {
       for (Run* r : R) device_round_replay(c, *r, round, rl);
     } else {
       for (Run* r : R) host_round(c, *r, round);
+      last_device = false;
     }
     for (Run* r : R) {
       whirgpu_prover_data_destroy(r->prev);

@@ -144,10 +144,12 @@ struct BlockChallenger {
 // Observes a step's input with the whole block: n_obs words at obs, or (when
 // obs_sum is given) the sum of obs_nb partial sums as one extension element,
 // reduced here instead of by a launch of its own.  The lead thread logs an
-// extension-element input (four words) to *log when given.  stage: OBS_STAGE
-// words of LDS; red: 16 extension elements of LDS.
+// extension-element input (four words) to *log when given, and copies words
+// observed from memory to copy when given.  stage: OBS_STAGE words of LDS;
+// red: 16 extension elements of LDS.
 __device__ __forceinline__ void observe_input(BlockChallenger& ch, const u32* obs, int n_obs, const ef* obs_sum,
-                                              int obs_nb, ef* log, bool lead, u32* stage, ef* red) {
+                                              int obs_nb, ef* log, bool lead, u32* stage, ef* red,
+                                              u32* copy = nullptr) {
   if (obs_sum) {
     ef acc[1] = {ef_zero()};
     for (int b = threadIdx.x; b < obs_nb; b += 256) acc[0] = ef_add(acc[0], obs_sum[b]);
@@ -160,6 +162,8 @@ __device__ __forceinline__ void observe_input(BlockChallenger& ch, const u32* ob
   }
   ch.observe_words(obs, n_obs, stage);
   if (lead && threadIdx.x == 0 && n_obs == 4 && log) *log = ef{{obs[0], obs[1], obs[2], obs[3]}};
+  if (lead && copy)
+    for (int i = threadIdx.x; i < n_obs; i += blockDim.x) copy[i] = stage[i];
 }
 
 // Runs one transcript step with the whole block.  TR_COEFFS_GRIND leaves the
@@ -177,7 +181,7 @@ __device__ __forceinline__ void tr_prologue(int mode, const ScTr& a, const P2C16
   u32 err = 0;
   if (mode == TR_OBS_GRIND) {
     __shared__ u32 stage[OBS_STAGE];
-    observe_input(ch, a.obs, a.n_obs, a.obs_sum, a.obs_nb, a.log_obs, lead, stage, red);
+    observe_input(ch, a.obs, a.n_obs, a.obs_sum, a.obs_nb, a.log_obs, lead, stage, red, a.obs_copy);
     if (t < 16) gb->s[t] = ch.absorbed();
     if (t == 0) gb->wpos = (u32)ch.in_len;
     __syncthreads();
