@@ -88,7 +88,8 @@ def test_lde_ext(W, m, k, R):
 
 @pytest.mark.parametrize("leaf_width", [16, 24])
 @pytest.mark.parametrize("log_h,row_len", [(0, 16), (1, 16), (5, 16), (7, 16), (8, 16), (10, 16), (12, 16),
-                                           (9, 64), (3, 2), (4, 20), (14, 16), (15, 16), (16, 16)])
+                                           (9, 64), (3, 2), (4, 20), (14, 16), (15, 16), (16, 16), (18, 16),
+                                           (19, 8)])
 def test_merkle_root_and_openings(W, leaf_width, log_h, row_len):
     cfg = ctx(W, leaf_hash_width=leaf_width)
     rng = np.random.default_rng(log_h * 7 + row_len + leaf_width)
