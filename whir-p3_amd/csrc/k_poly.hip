@@ -337,6 +337,21 @@ __global__ void __launch_bounds__(256) k_eq_combine(Group<CombineArgs> g, u64 n,
         const ef t = ef_mul(tabs.hi[k][x >> lo], tabs.lo[k][x & mask]);
         acc = ef_add(acc, ef_mul(scale[k], t));
       }
+    } else if (K > 0 && lo >= 8) {
+      // a block's 256 points share x >> lo: the hi entries are block-uniform
+      // (scalar loads), and the factor 3 of the wrapping terms goes on them
+      // (three scalar operations per term) instead of on the lo entries
+      u64 c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+      const u32 xh = __builtin_amdgcn_readfirstlane((u32)(x >> lo));
+      for (int k = 0; k < K; k++) {
+        const ef a = tabs.hi[k][xh], b = tabs.lo[k][x & mask];
+        const u32 a1 = add(dbl(a.c[1]), a.c[1]), a2 = add(dbl(a.c[2]), a.c[2]), a3 = add(dbl(a.c[3]), a.c[3]);
+        c0 = fold32(c0 + (u64)a.c[0] * b.c[0] + (u64)a1 * b.c[3] + (u64)a2 * b.c[2] + (u64)a3 * b.c[1]);
+        c1 = fold32(c1 + (u64)a.c[0] * b.c[1] + (u64)a.c[1] * b.c[0] + (u64)a2 * b.c[3] + (u64)a3 * b.c[2]);
+        c2 = fold32(c2 + (u64)a.c[0] * b.c[2] + (u64)a.c[1] * b.c[1] + (u64)a.c[2] * b.c[0] + (u64)a3 * b.c[3]);
+        c3 = fold32(c3 + (u64)a.c[0] * b.c[3] + (u64)a.c[1] * b.c[2] + (u64)a.c[2] * b.c[1] + (u64)a.c[3] * b.c[0]);
+      }
+      acc = ef_add(acc, ef{{redc_wide(c0), redc_wide(c1), redc_wide(c2), redc_wide(c3)}});
     } else if (K > 0) {
       u64 c0 = 0, c1 = 0, c2 = 0, c3 = 0;
       for (int k = 0; k < K; k++) {
