@@ -610,7 +610,7 @@ __global__ void __launch_bounds__(256) k_sc_fold_coeffs(Group<ScArgs> g, u64 h, 
   ef r = a.r;
   if (tr_mode != TR_NONE) {
     __shared__ ef rs;
-    tr_prologue(tr_mode, a.tr, *cp, nullptr, &rs);
+    tr_prologue_wave0(tr_mode, a.tr, *cp, &rs);
     r = rs;
   }
   ef acc[2] = {ef_zero(), ef_zero()};
@@ -648,7 +648,7 @@ __global__ void __launch_bounds__(256) k_sc_fold(Group<ScArgs> g, u64 h, int tr_
   ef r = a.r;
   if (tr_mode != TR_NONE) {
     __shared__ ef rs;
-    tr_prologue(tr_mode, a.tr, *cp, nullptr, &rs);
+    tr_prologue_wave0(tr_mode, a.tr, *cp, &rs);
     r = rs;
   }
   for (u64 j = j0; j < h; j += stride) {

@@ -233,4 +233,66 @@ __device__ __forceinline__ void tr_prologue(int mode, const ScTr& a, const P2C16
   __syncthreads();
 }
 
+// The sampling modes (TR_COEFFS_SAMPLE, TR_WITNESS_SAMPLE) run by wave 0 of
+// the block alone, for kernels with several blocks per CU (the large folds):
+// every wave of every block would otherwise compute the same permutation, and
+// the waves sharing a SIMD would multiply its latency.  The other waves wait at
+// the block barrier, with their own loads in flight.  r is left in *r (LDS).
+// Requires blockDim.x to be a multiple of 64.
+__device__ __forceinline__ void tr_prologue_wave0(int mode, const ScTr& a, const P2C16& c, ef* r) {
+  const int t = threadIdx.x;
+  if (t < 64) {
+    const WideLane wl = wide_lane(c, t & 15);
+    BlockChallenger ch{0, 0, 0, 0, 0, &wl, &c};
+    ch.load(*a.ch_in);
+    const bool lead = blockIdx.x == 0 && blockIdx.y == 0;
+    u32 err = 0;
+    if (mode == TR_COEFFS_SAMPLE) {
+      ef acc0 = ef_zero(), acc1 = ef_zero();
+      for (int b = t; b < a.nb; b += 64) {
+        acc0 = ef_add(acc0, a.partial[2 * b]);
+        acc1 = ef_add(acc1, a.partial[2 * b + 1]);
+      }
+      acc0 = row_sum16(acc0);
+      acc1 = row_sum16(acc1);
+      // the wave's four row sums, the same in every lane
+      u32 v[8];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const u32 x0 = add(__builtin_amdgcn_readlane((int)acc0.c[k], 0), __builtin_amdgcn_readlane((int)acc0.c[k], 16));
+        const u32 x1 = add(__builtin_amdgcn_readlane((int)acc0.c[k], 32), __builtin_amdgcn_readlane((int)acc0.c[k], 48));
+        const u32 y0 = add(__builtin_amdgcn_readlane((int)acc1.c[k], 0), __builtin_amdgcn_readlane((int)acc1.c[k], 16));
+        const u32 y1 = add(__builtin_amdgcn_readlane((int)acc1.c[k], 32), __builtin_amdgcn_readlane((int)acc1.c[k], 48));
+        v[k] = add(x0, x1);
+        v[4 + k] = add(y0, y1);
+      }
+      if (lead && t == 0) {
+        a.log->c0 = ef{{v[0], v[1], v[2], v[3]}};
+        a.log->c2 = ef{{v[4], v[5], v[6], v[7]}};
+        a.log->witness = 0;
+      }
+      ch.observe8(v);
+    } else {
+      const u32 w = *a.grind;
+      const u32 wm = w < P ? to_monty(w) : 0u;
+      if (w >= P) err = 1;
+      ch.observe(wm);
+      const u32 mask = a.pow_bits >= 32 ? 0xffffffffu : ((1u << a.pow_bits) - 1);
+      if ((from_monty(ch.sample()) & mask) != 0) err = 1;
+      if (lead && t == 0) a.log->witness = wm;
+    }
+    const ef rr = ch.sample_ef();
+    if (t == 0) *r = rr;
+    if (lead) {
+      ch.store(a.ch_out);
+      if (t == 0) {
+        a.log->r = rr;
+        a.log->err = err;
+        if (a.grind_next) *a.grind_next = P;
+      }
+    }
+  }
+  __syncthreads();
+}
+
 }  // namespace kb
