@@ -53,3 +53,20 @@ def test_schedule_candidates():
     assert (4, None, 3) in c and (3, 5, 1) in c
     assert len(c) == 8 and len(set(c)) == 8
     assert all(k0 != k for k, k0, _ in c)
+
+
+def test_committed_cache_keys_match_params():
+    """Every launch-shape entry of the committed cache is keyed by the current
+    parameter fields, so that a configuration can find its entry (a field
+    added to the parameters orphans every older key)."""
+    W = product()
+    A = W.autotune
+    c = A.load_cache(A.DEFAULT_CACHE)
+    assert c["devices"], "committed tuning cache is empty"
+    fields = [f for f, _ in W.params(20)._fields_]
+    for dev in c["devices"].values():
+        for key in dev["kernels"]:
+            assert [kv.split("=")[0] for kv in key.split(",")] == fields, key
+    prm = W.params(24)
+    hits = [d for d in c["devices"].values() if A.params_key(prm) in d["kernels"]]
+    assert hits, "no cached launch shapes for the n=24 benchmark parameters"
